@@ -1,0 +1,9 @@
+"""dgtd_amd -- MI355X-native (gfx950) hesthaven DGTD Maxwell operator.
+
+The compute path is libdgtd_hip.so (hand-written HIP kernels behind the C ABI
+in include/dgtd_hip.h); this package is the host-side mirror of the
+reference's operator and solver surfaces.
+"""
+from ._abi import DgtdError, LIB_PATH, lib  # noqa: F401
+from .hesthaven import (HesthavenEvolution, Solver, cube_mesh, estimate_time_step,  # noqa: F401
+                        mesh_geometry, mesh_vmaps, num_face_nodes, num_nodes, ref_element)

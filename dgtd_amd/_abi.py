@@ -1,0 +1,105 @@
+"""ctypes binding of libdgtd_hip.so (C ABI declared in include/dgtd_hip.h).
+
+The library is the product: there is no Python or CPU fallback.  If it is
+missing, importing this module raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdgtd_hip.so")
+
+c_int32 = ctypes.c_int32
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+DGTD_OK = 0
+DGTD_BC_NONE, DGTD_BC_PEC, DGTD_BC_PMC, DGTD_BC_SMA = 0, 1, 2, 3
+DGTD_RK4, DGTD_LSERK4 = 0, 1
+
+
+class DgtdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dgtd error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("K", c_int32), ("order", c_int32), ("vx", _dp), ("nv", c_int32),
+                ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("alpha", c_double), ("precision", c_int32), ("integrator", c_int32),
+                ("device", c_int32), ("reserved", c_int32)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/dgtd_hip.h.
+SIGNATURES = {
+    "dgtd_last_error": (ctypes.c_char_p, []),
+    "dgtd_version": (ctypes.c_char_p, []),
+    "dgtd_op_create": (c_int32, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Opts),
+                                 ctypes.POINTER(c_void_p)]),
+    "dgtd_op_destroy": (None, [c_void_p]),
+    "dgtd_op_ndofs": (c_int32, [c_void_p, _i64p]),
+    "dgtd_op_np": (c_int32, [c_void_p, _i32p]),
+    "dgtd_op_max_fscale": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_node_positions": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_mult": (c_int32, [c_void_p, c_double, _dp, _dp]),
+    "dgtd_op_mult_device": (c_int32, [c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "dgtd_op_set_state": (c_int32, [c_void_p, _dp, c_double]),
+    "dgtd_op_get_state": (c_int32, [c_void_p, _dp, _dp]),
+    "dgtd_op_advance": (c_int32, [c_void_p, c_double, c_int32]),
+    "dgtd_op_run": (c_int32, [c_void_p, c_double, c_double, _i32p]),
+    "dgtd_op_time": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_step": (c_int32, [c_void_p, _dp, _dp, c_double, c_int32]),
+    "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_set_timing": (c_int32, [c_void_p, c_int32]),
+    "dgtd_op_last_kernel_us": (c_int32, [c_void_p, _dp, _i32p]),
+    "dgtd_op_work_per_element": (c_int32, [c_void_p, _dp, _dp]),
+    "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
+    "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),
+    "dgtd_mesh_vmaps": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _i32p, _i32p, _i8p]),
+    "dgtd_mesh_geometry": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libdgtd_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                "g.build()' or make -C dgtd_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != DGTD_OK:
+        raise DgtdError(rc, lib().dgtd_last_error().decode())
+    return rc
+
+
+def dptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def i32ptr(a):
+    return None if a is None else a.ctypes.data_as(_i32p)
+
+
+def i8ptr(a):
+    return None if a is None else a.ctypes.data_as(_i8p)

@@ -1,0 +1,627 @@
+// C ABI of the operator (include/dgtd_hip.h): setup on the host in FP64,
+// device-resident buffers, fused stage launches.
+#include "dgtd_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "mesh.hpp"
+#include "reftet.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// LSERK4: Carpenter & Kennedy (1994), 5-stage 4th-order 2N-storage
+// coefficients as published (also tabulated in Hesthaven & Warburton 2008).
+const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                      -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                      1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                      2277821191437.0 / 14882151754819.0};
+
+template <typename T>
+void* upload(const std::vector<T>& h)
+{
+    void* d = nullptr;
+    hip_check(hipMalloc(&d, sizeof(T) * h.size()), "hipMalloc");
+    hip_check(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return d;
+}
+
+}  // namespace
+
+struct dgtd_op {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = 64;
+    int integrator = DGTD_LSERK4;
+    double alpha = 1.0;
+    dgtd::RefTet R;
+    int K = 0, Np = 0, Nfp = 0;
+    size_t N = 0;
+    int64_t ndofs = 0;
+    double max_fscale = 0.0;
+    std::vector<double> vx;
+    std::vector<int32_t> EToV;
+    int nv = 0;
+
+    void* d_geo = nullptr;
+    int* d_conn = nullptr;
+    void* d_mats = nullptr;
+    int* d_tabs = nullptr;
+    double* d_M = nullptr;
+    double* d_jac = nullptr;
+    double* d_eps = nullptr;
+    double* d_mu = nullptr;
+    double* d_partials = nullptr;
+    double* d_out3 = nullptr;
+    int nparts = 0;
+    void* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    int cur = 0;  // LSERK: buffer holding the state
+    double t = 0.0;
+    int grid = 1, nblk = 1;
+
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_us = 0.0;
+    int last_launches = 0;
+
+    size_t real_size() const { return precision == 64 ? sizeof(double) : sizeof(float); }
+    void* state() const { return integrator == DGTD_LSERK4 ? d_buf[cur] : d_buf[0]; }
+
+    ~dgtd_op()
+    {
+        (void)hipSetDevice(device);
+        for (void* p : {d_geo, (void*)d_conn, d_mats, (void*)d_tabs, (void*)d_M, (void*)d_jac,
+                        (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, d_buf[0],
+                        d_buf[1], d_buf[2], d_buf[3]})
+            if (p) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    dgtd::StageArgs<T> base_args() const
+    {
+        dgtd::StageArgs<T> a{};
+        a.K = K;
+        a.nblk = nblk;
+        a.alpha = (T)alpha;
+        a.geo = static_cast<const T*>(d_geo);
+        a.conn = d_conn;
+        a.mats = static_cast<const T*>(d_mats);
+        a.tabs = d_tabs;
+        return a;
+    }
+
+    template <typename T>
+    void launch(const dgtd::StageArgs<T>& a, int mode)
+    {
+        hip_check(dgtd::launch_stage<T>(R.p, a, mode, grid, stream), "stage kernel launch");
+    }
+
+    template <typename T>
+    int one_step(double dt)
+    {
+        auto a = base_args<T>();
+        a.dt = (T)dt;
+        if (integrator == DGTD_LSERK4) {
+            for (int s = 0; s < 5; ++s) {
+                a.qin = static_cast<const T*>(d_buf[cur]);
+                a.out = static_cast<T*>(d_buf[cur ^ 1]);
+                a.res = static_cast<T*>(d_buf[2]);
+                a.ca = (T)kA[s];
+                a.cb = (T)kB[s];
+                launch<T>(a, dgtd::kModeLserk);
+                cur ^= 1;
+            }
+            return 5;
+        }
+        // mfem::RK4Solver stage order: buffers x = 0, y ping-pong 1/2, z = 3
+        T* x = static_cast<T*>(d_buf[0]);
+        T* y1 = static_cast<T*>(d_buf[1]);
+        T* y2 = static_cast<T*>(d_buf[2]);
+        T* z = static_cast<T*>(d_buf[3]);
+        a.res = z;
+        a.x = x;
+        a.qin = x; a.out = y1; a.ca = (T)(dt / 2); a.cb = (T)(dt / 6);
+        launch<T>(a, dgtd::kModeRk1);
+        a.qin = y1; a.out = y2; a.ca = (T)(dt / 2); a.cb = (T)(dt / 3);
+        launch<T>(a, dgtd::kModeRk2);
+        a.qin = y2; a.out = y1; a.ca = (T)dt; a.cb = (T)(dt / 3);
+        launch<T>(a, dgtd::kModeRk3);
+        a.qin = y1; a.out = x; a.ca = (T)(dt / 6);
+        launch<T>(a, dgtd::kModeRk4);
+        return 4;
+    }
+
+    int step(double dt) { return precision == 64 ? one_step<double>(dt) : one_step<float>(dt); }
+
+    template <typename T>
+    void mult_dev(const void* in, void* out, hipStream_t s)
+    {
+        auto a = base_args<T>();
+        a.qin = static_cast<const T*>(in);
+        a.out = static_cast<T*>(out);
+        hip_check(dgtd::launch_stage<T>(R.p, a, dgtd::kModeMult, grid, s), "stage kernel launch");
+    }
+
+    void to_device(const double* h, void* d)
+    {
+        if (precision == 64) {
+            hip_check(hipMemcpyAsync(d, h, sizeof(double) * ndofs, hipMemcpyHostToDevice, stream), "H2D");
+        } else {
+            std::vector<float> tmp(h, h + ndofs);
+            hip_check(hipMemcpyAsync(d, tmp.data(), sizeof(float) * ndofs, hipMemcpyHostToDevice, stream),
+                      "H2D");
+            hip_check(hipStreamSynchronize(stream), "sync");
+        }
+    }
+
+    void to_host(const void* d, double* h)
+    {
+        if (precision == 64) {
+            hip_check(hipMemcpyAsync(h, d, sizeof(double) * ndofs, hipMemcpyDeviceToHost, stream), "D2H");
+            hip_check(hipStreamSynchronize(stream), "sync");
+        } else {
+            std::vector<float> tmp(ndofs);
+            hip_check(hipMemcpyAsync(tmp.data(), d, sizeof(float) * ndofs, hipMemcpyDeviceToHost, stream),
+                      "D2H");
+            hip_check(hipStreamSynchronize(stream), "sync");
+            std::copy(tmp.begin(), tmp.end(), h);
+        }
+    }
+};
+
+namespace {
+
+template <typename T>
+std::vector<T> pack_mats(const dgtd::RefTet& R)
+{
+    const int Np = R.Np, NF = 4 * R.Nfp;
+    std::vector<T> m((size_t)3 * Np * Np + (size_t)NF * Np);
+    const std::vector<double>* D[3] = {&R.Dr, &R.Ds, &R.Dt};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < Np; ++i)
+            for (int j = 0; j < Np; ++j) m[(size_t)d * Np * Np + (size_t)j * Np + i] = (T)(*D[d])[(size_t)i * Np + j];
+    T* L = m.data() + 3 * (size_t)Np * Np;
+    for (int i = 0; i < Np; ++i)
+        for (int k = 0; k < NF; ++k) L[(size_t)k * Np + i] = (T)R.LIFT[(size_t)i * NF + k];
+    return m;
+}
+
+template <typename T>
+std::vector<T> cast_vec(const std::vector<double>& v)
+{
+    return std::vector<T>(v.begin(), v.end());
+}
+
+}  // namespace
+
+namespace {
+
+dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
+{
+    dgtd::MeshInput in;
+    in.K = mesh->K;
+    in.nv = mesh->nv;
+    in.vx = mesh->vx;
+    in.EToV = mesh->EToV;
+    in.face_bc = mesh->face_bc;
+    in.eps_r = mesh->eps_r;
+    in.mu_r = mesh->mu_r;
+    return in;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dgtd_last_error(void) { return g_last_error.c_str(); }
+const char* dgtd_version(void) { return "dgtd_hip 0.1 (gfx950)"; }
+
+int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** out)
+{
+    if (!mesh || !opts || !out) return fail(DGTD_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (opts->precision != 64 && opts->precision != 32)
+        return fail(DGTD_E_ARG, "precision must be 64 or 32");
+    if (opts->integrator != DGTD_RK4 && opts->integrator != DGTD_LSERK4)
+        return fail(DGTD_E_ARG, "integrator must be DGTD_RK4 or DGTD_LSERK4");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder)
+        return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    if (!std::isfinite(opts->alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
+    dgtd_op* op = nullptr;
+    try {
+        op = new dgtd_op;
+        op->device = opts->device;
+        op->precision = opts->precision;
+        op->integrator = opts->integrator;
+        op->alpha = opts->alpha;
+        op->R = dgtd::build_reftet(mesh->order);
+        op->K = mesh->K;
+        op->Np = op->R.Np;
+        op->Nfp = op->R.Nfp;
+        op->N = (size_t)op->Np * mesh->K;
+        op->ndofs = 6 * (int64_t)op->N;
+        dgtd::MeshInput in;
+        in.K = mesh->K;
+        in.nv = mesh->nv;
+        in.vx = mesh->vx;
+        in.EToV = mesh->EToV;
+        in.face_bc = mesh->face_bc;
+        in.eps_r = mesh->eps_r;
+        in.mu_r = mesh->mu_r;
+        dgtd::MeshData md = dgtd::build_mesh(in, op->R, op->alpha);
+        op->max_fscale = md.max_fscale;
+        op->nv = mesh->nv;
+        op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
+        op->EToV.assign(mesh->EToV, mesh->EToV + 4 * (size_t)mesh->K);
+
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&op->ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&op->ev1), "hipEventCreate");
+        std::vector<int> tabs(op->R.Fmask.begin(), op->R.Fmask.end());
+        tabs.insert(tabs.end(), op->R.face_perm.begin(), op->R.face_perm.end());
+        op->d_tabs = static_cast<int*>(upload(tabs));
+        op->d_conn = static_cast<int*>(upload(md.conn));
+        int bpc;
+        if (op->precision == 64) {
+            op->d_geo = upload(md.geo);
+            op->d_mats = upload(pack_mats<double>(op->R));
+            bpc = dgtd::stage_blocks_per_cu<double>(op->R.p);
+        } else {
+            op->d_geo = upload(cast_vec<float>(md.geo));
+            op->d_mats = upload(pack_mats<float>(op->R));
+            bpc = dgtd::stage_blocks_per_cu<float>(op->R.p);
+        }
+        op->d_M = static_cast<double*>(upload(op->R.M));
+        op->d_jac = static_cast<double*>(upload(md.jac));
+        op->d_eps = static_cast<double*>(upload(md.eps));
+        op->d_mu = static_cast<double*>(upload(md.mu));
+        op->nparts = std::max(1, std::min(1024, (op->K + 255) / 256));
+        hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+        hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+        const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
+        for (int b = 0; b < nbuf; ++b) {
+            hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
+            hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
+        }
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, op->device), "hipGetDeviceProperties");
+        const int epb = dgtd::elements_per_block(op->R.p);
+        op->nblk = (op->K + epb - 1) / epb;
+        op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * bpc));
+    } catch (const HipError& e) {
+        delete op;
+        return fail(DGTD_E_HIP, e.what());
+    } catch (const std::bad_alloc&) {
+        delete op;
+        return fail(DGTD_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        delete op;
+        return fail(DGTD_E_ARG, e.what());
+    }
+    *out = op;
+    return DGTD_OK;
+}
+
+void dgtd_op_destroy(dgtd_op* op) { delete op; }
+
+int dgtd_op_ndofs(const dgtd_op* op, int64_t* n)
+{
+    if (!op || !n) return fail(DGTD_E_ARG, "NULL argument");
+    *n = op->ndofs;
+    return DGTD_OK;
+}
+
+int dgtd_op_np(const dgtd_op* op, int32_t* Np)
+{
+    if (!op || !Np) return fail(DGTD_E_ARG, "NULL argument");
+    *Np = op->Np;
+    return DGTD_OK;
+}
+
+int dgtd_op_max_fscale(const dgtd_op* op, double* out)
+{
+    if (!op || !out) return fail(DGTD_E_ARG, "NULL argument");
+    *out = op->max_fscale;
+    return DGTD_OK;
+}
+
+int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
+{
+    if (!op || !xyz) return fail(DGTD_E_ARG, "NULL argument");
+    dgtd::MeshInput in;
+    in.K = op->K;
+    in.nv = op->nv;
+    in.vx = op->vx.data();
+    in.EToV = op->EToV.data();
+    dgtd::node_positions(in, op->R, xyz);
+    return DGTD_OK;
+}
+
+int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, void* stream)
+{
+    (void)t;
+    if (!op || !d_in || !d_out) return fail(DGTD_E_ARG, "NULL argument");
+    if (d_in == d_out) return fail(DGTD_E_ARG, "in and out must not alias");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : op->stream;
+        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, s);
+        else op->mult_dev<float>(d_in, d_out, s);
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out)
+{
+    (void)t;
+    if (!op || !in || !out) return fail(DGTD_E_ARG, "NULL argument");
+    void *d_in = nullptr, *d_out = nullptr;
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hip_check(hipMalloc(&d_in, op->real_size() * op->ndofs), "hipMalloc");
+        hip_check(hipMalloc(&d_out, op->real_size() * op->ndofs), "hipMalloc");
+        op->to_device(in, d_in);
+        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, op->stream);
+        else op->mult_dev<float>(d_in, d_out, op->stream);
+        op->to_host(d_out, out);
+    } catch (const std::exception& e) {
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+        return fail(DGTD_E_HIP, e.what());
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return DGTD_OK;
+}
+
+int dgtd_op_set_state(dgtd_op* op, const double* x, double t)
+{
+    if (!op || !x) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        op->cur = 0;
+        op->to_device(x, op->d_buf[0]);
+        // LSERK residual starts at zero (rk4a[0] = 0 makes the first stage ignore it)
+        if (op->integrator == DGTD_LSERK4)
+            hip_check(hipMemsetAsync(op->d_buf[2], 0, op->real_size() * op->ndofs, op->stream), "memset");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        op->t = t;
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_get_state(dgtd_op* op, double* x, double* t)
+{
+    if (!op || !x) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        op->to_host(op->state(), x);
+        if (t) *t = op->t;
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_advance(dgtd_op* op, double dt, int32_t nsteps)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    if (nsteps < 0 || !std::isfinite(dt)) return fail(DGTD_E_ARG, "bad dt or nsteps");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        if (op->timing) hip_check(hipEventRecord(op->ev0, op->stream), "hipEventRecord");
+        int launches = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            launches += op->step(dt);
+            op->t += dt;
+        }
+        if (op->timing) {
+            hip_check(hipEventRecord(op->ev1, op->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(op->ev1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, op->ev0, op->ev1), "hipEventElapsedTime");
+            op->last_launches = launches;
+            op->last_us = launches ? 1000.0 * ms / launches : 0.0;
+        }
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_run(dgtd_op* op, double dt, double final_time, int32_t* steps)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    if (!(dt > 0.0) || !std::isfinite(final_time)) return fail(DGTD_E_ARG, "bad dt or final_time");
+    int n = 0;
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        if (op->timing) hip_check(hipEventRecord(op->ev0, op->stream), "hipEventRecord");
+        int launches = 0;
+        while (op->t <= final_time - 1e-8 * dt) {
+            const double truedt = std::min(dt, final_time - op->t);
+            launches += op->step(truedt);
+            op->t += truedt;
+            ++n;
+        }
+        if (op->timing) {
+            hip_check(hipEventRecord(op->ev1, op->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(op->ev1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, op->ev0, op->ev1), "hipEventElapsedTime");
+            op->last_launches = launches;
+            op->last_us = launches ? 1000.0 * ms / launches : 0.0;
+        }
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    if (steps) *steps = n;
+    return DGTD_OK;
+}
+
+int dgtd_op_time(const dgtd_op* op, double* t)
+{
+    if (!op || !t) return fail(DGTD_E_ARG, "NULL argument");
+    *t = op->t;
+    return DGTD_OK;
+}
+
+int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps)
+{
+    if (!op || !x || !t) return fail(DGTD_E_ARG, "NULL argument");
+    int rc = dgtd_op_set_state(op, x, *t);
+    if (rc) return rc;
+    rc = dgtd_op_advance(op, dt, nsteps);
+    if (rc) return rc;
+    return dgtd_op_get_state(op, x, t);
+}
+
+int dgtd_op_norms(dgtd_op* op, double* out3)
+{
+    if (!op || !out3) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hipError_t e;
+        if (op->precision == 64)
+            e = dgtd::launch_norms<double>(static_cast<const double*>(op->state()), op->K, op->Np, op->d_M,
+                                           op->d_jac, op->d_eps, op->d_mu, op->d_partials, op->nparts,
+                                           op->d_out3, op->stream);
+        else
+            e = dgtd::launch_norms<float>(static_cast<const float*>(op->state()), op->K, op->Np, op->d_M,
+                                          op->d_jac, op->d_eps, op->d_mu, op->d_partials, op->nparts,
+                                          op->d_out3, op->stream);
+        hip_check(e, "norms kernel launch");
+        hip_check(hipMemcpyAsync(out3, op->d_out3, 3 * sizeof(double), hipMemcpyDeviceToHost, op->stream), "D2H");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_set_timing(dgtd_op* op, int32_t on)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    op->timing = on != 0;
+    return DGTD_OK;
+}
+
+int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    if (avg_us) *avg_us = op->last_us;
+    if (launches) *launches = op->last_launches;
+    return DGTD_OK;
+}
+
+int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    const double s = (double)op->real_size();
+    const double Np = op->Np, Nfp = op->Nfp;
+    if (bytes) *bytes = 24.0 * Np * s + 27.0 * s + 32.0;
+    if (flops) *flops = 36.0 * Np * Np + 48.0 * Np * Nfp + 240.0 * Nfp + 96.0 * Np;
+    return DGTD_OK;
+}
+
+int dgtd_cube_mesh(int32_t n, double L, const int8_t* side_bc, double* vx, int32_t* EToV, int8_t* face_bc)
+{
+    if (n < 1 || !(L > 0.0)) return fail(DGTD_E_ARG, "n must be >= 1 and L > 0");
+    dgtd::cube_mesh(n, L, side_bc, vx, EToV, face_bc);
+    return DGTD_OK;
+}
+
+int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* Ds, double* Dt, double* lift,
+                     int32_t* fmask)
+{
+    try {
+        const dgtd::RefTet R = dgtd::build_reftet(order);
+        if (rst)
+            for (size_t i = 0; i < R.xi.size(); ++i) rst[i] = 2.0 * R.xi[i] - 1.0;
+        if (M) std::copy(R.M.begin(), R.M.end(), M);
+        if (Dr) std::copy(R.Dr.begin(), R.Dr.end(), Dr);
+        if (Ds) std::copy(R.Ds.begin(), R.Ds.end(), Ds);
+        if (Dt) std::copy(R.Dt.begin(), R.Dt.end(), Dt);
+        if (lift) std::copy(R.LIFT.begin(), R.LIFT.end(), lift);
+        if (fmask) std::copy(R.Fmask.begin(), R.Fmask.end(), fmask);
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP, int8_t* face_codes)
+{
+    if (!mesh || !vmapM || !vmapP) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+        const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
+        const int Np = R.Np, Nfp = R.Nfp;
+        for (int e = 0; e < mesh->K; ++e)
+            for (int f = 0; f < 4; ++f) {
+                const int nb = md.conn[(size_t)dgtd::kConnStride * e + f];
+                const int code = md.conn[(size_t)dgtd::kConnStride * e + 4 + f];
+                if (face_codes) face_codes[4 * (size_t)e + f] = (int8_t)(nb >= 0 ? 0 : -nb - 1);
+                for (int b = 0; b < Nfp; ++b) {
+                    const size_t n = ((size_t)e * 4 + f) * Nfp + b;
+                    vmapM[n] = e * Np + R.Fmask[f * Nfp + b];
+                    vmapP[n] = nb >= 0 ? nb * Np + R.face_perm[(size_t)code * Nfp + b] : vmapM[n];
+                }
+            }
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo)
+{
+    if (!mesh || !geo) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+        const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
+        std::copy(md.geo.begin(), md.geo.end(), geo);
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+}  // extern "C"

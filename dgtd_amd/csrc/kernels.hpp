@@ -1,0 +1,56 @@
+// Device-kernel interface shared by the host code and the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dgtd {
+
+constexpr int kGeoStrideDev = 27;  // matches mesh.hpp kGeoStride
+constexpr int kConnStrideDev = 8;  // matches mesh.hpp kConnStride
+
+// Stage variants of the fused kernel.
+enum StageMode : int {
+    kModeMult = 0,   // out = L(qin)
+    kModeLserk = 1,  // res = ca*res + dt*L(qin); out = qin + cb*res
+    kModeRk1 = 2,    // k = L(x):  out(y) = x + ca k, res(z) = x + cb k   (qin = x)
+    kModeRk2 = 3,    // k = L(y):  out(y') = x + ca k, res(z) += cb k
+    kModeRk3 = 4,    // same as kModeRk2 with other coefficients
+    kModeRk4 = 5,    // k = L(y):  out(x) = res(z) + ca k
+};
+
+template <typename T>
+struct StageArgs {
+    int K;
+    int nblk;            // number of element blocks
+    const T* qin;        // operator input, 6 blocks of Np*K
+    T* out;              // see StageMode
+    T* res;              // LSERK residual / RK4 accumulator z
+    const T* x;          // RK4 base state (stages 2 and 3)
+    T ca, cb, dt, alpha;
+    const T* geo;        // kGeoStrideDev per element
+    const int* conn;     // kConnStrideDev per element
+    const T* mats;       // Dr^T, Ds^T, Dt^T ([j][i] each), then LIFT^T ([m][i])
+    const int* tabs;     // Fmask[4*Nfp], then face permutation table [96*Nfp]
+};
+
+template <typename T>
+hipError_t launch_stage(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+
+// Sets the kernels' LDS limits and returns resident blocks per CU.
+template <typename T>
+int stage_blocks_per_cu(int order);
+
+int elements_per_block(int order);
+
+// FP64 reductions over a state in T: per-block partials then a fixed-order
+// final pass (deterministic, no atomics).
+//   out[0] = sum of squares, out[1] = energy, out[2] = non-finite count.
+template <typename T>
+hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const double* jac,
+                        const double* eps, const double* mu, double* partials, int nparts,
+                        double* out3, hipStream_t s);
+
+}  // namespace dgtd

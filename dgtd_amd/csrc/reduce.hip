@@ -1,0 +1,103 @@
+// FP64 reductions over a state vector: DOF 2-norm (Fields::getNorml2,
+// src/evolution/Fields.h:21, used by the stability check of Solver::run,
+// Solver.cpp:500-516), electromagnetic energy and a non-finite count.
+// Fixed-order two-pass tree reduction, no atomics: bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace dgtd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ void block_reduce3(double v[3], double* sh)
+{
+    const int t = threadIdx.x;
+    for (int k = 0; k < 3; ++k) sh[k * kThreads + t] = v[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 3; ++k) sh[k * kThreads + t] += sh[k * kThreads + t + w];
+        __syncthreads();
+    }
+    for (int k = 0; k < 3; ++k) v[k] = sh[k * kThreads];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void norms_partial(const T* __restrict__ q, int K, int Np,
+                                                          const double* __restrict__ M,
+                                                          const double* __restrict__ jac,
+                                                          const double* __restrict__ eps,
+                                                          const double* __restrict__ mu,
+                                                          double* __restrict__ partials)
+{
+    extern __shared__ double sh[];  // Np*Np mass, then 3*kThreads scratch
+    double* sM = sh;
+    double* sR = sh + Np * Np;
+    for (int i = threadIdx.x; i < Np * Np; i += kThreads) sM[i] = M[i];
+    __syncthreads();
+    const size_t N = (size_t)Np * K;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = blockIdx.x * kThreads + threadIdx.x; e < K; e += gridDim.x * kThreads) {
+        double sE = 0.0, sH = 0.0;
+        for (int c = 0; c < 6; ++c) {
+            const T* u = q + c * N + (size_t)e * Np;
+            double s = 0.0;
+            for (int i = 0; i < Np; ++i) {
+                const double ui = (double)u[i];
+                acc[0] += ui * ui;
+                if (!isfinite(ui)) acc[2] += 1.0;
+                double r = 0.0;
+                for (int j = 0; j < Np; ++j) r += sM[i * Np + j] * (double)u[j];
+                s += ui * r;
+            }
+            if (c < 3) sE += s; else sH += s;
+        }
+        acc[1] += 0.5 * jac[e] * (eps[e] * sE + mu[e] * sH);
+    }
+    block_reduce3(acc, sR);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partials[3 * blockIdx.x + k] = acc[k];
+}
+
+__global__ __launch_bounds__(kThreads) void norms_final(const double* __restrict__ partials, int n,
+                                                        double* __restrict__ out)
+{
+    __shared__ double sR[3 * kThreads];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < n; b += kThreads)
+        for (int k = 0; k < 3; ++k) acc[k] += partials[3 * b + k];
+    block_reduce3(acc, sR);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(acc[0]);
+        out[1] = acc[1];
+        out[2] = acc[2];
+    }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const double* jac,
+                        const double* eps, const double* mu, double* partials, int nparts,
+                        double* out3, hipStream_t s)
+{
+    const size_t sh = sizeof(double) * ((size_t)Np * Np + 3 * kThreads);
+    hipLaunchKernelGGL(norms_partial<T>, dim3(nparts), dim3(kThreads), sh, s, q, K, Np, Mref, jac, eps,
+                       mu, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(norms_final, dim3(1), dim3(kThreads), 0, s, partials, nparts, out3);
+    return hipGetLastError();
+}
+
+template hipError_t launch_norms<double>(const double*, int, int, const double*, const double*,
+                                         const double*, const double*, double*, int, double*,
+                                         hipStream_t);
+template hipError_t launch_norms<float>(const float*, int, int, const double*, const double*,
+                                        const double*, const double*, double*, int, double*,
+                                        hipStream_t);
+
+}  // namespace dgtd

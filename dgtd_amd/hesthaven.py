@@ -1,0 +1,245 @@
+"""Host-side mirror of the reference's operator and solver surfaces over the C ABI.
+
+- HesthavenEvolution mirrors maxwell::HesthavenEvolution
+  (src/evolution/HesthavenEvolution.h:14-50): an mfem::TimeDependentOperator
+  with Mult(in, out), SetTime/GetTime and height 6*NDofs.
+- Solver mirrors maxwell::Solver's run/step/getTimeStep for the hesthaven
+  operator (src/solver/Solver.cpp:79-149, 302-354, 483-551), with the state
+  resident on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, dptr, i32ptr, i8ptr, lib
+
+INTEGRATORS = {"rk4": _abi.DGTD_RK4, "lserk4": _abi.DGTD_LSERK4}
+
+
+def num_nodes(order):
+    return (order + 1) * (order + 2) * (order + 3) // 6
+
+
+def num_face_nodes(order):
+    return (order + 1) * (order + 2) // 2
+
+
+def cube_mesh(n, L=1.0, bc="pec"):
+    """n^3 hexahedra on [0,L]^3, six tetrahedra each (library generator).
+
+    bc: one of none/pec/pmc/sma for all sides or a list of six codes
+    (x=0, x=L, y=0, y=L, z=0, z=L).  Returns (vx, EToV, face_bc)."""
+    names = {"none": 0, "pec": 1, "pmc": 2, "sma": 3}
+    sides = [bc] * 6 if isinstance(bc, (str, int)) else list(bc)
+    side = np.array([names[s] if isinstance(s, str) else int(s) for s in sides], dtype=np.int8)
+    K = 6 * n ** 3
+    vx = np.zeros(3 * (n + 1) ** 3)
+    EToV = np.zeros(4 * K, dtype=np.int32)
+    face_bc = np.zeros(4 * K, dtype=np.int8)
+    check(lib().dgtd_cube_mesh(n, float(L), i8ptr(side), dptr(vx), i32ptr(EToV), i8ptr(face_bc)))
+    return vx.reshape(-1, 3), EToV.reshape(-1, 4), face_bc.reshape(-1, 4)
+
+
+class _Mesh:
+    """Keeps the numpy arrays alive while the C descriptor points at them."""
+
+    def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None):
+        self.vx = np.ascontiguousarray(vx, dtype=np.float64).reshape(-1, 3)
+        self.EToV = np.ascontiguousarray(EToV, dtype=np.int32).reshape(-1, 4)
+        self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, 4)
+        self.eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+        self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        self.desc = _abi.MeshDesc(self.EToV.shape[0], order, dptr(self.vx), self.vx.shape[0],
+                                  i32ptr(self.EToV), i8ptr(self.face_bc), dptr(self.eps),
+                                  dptr(self.mu))
+
+
+def ref_element(order):
+    """The library's reference matrices (host FP64 setup)."""
+    Np, Nfp = num_nodes(order), num_face_nodes(order)
+    out = dict(rst=np.zeros((Np, 3)), M=np.zeros((Np, Np)), Dr=np.zeros((Np, Np)),
+               Ds=np.zeros((Np, Np)), Dt=np.zeros((Np, Np)), LIFT=np.zeros((Np, 4 * Nfp)),
+               Fmask=np.zeros(4 * Nfp, dtype=np.int32))
+    check(lib().dgtd_ref_element(order, dptr(out["rst"]), dptr(out["M"]), dptr(out["Dr"]),
+                                 dptr(out["Ds"]), dptr(out["Dt"]), dptr(out["LIFT"]),
+                                 i32ptr(out["Fmask"])))
+    return out
+
+
+def mesh_vmaps(vx, EToV, face_bc, order, alpha=1.0):
+    """(vmapM, vmapP, face_codes) exactly as the reference's Connectivities::global."""
+    m = _Mesh(vx, EToV, face_bc, order)
+    K = m.EToV.shape[0]
+    n = 4 * num_face_nodes(order) * K
+    M = np.zeros(n, dtype=np.int32)
+    P = np.zeros(n, dtype=np.int32)
+    codes = np.zeros(4 * K, dtype=np.int8)
+    check(lib().dgtd_mesh_vmaps(ctypes.byref(m.desc), float(alpha), i32ptr(M), i32ptr(P),
+                                i8ptr(codes)))
+    return M, P, codes.reshape(K, 4)
+
+
+def mesh_geometry(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
+    m = _Mesh(vx, EToV, face_bc, order, eps, mu)
+    geo = np.zeros(27 * m.EToV.shape[0])
+    check(lib().dgtd_mesh_geometry(ctypes.byref(m.desc), float(alpha), dptr(geo)))
+    return geo.reshape(-1, 27)
+
+
+class HesthavenEvolution:
+    """GPU hesthaven operator: Mult(in, out) over 6*Np*K FP64 host vectors.
+
+    Construction errors raise DgtdError, as the reference constructor throws
+    std::runtime_error (HesthavenEvolution.cpp:37,42,146)."""
+
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, precision=64,
+                 integrator="lserk4", device=0):
+        self.order = order
+        self.precision = precision
+        self.integrator = integrator
+        self.alpha = alpha
+        mesh = _Mesh(vx, EToV, face_bc, order, eps, mu)
+        self.K = mesh.EToV.shape[0]
+        self.Np = num_nodes(order)
+        self.Nfp = num_face_nodes(order)
+        opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device), 0)
+        h = ctypes.c_void_p()
+        check(lib().dgtd_op_create(ctypes.byref(mesh.desc), ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        check(lib().dgtd_op_ndofs(h, ctypes.byref(n)))
+        self.ndofs = n.value
+        self._t = 0.0
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().dgtd_op_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # --- mfem::TimeDependentOperator surface ---
+    def Height(self):
+        return self.ndofs
+
+    def Width(self):
+        return self.ndofs
+
+    def SetTime(self, t):
+        self._t = float(t)
+
+    def GetTime(self):
+        return self._t
+
+    def Mult(self, x, y):
+        """y = L(x); y is overwritten (HesthavenEvolution.cpp:461-468,532-533)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.ndofs or y.size != self.ndofs or y.dtype != np.float64 or not y.flags.c_contiguous:
+            raise ValueError("Mult expects contiguous float64 vectors of length Height()")
+        check(lib().dgtd_op_mult(self._h, self._t, dptr(x), dptr(y)))
+
+    def mult(self, x):
+        y = np.empty(self.ndofs)
+        self.Mult(x, y)
+        return y
+
+    def mult_device(self, d_in, d_out, stream=None):
+        """Device pointers (ints) in the operator precision; asynchronous."""
+        check(lib().dgtd_op_mult_device(self._h, self._t, ctypes.c_void_p(d_in),
+                                        ctypes.c_void_p(d_out), ctypes.c_void_p(stream or 0)))
+
+    # --- setup data ---
+    def max_fscale(self):
+        v = ctypes.c_double()
+        check(lib().dgtd_op_max_fscale(self._h, ctypes.byref(v)))
+        return v.value
+
+    def node_positions(self):
+        xyz = np.zeros(3 * self.Np * self.K)
+        check(lib().dgtd_op_node_positions(self._h, dptr(xyz)))
+        return xyz.reshape(3, self.K, self.Np)
+
+    def work_per_element(self):
+        b, f = ctypes.c_double(), ctypes.c_double()
+        check(lib().dgtd_op_work_per_element(self._h, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+    # --- resident state ---
+    def set_state(self, x, t=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(lib().dgtd_op_set_state(self._h, dptr(x), float(t)))
+        self._t = float(t)
+
+    def get_state(self):
+        x = np.empty(self.ndofs)
+        t = ctypes.c_double()
+        check(lib().dgtd_op_get_state(self._h, dptr(x), ctypes.byref(t)))
+        self._t = t.value
+        return x
+
+    def advance(self, dt, nsteps):
+        check(lib().dgtd_op_advance(self._h, float(dt), int(nsteps)))
+        t = ctypes.c_double()
+        check(lib().dgtd_op_time(self._h, ctypes.byref(t)))
+        self._t = t.value
+
+    def run(self, dt, final_time):
+        n = ctypes.c_int32()
+        check(lib().dgtd_op_run(self._h, float(dt), float(final_time), ctypes.byref(n)))
+        t = ctypes.c_double()
+        check(lib().dgtd_op_time(self._h, ctypes.byref(t)))
+        self._t = t.value
+        return n.value
+
+    def norms(self):
+        out = np.zeros(3)
+        check(lib().dgtd_op_norms(self._h, dptr(out)))
+        return {"norm2": out[0], "energy": out[1], "nonfinite": int(out[2])}
+
+    def set_timing(self, on=True):
+        check(lib().dgtd_op_set_timing(self._h, 1 if on else 0))
+
+    def last_kernel_us(self):
+        us = ctypes.c_double()
+        n = ctypes.c_int32()
+        check(lib().dgtd_op_last_kernel_us(self._h, ctypes.byref(us), ctypes.byref(n)))
+        return us.value, n.value
+
+
+def estimate_time_step(op, cfl=1.0):
+    """Hesthaven 3D rule of Solver::estimateTimeStep (Solver.cpp:334-346):
+    dt = cfl / (max fscale * p^2)."""
+    return 1.0 * cfl / (op.max_fscale() * op.order * op.order)
+
+
+class Solver:
+    """Solver::run / step over the GPU operator with the state resident on device.
+
+    time_step = 0 selects the automatic rule, as SolverOptions::time_step = 0
+    does (Solver.cpp:111-116)."""
+
+    def __init__(self, op, initial_state, final_time=2.0, time_step=0.0, cfl=1.0):
+        self.op = op
+        self.final_time = final_time
+        self.dt = estimate_time_step(op, cfl) if time_step == 0.0 else time_step
+        self.time = 0.0
+        op.set_state(initial_state, 0.0)
+
+    def step(self):
+        truedt = min(self.dt, self.final_time - self.time)
+        self.op.advance(truedt, 1)
+        self.time += truedt
+
+    def run(self):
+        steps = self.op.run(self.dt, self.final_time)
+        self.time = self.op.GetTime()
+        return steps
+
+    def getFields(self):
+        return self.op.get_state()
+
+    def getTimeStep(self):
+        return self.dt

@@ -1,0 +1,178 @@
+/*
+ * dgtd_hip.h -- C ABI of the MI355X (gfx950) implementation of OpenSEMBA/dgtd's
+ * "hesthaven" DGTD Maxwell evolution operator and its Runge-Kutta stage update.
+ *
+ * The library replaces, for one GPU:
+ *   - maxwell::HesthavenEvolution (src/evolution/HesthavenEvolution.h:14-50),
+ *     its constructor (HesthavenEvolution.cpp:315-437) and
+ *     Mult(const mfem::Vector&, mfem::Vector&) const (HesthavenEvolution.cpp:450-542);
+ *   - the RK stage arithmetic of mfem::RK4Solver as driven by Solver::step /
+ *     Solver::run (src/solver/Solver.cpp:483-551, RK4 chosen at :45-47);
+ *   - the Hesthaven 3D time-step rule (Solver.cpp:334-346), via max fscale.
+ *
+ * Conventions
+ *   - State vectors use the reference layout (src/evolution/Fields.h:48-56):
+ *     x[c*N + e*Np + i], c in {Ex,Ey,Ez,Hx,Hy,Hz}, N = Np*K, Np = (p+1)(p+2)(p+3)/6.
+ *   - Host entry points take host pointers and FP64 values whatever the
+ *     operator's precision; *_device entry points take device pointers in the
+ *     operator's precision (double for 64, float for 32).
+ *   - Every function returns 0 on success or a negative DGTD_E* code; the
+ *     message of the last failure on the calling thread is dgtd_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - A handle owns its device buffers and one HIP stream.  Calls on one handle
+ *     must be serialised by the caller; different handles are independent.
+ *     The library keeps no caller pointer after a call returns.
+ */
+#ifndef DGTD_HIP_H
+#define DGTD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGTD_OK 0
+#define DGTD_E_ARG (-1)         /* bad argument */
+#define DGTD_E_NOMEM (-2)       /* device or host allocation failed */
+#define DGTD_E_HIP (-3)         /* HIP runtime error */
+#define DGTD_E_UNSUPPORTED (-4) /* feature outside this implementation */
+#define DGTD_E_STATE (-5)       /* call not valid in the handle's state */
+#define DGTD_E_CAPACITY (-6)    /* admission refused: HBM budget or slots */
+
+/* Face boundary codes (face f of a tetrahedron is opposite local vertex f,
+ * as MFEM's tetrahedron face table orders them).  Mirrors BdrCond PEC/PMC/SMA
+ * (src/components/Types.h:47-55) as applied in
+ * HesthavenEvolution::applyBoundaryConditionsToNodes (HesthavenEvolution.cpp:275-313). */
+#define DGTD_BC_NONE 0 /* interior face, or boundary face with zero jump */
+#define DGTD_BC_PEC 1
+#define DGTD_BC_PMC 2
+#define DGTD_BC_SMA 3
+
+/* Integrators. */
+#define DGTD_RK4 0    /* mfem::RK4Solver stage order (the reference's default) */
+#define DGTD_LSERK4 1 /* Carpenter-Kennedy 5-stage low-storage RK4 */
+
+typedef struct dgtd_op dgtd_op;
+
+/* Mesh description; host arrays borrowed for the duration of dgtd_op_create.
+ * Replaces the (ParFiniteElementSpace, Model) pair of the reference
+ * constructor (HesthavenEvolution.cpp:315): affine tetrahedra only. */
+typedef struct {
+    int32_t K;              /* number of tetrahedra */
+    int32_t order;          /* polynomial order p, 1..5 */
+    const double* vx;       /* 3*nv vertex coordinates (x,y,z interleaved) */
+    int32_t nv;             /* number of vertices */
+    const int32_t* EToV;    /* 4*K vertex ids; local vertex order fixes node/face order */
+    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes, NULL = all DGTD_BC_NONE */
+    const double* eps_r;    /* K relative permittivity, NULL = 1 */
+    const double* mu_r;     /* K relative permeability, NULL = 1 */
+} dgtd_mesh_desc;
+
+/* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)
+ * plus the implementation's precision and integrator. */
+typedef struct {
+    double alpha;       /* upwind weight: 1 upwind, 0 centred (EvolutionOptions::alpha) */
+    int32_t precision;  /* 64 or 32 */
+    int32_t integrator; /* DGTD_RK4 or DGTD_LSERK4 */
+    int32_t device;     /* HIP device ordinal */
+    int32_t reserved;
+} dgtd_opts;
+
+const char* dgtd_last_error(void);
+const char* dgtd_version(void);
+
+/* Build the operator: geometry, connectivity, reference matrices (FP64 setup),
+ * device upload.  Replaces HesthavenEvolution::HesthavenEvolution
+ * (HesthavenEvolution.cpp:315-437) and Connectivities (HesthavenEvolutionMethods.cpp:501-534). */
+int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** out);
+void dgtd_op_destroy(dgtd_op* op);
+
+/* Sizes: ndofs = 6*Np*K, the operator height (HesthavenEvolution.cpp:316). */
+int dgtd_op_ndofs(const dgtd_op* op, int64_t* ndofs);
+int dgtd_op_np(const dgtd_op* op, int32_t* Np);
+
+/* Largest fscale over all element faces; Solver::estimateTimeStep reads it
+ * through getHesthavenElement(e).fscale (Solver.cpp:337-340). */
+int dgtd_op_max_fscale(const dgtd_op* op, double* out);
+
+/* Physical node positions, 3*Np*K doubles: x block, y block, z block. */
+int dgtd_op_node_positions(const dgtd_op* op, double* xyz);
+
+/* out = L(in) -- HesthavenEvolution::Mult (HesthavenEvolution.cpp:450-542).
+ * Host pointers, 6*Np*K doubles each; `t` mirrors TimeDependentOperator::SetTime
+ * (used by the reference only for TF/SF sources, which are not supported). */
+int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out);
+
+/* Same on device pointers in the operator's precision, on `stream` (a
+ * hipStream_t; NULL = the handle's stream).  Asynchronous. */
+int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, void* stream);
+
+/* Device-resident state: upload / download the state vector (host FP64). */
+int dgtd_op_set_state(dgtd_op* op, const double* x, double t);
+int dgtd_op_get_state(dgtd_op* op, double* x, double* t);
+
+/* Advance the resident state by nsteps steps of size dt with the handle's
+ * integrator; stage updates are fused into the operator kernel (one read and
+ * one write of each state buffer per stage).  Mirrors Solver::step
+ * (Solver.cpp:535-551) without the final-time truncation. */
+int dgtd_op_advance(dgtd_op* op, double dt, int32_t nsteps);
+
+/* Solver::run (Solver.cpp:483-533) on the resident state: steps of dt while
+ * t <= final_time - 1e-8*dt, the last truncated to final_time - t (:537).
+ * Writes the number of steps taken. */
+int dgtd_op_run(dgtd_op* op, double dt, double final_time, int32_t* steps);
+
+/* Time of the resident state (TimeDependentOperator::GetTime after a run). */
+int dgtd_op_time(const dgtd_op* op, double* t);
+
+/* Host convenience: x (host FP64) advanced by nsteps steps; t updated. */
+int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps);
+
+/* FP64 reductions on the resident state (fixed-order, no atomics):
+ * out[0] = DOF 2-norm (Fields::getNorml2, Fields.h:21),
+ * out[1] = energy 0.5*sum_e J_e (eps E^T M E + mu H^T M H),
+ * out[2] = 1 if any entry is non-finite, else 0. */
+int dgtd_op_norms(dgtd_op* op, double* out3);
+
+/* Kernel timing of the last dgtd_op_advance/dgtd_op_run call: average
+ * operator-kernel duration in microseconds over its launches, measured with
+ * HIP events on the handle's stream (0 when timing is off). */
+int dgtd_op_set_timing(dgtd_op* op, int32_t on);
+int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches);
+
+/* Algorithmic HBM bytes and flops per element per stage for this operator. */
+int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops);
+
+/* Structured cube generator: n^3 hexahedra on [0,L]^3, each split into six
+ * tetrahedra around the main diagonal; negative tetrahedra get local vertices
+ * 2 and 3 swapped.  side_bc[6] = codes for x=0, x=L, y=0, y=L, z=0, z=L.
+ * Sizes: vx 3*(n+1)^3, EToV 4*6n^3, face_bc 4*6n^3 (any may be NULL to query). */
+int dgtd_cube_mesh(int32_t n, double L, const int8_t* side_bc, double* vx, int32_t* EToV,
+                   int8_t* face_bc);
+
+/* Host-only introspection of the setup (no GPU needed). */
+
+/* Reference matrices of order p on the Hesthaven tetrahedron (volume 4/3),
+ * nodes in the MFEM L2 GaussLobatto order (HesthavenEvolution.cpp:46-81,365):
+ * rst 3*Np, M/Dr/Ds/Dt Np*Np, lift Np*4Nfp (row-major), fmask 4*Nfp; any NULL. */
+int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* Ds, double* Dt,
+                     double* lift, int32_t* fmask);
+
+/* Expands the compact connectivity into the reference's (vmapM, vmapP) lists,
+ * 4*Nfp*K each, in Connectivities::global order (HesthavenEvolutionMethods.cpp
+ * :721-791): element, local face, face node ascending; P = M on boundary faces.
+ * face_codes (4*K, may be NULL) receives the boundary code of each face
+ * (0 for interior faces). */
+int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP,
+                    int8_t* face_codes);
+
+/* Per-element geometric factors, 27 doubles per element: dr/dx..dt/dz (9),
+ * per face nx ny nz fscale/2 (16), 1/eps_r, 1/mu_r. */
+int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
