@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the hesthaven DGTD operator + fused LSERK4 stage on one MI355X.
+
+Workload (a "step" is one LSERK4 time step = 5 fused stage launches over the
+whole mesh): PEC cavity [0,1]^3, 32^3 hexahedra x 6 = 196,608 tetrahedra,
+order 4 (Np = 35, 41,287,680 DOF), initial fields a seeded Gaussian pulse
+(centre uniform in [0.3,0.7]^3, width 0.05, amplitude 1 on each of the six
+components), dt from the Hesthaven rule dt = 1/(max fscale * p^2).
+
+The headline value is FP32 DOF*RK-stages/s; the FP64 path runs the same steps
+beside it and the relative L2 difference FP32 vs FP64 is reported.  roofline
+uses the algorithmic bytes per element per stage (24*Np*s + 27*s + 32) times
+K over the average stage-kernel duration from HIP events on the operator's
+stream.  cpu_baseline times the CPU FP64 oracle (a restatement of the
+reference algorithm) with OpenMP on the host cores for one LSERK4 step of the
+same mesh.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def gaussian_all_components(pos, seed, spread=0.05):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.3, 0.7, 3)
+    r2 = (pos[0] - c[0]) ** 2 + (pos[1] - c[1]) ** 2 + (pos[2] - c[2]) ** 2
+    g = np.exp(-r2 / (2.0 * spread ** 2))
+    return np.tile(g.ravel(), 6), c
+
+
+def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup):
+    op = D.HesthavenEvolution(vx, E, fbc, order, precision=precision, integrator="lserk4")
+    op.set_state(q0, 0.0)
+    if warmup:
+        op.advance(dt, warmup)
+    op.set_timing(True)
+    t0 = time.perf_counter()
+    op.advance(dt, steps)  # returns after the operator's stream has drained
+    wall = time.perf_counter() - t0
+    kern_us, launches = op.last_kernel_us()
+    state = op.get_state()
+    nm = op.norms()
+    bytes_el, flops_el = op.work_per_element()
+    op.close()
+    return dict(wall=wall, kern_us=kern_us, launches=launches, state=state, norms=nm,
+                bytes_el=bytes_el, flops_el=flops_el)
+
+
+def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0):
+    """LSERK4 steps (5 operator applications each) of the oracle on the host,
+    as many as fit in about budget_s seconds (at least one)."""
+    from oracle import oracle as O
+
+    ref = O.Operator(vx, E.reshape(-1, 4), fbc.reshape(-1, 4).astype(np.int32), order)
+    q = q0.copy()
+    res = np.zeros_like(q)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ref.lserk4_step(q, res, steps * dt, dt)
+        steps += 1
+        wall = time.perf_counter() - t0
+        if wall >= budget_s or wall / steps * (steps + 1) > 2.0 * budget_s:
+            break
+    return wall, steps, O.threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32, help="hexahedra per cube edge")
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note, not an implementation)"}))
+        sys.exit(2)
+
+    import dgtd_amd as D
+
+    n, order = args.n, args.order
+    vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
+    K = E.shape[0]
+    Np = D.num_nodes(order)
+    dof = 6 * Np * K
+    probe = D.HesthavenEvolution(vx, E, fbc, order, precision=64)
+    pos = probe.node_positions()
+    dt = D.estimate_time_step(probe)
+    probe.close()
+    q0, centre = gaussian_all_components(pos, args.seed)
+
+    r32 = time_path(D, vx, E, fbc, order, 32, q0, dt, args.steps, args.warmup)
+    stages = 5 * args.steps
+    value = dof * stages / r32["wall"]
+    kern_s = r32["kern_us"] * 1e-6
+    achieved_gbs = r32["bytes_el"] * K / kern_s / 1e9
+    achieved_tf = r32["flops_el"] * K / kern_s / 1e12
+    t_star = max(r32["bytes_el"] * K / (HBM_PEAK_GBS * 1e9), r32["flops_el"] * K / (FP32_PEAK_TFLOPS * 1e12))
+    out = {
+        "metric": "DOF*RK-stages/s (FP32 fused LSERK4 stage, hesthaven DGTD operator)",
+        "value": value,
+        "unit": "DOF*RK-stages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian pulse, structured PEC cavity)",
+        "config": {
+            "workload": f"PEC cavity [0,1]^3, {n}^3 hexes x 6 = {K} tets, order {order} (Np={Np}), "
+                        f"{dof} DOF, LSERK4 dt={dt:.6e}, Gaussian width 0.05 centre "
+                        f"({centre[0]:.4f},{centre[1]:.4f},{centre[2]:.4f}) on all 6 components",
+            "K": K, "order": order, "dof": dof, "dt": dt, "integrator": "lserk4",
+            "stages_per_step": 5,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "dg_stage_kernel<4,float,LSERK>",
+            "kernel_avg_us": r32["kern_us"],
+            "launches": r32["launches"],
+            "algorithmic_bytes_per_launch": r32["bytes_el"] * K,
+            "algorithmic_flops_per_launch": r32["flops_el"] * K,
+            "achieved_tflops": achieved_tf,
+            "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+            "attainable_frac": t_star / kern_s,
+        },
+    }
+    if not args.no_fp64:
+        r64 = time_path(D, vx, E, fbc, order, 64, q0, dt, args.steps, args.warmup)
+        v64 = dof * stages / r64["wall"]
+        k64 = r64["kern_us"] * 1e-6
+        out["fp64"] = {
+            "value": v64,
+            "ms_per_step": 1e3 * r64["wall"] / args.steps,
+            "kernel_avg_us": r64["kern_us"],
+            "hbm_frac": r64["bytes_el"] * K / k64 / 1e9 / HBM_PEAK_GBS,
+            "achieved_tflops": r64["flops_el"] * K / k64 / 1e12,
+        }
+        out["fp32_over_fp64"] = value / v64
+        a, b = r32["state"], r64["state"]
+        out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        out["energy_fp64"] = r64["norms"]["energy"]
+    out["energy_fp32"] = r32["norms"]["energy"]
+    if not args.no_cpu:
+        wall, csteps, threads = cpu_baseline(vx, E, fbc, order, q0, dt)
+        out["cpu_baseline"] = {
+            "value": dof * 5 * csteps / wall,
+            "unit": "DOF*RK-stages/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{csteps} LSERK4 steps (5 stages each) of the same {K}-tet p={order} mesh, "
+                      f"FP64 CPU oracle, OpenMP {threads} threads, {wall:.2f} s",
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
