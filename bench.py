@@ -39,8 +39,10 @@ def gaussian_all_components(pos, seed, spread=0.05):
     return np.tile(g.ravel(), 6), c
 
 
-def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup):
-    op = D.HesthavenEvolution(vx, E, fbc, order, precision=precision, integrator="lserk4")
+def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup, kernel="auto"):
+    op = D.HesthavenEvolution(vx, E, fbc, order, precision=precision, integrator="lserk4",
+                              kernel=kernel)
+    variant = op.kernel_variant()
     op.set_state(q0, 0.0)
     if warmup:
         op.advance(dt, warmup)
@@ -54,7 +56,7 @@ def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup):
     bytes_el, flops_el = op.work_per_element()
     op.close()
     return dict(wall=wall, kern_us=kern_us, launches=launches, state=state, norms=nm,
-                bytes_el=bytes_el, flops_el=flops_el)
+                bytes_el=bytes_el, flops_el=flops_el, variant=variant)
 
 
 def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0):
@@ -86,7 +88,19 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (HBM-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+
+        src = torch.ones(args.copy_calib, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": args.copy_calib}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not an implementation)"}))
@@ -105,7 +119,7 @@ def main():
     probe.close()
     q0, centre = gaussian_all_components(pos, args.seed)
 
-    r32 = time_path(D, vx, E, fbc, order, 32, q0, dt, args.steps, args.warmup)
+    r32 = time_path(D, vx, E, fbc, order, 32, q0, dt, args.steps, args.warmup, args.kernel)
     stages = 5 * args.steps
     value = dof * stages / r32["wall"]
     kern_s = r32["kern_us"] * 1e-6
@@ -139,7 +153,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "dg_stage_kernel<4,float,LSERK>",
+            "kernel": ("dg_stage_mfma" if r32["variant"] == "mfma" else "dg_stage_kernel")
+                      + f"<{order},float,LSERK>",
             "kernel_avg_us": r32["kern_us"],
             "launches": r32["launches"],
             "algorithmic_bytes_per_launch": r32["bytes_el"] * K,
@@ -150,13 +165,14 @@ def main():
         },
     }
     if not args.no_fp64:
-        r64 = time_path(D, vx, E, fbc, order, 64, q0, dt, args.steps, args.warmup)
+        r64 = time_path(D, vx, E, fbc, order, 64, q0, dt, args.steps, args.warmup, args.kernel)
         v64 = dof * stages / r64["wall"]
         k64 = r64["kern_us"] * 1e-6
         out["fp64"] = {
             "value": v64,
             "ms_per_step": 1e3 * r64["wall"] / args.steps,
             "kernel_avg_us": r64["kern_us"],
+            "kernel_variant": r64["variant"],
             "hbm_frac": r64["bytes_el"] * K / k64 / 1e9 / HBM_PEAK_GBS,
             "achieved_tflops": r64["flops_el"] * K / k64 / 1e12,
         }

@@ -20,6 +20,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 DGTD_OK = 0
 DGTD_BC_NONE, DGTD_BC_PEC, DGTD_BC_PMC, DGTD_BC_SMA = 0, 1, 2, 3
 DGTD_RK4, DGTD_LSERK4 = 0, 1
+DGTD_KERNEL_AUTO, DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA = 0, 1, 2
 
 
 class DgtdError(RuntimeError):
@@ -35,7 +36,7 @@ class MeshDesc(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("alpha", c_double), ("precision", c_int32), ("integrator", c_int32),
-                ("device", c_int32), ("reserved", c_int32)]
+                ("device", c_int32), ("kernel", c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/dgtd_hip.h.
@@ -60,6 +61,7 @@ SIGNATURES = {
     "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
     "dgtd_op_set_timing": (c_int32, [c_void_p, c_int32]),
     "dgtd_op_last_kernel_us": (c_int32, [c_void_p, _dp, _i32p]),
+    "dgtd_op_kernel_variant": (c_int32, [c_void_p, _i32p]),
     "dgtd_op_work_per_element": (c_int32, [c_void_p, _dp, _dp]),
     "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
     "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),

@@ -162,24 +162,32 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<T> a)
                 rhs[x] = (Dd(y, 3 + z) - Dd(z, 3 + y) + lf[x]) * g[25];
             }
             const size_t idx = (size_t)(e0 + el) * NP + i;
+            // all reads before the first store (see the MFMA variant)
+            T rv[6], xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
+                    rv[c] = a.res[c * N + idx];
+                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
+            }
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
                 const size_t o = c * N + idx;
+                const T qv = sQ[c * EB * NP + el * NP + i];
                 if constexpr (MODE == kModeMult) {
                     a.out[o] = rhs[c];
                 } else if constexpr (MODE == kModeLserk) {
-                    const T r = a.ca * a.res[o] + a.dt * rhs[c];
+                    const T r = a.ca * rv[c] + a.dt * rhs[c];
                     a.res[o] = r;
-                    a.out[o] = sQ[c * EB * NP + el * NP + i] + a.cb * r;
+                    a.out[o] = qv + a.cb * r;
                 } else if constexpr (MODE == kModeRk1) {
-                    const T xv = sQ[c * EB * NP + el * NP + i];
-                    a.out[o] = xv + a.ca * rhs[c];
-                    a.res[o] = xv + a.cb * rhs[c];
+                    a.out[o] = qv + a.ca * rhs[c];
+                    a.res[o] = qv + a.cb * rhs[c];
                 } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                    a.out[o] = a.x[o] + a.ca * rhs[c];
-                    a.res[o] = a.res[o] + a.cb * rhs[c];
+                    a.out[o] = xv[c] + a.ca * rhs[c];
+                    a.res[o] = rv[c] + a.cb * rhs[c];
                 } else {  // kModeRk4
-                    a.out[o] = a.res[o] + a.ca * rhs[c];
+                    a.out[o] = rv[c] + a.ca * rhs[c];
                 }
             }
         }

@@ -72,6 +72,9 @@ struct dgtd_op {
     void* d_geo = nullptr;
     int* d_conn = nullptr;
     void* d_mats = nullptr;
+    void* d_mats_mfma = nullptr;
+    int variant = DGTD_KERNEL_VALU;
+    int grid_mfma = 0;
     int* d_tabs = nullptr;
     double* d_M = nullptr;
     double* d_jac = nullptr;
@@ -96,7 +99,7 @@ struct dgtd_op {
     ~dgtd_op()
     {
         (void)hipSetDevice(device);
-        for (void* p : {d_geo, (void*)d_conn, d_mats, (void*)d_tabs, (void*)d_M, (void*)d_jac,
+        for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_tabs, (void*)d_M, (void*)d_jac,
                         (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
@@ -120,9 +123,44 @@ struct dgtd_op {
     }
 
     template <typename T>
+    hipError_t launch_on(const dgtd::StageArgs<T>& a0, int mode, int var, hipStream_t s) const
+    {
+        if (var == DGTD_KERNEL_MFMA) {
+            dgtd::StageArgs<T> a = a0;
+            a.mats = static_cast<const T*>(d_mats_mfma);
+            return dgtd::launch_stage_mfma<T>(R.p, a, mode, grid_mfma, s);
+        }
+        return dgtd::launch_stage<T>(R.p, a0, mode, grid, s);
+    }
+
+    template <typename T>
     void launch(const dgtd::StageArgs<T>& a, int mode)
     {
-        hip_check(dgtd::launch_stage<T>(R.p, a, mode, grid, stream), "stage kernel launch");
+        hip_check(launch_on<T>(a, mode, variant, stream), "stage kernel launch");
+    }
+
+    // Times each available variant (operator application on the zeroed state
+    // buffers) and keeps the fastest.
+    template <typename T>
+    void autotune()
+    {
+        if (grid_mfma <= 0) { variant = DGTD_KERNEL_VALU; return; }
+        auto a = base_args<T>();
+        a.qin = static_cast<const T*>(d_buf[0]);
+        a.out = static_cast<T*>(d_buf[1]);
+        float best = 1e30f;
+        int best_var = DGTD_KERNEL_VALU;
+        for (int var : {DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA}) {
+            hip_check(launch_on<T>(a, dgtd::kModeMult, var, stream), "autotune launch");
+            hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+            for (int r = 0; r < 3; ++r) hip_check(launch_on<T>(a, dgtd::kModeMult, var, stream), "autotune launch");
+            hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+            if (ms < best) { best = ms; best_var = var; }
+        }
+        variant = best_var;
     }
 
     template <typename T>
@@ -168,7 +206,7 @@ struct dgtd_op {
         auto a = base_args<T>();
         a.qin = static_cast<const T*>(in);
         a.out = static_cast<T*>(out);
-        hip_check(dgtd::launch_stage<T>(R.p, a, dgtd::kModeMult, grid, s), "stage kernel launch");
+        hip_check(launch_on<T>(a, dgtd::kModeMult, variant, s), "stage kernel launch");
     }
 
     void to_device(const double* h, void* d)
@@ -215,6 +253,24 @@ std::vector<T> pack_mats(const dgtd::RefTet& R)
     return m;
 }
 
+// B operands of the MFMA variant: Bv[j][kd*NPG + i] = D_kd[i][j] (KP rows),
+// then Bl[m][i] = LIFT[i][m] (4*Nfp rows); zero outside i, j < Np.
+template <typename T>
+std::vector<T> pack_mats_mfma(const dgtd::RefTet& R)
+{
+    const int Np = R.Np, NF = 4 * R.Nfp;
+    const int KP = (Np + 3) / 4 * 4, NPG = (Np + 15) / 16 * 16;
+    std::vector<T> m((size_t)KP * 3 * NPG + (size_t)NF * NPG, T(0));
+    const std::vector<double>* D[3] = {&R.Dr, &R.Ds, &R.Dt};
+    for (int j = 0; j < Np; ++j)
+        for (int kd = 0; kd < 3; ++kd)
+            for (int i = 0; i < Np; ++i) m[(size_t)j * 3 * NPG + kd * NPG + i] = (T)(*D[kd])[(size_t)i * Np + j];
+    T* L = m.data() + (size_t)KP * 3 * NPG;
+    for (int k = 0; k < NF; ++k)
+        for (int i = 0; i < Np; ++i) L[(size_t)k * NPG + i] = (T)R.LIFT[(size_t)i * NF + k];
+    return m;
+}
+
 template <typename T>
 std::vector<T> cast_vec(const std::vector<double>& v)
 {
@@ -256,6 +312,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder)
         return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(opts->alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
+    if (opts->kernel < DGTD_KERNEL_AUTO || opts->kernel > DGTD_KERNEL_MFMA)
+        return fail(DGTD_E_ARG, "kernel must be DGTD_KERNEL_AUTO, _VALU or _MFMA");
     dgtd_op* op = nullptr;
     try {
         op = new dgtd_op;
@@ -291,15 +349,21 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         tabs.insert(tabs.end(), op->R.face_perm.begin(), op->R.face_perm.end());
         op->d_tabs = static_cast<int*>(upload(tabs));
         op->d_conn = static_cast<int*>(upload(md.conn));
-        int bpc;
+        int bpc, mbpc, mwaves;
         if (op->precision == 64) {
             op->d_geo = upload(md.geo);
             op->d_mats = upload(pack_mats<double>(op->R));
+            op->d_mats_mfma = upload(pack_mats_mfma<double>(op->R));
             bpc = dgtd::stage_blocks_per_cu<double>(op->R.p);
+            mbpc = dgtd::mfma_blocks_per_cu<double>(op->R.p);
+            mwaves = dgtd::mfma_waves_per_block<double>(op->R.p);
         } else {
             op->d_geo = upload(cast_vec<float>(md.geo));
             op->d_mats = upload(pack_mats<float>(op->R));
+            op->d_mats_mfma = upload(pack_mats_mfma<float>(op->R));
             bpc = dgtd::stage_blocks_per_cu<float>(op->R.p);
+            mbpc = dgtd::mfma_blocks_per_cu<float>(op->R.p);
+            mwaves = dgtd::mfma_waves_per_block<float>(op->R.p);
         }
         op->d_M = static_cast<double*>(upload(op->R.M));
         op->d_jac = static_cast<double*>(upload(md.jac));
@@ -319,6 +383,20 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         const int epb = dgtd::elements_per_block(op->R.p);
         op->nblk = (op->K + epb - 1) / epb;
         op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * bpc));
+        if (mbpc > 0) {
+            const int ntasks = (op->K + 15) / 16;
+            int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
+            if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)
+            op->grid_mfma = std::max(1, g);
+        }
+        if (opts->kernel == DGTD_KERNEL_MFMA && op->grid_mfma <= 0)
+            throw std::runtime_error("the MFMA kernel does not fit this order and precision");
+        if (opts->kernel == DGTD_KERNEL_AUTO) {
+            if (op->precision == 64) op->autotune<double>();
+            else op->autotune<float>();
+        } else {
+            op->variant = opts->kernel;
+        }
     } catch (const HipError& e) {
         delete op;
         return fail(DGTD_E_HIP, e.what());
@@ -548,6 +626,13 @@ int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches)
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     if (avg_us) *avg_us = op->last_us;
     if (launches) *launches = op->last_launches;
+    return DGTD_OK;
+}
+
+int dgtd_op_kernel_variant(const dgtd_op* op, int32_t* variant)
+{
+    if (!op || !variant) return fail(DGTD_E_ARG, "NULL argument");
+    *variant = op->variant;
     return DGTD_OK;
 }
 

@@ -45,6 +45,16 @@ int stage_blocks_per_cu(int order);
 
 int elements_per_block(int order);
 
+// MFMA variant (dg_mfma.hip): 16-element wave-tasks, waves_per_block waves.
+// mats layout: Bv [KP][3*NPG] then Bl [4*Nfp][NPG] (see pack in dgtd_api.cpp).
+template <typename T>
+hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+// Sets LDS limits; returns resident blocks per CU, 0 if the variant does not fit.
+template <typename T>
+int mfma_blocks_per_cu(int order);
+template <typename T>
+int mfma_waves_per_block(int order);
+
 // FP64 reductions over a state in T: per-block partials then a fixed-order
 // final pass (deterministic, no atomics).
 //   out[0] = sum of squares, out[1] = energy, out[2] = non-finite count.

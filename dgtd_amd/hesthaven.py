@@ -15,6 +15,7 @@ from . import _abi
 from ._abi import check, dptr, i32ptr, i8ptr, lib
 
 INTEGRATORS = {"rk4": _abi.DGTD_RK4, "lserk4": _abi.DGTD_LSERK4}
+KERNELS = {"auto": _abi.DGTD_KERNEL_AUTO, "valu": _abi.DGTD_KERNEL_VALU, "mfma": _abi.DGTD_KERNEL_MFMA}
 
 
 def num_nodes(order):
@@ -94,7 +95,7 @@ class HesthavenEvolution:
     std::runtime_error (HesthavenEvolution.cpp:37,42,146)."""
 
     def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, precision=64,
-                 integrator="lserk4", device=0):
+                 integrator="lserk4", device=0, kernel="auto"):
         self.order = order
         self.precision = precision
         self.integrator = integrator
@@ -103,7 +104,8 @@ class HesthavenEvolution:
         self.K = mesh.EToV.shape[0]
         self.Np = num_nodes(order)
         self.Nfp = num_face_nodes(order)
-        opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device), 0)
+        opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device),
+                         KERNELS[kernel])
         h = ctypes.c_void_p()
         check(lib().dgtd_op_create(ctypes.byref(mesh.desc), ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
@@ -161,6 +163,11 @@ class HesthavenEvolution:
         xyz = np.zeros(3 * self.Np * self.K)
         check(lib().dgtd_op_node_positions(self._h, dptr(xyz)))
         return xyz.reshape(3, self.K, self.Np)
+
+    def kernel_variant(self):
+        v = ctypes.c_int32()
+        check(lib().dgtd_op_kernel_variant(self._h, ctypes.byref(v)))
+        return {v_: k for k, v_ in KERNELS.items()}[v.value]
 
     def work_per_element(self):
         b, f = ctypes.c_double(), ctypes.c_double()

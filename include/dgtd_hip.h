@@ -76,8 +76,13 @@ typedef struct {
     int32_t precision;  /* 64 or 32 */
     int32_t integrator; /* DGTD_RK4 or DGTD_LSERK4 */
     int32_t device;     /* HIP device ordinal */
-    int32_t reserved;
+    int32_t kernel;     /* DGTD_KERNEL_*: stage-kernel variant, 0 = autotune */
 } dgtd_opts;
+
+/* Stage-kernel variants (both fuse gather, flux, curl, LIFT and the RK update). */
+#define DGTD_KERNEL_AUTO 0 /* time the available variants on this mesh at create, keep the fastest */
+#define DGTD_KERNEL_VALU 1 /* per-node vector FMA, reference matrices in LDS */
+#define DGTD_KERNEL_MFMA 2 /* matrix cores (v_mfma 16x16x4), 16-element wave tasks */
 
 const char* dgtd_last_error(void);
 const char* dgtd_version(void);
@@ -140,6 +145,9 @@ int dgtd_op_norms(dgtd_op* op, double* out3);
  * HIP events on the handle's stream (0 when timing is off). */
 int dgtd_op_set_timing(dgtd_op* op, int32_t on);
 int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches);
+
+/* Stage-kernel variant in use (DGTD_KERNEL_VALU or DGTD_KERNEL_MFMA). */
+int dgtd_op_kernel_variant(const dgtd_op* op, int32_t* variant);
 
 /* Algorithmic HBM bytes and flops per element per stage for this operator. */
 int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops);

@@ -29,44 +29,60 @@ def mixed_case(n, order, alpha=1.0, seed=0, media=True, bc=("pec", "pmc", "sma",
     return vx, E, fbc, eps, mu
 
 
+KERNELS = ["valu", "mfma"]
+
+
+def make_gpu(*args, kernel="valu", **kw):
+    """The MFMA variant has no FP64 build at order 5 (its LDS tiles exceed 160 KiB)."""
+    if kernel == "mfma" and kw.get("precision") == 64 and args[3] == 5:
+        with pytest.raises(D.DgtdError):
+            D.HesthavenEvolution(*args, kernel=kernel, **kw)
+        pytest.skip("no FP64 MFMA variant at order 5 (creation refused, as checked)")
+    return D.HesthavenEvolution(*args, kernel=kernel, **kw)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("alpha", [1.0, 0.5])
-def test_mult_fp64_matches_oracle(order, alpha):
+def test_mult_fp64_matches_oracle(order, alpha, kernel):
     vx, E, fbc, eps, mu = mixed_case(2, order, alpha, seed=order)
     ref = O.Operator(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu)
-    gpu = D.HesthavenEvolution(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu, precision=64)
+    gpu = make_gpu(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu, precision=64, kernel=kernel)
     x = np.random.default_rng(7).standard_normal(ref.ndofs)
     y = gpu.mult(x)
     yr = ref.mult(x)
     assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", [1, 3])
-def test_mult_centred_pec_pmc(order):
+def test_mult_centred_pec_pmc(order, kernel):
     vx, E, fbc, eps, mu = mixed_case(2, order, bc=("pec", "pmc", "pec", "pmc", "none", "pec"))
     ref = O.Operator(vx, E, fbc, order, alpha=0.0)
-    gpu = D.HesthavenEvolution(vx, E, fbc, order, alpha=0.0, precision=64)
+    gpu = make_gpu(vx, E, fbc, order, alpha=0.0, precision=64, kernel=kernel)
     x = np.random.default_rng(3).standard_normal(ref.ndofs)
     yr = ref.mult(x)
     assert np.abs(gpu.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
-def test_mult_fp32_close_to_oracle(order):
+def test_mult_fp32_close_to_oracle(order, kernel):
     vx, E, fbc, eps, mu = mixed_case(3, order, seed=11)
     ref = O.Operator(vx, E, fbc, order, eps=eps, mu=mu)
-    gpu = D.HesthavenEvolution(vx, E, fbc, order, eps=eps, mu=mu, precision=32)
+    gpu = make_gpu(vx, E, fbc, order, eps=eps, mu=mu, precision=32, kernel=kernel)
     pos = ref.positions()
     x = MG.gaussian(pos, (0.45, 0.5, 0.55), 0.2, comps=range(6))
     assert rel(gpu.mult(x), ref.mult(x)) <= 1e-5
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("integrator", ["rk4", "lserk4"])
 @pytest.mark.parametrize("order", [2, 4])
-def test_steps_fp64_match_oracle(integrator, order):
+def test_steps_fp64_match_oracle(integrator, order, kernel):
     vx, E, fbc, eps, mu = mixed_case(2, order, seed=5)
     ref = O.Operator(vx, E, fbc, order, eps=eps, mu=mu)
-    gpu = D.HesthavenEvolution(vx, E, fbc, order, eps=eps, mu=mu, precision=64, integrator=integrator)
+    gpu = make_gpu(vx, E, fbc, order, eps=eps, mu=mu, precision=64, integrator=integrator, kernel=kernel)
     x0 = np.random.default_rng(1).standard_normal(ref.ndofs)
     dt = 0.25 / (ref.max_fscale() * order ** 2)
     xr = x0.copy()
@@ -83,11 +99,12 @@ def test_steps_fp64_match_oracle(integrator, order):
     assert abs(gpu.GetTime() - 3 * dt) < 1e-15
 
 
-def test_te110_cavity_fp64_end_to_end():
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_te110_cavity_fp64_end_to_end(kernel):
     """8^3 PEC cavity, p=3, TE110, LSERK4, dt=2e-3, T=1 (500 steps)."""
     vx, E, fbc = MG.cube_mesh(8, 1.0, "pec")
     ref = O.Operator(vx, E, fbc, 3)
-    gpu = D.HesthavenEvolution(vx, E, fbc, 3, precision=64, integrator="lserk4")
+    gpu = make_gpu(vx, E, fbc, 3, precision=64, integrator="lserk4", kernel=kernel)
     pos = ref.positions()
     q0 = MG.te110(pos)
     xr = q0.copy()
@@ -132,10 +149,11 @@ def test_norms_and_energy():
         assert gpu.norms()["nonfinite"] == 1
 
 
-def test_fp32_cavity_tracks_fp64():
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_fp32_cavity_tracks_fp64(kernel):
     vx, E, fbc = MG.cube_mesh(6, 1.0, "pec")
-    g64 = D.HesthavenEvolution(vx, E, fbc, 4, precision=64)
-    g32 = D.HesthavenEvolution(vx, E, fbc, 4, precision=32)
+    g64 = D.HesthavenEvolution(vx, E, fbc, 4, precision=64, kernel="valu")
+    g32 = make_gpu(vx, E, fbc, 4, precision=32, kernel=kernel)
     pos = g64.node_positions()
     q0 = MG.gaussian(pos, (0.45, 0.5, 0.55), 0.1, comps=range(6))
     dt = D.estimate_time_step(g64)
@@ -143,3 +161,21 @@ def test_fp32_cavity_tracks_fp64():
         g.set_state(q0)
         g.run(dt, 0.3)
     assert rel(g32.get_state(), g64.get_state()) <= 1e-5
+
+
+def test_partial_last_task_and_odd_sizes():
+    """K not a multiple of 16 (or of the VALU block): 5 hexes per edge -> 750 tets."""
+    vx, E, fbc, eps, mu = mixed_case(5, 3, seed=9)
+    ref = O.Operator(vx, E, fbc, 3, eps=eps, mu=mu)
+    x = np.random.default_rng(4).standard_normal(ref.ndofs)
+    yr = ref.mult(x)
+    for kernel in KERNELS:
+        for prec, tol in ((64, 1e-12), (32, 1e-5)):
+            g = D.HesthavenEvolution(vx, E, fbc, 3, eps=eps, mu=mu, precision=prec, kernel=kernel)
+            assert rel(g.mult(x), yr) <= tol, (kernel, prec)
+
+
+def test_autotune_picks_a_variant():
+    vx, E, fbc = MG.cube_mesh(4, 1.0, "pec")
+    g = D.HesthavenEvolution(vx, E, fbc, 4, precision=32, kernel="auto")
+    assert g.kernel_variant() in ("valu", "mfma")
