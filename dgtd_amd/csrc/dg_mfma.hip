@@ -134,7 +134,8 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
     const int t_begin = xcd * per_xcd;
     const int t_end = min(ntasks, t_begin + per_xcd);
 
-    for (int task = t_begin + bin * WAVES + wave; task < t_end; task += bpx * WAVES) {
+    for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES) {
+        const int task = a.order ? a.order[tix] : tix;
         const int e0 = task * 16;
         const int ne = min(16, K - e0);
 
@@ -196,31 +197,40 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
             for (int g = 0; g < G; ++g) R[c][g] = acc_t{0, 0, 0, 0};
 
         // ---- surface: flux of face q, node b of element li -> LIFT ----
-        // Branch-free gather with one step of prefetch: on a boundary face the
-        // "neighbour" read is the element's own trace, and the jump uses the
-        // boundary coefficients instead.
+        // Neighbour traces: from LDS when the neighbour is in this task, else
+        // a global gather (prefetched two face nodes ahead); boundary faces
+        // use the element's own trace with the boundary coefficients.
         const T* Ue = U + li * US;
         const bool interior = nb >= 0;
-        const T* qP = a.qin + (size_t)(interior ? nb : e) * NP;
+        const bool local = interior && (unsigned)(nb - e0) < (unsigned)ne;
+        const bool remote = interior && !local;
+        const T* qP = a.qin + (size_t)(remote ? nb : e) * NP;
+        const T* Un = U + (local ? nb - e0 : li) * US;
         const int* pmap = interior ? sPt + code * NFP : sFm + q * NFP;
-        const T cI = interior ? T(1) : T(0);  // weight of the gathered trace
+        const T cI = interior ? T(1) : T(0);  // weight of the neighbour trace
         const T cE = interior ? T(-1) : ce;   // weight of the own trace, E components
         const T cH = interior ? T(-1) : ch;   // weight of the own trace, H components
-        T nxt[6];
-        {
-            const int ip = pmap[0];
+        T pf0[6], pf1[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) nxt[c] = qP[c * N + ip];
+        for (int c = 0; c < 6; ++c) { pf0[c] = T(0); pf1[c] = T(0); }
+        if (remote) {
+            const int i0 = pmap[0], i1 = pmap[NFP > 1 ? 1 : 0];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { pf0[c] = qP[c * N + i0]; pf1[c] = qP[c * N + i1]; }
         }
 #pragma unroll
         for (int b = 0; b < NFP; ++b) {
+            const int ip = pmap[b];
             T cur[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) cur[c] = nxt[c];
-            if (b + 1 < NFP) {
-                const int ip = pmap[b + 1];
+            for (int c = 0; c < 6; ++c) {
+                cur[c] = remote ? pf0[c] : Un[c * 16 * US + ip];
+                pf0[c] = pf1[c];
+            }
+            if (remote && b + 2 < NFP) {
+                const int i2 = pmap[b + 2];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) nxt[c] = qP[c * N + ip];
+                for (int c = 0; c < 6; ++c) pf1[c] = qP[c * N + i2];
             }
             const int iM = sFm[q * NFP + b];
             T d[6];

@@ -73,6 +73,7 @@ struct dgtd_op {
     int* d_conn = nullptr;
     void* d_mats = nullptr;
     void* d_mats_mfma = nullptr;
+    int* d_order = nullptr;
     int variant = DGTD_KERNEL_VALU;
     int grid_mfma = 0;
     int* d_tabs = nullptr;
@@ -99,7 +100,7 @@ struct dgtd_op {
     ~dgtd_op()
     {
         (void)hipSetDevice(device);
-        for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_tabs, (void*)d_M, (void*)d_jac,
+        for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
                         (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
@@ -128,6 +129,7 @@ struct dgtd_op {
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
             a.mats = static_cast<const T*>(d_mats_mfma);
+            a.order = d_order;
             return dgtd::launch_stage_mfma<T>(R.p, a, mode, grid_mfma, s);
         }
         return dgtd::launch_stage<T>(R.p, a0, mode, grid, s);
@@ -271,6 +273,50 @@ std::vector<T> pack_mats_mfma(const dgtd::RefTet& R)
     return m;
 }
 
+// Processing order of the 16-element tasks of the MFMA variant: Morton
+// (Z-order) of the task centroids, so that tasks in flight together (and the
+// contiguous ranges the kernel gives each XCD) are compact in space and their
+// neighbour traces are reused from L2.
+std::vector<int> morton_task_order(const dgtd_mesh_desc* mesh)
+{
+    const int K = mesh->K, nt = (K + 15) / 16;
+    std::vector<double> c(3 * (size_t)nt, 0.0);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int t = 0; t < nt; ++t) {
+        const int e0 = 16 * t, e1 = std::min(K, e0 + 16);
+        for (int e = e0; e < e1; ++e)
+            for (int v = 0; v < 4; ++v)
+                for (int d = 0; d < 3; ++d) c[3 * (size_t)t + d] += mesh->vx[3 * (size_t)mesh->EToV[4 * (size_t)e + v] + d];
+        for (int d = 0; d < 3; ++d) {
+            c[3 * (size_t)t + d] /= 4.0 * (e1 - e0);
+            lo[d] = std::min(lo[d], c[3 * (size_t)t + d]);
+            hi[d] = std::max(hi[d], c[3 * (size_t)t + d]);
+        }
+    }
+    auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, int>> key(nt);
+    for (int t = 0; t < nt; ++t) {
+        uint64_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double w = hi[d] > lo[d] ? (c[3 * (size_t)t + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+            code |= spread((uint64_t)(w * ((1 << 20) - 1))) << d;
+        }
+        key[t] = {code, t};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> order(nt);
+    for (int t = 0; t < nt; ++t) order[t] = key[t].second;
+    return order;
+}
+
 template <typename T>
 std::vector<T> cast_vec(const std::vector<double>& v)
 {
@@ -385,6 +431,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * bpc));
         if (mbpc > 0) {
             const int ntasks = (op->K + 15) / 16;
+            op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
             int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
             if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)
             op->grid_mfma = std::max(1, g);

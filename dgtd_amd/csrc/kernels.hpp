@@ -34,6 +34,7 @@ struct StageArgs {
     const int* conn;     // kConnStrideDev per element
     const T* mats;       // Dr^T, Ds^T, Dt^T ([j][i] each), then LIFT^T ([m][i])
     const int* tabs;     // Fmask[4*Nfp], then face permutation table [96*Nfp]
+    const int* order;    // MFMA variant: processing order of the 16-element tasks (NULL = identity)
 };
 
 template <typename T>
