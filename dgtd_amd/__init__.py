@@ -5,5 +5,6 @@ in include/dgtd_hip.h); this package is the host-side mirror of the
 reference's operator and solver surfaces.
 """
 from ._abi import DgtdError, LIB_PATH, lib  # noqa: F401
+from .batch import Batch, CapacityError, footprint  # noqa: F401
 from .hesthaven import (HesthavenEvolution, Solver, cube_mesh, estimate_time_step,  # noqa: F401
                         mesh_geometry, mesh_vmaps, num_face_nodes, num_nodes, ref_element)

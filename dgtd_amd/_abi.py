@@ -39,6 +39,14 @@ class Opts(ctypes.Structure):
                 ("device", c_int32), ("kernel", c_int32)]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("alpha", c_double), ("precision", c_int32), ("device", c_int32),
+                ("max_instances", c_int32), ("reserved", c_int32), ("hbm_budget", ctypes.c_int64)]
+
+
+DGTD_E_CAPACITY = -6
+DGTD_INST_FREE, DGTD_INST_RUNNING, DGTD_INST_DONE = 0, 1, 2
+
 # name -> (restype, argtypes); every symbol declared in include/dgtd_hip.h.
 SIGNATURES = {
     "dgtd_last_error": (ctypes.c_char_p, []),
@@ -64,6 +72,16 @@ SIGNATURES = {
     "dgtd_op_kernel_variant": (c_int32, [c_void_p, _i32p]),
     "dgtd_op_work_per_element": (c_int32, [c_void_p, _dp, _dp]),
     "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
+    "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
+    "dgtd_batch_destroy": (None, [c_void_p]),
+    "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),
+    "dgtd_batch_add": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc), _dp, c_double, c_double, _i32p]),
+    "dgtd_batch_step": (c_int32, [c_void_p, c_int32]),
+    "dgtd_batch_poll": (c_int32, [c_void_p, _i32p, c_int32, _i32p, _i32p]),
+    "dgtd_batch_info": (c_int32, [c_void_p, c_int32, _dp, _i32p, _i32p, _dp]),
+    "dgtd_batch_get_state": (c_int32, [c_void_p, c_int32, _dp]),
+    "dgtd_batch_remove": (c_int32, [c_void_p, c_int32]),
+    "dgtd_batch_stats": (c_int32, [c_void_p, _i64p, _i32p, _i32p, _i32p]),
     "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),
     "dgtd_mesh_vmaps": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _i32p, _i32p, _i8p]),
     "dgtd_mesh_geometry": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _dp]),

@@ -1,0 +1,493 @@
+// Host side of the multi-instance batch (include/dgtd_hip.h, dgtd_batch_*):
+// admission by device footprint, resident instance storage, the device
+// instance table and work list, and the hipGraph of one time step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dgtd_hip.h"
+#include "kernels.hpp"
+#include "mesh.hpp"
+#include "packing.hpp"
+#include "reftet.hpp"
+
+namespace dgtd {
+int api_fail(int code, const std::string& msg);  // dgtd_api.cpp
+}
+
+namespace {
+
+struct HipErr : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hcheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw HipErr(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// LSERK4 (Carpenter & Kennedy 1994) as published.
+const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                      -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                      1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                      2277821191437.0 / 14882151754819.0};
+
+int64_t footprint(int K, int order, int precision)
+{
+    const int64_t s = precision == 64 ? 8 : 4;
+    const int64_t Np = dgtd::num_nodes(order);
+    return 3 * 6 * Np * K * s + (int64_t)K * dgtd::kGeoStride * s + (int64_t)K * dgtd::kConnStride * 4;
+}
+
+}  // namespace
+
+struct dgtd_batch {
+    int device = 0, precision = 32;
+    double alpha = 1.0;
+    int max_slots = 0;
+    int64_t budget = 0, used = 0;
+    hipStream_t stream = nullptr;
+    void* d_inst = nullptr;  // BatchInst<T>[max_slots]
+    dgtd::BatchItem* d_items = nullptr;
+    int item_cap = 0;
+    int* d_count = nullptr;
+    void* d_mats[dgtd::kMaxBatchOrder + 1] = {};
+    int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
+    int num_cu = 1;
+
+    struct Slot {
+        int state = dgtd::kSlotFree;
+        int order = 0, K = 0, nblk = 0;
+        int64_t bytes = 0;
+        void* buf[5] = {};  // q0, q1, res, geo, conn
+        bool reported = false;
+        // host copy of the device record (authoritative only while dirty)
+        double t = 0, dt = 0, dt_step = 0, final_time = 0;
+        int cur = 0, steps = 0;
+    };
+    std::vector<Slot> slots;
+    bool dirty = true;
+    int nitems = 0, grid = 1;
+    int graph_order = 0;
+    size_t smem = 0;
+    hipGraphExec_t graph = nullptr;
+
+    size_t rs() const { return precision == 64 ? 8 : 4; }
+
+    ~dgtd_batch()
+    {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& s : slots)
+            for (void* p : s.buf)
+                if (p) (void)hipFree(p);
+        for (void* p : {d_inst, (void*)d_items, (void*)d_count}) if (p) (void)hipFree(p);
+        for (int o = 0; o <= dgtd::kMaxBatchOrder; ++o) {
+            if (d_mats[o]) (void)hipFree(d_mats[o]);
+            if (d_tabs[o]) (void)hipFree(d_tabs[o]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Device table <-> host mirror.  The device owns t/cur/steps/state of
+    // running instances between flushes.
+    template <typename T>
+    void pull()
+    {
+        std::vector<dgtd::BatchInst<T>> h(max_slots);
+        hcheck(hipMemcpyAsync(h.data(), d_inst, sizeof(h[0]) * max_slots, hipMemcpyDeviceToHost, stream), "D2H table");
+        hcheck(hipStreamSynchronize(stream), "sync");
+        for (int i = 0; i < max_slots; ++i) {
+            Slot& s = slots[i];
+            if (s.state == dgtd::kSlotFree) continue;
+            s.state = h[i].state;
+            s.t = h[i].t;
+            s.cur = h[i].cur;
+            s.steps = h[i].steps;
+            s.dt_step = h[i].dt_step;
+        }
+    }
+
+    template <typename T>
+    void push()
+    {
+        std::vector<dgtd::BatchInst<T>> h(max_slots);
+        for (int i = 0; i < max_slots; ++i) {
+            const Slot& s = slots[i];
+            dgtd::BatchInst<T>& r = h[i];
+            r = dgtd::BatchInst<T>{};
+            r.state = s.state;
+            if (s.state == dgtd::kSlotFree) continue;
+            r.order = s.order;
+            r.K = s.K;
+            r.nblk = s.nblk;
+            r.cur = s.cur;
+            r.steps = s.steps;
+            r.q[0] = static_cast<T*>(s.buf[0]);
+            r.q[1] = static_cast<T*>(s.buf[1]);
+            r.res = static_cast<T*>(s.buf[2]);
+            r.geo = static_cast<const T*>(s.buf[3]);
+            r.conn = static_cast<const int*>(s.buf[4]);
+            r.alpha = alpha;
+            r.t = s.t;
+            r.dt = s.dt;
+            r.dt_step = s.dt_step;
+            r.final_time = s.final_time;
+        }
+        hcheck(hipMemcpyAsync(d_inst, h.data(), sizeof(h[0]) * max_slots, hipMemcpyHostToDevice, stream), "H2D table");
+        // work list: running instances' element blocks, sorted by order
+        std::vector<dgtd::BatchItem> items;
+        int max_order = 1;
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o)
+            for (int i = 0; i < max_slots; ++i)
+                if (slots[i].state == dgtd::kSlotRunning && slots[i].order == o) {
+                    max_order = std::max(max_order, o);
+                    for (int blk = 0; blk < slots[i].nblk; ++blk) items.push_back({i, blk});
+                }
+        if ((int)items.size() > item_cap) {
+            if (d_items) hcheck(hipFree(d_items), "hipFree");
+            item_cap = std::max<int>((int)items.size(), 2 * item_cap);
+            hcheck(hipMalloc(&d_items, sizeof(dgtd::BatchItem) * item_cap), "hipMalloc items");
+            if (graph) {
+                hcheck(hipGraphExecDestroy(graph), "hipGraphExecDestroy");
+                graph = nullptr;
+            }
+        }
+        nitems = (int)items.size();
+        if (nitems)
+            hcheck(hipMemcpyAsync(d_items, items.data(), sizeof(items[0]) * nitems, hipMemcpyHostToDevice, stream),
+                   "H2D items");
+        hcheck(hipMemcpyAsync(d_count, &nitems, sizeof(int), hipMemcpyHostToDevice, stream), "H2D count");
+        hcheck(hipStreamSynchronize(stream), "sync");  // host vectors die here
+        if (max_order != graph_order || !graph) capture<T>(max_order);
+    }
+
+    template <typename T>
+    void capture(int max_order)
+    {
+        if (graph) {
+            hcheck(hipGraphExecDestroy(graph), "hipGraphExecDestroy");
+            graph = nullptr;
+        }
+        smem = dgtd::batch_smem_bytes<T>(max_order);
+        const int bpc = dgtd::batch_blocks_per_cu<T>(smem);
+        grid = std::max(1, num_cu * bpc);
+        dgtd::BatchArgs<T> a{};
+        a.inst = static_cast<dgtd::BatchInst<T>*>(d_inst);
+        a.items = d_items;
+        a.count = d_count;
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+            a.mats[o] = static_cast<const T*>(d_mats[o]);
+            a.tabs[o] = d_tabs[o];
+        }
+        hipGraph_t g = nullptr;
+        hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipError_t err = hipSuccess;
+        for (int s = 0; s < 5 && err == hipSuccess; ++s) {
+            a.stage = s;
+            a.ca = (T)kA[s];
+            a.cb = (T)kB[s];
+            err = dgtd::launch_batch_stage<T>(a, grid, smem, stream);
+        }
+        if (err == hipSuccess) err = dgtd::launch_batch_advance<T>(a.inst, max_slots, stream);
+        hipError_t e2 = hipStreamEndCapture(stream, &g);
+        hcheck(err, "batch launch during capture");
+        hcheck(e2, "hipStreamEndCapture");
+        hipError_t e3 = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hcheck(e3, "hipGraphInstantiate");
+        graph_order = max_order;
+    }
+
+    void flush()
+    {
+        if (!dirty) return;
+        if (precision == 64) push<double>();
+        else push<float>();
+        dirty = false;
+    }
+
+    void sync_pull()
+    {
+        hcheck(hipStreamSynchronize(stream), "sync");
+        if (precision == 64) pull<double>();
+        else pull<float>();
+    }
+};
+
+namespace {
+
+template <typename T>
+void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const dgtd::MeshData& md, const double* x0,
+                     size_t ndofs)
+{
+    const size_t bytes_q = sizeof(T) * ndofs;
+    for (int k = 0; k < 3; ++k) hcheck(hipMalloc(&s.buf[k], bytes_q), "hipMalloc instance");
+    hcheck(hipMalloc(&s.buf[3], sizeof(T) * md.geo.size()), "hipMalloc geo");
+    hcheck(hipMalloc(&s.buf[4], sizeof(int) * md.conn.size()), "hipMalloc conn");
+    std::vector<T> tmp(ndofs, T(0));
+    if (x0)
+        for (size_t i = 0; i < ndofs; ++i) tmp[i] = (T)x0[i];
+    hcheck(hipMemcpyAsync(s.buf[0], tmp.data(), bytes_q, hipMemcpyHostToDevice, b->stream), "H2D x0");
+    hcheck(hipMemsetAsync(s.buf[2], 0, bytes_q, b->stream), "memset res");
+    const std::vector<T> g = dgtd::cast_vec<T>(md.geo);
+    hcheck(hipMemcpyAsync(s.buf[3], g.data(), sizeof(T) * g.size(), hipMemcpyHostToDevice, b->stream), "H2D geo");
+    hcheck(hipMemcpyAsync(s.buf[4], md.conn.data(), sizeof(int) * md.conn.size(), hipMemcpyHostToDevice, b->stream),
+           "H2D conn");
+    hcheck(hipStreamSynchronize(b->stream), "sync");
+}
+
+template <typename T>
+void upload_refs(dgtd_batch* b)
+{
+    for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+        const dgtd::RefTet R = dgtd::build_reftet(o);
+        const std::vector<T> m = dgtd::pack_mats<T>(R);
+        std::vector<int> tabs(R.Fmask.begin(), R.Fmask.end());
+        tabs.insert(tabs.end(), R.face_perm.begin(), R.face_perm.end());
+        hcheck(hipMalloc(&b->d_mats[o], sizeof(T) * m.size()), "hipMalloc mats");
+        hcheck(hipMemcpy(b->d_mats[o], m.data(), sizeof(T) * m.size(), hipMemcpyHostToDevice), "H2D mats");
+        hcheck(hipMalloc(&b->d_tabs[o], sizeof(int) * tabs.size()), "hipMalloc tabs");
+        hcheck(hipMemcpy(b->d_tabs[o], tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice), "H2D tabs");
+    }
+}
+
+int batch_error(const std::exception& e)
+{
+    if (dynamic_cast<const HipErr*>(&e)) return dgtd::api_fail(DGTD_E_HIP, e.what());
+    return dgtd::api_fail(DGTD_E_ARG, e.what());
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t* bytes)
+{
+    if (!mesh || !bytes) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    *bytes = footprint(mesh->K, mesh->order, precision);
+    return DGTD_OK;
+}
+
+int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out)
+{
+    if (!opts || !out) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (opts->precision != 32 && opts->precision != 64) return dgtd::api_fail(DGTD_E_ARG, "precision must be 64 or 32");
+    if (opts->max_instances < 1 || opts->max_instances > 4096)
+        return dgtd::api_fail(DGTD_E_ARG, "max_instances must be in 1..4096");
+    if (!std::isfinite(opts->alpha)) return dgtd::api_fail(DGTD_E_ARG, "alpha must be finite");
+    dgtd_batch* b = nullptr;
+    try {
+        b = new dgtd_batch;
+        b->device = opts->device;
+        b->precision = opts->precision;
+        b->alpha = opts->alpha;
+        b->max_slots = opts->max_instances;
+        b->slots.resize(b->max_slots);
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        hcheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipDeviceProp_t prop;
+        hcheck(hipGetDeviceProperties(&prop, b->device), "hipGetDeviceProperties");
+        b->num_cu = prop.multiProcessorCount;
+        size_t free_b = 0, total_b = 0;
+        hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        b->budget = opts->hbm_budget > 0 ? opts->hbm_budget : (int64_t)(0.8 * (double)free_b);
+        const size_t rec = b->precision == 64 ? sizeof(dgtd::BatchInst<double>) : sizeof(dgtd::BatchInst<float>);
+        hcheck(hipMalloc(&b->d_inst, rec * b->max_slots), "hipMalloc table");
+        hcheck(hipMalloc(&b->d_count, sizeof(int)), "hipMalloc count");
+        b->item_cap = 1 << 16;
+        hcheck(hipMalloc(&b->d_items, sizeof(dgtd::BatchItem) * b->item_cap), "hipMalloc items");
+        if (b->precision == 64) upload_refs<double>(b);
+        else upload_refs<float>(b);
+        b->flush();
+    } catch (const std::exception& e) {
+        delete b;
+        return batch_error(e);
+    }
+    *out = b;
+    return DGTD_OK;
+}
+
+void dgtd_batch_destroy(dgtd_batch* b) { delete b; }
+
+int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt, double final_time,
+                   int32_t* id)
+{
+    if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxBatchOrder)
+        return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");
+    const int64_t bytes = footprint(mesh->K, mesh->order, b->precision);
+    int slot = -1;
+    for (int i = 0; i < b->max_slots && slot < 0; ++i)
+        if (b->slots[i].state == dgtd::kSlotFree) slot = i;
+    if (slot < 0) return dgtd::api_fail(DGTD_E_CAPACITY, "no free instance slot");
+    if (b->used + bytes > b->budget) return dgtd::api_fail(DGTD_E_CAPACITY, "HBM budget exceeded");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+        dgtd::MeshInput in;
+        in.K = mesh->K;
+        in.nv = mesh->nv;
+        in.vx = mesh->vx;
+        in.EToV = mesh->EToV;
+        in.face_bc = mesh->face_bc;
+        in.eps_r = mesh->eps_r;
+        in.mu_r = mesh->mu_r;
+        const dgtd::MeshData md = dgtd::build_mesh(in, R, b->alpha);
+        if (!b->dirty) b->sync_pull();  // device values are authoritative: fetch before editing
+        dgtd_batch::Slot& s = b->slots[slot];
+        s = dgtd_batch::Slot{};
+        const size_t ndofs = 6 * (size_t)R.Np * mesh->K;
+        if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
+        else upload_instance<float>(b, s, md, x0, ndofs);
+        s.order = mesh->order;
+        s.K = mesh->K;
+        s.nblk = (mesh->K + dgtd::elements_per_block(mesh->order) - 1) / dgtd::elements_per_block(mesh->order);
+        s.bytes = bytes;
+        s.dt = dt > 0.0 ? dt : 1.0 / (md.max_fscale * mesh->order * mesh->order);
+        s.final_time = final_time;
+        s.t = 0.0;
+        s.cur = 0;
+        s.steps = 0;
+        if (0.0 <= final_time - 1e-8 * s.dt) {
+            s.state = dgtd::kSlotRunning;
+            s.dt_step = std::min(s.dt, final_time);
+        } else {
+            s.state = dgtd::kSlotDone;
+        }
+        b->used += bytes;
+        b->dirty = true;
+    } catch (const std::exception& e) {
+        dgtd_batch::Slot& s = b->slots[slot];
+        for (void*& p : s.buf)
+            if (p) { (void)hipFree(p); p = nullptr; }
+        s.state = dgtd::kSlotFree;
+        return batch_error(e);
+    }
+    *id = slot;
+    return DGTD_OK;
+}
+
+int dgtd_batch_step(dgtd_batch* b, int32_t nsteps)
+{
+    if (!b || nsteps < 0) return dgtd::api_fail(DGTD_E_ARG, "bad argument");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        b->flush();
+        for (int s = 0; s < nsteps; ++s) hcheck(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    return DGTD_OK;
+}
+
+int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndone, int32_t* nrunning)
+{
+    if (!b) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        b->flush();
+        b->sync_pull();
+        int nd = 0, nr = 0;
+        for (int i = 0; i < b->max_slots; ++i) {
+            auto& s = b->slots[i];
+            if (s.state == dgtd::kSlotRunning) ++nr;
+            if (s.state == dgtd::kSlotDone && !s.reported && nd < cap) {
+                if (done_ids) done_ids[nd] = i;
+                s.reported = true;
+                ++nd;
+            }
+        }
+        if (ndone) *ndone = nd;
+        if (nrunning) *nrunning = nr;
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    return DGTD_OK;
+}
+
+int dgtd_batch_info(dgtd_batch* b, int32_t id, double* t, int32_t* steps, int32_t* state, double* dt)
+{
+    if (!b || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad instance id");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        b->flush();
+        b->sync_pull();
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    const auto& s = b->slots[id];
+    if (t) *t = s.t;
+    if (steps) *steps = s.steps;
+    if (state) *state = s.state;
+    if (dt) *dt = s.dt;
+    return DGTD_OK;
+}
+
+int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x)
+{
+    if (!b || !x || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad argument");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        b->flush();
+        b->sync_pull();
+        const auto& s = b->slots[id];
+        if (s.state == dgtd::kSlotFree) return dgtd::api_fail(DGTD_E_STATE, "instance slot is free");
+        const size_t ndofs = 6 * (size_t)dgtd::num_nodes(s.order) * s.K;
+        if (b->precision == 64) {
+            hcheck(hipMemcpy(x, s.buf[s.cur], sizeof(double) * ndofs, hipMemcpyDeviceToHost), "D2H state");
+        } else {
+            std::vector<float> tmp(ndofs);
+            hcheck(hipMemcpy(tmp.data(), s.buf[s.cur], sizeof(float) * ndofs, hipMemcpyDeviceToHost), "D2H state");
+            std::copy(tmp.begin(), tmp.end(), x);
+        }
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    return DGTD_OK;
+}
+
+int dgtd_batch_remove(dgtd_batch* b, int32_t id)
+{
+    if (!b || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad instance id");
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        if (!b->dirty) b->sync_pull();
+        else hcheck(hipStreamSynchronize(b->stream), "sync");
+        auto& s = b->slots[id];
+        if (s.state == dgtd::kSlotFree) return dgtd::api_fail(DGTD_E_STATE, "instance slot is free");
+        for (void*& p : s.buf)
+            if (p) { hcheck(hipFree(p), "hipFree"); p = nullptr; }
+        b->used -= s.bytes;
+        s = dgtd_batch::Slot{};
+        b->dirty = true;
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    return DGTD_OK;
+}
+
+int dgtd_batch_stats(dgtd_batch* b, int64_t* bytes_in_use, int32_t* running, int32_t* items, int32_t* grid)
+{
+    if (!b) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    int nr = 0;
+    for (const auto& s : b->slots) nr += s.state == dgtd::kSlotRunning;
+    if (bytes_in_use) *bytes_in_use = b->used;
+    if (running) *running = nr;
+    if (items) *items = b->nitems;
+    if (grid) *grid = b->grid;
+    return DGTD_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,7 @@
 
 #include "kernels.hpp"
 #include "mesh.hpp"
+#include "packing.hpp"
 #include "reftet.hpp"
 
 namespace {
@@ -240,39 +241,6 @@ struct dgtd_op {
 
 namespace {
 
-template <typename T>
-std::vector<T> pack_mats(const dgtd::RefTet& R)
-{
-    const int Np = R.Np, NF = 4 * R.Nfp;
-    std::vector<T> m((size_t)3 * Np * Np + (size_t)NF * Np);
-    const std::vector<double>* D[3] = {&R.Dr, &R.Ds, &R.Dt};
-    for (int d = 0; d < 3; ++d)
-        for (int i = 0; i < Np; ++i)
-            for (int j = 0; j < Np; ++j) m[(size_t)d * Np * Np + (size_t)j * Np + i] = (T)(*D[d])[(size_t)i * Np + j];
-    T* L = m.data() + 3 * (size_t)Np * Np;
-    for (int i = 0; i < Np; ++i)
-        for (int k = 0; k < NF; ++k) L[(size_t)k * Np + i] = (T)R.LIFT[(size_t)i * NF + k];
-    return m;
-}
-
-// B operands of the MFMA variant: Bv[j][kd*NPG + i] = D_kd[i][j] (KP rows),
-// then Bl[m][i] = LIFT[i][m] (4*Nfp rows); zero outside i, j < Np.
-template <typename T>
-std::vector<T> pack_mats_mfma(const dgtd::RefTet& R)
-{
-    const int Np = R.Np, NF = 4 * R.Nfp;
-    const int KP = (Np + 3) / 4 * 4, NPG = (Np + 15) / 16 * 16;
-    std::vector<T> m((size_t)KP * 3 * NPG + (size_t)NF * NPG, T(0));
-    const std::vector<double>* D[3] = {&R.Dr, &R.Ds, &R.Dt};
-    for (int j = 0; j < Np; ++j)
-        for (int kd = 0; kd < 3; ++kd)
-            for (int i = 0; i < Np; ++i) m[(size_t)j * 3 * NPG + kd * NPG + i] = (T)(*D[kd])[(size_t)i * Np + j];
-    T* L = m.data() + (size_t)KP * 3 * NPG;
-    for (int k = 0; k < NF; ++k)
-        for (int i = 0; i < Np; ++i) L[(size_t)k * NPG + i] = (T)R.LIFT[(size_t)i * NF + k];
-    return m;
-}
-
 // Processing order of the 16-element tasks of the MFMA variant: Morton
 // (Z-order) of the task centroids, so that tasks in flight together (and the
 // contiguous ranges the kernel gives each XCD) are compact in space and their
@@ -317,11 +285,6 @@ std::vector<int> morton_task_order(const dgtd_mesh_desc* mesh)
     return order;
 }
 
-template <typename T>
-std::vector<T> cast_vec(const std::vector<double>& v)
-{
-    return std::vector<T>(v.begin(), v.end());
-}
 
 }  // namespace
 
@@ -398,15 +361,15 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         int bpc, mbpc, mwaves;
         if (op->precision == 64) {
             op->d_geo = upload(md.geo);
-            op->d_mats = upload(pack_mats<double>(op->R));
-            op->d_mats_mfma = upload(pack_mats_mfma<double>(op->R));
+            op->d_mats = upload(dgtd::pack_mats<double>(op->R));
+            op->d_mats_mfma = upload(dgtd::pack_mats_mfma<double>(op->R));
             bpc = dgtd::stage_blocks_per_cu<double>(op->R.p);
             mbpc = dgtd::mfma_blocks_per_cu<double>(op->R.p);
             mwaves = dgtd::mfma_waves_per_block<double>(op->R.p);
         } else {
-            op->d_geo = upload(cast_vec<float>(md.geo));
-            op->d_mats = upload(pack_mats<float>(op->R));
-            op->d_mats_mfma = upload(pack_mats_mfma<float>(op->R));
+            op->d_geo = upload(dgtd::cast_vec<float>(md.geo));
+            op->d_mats = upload(dgtd::pack_mats<float>(op->R));
+            op->d_mats_mfma = upload(dgtd::pack_mats_mfma<float>(op->R));
             bpc = dgtd::stage_blocks_per_cu<float>(op->R.p);
             mbpc = dgtd::mfma_blocks_per_cu<float>(op->R.p);
             mwaves = dgtd::mfma_waves_per_block<float>(op->R.p);
@@ -757,3 +720,8 @@ int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo)
 }
 
 }  // extern "C"
+
+namespace dgtd {
+// Shared error reporting for the other ABI sources.
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace dgtd

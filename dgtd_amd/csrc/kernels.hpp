@@ -56,6 +56,51 @@ int mfma_blocks_per_cu(int order);
 template <typename T>
 int mfma_waves_per_block(int order);
 
+// ---------------------------------------------------------------------------
+// Multi-instance batch (dg_batch.hip): many meshes of different size and order
+// advance together; one persistent launch per LSERK stage walks a work list
+// of (instance, element block) items sorted by order.
+constexpr int kMaxBatchOrder = 5;
+
+enum BatchState : int { kSlotFree = 0, kSlotRunning = 1, kSlotDone = 2 };
+
+template <typename T>
+struct BatchInst {
+    int order, K, nblk, state;  // state: BatchState
+    int cur, steps, pad0, pad1; // cur: which q buffer holds the state
+    T* q[2];
+    T* res;
+    const T* geo;
+    const int* conn;
+    double alpha, t, dt, dt_step, final_time;
+};
+
+struct BatchItem {
+    int inst, blk;
+};
+
+template <typename T>
+struct BatchArgs {
+    BatchInst<T>* inst;
+    const BatchItem* items;
+    const int* count;  // number of valid items (device)
+    const T* mats[kMaxBatchOrder + 1];
+    const int* tabs[kMaxBatchOrder + 1];
+    int stage;
+    T ca, cb;
+};
+
+template <typename T>
+hipError_t launch_batch_stage(const BatchArgs<T>& b, int grid, size_t smem, hipStream_t s);
+template <typename T>
+hipError_t launch_batch_advance(BatchInst<T>* inst, int nslots, hipStream_t s);
+// LDS bytes of the batch kernel when orders up to max_order are present,
+// and resident blocks per CU at that size.
+template <typename T>
+size_t batch_smem_bytes(int max_order);
+template <typename T>
+int batch_blocks_per_cu(size_t smem);
+
 // FP64 reductions over a state in T: per-block partials then a fixed-order
 // final pass (deterministic, no atomics).
 //   out[0] = sum of squares, out[1] = energy, out[2] = non-finite count.

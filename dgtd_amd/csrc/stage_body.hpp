@@ -1,0 +1,198 @@
+// Device body of the per-node (VALU) fused DGTD stage, shared by the
+// single-mesh kernel (dg_kernels.hip) and the multi-instance batch kernel
+// (dg_batch.hip).  See dg_kernels.hip for the algorithm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace dgtd {
+namespace valu {
+
+template <int P>
+struct Shape {
+    static constexpr int NP = (P + 1) * (P + 2) * (P + 3) / 6;
+    static constexpr int NFP = (P + 1) * (P + 2) / 2;
+    static constexpr int NF = 4 * NFP;
+    static constexpr int EB = (256 / NP) > 0 ? (256 / NP) : 1;  // elements per block
+};
+
+template <int P, typename T>
+struct Smem {
+    using S = Shape<P>;
+    static constexpr int nD = 3 * S::NP * S::NP;
+    static constexpr int nL = S::NF * S::NP;
+    static constexpr int nQ = 6 * S::EB * S::NP;
+    static constexpr int nF = 6 * S::EB * S::NF;
+    static constexpr int nG = S::EB * kGeoStrideDev;
+    static constexpr int nReal = nD + nL + nQ + nF + nG;
+    static constexpr int nInt = S::EB * kConnStrideDev + 4 * S::NFP + 96 * S::NFP;
+    static constexpr size_t bytes = sizeof(T) * (size_t)nReal + sizeof(int) * (size_t)nInt;
+};
+
+// Loads the order-P reference matrices and tables into LDS (block-wide;
+// the caller synchronises before use).
+template <int P, typename T>
+__device__ __forceinline__ void stage_load_ref(const T* __restrict__ mats, const int* __restrict__ tabs,
+                                               unsigned char* smem_raw)
+{
+    using S = Shape<P>;
+    using L = Smem<P, T>;
+    constexpr int NFP = S::NFP, EB = S::EB;
+    T* sD = reinterpret_cast<T*>(smem_raw);
+    int* sFm = reinterpret_cast<int*>(sD + L::nReal) + EB * kConnStrideDev;
+    for (int i = threadIdx.x; i < L::nD + L::nL; i += blockDim.x) sD[i] = mats[i];
+    for (int i = threadIdx.x; i < 4 * NFP + 96 * NFP; i += blockDim.x) sFm[i] = tabs[i];
+}
+
+// One element block (EB elements from e0 = blk*EB) of one stage: gather,
+// flux, curl, LIFT and the stage update.  Starts and ends with a block-wide
+// barrier; the reference matrices must already be in LDS.
+template <int P, typename T, int MODE>
+__device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char* smem_raw, int blk)
+{
+    using S = Shape<P>;
+    using L = Smem<P, T>;
+    constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EB = S::EB;
+    T* sD = reinterpret_cast<T*>(smem_raw);  // [3][j][i]
+    T* sL = sD + L::nD;                      // [m][i]
+    T* sQ = sL + L::nL;                      // [c][el*NP + j]
+    T* sF = sQ + L::nQ;                      // [c][el*NF + m]
+    T* sG = sF + L::nF;                      // [el][kGeoStride]
+    int* sC = reinterpret_cast<int*>(sG + L::nG);  // [el][kConnStride]
+    int* sFm = sC + EB * kConnStrideDev;            // [4*NFP]
+    int* sPt = sFm + 4 * NFP;                       // [96*NFP]
+
+    const int tid = threadIdx.x;
+    const int K = a.K;
+    const size_t N = (size_t)NP * K;
+    {
+        const int e0 = blk * EB;
+        const int ne = min(EB, K - e0);
+        __syncthreads();  // previous iteration finished with sQ/sF/sG/sC (and tables loaded)
+        {
+            const size_t base = (size_t)e0 * NP;
+            const int cnt = ne * NP;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                for (int i = tid; i < cnt; i += blockDim.x) sQ[c * EB * NP + i] = a.qin[c * N + base + i];
+            for (int i = tid; i < ne * kGeoStrideDev; i += blockDim.x)
+                sG[i] = a.geo[(size_t)e0 * kGeoStrideDev + i];
+            for (int i = tid; i < ne * kConnStrideDev; i += blockDim.x)
+                sC[i] = a.conn[(size_t)e0 * kConnStrideDev + i];
+        }
+        __syncthreads();
+
+        // ---- surface: jumps, boundary conditions, flux, fscale/2 ----
+        for (int s = tid; s < ne * NF; s += blockDim.x) {
+            const int el = s / NF;
+            const int m = s - el * NF;
+            const int f = m / NFP;
+            const int b = m - f * NFP;
+            const T* g = sG + el * kGeoStrideDev;
+            const int nb = sC[el * kConnStrideDev + f];
+            const int iM = sFm[f * NFP + b];
+            T uM[6], d[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) uM[c] = sQ[c * EB * NP + el * NP + iM];
+            if (nb >= 0) {
+                const int code = sC[el * kConnStrideDev + 4 + f];
+                const size_t gP = (size_t)nb * NP + sPt[code * NFP + b];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) d[c] = a.qin[c * N + gP] - uM[c];
+            } else {
+                const int bc = -nb - 1;
+                // HesthavenEvolution.cpp:290-292: PEC (-2, 0), PMC (0, -2), SMA (-1/alpha, -1/alpha)
+                T ce = T(0), ch = T(0);
+                if (bc == 1) { ce = T(-2); ch = T(0); }
+                else if (bc == 2) { ce = T(0); ch = T(-2); }
+                else if (bc == 3) { ce = T(-1) / a.alpha; ch = ce; }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { d[c] = ce * uM[c]; d[3 + c] = ch * uM[3 + c]; }
+            }
+            const T nx = g[9 + 4 * f], ny = g[10 + 4 * f], nz = g[11 + 4 * f], fs = g[12 + 4 * f];
+            const T n3[3] = {nx, ny, nz};
+            const T ndH = nx * d[3] + ny * d[4] + nz * d[5];
+            const T ndE = nx * d[0] + ny * d[1] + nz * d[2];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                const T fH = -n3[y] * d[z] + n3[z] * d[y] + a.alpha * (d[3 + x] - ndH * n3[x]);
+                const T fE = n3[y] * d[3 + z] - n3[z] * d[3 + y] + a.alpha * (d[x] - ndE * n3[x]);
+                sF[(3 + x) * EB * NF + el * NF + m] = fH * fs;
+                sF[x * EB * NF + el * NF + m] = fE * fs;
+            }
+        }
+        __syncthreads();
+
+        // ---- volume: curl + LIFT, stage update ----
+        for (int t = tid; t < ne * NP; t += blockDim.x) {
+            const int el = t / NP;
+            const int i = t - el * NP;
+            T ar[6], as[6], at[6], lf[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { ar[c] = T(0); as[c] = T(0); at[c] = T(0); lf[c] = T(0); }
+#pragma unroll 5
+            for (int j = 0; j < NP; ++j) {
+                const T dr = sD[j * NP + i], ds = sD[NP * NP + j * NP + i], dt = sD[2 * NP * NP + j * NP + i];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const T u = sQ[c * EB * NP + el * NP + j];
+                    ar[c] += dr * u;
+                    as[c] += ds * u;
+                    at[c] += dt * u;
+                }
+            }
+#pragma unroll 4
+            for (int m = 0; m < NF; ++m) {
+                const T l = sL[m * NP + i];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) lf[c] += l * sF[c * EB * NF + el * NF + m];
+            }
+            const T* g = sG + el * kGeoStrideDev;
+            // D_d u = r_d ur + s_d us + t_d ut
+            auto Dd = [&](int dd, int c) { return g[dd] * ar[c] + g[3 + dd] * as[c] + g[6 + dd] * at[c]; };
+            T rhs[6];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                rhs[3 + x] = (-Dd(y, z) + Dd(z, y) + lf[3 + x]) * g[26];
+                rhs[x] = (Dd(y, 3 + z) - Dd(z, 3 + y) + lf[x]) * g[25];
+            }
+            const size_t idx = (size_t)(e0 + el) * NP + i;
+            // all reads before the first store (see the MFMA variant)
+            T rv[6], xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
+                    rv[c] = a.res[c * N + idx];
+                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const size_t o = c * N + idx;
+                const T qv = sQ[c * EB * NP + el * NP + i];
+                if constexpr (MODE == kModeMult) {
+                    a.out[o] = rhs[c];
+                } else if constexpr (MODE == kModeLserk) {
+                    const T r = a.ca * rv[c] + a.dt * rhs[c];
+                    a.res[o] = r;
+                    a.out[o] = qv + a.cb * r;
+                } else if constexpr (MODE == kModeRk1) {
+                    a.out[o] = qv + a.ca * rhs[c];
+                    a.res[o] = qv + a.cb * rhs[c];
+                } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
+                    a.out[o] = xv[c] + a.ca * rhs[c];
+                    a.res[o] = rv[c] + a.cb * rhs[c];
+                } else {  // kModeRk4
+                    a.out[o] = rv[c] + a.ca * rhs[c];
+                }
+            }
+        }
+    }
+    __syncthreads();  // LDS tiles free for the next block
+}
+
+}  // namespace valu
+}  // namespace dgtd

@@ -159,6 +159,65 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops);
 int dgtd_cube_mesh(int32_t n, double L, const int8_t* side_bc, double* vx, int32_t* EToV,
                    int8_t* face_bc);
 
+/* ------------------------------------------------------------------------
+ * Batched instances: many independent meshes of uneven size and order
+ * advance together (no counterpart in the reference, which runs one mesh per
+ * process).  Each time step is one persistent launch per LSERK4 stage over a
+ * device work list of (instance, element block) items plus a bookkeeping
+ * launch, captured once in a hipGraph; instances keep their fields resident
+ * in HBM, carry their own dt and final time, and retire on the device with
+ * the Solver::run loop rule (Solver.cpp:497,537).
+ * ------------------------------------------------------------------------ */
+typedef struct dgtd_batch dgtd_batch;
+
+typedef struct {
+    double alpha;          /* upwind weight for all instances */
+    int32_t precision;     /* 64 or 32 */
+    int32_t device;        /* HIP device ordinal */
+    int32_t max_instances; /* slots, <= 4096 */
+    int32_t reserved;
+    int64_t hbm_budget;    /* bytes of instance storage admitted at once (0 = 80% of free HBM) */
+} dgtd_batch_opts;
+
+/* Instance states reported by dgtd_batch_info. */
+#define DGTD_INST_FREE 0
+#define DGTD_INST_RUNNING 1
+#define DGTD_INST_DONE 2
+
+int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out);
+void dgtd_batch_destroy(dgtd_batch* b);
+
+/* Device bytes one instance of this mesh holds (3 state buffers, geometry,
+ * connectivity) -- the admission measure. */
+int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t* bytes);
+
+/* Admit a mesh with initial fields x0 (host FP64, reference layout; NULL =
+ * zero), time step dt (<= 0: the Hesthaven rule 1/(max fscale p^2)) and final
+ * time.  Returns DGTD_E_CAPACITY when the HBM budget or the slots are full. */
+int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
+                   double final_time, int32_t* id);
+
+/* Launch nsteps time steps of all running instances (asynchronous). */
+int dgtd_batch_step(dgtd_batch* b, int32_t nsteps);
+
+/* Wait for the device, then report instances that finished since the last
+ * poll (up to cap ids) and the number still running. */
+int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndone, int32_t* nrunning);
+
+/* Per instance: time, steps taken, DGTD_INST_* state, dt in use. */
+int dgtd_batch_info(dgtd_batch* b, int32_t id, double* t, int32_t* steps, int32_t* state, double* dt);
+
+/* Fields of an instance (host FP64, reference layout). */
+int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x);
+
+/* Release an instance's device storage and slot. */
+int dgtd_batch_remove(dgtd_batch* b, int32_t id);
+
+/* Bytes admitted, running instances, work-list items, resident blocks of the
+ * persistent stage launch. */
+int dgtd_batch_stats(dgtd_batch* b, int64_t* bytes_in_use, int32_t* running, int32_t* items,
+                     int32_t* grid);
+
 /* Host-only introspection of the setup (no GPU needed). */
 
 /* Reference matrices of order p on the Hesthaven tetrahedron (volume 4/3),

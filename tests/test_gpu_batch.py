@@ -1,0 +1,115 @@
+"""Multi-instance batch on the GPU: uneven sizes and orders in one persistent
+launch per stage, hipGraph-replayed steps with host polls in between,
+per-instance dt and final time, admission by footprint, continuous batching.
+
+Bars: FP64 instances agree with the oracle run to 1e-11 relative L2 and take
+the same number of steps; an FP32 instance's fields are bitwise identical
+alone and in a mix (no cross-instance interference); FP32 vs FP64 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import dgtd_amd as D
+import meshgen as MG
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # n, order, final_time, bc
+    (2, 1, 0.30, "pec"),
+    (3, 2, 0.25, ["pec", "pmc", "sma", "pec", "pmc", "sma"]),
+    (2, 3, 0.20, "sma"),
+    (3, 4, 0.15, "pec"),
+    (2, 5, 0.10, "pmc"),
+]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def case_input(n, order, bc, seed):
+    vx, E, fbc = MG.cube_mesh(n, 1.0, bc)
+    ref = O.Operator(vx, E, fbc, order)
+    rng = np.random.default_rng(seed)
+    q0 = MG.gaussian(ref.positions(), rng.uniform(0.3, 0.7, 3), 0.15, comps=range(6))
+    return vx, E, fbc, ref, q0
+
+
+def test_mixed_orders_fp64_match_oracle():
+    b = D.Batch(precision=64, max_instances=16)
+    ids, refs = [], []
+    for k, (n, order, T, bc) in enumerate(CASES):
+        vx, E, fbc, ref, q0 = case_input(n, order, bc, k)
+        i = b.add(vx, E, fbc, order, x0=q0, dt=0.0, final_time=T)
+        dt = b.info(i)["dt"]
+        assert dt == pytest.approx(1.0 / (ref.max_fscale() * order ** 2), rel=1e-14)
+        x = q0.copy()
+        steps, t = ref.run(x, dt, T, "lserk4")
+        ids.append(i)
+        refs.append((x, steps, t))
+    total = 0
+    while True:
+        done, running = b.poll()
+        if running == 0:
+            break
+        b.step(3)  # graph replays, eager polls between them
+        total += 3
+    assert total >= max(r[1] for r in refs)
+    for i, (x, steps, t) in zip(ids, refs):
+        info = b.info(i)
+        assert info["state"] == "done"
+        assert info["steps"] == steps
+        assert info["t"] == pytest.approx(t, abs=1e-14)
+        assert rel(b.get_state(i), x) <= 1e-11
+
+
+def test_alone_equals_in_mix_fp32_and_tracks_fp64():
+    n, order, T, bc = 3, 3, 0.2, "pec"
+    vx, E, fbc, ref, q0 = case_input(n, order, bc, 42)
+    alone = D.Batch(precision=32)
+    a = alone.add(vx, E, fbc, order, x0=q0, final_time=T)
+    alone.run(poll_every=4)
+    xa = alone.get_state(a)
+
+    mix = D.Batch(precision=32)
+    others = []
+    for k, (n2, o2, T2, bc2) in enumerate(CASES):
+        vx2, E2, fbc2, _, q2 = case_input(n2, o2, bc2, 100 + k)
+        others.append(mix.add(vx2, E2, fbc2, o2, x0=q2, final_time=T2))
+    m = mix.add(vx, E, fbc, order, x0=q0, final_time=T)
+    mix.run(poll_every=5)
+    xm = mix.get_state(m)
+    np.testing.assert_array_equal(xm, xa)  # same result alone and in the mix
+
+    g64 = D.Batch(precision=64)
+    r = g64.add(vx, E, fbc, order, x0=q0, final_time=T)
+    g64.run()
+    assert rel(xm, g64.get_state(r)) <= 1e-5
+
+
+def test_continuous_admission_and_removal():
+    b = D.Batch(precision=32, max_instances=4)
+    vx, E, fbc, ref, q0 = case_input(2, 2, "pec", 7)
+    first = b.add(vx, E, fbc, 2, x0=q0, final_time=0.3)
+    b.step(5)
+    late = b.add(vx, E, fbc, 2, x0=q0, final_time=0.3)  # joins while the first runs
+    b.run(poll_every=2)
+    assert b.info(first)["steps"] == b.info(late)["steps"]
+    np.testing.assert_array_equal(b.get_state(first), b.get_state(late))
+    b.remove(first)
+    b.remove(late)
+    assert b.stats()["bytes_in_use"] == 0
+
+
+def test_admission_by_footprint():
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    fp = D.footprint(vx, E, fbc, 3, precision=32)
+    b = D.Batch(precision=32, max_instances=8, hbm_budget=int(2.5 * fp))
+    i1 = b.add(vx, E, fbc, 3, final_time=0.1)
+    b.add(vx, E, fbc, 3, final_time=0.1)
+    with pytest.raises(D.CapacityError):
+        b.add(vx, E, fbc, 3, final_time=0.1)
+    b.remove(i1)
+    b.add(vx, E, fbc, 3, final_time=0.1)
+    assert b.stats()["bytes_in_use"] == 2 * fp
