@@ -17,6 +17,7 @@ same mesh.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -78,6 +79,113 @@ def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0):
     return wall, steps, O.threads()
 
 
+def bench_c3(D, args):
+    """48^3 hexes x 6 = 663,552 tets on [0,2]^3, p=3, eps_r ~ U[1,4] per element
+    (seeded), mu_r = 1, SMA on all sides, Ez Gaussian pulse, dt = 5e-4,
+    FP32 state with FP64 energy reductions, 2000 LSERK4 steps (FP64 beside it)."""
+    n, L, order, dt = 48, 2.0, 3, 5e-4
+    steps = args.steps if args.steps != 100 else 2000
+    vx, E, fbc = D.cube_mesh(n, L, "sma")
+    K = E.shape[0]
+    rng = np.random.default_rng(args.seed)
+    eps = rng.uniform(1.0, 4.0, K)
+    c = rng.uniform(0.3, 0.7, 3) * L
+    x0 = D.project_initial_fields(vx, E, order, [D.gaussian("E", c, 0.05 * L, (0, 0, 1))])
+    dof = 6 * D.num_nodes(order) * K
+    out = {"workload": f"SMA box [0,{L}]^3, {n}^3 hexes x 6 = {K} tets, p={order}, eps_r~U[1,4], "
+                       f"Ez Gaussian at ({c[0]:.3f},{c[1]:.3f},{c[2]:.3f}), dt={dt}, {steps} LSERK4 steps",
+           "dof": dof}
+    states = {}
+    for prec in (32, 64):
+        op = D.HesthavenEvolution(vx, E, fbc, order, eps=eps, precision=prec, integrator="lserk4",
+                                  kernel=args.kernel)
+        op.set_state(x0)
+        e0 = op.norms()["energy"]
+        op.advance(dt, args.warmup)
+        op.set_timing(True)
+        t0 = time.perf_counter()
+        op.advance(dt, steps)
+        wall = time.perf_counter() - t0
+        us, launches = op.last_kernel_us()
+        nm = op.norms()
+        b, f = op.work_per_element()
+        states[prec] = op.get_state()
+        out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
+                            "kernel_avg_us": us, "kernel_variant": op.kernel_variant(),
+                            "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                            "energy_initial": e0, "energy_final": nm["energy"], "nonfinite": nm["nonfinite"]}
+        op.close()
+    out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(states[32] - states[64]) / np.linalg.norm(states[64]))
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    print(json.dumps(out))
+
+
+def bench_c4(D, args):
+    """64 independent PEC cavities, edge n_i log-uniform over {2..24} hexes,
+    order uniform over {1..5}, seeded Gaussian pulses, FP32, one persistent
+    launch per stage, hipGraph-replayed steps; all run to final time 0.1."""
+    rng = np.random.default_rng(args.seed)
+    T = 0.1
+    insts = []
+    for k in range(64):
+        n = int(round(np.exp(rng.uniform(np.log(2), np.log(24)))))
+        order = int(rng.integers(1, 6))
+        vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
+        c = rng.uniform(0.3, 0.7, 3)
+        x0 = D.project_initial_fields(vx, E, order, [D.gaussian("E", c, 0.1, (0, 0, 1))])
+        insts.append((n, order, vx, E, fbc, x0))
+    res = {}
+    for prec in (32, 64):
+        b = D.Batch(precision=prec, max_instances=64)
+        ids = [b.add(vx, E, fbc, order, x0=x0, final_time=T) for (n, order, vx, E, fbc, x0) in insts]
+        b.poll()
+        st = b.stats()
+        t0 = time.perf_counter()
+        launched = b.run(poll_every=16)
+        wall = time.perf_counter() - t0
+        work = 0
+        states = []
+        for (n, order, vx, E, fbc, x0), i in zip(insts, ids):
+            info = b.info(i)
+            work += 6 * D.num_nodes(order) * E.shape[0] * 5 * info["steps"]
+            states.append(b.get_state(i))
+        res[prec] = dict(value=work / wall, wall_s=wall, graph_steps=launched, items=st["items"],
+                         grid=st["grid"], states=states)
+        b.close()
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(res[32]["states"], res[64]["states"])]
+    out = {"workload": "64 PEC cavities, n~logU{2..24}, p~U{1..5}, Gaussian pulses, T=0.1, LSERK4",
+           "instances": [(n, o, int(E.shape[0])) for (n, o, vx, E, fbc, x0) in insts],
+           "total_dof": int(sum(6 * D.num_nodes(o) * E.shape[0] for (n, o, vx, E, fbc, x0) in insts))}
+    for prec in (32, 64):
+        r = res[prec]
+        out[f"fp{prec}"] = {"value": r["value"], "unit": "DOF*RK-stages/s", "wall_s": r["wall_s"],
+                            "graph_steps": r["graph_steps"], "work_items": r["items"], "grid": r["grid"]}
+    out["max_rel_l2_fp32_vs_fp64"] = max(errs)
+    out["fp32_over_fp64"] = res[32]["value"] / res[64]["value"]
+    print(json.dumps(out))
+
+
+def bench_c5(D, args):
+    """Poisson arrivals of cavity / SMA-box solves, K log-uniform 1k..400k tets,
+    p in {2,3,4}, final time U[0.5,2], eps_r ~ U[1,4] per element; FP32."""
+    from dgtd_amd import serve as S
+
+    reqs = S.poisson_stream(args.rate, args.duration, seed=args.seed)
+    srv = S.Server(precision=32, hbm_budget=int(64e9), max_instances=512, poll_every=4, workers=8)
+    recs = srv.serve(reqs, max_wall=args.duration * 3 + 60)
+    srv.close()
+    summ = S.summarize(recs, srv.wall)
+    done = [r for r in recs.values() if r.finished > 0]
+    busy = sum(r.dof * 5 * r.steps for r in done)
+    summ.update({"workload": f"Poisson {args.rate}/s for {args.duration}s, K~logU[1k,400k], p in {{2,3,4}}, "
+                             f"T~U[0.5,2], eps_r~U[1,4], FP32",
+                 "offered_dof_stages": float(sum(r.dof * 5 * math.ceil(r.final_time * 2 * math.sqrt(2) * r.n
+                                                                         * r.order ** 2) for r in reqs))})
+    if done:
+        summ["sustainable_rate_estimate_per_s"] = len(done) / srv.wall
+    print(json.dumps(summ))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +197,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default, the headline): 32^3 cavity p=4; c3: 48^3 SMA box with random eps, "
+                         "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream")
+    ap.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM-counter calibration)")
     args = ap.parse_args()
@@ -108,6 +221,12 @@ def main():
 
     import dgtd_amd as D
 
+    if args.workload == "c3":
+        return bench_c3(D, args)
+    if args.workload == "c4":
+        return bench_c4(D, args)
+    if args.workload == "c5":
+        return bench_c5(D, args)
     n, order = args.n, args.order
     vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
     K = E.shape[0]

@@ -39,6 +39,12 @@ class Opts(ctypes.Structure):
                 ("device", c_int32), ("kernel", c_int32)]
 
 
+class InitialField(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("field", c_int32), ("center", c_double * 3),
+                ("polarization", c_double * 3), ("spread", c_double), ("dimension", c_int32),
+                ("nmodes", c_int32), ("modes", c_int32 * 3)]
+
+
 class BatchOpts(ctypes.Structure):
     _fields_ = [("alpha", c_double), ("precision", c_int32), ("device", c_int32),
                 ("max_instances", c_int32), ("reserved", c_int32), ("hbm_budget", ctypes.c_int64)]
@@ -72,6 +78,8 @@ SIGNATURES = {
     "dgtd_op_kernel_variant": (c_int32, [c_void_p, _i32p]),
     "dgtd_op_work_per_element": (c_int32, [c_void_p, _dp, _dp]),
     "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
+    "dgtd_project_initial_fields": (c_int32, [ctypes.POINTER(MeshDesc), ctypes.POINTER(InitialField),
+                                              c_int32, _dp]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),
     "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -49,6 +50,10 @@ int64_t footprint(int K, int order, int precision)
 }  // namespace
 
 struct dgtd_batch {
+    // Every entry point holds `mu` while it touches the tables or the stream;
+    // dgtd_batch_add does its host setup and upload outside it, so callers may
+    // admit from worker threads while another thread steps the batch.
+    std::mutex mu;
     int device = 0, precision = 32;
     double alpha = 1.0;
     int max_slots = 0;
@@ -61,6 +66,8 @@ struct dgtd_batch {
     void* d_mats[dgtd::kMaxBatchOrder + 1] = {};
     int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
+
+    static constexpr int kSlotReserved = 3;  // host only: being set up by dgtd_batch_add
 
     struct Slot {
         int state = dgtd::kSlotFree;
@@ -107,7 +114,7 @@ struct dgtd_batch {
         hcheck(hipStreamSynchronize(stream), "sync");
         for (int i = 0; i < max_slots; ++i) {
             Slot& s = slots[i];
-            if (s.state == dgtd::kSlotFree) continue;
+            if (s.state == dgtd::kSlotFree || s.state == kSlotReserved) continue;
             s.state = h[i].state;
             s.t = h[i].t;
             s.cur = h[i].cur;
@@ -124,8 +131,8 @@ struct dgtd_batch {
             const Slot& s = slots[i];
             dgtd::BatchInst<T>& r = h[i];
             r = dgtd::BatchInst<T>{};
-            r.state = s.state;
-            if (s.state == dgtd::kSlotFree) continue;
+            r.state = s.state == kSlotReserved ? dgtd::kSlotFree : s.state;
+            if (r.state == dgtd::kSlotFree) continue;
             r.order = s.order;
             r.K = s.K;
             r.nblk = s.nblk;
@@ -225,10 +232,13 @@ struct dgtd_batch {
 
 namespace {
 
+// Runs without the batch lock: synchronous copies on the null stream (the
+// batch stream is non-blocking, so a step in flight is not waited for).
 template <typename T>
 void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const dgtd::MeshData& md, const double* x0,
                      size_t ndofs)
 {
+    (void)b;
     const size_t bytes_q = sizeof(T) * ndofs;
     for (int k = 0; k < 3; ++k) hcheck(hipMalloc(&s.buf[k], bytes_q), "hipMalloc instance");
     hcheck(hipMalloc(&s.buf[3], sizeof(T) * md.geo.size()), "hipMalloc geo");
@@ -236,13 +246,12 @@ void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const dgtd::MeshData& m
     std::vector<T> tmp(ndofs, T(0));
     if (x0)
         for (size_t i = 0; i < ndofs; ++i) tmp[i] = (T)x0[i];
-    hcheck(hipMemcpyAsync(s.buf[0], tmp.data(), bytes_q, hipMemcpyHostToDevice, b->stream), "H2D x0");
-    hcheck(hipMemsetAsync(s.buf[2], 0, bytes_q, b->stream), "memset res");
+    hcheck(hipMemcpy(s.buf[0], tmp.data(), bytes_q, hipMemcpyHostToDevice), "H2D x0");
+    std::fill(tmp.begin(), tmp.end(), T(0));
+    hcheck(hipMemcpy(s.buf[2], tmp.data(), bytes_q, hipMemcpyHostToDevice), "H2D res");
     const std::vector<T> g = dgtd::cast_vec<T>(md.geo);
-    hcheck(hipMemcpyAsync(s.buf[3], g.data(), sizeof(T) * g.size(), hipMemcpyHostToDevice, b->stream), "H2D geo");
-    hcheck(hipMemcpyAsync(s.buf[4], md.conn.data(), sizeof(int) * md.conn.size(), hipMemcpyHostToDevice, b->stream),
-           "H2D conn");
-    hcheck(hipStreamSynchronize(b->stream), "sync");
+    hcheck(hipMemcpy(s.buf[3], g.data(), sizeof(T) * g.size(), hipMemcpyHostToDevice), "H2D geo");
+    hcheck(hipMemcpy(s.buf[4], md.conn.data(), sizeof(int) * md.conn.size(), hipMemcpyHostToDevice), "H2D conn");
 }
 
 template <typename T>
@@ -329,10 +338,18 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
     if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");
     const int64_t bytes = footprint(mesh->K, mesh->order, b->precision);
     int slot = -1;
-    for (int i = 0; i < b->max_slots && slot < 0; ++i)
-        if (b->slots[i].state == dgtd::kSlotFree) slot = i;
-    if (slot < 0) return dgtd::api_fail(DGTD_E_CAPACITY, "no free instance slot");
-    if (b->used + bytes > b->budget) return dgtd::api_fail(DGTD_E_CAPACITY, "HBM budget exceeded");
+    {  // reserve a slot and the bytes
+        std::lock_guard<std::mutex> lk(b->mu);
+        for (int i = 0; i < b->max_slots && slot < 0; ++i)
+            if (b->slots[i].state == dgtd::kSlotFree) slot = i;
+        if (slot < 0) return dgtd::api_fail(DGTD_E_CAPACITY, "no free instance slot");
+        if (b->used + bytes > b->budget) return dgtd::api_fail(DGTD_E_CAPACITY, "HBM budget exceeded");
+        b->slots[slot] = dgtd_batch::Slot{};
+        b->slots[slot].state = dgtd_batch::kSlotReserved;
+        b->slots[slot].bytes = bytes;
+        b->used += bytes;
+    }
+    dgtd_batch::Slot s;  // built outside the lock
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
@@ -345,9 +362,6 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
         in.eps_r = mesh->eps_r;
         in.mu_r = mesh->mu_r;
         const dgtd::MeshData md = dgtd::build_mesh(in, R, b->alpha);
-        if (!b->dirty) b->sync_pull();  // device values are authoritative: fetch before editing
-        dgtd_batch::Slot& s = b->slots[slot];
-        s = dgtd_batch::Slot{};
         const size_t ndofs = 6 * (size_t)R.Np * mesh->K;
         if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
         else upload_instance<float>(b, s, md, x0, ndofs);
@@ -357,23 +371,29 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
         s.bytes = bytes;
         s.dt = dt > 0.0 ? dt : 1.0 / (md.max_fscale * mesh->order * mesh->order);
         s.final_time = final_time;
-        s.t = 0.0;
-        s.cur = 0;
-        s.steps = 0;
         if (0.0 <= final_time - 1e-8 * s.dt) {
             s.state = dgtd::kSlotRunning;
             s.dt_step = std::min(s.dt, final_time);
         } else {
             s.state = dgtd::kSlotDone;
         }
-        b->used += bytes;
-        b->dirty = true;
     } catch (const std::exception& e) {
-        dgtd_batch::Slot& s = b->slots[slot];
         for (void*& p : s.buf)
             if (p) { (void)hipFree(p); p = nullptr; }
-        s.state = dgtd::kSlotFree;
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->slots[slot] = dgtd_batch::Slot{};
+        b->used -= bytes;
         return batch_error(e);
+    }
+    {  // publish
+        std::lock_guard<std::mutex> lk(b->mu);
+        try {
+            if (!b->dirty) b->sync_pull();  // device values are authoritative: fetch before editing
+        } catch (const std::exception& e) {
+            return batch_error(e);
+        }
+        b->slots[slot] = s;
+        b->dirty = true;
     }
     *id = slot;
     return DGTD_OK;
@@ -382,6 +402,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 int dgtd_batch_step(dgtd_batch* b, int32_t nsteps)
 {
     if (!b || nsteps < 0) return dgtd::api_fail(DGTD_E_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(b->mu);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         b->flush();
@@ -395,6 +416,7 @@ int dgtd_batch_step(dgtd_batch* b, int32_t nsteps)
 int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndone, int32_t* nrunning)
 {
     if (!b) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(b->mu);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         b->flush();
@@ -420,6 +442,7 @@ int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndon
 int dgtd_batch_info(dgtd_batch* b, int32_t id, double* t, int32_t* steps, int32_t* state, double* dt)
 {
     if (!b || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad instance id");
+    std::lock_guard<std::mutex> lk(b->mu);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         b->flush();
@@ -438,12 +461,14 @@ int dgtd_batch_info(dgtd_batch* b, int32_t id, double* t, int32_t* steps, int32_
 int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x)
 {
     if (!b || !x || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(b->mu);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         b->flush();
         b->sync_pull();
         const auto& s = b->slots[id];
-        if (s.state == dgtd::kSlotFree) return dgtd::api_fail(DGTD_E_STATE, "instance slot is free");
+        if (s.state == dgtd::kSlotFree || s.state == dgtd_batch::kSlotReserved)
+            return dgtd::api_fail(DGTD_E_STATE, "instance slot is free or still being set up");
         const size_t ndofs = 6 * (size_t)dgtd::num_nodes(s.order) * s.K;
         if (b->precision == 64) {
             hcheck(hipMemcpy(x, s.buf[s.cur], sizeof(double) * ndofs, hipMemcpyDeviceToHost), "D2H state");
@@ -461,12 +486,14 @@ int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x)
 int dgtd_batch_remove(dgtd_batch* b, int32_t id)
 {
     if (!b || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad instance id");
+    std::lock_guard<std::mutex> lk(b->mu);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         if (!b->dirty) b->sync_pull();
         else hcheck(hipStreamSynchronize(b->stream), "sync");
         auto& s = b->slots[id];
-        if (s.state == dgtd::kSlotFree) return dgtd::api_fail(DGTD_E_STATE, "instance slot is free");
+        if (s.state == dgtd::kSlotFree || s.state == dgtd_batch::kSlotReserved)
+            return dgtd::api_fail(DGTD_E_STATE, "instance slot is free or still being set up");
         for (void*& p : s.buf)
             if (p) { hcheck(hipFree(p), "hipFree"); p = nullptr; }
         b->used -= s.bytes;
@@ -481,6 +508,7 @@ int dgtd_batch_remove(dgtd_batch* b, int32_t id)
 int dgtd_batch_stats(dgtd_batch* b, int64_t* bytes_in_use, int32_t* running, int32_t* items, int32_t* grid)
 {
     if (!b) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(b->mu);
     int nr = 0;
     for (const auto& s : b->slots) nr += s.state == dgtd::kSlotRunning;
     if (bytes_in_use) *bytes_in_use = b->used;

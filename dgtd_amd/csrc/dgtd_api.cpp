@@ -725,3 +725,43 @@ namespace dgtd {
 // Shared error reporting for the other ABI sources.
 int api_fail(int code, const std::string& msg) { return fail(code, msg); }
 }  // namespace dgtd
+
+extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_field* src,
+                                           int32_t nsrc, double* x)
+{
+    if (!mesh || !x || (nsrc > 0 && !src)) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+        const size_t N = (size_t)R.Np * mesh->K;
+        std::vector<double> xyz(3 * N);
+        dgtd::node_positions(mesh_input(mesh), R, xyz.data());
+        std::fill(x, x + 6 * N, 0.0);
+        for (int s = 0; s < nsrc; ++s) {
+            const dgtd_initial_field& f = src[s];
+            if (f.field != 0 && f.field != 1) throw std::runtime_error("field must be 0 (E) or 1 (H)");
+            if (f.kind == DGTD_FIELD_GAUSSIAN && (f.dimension < 1 || f.dimension > 3 || !(f.spread > 0)))
+                throw std::runtime_error("gaussian needs dimension 1..3 and spread > 0");
+            if (f.kind == DGTD_FIELD_RESONANT && (f.nmodes < 1 || f.nmodes > 3))
+                throw std::runtime_error("resonant mode needs 1..3 modes");
+            if (f.kind != DGTD_FIELD_GAUSSIAN && f.kind != DGTD_FIELD_RESONANT)
+                throw std::runtime_error("unknown initial field kind");
+            for (size_t n = 0; n < N; ++n) {
+                double y[3];
+                for (int d = 0; d < 3; ++d) y[d] = xyz[d * N + n] - f.center[d];
+                double v;
+                if (f.kind == DGTD_FIELD_GAUSSIAN) {
+                    double r2 = 0.0;
+                    for (int d = 0; d < f.dimension; ++d) r2 += y[d] * y[d];
+                    v = std::exp(-r2 / (2.0 * f.spread * f.spread));
+                } else {
+                    v = 1.0;
+                    for (int d = 0; d < f.nmodes; ++d) v *= std::sin(f.modes[d] * M_PI * y[d]);
+                }
+                for (int d = 0; d < 3; ++d) x[(3 * f.field + d) * N + n] += v * f.polarization[d];
+            }
+        }
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}

@@ -56,6 +56,35 @@ class _Mesh:
                                   dptr(self.mu))
 
 
+def gaussian(field="E", center=(0.5, 0.5, 0.5), spread=0.1, polarization=(0, 0, 1), dimension=3):
+    """Gaussian initial field (the reference's "gaussian" source, driver.cpp:368-380)."""
+    f = _abi.InitialField()
+    f.kind, f.field = 0, 0 if field == "E" else 1
+    f.center[:] = list(center)
+    f.polarization[:] = list(polarization)
+    f.spread, f.dimension = float(spread), int(dimension)
+    return f
+
+
+def resonant(modes=(1, 1), field="E", polarization=(0, 0, 1)):
+    """Sinusoidal cavity mode prod sin(m_k pi x_k) (the "resonant" source, driver.cpp:382-391)."""
+    f = _abi.InitialField()
+    f.kind, f.field = 1, 0 if field == "E" else 1
+    f.polarization[:] = list(polarization)
+    f.nmodes = len(modes)
+    f.modes[: len(modes)] = list(modes)
+    return f
+
+
+def project_initial_fields(vx, EToV, order, sources):
+    """Sum of the sources evaluated at the nodes: a 6*Np*K state vector."""
+    m = _Mesh(vx, EToV, None, order)
+    arr = (_abi.InitialField * len(sources))(*sources)
+    x = np.empty(6 * num_nodes(order) * m.EToV.shape[0])
+    check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), arr, len(sources), dptr(x)))
+    return x
+
+
 def ref_element(order):
     """The library's reference matrices (host FP64 setup)."""
     Np, Nfp = num_nodes(order), num_face_nodes(order)

@@ -159,6 +159,30 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops);
 int dgtd_cube_mesh(int32_t n, double L, const int8_t* side_bc, double* vx, int32_t* EToV,
                    int8_t* face_bc);
 
+/* Initial fields, evaluated at the nodes (the reference's L2 nodal
+ * projection of InitialField sources, SourcesManager.cpp:16-36 with
+ * InitialField::eval, Sources.cpp:37-57):
+ *   value(x) = f(x - center) * polarization[d] added to component (field, d)
+ *   gaussian: f = exp(-sum_{k<dimension} y_k^2 / (2 spread^2))  (Function.h:45-100)
+ *   resonant: f = prod_{k<nmodes} sin(modes[k] pi y_k)          (Function.h:145-168)
+ * Several sources add up, as setInitialFields does. */
+#define DGTD_FIELD_GAUSSIAN 0
+#define DGTD_FIELD_RESONANT 1
+typedef struct {
+    int32_t kind;            /* DGTD_FIELD_* */
+    int32_t field;           /* 0 electric, 1 magnetic */
+    double center[3];
+    double polarization[3];
+    double spread;           /* gaussian */
+    int32_t dimension;       /* gaussian: 1..3 coordinates used */
+    int32_t nmodes;          /* resonant: 1..3 */
+    int32_t modes[3];
+} dgtd_initial_field;
+
+/* x (6*Np*K doubles, reference layout) = sum of the sources at the nodes. */
+int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_field* sources,
+                                int32_t nsources, double* x);
+
 /* ------------------------------------------------------------------------
  * Batched instances: many independent meshes of uneven size and order
  * advance together (no counterpart in the reference, which runs one mesh per

@@ -123,3 +123,22 @@ def test_create_rejects_invalid_input_before_touching_the_gpu():
     E2[0, 0] = 10 ** 6
     with pytest.raises(D.DgtdError, match="out of range"):
         D.mesh_vmaps(vx, E2, fbc, 2)
+
+
+def test_initial_field_projection_matches_reference_sources():
+    """TE110 = "resonant" modes [1,1] polarised along z; a Gaussian on all six
+    components = six unit-polarised sources; evaluated at the oracle's nodes."""
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    ref = O.Operator(vx, E, fbc, 3)
+    pos = ref.positions()
+    x = D.project_initial_fields(vx, E, 3, [D.resonant((1, 1))])
+    np.testing.assert_allclose(x, MG.te110(pos), atol=1e-14)
+    c = (0.4, 0.55, 0.5)
+    srcs = [D.gaussian(f, c, 0.1, tuple(float(k == d) for k in range(3))) for f in "EH" for d in range(3)]
+    x = D.project_initial_fields(vx, E, 3, srcs)
+    np.testing.assert_allclose(x, MG.gaussian(pos, c, 0.1, comps=range(6)), atol=1e-14)
+    # a 1D Gaussian uses x only (Function.h:73-78)
+    x = D.project_initial_fields(vx, E, 3, [D.gaussian("H", c, 0.2, (0, 1, 0), dimension=1)])
+    g = np.exp(-(pos[0] - c[0]) ** 2 / (2 * 0.04))
+    np.testing.assert_allclose(x.reshape(6, -1)[4], g.ravel(), atol=1e-14)
+    assert np.abs(np.delete(x.reshape(6, -1), 4, axis=0)).max() == 0
