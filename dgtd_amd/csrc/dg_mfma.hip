@@ -6,22 +6,23 @@
 // metric-weighted curl combinations of two field components
 // (e.g. r_Hx: w_k = (g_kz E_y - g_ky E_z)/mu, g_kd = dr_k/dx_d) and f_c the
 // fscale/2-scaled face flux (with 1/eps or 1/mu folded in).  Both terms are
-// products with matrices shared by all elements, so for a wave-task of 16
-// elements the kernel evaluates
-//     R^T (16 x Np) = [W_r W_s W_t F]^T (16 x (3Np + 4Nfp)) * [D_r D_s D_t LIFT]^T
-// with v_mfma_{f32,f64}_16x16x4: rows = the 16 elements, columns = output
-// nodes in groups of 16, K = input nodes (for D_k) and face nodes (for LIFT).
+// products with matrices shared by all elements, so a wave-task of 8 elements
+// evaluates, for x = 0..2,
+//     [r_Ex ; r_Hx]^T (16 x Np) = [W_x F_x]^T (16 x (3Np + 4Nfp)) * [D_r D_s D_t LIFT]^T
+// with v_mfma_{f32,f64}_16x16x4: A rows 0-7 carry the E_x operands and rows
+// 8-15 the H_x operands of the same 8 elements, columns = output nodes in
+// groups of 16, K = input nodes (for D_k) and face nodes (for LIFT).
 //   - A operands (W, F) are formed in registers from the LDS-staged fields:
-//     lane (element l%16, slot l/16) -- for the LIFT the slot is the face, so
-//     each lane gathers the neighbour trace and forms the flux of one face
-//     node per k-step; fluxes never touch LDS or HBM.
-//   - B operands (the reference matrices, zero-padded to 16-column groups)
-//     stay in LDS for the life of the block.
+//     lane (row l%16, slot l/16); for the LIFT a k-step covers four
+//     consecutive nodes of one face, each lane gathering the neighbour trace
+//     of its node and forming the flux there; fluxes never touch LDS or HBM.
+//   - B operands (the reference matrices, zero-padded to 16-column groups and
+//     to whole k-steps per face) stay in LDS for the life of the block.
 //   - The accumulators are the finished right-hand sides; the stage update is
 //     applied straight from them.
 // Waves run independent tasks (only the initial matrix load is block-wide);
-// blocks take contiguous task ranges per XCD so neighbour re-reads stay in
-// that XCD's L2.
+// tasks run in Morton order of their centroids, in contiguous ranges per XCD
+// so neighbour re-reads stay in that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -70,18 +71,23 @@ struct MShape {
     static constexpr int NPG = (NP + 15) / 16 * 16;
     static constexpr int G = NPG / 16;           // output-node groups
     static constexpr int US = (KP % 2 == 0) ? KP + 1 : KP;  // LDS row stride of a task's fields
+    static constexpr int NFPP = (NFP + 3) / 4 * 4;  // face nodes padded to whole k-steps
+    static constexpr int SF = NFPP / 4;             // LIFT k-steps per face
 };
+
+constexpr int kTaskElems = kMfmaTaskElems;  // MFMA rows: 8 elements x {E, H} outputs
 
 // Waves per block, chosen so the block's LDS fits in 160 KiB.
 template <int P, typename T>
 struct MWaves {
+    static constexpr int kMaxWaves = sizeof(T) == 4 ? 12 : 8;
     using S = MShape<P>;
-    static constexpr size_t mats = sizeof(T) * ((size_t)S::KP * 3 * S::NPG + (size_t)S::NF * S::NPG) +
+    static constexpr size_t mats = sizeof(T) * ((size_t)S::KP * 3 * S::NPG + (size_t)4 * S::NFPP * S::NPG) +
                                    sizeof(int) * (size_t)(4 * S::NFP + 96 * S::NFP);
-    static constexpr size_t per_wave = sizeof(T) * 6 * 16 * (size_t)S::US;
+    static constexpr size_t per_wave = sizeof(T) * (6 * kTaskElems * (size_t)S::US + 2 * kTaskElems * (size_t)S::NP);
     static constexpr size_t budget = 160 * 1024;
     static constexpr int fit = (int)((budget - mats) / per_wave);
-    static constexpr int W = fit >= 8 ? 8 : (fit < 1 ? 1 : fit);
+    static constexpr int W = fit >= kMaxWaves ? kMaxWaves : (fit < 1 ? 1 : fit);
     static constexpr size_t bytes = mats + per_wave * W;
     static constexpr bool ok = bytes <= budget;  // FP64 at order 5 does not fit
 };
@@ -99,36 +105,42 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
     using S = MShape<P>;
     using MM = Mfma<T>;
     using acc_t = typename MM::acc_t;
-    constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, KP = S::KP, NPG = S::NPG, G = S::G, US = S::US;
+    constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, G = S::G, US = S::US;
+    constexpr int NFPP = S::NFPP, SF = S::SF;
+    constexpr int TE = kTaskElems;
     constexpr int WAVES = MWaves<P, T>::W;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* sBv = reinterpret_cast<T*>(smem_raw);     // [KP][3*NPG]
-    T* sBl = sBv + KP * 3 * NPG;                 // [NF][NPG]
-    T* sU = sBl + NF * NPG;                      // [WAVES][6][16][US]
-    int* sFm = reinterpret_cast<int*>(sU + WAVES * 6 * 16 * US);  // [4*NFP]
+    T* sBl = sBv + KP * 3 * NPG;                 // [4*NFPP][NPG], row f*NFPP + b
+    T* sU = sBl + 4 * NFPP * NPG;                // [WAVES][6][TE][US]
+    T* sS = sU + WAVES * 6 * TE * US;            // [WAVES][2][TE*NP] output staging
+    int* sFm = reinterpret_cast<int*>(sS + WAVES * 2 * TE * NP);  // [4*NFP]
     int* sPt = sFm + 4 * NFP;                                     // [96*NFP]
 
     const int tid = threadIdx.x;
     const int nthreads = WAVES * 64;
-    for (int i = tid; i < KP * 3 * NPG + NF * NPG; i += nthreads) sBv[i] = a.mats[i];
+    for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) sBv[i] = a.mats[i];
     for (int i = tid; i < 4 * NFP + 96 * NFP; i += nthreads) sFm[i] = a.tabs[i];
-    for (int i = tid; i < WAVES * 6 * 16 * US; i += nthreads) sU[i] = T(0);  // padding j >= NP stays 0
+    for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) sU[i] = T(0);  // padding j >= NP stays 0
     __syncthreads();
 
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int q = lane >> 4;   // slot: k within a step / face for the LIFT
-    const int li = lane & 15;  // A-row element / output node within a group
-    T* U = sU + wave * 6 * 16 * US;
+    const int q = lane >> 4;    // k slot within a step
+    const int li = lane & 15;   // A row (= output column within a node group)
+    const int el = li & 7;      // element of the A row
+    const int half = li >> 3;   // A rows 0-7: E outputs, 8-15: H outputs of the same elements
+    T* U = sU + wave * 6 * TE * US;
+    T* Sg = sS + wave * 2 * TE * NP;
 
     const int K = a.K;
     const size_t N = (size_t)NP * K;
-    const int ntasks = (K + 15) / 16;
+    const int ntasks = (K + TE - 1) / TE;
     // XCD-contiguous task ranges: block b runs on XCD b % 8.
     const int nxcd = (gridDim.x >= 8 && gridDim.x % 8 == 0) ? 8 : 1;
     const int xcd = blockIdx.x % nxcd;
-    const int bpx = gridDim.x / nxcd;  // blocks per XCD (grid is a multiple of nxcd)
+    const int bpx = gridDim.x / nxcd;
     const int bin = blockIdx.x / nxcd;
     const int per_xcd = (ntasks + nxcd - 1) / nxcd;
     const int t_begin = xcd * per_xcd;
@@ -136,15 +148,13 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
 
     for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES) {
         const int task = a.order ? a.order[tix] : tix;
-        const int e0 = task * 16;
-        const int ne = min(16, K - e0);
+        const int e0 = task * TE;
+        const int ne = min(TE, K - e0);
 
-        // ---- stage the 16 elements' fields in LDS (coalesced per component) ----
-        // All loads of the task are issued before the first LDS write so the
-        // wave has 6*ceil(16*Np/64) requests in flight.
+        // ---- stage the task's fields in LDS: all loads in flight, then the writes ----
         wave_sync();  // previous task's reads of U are done
         {
-            constexpr int NL = (16 * NP + 63) / 64;
+            constexpr int NL = (TE * NP + 63) / 64;
             const int cnt = ne * NP;
             T v[6][NL];
 #pragma unroll
@@ -162,179 +172,209 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
                 for (int k = 0; k < NL; ++k) {
                     const int t = lane + 64 * k;
                     if (t < cnt) {
-                        const int el = t / NP;
-                        U[(c * 16 + el) * US + (t - el * NP)] = v[c][k];
+                        const int r = t / NP;
+                        U[(c * TE + r) * US + (t - r * NP)] = v[c][k];
                     }
                 }
         }
         wave_sync();
 
-        // ---- per-lane element (li) and face (q) data ----
-        const bool valid = li < ne;
-        const int e = e0 + (valid ? li : 0);
+        // ---- per-lane element data; half 0 forms E-output operands, half 1 H-output ----
+        const bool valid = el < ne;
+        const int e = e0 + (valid ? el : 0);
         const T* ge = a.geo + (size_t)e * kGeoStrideDev;
         T gm[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) gm[k] = ge[k];
-        const T ieps = ge[25], imu = ge[26];
-        const T nx = ge[9 + 4 * q], ny = ge[10 + 4 * q], nz = ge[11 + 4 * q], fs = ge[12 + 4 * q];
-        const int nb = a.conn[(size_t)e * kConnStrideDev + q];
-        const int code = a.conn[(size_t)e * kConnStrideDev + 4 + q];
-        T ce = T(0), ch = T(0);  // boundary jump coefficients (HesthavenEvolution.cpp:290-292)
-        if (nb < 0) {
-            const int bc = -nb - 1;
-            if (bc == 1) ce = T(-2);
-            else if (bc == 2) ch = T(-2);
-            else if (bc == 3) { ce = T(-1) / a.alpha; ch = ce; }
+        // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
+        const T scale = valid ? (half ? ge[26] : ge[25]) : T(0);
+        const T sign = half ? T(-1) : T(1);
+        const T sscale = sign * scale;
+
+        acc_t R[3][G];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
+
+        // ---- surface -> LIFT ----
+        // k-step s covers face f = s / SF, nodes b = 4*(s % SF) + q, so the
+        // lanes of an element read the same neighbour element.  Neighbours in
+        // this task come from LDS; others from memory, prefetched one step
+        // ahead; boundary faces use the own trace with the boundary
+        // coefficients (HesthavenEvolution.cpp:290-292).
+        const T* Ue = U + el * US;
+        int fnb[4], fcode[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
+            fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
         }
-        const T sE = valid ? fs * ieps : T(0);  // fold 1/eps, 1/mu and fscale/2 into F
-        const T sH = valid ? fs * imu : T(0);
-
-        acc_t R[6][G];
+        auto remote_of = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
+        auto node_of = [&](int f, int b) {
+            const int bb = b < NFP ? b : NFP - 1;
+            return fnb[f] >= 0 ? sPt[fcode[f] * NFP + bb] : sFm[f * NFP + bb];
+        };
+        T pf[6];
+        if (remote_of(0)) {
+            const T* src = a.qin + (size_t)fnb[0] * NP + node_of(0, q);
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int g = 0; g < G; ++g) R[c][g] = acc_t{0, 0, 0, 0};
-
-        // ---- surface: flux of face q, node b of element li -> LIFT ----
-        // Neighbour traces: from LDS when the neighbour is in this task, else
-        // a global gather (prefetched two face nodes ahead); boundary faces
-        // use the element's own trace with the boundary coefficients.
-        const T* Ue = U + li * US;
-        const bool interior = nb >= 0;
-        const bool local = interior && (unsigned)(nb - e0) < (unsigned)ne;
-        const bool remote = interior && !local;
-        const T* qP = a.qin + (size_t)(remote ? nb : e) * NP;
-        const T* Un = U + (local ? nb - e0 : li) * US;
-        const int* pmap = interior ? sPt + code * NFP : sFm + q * NFP;
-        const T cI = interior ? T(1) : T(0);  // weight of the neighbour trace
-        const T cE = interior ? T(-1) : ce;   // weight of the own trace, E components
-        const T cH = interior ? T(-1) : ch;   // weight of the own trace, H components
-        T pf0[6], pf1[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) { pf0[c] = T(0); pf1[c] = T(0); }
-        if (remote) {
-            const int i0 = pmap[0], i1 = pmap[NFP > 1 ? 1 : 0];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) { pf0[c] = qP[c * N + i0]; pf1[c] = qP[c * N + i1]; }
+            for (int c = 0; c < 6; ++c) pf[c] = src[c * N];
         }
 #pragma unroll
-        for (int b = 0; b < NFP; ++b) {
-            const int ip = pmap[b];
+        for (int s = 0; s < 4 * SF; ++s) {
+            const int f = s / SF;
+            const int b = 4 * (s % SF) + q;
+            const int nb = fnb[f];
+            const bool interior = nb >= 0;
+            const bool remote = remote_of(f);
+            const int ip = node_of(f, b);
             T cur[6];
+            {
+                const T* Un = U + (interior && !remote ? nb - e0 : el) * US;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                cur[c] = remote ? pf0[c] : Un[c * 16 * US + ip];
-                pf0[c] = pf1[c];
+                for (int c = 0; c < 6; ++c) cur[c] = remote ? pf[c] : Un[c * TE * US + ip];
             }
-            if (remote && b + 2 < NFP) {
-                const int i2 = pmap[b + 2];
+            if (s + 1 < 4 * SF) {
+                const int f1 = (s + 1) / SF;
+                if (remote_of(f1)) {
+                    const T* src = a.qin + (size_t)fnb[f1] * NP + node_of(f1, 4 * ((s + 1) % SF) + q);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) pf1[c] = qP[c * N + i2];
+                    for (int c = 0; c < 6; ++c) pf[c] = src[c * N];
+                }
             }
-            const int iM = sFm[q * NFP + b];
-            T d[6];
+            T ce = T(-1), ch = T(-1), cI = T(1);
+            if (!interior) {
+                const int bc = -nb - 1;
+                cI = T(0);
+                ce = bc == 1 ? T(-2) : (bc == 3 ? T(-1) / a.alpha : T(0));
+                ch = bc == 2 ? T(-2) : (bc == 3 ? T(-1) / a.alpha : T(0));
+            }
+            const int iM = sFm[f * NFP + (b < NFP ? b : NFP - 1)];
+            // A = jump of the "other" field, B = jump of the output's own field
+            T A3[3], B3[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                d[c] = cI * cur[c] + cE * Ue[c * 16 * US + iM];
-                d[3 + c] = cI * cur[3 + c] + cH * Ue[(3 + c) * 16 * US + iM];
+                const T dE = cI * cur[c] + ce * Ue[c * TE * US + iM];
+                const T dH = cI * cur[3 + c] + ch * Ue[(3 + c) * TE * US + iM];
+                A3[c] = half ? dE : dH;
+                B3[c] = half ? dH : dE;
             }
-            const T n3[3] = {nx, ny, nz};
-            const T ndH = nx * d[3] + ny * d[4] + nz * d[5];
-            const T ndE = nx * d[0] + ny * d[1] + nz * d[2];
-            T F[6];
+            const T* gf = ge + 9 + 4 * f;
+            const T n3[3] = {gf[0], gf[1], gf[2]};
+            const T fs = (b < NFP) ? gf[3] : T(0);  // padded face slots add nothing
+            const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
+            // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
+            T F[3];
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
                 const int y = (x + 1) % 3, z = (x + 2) % 3;
-                F[3 + x] = (-n3[y] * d[z] + n3[z] * d[y] + a.alpha * (d[3 + x] - ndH * n3[x])) * sH;
-                F[x] = (n3[y] * d[3 + z] - n3[z] * d[3 + y] + a.alpha * (d[x] - ndE * n3[x])) * sE;
+                F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * (fs * scale);
             }
-            const T* bl = sBl + (q * NFP + b) * NPG + li;
+            const T* bl = sBl + (f * NFPP + b) * NPG + li;
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const T bv = bl[16 * g];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) R[c][g] = MM::mma(F[c], bv, R[c][g]);
+                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
             }
         }
 
-        // ---- volume: sum_k D_k w_{c,k} ----
-        const T wE = valid ? ieps : T(0), wH = valid ? imu : T(0);
+        // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
+        // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
 #pragma unroll 3
         for (int ks = 0; ks < S::S; ++ks) {
             const int j = q * S::S + ks;
-            T u[6];
+            T src[3];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) u[c] = Ue[c * 16 * US + j];
+            for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
             const T* bv = sBv + j * 3 * NPG + li;
 #pragma unroll
             for (int kd = 0; kd < 3; ++kd) {
-                const T* gk = gm + 3 * kd;  // dr_kd/dx_d, d = 0..2
-                T w[6];
+                const T* gk = gm + 3 * kd;
+                T w[3];
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
                     const int y = (x + 1) % 3, z = (x + 2) % 3;
-                    w[3 + x] = (gk[z] * u[y] - gk[y] * u[z]) * wH;          // H_x: -D_y E_z + D_z E_y
-                    w[x] = (gk[y] * u[3 + z] - gk[z] * u[3 + y]) * wE;      // E_x:  D_y H_z - D_z H_y
+                    w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
                 }
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     const T bval = bv[kd * NPG + 16 * g];
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) R[c][g] = MM::mma(w[c], bval, R[c][g]);
+                    for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
                 }
             }
         }
 
-        // ---- stage update straight from the accumulators ----
-        // Per output-node group: every global read of the group is issued
-        // before its first store (reads and stores may alias as far as the
-        // compiler knows, which would otherwise serialise them pairwise).
+        // ---- stage update: per x, the E_x and H_x right-hand sides go through
+        // the wave's LDS staging buffer and the update runs as a linear pass
+        // over the task's contiguous chunk (whole sectors, coalesced).  The
+        // residual / base state of the pair x+1 (linear index t over
+        // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
+        constexpr int NLIN = (2 * TE * NP + 63) / 64;
+        constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
+        constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
+        auto lin_index = [&](int x, int t, bool& okt) {
+            const int h = t >= TE * NP;
+            const int u = t - h * TE * NP;
+            okt = t < 2 * TE * NP && u < ne * NP;
+            return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (okt ? u : 0);
+        };
+        T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
+        auto fetch = [&](int x, int slot) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int i = 16 * g + li;
-            size_t o[4];
-            bool ok[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int el = MM::row(q, r);
-                ok[r] = (i < NP) && (el < ne);
-                o[r] = (size_t)(e0 + (ok[r] ? el : 0)) * NP + (ok[r] ? i : 0);
+            for (int k = 0; k < NLIN; ++k) {
+                bool okt;
+                const size_t o = lin_index(x, lane + 64 * k, okt);
+                if constexpr (kNeedRes) rl[slot][k] = okt ? a.res[o] : T(0);
+                if constexpr (kNeedX) xl[slot][k] = okt ? a.x[o] : T(0);
             }
-            T rv[6][4], xv[6][4];
+        };
+        fetch(0, 0);
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
+        for (int x = 0; x < 3; ++x) {
+            const int cur = x & 1;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
-                        rv[c][r] = ok[r] ? a.res[c * N + o[r]] : T(0);
-                    if constexpr (MODE == kModeRk2 || MODE == kModeRk3)
-                        xv[c][r] = ok[r] ? a.x[c * N + o[r]] : T(0);
-                }
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
+            for (int g = 0; g < G; ++g) {
+                const int i = 16 * g + li;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    if (!ok[r]) continue;
-                    const T rhs = R[c][g][r];
-                    const size_t oo = c * N + o[r];
-                    const T qv = U[(c * 16 + MM::row(q, r)) * US + i];
-                    if constexpr (MODE == kModeMult) {
-                        a.out[oo] = rhs;
-                    } else if constexpr (MODE == kModeLserk) {
-                        const T rr = a.ca * rv[c][r] + a.dt * rhs;
-                        a.res[oo] = rr;
-                        a.out[oo] = qv + a.cb * rr;
-                    } else if constexpr (MODE == kModeRk1) {
-                        a.out[oo] = qv + a.ca * rhs;
-                        a.res[oo] = qv + a.cb * rhs;
-                    } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                        a.out[oo] = xv[c][r] + a.ca * rhs;
-                        a.res[oo] = rv[c][r] + a.cb * rhs;
-                    } else {
-                        a.out[oo] = rv[c][r] + a.ca * rhs;
-                    }
+                    const int row = MM::row(q, r);
+                    if (i < NP) Sg[(row >> 3) * TE * NP + (row & 7) * NP + i] = R[x][g][r];
                 }
+            }
+            wave_sync();
+            if (x + 1 < 3) fetch(x + 1, cur ^ 1);
+#pragma unroll
+            for (int k = 0; k < NLIN; ++k) {
+                const int t = lane + 64 * k;
+                bool okt;
+                const size_t o = lin_index(x, t, okt);
+                if (!okt) continue;
+                const int h = t >= TE * NP;
+                const int u = t - h * TE * NP;
+                const int c = h ? 3 + x : x;
+                const int r_el = u / NP;
+                const T rhs = Sg[t];
+                const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
+                if constexpr (MODE == kModeMult) {
+                    a.out[o] = rhs;
+                } else if constexpr (MODE == kModeLserk) {
+                    const T rr = a.ca * rl[cur][k] + a.dt * rhs;
+                    a.res[o] = rr;
+                    a.out[o] = qv + a.cb * rr;
+                } else if constexpr (MODE == kModeRk1) {
+                    a.out[o] = qv + a.ca * rhs;
+                    a.res[o] = qv + a.cb * rhs;
+                } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
+                    a.out[o] = xl[cur][k] + a.ca * rhs;
+                    a.res[o] = rl[cur][k] + a.cb * rhs;
+                } else {
+                    a.out[o] = rl[cur][k] + a.ca * rhs;
+                }
+            }
+            wave_sync();  // staging buffer free for the next x
         }
     }
 }

@@ -241,17 +241,18 @@ struct dgtd_op {
 
 namespace {
 
-// Processing order of the 16-element tasks of the MFMA variant: Morton
+// Processing order of the wave-tasks of the MFMA variant: Morton
 // (Z-order) of the task centroids, so that tasks in flight together (and the
 // contiguous ranges the kernel gives each XCD) are compact in space and their
 // neighbour traces are reused from L2.
 std::vector<int> morton_task_order(const dgtd_mesh_desc* mesh)
 {
-    const int K = mesh->K, nt = (K + 15) / 16;
+    const int TE = dgtd::kMfmaTaskElems;
+    const int K = mesh->K, nt = (K + TE - 1) / TE;
     std::vector<double> c(3 * (size_t)nt, 0.0);
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int t = 0; t < nt; ++t) {
-        const int e0 = 16 * t, e1 = std::min(K, e0 + 16);
+        const int e0 = TE * t, e1 = std::min(K, e0 + TE);
         for (int e = e0; e < e1; ++e)
             for (int v = 0; v < 4; ++v)
                 for (int d = 0; d < 3; ++d) c[3 * (size_t)t + d] += mesh->vx[3 * (size_t)mesh->EToV[4 * (size_t)e + v] + d];
@@ -393,7 +394,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->nblk = (op->K + epb - 1) / epb;
         op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * bpc));
         if (mbpc > 0) {
-            const int ntasks = (op->K + 15) / 16;
+            const int ntasks = (op->K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
             op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
             int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
             if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)

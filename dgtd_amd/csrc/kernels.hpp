@@ -46,7 +46,9 @@ int stage_blocks_per_cu(int order);
 
 int elements_per_block(int order);
 
-// MFMA variant (dg_mfma.hip): 16-element wave-tasks, waves_per_block waves.
+// MFMA variant (dg_mfma.hip): wave-tasks of kMfmaTaskElems elements,
+// waves_per_block waves.
+constexpr int kMfmaTaskElems = 8;
 // mats layout: Bv [KP][3*NPG] then Bl [4*Nfp][NPG] (see pack in dgtd_api.cpp).
 template <typename T>
 hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);

@@ -25,20 +25,23 @@ inline std::vector<T> pack_mats(const RefTet& R)
 }
 
 // B operands of the MFMA variant: Bv[j][kd*NPG + i] = D_kd[i][j] (KP rows),
-// then Bl[m][i] = LIFT[i][m] (4*Nfp rows); zero outside i, j < Np.
+// then Bl[f*NFPP + b][i] = LIFT[i][f*Nfp + b] (4*NFPP rows, faces padded to a
+// multiple of 4 nodes); zero outside i, j < Np and b < Nfp.
 template <typename T>
 inline std::vector<T> pack_mats_mfma(const RefTet& R)
 {
-    const int Np = R.Np, NF = 4 * R.Nfp;
-    const int KP = (Np + 3) / 4 * 4, NPG = (Np + 15) / 16 * 16;
-    std::vector<T> m((size_t)KP * 3 * NPG + (size_t)NF * NPG, T(0));
+    const int Np = R.Np, Nfp = R.Nfp, NF = 4 * Nfp;
+    const int KP = (Np + 3) / 4 * 4, NPG = (Np + 15) / 16 * 16, NFPP = (Nfp + 3) / 4 * 4;
+    std::vector<T> m((size_t)KP * 3 * NPG + (size_t)4 * NFPP * NPG, T(0));
     const std::vector<double>* D[3] = {&R.Dr, &R.Ds, &R.Dt};
     for (int j = 0; j < Np; ++j)
         for (int kd = 0; kd < 3; ++kd)
             for (int i = 0; i < Np; ++i) m[(size_t)j * 3 * NPG + kd * NPG + i] = (T)(*D[kd])[(size_t)i * Np + j];
     T* L = m.data() + (size_t)KP * 3 * NPG;
-    for (int k = 0; k < NF; ++k)
-        for (int i = 0; i < Np; ++i) L[(size_t)k * NPG + i] = (T)R.LIFT[(size_t)i * NF + k];
+    for (int f = 0; f < 4; ++f)
+        for (int b = 0; b < Nfp; ++b)
+            for (int i = 0; i < Np; ++i)
+                L[(size_t)(f * NFPP + b) * NPG + i] = (T)R.LIFT[(size_t)i * NF + f * Nfp + b];
     return m;
 }
 

@@ -82,7 +82,7 @@ typedef struct {
 /* Stage-kernel variants (both fuse gather, flux, curl, LIFT and the RK update). */
 #define DGTD_KERNEL_AUTO 0 /* time the available variants on this mesh at create, keep the fastest */
 #define DGTD_KERNEL_VALU 1 /* per-node vector FMA, reference matrices in LDS */
-#define DGTD_KERNEL_MFMA 2 /* matrix cores (v_mfma 16x16x4), 16-element wave tasks */
+#define DGTD_KERNEL_MFMA 2 /* matrix cores (v_mfma 16x16x4), 8-element wave tasks */
 
 const char* dgtd_last_error(void);
 const char* dgtd_version(void);
