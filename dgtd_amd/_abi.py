@@ -50,7 +50,15 @@ class BatchOpts(ctypes.Structure):
                 ("max_instances", c_int32), ("reserved", c_int32), ("hbm_budget", ctypes.c_int64)]
 
 
+class CaseInfo(ctypes.Structure):
+    _fields_ = [("K", c_int32), ("nv", c_int32), ("order", c_int32), ("alpha", c_double),
+                ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
+                ("evolution_operator", c_int32), ("nsources", c_int32), ("nprobes", c_int32),
+                ("exporter_steps", c_int32), ("refinement", c_int32)]
+
+
 DGTD_E_CAPACITY = -6
+DGTD_OPERATOR_MAXWELL, DGTD_OPERATOR_GLOBAL, DGTD_OPERATOR_HESTHAVEN = 0, 1, 2
 DGTD_INST_FREE, DGTD_INST_RUNNING, DGTD_INST_DONE = 0, 1, 2
 
 # name -> (restype, argtypes); every symbol declared in include/dgtd_hip.h.
@@ -73,6 +81,8 @@ SIGNATURES = {
     "dgtd_op_time": (c_int32, [c_void_p, _dp]),
     "dgtd_op_step": (c_int32, [c_void_p, _dp, _dp, c_double, c_int32]),
     "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_probe_set": (c_int32, [c_void_p, c_int32, _dp, _i32p]),
+    "dgtd_op_probe_sample": (c_int32, [c_void_p, _dp]),
     "dgtd_op_set_timing": (c_int32, [c_void_p, c_int32]),
     "dgtd_op_last_kernel_us": (c_int32, [c_void_p, _dp, _i32p]),
     "dgtd_op_kernel_variant": (c_int32, [c_void_p, _i32p]),
@@ -80,6 +90,12 @@ SIGNATURES = {
     "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
     "dgtd_project_initial_fields": (c_int32, [ctypes.POINTER(MeshDesc), ctypes.POINTER(InitialField),
                                               c_int32, _dp]),
+    "dgtd_case_load": (c_int32, [ctypes.c_char_p, ctypes.POINTER(c_void_p)]),
+    "dgtd_case_destroy": (None, [c_void_p]),
+    "dgtd_case_get_info": (c_int32, [c_void_p, ctypes.POINTER(CaseInfo)]),
+    "dgtd_case_mesh": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc)]),
+    "dgtd_case_sources": (c_int32, [c_void_p, ctypes.POINTER(InitialField)]),
+    "dgtd_case_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),
     "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),
@@ -93,6 +109,7 @@ SIGNATURES = {
     "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),
     "dgtd_mesh_vmaps": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _i32p, _i32p, _i8p]),
     "dgtd_mesh_geometry": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _dp]),
+    "dgtd_mesh_locate": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _dp, _i32p, _dp]),
 }
 
 _lib = None

@@ -1,0 +1,106 @@
+"""Case files of the reference's JSON launcher (driver.cpp:531-706, 940-1035,
+1242-1300) through the library's C reader: mesh file (Gmsh 2.2 / MFEM v1.0),
+uniform refinement, boundary tags, materials, initial-field sources, point
+probes and solver options.  `run_case` mirrors Solver::run for the case
+(src/solver/Solver.cpp:79-149, 483-551) on the hesthaven operator.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, dptr, i32ptr, lib
+from .hesthaven import HesthavenEvolution, num_nodes
+
+OPERATORS = {_abi.DGTD_OPERATOR_MAXWELL: "maxwell", _abi.DGTD_OPERATOR_GLOBAL: "global",
+             _abi.DGTD_OPERATOR_HESTHAVEN: "hesthaven"}
+
+
+class Case:
+    """A loaded case: `info` (dict), mesh arrays `vx`, `EToV`, `face_bc`, `eps`,
+    `mu`, `sources` (InitialField list) and `probes` (n x 3)."""
+
+    def __init__(self, path):
+        self.path = os.path.abspath(path)
+        h = ctypes.c_void_p()
+        check(lib().dgtd_case_load(self.path.encode(), ctypes.byref(h)))
+        self._h = h
+        try:
+            ci = _abi.CaseInfo()
+            check(lib().dgtd_case_get_info(h, ctypes.byref(ci)))
+            self.info = {k: getattr(ci, k) for k, _ in _abi.CaseInfo._fields_}
+            self.info["evolution_operator"] = OPERATORS[ci.evolution_operator]
+            d = _abi.MeshDesc()
+            check(lib().dgtd_case_mesh(h, ctypes.byref(d)))
+            K, nv = d.K, d.nv
+            self.vx = np.ctypeslib.as_array(d.vx, shape=(3 * nv,)).reshape(nv, 3).copy()
+            self.EToV = np.ctypeslib.as_array(d.EToV, shape=(4 * K,)).reshape(K, 4).copy()
+            self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(4 * K,)).reshape(K, 4).copy()
+            self.eps = None if not d.eps_r else np.ctypeslib.as_array(d.eps_r, shape=(K,)).copy()
+            self.mu = None if not d.mu_r else np.ctypeslib.as_array(d.mu_r, shape=(K,)).copy()
+            n = ci.nsources
+            arr = (_abi.InitialField * max(n, 1))()
+            check(lib().dgtd_case_sources(h, arr))
+            self.sources = [arr[i] for i in range(n)]
+            self.probes = np.zeros((ci.nprobes, 3))
+            self.probe_steps = np.zeros(ci.nprobes, dtype=np.int32)
+            self.probe_saves = np.zeros(ci.nprobes, dtype=np.int32)
+            check(lib().dgtd_case_probes(h, dptr(self.probes), i32ptr(self.probe_steps),
+                                         i32ptr(self.probe_saves)))
+        finally:
+            lib().dgtd_case_destroy(h)
+            self._h = None
+
+    @property
+    def order(self):
+        return self.info["order"]
+
+    @property
+    def K(self):
+        return self.EToV.shape[0]
+
+    @property
+    def ndofs(self):
+        return 6 * num_nodes(self.order) * self.K
+
+    def initial_state(self):
+        from .hesthaven import _Mesh
+
+        m = _Mesh(self.vx, self.EToV, self.face_bc, self.order)
+        x = np.zeros(self.ndofs)
+        srcs = (_abi.InitialField * max(len(self.sources), 1))(*self.sources)
+        check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), srcs, len(self.sources), dptr(x)))
+        return x
+
+    def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
+        return HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
+                                  eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
+                                  kernel=kernel, device=device)
+
+    def time_step(self, op):
+        """time_step, or the Hesthaven rule cfl / (max fscale * p^2) when it is 0
+        (Solver.cpp:110-115, 335-345)."""
+        if self.info["time_step"] > 0:
+            return self.info["time_step"]
+        return self.info["cfl"] / (op.max_fscale() * self.order ** 2)
+
+
+def run_case(path, precision=64, integrator="rk4", final_time=None, kernel="auto"):
+    """Load a case, project its initial fields and run it to final time on the
+    GPU.  Returns (fields, info) with info holding steps, dt and time."""
+    c = Case(path)
+    op = c.operator(precision=precision, integrator=integrator, kernel=kernel)
+    try:
+        dt = c.time_step(op)
+        op.set_state(c.initial_state(), 0.0)
+        T = c.info["final_time"] if final_time is None else final_time
+        steps = op.run(dt, T)
+        x = op.get_state()
+        return x, {"steps": steps, "dt": dt, "time": op.GetTime(), "K": c.K, "order": c.order,
+                   "ndofs": c.ndofs}
+    finally:
+        op.close()
+
+
+__all__ = ["Case", "run_case", "OPERATORS"]

@@ -1,0 +1,293 @@
+// Case files: the solver_options / model / sources / probes subset of the
+// reference's JSON launcher (driver.cpp:531-706, 940-1035, 1242-1300) that the
+// hesthaven path can run, read into plain arrays for dgtd_op_create.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dgtd_hip.h"
+#include "json.hpp"
+#include "meshio.hpp"
+
+namespace dgtd {
+int api_fail(int code, const std::string& msg);  // dgtd_api.cpp
+}
+
+struct dgtd_case {
+    dgtd::TetMesh mesh;
+    std::vector<int32_t> EToV;
+    std::vector<int8_t> face_bc;
+    std::vector<double> eps, mu;
+    dgtd_case_info info{};
+    std::vector<dgtd_initial_field> sources;
+    std::vector<double> probes;  // 3 per point probe
+    std::vector<int32_t> probe_steps, probe_saves;
+};
+
+namespace {
+
+struct Unsupported : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+std::string dir_of(const std::string& p)
+{
+    const size_t s = p.find_last_of('/');
+    return s == std::string::npos ? std::string(".") : p.substr(0, s);
+}
+
+// assemble3DVector (driver.cpp:346-353): up to three entries, the rest zero.
+void vec3(const dgtd::Json& j, double out[3])
+{
+    const std::vector<double> v = j.numbers();
+    if (v.size() > 3) throw std::runtime_error("vector with more than 3 entries");
+    for (int d = 0; d < 3; ++d) out[d] = d < (int)v.size() ? v[d] : 0.0;
+}
+
+int field_type(const dgtd::Json& j)
+{
+    const std::string& s = j.string();
+    if (s == "electric" || s == "E") return 0;
+    if (s == "magnetic" || s == "H") return 1;
+    throw std::runtime_error("unknown field_type '" + s + "'");
+}
+
+void read_solver_options(const dgtd::Json& c, dgtd_case_info& in)
+{
+    // SolverOptions defaults (SolverOptions.h:20-23, EvolutionOptions.h:13-17)
+    in.order = 2;
+    in.alpha = 1.0;
+    in.time_step = 0.0;
+    in.final_time = 2.0;
+    in.cfl = 1.0;
+    in.evolution_operator = DGTD_OPERATOR_GLOBAL;
+    if (!c.contains("solver_options")) return;
+    const dgtd::Json& s = c.at("solver_options");
+    if (s.contains("upwind_alpha")) in.alpha = s.at("upwind_alpha").number();
+    if (s.contains("time_step")) in.time_step = s.at("time_step").number();
+    if (s.contains("final_time")) in.final_time = s.at("final_time").number();
+    if (s.contains("cfl")) in.cfl = s.at("cfl").number();
+    if (s.contains("order")) in.order = s.at("order").integer();
+    if (s.contains("evolution_operator")) {
+        const std::string& op = s.at("evolution_operator").string();
+        if (op == "hesthaven") in.evolution_operator = DGTD_OPERATOR_HESTHAVEN;
+        else if (op == "global") in.evolution_operator = DGTD_OPERATOR_GLOBAL;
+        else if (op == "maxwell") in.evolution_operator = DGTD_OPERATOR_MAXWELL;
+        else
+            throw std::runtime_error("Wrong type of Evolution Operator defined, please choose: 'maxwell', 'global' or "
+                                     "'hesthaven'.");
+    }
+    if (s.contains("basis_type") && s.at("basis_type").integer() != 1)
+        throw Unsupported("only GaussLobatto nodes (basis_type 1) are supported");
+    if (s.contains("ode_type") && s.at("ode_type").integer() != 0)
+        throw Unsupported("only the explicit RK4 integrator (ode_type 0) is supported");
+    if (in.order < 1 || in.order > 5) throw Unsupported("order must be in 1..5");
+    if (!(in.final_time >= 0) || !(in.time_step >= 0) || !(in.cfl > 0))
+        throw std::runtime_error("final_time, time_step must be >= 0 and cfl > 0");
+}
+
+void read_sources(const dgtd::Json& c, dgtd_case& cs)
+{
+    if (!c.contains("sources")) return;
+    const dgtd::Json& ss = c.at("sources");
+    for (size_t i = 0; i < ss.size(); ++i) {
+        const dgtd::Json& s = ss.at(i);
+        const std::string& type = s.at("type").string();
+        if (type != "initial") throw Unsupported("source type '" + type + "' (TF/SF planewave, dipole) is not supported");
+        const dgtd::Json& mag = s.at("magnitude");
+        const std::string& mt = mag.at("type").string();
+        dgtd_initial_field f{};
+        f.field = field_type(s.at("field_type"));
+        vec3(s.at("polarization"), f.polarization);
+        if (mt == "gaussian") {  // buildGaussianInitialField, driver.cpp:368-380
+            f.kind = DGTD_FIELD_GAUSSIAN;
+            f.spread = mag.at("spread").number();
+            f.dimension = s.at("dimension").integer();
+            vec3(s.at("center"), f.center);
+        } else if (mt == "resonant") {  // buildResonantModeInitialField, driver.cpp:382-391
+            f.kind = DGTD_FIELD_RESONANT;
+            const std::vector<double> modes = mag.at("modes").numbers();
+            if (modes.empty() || modes.size() > 3) throw std::runtime_error("resonant modes must have 1..3 entries");
+            f.nmodes = (int32_t)modes.size();
+            for (size_t k = 0; k < modes.size(); ++k) f.modes[k] = (int32_t)modes[k];
+        } else {
+            throw Unsupported("initial field magnitude '" + mt + "' is not supported");
+        }
+        cs.sources.push_back(f);
+    }
+}
+
+void read_probes(const dgtd::Json& c, dgtd_case& cs)
+{
+    cs.info.exporter_steps = 0;
+    if (!c.contains("probes")) return;
+    const dgtd::Json& p = c.at("probes");
+    if (p.contains("exporter") && p.at("exporter").contains("steps"))
+        cs.info.exporter_steps = p.at("exporter").at("steps").integer();
+    if (p.contains("point")) {
+        const dgtd::Json& pts = p.at("point");
+        for (size_t i = 0; i < pts.size(); ++i) {
+            const dgtd::Json& pj = pts.at(i);
+            double x[3];
+            vec3(pj.at("position"), x);
+            cs.probes.insert(cs.probes.end(), x, x + 3);
+            cs.probe_steps.push_back(pj.contains("steps") ? pj.at("steps").integer() : 0);
+            cs.probe_saves.push_back(pj.contains("saves") ? pj.at("saves").integer() : 0);
+        }
+    }
+}
+
+void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs)
+{
+    if (!c.contains("model")) throw std::runtime_error("JSON data does not include 'model'.");
+    const dgtd::Json& m = c.at("model");
+    std::string file = m.at("filename").string();
+    if (file.empty() || file[0] != '/') file = dir_of(case_path) + "/" + file;
+    cs.mesh = dgtd::read_mesh_file(file);
+    dgtd::orient_positive(cs.mesh);
+    cs.info.refinement = m.contains("refinement") ? m.at("refinement").integer() : 0;
+    if (cs.info.refinement < 0 || cs.info.refinement > 6) throw std::runtime_error("refinement must be in 0..6");
+    for (int r = 0; r < cs.info.refinement; ++r) cs.mesh = dgtd::refine_uniform(cs.mesh);
+
+    // materials by element attribute (driver.cpp:946-966); the hesthaven
+    // operator applies eps_r, mu_r per element as 1/eps, 1/mu on the rhs
+    std::map<int, std::pair<double, double>> mat;
+    if (!m.contains("materials")) throw std::runtime_error("JSON data does not include 'materials'.");
+    const dgtd::Json& mats = m.at("materials");
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const dgtd::Json& mj = mats.at(i);
+        double eps = 1.0, mu = 1.0;
+        if (mj.contains("relative_permittivity")) eps = mj.at("relative_permittivity").number();
+        if (mj.contains("relative_permeability")) mu = mj.at("relative_permeability").number();
+        if (mj.contains("bulk_conductivity") && mj.at("bulk_conductivity").number() != 0.0)
+            throw Unsupported("conductive materials are not supported");
+        if (!(eps > 0) || !(mu > 0)) throw std::runtime_error("relative permittivity/permeability must be > 0");
+        const dgtd::Json& tags = mj.at("tags");
+        for (size_t t = 0; t < tags.size(); ++t) mat[tags.at(t).integer()] = {eps, mu};
+    }
+    const int32_t K = cs.mesh.K();
+    cs.eps.assign(K, 1.0);
+    cs.mu.assign(K, 1.0);
+    bool any = false;
+    for (int32_t e = 0; e < K; ++e) {
+        auto it = mat.find(cs.mesh.tet_attr[e]);
+        if (it == mat.end()) continue;
+        cs.eps[e] = it->second.first;
+        cs.mu[e] = it->second.second;
+        any = any || it->second.first != 1.0 || it->second.second != 1.0;
+    }
+    if (!any) {
+        cs.eps.clear();
+        cs.mu.clear();
+    }
+
+    // boundaries by triangle attribute (driver.cpp:870-887, 994-1035)
+    if (!m.contains("boundaries")) throw std::runtime_error("JSON data does not include 'boundaries'.");
+    int max_attr = 0;
+    for (int a : cs.mesh.tri_attr) max_attr = std::max(max_attr, a);
+    std::vector<int> attr_bc(max_attr + 1, -1);
+    const dgtd::Json& bs = m.at("boundaries");
+    for (size_t i = 0; i < bs.size(); ++i) {
+        const dgtd::Json& b = bs.at(i);
+        const std::string& type = b.at("type").string();
+        int code;
+        if (type == "PEC") code = DGTD_BC_PEC;
+        else if (type == "PMC") code = DGTD_BC_PMC;
+        else if (type == "SMA") code = DGTD_BC_SMA;
+        else if (type == "SGBC") throw Unsupported("SGBC boundaries are not supported");
+        else throw std::runtime_error("The defined Boundary Type " + type + " is incorrect.");
+        const dgtd::Json& tags = b.at("tags");
+        if (tags.size() == 0) throw std::runtime_error("boundary without tags");
+        for (size_t t = 0; t < tags.size(); ++t) {
+            const int a = tags.at(t).integer();
+            if (a < 0 || a > max_attr)
+                throw std::runtime_error("There is no bdr_attribute " + std::to_string(a) +
+                                         " defined in the mesh, but it is defined in the JSON.");
+            attr_bc[a] = code;
+        }
+    }
+    cs.face_bc = dgtd::face_codes(cs.mesh, attr_bc);
+    cs.EToV.resize(4 * (size_t)K);
+    for (int32_t e = 0; e < K; ++e)
+        for (int j = 0; j < 4; ++j) cs.EToV[4 * (size_t)e + j] = cs.mesh.tets[e][j];
+    cs.info.K = K;
+    cs.info.nv = cs.mesh.nv();
+}
+
+}  // namespace
+
+extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
+{
+    if (!json_path || !out) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    *out = nullptr;
+    try {
+        auto cs = std::make_unique<dgtd_case>();
+        const dgtd::Json c = dgtd::Json::parse_file(json_path);
+        if (!c.is_object()) throw std::runtime_error("case file is not a JSON object");
+        read_solver_options(c, cs->info);
+        read_model(c, json_path, *cs);
+        read_sources(c, *cs);
+        read_probes(c, *cs);
+        cs->info.nsources = (int32_t)cs->sources.size();
+        cs->info.nprobes = (int32_t)(cs->probes.size() / 3);
+        if (cs->info.alpha == 0.0)
+            for (int8_t b : cs->face_bc)
+                if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598
+                    throw Unsupported("Centered SMA with Hesthaven Evolution Operator not supported yet.");
+        *out = cs.release();
+    } catch (const Unsupported& e) {
+        return dgtd::api_fail(DGTD_E_UNSUPPORTED, e.what());
+    } catch (const dgtd::MeshUnsupported& e) {
+        return dgtd::api_fail(DGTD_E_UNSUPPORTED, e.what());
+    } catch (const std::exception& e) {
+        return dgtd::api_fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+extern "C" void dgtd_case_destroy(dgtd_case* c) { delete c; }
+
+extern "C" int dgtd_case_get_info(const dgtd_case* c, dgtd_case_info* info)
+{
+    if (!c || !info) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    *info = c->info;
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
+{
+    if (!c || !mesh) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    std::memset(mesh, 0, sizeof(*mesh));
+    mesh->K = c->info.K;
+    mesh->order = c->info.order;
+    mesh->vx = c->mesh.vx.data();
+    mesh->nv = c->info.nv;
+    mesh->EToV = c->EToV.data();
+    mesh->face_bc = c->face_bc.data();
+    mesh->eps_r = c->eps.empty() ? nullptr : c->eps.data();
+    mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out)
+{
+    if (!c || (!out && !c->sources.empty())) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    for (size_t i = 0; i < c->sources.size(); ++i) out[i] = c->sources[i];
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_probes(const dgtd_case* c, double* xyz, int32_t* steps, int32_t* saves)
+{
+    if (!c || (!xyz && !c->probes.empty())) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    for (size_t i = 0; i < c->probes.size(); ++i) xyz[i] = c->probes[i];
+    for (size_t i = 0; i < c->probe_steps.size(); ++i) {
+        if (steps) steps[i] = c->probe_steps[i];
+        if (saves) saves[i] = c->probe_saves[i];
+    }
+    return DGTD_OK;
+}

@@ -90,6 +90,11 @@ struct dgtd_op {
     double t = 0.0;
     int grid = 1, nblk = 1;
 
+    int nprobes = 0;
+    int* d_probe_elem = nullptr;
+    double* d_probe_w = nullptr;
+    double* d_probe_out = nullptr;
+
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_us = 0.0;
@@ -102,7 +107,8 @@ struct dgtd_op {
     {
         (void)hipSetDevice(device);
         for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
-                        (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, d_buf[0],
+                        (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_probe_elem,
+                        (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -625,6 +631,66 @@ int dgtd_op_norms(dgtd_op* op, double* out3)
     return DGTD_OK;
 }
 
+int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem)
+{
+    if (!op || n < 0 || (n > 0 && !xyz)) return fail(DGTD_E_ARG, "NULL argument or negative count");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        std::vector<int32_t> el(n, -1);
+        std::vector<double> w((size_t)n * op->Np, 0.0);
+        dgtd::MeshInput in;
+        in.K = op->K;
+        in.nv = op->nv;
+        in.vx = op->vx.data();
+        in.EToV = op->EToV.data();
+        dgtd::locate_points(in, op->R, n, xyz, el.data(), w.data());
+        for (void* q : {(void*)op->d_probe_elem, (void*)op->d_probe_w, (void*)op->d_probe_out})
+            if (q) hip_check(hipFree(q), "hipFree");
+        op->d_probe_elem = nullptr;
+        op->d_probe_w = nullptr;
+        op->d_probe_out = nullptr;
+        op->nprobes = n;
+        if (n > 0) {
+            op->d_probe_elem = static_cast<int*>(upload(el));
+            op->d_probe_w = static_cast<double*>(upload(w));
+            hip_check(hipMalloc(&op->d_probe_out, 6 * sizeof(double) * (size_t)n), "hipMalloc");
+        }
+        if (elem)
+            for (int p = 0; p < n; ++p) elem[p] = el[p];
+    } catch (const std::bad_alloc&) {
+        return fail(DGTD_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_probe_sample(dgtd_op* op, double* out)
+{
+    if (!op || (op->nprobes > 0 && !out)) return fail(DGTD_E_ARG, "NULL argument");
+    if (op->nprobes == 0) return DGTD_OK;
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hipError_t e;
+        if (op->precision == 64)
+            e = dgtd::launch_probe_sample<double>(static_cast<const double*>(op->state()), op->N, op->Np,
+                                                  op->nprobes, op->d_probe_elem, op->d_probe_w, op->d_probe_out,
+                                                  op->stream);
+        else
+            e = dgtd::launch_probe_sample<float>(static_cast<const float*>(op->state()), op->N, op->Np,
+                                                 op->nprobes, op->d_probe_elem, op->d_probe_w, op->d_probe_out,
+                                                 op->stream);
+        hip_check(e, "probe kernel launch");
+        hip_check(hipMemcpyAsync(out, op->d_probe_out, 6 * sizeof(double) * (size_t)op->nprobes,
+                                 hipMemcpyDeviceToHost, op->stream),
+                  "D2H");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
 int dgtd_op_set_timing(dgtd_op* op, int32_t on)
 {
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
@@ -701,6 +767,19 @@ int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, in
                     vmapP[n] = nb >= 0 ? nb * Np + R.face_perm[(size_t)code * Nfp + b] : vmapM[n];
                 }
             }
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, int32_t* elem, double* weights)
+{
+    if (!mesh || n < 0 || (n > 0 && (!xyz || !elem || !weights))) return fail(DGTD_E_ARG, "NULL argument");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    try {
+        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+        dgtd::locate_points(mesh_input(mesh), R, n, xyz, elem, weights);
     } catch (const std::exception& e) {
         return fail(DGTD_E_ARG, e.what());
     }

@@ -111,4 +111,10 @@ hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const dou
                         const double* eps, const double* mu, double* partials, int nparts,
                         double* out3, hipStream_t s);
 
+// Point-probe values: out[6*p + c] = sum_n w[p*Np + n] q[c*N + elem[p]*Np + n]
+// (NaN where elem[p] < 0).  elem[p] < K and w hold Np weights per probe.
+template <typename T>
+hipError_t launch_probe_sample(const T* q, size_t N, int Np, int nprobes, const int* elem, const double* w,
+                               double* out, hipStream_t s);
+
 }  // namespace dgtd

@@ -1,0 +1,192 @@
+// dgtd_launch: runs a case file of the reference's JSON launcher
+// (src/launcher/launcher.cpp, src/driver/driver.cpp) on the GPU hesthaven
+// operator.  Mirrors Solver::run (src/solver/Solver.cpp:483-551) with point
+// probes written as ProbesManager writes them (ProbesManager.cpp:183-218,
+// 375-406): Exports/<case>/PointProbes/PointProbe<ID>.dat, one line per
+// sample with time / c (seconds) and Ex Ey Ez Hx Hy Hz.
+//
+//   dgtd_launch CASE.json [--precision 64|32] [--integrator rk4|lserk4]
+//               [--kernel auto|valu|mfma] [--final-time T] [--out DIR]
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <string>
+#include <vector>
+
+#include "dgtd_hip.h"
+
+namespace {
+
+constexpr double kSpeedOfLightSI = 299792458.0;  // physicalConstants::speedOfLight_SI
+
+void die(const char* what)
+{
+    std::fprintf(stderr, "dgtd_launch: %s: %s\n", what, dgtd_last_error());
+    std::exit(1);
+}
+
+void check(int rc, const char* what)
+{
+    if (rc != DGTD_OK) die(what);
+}
+
+std::string stem(const std::string& path)
+{
+    std::string s = std::filesystem::path(path).filename().string();
+    const size_t dot = s.rfind('.');
+    return dot == std::string::npos ? s : s.substr(0, dot);
+}
+
+void usage()
+{
+    std::fprintf(stderr,
+                 "usage: dgtd_launch CASE.json [--precision 64|32] [--integrator rk4|lserk4]\n"
+                 "                   [--kernel auto|valu|mfma] [--final-time T] [--out DIR]\n");
+    std::exit(2);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) usage();
+    const std::string case_path = argv[1];
+    int precision = 64, integrator = DGTD_RK4, kernel = DGTD_KERNEL_AUTO;
+    double final_time = -1.0;
+    std::string out_dir;
+    for (int i = 2; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto val = [&]() -> std::string {
+            if (i + 1 >= argc) usage();
+            return argv[++i];
+        };
+        if (a == "--precision") precision = std::atoi(val().c_str());
+        else if (a == "--integrator") {
+            const std::string v = val();
+            integrator = v == "lserk4" ? DGTD_LSERK4 : v == "rk4" ? DGTD_RK4 : -1;
+            if (integrator < 0) usage();
+        } else if (a == "--kernel") {
+            const std::string v = val();
+            kernel = v == "auto" ? DGTD_KERNEL_AUTO : v == "valu" ? DGTD_KERNEL_VALU : v == "mfma" ? DGTD_KERNEL_MFMA : -1;
+            if (kernel < 0) usage();
+        } else if (a == "--final-time") final_time = std::atof(val().c_str());
+        else if (a == "--out") out_dir = val();
+        else usage();
+    }
+
+    const auto t_setup = std::chrono::steady_clock::now();
+    dgtd_case* cs = nullptr;
+    check(dgtd_case_load(case_path.c_str(), &cs), "case");
+    dgtd_case_info info{};
+    check(dgtd_case_get_info(cs, &info), "case info");
+    if (final_time < 0) final_time = info.final_time;
+    dgtd_mesh_desc mesh{};
+    check(dgtd_case_mesh(cs, &mesh), "case mesh");
+
+    dgtd_opts o{};
+    o.alpha = info.alpha;
+    o.precision = precision;
+    o.integrator = integrator;
+    o.kernel = kernel;
+    dgtd_op* op = nullptr;
+    check(dgtd_op_create(&mesh, &o, &op), "operator");
+
+    int64_t ndofs = 0;
+    check(dgtd_op_ndofs(op, &ndofs), "ndofs");
+    std::vector<dgtd_initial_field> src(info.nsources > 0 ? info.nsources : 1);
+    check(dgtd_case_sources(cs, src.data()), "sources");
+    std::vector<double> x((size_t)ndofs, 0.0);
+    check(dgtd_project_initial_fields(&mesh, src.data(), info.nsources, x.data()), "initial fields");
+    check(dgtd_op_set_state(op, x.data(), 0.0), "set state");
+
+    // dt: time_step, or the Hesthaven rule (Solver.cpp:110-115, 335-345)
+    double dt = info.time_step;
+    if (dt == 0.0) {
+        double fs = 0.0;
+        check(dgtd_op_max_fscale(op, &fs), "max fscale");
+        dt = info.cfl / (fs * info.order * info.order);
+    }
+
+    // point probes and their sampling intervals (driver.cpp:717-733)
+    const int np = info.nprobes;
+    std::vector<double> pxyz(3 * (size_t)np);
+    std::vector<int32_t> psteps(np), psaves(np), pelem(np), interval(np, 1);
+    check(dgtd_case_probes(cs, pxyz.data(), psteps.data(), psaves.data()), "probes");
+    if (np > 0) check(dgtd_op_probe_set(op, np, pxyz.data(), pelem.data()), "probe set");
+    const int total_steps = (int)std::ceil(final_time / dt);
+    for (int p = 0; p < np; ++p)
+        interval[p] = psteps[p] > 0 ? psteps[p] : psaves[p] > 0 ? std::max(1, total_steps / psaves[p]) : 1;
+
+    if (out_dir.empty()) out_dir = "Exports/" + stem(case_path);
+    std::vector<std::ofstream> files(np);
+    if (np > 0) {
+        const std::string base = out_dir + "/PointProbes/";
+        std::filesystem::create_directories(base);
+        for (int p = 0; p < np; ++p) {
+            if (pelem[p] < 0) continue;  // outside the mesh: no file, as elementId -2
+            files[p].open(base + "PointProbe" + std::to_string(p) + ".dat", std::ios::trunc);
+            files[p] << "PointProbe ID " << p << "\n";
+            files[p] << "Spatial Position (X, Y, Z) \n";
+            files[p] << std::scientific << std::setprecision(5);
+            files[p] << std::to_string(pxyz[3 * p]) + " " + std::to_string(pxyz[3 * p + 1]) + " " +
+                            std::to_string(pxyz[3 * p + 2])
+                     << "\n";
+            files[p] << "Time (s) // Ex // Ey // Ez // Hx // Hy // Hz \n";
+        }
+    }
+    std::vector<double> vals(6 * (size_t)np);
+    auto sample = [&](long cycle, double t, bool final) {
+        bool any = false;
+        for (int p = 0; p < np; ++p) any = any || (pelem[p] >= 0 && (final || cycle % interval[p] == 0));
+        if (!any) return;
+        check(dgtd_op_probe_sample(op, vals.data()), "probe sample");
+        for (int p = 0; p < np; ++p) {
+            if (pelem[p] < 0 || !(final || cycle % interval[p] == 0)) continue;
+            files[p] << t / kSpeedOfLightSI;
+            for (int c = 0; c < 6; ++c) files[p] << " " << vals[6 * p + c];
+            files[p] << "\n";
+        }
+    };
+    const double setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count();
+
+    const auto t_run = std::chrono::steady_clock::now();
+    double t = 0.0;
+    long steps = 0;
+    if (np == 0) {
+        int32_t n = 0;
+        check(dgtd_op_run(op, dt, final_time, &n), "run");
+        steps = n;
+        check(dgtd_op_time(op, &t), "time");
+    } else {
+        long cycle = 0;
+        sample(cycle++, t, std::fabs(t - final_time) < 1e-8);
+        while (t <= final_time - 1e-8 * dt) {  // Solver::run loop (Solver.cpp:497)
+            const double truedt = std::min(dt, final_time - t);  // Solver::step (:537)
+            check(dgtd_op_advance(op, truedt, 1), "advance");
+            check(dgtd_op_time(op, &t), "time");
+            ++steps;
+            sample(cycle++, t, std::fabs(t - final_time) < 1e-8);
+        }
+    }
+    double norms[3] = {0, 0, 0};
+    check(dgtd_op_norms(op, norms), "norms");
+    const double run_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_run).count();
+    const int stages = integrator == DGTD_LSERK4 ? 5 : 4;
+    std::printf("{\"case\": \"%s\", \"K\": %d, \"order\": %d, \"ndofs\": %lld, \"precision\": %d, "
+                "\"integrator\": \"%s\", \"dt\": %.10e, \"steps\": %ld, \"time\": %.10e, \"norm2\": %.10e, "
+                "\"energy\": %.10e, \"nonfinite\": %d, \"probes\": %d, \"setup_s\": %.3f, \"run_s\": %.3f, "
+                "\"dof_stages_per_s\": %.4e, \"out\": \"%s\"}\n",
+                stem(case_path).c_str(), info.K, info.order, (long long)ndofs, precision,
+                integrator == DGTD_LSERK4 ? "lserk4" : "rk4", dt, steps, t, norms[0], norms[1], (int)norms[2], np,
+                setup_s, run_s, run_s > 0 ? (double)ndofs * stages * steps / run_s : 0.0, out_dir.c_str());
+    for (auto& f : files)
+        if (f.is_open()) f.close();
+    dgtd_op_destroy(op);
+    dgtd_case_destroy(cs);
+    return norms[2] != 0.0 ? 3 : 0;
+}

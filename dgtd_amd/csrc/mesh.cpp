@@ -173,6 +173,42 @@ void node_positions(const MeshInput& in, const RefTet& R, double* xyz)
     }
 }
 
+void locate_points(const MeshInput& in, const RefTet& R, int n, const double* xyz, int32_t* elem, double* w)
+{
+    auto det3 = [](const double A[3][3]) {
+        return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+               A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    };
+    const double tol = 1e-10;
+    for (int p = 0; p < n; ++p) {
+        const double* x = xyz + 3 * (size_t)p;
+        elem[p] = -1;
+        std::fill(w + (size_t)p * R.Np, w + (size_t)(p + 1) * R.Np, 0.0);
+        for (int e = 0; e < in.K; ++e) {
+            const int32_t* v = &in.EToV[4 * (size_t)e];
+            const double* x0 = &in.vx[3 * (size_t)v[0]];
+            double A[3][3], b[3];
+            for (int d = 0; d < 3; ++d) {
+                for (int k = 0; k < 3; ++k) A[d][k] = in.vx[3 * (size_t)v[k + 1] + d] - x0[d];
+                b[d] = x[d] - x0[d];
+            }
+            const double det = det3(A);
+            double xi[3];
+            for (int k = 0; k < 3; ++k) {  // Cramer's rule for x = x0 + A xi
+                double M[3][3];
+                for (int d = 0; d < 3; ++d)
+                    for (int j = 0; j < 3; ++j) M[d][j] = j == k ? b[d] : A[d][j];
+                xi[k] = det3(M) / det;
+            }
+            if (xi[0] >= -tol && xi[1] >= -tol && xi[2] >= -tol && xi[0] + xi[1] + xi[2] <= 1 + tol) {
+                elem[p] = e;
+                interp_weights(R, xi, w + (size_t)p * R.Np);
+                break;
+            }
+        }
+    }
+}
+
 void cube_mesh(int n, double L, const int8_t side_bc[6], double* vx, int32_t* EToV, int8_t* face_bc)
 {
     const int n1 = n + 1;

@@ -45,6 +45,10 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha);
 // Physical node positions (3 blocks of Np*K).
 void node_positions(const MeshInput& in, const RefTet& R, double* xyz);
 
+// Point location for probes: elem[p] = lowest-index element whose closure
+// holds point p (-1 if none), w[p*Np..) = nodal interpolation weights there.
+void locate_points(const MeshInput& in, const RefTet& R, int n, const double* xyz, int32_t* elem, double* w);
+
 // n^3 hexahedra on [0,L]^3 split into six tetrahedra each (see dgtd_hip.h).
 void cube_mesh(int n, double L, const int8_t side_bc[6], double* vx, int32_t* EToV,
                int8_t* face_bc);

@@ -1,0 +1,315 @@
+// Mesh file readers and uniform refinement (see meshio.hpp).
+#include "meshio.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace dgtd {
+
+namespace {
+
+[[noreturn]] void bad(const std::string& path, const std::string& what)
+{
+    throw std::runtime_error(path + ": " + what);
+}
+
+double signed_volume6(const double* vx, const std::array<int32_t, 4>& t)
+{
+    const double* a = vx + 3 * t[0];
+    double u[3], v[3], w[3];
+    for (int d = 0; d < 3; ++d) {
+        u[d] = vx[3 * t[1] + d] - a[d];
+        v[d] = vx[3 * t[2] + d] - a[d];
+        w[d] = vx[3 * t[3] + d] - a[d];
+    }
+    return u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+}
+
+bool ends_with(const std::string& s, const std::string& suf)
+{
+    return s.size() >= suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0;
+}
+
+}  // namespace
+
+TetMesh read_gmsh22(const std::string& path)
+{
+    std::ifstream f(path);
+    if (!f) bad(path, "cannot open");
+    TetMesh m;
+    std::unordered_map<long long, int32_t> node_index;
+    std::string line;
+    bool have_format = false, have_nodes = false, have_elements = false;
+    while (std::getline(f, line)) {
+        if (line.rfind("$MeshFormat", 0) == 0) {
+            std::getline(f, line);
+            std::istringstream is(line);
+            double ver = 0;
+            int ftype = -1;
+            is >> ver >> ftype;
+            if (ver < 2.0 || ver >= 3.0) bad(path, "only Gmsh format 2.x is supported");
+            if (ftype != 0) bad(path, "binary Gmsh files are not supported");
+            have_format = true;
+        } else if (line.rfind("$Nodes", 0) == 0) {
+            long long n = 0;
+            f >> n;
+            if (n <= 0) bad(path, "bad node count");
+            m.vx.resize(3 * (size_t)n);
+            for (long long i = 0; i < n; ++i) {
+                long long id;
+                double x, y, z;
+                if (!(f >> id >> x >> y >> z)) bad(path, "truncated $Nodes");
+                node_index[id] = (int32_t)i;
+                m.vx[3 * i] = x;
+                m.vx[3 * i + 1] = y;
+                m.vx[3 * i + 2] = z;
+            }
+            have_nodes = true;
+        } else if (line.rfind("$Elements", 0) == 0) {
+            if (!have_nodes) bad(path, "$Elements before $Nodes");
+            long long n = 0;
+            f >> n;
+            std::getline(f, line);
+            for (long long i = 0; i < n; ++i) {
+                if (!std::getline(f, line)) bad(path, "truncated $Elements");
+                std::istringstream is(line);
+                long long id;
+                int type, ntags;
+                if (!(is >> id >> type >> ntags) || ntags < 0) bad(path, "bad element line");
+                std::vector<long long> tags(ntags);
+                for (auto& t : tags) is >> t;
+                const int attr = ntags >= 2 ? (int)tags[1] : (ntags == 1 ? (int)tags[0] : 1);
+                int nn = 0;
+                if (type == 4) nn = 4;
+                else if (type == 2) nn = 3;
+                else if (type == 15 || type == 1) continue;  // points and lines dropped
+                else throw MeshUnsupported(path + ": element type " + std::to_string(type) + " is not supported (tetrahedra only)");
+                int32_t vs[4];
+                for (int k = 0; k < nn; ++k) {
+                    long long nid;
+                    if (!(is >> nid)) bad(path, "bad element line");
+                    auto it = node_index.find(nid);
+                    if (it == node_index.end()) bad(path, "element references unknown node");
+                    vs[k] = it->second;
+                }
+                if (nn == 4) {
+                    m.tets.push_back({vs[0], vs[1], vs[2], vs[3]});
+                    m.tet_attr.push_back(attr);
+                } else {
+                    m.tris.push_back({vs[0], vs[1], vs[2]});
+                    m.tri_attr.push_back(attr);
+                }
+            }
+            have_elements = true;
+        }
+    }
+    if (!have_format || !have_nodes || !have_elements) bad(path, "missing $MeshFormat, $Nodes or $Elements");
+    if (m.tets.empty()) bad(path, "no tetrahedra");
+    return m;
+}
+
+TetMesh read_mfem_mesh(const std::string& path)
+{
+    std::ifstream f(path);
+    if (!f) bad(path, "cannot open");
+    // tokens, with '#' comments removed
+    std::vector<std::string> tok;
+    std::string line;
+    bool first = true;
+    while (std::getline(f, line)) {
+        if (first) {
+            if (line.rfind("MFEM mesh v1.", 0) != 0) bad(path, "not an MFEM v1.x mesh");
+            first = false;
+            continue;
+        }
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream is(line);
+        std::string t;
+        while (is >> t) tok.push_back(t);
+    }
+    size_t p = 0;
+    auto next = [&]() -> const std::string& {
+        if (p >= tok.size()) bad(path, "unexpected end of file");
+        return tok[p++];
+    };
+    auto next_int = [&]() { return std::stol(next()); };
+    TetMesh m;
+    bool have_vertices = false;
+    while (p < tok.size()) {
+        const std::string sec = next();
+        if (sec == "dimension") {
+            if (next_int() != 3) bad(path, "only 3D meshes are supported");
+        } else if (sec == "elements") {
+            const long n = next_int();
+            for (long i = 0; i < n; ++i) {
+                const int attr = (int)next_int();
+                const int geom = (int)next_int();
+                if (geom != 4)
+                    throw MeshUnsupported(path + ": element geometry " + std::to_string(geom) +
+                                          " is not supported (tetrahedra only)");
+                std::array<int32_t, 4> t;
+                for (auto& v : t) v = (int32_t)next_int();
+                m.tets.push_back(t);
+                m.tet_attr.push_back(attr);
+            }
+        } else if (sec == "boundary") {
+            const long n = next_int();
+            for (long i = 0; i < n; ++i) {
+                const int attr = (int)next_int();
+                const int geom = (int)next_int();
+                if (geom != 2) bad(path, "boundary geometry must be triangles");
+                std::array<int32_t, 3> t;
+                for (auto& v : t) v = (int32_t)next_int();
+                m.tris.push_back(t);
+                m.tri_attr.push_back(attr);
+            }
+        } else if (sec == "vertices") {
+            const long n = next_int();
+            if (p < tok.size() && tok[p] == "nodes")
+                throw MeshUnsupported(path + ": curved meshes (nodes) are not supported");
+            const long vdim = next_int();
+            if (vdim != 3) bad(path, "vertex dimension must be 3");
+            m.vx.resize(3 * (size_t)n);
+            for (long i = 0; i < 3 * n; ++i) m.vx[i] = std::stod(next());
+            have_vertices = true;
+        } else if (sec == "nodes") {
+            throw MeshUnsupported(path + ": curved meshes (nodes) are not supported");
+        } else if (sec == "mfem_mesh_end") {
+            break;
+        } else {
+            bad(path, "unknown section '" + sec + "'");
+        }
+    }
+    if (!have_vertices || m.tets.empty()) bad(path, "missing vertices or elements");
+    const int32_t nv = m.nv();
+    for (const auto& t : m.tets)
+        for (int v : t)
+            if (v < 0 || v >= nv) bad(path, "element vertex out of range");
+    for (const auto& t : m.tris)
+        for (int v : t)
+            if (v < 0 || v >= nv) bad(path, "boundary vertex out of range");
+    return m;
+}
+
+TetMesh read_mesh_file(const std::string& path)
+{
+    if (ends_with(path, ".msh")) return read_gmsh22(path);
+    if (ends_with(path, ".mesh")) return read_mfem_mesh(path);
+    throw std::runtime_error("File format for mesh must be name.msh or name.mesh");
+}
+
+void orient_positive(TetMesh& m)
+{
+    for (auto& t : m.tets) {
+        const double v6 = signed_volume6(m.vx.data(), t);
+        if (v6 == 0.0) throw std::runtime_error("degenerate tetrahedron in mesh");
+        if (v6 < 0) std::swap(t[2], t[3]);
+    }
+}
+
+TetMesh refine_uniform(const TetMesh& m)
+{
+    TetMesh r;
+    r.vx = m.vx;
+    const int64_t nv0 = m.nv();
+    std::unordered_map<int64_t, int32_t> mid;
+    auto midpoint = [&](int32_t a, int32_t b) {
+        const int64_t key = (int64_t)std::min(a, b) * nv0 + std::max(a, b);
+        auto it = mid.find(key);
+        if (it != mid.end()) return it->second;
+        const int32_t id = (int32_t)(r.vx.size() / 3);
+        for (int d = 0; d < 3; ++d) r.vx.push_back(0.5 * (m.vx[3 * a + d] + m.vx[3 * b + d]));
+        mid.emplace(key, id);
+        return id;
+    };
+    auto dist2 = [&](int32_t a, int32_t b) {
+        double s = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double t = r.vx[3 * a + d] - r.vx[3 * b + d];
+            s += t * t;
+        }
+        return s;
+    };
+    r.tets.reserve(8 * m.tets.size());
+    for (size_t e = 0; e < m.tets.size(); ++e) {
+        const auto& t = m.tets[e];
+        const int32_t v0 = t[0], v1 = t[1], v2 = t[2], v3 = t[3];
+        const int32_t m01 = midpoint(v0, v1), m02 = midpoint(v0, v2), m03 = midpoint(v0, v3);
+        const int32_t m12 = midpoint(v1, v2), m13 = midpoint(v1, v3), m23 = midpoint(v2, v3);
+        std::array<int32_t, 4> kids[8] = {
+            {v0, m01, m02, m03}, {m01, v1, m12, m13}, {m02, m12, v2, m23}, {m03, m13, m23, v3}};
+        // inner octahedron: diagonal (a, b) and the cycle of the other four
+        const double d0 = dist2(m01, m23), d1 = dist2(m02, m13), d2 = dist2(m03, m12);
+        int32_t a, b, c[4];
+        if (d0 <= d1 && d0 <= d2) {
+            a = m01; b = m23; c[0] = m02; c[1] = m03; c[2] = m13; c[3] = m12;
+        } else if (d1 <= d2) {
+            a = m02; b = m13; c[0] = m01; c[1] = m03; c[2] = m23; c[3] = m12;
+        } else {
+            a = m03; b = m12; c[0] = m01; c[1] = m02; c[2] = m23; c[3] = m13;
+        }
+        for (int i = 0; i < 4; ++i) kids[4 + i] = {a, b, c[i], c[(i + 1) % 4]};
+        for (auto& k : kids) {
+            if (signed_volume6(r.vx.data(), k) < 0) std::swap(k[2], k[3]);
+            r.tets.push_back(k);
+            r.tet_attr.push_back(m.tet_attr[e]);
+        }
+    }
+    for (size_t i = 0; i < m.tris.size(); ++i) {
+        const auto& t = m.tris[i];
+        const int32_t a = t[0], b = t[1], c = t[2];
+        const int32_t mab = midpoint(a, b), mbc = midpoint(b, c), mca = midpoint(c, a);
+        const std::array<int32_t, 3> kids[4] = {{a, mab, mca}, {mab, b, mbc}, {mca, mbc, c}, {mab, mbc, mca}};
+        for (const auto& k : kids) {
+            r.tris.push_back(k);
+            r.tri_attr.push_back(m.tri_attr[i]);
+        }
+    }
+    return r;
+}
+
+std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
+{
+    static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+    const int64_t nv = m.nv();
+    auto key_of = [&](int32_t a, int32_t b, int32_t c) {
+        int32_t s[3] = {a, b, c};
+        std::sort(s, s + 3);
+        return ((int64_t)s[0] * nv + s[1]) * nv + s[2];  // nv^3 < 2^63 for nv < 2e6
+    };
+    if (nv >= 2000000) throw std::runtime_error("mesh too large for the face table (2e6 vertices)");
+    std::unordered_map<int64_t, std::pair<int64_t, int64_t>> faces;  // key -> (slot, slot or -1)
+    faces.reserve(4 * m.tets.size());
+    for (size_t e = 0; e < m.tets.size(); ++e)
+        for (int f = 0; f < 4; ++f) {
+            const auto& t = m.tets[e];
+            const int64_t k = key_of(t[kFace[f][0]], t[kFace[f][1]], t[kFace[f][2]]);
+            auto it = faces.find(k);
+            if (it == faces.end())
+                faces.emplace(k, std::make_pair((int64_t)(4 * e + f), (int64_t)-1));
+            else
+                it->second.second = (int64_t)(4 * e + f);
+        }
+    std::vector<int8_t> bc(4 * m.tets.size(), 0);
+    for (size_t i = 0; i < m.tris.size(); ++i) {
+        const int attr = m.tri_attr[i];
+        const int code = (attr >= 0 && attr < (int)attr_to_bc.size()) ? attr_to_bc[attr] : -1;
+        if (code < 0) continue;
+        const auto& t = m.tris[i];
+        auto it = faces.find(key_of(t[0], t[1], t[2]));
+        if (it == faces.end())
+            throw std::runtime_error("boundary triangle with attribute " + std::to_string(attr) + " is no element face");
+        if (it->second.second >= 0)
+            throw MeshUnsupported("boundary attribute " + std::to_string(attr) +
+                                  " lies on interior faces: interior boundary conditions are not supported");
+        bc[it->second.first] = (int8_t)code;
+    }
+    return bc;
+}
+
+}  // namespace dgtd

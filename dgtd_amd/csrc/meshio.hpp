@@ -1,0 +1,59 @@
+// Tetrahedral mesh files the reference's launcher reads (driver.cpp:1083-1188):
+// Gmsh 2.2 ASCII (.msh) and MFEM v1.0 (.mesh), plus uniform refinement.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dgtd {
+
+// Input the readers understand but the operator does not support (curved
+// meshes, non-tetrahedral elements, interior boundary conditions).
+struct MeshUnsupported : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+struct TetMesh {
+    std::vector<double> vx;                 // 3 per vertex
+    std::vector<std::array<int32_t, 4>> tets;
+    std::vector<int32_t> tet_attr;
+    std::vector<std::array<int32_t, 3>> tris;  // boundary (and interior-boundary) triangles
+    std::vector<int32_t> tri_attr;
+
+    int32_t nv() const { return (int32_t)(vx.size() / 3); }
+    int32_t K() const { return (int32_t)tets.size(); }
+};
+
+// Gmsh 2.2 ASCII.  Tetrahedra (type 4) are elements, triangles (type 2)
+// boundary elements; points and lines are dropped, as the reference's
+// fixGmshMesh does for 3D meshes (driver.cpp:1076-1080).  The attribute is
+// the elementary-entity tag (the reference swaps it into the physical-tag
+// slot before MFEM reads the file, driver.cpp:1140-1142).
+TetMesh read_gmsh22(const std::string& path);
+
+// MFEM mesh v1.0 with tetrahedra (geometry 4) and triangle boundary elements
+// (geometry 2), straight-sided ("vertices" section; curved "nodes" refused).
+TetMesh read_mfem_mesh(const std::string& path);
+
+// Dispatch on the extension (.msh / .mesh), as driver.cpp:1276-1282.
+TetMesh read_mesh_file(const std::string& path);
+
+// Make every tetrahedron positively oriented (swap local vertices 2 and 3).
+void orient_positive(TetMesh& m);
+
+// One level of uniform (1:8) refinement: edge midpoints, four corner
+// children and the inner octahedron split along its shortest diagonal;
+// boundary triangles split into four with the parent's attribute.  Children
+// are positively oriented when the parent is.
+TetMesh refine_uniform(const TetMesh& m);
+
+// face_bc[4e+f] from the triangles' attributes via attr_to_bc (index =
+// attribute, value = DGTD_BC_* or -1 for "no condition"); throws if a
+// triangle with a condition lies on an interior face (interior boundary
+// conditions are not supported) or matches no element face.
+std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc);
+
+}  // namespace dgtd

@@ -100,4 +100,35 @@ template hipError_t launch_norms<float>(const float*, int, int, const double*, c
                                         const double*, const double*, double*, int, double*,
                                         hipStream_t);
 
+// Point probes: one wave per (probe, component) forms
+// sum_n w[probe][n] * q[c*N + elem[probe]*Np + n] in FP64; probes outside
+// the mesh (elem < 0) give NaN.
+template <typename T>
+__global__ __launch_bounds__(64) void probe_sample(const T* __restrict__ q, size_t N, int Np,
+                                                   const int* __restrict__ elem,
+                                                   const double* __restrict__ w, double* __restrict__ out)
+{
+    const int p = blockIdx.x / 6, c = blockIdx.x % 6;
+    const int e = elem[p];
+    double s = 0.0;
+    if (e >= 0)
+        for (int n = threadIdx.x; n < Np; n += 64) s += w[(size_t)p * Np + n] * (double)q[c * N + (size_t)e * Np + n];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) out[blockIdx.x] = e >= 0 ? s : __builtin_nan("");
+}
+
+template <typename T>
+hipError_t launch_probe_sample(const T* q, size_t N, int Np, int nprobes, const int* elem, const double* w,
+                               double* out, hipStream_t s)
+{
+    if (nprobes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(probe_sample<T>, dim3(6 * nprobes), dim3(64), 0, s, q, N, Np, elem, w, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_probe_sample<double>(const double*, size_t, int, int, const int*, const double*,
+                                                double*, hipStream_t);
+template hipError_t launch_probe_sample<float>(const float*, size_t, int, int, const int*, const double*,
+                                               double*, hipStream_t);
+
 }  // namespace dgtd

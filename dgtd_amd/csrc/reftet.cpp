@@ -191,6 +191,25 @@ const int kPerms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1},
 
 }  // namespace
 
+void interp_weights(const RefTet& R, const double xi[3], double* w)
+{
+    const int p = R.p, Np = R.Np;
+    const double r = 2 * xi[0] - 1, s = 2 * xi[1] - 1, t = 2 * xi[2] - 1;
+    std::vector<double> P(Np);
+    int m = 0;
+    for (int i = 0; i <= p; ++i)
+        for (int j = 0; i + j <= p; ++j)
+            for (int k = 0; i + j + k <= p; ++k, ++m) {
+                double vr, vs, vt;
+                pkd3(r, s, t, i, j, k, P[m], vr, vs, vt);
+            }
+    for (int n = 0; n < Np; ++n) {
+        double acc = 0.0;
+        for (int mm = 0; mm < Np; ++mm) acc += R.Vinv[(size_t)mm * Np + n] * P[mm];
+        w[n] = acc;
+    }
+}
+
 int perm_code(const int pi[3])
 {
     for (int c = 0; c < 6; ++c)
@@ -234,6 +253,7 @@ RefTet build_reftet(int p)
                     pkd3(r, s, t, i, j, k, V(n, m), Vr(n, m), Vs(n, m), Vt(n, m));
     }
     const Dense Vi = inv(V);
+    R.Vinv = Vi.a;
     const Dense Minv = mul_bt(V, V);
     R.M = inv(Minv).a;
     R.Dr = mul(Vr, Vi).a;

@@ -23,6 +23,7 @@ struct RefTet {
     // Barycentric lattice index of each node w.r.t. vertices 0..3 (sum = p).
     std::vector<int> lattice;      // 4*Np
     std::vector<double> xi;        // 3*Np, unit-tetrahedron coordinates
+    std::vector<double> Vinv;      // Np*Np inverse Vandermonde of the orthonormal basis
     std::vector<double> M;         // Np*Np mass on the reference tet of volume 4/3
     std::vector<double> Dr, Ds, Dt;// Np*Np strong derivatives in r = 2 xi - 1, ...
     std::vector<double> LIFT;      // Np*4Nfp
@@ -36,6 +37,10 @@ struct RefTet {
 // Builds the reference element of order p (1..kMaxOrder).  Throws
 // std::runtime_error on failure.
 RefTet build_reftet(int p);
+
+// Weights w[0..Np) of the nodal interpolant at unit-tetrahedron point xi:
+// u(xi) = sum_n w[n] u_n (orthonormal basis at xi times the inverse Vandermonde).
+void interp_weights(const RefTet& R, const double xi[3], double* w);
 
 // Index 0..5 of the permutation pi with own face vertex k equal to neighbour
 // face vertex pi[k]; -1 if `pi` is not a permutation of {0,1,2}.

@@ -110,6 +110,16 @@ def mesh_vmaps(vx, EToV, face_bc, order, alpha=1.0):
     return M, P, codes.reshape(K, 4)
 
 
+def mesh_locate(vx, EToV, order, points):
+    """Host point location: (elem (n,), weights (n, Np)) as the probes use them."""
+    m = _Mesh(vx, EToV, None, order)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    elem = np.zeros(pts.shape[0], dtype=np.int32)
+    w = np.zeros((pts.shape[0], num_nodes(order)))
+    check(lib().dgtd_mesh_locate(ctypes.byref(m.desc), pts.shape[0], dptr(pts), i32ptr(elem), dptr(w)))
+    return elem, w
+
+
 def mesh_geometry(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
     m = _Mesh(vx, EToV, face_bc, order, eps, mu)
     geo = np.zeros(27 * m.EToV.shape[0])
@@ -229,6 +239,21 @@ class HesthavenEvolution:
         check(lib().dgtd_op_time(self._h, ctypes.byref(t)))
         self._t = t.value
         return n.value
+
+    # --- point probes (ProbesManager::updateProbe, ProbesManager.cpp:375-406) ---
+    def probe_set(self, points):
+        """Locate points (n x 3); returns the element of each (-1 outside)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        elem = np.zeros(pts.shape[0], dtype=np.int32)
+        check(lib().dgtd_op_probe_set(self._h, pts.shape[0], dptr(pts), i32ptr(elem)))
+        self._nprobes = pts.shape[0]
+        return elem
+
+    def probe_sample(self):
+        """(n x 6) Ex, Ey, Ez, Hx, Hy, Hz of the resident state at the probes."""
+        out = np.zeros((getattr(self, "_nprobes", 0), 6))
+        check(lib().dgtd_op_probe_sample(self._h, dptr(out)))
+        return out
 
     def norms(self):
         out = np.zeros(3)

@@ -140,6 +140,18 @@ int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps);
  * out[2] = 1 if any entry is non-finite, else 0. */
 int dgtd_op_norms(dgtd_op* op, double* out3);
 
+/* Point probes (PointProbe, src/components/Probes.h:132-159, evaluated by
+ * ProbesManager::updateProbe, src/solver/ProbesManager.cpp:375-406): the
+ * six field components of the nodal solution at n points.  probe_set
+ * locates each point (the lowest-index element whose closure holds it; the
+ * DG field may jump across element faces) and stores its interpolation
+ * weights on the device; elem[p] (may be NULL) receives the element or -1
+ * for a point outside the mesh.  A new call replaces the previous set. */
+int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem);
+/* out[6p + c] = component c (Ex,Ey,Ez,Hx,Hy,Hz) of the resident state at
+ * probe p, FP64 (NaN for a probe outside the mesh).  Synchronous. */
+int dgtd_op_probe_sample(dgtd_op* op, double* out);
+
 /* Kernel timing of the last dgtd_op_advance/dgtd_op_run call: average
  * operator-kernel duration in microseconds over its launches, measured with
  * HIP events on the handle's stream (0 when timing is off). */
@@ -182,6 +194,51 @@ typedef struct {
 /* x (6*Np*K doubles, reference layout) = sum of the sources at the nodes. */
 int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_field* sources,
                                 int32_t nsources, double* x);
+
+/* ------------------------------------------------------------------------
+ * Case files: the reference's JSON launcher input (driver.cpp:531-706,
+ * 940-1035, 1242-1300) as far as the hesthaven path runs it.
+ *   solver_options: upwind_alpha, time_step (0 = Hesthaven dt rule), final_time,
+ *                   cfl, order, evolution_operator, basis_type (GaussLobatto),
+ *                   ode_type (RK4)
+ *   model: filename (Gmsh 2.2 ASCII .msh or MFEM v1.0 .mesh, relative to the
+ *          case file; driver.cpp:1083-1188), refinement (uniform, 1:8),
+ *          materials (relative_permittivity / _permeability per element tag),
+ *          boundaries (PEC / PMC / SMA per boundary tag)
+ *   sources: "initial" gaussian / resonant fields
+ *   probes: point positions and exporter steps
+ * TF/SF sources, SGBC, conductivity, interior boundary conditions and
+ * non-RK4 integrators are refused with DGTD_E_UNSUPPORTED.
+ * ------------------------------------------------------------------------ */
+typedef struct dgtd_case dgtd_case;
+
+#define DGTD_OPERATOR_MAXWELL 0   /* EvolutionOperatorType (EvolutionOptions.h:7-11) */
+#define DGTD_OPERATOR_GLOBAL 1
+#define DGTD_OPERATOR_HESTHAVEN 2
+
+typedef struct {
+    int32_t K, nv, order;
+    double alpha, time_step, final_time, cfl;
+    int32_t evolution_operator;   /* as named in the case; all run the hesthaven operator */
+    int32_t nsources, nprobes;
+    int32_t exporter_steps;       /* probes.exporter.steps, 0 if absent */
+    int32_t refinement;           /* uniform refinement levels applied */
+} dgtd_case_info;
+
+/* Parse the case file, read and refine its mesh, map tags to boundary codes
+ * and materials (buildModel, driver.cpp:1242-1300). */
+int dgtd_case_load(const char* json_path, dgtd_case** out);
+void dgtd_case_destroy(dgtd_case* c);
+int dgtd_case_get_info(const dgtd_case* c, dgtd_case_info* info);
+/* Mesh description for dgtd_op_create; its arrays belong to the case. */
+int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh);
+/* info.nsources initial fields (buildSources, driver.cpp:531-564). */
+int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out);
+/* Point probes (buildProbes, driver.cpp:707-864): 3 * info.nprobes
+ * coordinates, and per probe its "steps" and "saves" entries (0 if absent;
+ * either array may be NULL).  The sampling interval is steps, else
+ * max(1, ceil(final_time/dt)/saves), else 1 (driver.cpp:717-733). */
+int dgtd_case_probes(const dgtd_case* c, double* xyz, int32_t* steps, int32_t* saves);
 
 /* ------------------------------------------------------------------------
  * Batched instances: many independent meshes of uneven size and order
@@ -261,6 +318,10 @@ int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, in
 /* Per-element geometric factors, 27 doubles per element: dr/dx..dt/dz (9),
  * per face nx ny nz fscale/2 (16), 1/eps_r, 1/mu_r. */
 int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo);
+
+/* Point location as dgtd_op_probe_set does it: elem[p] (or -1) and Np nodal
+ * interpolation weights per point (u(x_p) = sum_n weights[p*Np + n] u_n). */
+int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, int32_t* elem, double* weights);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,244 @@
+"""Case files of the reference's JSON launcher on the CPU: the library's JSON,
+Gmsh 2.2 and MFEM mesh readers against independent Python readers of the
+reference's own test-data files (tests/golden/cases: 3D_PEC and
+3D_Resonant_Box_TM55_H1_P2 from testData/maxwellInputs), uniform refinement
+against closed-form invariants, initial fields against their formulas, point
+location against polynomial reproduction, and the documented refusals."""
+import json
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import dgtd_amd as D
+import meshgen as MG
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CASES = os.path.join(ROOT, "tests", "golden", "cases")
+PEC = os.path.join(CASES, "3D_PEC.json")
+TM55 = os.path.join(CASES, "3D_Resonant_Box_TM55_H1_P2.json")
+
+
+def volumes(vx, E):
+    v = vx[E]
+    return np.einsum("ij,ij->i", np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), v[:, 3] - v[:, 0]) / 6.0
+
+
+def face_counts(E):
+    keys = {}
+    for f in range(4):
+        for k in map(tuple, np.sort(E[:, list(MG.FACE_VERTS[f])], axis=1)):
+            keys[k] = keys.get(k, 0) + 1
+    return keys
+
+
+def test_pec_case_options_sources_probes():
+    c = D.Case(PEC)
+    ref = json.load(open(PEC))
+    so = ref["solver_options"]
+    assert c.info["order"] == so["order"]
+    assert c.info["alpha"] == so["upwind_alpha"]
+    assert c.info["time_step"] == so["time_step"]
+    assert c.info["final_time"] == so["final_time"]
+    assert c.info["evolution_operator"] == so["evolution_operator"]
+    assert c.info["exporter_steps"] == ref["probes"]["exporter"]["steps"]
+    np.testing.assert_array_equal(c.probes, [p["position"] for p in ref["probes"]["point"]])
+    np.testing.assert_array_equal(c.probe_steps, 0)
+    (s,) = c.sources
+    src = ref["sources"][0]
+    assert (s.kind, s.field, s.dimension) == (0, 0, src["dimension"])
+    assert s.spread == src["magnitude"]["spread"]
+    assert list(s.center) == src["center"] and list(s.polarization) == src["polarization"]
+
+
+def test_gmsh_reader_matches_independent_reader():
+    c = D.Case(PEC)
+    vx, E, tris = MG.read_gmsh22(os.path.join(CASES, "3D_PEC.msh"))
+    E = MG.fix_orientation(vx, E)
+    np.testing.assert_array_equal(c.vx, vx)
+    np.testing.assert_array_equal(c.EToV, E)
+    assert (volumes(c.vx, c.EToV) > 0).all()
+    codes = {1: 1, 2: 1, 5: 1, 6: 1, 3: 2, 4: 2}  # PEC tags 1,2,5,6; PMC 3,4
+    np.testing.assert_array_equal(c.face_bc, MG.face_bc_from_triangles(E, tris, codes))
+    assert np.bincount(c.face_bc.ravel()).tolist() == [4 * c.K - 704, 384, 320]
+    assert c.eps is None and c.mu is None  # vacuum everywhere
+
+
+def test_initial_fields_follow_their_formulas():
+    c = D.Case(PEC)
+    pos = O.Operator(c.vx, c.EToV, c.face_bc, c.order).positions()
+    x = c.initial_state().reshape(6, -1)
+    # 1D gaussian along x about 0.5, spread 0.07, on Ez (Function.h:45-100)
+    np.testing.assert_allclose(x[2], np.exp(-(pos[0].ravel() - 0.5) ** 2 / (2 * 0.07 ** 2)), atol=1e-14)
+    assert np.abs(np.delete(x, 2, axis=0)).max() == 0
+    t = D.Case(TM55)
+    pos = O.Operator(t.vx, t.EToV, t.face_bc, t.order).positions()
+    x = t.initial_state().reshape(6, -1)
+    # resonant modes [5, 5] on Ez: sin(5 pi x) sin(5 pi y) (Function.h:145-168)
+    np.testing.assert_allclose(x[2], (np.sin(5 * np.pi * pos[0]) * np.sin(5 * np.pi * pos[1])).ravel(),
+                               atol=1e-14)
+
+
+def test_uniform_refinement_invariants():
+    t = D.Case(TM55)  # "refinement": 1
+    vx0, E0, tris0 = MG.read_gmsh22(os.path.join(CASES, "3D_Resonant_Box_TM55_H1_P2.msh"))
+    E0 = MG.fix_orientation(vx0, E0)
+    assert t.info["refinement"] == 1 and t.K == 8 * E0.shape[0]
+    v0, v = volumes(vx0, E0), volumes(t.vx, t.EToV)
+    assert (v > 0).all()
+    np.testing.assert_allclose(v.reshape(-1, 8), np.repeat(v0[:, None] / 8, 8, axis=1), rtol=1e-12)
+    np.testing.assert_allclose(v.sum(), 1.0, rtol=1e-12)
+    edges = {tuple(sorted((a, b))) for tet in E0 for i, a in enumerate(tet) for b in tet[i + 1:]}
+    assert t.vx.shape[0] == vx0.shape[0] + len(edges)
+    # conforming: every face is shared by two children or lies on the boundary
+    cnt = face_counts(t.EToV)
+    assert set(cnt.values()) <= {1, 2}
+    nb = sum(1 for k in cnt.values() if k == 1)
+    assert nb == 4 * len(tris0)
+    # every boundary face PEC, every interior face none
+    bnd = MG.boundary_faces(t.EToV)
+    np.testing.assert_array_equal(t.face_bc, np.where(bnd, 1, 0))
+
+
+def write_case(tmp, mesh_name, boundaries, refinement=0):
+    case = {"solver_options": {"order": 2, "final_time": 0.1},
+            "model": {"filename": mesh_name, "refinement": refinement,
+                      "materials": [{"tags": [1], "type": "vacuum"}], "boundaries": boundaries},
+            "sources": []}
+    p = os.path.join(tmp, "case.json")
+    json.dump(case, open(p, "w"))
+    return p
+
+
+def test_mfem_mesh_reader_and_refinement(tmp_path):
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "onetetra.mesh"), tmp_path / "onetetra.mesh")
+    bnd = [{"tags": [1], "type": "PEC"}, {"tags": [2], "type": "PMC"}, {"tags": [3], "type": "SMA"},
+           {"tags": [4], "type": "PEC"}]
+    c = D.Case(write_case(str(tmp_path), "onetetra.mesh", bnd))
+    np.testing.assert_array_equal(c.vx, [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    np.testing.assert_array_equal(c.EToV, [[0, 1, 2, 3]])
+    # boundary element b lies opposite vertex b-1 (faces {1,2,3},{0,3,2},{0,1,3},{0,2,1})
+    np.testing.assert_array_equal(c.face_bc, [[1, 2, 3, 1]])
+    r = D.Case(write_case(str(tmp_path), "onetetra.mesh", bnd, refinement=2))
+    assert r.K == 64
+    np.testing.assert_allclose(volumes(r.vx, r.EToV), 1.0 / 6 / 64, rtol=1e-12)
+    bf = MG.boundary_faces(r.EToV)
+    assert bf.sum() == 4 * 16 and (r.face_bc[bf] > 0).all() and (r.face_bc[~bf] == 0).all()
+    # children of a boundary face keep its attribute: 16 faces per parent face
+    assert np.bincount(r.face_bc[bf]).tolist() == [0, 32, 16, 16]
+
+
+def test_materials_by_tag(tmp_path):
+    shutil.copy(os.path.join(CASES, "3D_PEC.msh"), tmp_path / "m.msh")
+    p = write_case(str(tmp_path), "m.msh", [{"tags": [1, 2, 3, 4, 5, 6], "type": "SMA"}])
+    case = json.load(open(p))
+    case["model"]["materials"] = [{"tags": [1], "type": "dielectric", "relative_permittivity": 4.0,
+                                   "relative_permeability": 2.0}]
+    json.dump(case, open(p, "w"))
+    c = D.Case(p)
+    np.testing.assert_array_equal(c.eps, 4.0)
+    np.testing.assert_array_equal(c.mu, 2.0)
+    assert set(np.unique(c.face_bc)) == {0, 3}
+
+
+@pytest.mark.parametrize("mutate,code,msg", [
+    (lambda c: c["sources"].append({"type": "planewave"}), -4, "planewave"),
+    (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "XYZ"}), -1, "incorrect"),
+    (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [99], "type": "PEC"}), -1, "bdr_attribute"),
+    (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "SGBC"}), -4, "SGBC"),
+    (lambda c: c["model"]["materials"][0].__setitem__("bulk_conductivity", 1.0), -4, "conductive"),
+    (lambda c: c["solver_options"].__setitem__("ode_type", 1), -4, "RK4"),
+    (lambda c: c["solver_options"].__setitem__("evolution_operator", "foo"), -1, "Evolution Operator"),
+    (lambda c: c["model"].__setitem__("filename", "missing.msh"), -1, "cannot open"),
+    (lambda c: c["model"].__setitem__("filename", "m.vtk"), -1, "name.msh or name.mesh"),
+    (lambda c: (c["solver_options"].__setitem__("upwind_alpha", 0.0),
+                c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "SMA"})), -4, "Centered SMA"),
+])
+def test_case_refusals(tmp_path, mutate, code, msg):
+    shutil.copy(os.path.join(CASES, "3D_PEC.msh"), tmp_path / "m.msh")
+    p = write_case(str(tmp_path), "m.msh", [{"tags": [1, 2, 3, 4, 5, 6], "type": "PEC"}])
+    case = json.load(open(p))
+    mutate(case)
+    json.dump(case, open(p, "w"))
+    with pytest.raises(D.DgtdError, match=msg) as ei:
+        D.Case(p)
+    assert ei.value.code == code
+
+
+def test_interior_boundary_tags_are_refused(tmp_path):
+    # two tetrahedra sharing face {1,2,3}; a boundary element on the shared face
+    txt = """MFEM mesh v1.0
+
+dimension
+3
+
+elements
+2
+1 4 0 1 2 3
+1 4 4 1 3 2
+
+boundary
+1
+7 2 1 2 3
+
+vertices
+5
+3
+0 0 0
+1 0 0
+0 1 0
+0 0 1
+1 1 1
+"""
+    (tmp_path / "two.mesh").write_text(txt)
+    p = write_case(str(tmp_path), "two.mesh", [{"tags": [7], "type": "PEC"}])
+    with pytest.raises(D.DgtdError, match="interior boundary") as ei:
+        D.Case(p)
+    assert ei.value.code == -4
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_point_location_reproduces_polynomials(p):
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    pos = O.Operator(vx, E, fbc, p).positions().reshape(3, -1)
+    rng = np.random.default_rng(p)
+    pts = rng.uniform(0.02, 0.98, (40, 3))
+    elem, w = D.mesh_locate(vx, E, p, pts)
+    assert (elem >= 0).all()
+    # random polynomial of total degree p, evaluated at nodes and at the points
+    coef = rng.normal(size=(p + 1, p + 1, p + 1))
+
+    def f(x, y, z):
+        return sum(coef[a, b, c] * x ** a * y ** b * z ** c
+                   for a in range(p + 1) for b in range(p + 1 - a) for c in range(p + 1 - a - b))
+
+    u = f(*pos).reshape(E.shape[0], -1)
+    got = np.einsum("pn,pn->p", w, u[elem])
+    np.testing.assert_allclose(got, f(*pts.T), rtol=1e-11, atol=1e-11)
+    # the containing element really contains the point
+    v = vx[E[elem]]
+    A = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=2)
+    xi = np.linalg.solve(A, (pts - v[:, 0])[..., None])[..., 0]
+    assert (xi > -1e-10).all() and (xi.sum(1) < 1 + 1e-10).all()
+
+
+def test_point_location_nodes_outside_and_ties():
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    p = 4  # one interior node per element
+    Np = D.num_nodes(p)
+    pos = O.Operator(vx, E, fbc, p).positions().reshape(3, -1)
+    v = vx[E[5]]
+    A = np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]], axis=1)
+    lam = np.linalg.solve(A, pos[:, 5 * Np: 6 * Np] - v[0][:, None])
+    bary = np.vstack([1 - lam.sum(0), lam])
+    k = int(np.argmax(bary.min(0)))  # the interior node
+    assert bary[:, k].min() > 0.1
+    elem, w = D.mesh_locate(vx, E, p, [pos[:, 5 * Np + k], [1.5, 0.5, 0.5], [0.5, 0.5, 0.5]])
+    assert elem[0] == 5
+    np.testing.assert_allclose(w[0], np.eye(Np)[k], atol=1e-11)
+    assert elem[1] == -1 and np.abs(w[1]).max() == 0
+    # a shared vertex goes to the lowest-index element holding it
+    holds = [e for e in range(E.shape[0]) if (np.abs(vx[E[e]] - 0.5).sum(1) < 1e-14).any()]
+    assert elem[2] == min(holds)
