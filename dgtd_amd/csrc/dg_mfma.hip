@@ -27,6 +27,7 @@
 
 #include <cstdint>
 
+#include "bc_coeffs.hpp"
 #include "kernels.hpp"
 
 namespace dgtd {
@@ -151,8 +152,15 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
         const int e0 = task * TE;
         const int ne = min(TE, K - e0);
 
-        // ---- stage the task's fields in LDS: all loads in flight, then the writes ----
+        // ---- stage the task's fields in LDS: all loads in flight (fields and the
+        // lane's element record), then the LDS writes ----
         wave_sync();  // previous task's reads of U are done
+        const bool valid = el < ne;
+        const int e = e0 + (valid ? el : 0);
+        const T* ge = a.geo + (size_t)e * kGeoStrideDev;
+        T gm[9];
+        T scale;
+        int fnb[4], fcode[4];
         {
             constexpr int NL = (TE * NP + 63) / 64;
             const int cnt = ne * NP;
@@ -167,6 +175,15 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
                 }
             }
 #pragma unroll
+            for (int k = 0; k < 9; ++k) gm[k] = ge[k];
+            // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
+            scale = valid ? ge[25 + half] : T(0);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
+                fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
+            }
+#pragma unroll
             for (int c = 0; c < 6; ++c)
 #pragma unroll
                 for (int k = 0; k < NL; ++k) {
@@ -179,15 +196,7 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
         }
         wave_sync();
 
-        // ---- per-lane element data; half 0 forms E-output operands, half 1 H-output ----
-        const bool valid = el < ne;
-        const int e = e0 + (valid ? el : 0);
-        const T* ge = a.geo + (size_t)e * kGeoStrideDev;
-        T gm[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) gm[k] = ge[k];
-        // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
-        const T scale = valid ? (half ? ge[26] : ge[25]) : T(0);
+        // half 0 forms E-output operands, half 1 H-output
         const T sign = half ? T(-1) : T(1);
         const T sscale = sign * scale;
 
@@ -204,12 +213,6 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
         // ahead; boundary faces use the own trace with the boundary
         // coefficients (HesthavenEvolution.cpp:290-292).
         const T* Ue = U + el * US;
-        int fnb[4], fcode[4];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
-            fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
-        }
         auto remote_of = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
         auto node_of = [&](int f, int b) {
             const int bb = b < NFP ? b : NFP - 1;
@@ -245,10 +248,8 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
             }
             T ce = T(-1), ch = T(-1), cI = T(1);
             if (!interior) {
-                const int bc = -nb - 1;
                 cI = T(0);
-                ce = bc == 1 ? T(-2) : (bc == 3 ? T(-1) / a.alpha : T(0));
-                ch = bc == 2 ? T(-2) : (bc == 3 ? T(-1) / a.alpha : T(0));
+                bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
             }
             const int iM = sFm[f * NFP + (b < NFP ? b : NFP - 1)];
             // A = jump of the "other" field, B = jump of the output's own field

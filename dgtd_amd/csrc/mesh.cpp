@@ -123,11 +123,26 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
         while (j < keys.size() && same(keys[i], keys[j])) ++j;
         if (j - i > 2) throw std::runtime_error("non-manifold mesh: a face is shared by more than two elements");
         if (j - i == 2) {
+            // interior face: 0, or an interior condition 4-6 given on either side
+            int ic = 0;
+            if (in.face_bc) {
+                const int c0 = in.face_bc[keys[i].ef], c1 = in.face_bc[keys[i + 1].ef];
+                if (c0 < 0 || c0 > 6 || c1 < 0 || c1 > 6) throw std::runtime_error("face_bc code must be 0..6");
+                if (c0 != 0 && c1 != 0 && c0 != c1)
+                    throw std::runtime_error("the two sides of an interior face carry different codes");
+                ic = c0 != 0 ? c0 : c1;
+                if (ic >= 1 && ic <= 3)
+                    throw std::runtime_error("PEC/PMC/SMA codes 1-3 on an interior face (interior conditions are 4-6)");
+            }
             for (int side = 0; side < 2; ++side) {
                 const int ef = keys[i + side].ef, og = keys[i + 1 - side].ef;
                 const int e = ef / 4, f = ef % 4, e2 = og / 4, g = og % 4;
-                if (in.face_bc && in.face_bc[ef] != 0)
-                    throw std::runtime_error("boundary conditions on interior faces are not supported");
+                if (ic != 0) {  // the jump uses the own trace only: no neighbour gather
+                    m.conn[(size_t)kConnStride * e + f] = -(ic + 1);
+                    m.conn[(size_t)kConnStride * e + 4 + f] = 0;
+                    m.n_boundary_faces[ic]++;
+                    continue;
+                }
                 int pi[3];
                 for (int k = 0; k < 3; ++k) {
                     const int vid = in.EToV[4 * (size_t)e + kFaceVerts[f][k]];
@@ -143,7 +158,8 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
         } else {
             const int ef = keys[i].ef, e = ef / 4, f = ef % 4;
             const int bc = in.face_bc ? in.face_bc[ef] : 0;
-            if (bc < 0 || bc > 3) throw std::runtime_error("face_bc code must be 0..3");
+            if (bc < 0 || bc > 6) throw std::runtime_error("face_bc code must be 0..6");
+            if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-6 on a boundary face");
             if (bc == 3 && alpha == 0.0)
                 throw std::runtime_error("SMA boundaries require upwind flux (alpha != 0)");
             m.conn[(size_t)kConnStride * e + f] = -(bc + 1);

@@ -36,7 +36,7 @@ struct MeshData {
     std::vector<double> jac;     // K: V_e / V_ref, for mass-weighted reductions
     std::vector<double> eps, mu; // K
     double max_fscale = 0.0;
-    int n_boundary_faces[4] = {0, 0, 0, 0};
+    int n_boundary_faces[7] = {0, 0, 0, 0, 0, 0, 0};  // per face code (interior conditions count each side)
 };
 
 // Throws std::runtime_error (message is user-facing) on invalid input.

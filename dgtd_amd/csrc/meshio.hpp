@@ -11,7 +11,7 @@
 namespace dgtd {
 
 // Input the readers understand but the operator does not support (curved
-// meshes, non-tetrahedral elements, interior boundary conditions).
+// meshes, non-tetrahedral elements).
 struct MeshUnsupported : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
@@ -51,9 +51,10 @@ void orient_positive(TetMesh& m);
 TetMesh refine_uniform(const TetMesh& m);
 
 // face_bc[4e+f] from the triangles' attributes via attr_to_bc (index =
-// attribute, value = DGTD_BC_* or -1 for "no condition"); throws if a
-// triangle with a condition lies on an interior face (interior boundary
-// conditions are not supported) or matches no element face.
+// attribute, value DGTD_BC_PEC/PMC/SMA or -1 for "no condition"); a
+// triangle on an interior face gives the interior code (value + 3) to both
+// sides, as the reference marks such tags interior (driver.cpp:1024-1035).
+// Throws if a triangle matches no element face.
 std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc);
 
 }  // namespace dgtd

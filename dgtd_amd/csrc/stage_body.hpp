@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bc_coeffs.hpp"
 #include "kernels.hpp"
 
 namespace dgtd {
@@ -102,12 +103,8 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
 #pragma unroll
                 for (int c = 0; c < 6; ++c) d[c] = a.qin[c * N + gP] - uM[c];
             } else {
-                const int bc = -nb - 1;
-                // HesthavenEvolution.cpp:290-292: PEC (-2, 0), PMC (0, -2), SMA (-1/alpha, -1/alpha)
-                T ce = T(0), ch = T(0);
-                if (bc == 1) { ce = T(-2); ch = T(0); }
-                else if (bc == 2) { ce = T(0); ch = T(-2); }
-                else if (bc == 3) { ce = T(-1) / a.alpha; ch = ce; }
+                T ce, ch;
+                bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) { d[c] = ce * uM[c]; d[3 + c] = ch * uM[3 + c]; }
             }

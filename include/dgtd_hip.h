@@ -48,6 +48,12 @@ extern "C" {
 #define DGTD_BC_PEC 1
 #define DGTD_BC_PMC 2
 #define DGTD_BC_SMA 3
+/* Interior boundary conditions on interior faces (given on either side; the
+ * jump on both sides becomes (-1, 0), (0, -1), (-1/2, -1/2) times the own
+ * trace, HesthavenEvolution.cpp:294-311). */
+#define DGTD_BC_INT_PEC 4
+#define DGTD_BC_INT_PMC 5
+#define DGTD_BC_INT_SMA 6
 
 /* Integrators. */
 #define DGTD_RK4 0    /* mfem::RK4Solver stage order (the reference's default) */
@@ -64,7 +70,7 @@ typedef struct {
     const double* vx;       /* 3*nv vertex coordinates (x,y,z interleaved) */
     int32_t nv;             /* number of vertices */
     const int32_t* EToV;    /* 4*K vertex ids; local vertex order fixes node/face order */
-    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes, NULL = all DGTD_BC_NONE */
+    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (1-3 boundary, 4-6 interior faces), NULL = none */
     const double* eps_r;    /* K relative permittivity, NULL = 1 */
     const double* mu_r;     /* K relative permeability, NULL = 1 */
 } dgtd_mesh_desc;
@@ -204,11 +210,12 @@ int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_f
  *   model: filename (Gmsh 2.2 ASCII .msh or MFEM v1.0 .mesh, relative to the
  *          case file; driver.cpp:1083-1188), refinement (uniform, 1:8),
  *          materials (relative_permittivity / _permeability per element tag),
- *          boundaries (PEC / PMC / SMA per boundary tag)
+ *          boundaries (PEC / PMC / SMA per tag; a tag on interior faces
+ *          becomes the interior condition DGTD_BC_INT_*)
  *   sources: "initial" gaussian / resonant fields
  *   probes: point positions and exporter steps
- * TF/SF sources, SGBC, conductivity, interior boundary conditions and
- * non-RK4 integrators are refused with DGTD_E_UNSUPPORTED.
+ * TF/SF sources, SGBC, conductivity and non-RK4 integrators are refused
+ * with DGTD_E_UNSUPPORTED.
  * ------------------------------------------------------------------------ */
 typedef struct dgtd_case dgtd_case;
 
@@ -309,9 +316,10 @@ int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* 
 
 /* Expands the compact connectivity into the reference's (vmapM, vmapP) lists,
  * 4*Nfp*K each, in Connectivities::global order (HesthavenEvolutionMethods.cpp
- * :721-791): element, local face, face node ascending; P = M on boundary faces.
- * face_codes (4*K, may be NULL) receives the boundary code of each face
- * (0 for interior faces). */
+ * :721-791): element, local face, face node ascending; P = M on boundary faces
+ * and on faces with an interior condition (their jump uses the own trace only).
+ * face_codes (4*K, may be NULL) receives the code of each face (0 for plain
+ * interior faces, 4-6 on both sides of a face with an interior condition). */
 int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP,
                     int8_t* face_codes);
 

@@ -556,13 +556,25 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
                 int e2 = -1, f2 = -1;
                 for (auto& pr : lst)
                     if (pr.first != e || pr.second != f) { e2 = pr.first; f2 = pr.second; }
-                const int code = face_bc[4 * e + f];
-                if (code < 0 || code > 3) throw std::runtime_error("bad face_bc code");
-                if (e2 >= 0 && code != 0)
-                    throw std::runtime_error("boundary conditions on interior faces are not supported");
+                /* codes 1-3 on boundary faces; 4-6 (interior PEC/PMC/SMA) on
+                 * interior faces, given on either side (the condition belongs
+                 * to the face, :294-311 applies it to both elements) */
+                int code = face_bc[4 * e + f];
+                if (code < 0 || code > 6) throw std::runtime_error("bad face_bc code");
+                if (e2 >= 0) {
+                    const int other = face_bc[4 * e2 + f2];
+                    if (other < 0 || other > 6) throw std::runtime_error("bad face_bc code");
+                    if (code != 0 && other != 0 && code != other)
+                        throw std::runtime_error("the two sides of an interior face carry different codes");
+                    if (code == 0) code = other;
+                    if (code >= 1 && code <= 3)
+                        throw std::runtime_error("PEC/PMC/SMA codes 1-3 on an interior face (use 4-6)");
+                } else if (code >= 4) {
+                    throw std::runtime_error("interior-boundary codes 4-6 on a boundary face");
+                }
                 if (code == 3 && alpha == 0.0)
                     throw std::runtime_error("SMA boundaries need upwind flux (alpha != 0)");
-                op->bc[4 * (size_t)e + f] = (e2 >= 0) ? 0 : code;
+                op->bc[4 * (size_t)e + f] = code;
                 for (int b = 0; b < Nfp; ++b) {
                     const size_t n = (size_t)e * 4 * Nfp + (size_t)f * Nfp + b;
                     const int iM = op->R.Fmask[f * Nfp + b];
@@ -668,6 +680,11 @@ void oracle_mult(const oracle_op* op, const double* in, double* out)
                         for (int c = 3; c < 6; ++c) dF[c][f * Nfp + b] = -2.0 * in[c * N + M];
                     } else if (code == 3) { /* SMA: dE = -E/alpha, dH = -H/alpha */
                         for (int c = 0; c < 6; ++c) dF[c][f * Nfp + b] = (-1.0 / alpha) * in[c * N + M];
+                    } else if (code >= 4) { /* interior PEC (-1, 0), PMC (0, -1), SMA (-1/2, -1/2) (:309-311) */
+                        const double ce = code == 4 ? -1.0 : (code == 6 ? -0.5 : 0.0);
+                        const double ch = code == 5 ? -1.0 : (code == 6 ? -0.5 : 0.0);
+                        for (int c = 0; c < 3; ++c) dF[c][f * Nfp + b] = ce * in[c * N + M];
+                        for (int c = 3; c < 6; ++c) dF[c][f * Nfp + b] = ch * in[c * N + M];
                     }
                 }
             }

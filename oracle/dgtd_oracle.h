@@ -47,8 +47,9 @@ void oracle_gll01(int p, double* x);
 /* Operator over an affine tetrahedral mesh.
  *   vx      : 3*nv vertex coordinates
  *   EToV    : 4*K vertex ids; local vertex order fixes node and face order
- *   face_bc : 4*K, face f is opposite local vertex f.  0 interior, 1 PEC,
- *             2 PMC, 3 SMA.  A face with no neighbour and code 0 is an error.
+ *   face_bc : 4*K, face f is opposite local vertex f.  On boundary faces
+ *             0 (zero jump), 1 PEC, 2 PMC, 3 SMA; on interior faces 0 or
+ *             the interior conditions 4 PEC, 5 PMC, 6 SMA (either side).
  *   eps,mu  : K relative permittivity/permeability, NULL means 1
  * Returns NULL on error (message via oracle_last_error). */
 oracle_op* oracle_create(int K, int order, const double* vx, int nv,

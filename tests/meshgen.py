@@ -178,3 +178,27 @@ def sort_by_position(pos, q, decimals=9):
     order = np.lexsort((z, y, x))
     qq = q.reshape(6, -1)
     return qq[:, order]
+
+
+def plane_faces(vx, EToV, axis, value, tol=1e-12):
+    """(K, 4) mask of element faces lying in the plane x[axis] = value."""
+    mask = np.zeros(EToV.shape, dtype=bool)
+    for f in range(4):
+        pts = vx[EToV[:, list(FACE_VERTS[f])]]
+        mask[:, f] = (np.abs(pts[..., axis] - value) < tol).all(axis=1)
+    return mask
+
+
+def cut_along_plane(vx, EToV, axis, value, tol=1e-12):
+    """Copy of the mesh in which the elements on the far side of the plane
+    x[axis] = value use their own copies of the plane's vertices, so the plane
+    becomes two boundary surfaces.  Element order and local vertex order (hence
+    node order) are unchanged."""
+    on = np.where(np.abs(vx[:, axis] - value) < tol)[0]
+    copy = {int(v): vx.shape[0] + i for i, v in enumerate(on)}
+    vx2 = np.vstack([vx, vx[on]])
+    far = vx[EToV].mean(axis=1)[:, axis] > value
+    E2 = EToV.copy()
+    for e in np.where(far)[0]:
+        E2[e] = [copy.get(int(v), int(v)) for v in EToV[e]]
+    return vx2, E2

@@ -167,7 +167,8 @@ def test_case_refusals(tmp_path, mutate, code, msg):
     assert ei.value.code == code
 
 
-def test_interior_boundary_tags_are_refused(tmp_path):
+@pytest.mark.parametrize("kind,code", [("PEC", 4), ("PMC", 5), ("SMA", 6)])
+def test_interior_boundary_tags_become_interior_conditions(tmp_path, kind, code):
     # two tetrahedra sharing face {1,2,3}; a boundary element on the shared face
     txt = """MFEM mesh v1.0
 
@@ -193,10 +194,9 @@ vertices
 1 1 1
 """
     (tmp_path / "two.mesh").write_text(txt)
-    p = write_case(str(tmp_path), "two.mesh", [{"tags": [7], "type": "PEC"}])
-    with pytest.raises(D.DgtdError, match="interior boundary") as ei:
-        D.Case(p)
-    assert ei.value.code == -4
+    c = D.Case(write_case(str(tmp_path), "two.mesh", [{"tags": [7], "type": kind}]))
+    # the shared face is face 0 of element 0 and face 0 of element 1 (opposite vertex 4)
+    np.testing.assert_array_equal(c.face_bc, [[code, 0, 0, 0], [code, 0, 0, 0]])
 
 
 @pytest.mark.parametrize("p", [1, 3, 5])

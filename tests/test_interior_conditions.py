@@ -1,0 +1,97 @@
+"""Interior boundary conditions (HesthavenEvolution.cpp:294-311) in the
+oracle, pinned without retyping their formula: the flux is linear in the
+jumps, so on a face with an interior condition (ce, ch) the right-hand side
+equals the mean of two right-hand sides of the mesh cut along that face —
+one with the boundary condition (2ce, 2ch) on both cut surfaces and one with
+zero jump there.  Interior PEC (-1, 0) is the mean of PEC (-2, 0) and none;
+PMC (0, -1) likewise; interior SMA (-1/2, -1/2) is the mean of SMA (-1/alpha)
+at alpha = 1 and none.  The product's host setup is checked to classify the
+same faces."""
+import numpy as np
+import pytest
+
+import dgtd_amd as D
+import meshgen as MG
+from oracle import oracle as O
+
+
+def pec_box(n):
+    vx, E, fbc = MG.cube_mesh(n, 1.0, "pec")
+    return vx, E, fbc.astype(np.int32)
+
+
+@pytest.mark.parametrize("order", [1, 2, 4])
+@pytest.mark.parametrize("cond,bcode,alpha", [(4, 1, 1.0), (4, 1, 0.5), (4, 1, 0.0), (5, 2, 1.0), (5, 2, 0.0),
+                                              (6, 3, 1.0)])
+def test_interior_condition_is_mean_of_cut_meshes(order, cond, bcode, alpha):
+    vx, E, fbc = pec_box(2)
+    plane = MG.plane_faces(vx, E, 0, 0.5)
+    assert plane.sum() == 2 * 8  # 8 triangles of the x = 0.5 plane, seen from both sides
+    inner = fbc.copy()
+    inner[plane] = cond
+    vx2, E2 = MG.cut_along_plane(vx, E, 0, 0.5)
+    assert MG.boundary_faces(E2)[plane].all()
+    cut_bc, cut_none = fbc.copy(), fbc.copy()
+    cut_bc[plane] = bcode
+    cut_none[plane] = 0
+    rng = np.random.default_rng(order + cond)
+    eps, mu = rng.uniform(1, 3, E.shape[0]), rng.uniform(1, 2, E.shape[0])
+    A = O.Operator(vx, E, inner, order, alpha=alpha, eps=eps, mu=mu)
+    B1 = O.Operator(vx2, E2, cut_bc, order, alpha=alpha, eps=eps, mu=mu)
+    B0 = O.Operator(vx2, E2, cut_none, order, alpha=alpha, eps=eps, mu=mu)
+    x = rng.standard_normal(A.ndofs)
+    ya = A.mult(x)
+    yb = 0.5 * (B1.mult(x) + B0.mult(x))
+    assert np.abs(ya - yb).max() <= 1e-12 * np.abs(yb).max()
+    # the plain interior face differs (the condition is not a no-op)
+    yp = O.Operator(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu).mult(x)
+    assert np.abs(ya - yp).max() > 1e-3 * np.abs(yp).max()
+
+
+def test_one_sided_code_applies_to_the_face():
+    vx, E, fbc = pec_box(2)
+    plane = MG.plane_faces(vx, E, 1, 0.5)
+    both = fbc.copy()
+    both[plane] = 5
+    one = fbc.copy()
+    rows, cols = np.where(plane)
+    keep = vx[E[rows]].mean(axis=1)[:, 1] < 0.5  # only the y < 0.5 side carries the code
+    one[rows[keep], cols[keep]] = 5
+    x = np.random.default_rng(1).standard_normal(6 * D.num_nodes(2) * E.shape[0])
+    ya = O.Operator(vx, E, both, 2).mult(x)
+    yb = O.Operator(vx, E, one, 2).mult(x)
+    np.testing.assert_array_equal(ya, yb)
+    # host setup: both sides report the code, the neighbour is not gathered (P = M)
+    M, P, codes = D.mesh_vmaps(vx, E, one, 2)
+    np.testing.assert_array_equal(codes, np.where(plane, 5, np.where(MG.boundary_faces(E), fbc, 0)))
+    Nfp = D.num_face_nodes(2)
+    Pm = P.reshape(-1, 4, Nfp)
+    Mm = M.reshape(-1, 4, Nfp)
+    np.testing.assert_array_equal(Pm[plane], Mm[plane])
+
+
+@pytest.mark.parametrize("bad,msg", [("pec_interior", "1-3 on an interior face"),
+                                     ("int_on_boundary", "4-6 on a boundary face"),
+                                     ("mismatch", "different codes")])
+def test_inconsistent_codes_are_refused(bad, msg):
+    vx, E, fbc = pec_box(1)
+    plane = MG.plane_faces(vx, E, 0, 1.0)  # outer boundary x = 1
+    diag = ~MG.boundary_faces(E)
+    b = fbc.copy()
+    if bad == "pec_interior":
+        b[np.where(diag)[0][0], np.where(diag)[1][0]] = 1
+    elif bad == "int_on_boundary":
+        b[plane] = 4
+    else:
+        r, c = np.where(diag)
+        b[r[0], c[0]] = 4
+        # the other side of the same face gets 5
+        key = sorted(E[r[0], list(MG.FACE_VERTS[c[0]])])
+        for e in range(E.shape[0]):
+            for f in range(4):
+                if (e, f) != (r[0], c[0]) and sorted(E[e, list(MG.FACE_VERTS[f])]) == key:
+                    b[e, f] = 5
+    with pytest.raises(D.DgtdError, match=msg):
+        D.mesh_vmaps(vx, E, b, 2)
+    with pytest.raises(RuntimeError):
+        O.Operator(vx, E, b, 2)
