@@ -60,23 +60,62 @@ def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup, kernel="au
                 bytes_el=bytes_el, flops_el=flops_el, variant=variant)
 
 
-def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0):
+def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0, threads=None):
     """LSERK4 steps (5 operator applications each) of the oracle on the host,
-    as many as fit in about budget_s seconds (at least one)."""
+    as many as fit in about budget_s seconds (at least one), on `threads`
+    OpenMP threads (default: all)."""
     from oracle import oracle as O
 
     ref = O.Operator(vx, E.reshape(-1, 4), fbc.reshape(-1, 4).astype(np.int32), order)
-    q = q0.copy()
-    res = np.zeros_like(q)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        ref.lserk4_step(q, res, steps * dt, dt)
-        steps += 1
-        wall = time.perf_counter() - t0
-        if wall >= budget_s or wall / steps * (steps + 1) > 2.0 * budget_s:
-            break
-    return wall, steps, O.threads()
+    all_threads = O.threads()
+    if threads:
+        O.set_threads(threads)
+    try:
+        q = q0.copy()
+        res = np.zeros_like(q)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            ref.lserk4_step(q, res, steps * dt, dt)
+            steps += 1
+            wall = time.perf_counter() - t0
+            if wall >= budget_s or wall / steps * (steps + 1) > 2.0 * budget_s:
+                break
+        used = O.threads()
+    finally:
+        O.set_threads(all_threads)
+    return wall, steps, used
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (calibrated against a plain copy) for this kernel and workload, from the
+    counter summaries under profiles/; None when no summary matches."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith("_hbm_counters.md"):
+            continue
+        kv = {}
+        for ln in open(os.path.join(pdir, name)):
+            if " = " in ln:
+                k, v = ln.strip().split(" = ", 1)
+                kv[k] = v
+        if kv.get("kernel") == kernel and kv.get("workload") == workload and "traffic_bytes_per_launch" in kv:
+            return float(kv["traffic_bytes_per_launch"]), "profiles/" + name
+    return None, None
+
+
+def mass_norm(D, vx, E, fbc, order, v):
+    """sqrt(sum_e J_e v_e^T M v_e) over all six components, from the library's
+    FP64 energy reduction (vacuum: energy = norm^2 / 2)."""
+    op = D.HesthavenEvolution(vx, E, fbc, order, precision=64, kernel="valu")
+    try:
+        op.set_state(v, 0.0)
+        return math.sqrt(2.0 * op.norms()["energy"])
+    finally:
+        op.close()
 
 
 def bench_c3(D, args):
@@ -245,6 +284,8 @@ def main():
     achieved_gbs = r32["bytes_el"] * K / kern_s / 1e9
     achieved_tf = r32["flops_el"] * K / kern_s / 1e12
     t_star = max(r32["bytes_el"] * K / (HBM_PEAK_GBS * 1e9), r32["flops_el"] * K / (FP32_PEAK_TFLOPS * 1e12))
+    kname = ("dg_stage_mfma" if r32["variant"] == "mfma" else "dg_stage_kernel") + f"<{order},float,LSERK>"
+    traffic, traffic_src = profiled_traffic(kname, f"cavity{n}_p{order}_fp32_lserk4")
     out = {
         "metric": "DOF*RK-stages/s (FP32 fused LSERK4 stage, hesthaven DGTD operator)",
         "value": value,
@@ -271,9 +312,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": ("dg_stage_mfma" if r32["variant"] == "mfma" else "dg_stage_kernel")
-                      + f"<{order},float,LSERK>",
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": kname,
             "kernel_avg_us": r32["kern_us"],
             "launches": r32["launches"],
             "algorithmic_bytes_per_launch": r32["bytes_el"] * K,
@@ -298,10 +339,13 @@ def main():
         out["fp32_over_fp64"] = value / v64
         a, b = r32["state"], r64["state"]
         out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        out["rel_l2_mass_fp32_vs_fp64"] = mass_norm(D, vx, E, fbc, order, a - b) / mass_norm(D, vx, E, fbc,
+                                                                                              order, b)
         out["energy_fp64"] = r64["norms"]["energy"]
     out["energy_fp32"] = r32["norms"]["energy"]
     if not args.no_cpu:
         wall, csteps, threads = cpu_baseline(vx, E, fbc, order, q0, dt)
+        wall1, csteps1, _ = cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=8.0, threads=1)
         out["cpu_baseline"] = {
             "value": dof * 5 * csteps / wall,
             "unit": "DOF*RK-stages/s",
@@ -309,6 +353,9 @@ def main():
             "kind": "port",
             "sample": f"{csteps} LSERK4 steps (5 stages each) of the same {K}-tet p={order} mesh, "
                       f"FP64 CPU oracle, OpenMP {threads} threads, {wall:.2f} s",
+            "value_1_thread": dof * 5 * csteps1 / wall1,
+            "sample_1_thread": f"{csteps1} LSERK4 step(s) of the same mesh on 1 thread (the reference's "
+                               f"serial Mult), {wall1:.2f} s",
         }
     print(json.dumps(out))
 

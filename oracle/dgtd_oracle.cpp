@@ -414,6 +414,15 @@ int oracle_threads(void)
 #endif
 }
 
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 void oracle_gll01(int p, double* x)
 {
     std::vector<double> g = gll(p);

@@ -95,6 +95,8 @@ double oracle_energy(const oracle_op* op, const double* x);
 
 /* Number of OpenMP threads oracle_mult uses (1 when built without OpenMP). */
 int oracle_threads(void);
+/* Sets that number for later calls (n > 0). */
+void oracle_set_threads(int n);
 
 #ifdef __cplusplus
 }

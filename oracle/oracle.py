@@ -56,6 +56,8 @@ def lib():
         L.oracle_ref_element.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         L.oracle_gll01.argtypes = [ctypes.c_int, _dp]
         L.oracle_threads.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -171,3 +173,7 @@ class Operator:
 
 def threads():
     return lib().oracle_threads()
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
