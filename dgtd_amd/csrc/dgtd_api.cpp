@@ -385,7 +385,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->d_jac = static_cast<double*>(upload(md.jac));
         op->d_eps = static_cast<double*>(upload(md.eps));
         op->d_mu = static_cast<double*>(upload(md.mu));
-        op->nparts = std::max(1, std::min(1024, (op->K + 255) / 256));
+        const int te = dgtd::norms_tile(op->Np);
+        op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
         hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
         hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
         const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;

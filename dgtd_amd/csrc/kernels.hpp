@@ -106,6 +106,14 @@ int batch_blocks_per_cu(size_t smem);
 // FP64 reductions over a state in T: per-block partials then a fixed-order
 // final pass (deterministic, no atomics).
 //   out[0] = sum of squares, out[1] = energy, out[2] = non-finite count.
+// Elements per tile of the norms reduction: mass (Np^2), tile (6*TE*Np) and
+// reduction scratch (3*256) doubles stay within 60 KiB of LDS.
+inline int norms_tile(int Np)
+{
+    const int te = (61440 - 8 * Np * Np - 8 * 3 * 256) / (6 * 8 * Np);
+    return te < 1 ? 1 : (te > 32 ? 32 : te);
+}
+
 template <typename T>
 hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const double* jac,
                         const double* eps, const double* mu, double* partials, int nparts,

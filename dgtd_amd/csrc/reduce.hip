@@ -25,37 +25,50 @@ __device__ void block_reduce3(double v[3], double* sh)
     for (int k = 0; k < 3; ++k) v[k] = sh[k * kThreads];
 }
 
+// One block per tile of TE elements at a time: the tile's six components are
+// read once, coalesced, into LDS (2-norm and non-finite count on the way),
+// then each thread forms u^T M u for one (element, component) pair.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void norms_partial(const T* __restrict__ q, int K, int Np,
+__global__ __launch_bounds__(kThreads) void norms_partial(const T* __restrict__ q, int K, int Np, int TE,
                                                           const double* __restrict__ M,
                                                           const double* __restrict__ jac,
                                                           const double* __restrict__ eps,
                                                           const double* __restrict__ mu,
                                                           double* __restrict__ partials)
 {
-    extern __shared__ double sh[];  // Np*Np mass, then 3*kThreads scratch
+    extern __shared__ double sh[];  // Np*Np mass, 6*TE*Np tile, 3*kThreads scratch
     double* sM = sh;
-    double* sR = sh + Np * Np;
+    double* sU = sh + Np * Np;
+    double* sR = sU + 6 * TE * Np;
     for (int i = threadIdx.x; i < Np * Np; i += kThreads) sM[i] = M[i];
-    __syncthreads();
     const size_t N = (size_t)Np * K;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int e = blockIdx.x * kThreads + threadIdx.x; e < K; e += gridDim.x * kThreads) {
-        double sE = 0.0, sH = 0.0;
-        for (int c = 0; c < 6; ++c) {
-            const T* u = q + c * N + (size_t)e * Np;
+    const int ntiles = (K + TE - 1) / TE;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int e0 = tile * TE;
+        const int ne = min(TE, K - e0);
+        const int cnt = ne * Np;
+        __syncthreads();  // previous tile's reads done (and sM loaded)
+        for (int c = 0; c < 6; ++c)
+            for (int t = threadIdx.x; t < cnt; t += kThreads) {
+                const double v = (double)q[c * N + (size_t)e0 * Np + t];
+                acc[0] += v * v;
+                if (!isfinite(v)) acc[2] += 1.0;
+                sU[c * TE * Np + t] = v;
+            }
+        __syncthreads();
+        for (int pr = threadIdx.x; pr < 6 * ne; pr += kThreads) {
+            const int c = pr / ne, el = pr - c * ne;
+            const double* u = sU + c * TE * Np + el * Np;
             double s = 0.0;
             for (int i = 0; i < Np; ++i) {
-                const double ui = (double)u[i];
-                acc[0] += ui * ui;
-                if (!isfinite(ui)) acc[2] += 1.0;
                 double r = 0.0;
-                for (int j = 0; j < Np; ++j) r += sM[i * Np + j] * (double)u[j];
-                s += ui * r;
+                for (int j = 0; j < Np; ++j) r += sM[i * Np + j] * u[j];
+                s += u[i] * r;
             }
-            if (c < 3) sE += s; else sH += s;
+            const int e = e0 + el;
+            acc[1] += 0.5 * jac[e] * (c < 3 ? eps[e] : mu[e]) * s;
         }
-        acc[1] += 0.5 * jac[e] * (eps[e] * sE + mu[e] * sH);
     }
     block_reduce3(acc, sR);
     if (threadIdx.x == 0)
@@ -84,8 +97,9 @@ hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const dou
                         const double* eps, const double* mu, double* partials, int nparts,
                         double* out3, hipStream_t s)
 {
-    const size_t sh = sizeof(double) * ((size_t)Np * Np + 3 * kThreads);
-    hipLaunchKernelGGL(norms_partial<T>, dim3(nparts), dim3(kThreads), sh, s, q, K, Np, Mref, jac, eps,
+    const int TE = norms_tile(Np);
+    const size_t sh = sizeof(double) * ((size_t)Np * Np + 6 * (size_t)TE * Np + 3 * kThreads);
+    hipLaunchKernelGGL(norms_partial<T>, dim3(nparts), dim3(kThreads), sh, s, q, K, Np, TE, Mref, jac, eps,
                        mu, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
