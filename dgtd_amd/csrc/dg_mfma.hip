@@ -93,6 +93,21 @@ struct MWaves {
     static constexpr bool ok = bytes <= budget;  // FP64 at order 5 does not fit
 };
 
+// Value of the lane 8 away within the 16-lane row (DPP row_ror:8 swaps the
+// two halves of every row: rows l%16 < 8 and >= 8 of an MFMA A operand).
+__device__ __forceinline__ float dpp_swap8(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ double dpp_swap8(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffLL), 0x128, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), 0x128, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -207,77 +222,77 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
             for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
 
         // ---- surface -> LIFT ----
-        // k-step s covers face f = s / SF, nodes b = 4*(s % SF) + q, so the
-        // lanes of an element read the same neighbour element.  Neighbours in
-        // this task come from LDS; others from memory, prefetched one step
-        // ahead; boundary faces use the own trace with the boundary
-        // coefficients (HesthavenEvolution.cpp:290-292).
+        // Face by face; k-step t of face f covers nodes b = 4t + q, so the
+        // lanes of an element read the same neighbour element.  A lane forms
+        // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
+        // takes the other field's jumps from its partner row (lane ^ 8, one
+        // DPP row rotation).  Neighbours inside the task and boundary faces
+        // read LDS (a boundary face reads the element's own trace, weighted
+        // by the boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
+        // 309-311); a neighbour in another task is gathered from memory for
+        // all k-steps of the face when the face starts.
         const T* Ue = U + el * US;
-        auto remote_of = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
-        auto node_of = [&](int f, int b) {
-            const int bb = b < NFP ? b : NFP - 1;
-            return fnb[f] >= 0 ? sPt[fcode[f] * NFP + bb] : sFm[f * NFP + bb];
+        const int cbase = half ? 3 : 0;  // own field's first component
+        // per-face setup: neighbour kind, node table, LDS source, own-trace weight
+        auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
+        auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
+        auto gather = [&](int s, T (&v)[3]) {  // remote trace of k-step s (own field's 3 components)
+            const int f = s / SF, b = 4 * (s % SF) + q;
+            const T* src = a.qin + (size_t)cbase * N + (size_t)fnb[f] * NP + face_tab(f)[b < NFP ? b : NFP - 1];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = src[c * N];
         };
-        T pf[6];
-        if (remote_of(0)) {
-            const T* src = a.qin + (size_t)fnb[0] * NP + node_of(0, q);
+        T pf[3] = {T(0), T(0), T(0)};
+        if (face_remote(0)) gather(0, pf);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) pf[c] = src[c * N];
-        }
-#pragma unroll
-        for (int s = 0; s < 4 * SF; ++s) {
-            const int f = s / SF;
-            const int b = 4 * (s % SF) + q;
+        for (int f = 0; f < 4; ++f) {
             const int nb = fnb[f];
             const bool interior = nb >= 0;
-            const bool remote = remote_of(f);
-            const int ip = node_of(f, b);
-            T cur[6];
+            const bool remote = face_remote(f);
+            const int* tab = face_tab(f);
+            const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
+            T cown;
             {
-                const T* Un = U + (interior && !remote ? nb - e0 : el) * US;
-#pragma unroll
-                for (int c = 0; c < 6; ++c) cur[c] = remote ? pf[c] : Un[c * TE * US + ip];
-            }
-            if (s + 1 < 4 * SF) {
-                const int f1 = (s + 1) / SF;
-                if (remote_of(f1)) {
-                    const T* src = a.qin + (size_t)fnb[f1] * NP + node_of(f1, 4 * ((s + 1) % SF) + q);
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) pf[c] = src[c * N];
-                }
-            }
-            T ce = T(-1), ch = T(-1), cI = T(1);
-            if (!interior) {
-                cI = T(0);
+                T ce, ch;
                 bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
-            }
-            const int iM = sFm[f * NFP + (b < NFP ? b : NFP - 1)];
-            // A = jump of the "other" field, B = jump of the output's own field
-            T A3[3], B3[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T dE = cI * cur[c] + ce * Ue[c * TE * US + iM];
-                const T dH = cI * cur[3 + c] + ch * Ue[(3 + c) * TE * US + iM];
-                A3[c] = half ? dE : dH;
-                B3[c] = half ? dH : dE;
+                cown = interior ? T(-1) : (half ? ch : ce) - T(1);
             }
             const T* gf = ge + 9 + 4 * f;
             const T n3[3] = {gf[0], gf[1], gf[2]};
-            const T fs = (b < NFP) ? gf[3] : T(0);  // padded face slots add nothing
-            const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
-            // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
-            T F[3];
+            const T fsc = gf[3] * scale;
 #pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const int y = (x + 1) % 3, z = (x + 2) % 3;
-                F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * (fs * scale);
-            }
-            const T* bl = sBl + (f * NFPP + b) * NPG + li;
+            for (int t = 0; t < SF; ++t) {
+                const int s = f * SF + t;
+                const int b = 4 * t + q;
+                const int bb = b < NFP ? b : NFP - 1;
+                const int iM = sFm[f * NFP + bb];
+                const int ip = tab[bb];
+                T cur[3];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const T bv = bl[16 * g];
+                for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[c] : Ub[c * TE * US + ip];
+                if (s + 1 < 4 * SF && face_remote((s + 1) / SF)) gather(s + 1, pf);
+                T B3[3], A3[3];
 #pragma unroll
-                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
+                for (int c = 0; c < 3; ++c) {
+                    B3[c] = cur[c] + cown * Ue[(cbase + c) * TE * US + iM];
+                    A3[c] = dpp_swap8(B3[c]);  // partner row's jump: the other field
+                }
+                const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
+                const T w = (b < NFP) ? fsc : T(0);  // padded face slots add nothing
+                // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
+                T F[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int y = (x + 1) % 3, z = (x + 2) % 3;
+                    F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * w;
+                }
+                const T* bl = sBl + (f * NFPP + b) * NPG + li;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const T bv = bl[16 * g];
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
+                }
             }
         }
 
