@@ -54,7 +54,8 @@ class CaseInfo(ctypes.Structure):
     _fields_ = [("K", c_int32), ("nv", c_int32), ("order", c_int32), ("alpha", c_double),
                 ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
                 ("evolution_operator", c_int32), ("nsources", c_int32), ("nprobes", c_int32),
-                ("exporter_steps", c_int32), ("refinement", c_int32)]
+                ("exporter_steps", c_int32), ("refinement", c_int32), ("snapshot_steps", c_int32),
+                ("snapshot_saves", c_int32)]
 
 
 DGTD_E_CAPACITY = -6
@@ -96,6 +97,8 @@ SIGNATURES = {
     "dgtd_case_mesh": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc)]),
     "dgtd_case_sources": (c_int32, [c_void_p, ctypes.POINTER(InitialField)]),
     "dgtd_case_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p]),
+    "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
+    "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),
     "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),

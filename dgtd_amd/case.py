@@ -48,9 +48,22 @@ class Case:
             self.probe_saves = np.zeros(ci.nprobes, dtype=np.int32)
             check(lib().dgtd_case_probes(h, dptr(self.probes), i32ptr(self.probe_steps),
                                          i32ptr(self.probe_saves)))
-        finally:
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
             lib().dgtd_case_destroy(h)
             self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def save_mesh(self, path):
+        """The case's (refined) mesh in MFEM v1.0 format."""
+        check(lib().dgtd_case_save_mesh(self._h, os.fspath(path).encode()))
 
     @property
     def order(self):
@@ -86,6 +99,12 @@ class Case:
         return self.info["cfl"] / (op.max_fscale() * self.order ** 2)
 
 
+def save_fields(directory, order, x, t):
+    """Ex.gf ... Hz.gf (MFEM GridFunctions, L2 GaussLobatto, order p) and time.txt."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    check(lib().dgtd_save_fields(os.fspath(directory).encode(), int(order), x.size, dptr(x), float(t)))
+
+
 def run_case(path, precision=64, integrator="rk4", final_time=None, kernel="auto"):
     """Load a case, project its initial fields and run it to final time on the
     GPU.  Returns (fields, info) with info holding steps, dt and time."""
@@ -103,4 +122,4 @@ def run_case(path, precision=64, integrator="rk4", final_time=None, kernel="auto
         op.close()
 
 
-__all__ = ["Case", "run_case", "OPERATORS"]
+__all__ = ["Case", "run_case", "save_fields", "OPERATORS"]

@@ -3,6 +3,7 @@
 // hesthaven path can run, read into plain arrays for dgtd_op_create.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -125,10 +126,17 @@ void read_sources(const dgtd::Json& c, dgtd_case& cs)
 void read_probes(const dgtd::Json& c, dgtd_case& cs)
 {
     cs.info.exporter_steps = 0;
+    cs.info.snapshot_steps = -1;
+    cs.info.snapshot_saves = 0;
     if (!c.contains("probes")) return;
     const dgtd::Json& p = c.at("probes");
     if (p.contains("exporter") && p.at("exporter").contains("steps"))
         cs.info.exporter_steps = p.at("exporter").at("steps").integer();
+    if (p.contains("domain_snapshot")) {  // driver.cpp:808-820
+        const dgtd::Json& d = p.at("domain_snapshot");
+        cs.info.snapshot_steps = d.contains("steps") ? d.at("steps").integer() : 0;
+        cs.info.snapshot_saves = d.contains("saves") ? d.at("saves").integer() : 0;
+    }
     if (p.contains("point")) {
         const dgtd::Json& pts = p.at("point");
         for (size_t i = 0; i < pts.size(); ++i) {
@@ -271,6 +279,41 @@ extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
     mesh->face_bc = c->face_bc.data();
     mesh->eps_r = c->eps.empty() ? nullptr : c->eps.data();
     mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_save_mesh(const dgtd_case* c, const char* path)
+{
+    if (!c || !path) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    try {
+        dgtd::write_mfem_mesh(c->mesh, path);
+    } catch (const std::exception& e) {
+        return dgtd::api_fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t)
+{
+    if (!dir || !x) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    if (order < 1 || order > 5) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    if (ndofs <= 0 || ndofs % 6) return dgtd::api_fail(DGTD_E_ARG, "ndofs must be a positive multiple of 6");
+    static const char* names[6] = {"Ex.gf", "Ey.gf", "Ez.gf", "Hx.gf", "Hy.gf", "Hz.gf"};
+    const int64_t n = ndofs / 6;
+    const std::string base(dir);
+    const std::string fec = "L2_T1_3D_P" + std::to_string(order);  // L2, GaussLobatto basis (type 1)
+    for (int c = 0; c < 6; ++c) {
+        const std::string path = base + "/" + names[c];
+        FILE* f = std::fopen(path.c_str(), "w");
+        if (!f) return dgtd::api_fail(DGTD_E_ARG, "cannot write " + path);
+        std::fprintf(f, "FiniteElementSpace\nFiniteElementCollection: %s\nVDim: 1\nOrdering: 0\n\n", fec.c_str());
+        for (int64_t i = 0; i < n; ++i) std::fprintf(f, "%.16g\n", x[c * n + i]);
+        if (std::fclose(f) != 0) return dgtd::api_fail(DGTD_E_ARG, "write failed: " + path);
+    }
+    FILE* f = std::fopen((base + "/time.txt").c_str(), "w");
+    if (!f) return dgtd::api_fail(DGTD_E_ARG, "cannot write " + base + "/time.txt");
+    std::fprintf(f, "%.17g", t);
+    std::fclose(f);
     return DGTD_OK;
 }
 

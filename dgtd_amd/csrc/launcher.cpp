@@ -3,7 +3,9 @@
 // operator.  Mirrors Solver::run (src/solver/Solver.cpp:483-551) with point
 // probes written as ProbesManager writes them (ProbesManager.cpp:183-218,
 // 375-406): Exports/<case>/PointProbes/PointProbe<ID>.dat, one line per
-// sample with time / c (seconds) and Ex Ey Ez Hx Hy Hz.
+// sample with time / c (seconds) and Ex Ey Ez Hx Hy Hz; and domain snapshots
+// as ProbesManager writes them (:457-495): DomainSnapshotProbes/meshes/mesh_rank0
+// and DomainSnapshotProbes/rank_0/cycle_NNNNNN/{Ex,Ey,Ez,Hx,Hy,Hz}.gf, time.txt.
 //
 //   dgtd_launch CASE.json [--precision 64|32] [--integrator rk4|lserk4]
 //               [--kernel auto|valu|mfma] [--final-time T] [--out DIR]
@@ -139,6 +141,28 @@ int main(int argc, char** argv)
             files[p] << "Time (s) // Ex // Ey // Ez // Hx // Hy // Hz \n";
         }
     }
+    // domain snapshots (ProbesManager.cpp:457-495): the mesh once, then
+    // Ex..Hz GridFunctions and time.txt per sampled cycle
+    const bool snapshots = info.snapshot_steps >= 0;
+    const int snap_every = info.snapshot_steps > 0 ? info.snapshot_steps
+                           : info.snapshot_saves > 0 ? std::max(1, total_steps / info.snapshot_saves) : 1;
+    const std::string snap_base = out_dir + "/DomainSnapshotProbes/";
+    if (snapshots) {
+        std::filesystem::create_directories(snap_base + "meshes");
+        check(dgtd_case_save_mesh(cs, (snap_base + "meshes/mesh_rank0").c_str()), "snapshot mesh");
+    }
+    long nsnap = 0;
+    auto snapshot = [&](long cycle, double t, bool final) {
+        if (!snapshots || !(final || cycle % snap_every == 0)) return;
+        char name[32];
+        std::snprintf(name, sizeof(name), "cycle_%06ld", cycle);
+        const std::string dir = snap_base + "rank_0/" + name;
+        std::filesystem::create_directories(dir);
+        double tt = 0.0;
+        check(dgtd_op_get_state(op, x.data(), &tt), "snapshot state");
+        check(dgtd_save_fields(dir.c_str(), info.order, ndofs, x.data(), t), "snapshot fields");
+        ++nsnap;
+    };
     std::vector<double> vals(6 * (size_t)np);
     auto sample = [&](long cycle, double t, bool final) {
         bool any = false;
@@ -157,20 +181,26 @@ int main(int argc, char** argv)
     const auto t_run = std::chrono::steady_clock::now();
     double t = 0.0;
     long steps = 0;
-    if (np == 0) {
+    if (np == 0 && !snapshots) {
         int32_t n = 0;
         check(dgtd_op_run(op, dt, final_time, &n), "run");
         steps = n;
         check(dgtd_op_time(op, &t), "time");
     } else {
         long cycle = 0;
-        sample(cycle++, t, std::fabs(t - final_time) < 1e-8);
+        auto update = [&]() {  // ProbesManager::updateProbes (ProbesManager.cpp:498-528)
+            const bool final = std::fabs(t - final_time) < 1e-8;
+            sample(cycle, t, final);
+            snapshot(cycle, t, final);
+            ++cycle;
+        };
+        update();
         while (t <= final_time - 1e-8 * dt) {  // Solver::run loop (Solver.cpp:497)
             const double truedt = std::min(dt, final_time - t);  // Solver::step (:537)
             check(dgtd_op_advance(op, truedt, 1), "advance");
             check(dgtd_op_time(op, &t), "time");
             ++steps;
-            sample(cycle++, t, std::fabs(t - final_time) < 1e-8);
+            update();
         }
     }
     double norms[3] = {0, 0, 0};
@@ -180,10 +210,10 @@ int main(int argc, char** argv)
     std::printf("{\"case\": \"%s\", \"K\": %d, \"order\": %d, \"ndofs\": %lld, \"precision\": %d, "
                 "\"integrator\": \"%s\", \"dt\": %.10e, \"steps\": %ld, \"time\": %.10e, \"norm2\": %.10e, "
                 "\"energy\": %.10e, \"nonfinite\": %d, \"probes\": %d, \"setup_s\": %.3f, \"run_s\": %.3f, "
-                "\"dof_stages_per_s\": %.4e, \"out\": \"%s\"}\n",
+                "\"dof_stages_per_s\": %.4e, \"snapshots\": %ld, \"out\": \"%s\"}\n",
                 stem(case_path).c_str(), info.K, info.order, (long long)ndofs, precision,
                 integrator == DGTD_LSERK4 ? "lserk4" : "rk4", dt, steps, t, norms[0], norms[1], (int)norms[2], np,
-                setup_s, run_s, run_s > 0 ? (double)ndofs * stages * steps / run_s : 0.0, out_dir.c_str());
+                setup_s, run_s, run_s > 0 ? (double)ndofs * stages * steps / run_s : 0.0, nsnap, out_dir.c_str());
     for (auto& f : files)
         if (f.is_open()) f.close();
     dgtd_op_destroy(op);

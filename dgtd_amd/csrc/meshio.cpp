@@ -196,6 +196,24 @@ TetMesh read_mfem_mesh(const std::string& path)
     return m;
 }
 
+void write_mfem_mesh(const TetMesh& m, const std::string& path)
+{
+    std::ofstream o(path);
+    if (!o) throw std::runtime_error("cannot write " + path);
+    o.precision(16);
+    o << "MFEM mesh v1.0\n\ndimension\n3\n\nelements\n" << m.tets.size() << "\n";
+    for (size_t e = 0; e < m.tets.size(); ++e)
+        o << m.tet_attr[e] << " 4 " << m.tets[e][0] << " " << m.tets[e][1] << " " << m.tets[e][2] << " "
+          << m.tets[e][3] << "\n";
+    o << "\nboundary\n" << m.tris.size() << "\n";
+    for (size_t i = 0; i < m.tris.size(); ++i)
+        o << m.tri_attr[i] << " 2 " << m.tris[i][0] << " " << m.tris[i][1] << " " << m.tris[i][2] << "\n";
+    o << "\nvertices\n" << m.nv() << "\n3\n";
+    for (int32_t v = 0; v < m.nv(); ++v)
+        o << m.vx[3 * v] << " " << m.vx[3 * v + 1] << " " << m.vx[3 * v + 2] << "\n";
+    if (!o) throw std::runtime_error("write failed: " + path);
+}
+
 TetMesh read_mesh_file(const std::string& path)
 {
     if (ends_with(path, ".msh")) return read_gmsh22(path);

@@ -38,6 +38,12 @@ TetMesh read_gmsh22(const std::string& path);
 // (geometry 2), straight-sided ("vertices" section; curved "nodes" refused).
 TetMesh read_mfem_mesh(const std::string& path);
 
+// Writes the mesh in MFEM v1.0 format (elements with attributes, boundary
+// triangles with attributes, vertices), 16 significant digits, as
+// Mesh::Save(path, 16) does for the reference's snapshot meshes
+// (ProbesManager.cpp:481).
+void write_mfem_mesh(const TetMesh& m, const std::string& path);
+
 // Dispatch on the extension (.msh / .mesh), as driver.cpp:1276-1282.
 TetMesh read_mesh_file(const std::string& path);
 

@@ -230,6 +230,8 @@ typedef struct {
     int32_t nsources, nprobes;
     int32_t exporter_steps;       /* probes.exporter.steps, 0 if absent */
     int32_t refinement;           /* uniform refinement levels applied */
+    int32_t snapshot_steps;       /* probes.domain_snapshot: "steps" (0 if absent), -1 = no snapshots */
+    int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
 } dgtd_case_info;
 
 /* Parse the case file, read and refine its mesh, map tags to boundary codes
@@ -241,6 +243,13 @@ int dgtd_case_get_info(const dgtd_case* c, dgtd_case_info* info);
 int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh);
 /* info.nsources initial fields (buildSources, driver.cpp:531-564). */
 int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out);
+/* Domain snapshots (DomainSnapshotProbe, ProbesManager.cpp:457-495): the
+ * case's (refined) mesh in MFEM v1.0 format, and the six field components of
+ * a state as MFEM GridFunction files Ex.gf ... Hz.gf (L2 GaussLobatto space of
+ * the given order, 16 significant digits) plus time.txt in directory dir. */
+int dgtd_case_save_mesh(const dgtd_case* c, const char* path);
+int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t);
+
 /* Point probes (buildProbes, driver.cpp:707-864): 3 * info.nprobes
  * coordinates, and per probe its "steps" and "saves" entries (0 if absent;
  * either array may be NULL).  The sampling interval is steps, else

@@ -242,3 +242,34 @@ def test_point_location_nodes_outside_and_ties():
     # a shared vertex goes to the lowest-index element holding it
     holds = [e for e in range(E.shape[0]) if (np.abs(vx[E[e]] - 0.5).sum(1) < 1e-14).any()]
     assert elem[2] == min(holds)
+
+
+def read_gf(path):
+    lines = open(path).read().split("\n")
+    assert lines[0] == "FiniteElementSpace"
+    header = dict(ln.split(": ", 1) for ln in lines[1:4])
+    vals = np.array([float(v) for v in lines[5:] if v.strip()])
+    return header, vals
+
+
+def test_snapshot_mesh_and_fields_round_trip(tmp_path):
+    t = D.Case(TM55)  # refined once
+    t.save_mesh(tmp_path / "snap.mesh")
+    case = json.load(open(TM55))
+    case["model"]["filename"] = "snap.mesh"
+    case["model"]["refinement"] = 0
+    json.dump(case, open(tmp_path / "again.json", "w"))
+    r = D.Case(tmp_path / "again.json")
+    np.testing.assert_allclose(r.vx, t.vx, rtol=0, atol=1e-15)  # 16 significant digits, as Mesh::Save(path, 16)
+    np.testing.assert_array_equal(r.EToV, t.EToV)
+    np.testing.assert_array_equal(r.face_bc, t.face_bc)
+    x = t.initial_state()
+    x[1::7] += 0.25  # all six components distinct from zero somewhere
+    from dgtd_amd.case import save_fields
+    save_fields(tmp_path, t.order, x, 0.125)
+    n = x.size // 6
+    for c, name in enumerate(["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]):
+        header, vals = read_gf(tmp_path / f"{name}.gf")
+        assert header == {"FiniteElementCollection": "L2_T1_3D_P2", "VDim": "1", "Ordering": "0"}
+        np.testing.assert_allclose(vals, x[c * n:(c + 1) * n], rtol=1e-15, atol=0)  # 16 significant digits
+    assert float((tmp_path / "time.txt").read_text()) == 0.125
