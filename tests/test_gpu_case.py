@@ -80,3 +80,28 @@ def test_launcher_writes_reference_probe_files(tmp_path):
         # times are written as t / c in seconds with 6 significant digits
         np.testing.assert_allclose(rows[:, 0] * 299792458.0, np.arange(6) * 0.01, rtol=1e-5, atol=1e-12)
         np.testing.assert_allclose(rows[-1, 1:], last[p], rtol=1e-4, atol=1e-9)
+
+
+def test_launcher_domain_snapshots(tmp_path):
+    """3D_Resonant_Box_TM55_H1_P2 (refinement 1, domain_snapshot every 2828
+    steps): 3 steps give snapshots at cycle 0 and at the final time."""
+    tm55 = os.path.join(ROOT, "tests", "golden", "cases", "3D_Resonant_Box_TM55_H1_P2.json")
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, tm55, "--final-time", "3e-4", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["steps"] == 3 and summary["snapshots"] == 2
+    snap = out / "DomainSnapshotProbes"
+    assert (snap / "meshes" / "mesh_rank0").exists()
+    c = D.Case(tm55)
+    x, info = D.run_case(tm55, precision=64, integrator="rk4", final_time=3e-4)
+    n = x.size // 6
+    for cyc, want, t in (("cycle_000000", c.initial_state(), 0.0), ("cycle_000003", x, info["time"])):
+        d = snap / "rank_0" / cyc
+        assert float((d / "time.txt").read_text()) == pytest.approx(t, abs=1e-15)
+        for k, name in enumerate(["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]):
+            lines = (d / f"{name}.gf").read_text().split("\n")
+            assert lines[1] == "FiniteElementCollection: L2_T1_3D_P2"
+            vals = np.array([float(v) for v in lines[5:] if v.strip()])
+            np.testing.assert_allclose(vals, want[k * n:(k + 1) * n], rtol=1e-14, atol=1e-15)
