@@ -63,7 +63,10 @@ struct dgtd_batch {
     dgtd::BatchItem* d_items = nullptr;
     int item_cap = 0;
     int* d_count = nullptr;
+    int* d_ranges = nullptr;  // per order o: [2o, 2o+1] = matrix-core item range
     void* d_mats[dgtd::kMaxBatchOrder + 1] = {};
+    void* d_mats_mfma[dgtd::kMaxBatchOrder + 1] = {};
+    int grid_mfma[dgtd::kMaxBatchOrder + 1] = {};
     int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
 
@@ -88,6 +91,15 @@ struct dgtd_batch {
 
     size_t rs() const { return precision == 64 ? 8 : 4; }
 
+    // Orders 2-5 run on the matrix-core stage where it exists for the
+    // precision; order 1 (4 nodes, 16-column tiles mostly padding) and FP64
+    // order 5 run on the per-node stage.
+    bool use_mfma(int order) const
+    {
+        if (order < 2) return false;
+        return precision == 64 ? dgtd::mfma_available<double>(order) : dgtd::mfma_available<float>(order);
+    }
+
     ~dgtd_batch()
     {
         (void)hipSetDevice(device);
@@ -96,9 +108,10 @@ struct dgtd_batch {
         for (auto& s : slots)
             for (void* p : s.buf)
                 if (p) (void)hipFree(p);
-        for (void* p : {d_inst, (void*)d_items, (void*)d_count}) if (p) (void)hipFree(p);
+        for (void* p : {d_inst, (void*)d_items, (void*)d_count, (void*)d_ranges}) if (p) (void)hipFree(p);
         for (int o = 0; o <= dgtd::kMaxBatchOrder; ++o) {
             if (d_mats[o]) (void)hipFree(d_mats[o]);
+            if (d_mats_mfma[o]) (void)hipFree(d_mats_mfma[o]);
             if (d_tabs[o]) (void)hipFree(d_tabs[o]);
         }
         if (stream) (void)hipStreamDestroy(stream);
@@ -150,15 +163,30 @@ struct dgtd_batch {
             r.final_time = s.final_time;
         }
         hcheck(hipMemcpyAsync(d_inst, h.data(), sizeof(h[0]) * max_slots, hipMemcpyHostToDevice, stream), "H2D table");
-        // work list: running instances' element blocks, sorted by order
+        // work list: running instances' element blocks for the per-node stage
+        // (sorted by order), then per matrix-core order its (instance, task) items
         std::vector<dgtd::BatchItem> items;
         int max_order = 1;
-        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o)
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+            if (use_mfma(o)) continue;
             for (int i = 0; i < max_slots; ++i)
                 if (slots[i].state == dgtd::kSlotRunning && slots[i].order == o) {
                     max_order = std::max(max_order, o);
                     for (int blk = 0; blk < slots[i].nblk; ++blk) items.push_back({i, blk});
                 }
+        }
+        const int nvalu = (int)items.size();
+        int ranges[2 * (dgtd::kMaxBatchOrder + 1)] = {};
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+            ranges[2 * o] = (int)items.size();
+            if (use_mfma(o))
+                for (int i = 0; i < max_slots; ++i)
+                    if (slots[i].state == dgtd::kSlotRunning && slots[i].order == o) {
+                        const int ntask = (slots[i].K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
+                        for (int t = 0; t < ntask; ++t) items.push_back({i, t});
+                    }
+            ranges[2 * o + 1] = (int)items.size();
+        }
         if ((int)items.size() > item_cap) {
             if (d_items) hcheck(hipFree(d_items), "hipFree");
             item_cap = std::max<int>((int)items.size(), 2 * item_cap);
@@ -172,7 +200,8 @@ struct dgtd_batch {
         if (nitems)
             hcheck(hipMemcpyAsync(d_items, items.data(), sizeof(items[0]) * nitems, hipMemcpyHostToDevice, stream),
                    "H2D items");
-        hcheck(hipMemcpyAsync(d_count, &nitems, sizeof(int), hipMemcpyHostToDevice, stream), "H2D count");
+        hcheck(hipMemcpyAsync(d_count, &nvalu, sizeof(int), hipMemcpyHostToDevice, stream), "H2D count");
+        hcheck(hipMemcpyAsync(d_ranges, ranges, sizeof(ranges), hipMemcpyHostToDevice, stream), "H2D ranges");
         hcheck(hipStreamSynchronize(stream), "sync");  // host vectors die here
         if (max_order != graph_order || !graph) capture<T>(max_order);
     }
@@ -193,7 +222,12 @@ struct dgtd_batch {
         a.count = d_count;
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
             a.mats[o] = static_cast<const T*>(d_mats[o]);
+            a.mats_mfma[o] = static_cast<const T*>(d_mats_mfma[o]);
             a.tabs[o] = d_tabs[o];
+            if (use_mfma(o) && !grid_mfma[o]) {
+                const int bpc = dgtd::mfma_blocks_per_cu<T>(o);
+                grid_mfma[o] = std::max(1, bpc) * num_cu;
+            }
         }
         hipGraph_t g = nullptr;
         hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -203,6 +237,8 @@ struct dgtd_batch {
             a.ca = (T)kA[s];
             a.cb = (T)kB[s];
             err = dgtd::launch_batch_stage<T>(a, grid, smem, stream);
+            for (int o = 1; o <= dgtd::kMaxBatchOrder && err == hipSuccess; ++o)
+                if (use_mfma(o)) err = dgtd::launch_batch_stage_mfma<T>(o, a, d_ranges + 2 * o, grid_mfma[o], stream);
         }
         if (err == hipSuccess) err = dgtd::launch_batch_advance<T>(a.inst, max_slots, stream);
         hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -264,6 +300,11 @@ void upload_refs(dgtd_batch* b)
         tabs.insert(tabs.end(), R.face_perm.begin(), R.face_perm.end());
         hcheck(hipMalloc(&b->d_mats[o], sizeof(T) * m.size()), "hipMalloc mats");
         hcheck(hipMemcpy(b->d_mats[o], m.data(), sizeof(T) * m.size(), hipMemcpyHostToDevice), "H2D mats");
+        if (b->use_mfma(o)) {
+            const std::vector<T> mm = dgtd::pack_mats_mfma<T>(R);
+            hcheck(hipMalloc(&b->d_mats_mfma[o], sizeof(T) * mm.size()), "hipMalloc mats");
+            hcheck(hipMemcpy(b->d_mats_mfma[o], mm.data(), sizeof(T) * mm.size(), hipMemcpyHostToDevice), "H2D mats");
+        }
         hcheck(hipMalloc(&b->d_tabs[o], sizeof(int) * tabs.size()), "hipMalloc tabs");
         hcheck(hipMemcpy(b->d_tabs[o], tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice), "H2D tabs");
     }
@@ -314,6 +355,7 @@ int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out)
         const size_t rec = b->precision == 64 ? sizeof(dgtd::BatchInst<double>) : sizeof(dgtd::BatchInst<float>);
         hcheck(hipMalloc(&b->d_inst, rec * b->max_slots), "hipMalloc table");
         hcheck(hipMalloc(&b->d_count, sizeof(int)), "hipMalloc count");
+        hcheck(hipMalloc(&b->d_ranges, sizeof(int) * 2 * (dgtd::kMaxBatchOrder + 1)), "hipMalloc ranges");
         b->item_cap = 1 << 16;
         hcheck(hipMalloc(&b->d_items, sizeof(dgtd::BatchItem) * b->item_cap), "hipMalloc items");
         if (b->precision == 64) upload_refs<double>(b);

@@ -115,44 +115,307 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-block LDS carve-up of the MFMA stage (reference matrices shared by the
+// block, per-wave field and staging tiles).
+template <int P, typename T>
+struct MfmaLds {
+    T* sBv;   // [KP][3*NPG]
+    T* sBl;   // [4*NFPP][NPG], row f*NFPP + b
+    T* sU;    // [WAVES][6][TE][US]
+    T* sS;    // [WAVES][2][TE*NP] output staging
+    int* sFm; // [4*NFP]
+    int* sPt; // [96*NFP]
+};
+
+// Loads the reference matrices and node tables of order P into LDS and zeroes
+// the field tiles (their padding columns j >= NP stay 0); block-wide.
+template <int P, typename T>
+__device__ MfmaLds<P, T> mfma_load_ref(const T* mats, const int* tabs, unsigned char* smem_raw)
+{
+    using S = MShape<P>;
+    constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, US = S::US, NFPP = S::NFPP;
+    constexpr int TE = kTaskElems;
+    constexpr int WAVES = MWaves<P, T>::W;
+    MfmaLds<P, T> L;
+    L.sBv = reinterpret_cast<T*>(smem_raw);
+    L.sBl = L.sBv + KP * 3 * NPG;
+    L.sU = L.sBl + 4 * NFPP * NPG;
+    L.sS = L.sU + WAVES * 6 * TE * US;
+    L.sFm = reinterpret_cast<int*>(L.sS + WAVES * 2 * TE * NP);
+    L.sPt = L.sFm + 4 * NFP;
+    const int tid = threadIdx.x;
+    const int nthreads = WAVES * 64;
+    for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) L.sBv[i] = mats[i];
+    for (int i = tid; i < 4 * NFP + 96 * NFP; i += nthreads) L.sFm[i] = tabs[i];
+    for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) L.sU[i] = T(0);
+    __syncthreads();
+    return L;
+}
+
+// One wave-task: the fused stage of elements [task*8, task*8 + 8) of the mesh
+// described by `a`.  Only wave-level synchronisation inside.
 template <int P, typename T, int MODE>
-__global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArgs<T> a)
+__device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P, T>& L, const int task)
 {
     using S = MShape<P>;
     using MM = Mfma<T>;
     using acc_t = typename MM::acc_t;
-    constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, G = S::G, US = S::US;
+    constexpr int NP = S::NP, NFP = S::NFP, NPG = S::NPG, G = S::G, US = S::US;
     constexpr int NFPP = S::NFPP, SF = S::SF;
     constexpr int TE = kTaskElems;
-    constexpr int WAVES = MWaves<P, T>::W;
-
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sBv = reinterpret_cast<T*>(smem_raw);     // [KP][3*NPG]
-    T* sBl = sBv + KP * 3 * NPG;                 // [4*NFPP][NPG], row f*NFPP + b
-    T* sU = sBl + 4 * NFPP * NPG;                // [WAVES][6][TE][US]
-    T* sS = sU + WAVES * 6 * TE * US;            // [WAVES][2][TE*NP] output staging
-    int* sFm = reinterpret_cast<int*>(sS + WAVES * 2 * TE * NP);  // [4*NFP]
-    int* sPt = sFm + 4 * NFP;                                     // [96*NFP]
-
-    const int tid = threadIdx.x;
-    const int nthreads = WAVES * 64;
-    for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) sBv[i] = a.mats[i];
-    for (int i = tid; i < 4 * NFP + 96 * NFP; i += nthreads) sFm[i] = a.tabs[i];
-    for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) sU[i] = T(0);  // padding j >= NP stays 0
-    __syncthreads();
-
-    const int wave = tid >> 6;
-    const int lane = tid & 63;
+    const T* sBv = L.sBv;
+    const T* sBl = L.sBl;
+    const int* sFm = L.sFm;
+    const int* sPt = L.sPt;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int lane = threadIdx.x & 63;
     const int q = lane >> 4;    // k slot within a step
     const int li = lane & 15;   // A row (= output column within a node group)
     const int el = li & 7;      // element of the A row
     const int half = li >> 3;   // A rows 0-7: E outputs, 8-15: H outputs of the same elements
-    T* U = sU + wave * 6 * TE * US;
-    T* Sg = sS + wave * 2 * TE * NP;
-
+    T* U = L.sU + wave * 6 * TE * US;
+    T* Sg = L.sS + wave * 2 * TE * NP;
     const int K = a.K;
     const size_t N = (size_t)NP * K;
-    const int ntasks = (K + TE - 1) / TE;
+    const int e0 = task * TE;
+    const int ne = min(TE, K - e0);
+
+    // ---- stage the task's fields in LDS: all loads in flight (fields and the
+    // lane's element record), then the LDS writes ----
+    wave_sync();  // previous task's reads of U are done
+    const bool valid = el < ne;
+    const int e = e0 + (valid ? el : 0);
+    const T* ge = a.geo + (size_t)e * kGeoStrideDev;
+    T gm[9];
+    T scale;
+    int fnb[4], fcode[4];
+    {
+        constexpr int NL = (TE * NP + 63) / 64;
+        const int cnt = ne * NP;
+        T v[6][NL];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const T* src = a.qin + c * N + (size_t)e0 * NP;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int t = lane + 64 * k;
+                v[c][k] = (t < cnt) ? src[t] : T(0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) gm[k] = ge[k];
+        // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
+        scale = valid ? ge[25 + half] : T(0);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
+            fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int t = lane + 64 * k;
+                if (t < cnt) {
+                    const int r = t / NP;
+                    U[(c * TE + r) * US + (t - r * NP)] = v[c][k];
+                }
+            }
+    }
+    wave_sync();
+
+    // half 0 forms E-output operands, half 1 H-output
+    const T sign = half ? T(-1) : T(1);
+    const T sscale = sign * scale;
+
+    acc_t R[3][G];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
+
+    // ---- surface -> LIFT ----
+    // Face by face; k-step t of face f covers nodes b = 4t + q, so the
+    // lanes of an element read the same neighbour element.  A lane forms
+    // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
+    // takes the other field's jumps from its partner row (lane ^ 8, one
+    // DPP row rotation).  Neighbours inside the task and boundary faces
+    // read LDS (a boundary face reads the element's own trace, weighted
+    // by the boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
+    // 309-311); a neighbour in another task is gathered from memory for
+    // all k-steps of the face when the face starts.
+    const T* Ue = U + el * US;
+    const int cbase = half ? 3 : 0;  // own field's first component
+    // per-face setup: neighbour kind, node table, LDS source, own-trace weight
+    auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
+    auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
+    auto gather = [&](int s, T (&v)[3]) {  // remote trace of k-step s (own field's 3 components)
+        const int f = s / SF, b = 4 * (s % SF) + q;
+        const T* src = a.qin + (size_t)cbase * N + (size_t)fnb[f] * NP + face_tab(f)[b < NFP ? b : NFP - 1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = src[c * N];
+    };
+    T pf[3] = {T(0), T(0), T(0)};
+    if (face_remote(0)) gather(0, pf);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int nb = fnb[f];
+        const bool interior = nb >= 0;
+        const bool remote = face_remote(f);
+        const int* tab = face_tab(f);
+        const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
+        T cown;
+        {
+            T ce, ch;
+            bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
+            cown = interior ? T(-1) : (half ? ch : ce) - T(1);
+        }
+        const T* gf = ge + 9 + 4 * f;
+        const T n3[3] = {gf[0], gf[1], gf[2]};
+        const T fsc = gf[3] * scale;
+#pragma unroll
+        for (int t = 0; t < SF; ++t) {
+            const int s = f * SF + t;
+            const int b = 4 * t + q;
+            const int bb = b < NFP ? b : NFP - 1;
+            const int iM = sFm[f * NFP + bb];
+            const int ip = tab[bb];
+            T cur[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[c] : Ub[c * TE * US + ip];
+            if (s + 1 < 4 * SF && face_remote((s + 1) / SF)) gather(s + 1, pf);
+            T B3[3], A3[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                B3[c] = cur[c] + cown * Ue[(cbase + c) * TE * US + iM];
+                A3[c] = dpp_swap8(B3[c]);  // partner row's jump: the other field
+            }
+            const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
+            const T w = (b < NFP) ? fsc : T(0);  // padded face slots add nothing
+            // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
+            T F[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * w;
+            }
+            const T* bl = sBl + (f * NFPP + b) * NPG + li;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const T bv = bl[16 * g];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
+            }
+        }
+    }
+
+    // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
+    // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
+#pragma unroll 3
+    for (int ks = 0; ks < S::S; ++ks) {
+        const int j = q * S::S + ks;
+        T src[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
+        const T* bv = sBv + j * 3 * NPG + li;
+#pragma unroll
+        for (int kd = 0; kd < 3; ++kd) {
+            const T* gk = gm + 3 * kd;
+            T w[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const T bval = bv[kd * NPG + 16 * g];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
+            }
+        }
+    }
+
+    // ---- stage update: per x, the E_x and H_x right-hand sides go through
+    // the wave's LDS staging buffer and the update runs as a linear pass
+    // over the task's contiguous chunk (whole sectors, coalesced).  The
+    // residual / base state of the pair x+1 (linear index t over
+    // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
+    constexpr int NLIN = (2 * TE * NP + 63) / 64;
+    constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
+    constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
+    auto lin_index = [&](int x, int t, bool& okt) {
+        const int h = t >= TE * NP;
+        const int u = t - h * TE * NP;
+        okt = t < 2 * TE * NP && u < ne * NP;
+        return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (okt ? u : 0);
+    };
+    T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
+    auto fetch = [&](int x, int slot) {
+#pragma unroll
+        for (int k = 0; k < NLIN; ++k) {
+            bool okt;
+            const size_t o = lin_index(x, lane + 64 * k, okt);
+            if constexpr (kNeedRes) rl[slot][k] = okt ? a.res[o] : T(0);
+            if constexpr (kNeedX) xl[slot][k] = okt ? a.x[o] : T(0);
+        }
+    };
+    fetch(0, 0);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const int cur = x & 1;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int i = 16 * g + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = MM::row(q, r);
+                if (i < NP) Sg[(row >> 3) * TE * NP + (row & 7) * NP + i] = R[x][g][r];
+            }
+        }
+        wave_sync();
+        if (x + 1 < 3) fetch(x + 1, cur ^ 1);
+#pragma unroll
+        for (int k = 0; k < NLIN; ++k) {
+            const int t = lane + 64 * k;
+            bool okt;
+            const size_t o = lin_index(x, t, okt);
+            if (!okt) continue;
+            const int h = t >= TE * NP;
+            const int u = t - h * TE * NP;
+            const int c = h ? 3 + x : x;
+            const int r_el = u / NP;
+            const T rhs = Sg[t];
+            const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
+            if constexpr (MODE == kModeMult) {
+                a.out[o] = rhs;
+            } else if constexpr (MODE == kModeLserk) {
+                const T rr = a.ca * rl[cur][k] + a.dt * rhs;
+                a.res[o] = rr;
+                a.out[o] = qv + a.cb * rr;
+            } else if constexpr (MODE == kModeRk1) {
+                a.out[o] = qv + a.ca * rhs;
+                a.res[o] = qv + a.cb * rhs;
+            } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
+                a.out[o] = xl[cur][k] + a.ca * rhs;
+                a.res[o] = rl[cur][k] + a.cb * rhs;
+            } else {
+                a.out[o] = rl[cur][k] + a.ca * rhs;
+            }
+        }
+        wave_sync();  // staging buffer free for the next x
+    }
+}
+
+template <int P, typename T, int MODE>
+__global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArgs<T> a)
+{
+    constexpr int TE = kTaskElems;
+    constexpr int WAVES = MWaves<P, T>::W;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const MfmaLds<P, T> L = mfma_load_ref<P, T>(a.mats, a.tabs, smem_raw);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int ntasks = (a.K + TE - 1) / TE;
     // XCD-contiguous task ranges: block b runs on XCD b % 8.
     const int nxcd = (gridDim.x >= 8 && gridDim.x % 8 == 0) ? 8 : 1;
     const int xcd = blockIdx.x % nxcd;
@@ -161,237 +424,38 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
     const int per_xcd = (ntasks + nxcd - 1) / nxcd;
     const int t_begin = xcd * per_xcd;
     const int t_end = min(ntasks, t_begin + per_xcd);
+    for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES)
+        mfma_task<P, T, MODE>(a, L, a.order ? a.order[tix] : tix);
+}
 
-    for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES) {
-        const int task = a.order ? a.order[tix] : tix;
-        const int e0 = task * TE;
-        const int ne = min(TE, K - e0);
-
-        // ---- stage the task's fields in LDS: all loads in flight (fields and the
-        // lane's element record), then the LDS writes ----
-        wave_sync();  // previous task's reads of U are done
-        const bool valid = el < ne;
-        const int e = e0 + (valid ? el : 0);
-        const T* ge = a.geo + (size_t)e * kGeoStrideDev;
-        T gm[9];
-        T scale;
-        int fnb[4], fcode[4];
-        {
-            constexpr int NL = (TE * NP + 63) / 64;
-            const int cnt = ne * NP;
-            T v[6][NL];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const T* src = a.qin + c * N + (size_t)e0 * NP;
-#pragma unroll
-                for (int k = 0; k < NL; ++k) {
-                    const int t = lane + 64 * k;
-                    v[c][k] = (t < cnt) ? src[t] : T(0);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) gm[k] = ge[k];
-            // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
-            scale = valid ? ge[25 + half] : T(0);
-#pragma unroll
-            for (int f = 0; f < 4; ++f) {
-                fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
-                fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int k = 0; k < NL; ++k) {
-                    const int t = lane + 64 * k;
-                    if (t < cnt) {
-                        const int r = t / NP;
-                        U[(c * TE + r) * US + (t - r * NP)] = v[c][k];
-                    }
-                }
-        }
-        wave_sync();
-
-        // half 0 forms E-output operands, half 1 H-output
-        const T sign = half ? T(-1) : T(1);
-        const T sscale = sign * scale;
-
-        acc_t R[3][G];
-#pragma unroll
-        for (int x = 0; x < 3; ++x)
-#pragma unroll
-            for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
-
-        // ---- surface -> LIFT ----
-        // Face by face; k-step t of face f covers nodes b = 4t + q, so the
-        // lanes of an element read the same neighbour element.  A lane forms
-        // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
-        // takes the other field's jumps from its partner row (lane ^ 8, one
-        // DPP row rotation).  Neighbours inside the task and boundary faces
-        // read LDS (a boundary face reads the element's own trace, weighted
-        // by the boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
-        // 309-311); a neighbour in another task is gathered from memory for
-        // all k-steps of the face when the face starts.
-        const T* Ue = U + el * US;
-        const int cbase = half ? 3 : 0;  // own field's first component
-        // per-face setup: neighbour kind, node table, LDS source, own-trace weight
-        auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
-        auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
-        auto gather = [&](int s, T (&v)[3]) {  // remote trace of k-step s (own field's 3 components)
-            const int f = s / SF, b = 4 * (s % SF) + q;
-            const T* src = a.qin + (size_t)cbase * N + (size_t)fnb[f] * NP + face_tab(f)[b < NFP ? b : NFP - 1];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = src[c * N];
-        };
-        T pf[3] = {T(0), T(0), T(0)};
-        if (face_remote(0)) gather(0, pf);
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int nb = fnb[f];
-            const bool interior = nb >= 0;
-            const bool remote = face_remote(f);
-            const int* tab = face_tab(f);
-            const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
-            T cown;
-            {
-                T ce, ch;
-                bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
-                cown = interior ? T(-1) : (half ? ch : ce) - T(1);
-            }
-            const T* gf = ge + 9 + 4 * f;
-            const T n3[3] = {gf[0], gf[1], gf[2]};
-            const T fsc = gf[3] * scale;
-#pragma unroll
-            for (int t = 0; t < SF; ++t) {
-                const int s = f * SF + t;
-                const int b = 4 * t + q;
-                const int bb = b < NFP ? b : NFP - 1;
-                const int iM = sFm[f * NFP + bb];
-                const int ip = tab[bb];
-                T cur[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[c] : Ub[c * TE * US + ip];
-                if (s + 1 < 4 * SF && face_remote((s + 1) / SF)) gather(s + 1, pf);
-                T B3[3], A3[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    B3[c] = cur[c] + cown * Ue[(cbase + c) * TE * US + iM];
-                    A3[c] = dpp_swap8(B3[c]);  // partner row's jump: the other field
-                }
-                const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
-                const T w = (b < NFP) ? fsc : T(0);  // padded face slots add nothing
-                // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
-                T F[3];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) {
-                    const int y = (x + 1) % 3, z = (x + 2) % 3;
-                    F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * w;
-                }
-                const T* bl = sBl + (f * NFPP + b) * NPG + li;
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    const T bv = bl[16 * g];
-#pragma unroll
-                    for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
-                }
-            }
-        }
-
-        // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
-        // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
-#pragma unroll 3
-        for (int ks = 0; ks < S::S; ++ks) {
-            const int j = q * S::S + ks;
-            T src[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
-            const T* bv = sBv + j * 3 * NPG + li;
-#pragma unroll
-            for (int kd = 0; kd < 3; ++kd) {
-                const T* gk = gm + 3 * kd;
-                T w[3];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) {
-                    const int y = (x + 1) % 3, z = (x + 2) % 3;
-                    w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
-                }
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    const T bval = bv[kd * NPG + 16 * g];
-#pragma unroll
-                    for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
-                }
-            }
-        }
-
-        // ---- stage update: per x, the E_x and H_x right-hand sides go through
-        // the wave's LDS staging buffer and the update runs as a linear pass
-        // over the task's contiguous chunk (whole sectors, coalesced).  The
-        // residual / base state of the pair x+1 (linear index t over
-        // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
-        constexpr int NLIN = (2 * TE * NP + 63) / 64;
-        constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
-        constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
-        auto lin_index = [&](int x, int t, bool& okt) {
-            const int h = t >= TE * NP;
-            const int u = t - h * TE * NP;
-            okt = t < 2 * TE * NP && u < ne * NP;
-            return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (okt ? u : 0);
-        };
-        T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
-        auto fetch = [&](int x, int slot) {
-#pragma unroll
-            for (int k = 0; k < NLIN; ++k) {
-                bool okt;
-                const size_t o = lin_index(x, lane + 64 * k, okt);
-                if constexpr (kNeedRes) rl[slot][k] = okt ? a.res[o] : T(0);
-                if constexpr (kNeedX) xl[slot][k] = okt ? a.x[o] : T(0);
-            }
-        };
-        fetch(0, 0);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            const int cur = x & 1;
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const int i = 16 * g + li;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = MM::row(q, r);
-                    if (i < NP) Sg[(row >> 3) * TE * NP + (row & 7) * NP + i] = R[x][g][r];
-                }
-            }
-            wave_sync();
-            if (x + 1 < 3) fetch(x + 1, cur ^ 1);
-#pragma unroll
-            for (int k = 0; k < NLIN; ++k) {
-                const int t = lane + 64 * k;
-                bool okt;
-                const size_t o = lin_index(x, t, okt);
-                if (!okt) continue;
-                const int h = t >= TE * NP;
-                const int u = t - h * TE * NP;
-                const int c = h ? 3 + x : x;
-                const int r_el = u / NP;
-                const T rhs = Sg[t];
-                const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
-                if constexpr (MODE == kModeMult) {
-                    a.out[o] = rhs;
-                } else if constexpr (MODE == kModeLserk) {
-                    const T rr = a.ca * rl[cur][k] + a.dt * rhs;
-                    a.res[o] = rr;
-                    a.out[o] = qv + a.cb * rr;
-                } else if constexpr (MODE == kModeRk1) {
-                    a.out[o] = qv + a.ca * rhs;
-                    a.res[o] = qv + a.cb * rhs;
-                } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                    a.out[o] = xl[cur][k] + a.ca * rhs;
-                    a.res[o] = rl[cur][k] + a.cb * rhs;
-                } else {
-                    a.out[o] = rl[cur][k] + a.ca * rhs;
-                }
-            }
-            wave_sync();  // staging buffer free for the next x
-        }
+// Batch variant: the running instances of order P, one (instance, task) work
+// item per wave-task, items [range[0], range[1]) of the list; stage `b.stage`
+// of LSERK4 with the instance's own dt, buffers and geometry.
+template <int P, typename T>
+__global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_batch_stage_mfma(BatchArgs<T> b, const int* range)
+{
+    constexpr int WAVES = MWaves<P, T>::W;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const MfmaLds<P, T> L = mfma_load_ref<P, T>(b.mats_mfma[P], b.tabs[P], smem_raw);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int begin = range[0], end = range[1];
+    for (int it = begin + blockIdx.x * WAVES + wave; it < end; it += gridDim.x * WAVES) {
+        const BatchItem w = b.items[it];
+        const BatchInst<T> I = b.inst[w.inst];
+        if (I.state != kSlotRunning) continue;  // uniform across the wave
+        StageArgs<T> a{};
+        a.K = I.K;
+        const int src = I.cur ^ (b.stage & 1);
+        a.qin = I.q[src];
+        a.out = I.q[src ^ 1];
+        a.res = I.res;
+        a.ca = b.ca;
+        a.cb = b.cb;
+        a.dt = (T)I.dt_step;
+        a.alpha = (T)I.alpha;
+        a.geo = I.geo;
+        a.conn = I.conn;
+        mfma_task<P, T, kModeLserk>(a, L, w.blk);
     }
 }
 
@@ -413,6 +477,8 @@ int mfma_prepare()
     mfma_set_attr<P, T, kModeRk2>();
     mfma_set_attr<P, T, kModeRk3>();
     mfma_set_attr<P, T, kModeRk4>();
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)MWaves<P, T>::bytes);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_stage_mfma<P, T, kModeLserk>,
                                                      MWaves<P, T>::W * 64, MWaves<P, T>::bytes) != hipSuccess)
@@ -438,7 +504,46 @@ hipError_t mfma_launch(const StageArgs<T>& a, int mode, int grid, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int P, typename T>
+hipError_t batch_mfma_launch(const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
+{
+    constexpr int threads = MWaves<P, T>::W * 64;
+    constexpr size_t bytes = MWaves<P, T>::bytes;
+    if constexpr (!MWaves<P, T>::ok) {
+        return hipErrorInvalidConfiguration;
+    } else {
+        hipLaunchKernelGGL((dg_batch_stage_mfma<P, T>), dim3(grid), dim3(threads), bytes, s, b, range);
+        return hipGetLastError();
+    }
+}
+
 }  // namespace
+
+template <typename T>
+bool mfma_available(int order)
+{
+    switch (order) {
+        case 1: return MWaves<1, T>::ok;
+        case 2: return MWaves<2, T>::ok;
+        case 3: return MWaves<3, T>::ok;
+        case 4: return MWaves<4, T>::ok;
+        case 5: return MWaves<5, T>::ok;
+        default: return false;
+    }
+}
+
+template <typename T>
+hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
+{
+    switch (order) {
+        case 1: return batch_mfma_launch<1, T>(b, range, grid, s);
+        case 2: return batch_mfma_launch<2, T>(b, range, grid, s);
+        case 3: return batch_mfma_launch<3, T>(b, range, grid, s);
+        case 4: return batch_mfma_launch<4, T>(b, range, grid, s);
+        case 5: return batch_mfma_launch<5, T>(b, range, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 template <typename T>
 hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s)
@@ -485,5 +590,9 @@ template int mfma_blocks_per_cu<float>(int);
 template int mfma_blocks_per_cu<double>(int);
 template int mfma_waves_per_block<float>(int);
 template int mfma_waves_per_block<double>(int);
+template bool mfma_available<float>(int);
+template bool mfma_available<double>(int);
+template hipError_t launch_batch_stage_mfma<float>(int, const BatchArgs<float>&, const int*, int, hipStream_t);
+template hipError_t launch_batch_stage_mfma<double>(int, const BatchArgs<double>&, const int*, int, hipStream_t);
 
 }  // namespace dgtd

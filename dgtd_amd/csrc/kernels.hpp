@@ -85,8 +85,9 @@ template <typename T>
 struct BatchArgs {
     BatchInst<T>* inst;
     const BatchItem* items;
-    const int* count;  // number of valid items (device)
-    const T* mats[kMaxBatchOrder + 1];
+    const int* count;  // number of valid items of the per-node kernel (device)
+    const T* mats[kMaxBatchOrder + 1];       // per-node kernel layout
+    const T* mats_mfma[kMaxBatchOrder + 1];  // matrix-core kernel layout
     const int* tabs[kMaxBatchOrder + 1];
     int stage;
     T ca, cb;
@@ -94,6 +95,14 @@ struct BatchArgs {
 
 template <typename T>
 hipError_t launch_batch_stage(const BatchArgs<T>& b, int grid, size_t smem, hipStream_t s);
+// Matrix-core batch stage for the instances of one order: items (instance,
+// 8-element task) in [range[0], range[1]) of b.items (range in device memory).
+template <typename T>
+hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s);
+// Whether the matrix-core variant exists for this order and precision (its
+// LDS tiles fit; FP64 at order 5 does not).
+template <typename T>
+bool mfma_available(int order);
 template <typename T>
 hipError_t launch_batch_advance(BatchInst<T>* inst, int nslots, hipStream_t s);
 // LDS bytes of the batch kernel when orders up to max_order are present,
