@@ -19,6 +19,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 
 DGTD_OK = 0
 DGTD_BC_NONE, DGTD_BC_PEC, DGTD_BC_PMC, DGTD_BC_SMA = 0, 1, 2, 3
+DGTD_BC_INT_PEC, DGTD_BC_INT_PMC, DGTD_BC_INT_SMA, DGTD_BC_TFSF = 4, 5, 6, 7
 DGTD_RK4, DGTD_LSERK4 = 0, 1
 DGTD_KERNEL_AUTO, DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA = 0, 1, 2
 
@@ -45,6 +46,11 @@ class InitialField(ctypes.Structure):
                 ("nmodes", c_int32), ("modes", c_int32 * 3)]
 
 
+class PlaneWave(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("polarization", c_double * 3), ("propagation", c_double * 3),
+                ("spread", c_double), ("mean", c_double), ("frequency", c_double)]
+
+
 class BatchOpts(ctypes.Structure):
     _fields_ = [("alpha", c_double), ("precision", c_int32), ("device", c_int32),
                 ("max_instances", c_int32), ("reserved", c_int32), ("hbm_budget", ctypes.c_int64)]
@@ -55,7 +61,7 @@ class CaseInfo(ctypes.Structure):
                 ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
                 ("evolution_operator", c_int32), ("nsources", c_int32), ("nprobes", c_int32),
                 ("exporter_steps", c_int32), ("refinement", c_int32), ("snapshot_steps", c_int32),
-                ("snapshot_saves", c_int32)]
+                ("snapshot_saves", c_int32), ("nplanewaves", c_int32)]
 
 
 DGTD_E_CAPACITY = -6
@@ -82,6 +88,7 @@ SIGNATURES = {
     "dgtd_op_time": (c_int32, [c_void_p, _dp]),
     "dgtd_op_step": (c_int32, [c_void_p, _dp, _dp, c_double, c_int32]),
     "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_set_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave), c_int32]),
     "dgtd_op_probe_set": (c_int32, [c_void_p, c_int32, _dp, _i32p]),
     "dgtd_op_probe_sample": (c_int32, [c_void_p, _dp]),
     "dgtd_op_set_timing": (c_int32, [c_void_p, c_int32]),
@@ -98,6 +105,7 @@ SIGNATURES = {
     "dgtd_case_sources": (c_int32, [c_void_p, ctypes.POINTER(InitialField)]),
     "dgtd_case_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p]),
     "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
+    "dgtd_case_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave)]),
     "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),

@@ -19,7 +19,8 @@ OPERATORS = {_abi.DGTD_OPERATOR_MAXWELL: "maxwell", _abi.DGTD_OPERATOR_GLOBAL: "
 
 class Case:
     """A loaded case: `info` (dict), mesh arrays `vx`, `EToV`, `face_bc`, `eps`,
-    `mu`, `sources` (InitialField list) and `probes` (n x 3)."""
+    `mu`, `sources` (InitialField list), `planewaves` (PlaneWave list, the
+    TF/SF faces carry code 7) and `probes` (n x 3)."""
 
     def __init__(self, path):
         self.path = os.path.abspath(path)
@@ -43,6 +44,9 @@ class Case:
             arr = (_abi.InitialField * max(n, 1))()
             check(lib().dgtd_case_sources(h, arr))
             self.sources = [arr[i] for i in range(n)]
+            w = (_abi.PlaneWave * max(ci.nplanewaves, 1))()
+            check(lib().dgtd_case_planewaves(h, w))
+            self.planewaves = [w[i] for i in range(ci.nplanewaves)]
             self.probes = np.zeros((ci.nprobes, 3))
             self.probe_steps = np.zeros(ci.nprobes, dtype=np.int32)
             self.probe_saves = np.zeros(ci.nprobes, dtype=np.int32)
@@ -87,9 +91,12 @@ class Case:
         return x
 
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
-        return HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
-                                  eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
-                                  kernel=kernel, device=device)
+        op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
+                                eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
+                                kernel=kernel, device=device)
+        if self.planewaves:
+            op.set_planewaves(self.planewaves)
+        return op
 
     def time_step(self, op):
         """time_step, or the Hesthaven rule cfl / (max fscale * p^2) when it is 0

@@ -378,6 +378,10 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
     if (mesh->order < 1 || mesh->order > dgtd::kMaxBatchOrder)
         return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");
+    if (mesh->face_bc)
+        for (size_t i = 0; i < 4 * (size_t)std::max(0, mesh->K); ++i)
+            if (mesh->face_bc[i] == DGTD_BC_TFSF)
+                return dgtd::api_fail(DGTD_E_UNSUPPORTED, "TF/SF faces are not supported by batched instances");
     const int64_t bytes = footprint(mesh->K, mesh->order, b->precision);
     int slot = -1;
     {  // reserve a slot and the bytes

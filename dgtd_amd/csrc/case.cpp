@@ -28,6 +28,9 @@ struct dgtd_case {
     std::vector<dgtd_initial_field> sources;
     std::vector<double> probes;  // 3 per point probe
     std::vector<int32_t> probe_steps, probe_saves;
+    std::vector<dgtd_planewave> waves;
+    std::vector<int> tfsf_tags;
+    std::vector<bool> wave_auto_mean;  // mean from the TF/SF surface (no "mean" in the case)
 };
 
 namespace {
@@ -99,7 +102,37 @@ void read_sources(const dgtd::Json& c, dgtd_case& cs)
     for (size_t i = 0; i < ss.size(); ++i) {
         const dgtd::Json& s = ss.at(i);
         const std::string& type = s.at("type").string();
-        if (type != "initial") throw Unsupported("source type '" + type + "' (TF/SF planewave, dipole) is not supported");
+        if (type == "planewave") {  // driver.cpp:565-609
+            const dgtd::Json& mag = s.at("magnitude");
+            if (mag.contains("type") && mag.at("type").string() != "gaussian")
+                throw Unsupported("plane-wave magnitude '" + mag.at("type").string() + "' is not supported");
+            dgtd_planewave w{};
+            vec3(s.at("polarization"), w.polarization);
+            vec3(s.at("propagation"), w.propagation);
+            w.spread = mag.at("spread").number();
+            w.kind = DGTD_PLANEWAVE_GAUSSIAN;
+            if (mag.contains("frequency")) {
+                w.kind = DGTD_PLANEWAVE_MODULATED;
+                w.frequency = mag.at("frequency").number() / 299792458.0;  // f / c_SI (driver.cpp:515)
+            }
+            const double nd = std::sqrt(w.propagation[0] * w.propagation[0] + w.propagation[1] * w.propagation[1] +
+                                        w.propagation[2] * w.propagation[2]);
+            if (!(nd > 0)) throw std::runtime_error("plane-wave propagation must be nonzero");
+            const bool auto_mean = !mag.contains("mean");
+            if (!auto_mean) {  // projected mean k.mean (buildGaussianPlanewave, driver.cpp:491-496)
+                double m[3];
+                vec3(mag.at("mean"), m);
+                w.mean = (m[0] * w.propagation[0] + m[1] * w.propagation[1] + m[2] * w.propagation[2]) / nd;
+            }
+            if (s.contains("tags")) {
+                const dgtd::Json& tags = s.at("tags");
+                for (size_t t = 0; t < tags.size(); ++t) cs.tfsf_tags.push_back(tags.at(t).integer());
+            }
+            cs.waves.push_back(w);
+            cs.wave_auto_mean.push_back(auto_mean);
+            continue;
+        }
+        if (type != "initial") throw Unsupported("source type '" + type + "' (dipole) is not supported");
         const dgtd::Json& mag = s.at("magnitude");
         const std::string& mt = mag.at("type").string();
         dgtd_initial_field f{};
@@ -219,7 +252,36 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
             attr_bc[a] = code;
         }
     }
+    for (int a : cs.tfsf_tags) {
+        if (a < 0 || a > max_attr)
+            throw std::runtime_error("There is no bdr_attribute " + std::to_string(a) +
+                                     " defined in the mesh, but it is defined in the JSON.");
+        attr_bc[a] = DGTD_BC_TFSF;
+    }
     cs.face_bc = dgtd::face_codes(cs.mesh, attr_bc);
+    // automatic plane-wave delay: the pulse starts 5 spreads (x sqrt 2) before
+    // the upstream point of the TF/SF surface (driver.cpp:408, 455-472, 575-588)
+    for (size_t i = 0; i < cs.waves.size(); ++i) {
+        if (!cs.wave_auto_mean[i]) continue;
+        dgtd_planewave& w = cs.waves[i];
+        const double nd = std::sqrt(w.propagation[0] * w.propagation[0] + w.propagation[1] * w.propagation[1] +
+                                    w.propagation[2] * w.propagation[2]);
+        double min_phase = 0.0;
+        bool any = false;
+        for (size_t t = 0; t < cs.mesh.tris.size(); ++t) {
+            const int a = cs.mesh.tri_attr[t];
+            if (std::find(cs.tfsf_tags.begin(), cs.tfsf_tags.end(), a) == cs.tfsf_tags.end()) continue;
+            double ph = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double c = 0.0;
+                for (int k = 0; k < 3; ++k) c += cs.mesh.vx[3 * (size_t)cs.mesh.tris[t][k] + d];
+                ph += c / 3.0 * w.propagation[d] / nd;
+            }
+            min_phase = any ? std::min(min_phase, ph) : ph;
+            any = true;
+        }
+        w.mean = min_phase - 5.0 * w.spread * std::sqrt(2.0);
+    }
     cs.EToV.resize(4 * (size_t)K);
     for (int32_t e = 0; e < K; ++e)
         for (int j = 0; j < 4; ++j) cs.EToV[4 * (size_t)e + j] = cs.mesh.tets[e][j];
@@ -238,10 +300,11 @@ extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
         const dgtd::Json c = dgtd::Json::parse_file(json_path);
         if (!c.is_object()) throw std::runtime_error("case file is not a JSON object");
         read_solver_options(c, cs->info);
+        read_sources(c, *cs);  // plane-wave tags mark TF/SF faces in the model
         read_model(c, json_path, *cs);
-        read_sources(c, *cs);
         read_probes(c, *cs);
         cs->info.nsources = (int32_t)cs->sources.size();
+        cs->info.nplanewaves = (int32_t)cs->waves.size();
         cs->info.nprobes = (int32_t)(cs->probes.size() / 3);
         if (cs->info.alpha == 0.0)
             for (int8_t b : cs->face_bc)
@@ -314,6 +377,13 @@ extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, c
     if (!f) return dgtd::api_fail(DGTD_E_ARG, "cannot write " + base + "/time.txt");
     std::fprintf(f, "%.17g", t);
     std::fclose(f);
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_planewaves(const dgtd_case* c, dgtd_planewave* out)
+{
+    if (!c || (!out && !c->waves.empty())) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    for (size_t i = 0; i < c->waves.size(); ++i) out[i] = c->waves[i];
     return DGTD_OK;
 }
 

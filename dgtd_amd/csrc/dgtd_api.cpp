@@ -44,6 +44,10 @@ const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612
                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                       2277821191437.0 / 14882151754819.0};
 
+// LSERK4 stage times t + c_s dt (Carpenter & Kennedy 1994).
+const double kC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                      2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
+
 template <typename T>
 void* upload(const std::vector<T>& h)
 {
@@ -90,6 +94,15 @@ struct dgtd_op {
     double t = 0.0;
     int grid = 1, nblk = 1;
 
+    // TF/SF injection: elements owning TF/SF faces and the plane waves
+    int ntfsf = 0, nwaves = 0;
+    int* d_tfsf_elem = nullptr;
+    int* d_tfsf_sign = nullptr;
+    double* d_tfsf_xyz = nullptr;
+    double* d_tfsf_geo = nullptr;
+    double* d_lift = nullptr;
+    dgtd::TfsfWave* d_waves = nullptr;
+
     int nprobes = 0;
     int* d_probe_elem = nullptr;
     double* d_probe_w = nullptr;
@@ -108,6 +121,8 @@ struct dgtd_op {
         (void)hipSetDevice(device);
         for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
                         (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_probe_elem,
+                        (void*)d_tfsf_elem, (void*)d_tfsf_sign, (void*)d_tfsf_xyz, (void*)d_tfsf_geo,
+                        (void*)d_lift, (void*)d_waves,
                         (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
@@ -172,6 +187,35 @@ struct dgtd_op {
         variant = best_var;
     }
 
+    // Adds the TF/SF right-hand side at time ts: c_out * S to out and
+    // c_res * S to res (res may be null); no-op without TF/SF faces or waves.
+    template <typename T>
+    void tfsf(double ts, T* out, double c_out, T* res, double c_res, hipStream_t s)
+    {
+        if (ntfsf == 0 || nwaves == 0) return;
+        dgtd::TfsfArgs<T> a{};
+        a.nelem = ntfsf;
+        a.elem = d_tfsf_elem;
+        a.sign = d_tfsf_sign;
+        a.xyz = d_tfsf_xyz;
+        a.geo = d_tfsf_geo;
+        a.lift = d_lift;
+        a.waves = d_waves;
+        a.nwaves = nwaves;
+        a.Np = Np;
+        a.Nfp = Nfp;
+        a.N = N;
+        a.alpha = alpha;
+        a.t = ts;
+        a.out = out;
+        a.c_out = c_out;
+        a.res = res;
+        a.c_res = c_res;
+        hip_check(dgtd::launch_tfsf<T>(a, s), "TF/SF kernel launch");
+    }
+
+    // One step from time t (the stage kernels take the TF/SF right-hand side
+    // at the stage times: LSERK t + c_s dt, RK4 t, t + dt/2, t + dt/2, t + dt).
     template <typename T>
     int one_step(double dt)
     {
@@ -185,6 +229,8 @@ struct dgtd_op {
                 a.ca = (T)kA[s];
                 a.cb = (T)kB[s];
                 launch<T>(a, dgtd::kModeLserk);
+                // res' = res + dt S, q' = q + b res'  =>  q' += b dt S
+                tfsf<T>(t + kC[s] * dt, a.out, kB[s] * dt, a.res, dt, stream);
                 cur ^= 1;
             }
             return 5;
@@ -198,24 +244,29 @@ struct dgtd_op {
         a.x = x;
         a.qin = x; a.out = y1; a.ca = (T)(dt / 2); a.cb = (T)(dt / 6);
         launch<T>(a, dgtd::kModeRk1);
+        tfsf<T>(t, y1, dt / 2, z, dt / 6, stream);
         a.qin = y1; a.out = y2; a.ca = (T)(dt / 2); a.cb = (T)(dt / 3);
         launch<T>(a, dgtd::kModeRk2);
+        tfsf<T>(t + dt / 2, y2, dt / 2, z, dt / 3, stream);
         a.qin = y2; a.out = y1; a.ca = (T)dt; a.cb = (T)(dt / 3);
         launch<T>(a, dgtd::kModeRk3);
+        tfsf<T>(t + dt / 2, y1, dt, z, dt / 3, stream);
         a.qin = y1; a.out = x; a.ca = (T)(dt / 6);
         launch<T>(a, dgtd::kModeRk4);
+        tfsf<T>(t + dt, x, dt / 6, nullptr, 0.0, stream);
         return 4;
     }
 
     int step(double dt) { return precision == 64 ? one_step<double>(dt) : one_step<float>(dt); }
 
     template <typename T>
-    void mult_dev(const void* in, void* out, hipStream_t s)
+    void mult_dev(const void* in, void* out, hipStream_t s, double tm)
     {
         auto a = base_args<T>();
         a.qin = static_cast<const T*>(in);
         a.out = static_cast<T*>(out);
         hip_check(launch_on<T>(a, dgtd::kModeMult, variant, s), "stage kernel launch");
+        tfsf<T>(tm, static_cast<T*>(out), 1.0, nullptr, 0.0, s);
     }
 
     void to_device(const double* h, void* d)
@@ -394,6 +445,39 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
             hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
             hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
         }
+        if (!md.tfsf.empty()) {  // elements owning TF/SF faces, their face signs, nodes and records
+            std::vector<int> elems, signs;
+            std::vector<double> xyz, geo;
+            std::vector<int> slot(op->K, -1);
+            for (const auto& t : md.tfsf) {
+                const int e = t[0];
+                if (slot[e] < 0) {
+                    slot[e] = (int)elems.size();
+                    elems.push_back(e);
+                    signs.insert(signs.end(), {0, 0, 0, 0});
+                    geo.insert(geo.end(), md.geo.begin() + (size_t)dgtd::kGeoStride * e,
+                               md.geo.begin() + (size_t)dgtd::kGeoStride * (e + 1));
+                    const int32_t* v = &mesh->EToV[4 * (size_t)e];
+                    for (int m = 0; m < 4 * op->Nfp; ++m) {
+                        const int n = op->R.Fmask[m];
+                        const double a = op->R.xi[3 * n], b = op->R.xi[3 * n + 1], c = op->R.xi[3 * n + 2];
+                        for (int d = 0; d < 3; ++d) {
+                            const double x0 = mesh->vx[3 * (size_t)v[0] + d];
+                            xyz.push_back(x0 + (mesh->vx[3 * (size_t)v[1] + d] - x0) * a +
+                                          (mesh->vx[3 * (size_t)v[2] + d] - x0) * b +
+                                          (mesh->vx[3 * (size_t)v[3] + d] - x0) * c);
+                        }
+                    }
+                }
+                signs[4 * (size_t)slot[e] + t[1]] = t[2];
+            }
+            op->ntfsf = (int)elems.size();
+            op->d_tfsf_elem = static_cast<int*>(upload(elems));
+            op->d_tfsf_sign = static_cast<int*>(upload(signs));
+            op->d_tfsf_xyz = static_cast<double*>(upload(xyz));
+            op->d_tfsf_geo = static_cast<double*>(upload(geo));
+            op->d_lift = static_cast<double*>(upload(op->R.LIFT));
+        }
         hip_check(hipStreamSynchronize(op->stream), "sync");
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, op->device), "hipGetDeviceProperties");
@@ -466,14 +550,13 @@ int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
 
 int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, void* stream)
 {
-    (void)t;
     if (!op || !d_in || !d_out) return fail(DGTD_E_ARG, "NULL argument");
     if (d_in == d_out) return fail(DGTD_E_ARG, "in and out must not alias");
     try {
         hip_check(hipSetDevice(op->device), "hipSetDevice");
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : op->stream;
-        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, s);
-        else op->mult_dev<float>(d_in, d_out, s);
+        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, s, t);
+        else op->mult_dev<float>(d_in, d_out, s, t);
     } catch (const std::exception& e) {
         return fail(DGTD_E_HIP, e.what());
     }
@@ -482,7 +565,6 @@ int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, vo
 
 int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out)
 {
-    (void)t;
     if (!op || !in || !out) return fail(DGTD_E_ARG, "NULL argument");
     void *d_in = nullptr, *d_out = nullptr;
     try {
@@ -490,8 +572,8 @@ int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out)
         hip_check(hipMalloc(&d_in, op->real_size() * op->ndofs), "hipMalloc");
         hip_check(hipMalloc(&d_out, op->real_size() * op->ndofs), "hipMalloc");
         op->to_device(in, d_in);
-        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, op->stream);
-        else op->mult_dev<float>(d_in, d_out, op->stream);
+        if (op->precision == 64) op->mult_dev<double>(d_in, d_out, op->stream, t);
+        else op->mult_dev<float>(d_in, d_out, op->stream, t);
         op->to_host(d_out, out);
     } catch (const std::exception& e) {
         if (d_in) (void)hipFree(d_in);
@@ -632,6 +714,47 @@ int dgtd_op_norms(dgtd_op* op, double* out3)
     return DGTD_OK;
 }
 
+int dgtd_op_set_planewaves(dgtd_op* op, const dgtd_planewave* w, int32_t n)
+{
+    if (!op || n < 0 || (n > 0 && !w)) return fail(DGTD_E_ARG, "NULL argument or negative count");
+    std::vector<dgtd::TfsfWave> dev((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const dgtd_planewave& p = w[i];
+        if (p.kind != DGTD_PLANEWAVE_GAUSSIAN && p.kind != DGTD_PLANEWAVE_MODULATED)
+            return fail(DGTD_E_ARG, "unknown plane-wave kind");
+        const double np = std::sqrt(p.polarization[0] * p.polarization[0] + p.polarization[1] * p.polarization[1] +
+                                    p.polarization[2] * p.polarization[2]);
+        const double nd = std::sqrt(p.propagation[0] * p.propagation[0] + p.propagation[1] * p.propagation[1] +
+                                    p.propagation[2] * p.propagation[2]);
+        if (!(np > 0) || !(nd > 0) || !(p.spread > 0) || !std::isfinite(p.mean) || !std::isfinite(p.frequency))
+            return fail(DGTD_E_ARG, "plane wave needs nonzero polarization and propagation and spread > 0");
+        dgtd::TfsfWave& d = dev[i];
+        for (int c = 0; c < 3; ++c) {  // Planewave normalises both (Function.h:331-340)
+            d.pol[c] = p.polarization[c] / np;
+            d.dir[c] = p.propagation[c] / nd;
+        }
+        for (int c = 0; c < 3; ++c) {  // H = k x pol for an electric-field plane wave (:365-374)
+            const int y = (c + 1) % 3, z = (c + 2) % 3;
+            d.kxp[c] = d.dir[y] * d.pol[z] - d.dir[z] * d.pol[y];
+        }
+        d.spread = p.spread;
+        d.mean = p.mean;
+        d.freq = p.frequency;
+        d.kind = p.kind;
+    }
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        if (op->d_waves) hip_check(hipFree(op->d_waves), "hipFree");
+        op->d_waves = nullptr;
+        op->nwaves = n;
+        if (n > 0) op->d_waves = static_cast<dgtd::TfsfWave*>(upload(dev));
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
 int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem)
 {
     if (!op || n < 0 || (n > 0 && !xyz)) return fail(DGTD_E_ARG, "NULL argument or negative count");
@@ -761,7 +884,7 @@ int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, in
             for (int f = 0; f < 4; ++f) {
                 const int nb = md.conn[(size_t)dgtd::kConnStride * e + f];
                 const int code = md.conn[(size_t)dgtd::kConnStride * e + 4 + f];
-                if (face_codes) face_codes[4 * (size_t)e + f] = (int8_t)(nb >= 0 ? 0 : -nb - 1);
+                if (face_codes) face_codes[4 * (size_t)e + f] = md.face_code[4 * (size_t)e + f];
                 for (int b = 0; b < Nfp; ++b) {
                     const size_t n = ((size_t)e * 4 + f) * Nfp + b;
                     vmapM[n] = e * Np + R.Fmask[f * Nfp + b];

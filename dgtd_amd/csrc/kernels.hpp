@@ -115,6 +115,37 @@ int batch_blocks_per_cu(size_t smem);
 // FP64 reductions over a state in T: per-block partials then a fixed-order
 // final pass (deterministic, no atomics).
 //   out[0] = sum of squares, out[1] = energy, out[2] = non-finite count.
+// TF/SF plane-wave injection (tfsf.hip): plane wave with normalised
+// polarisation and propagation, k x pol precomputed, Gaussian (kind 0) or
+// modulated Gaussian (kind 1) magnitude of u = k.x - t.
+struct TfsfWave {
+    double pol[3], kxp[3], dir[3];
+    double spread, mean, freq;
+    int kind;
+};
+
+template <typename T>
+struct TfsfArgs {
+    int nelem;           // elements owning TF/SF faces
+    const int* elem;     // [nelem] element ids
+    const int* sign;     // [4*nelem] per face: -1 scattered side, +1 total side, 0
+    const double* xyz;   // [nelem][4*Nfp][3] face node positions
+    const double* geo;   // [nelem][27] FP64 geometry records
+    const double* lift;  // [Np][4*Nfp]
+    const TfsfWave* waves;
+    int nwaves;
+    int Np, Nfp;
+    size_t N;
+    double alpha, t;
+    T* out;
+    double c_out;
+    T* res;  // may be null
+    double c_res;
+};
+
+template <typename T>
+hipError_t launch_tfsf(const TfsfArgs<T>& a, hipStream_t s);
+
 // Elements per tile of the norms reduction: mass (Np^2), tile (6*TE*Np) and
 // reduction scratch (3*256) doubles stay within 60 KiB of LDS.
 inline int norms_tile(int Np)

@@ -105,6 +105,11 @@ int main(int argc, char** argv)
     std::vector<double> x((size_t)ndofs, 0.0);
     check(dgtd_project_initial_fields(&mesh, src.data(), info.nsources, x.data()), "initial fields");
     check(dgtd_op_set_state(op, x.data(), 0.0), "set state");
+    if (info.nplanewaves > 0) {  // TF/SF plane waves
+        std::vector<dgtd_planewave> waves(info.nplanewaves);
+        check(dgtd_case_planewaves(cs, waves.data()), "plane waves");
+        check(dgtd_op_set_planewaves(op, waves.data(), info.nplanewaves), "set plane waves");
+    }
 
     // dt: time_step, or the Hesthaven rule (Solver.cpp:110-115, 335-345)
     double dt = info.time_step;

@@ -35,6 +35,7 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
     m.K = K;
     m.geo.assign((size_t)kGeoStride * K, 0.0);
     m.conn.assign((size_t)kConnStride * K, 0);
+    m.face_code.assign((size_t)4 * K, 0);
     m.jac.resize(K);
     m.eps.resize(K);
     m.mu.resize(K);
@@ -123,11 +124,12 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
         while (j < keys.size() && same(keys[i], keys[j])) ++j;
         if (j - i > 2) throw std::runtime_error("non-manifold mesh: a face is shared by more than two elements");
         if (j - i == 2) {
-            // interior face: 0, or an interior condition 4-6 given on either side
+            // interior face: 0, an interior condition 4-6 or the TF/SF code 7,
+            // given on either side
             int ic = 0;
             if (in.face_bc) {
                 const int c0 = in.face_bc[keys[i].ef], c1 = in.face_bc[keys[i + 1].ef];
-                if (c0 < 0 || c0 > 6 || c1 < 0 || c1 > 6) throw std::runtime_error("face_bc code must be 0..6");
+                if (c0 < 0 || c0 > 7 || c1 < 0 || c1 > 7) throw std::runtime_error("face_bc code must be 0..7");
                 if (c0 != 0 && c1 != 0 && c0 != c1)
                     throw std::runtime_error("the two sides of an interior face carry different codes");
                 ic = c0 != 0 ? c0 : c1;
@@ -137,7 +139,12 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
             for (int side = 0; side < 2; ++side) {
                 const int ef = keys[i + side].ef, og = keys[i + 1 - side].ef;
                 const int e = ef / 4, f = ef % 4, e2 = og / 4, g = og % 4;
-                if (ic != 0) {  // the jump uses the own trace only: no neighbour gather
+                m.face_code[ef] = (int8_t)ic;
+                if (ic == 7) {  // TF/SF: the lower-index element is the scattered-field side
+                    m.tfsf.push_back({e, f, e < e2 ? -1 : 1});
+                    m.n_boundary_faces[7]++;
+                }
+                if (ic >= 4 && ic <= 6) {  // the jump uses the own trace only: no neighbour gather
                     m.conn[(size_t)kConnStride * e + f] = -(ic + 1);
                     m.conn[(size_t)kConnStride * e + 4 + f] = 0;
                     m.n_boundary_faces[ic]++;
@@ -158,8 +165,9 @@ MeshData build_mesh(const MeshInput& in, const RefTet& R, double alpha)
         } else {
             const int ef = keys[i].ef, e = ef / 4, f = ef % 4;
             const int bc = in.face_bc ? in.face_bc[ef] : 0;
-            if (bc < 0 || bc > 6) throw std::runtime_error("face_bc code must be 0..6");
-            if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-6 on a boundary face");
+            if (bc < 0 || bc > 7) throw std::runtime_error("face_bc code must be 0..7");
+            if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-7 on a boundary face");
+            m.face_code[ef] = (int8_t)bc;
             if (bc == 3 && alpha == 0.0)
                 throw std::runtime_error("SMA boundaries require upwind flux (alpha != 0)");
             m.conn[(size_t)kConnStride * e + f] = -(bc + 1);

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <array>
 #include <vector>
 
 #include "reftet.hpp"
@@ -36,7 +37,9 @@ struct MeshData {
     std::vector<double> jac;     // K: V_e / V_ref, for mass-weighted reductions
     std::vector<double> eps, mu; // K
     double max_fscale = 0.0;
-    int n_boundary_faces[7] = {0, 0, 0, 0, 0, 0, 0};  // per face code (interior conditions count each side)
+    int n_boundary_faces[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per face code (interior codes count each side)
+    std::vector<int8_t> face_code;              // 4K: the code of each face (0 plain interior)
+    std::vector<std::array<int, 3>> tfsf;       // TF/SF face sides: (element, face, -1 scattered / +1 total)
 };
 
 // Throws std::runtime_error (message is user-facing) on invalid input.

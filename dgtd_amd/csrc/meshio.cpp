@@ -322,7 +322,12 @@ std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to
         auto it = faces.find(key_of(t[0], t[1], t[2]));
         if (it == faces.end())
             throw std::runtime_error("boundary triangle with attribute " + std::to_string(attr) + " is no element face");
-        if (it->second.second >= 0) {  // interior face: the interior condition on both sides
+        if (code == 7) {  // TF/SF surface: interior faces only
+            if (it->second.second < 0)
+                throw std::runtime_error("TF/SF attribute " + std::to_string(attr) + " lies on a boundary face");
+            bc[it->second.first] = 7;
+            bc[it->second.second] = 7;
+        } else if (it->second.second >= 0) {  // interior face: the interior condition on both sides
             bc[it->second.first] = (int8_t)(code + 3);
             bc[it->second.second] = (int8_t)(code + 3);
         } else {

@@ -57,10 +57,11 @@ void orient_positive(TetMesh& m);
 TetMesh refine_uniform(const TetMesh& m);
 
 // face_bc[4e+f] from the triangles' attributes via attr_to_bc (index =
-// attribute, value DGTD_BC_PEC/PMC/SMA or -1 for "no condition"); a
-// triangle on an interior face gives the interior code (value + 3) to both
-// sides, as the reference marks such tags interior (driver.cpp:1024-1035).
-// Throws if a triangle matches no element face.
+// attribute, value DGTD_BC_PEC/PMC/SMA, DGTD_BC_TFSF or -1 for "no
+// condition"); a PEC/PMC/SMA triangle on an interior face gives the interior
+// code (value + 3) to both sides, as the reference marks such tags interior
+// (driver.cpp:1024-1035); TF/SF triangles must be interior.  Throws if a
+// triangle matches no element face.
 std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc);
 
 }  // namespace dgtd

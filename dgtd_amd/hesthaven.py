@@ -76,6 +76,18 @@ def resonant(modes=(1, 1), field="E", polarization=(0, 0, 1)):
     return f
 
 
+def planewave(polarization, propagation, spread, mean=0.0, frequency=None):
+    """Gaussian (or, with a frequency, modulated Gaussian) plane wave for the
+    TF/SF faces; mean is the projected 1D mean k.mean (driver.cpp:483-520)."""
+    w = _abi.PlaneWave()
+    w.kind = 0 if frequency is None else 1
+    w.polarization[:] = list(polarization)
+    w.propagation[:] = list(propagation)
+    w.spread, w.mean = float(spread), float(mean)
+    w.frequency = 0.0 if frequency is None else float(frequency)
+    return w
+
+
 def project_initial_fields(vx, EToV, order, sources):
     """Sum of the sources evaluated at the nodes: a 6*Np*K state vector."""
     m = _Mesh(vx, EToV, None, order)
@@ -239,6 +251,14 @@ class HesthavenEvolution:
         check(lib().dgtd_op_time(self._h, ctypes.byref(t)))
         self._t = t.value
         return n.value
+
+    # --- TF/SF plane waves (HesthavenEvolution::evaluateTFSF, HesthavenEvolution.cpp:97-124) ---
+    def set_planewaves(self, waves):
+        """waves: iterable of planewave(...) structures (an empty list switches
+        the injection off)."""
+        waves = list(waves)
+        arr = (_abi.PlaneWave * max(1, len(waves)))(*waves)
+        check(lib().dgtd_op_set_planewaves(self._h, arr, len(waves)))
 
     # --- point probes (ProbesManager::updateProbe, ProbesManager.cpp:375-406) ---
     def probe_set(self, points):

@@ -54,6 +54,12 @@ extern "C" {
 #define DGTD_BC_INT_PEC 4
 #define DGTD_BC_INT_PMC 5
 #define DGTD_BC_INT_SMA 6
+/* Total-field / scattered-field surface (interior faces, either side): the
+ * plane waves of dgtd_op_set_planewaves are injected there
+ * (HesthavenEvolution::evaluateTFSF, HesthavenEvolution.cpp:97-124).  The
+ * face's lower-index element is the scattered-field side, the face's first
+ * element as MFEM orders them (HesthavenEvolutionMethods.cpp:562-574). */
+#define DGTD_BC_TFSF 7
 
 /* Integrators. */
 #define DGTD_RK4 0    /* mfem::RK4Solver stage order (the reference's default) */
@@ -70,7 +76,7 @@ typedef struct {
     const double* vx;       /* 3*nv vertex coordinates (x,y,z interleaved) */
     int32_t nv;             /* number of vertices */
     const int32_t* EToV;    /* 4*K vertex ids; local vertex order fixes node/face order */
-    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (1-3 boundary, 4-6 interior faces), NULL = none */
+    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (1-3 boundary, 4-7 interior faces), NULL = none */
     const double* eps_r;    /* K relative permittivity, NULL = 1 */
     const double* mu_r;     /* K relative permeability, NULL = 1 */
 } dgtd_mesh_desc;
@@ -146,6 +152,28 @@ int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps);
  * out[2] = 1 if any entry is non-finite, else 0. */
 int dgtd_op_norms(dgtd_op* op, double* out3);
 
+/* Plane waves for the TF/SF faces (Planewave with a Gaussian or modulated
+ * Gaussian magnitude, Function.h:329-418 and :45-140; built by
+ * buildGaussianPlanewave / buildModulatedGaussianPlanewave, driver.cpp:483-520):
+ *   u = k.x - t (c = 1),  g = exp(-(u - mean)^2 / (2 spread^2))
+ *   [* cos(2 pi frequency (u - mean)) for DGTD_PLANEWAVE_MODULATED],
+ *   E_inc = g pol,  H_inc = g (k x pol),  pol and k normalised here.
+ * mean is the projected 1D mean (k.mean of the case's 3D mean); frequency is
+ * normalised (f / c).  The incident field is evaluated at each stage time
+ * (LSERK t + c_s dt; RK4 t, t + dt/2, t + dt/2, t + dt; Mult: its t).
+ * Replaces the previous set; n = 0 switches the injection off. */
+#define DGTD_PLANEWAVE_GAUSSIAN 0
+#define DGTD_PLANEWAVE_MODULATED 1
+typedef struct {
+    int32_t kind;
+    double polarization[3];
+    double propagation[3];
+    double spread;
+    double mean;
+    double frequency;
+} dgtd_planewave;
+int dgtd_op_set_planewaves(dgtd_op* op, const dgtd_planewave* waves, int32_t n);
+
 /* Point probes (PointProbe, src/components/Probes.h:132-159, evaluated by
  * ProbesManager::updateProbe, src/solver/ProbesManager.cpp:375-406): the
  * six field components of the nodal solution at n points.  probe_set
@@ -212,9 +240,10 @@ int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_f
  *          materials (relative_permittivity / _permeability per element tag),
  *          boundaries (PEC / PMC / SMA per tag; a tag on interior faces
  *          becomes the interior condition DGTD_BC_INT_*)
- *   sources: "initial" gaussian / resonant fields
- *   probes: point positions and exporter steps
- * TF/SF sources, SGBC, conductivity and non-RK4 integrators are refused
+ *   sources: "initial" gaussian / resonant fields; "planewave" TF/SF
+ *            sources (Gaussian or modulated Gaussian, tags = TF/SF surface)
+ *   probes: point positions, exporter steps, domain snapshots
+ * Dipole sources, SGBC, conductivity and non-RK4 integrators are refused
  * with DGTD_E_UNSUPPORTED.
  * ------------------------------------------------------------------------ */
 typedef struct dgtd_case dgtd_case;
@@ -232,6 +261,7 @@ typedef struct {
     int32_t refinement;           /* uniform refinement levels applied */
     int32_t snapshot_steps;       /* probes.domain_snapshot: "steps" (0 if absent), -1 = no snapshots */
     int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
+    int32_t nplanewaves;          /* "planewave" sources (their tags are TF/SF faces) */
 } dgtd_case_info;
 
 /* Parse the case file, read and refine its mesh, map tags to boundary codes
@@ -241,6 +271,10 @@ void dgtd_case_destroy(dgtd_case* c);
 int dgtd_case_get_info(const dgtd_case* c, dgtd_case_info* info);
 /* Mesh description for dgtd_op_create; its arrays belong to the case. */
 int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh);
+/* info.nplanewaves plane waves for dgtd_op_set_planewaves (driver.cpp:565-609;
+ * without a "mean" the pulse starts 5*sqrt(2) spreads before the upstream
+ * point of the TF/SF surface, driver.cpp:575-588). */
+int dgtd_case_planewaves(const dgtd_case* c, dgtd_planewave* out);
 /* info.nsources initial fields (buildSources, driver.cpp:531-564). */
 int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out);
 /* Domain snapshots (DomainSnapshotProbe, ProbesManager.cpp:457-495): the

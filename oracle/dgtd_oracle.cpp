@@ -397,7 +397,26 @@ struct oracle_op {
     std::vector<int> vmapM, vmapP;                /* 4*Nfp*K */
     std::vector<int> bc;                          /* 4K: BC code of each face (0 if interior) */
     std::vector<double> pos;                      /* 3*Np*K */
+    std::vector<int> tfsf;                        /* 4K: -1 scattered-field side, +1 total-field side, 0 */
+    std::vector<oracle_planewave> waves;          /* plane waves injected on the TF/SF faces */
 };
+
+/* Planewave::eval (Function.h:361-406) with a Gaussian or modulated Gaussian
+ * magnitude: u = k.x / c - t, g = exp(-(u - mean)^2 / (2 spread^2))
+ * [* cos(2 pi freq (u - mean))]; E = g pol, H = g (k x pol), c = 1. */
+static void planewave_eval(const oracle_planewave& w, const double* x, double t, double E[3], double H[3])
+{
+    const double u = w.dir[0] * x[0] + w.dir[1] * x[1] + w.dir[2] * x[2] - t;
+    const double a = u - w.mean;
+    double g = std::exp(-a * a / (2.0 * w.spread * w.spread));
+    if (w.kind == 1) g *= std::cos(2.0 * M_PI * w.freq * a);
+    const double kxp[3] = {w.dir[1] * w.pol[2] - w.dir[2] * w.pol[1], w.dir[2] * w.pol[0] - w.dir[0] * w.pol[2],
+                           w.dir[0] * w.pol[1] - w.dir[1] * w.pol[0]};
+    for (int d = 0; d < 3; ++d) {
+        E[d] = g * w.pol[d];
+        H[d] = g * kxp[d];
+    }
+}
 
 extern "C" {
 
@@ -553,6 +572,7 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
         op->vmapM.resize(4 * (size_t)Nfp * K);
         op->vmapP.resize(4 * (size_t)Nfp * K);
         op->bc.assign(4 * (size_t)K, 0);
+        op->tfsf.assign(4 * (size_t)K, 0);
         double hmin = 1e300;
         for (int e = 0; e < K; ++e) hmin = std::min(hmin, std::cbrt(op->vol[e]));
         const double ptol = 1e-8 * hmin;
@@ -569,17 +589,24 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
                  * interior faces, given on either side (the condition belongs
                  * to the face, :294-311 applies it to both elements) */
                 int code = face_bc[4 * e + f];
-                if (code < 0 || code > 6) throw std::runtime_error("bad face_bc code");
+                if (code < 0 || code > 7) throw std::runtime_error("bad face_bc code");
                 if (e2 >= 0) {
                     const int other = face_bc[4 * e2 + f2];
-                    if (other < 0 || other > 6) throw std::runtime_error("bad face_bc code");
+                    if (other < 0 || other > 7) throw std::runtime_error("bad face_bc code");
                     if (code != 0 && other != 0 && code != other)
                         throw std::runtime_error("the two sides of an interior face carry different codes");
                     if (code == 0) code = other;
                     if (code >= 1 && code <= 3)
                         throw std::runtime_error("PEC/PMC/SMA codes 1-3 on an interior face (use 4-6)");
                 } else if (code >= 4) {
-                    throw std::runtime_error("interior-boundary codes 4-6 on a boundary face");
+                    throw std::runtime_error("interior-boundary codes 4-7 on a boundary face");
+                }
+                if (code == 7) {
+                    /* TF/SF face: the scattered-field side is the face's first
+                     * element (the lower index), as MFEM orders a face's two
+                     * elements (HesthavenEvolutionMethods.cpp:562-574) */
+                    op->tfsf[4 * (size_t)e + f] = e < e2 ? -1 : 1;
+                    code = 0;
                 }
                 if (code == 3 && alpha == 0.0)
                     throw std::runtime_error("SMA boundaries need upwind flux (alpha != 0)");
@@ -656,8 +683,26 @@ void oracle_geometry(const oracle_op* op, double* vol, double* fscale, double* n
     if (normals) std::copy(op->nrm.begin(), op->nrm.end(), normals);
 }
 
-/* HesthavenEvolution::Mult (HesthavenEvolution.cpp:450-542). */
-void oracle_mult(const oracle_op* op, const double* in, double* out)
+int oracle_set_planewaves(oracle_op* op, const oracle_planewave* w, int n)
+{
+    if (!op || n < 0 || (n > 0 && !w)) return -1;
+    op->waves.assign(w, w + n);
+    for (auto& v : op->waves) {  /* Planewave normalises both directions (Function.h:331-340) */
+        const double np = std::sqrt(v.pol[0] * v.pol[0] + v.pol[1] * v.pol[1] + v.pol[2] * v.pol[2]);
+        const double nd = std::sqrt(v.dir[0] * v.dir[0] + v.dir[1] * v.dir[1] + v.dir[2] * v.dir[2]);
+        if (!(np > 0) || !(nd > 0) || !(v.spread > 0)) return -1;
+        for (int d = 0; d < 3; ++d) {
+            v.pol[d] /= np;
+            v.dir[d] /= nd;
+        }
+    }
+    return 0;
+}
+
+void oracle_mult(const oracle_op* op, const double* in, double* out) { oracle_mult_t(op, in, out, 0.0); }
+
+/* HesthavenEvolution::Mult (HesthavenEvolution.cpp:450-542) at time t. */
+void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t)
 {
     const int K = op->K, Np = op->Np, Nfp = op->Nfp, Nf = 4 * Nfp;
     const size_t N = (size_t)Np * K;
@@ -694,6 +739,22 @@ void oracle_mult(const oracle_op* op, const double* in, double* out)
                         const double ch = code == 5 ? -1.0 : (code == 6 ? -0.5 : 0.0);
                         for (int c = 0; c < 3; ++c) dF[c][f * Nfp + b] = ce * in[c * N + M];
                         for (int c = 3; c < 6; ++c) dF[c][f * Nfp + b] = ch * in[c * N + M];
+                    }
+                    /* TF/SF (evaluateTFSF, :97-124): the incident field at the
+                     * face node, subtracted on the scattered-field side and
+                     * added on the total-field side */
+                    const int sg = op->tfsf[4 * (size_t)e + f];
+                    if (sg != 0) {
+                        double xM[3];
+                        for (int d = 0; d < 3; ++d) xM[d] = op->pos[d * N + M];
+                        for (const auto& w : op->waves) {
+                            double Ei[3], Hi[3];
+                            planewave_eval(w, xM, t, Ei, Hi);
+                            for (int d = 0; d < 3; ++d) {
+                                dF[d][f * Nfp + b] += sg * Ei[d];
+                                dF[3 + d][f * Nfp + b] += sg * Hi[d];
+                            }
+                        }
                     }
                 }
             }
@@ -768,27 +829,26 @@ void oracle_mult(const oracle_op* op, const double* in, double* out)
  *   k4 = f(y); x = z + dt/6 k4 */
 void oracle_rk4_step(const oracle_op* op, double* x, double t, double dt)
 {
-    (void)t;
+    /* stage times t, t + dt/2, t + dt/2, t + dt (the SetTime calls of RK4Solver) */
     const size_t n = (size_t)oracle_ndofs(op);
     std::vector<double> k(n), y(n), z(n);
-    oracle_mult(op, x, k.data());
+    oracle_mult_t(op, x, k.data(), t);
     for (size_t i = 0; i < n; ++i) { y[i] = x[i] + dt / 2 * k[i]; z[i] = x[i] + dt / 6 * k[i]; }
-    oracle_mult(op, y.data(), k.data());
+    oracle_mult_t(op, y.data(), k.data(), t + dt / 2);
     for (size_t i = 0; i < n; ++i) { y[i] = x[i] + dt / 2 * k[i]; z[i] += dt / 3 * k[i]; }
-    oracle_mult(op, y.data(), k.data());
+    oracle_mult_t(op, y.data(), k.data(), t + dt / 2);
     for (size_t i = 0; i < n; ++i) { y[i] = x[i] + dt * k[i]; z[i] += dt / 3 * k[i]; }
-    oracle_mult(op, y.data(), k.data());
+    oracle_mult_t(op, y.data(), k.data(), t + dt);
     for (size_t i = 0; i < n; ++i) x[i] = z[i] + dt / 6 * k[i];
 }
 
 /* LSERK4: res = a_s res + dt L(q); q = q + b_s res, s = 1..5. */
 void oracle_lserk4_step(const oracle_op* op, double* q, double* res, double t, double dt)
 {
-    (void)t;
     const size_t n = (size_t)oracle_ndofs(op);
     std::vector<double> k(n);
     for (int s = 0; s < 5; ++s) {
-        oracle_mult(op, q, k.data());
+        oracle_mult_t(op, q, k.data(), t + rk4c[s] * dt);
         for (size_t i = 0; i < n; ++i) {
             res[i] = rk4a[s] * res[i] + dt * k[i];
             q[i] = q[i] + rk4b[s] * res[i];

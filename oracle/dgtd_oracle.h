@@ -48,15 +48,28 @@ void oracle_gll01(int p, double* x);
  *   vx      : 3*nv vertex coordinates
  *   EToV    : 4*K vertex ids; local vertex order fixes node and face order
  *   face_bc : 4*K, face f is opposite local vertex f.  On boundary faces
- *             0 (zero jump), 1 PEC, 2 PMC, 3 SMA; on interior faces 0 or
- *             the interior conditions 4 PEC, 5 PMC, 6 SMA (either side).
+ *             0 (zero jump), 1 PEC, 2 PMC, 3 SMA; on interior faces 0, the
+ *             interior conditions 4 PEC, 5 PMC, 6 SMA, or 7 TF/SF surface
+ *             (either side; the lower-index element is the scattered-field side).
  *   eps,mu  : K relative permittivity/permeability, NULL means 1
  * Returns NULL on error (message via oracle_last_error). */
+/* Plane wave for TF/SF injection (Planewave, Function.h:329-418): kind 0
+ * Gaussian, 1 modulated Gaussian; pol and dir are normalised by
+ * oracle_set_planewaves; mean is the projected 1D mean k.mean. */
+typedef struct {
+    int kind;
+    double pol[3], dir[3];
+    double spread, mean, freq;
+} oracle_planewave;
+
 oracle_op* oracle_create(int K, int order, const double* vx, int nv,
                          const int* EToV, const int* face_bc,
                          const double* eps, const double* mu, double alpha);
 void oracle_destroy(oracle_op* op);
 const char* oracle_last_error(void);
+
+/* Plane waves injected on the TF/SF faces (replaces the previous set). */
+int oracle_set_planewaves(oracle_op* op, const oracle_planewave* w, int n);
 
 int64_t oracle_ndofs(const oracle_op* op); /* 6*Np*K */
 double oracle_max_fscale(const oracle_op* op);
@@ -77,6 +90,8 @@ void oracle_geometry(const oracle_op* op, double* vol, double* fscale,
 
 /* out = L(in); the reference's HesthavenEvolution::Mult. */
 void oracle_mult(const oracle_op* op, const double* in, double* out);
+/* The same at time t (the TF/SF incident field depends on it). */
+void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t);
 
 /* One classic RK4 step in mfem::RK4Solver's stage order. */
 void oracle_rk4_step(const oracle_op* op, double* x, double t, double dt);

@@ -56,6 +56,9 @@ def lib():
         L.oracle_ref_element.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         L.oracle_gll01.argtypes = [ctypes.c_int, _dp]
         L.oracle_threads.restype = ctypes.c_int
+        L.oracle_mult_t.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double]
+        L.oracle_set_planewaves.restype = ctypes.c_int
+        L.oracle_set_planewaves.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = None
         _lib = L
@@ -93,6 +96,11 @@ def ref_element(order):
     lib().oracle_ref_element(order, _d(out["rst"]), _d(out["M"]), _d(out["Dr"]), _d(out["Ds"]),
                              _d(out["Dt"]), _d(out["Emat"]), _d(out["LIFT"]), _i(out["Fmask"]))
     return out
+
+
+class PlaneWave(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("pol", ctypes.c_double * 3), ("dir", ctypes.c_double * 3),
+                ("spread", ctypes.c_double), ("mean", ctypes.c_double), ("freq", ctypes.c_double)]
 
 
 class Operator:
@@ -148,11 +156,24 @@ class Operator:
     def max_fscale(self):
         return lib().oracle_max_fscale(self.h)
 
-    def mult(self, x):
+    def mult(self, x, t=0.0):
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.empty(self.ndofs)
-        lib().oracle_mult(self.h, _d(x), _d(out))
+        lib().oracle_mult_t(self.h, _d(x), _d(out), float(t))
         return out
+
+    def set_planewaves(self, waves):
+        """waves: dicts with kind (0 Gaussian, 1 modulated), pol, dir, spread, mean, freq."""
+        arr = (PlaneWave * max(1, len(waves)))()
+        for i, w in enumerate(waves):
+            arr[i].kind = int(w.get("kind", 0))
+            arr[i].pol[:] = list(w["pol"])
+            arr[i].dir[:] = list(w["dir"])
+            arr[i].spread = float(w["spread"])
+            arr[i].mean = float(w.get("mean", 0.0))
+            arr[i].freq = float(w.get("freq", 0.0))
+        if lib().oracle_set_planewaves(self.h, arr, len(waves)) != 0:
+            raise ValueError("bad plane wave")
 
     def rk4_step(self, x, t, dt):
         lib().oracle_rk4_step(self.h, _d(x), t, dt)

@@ -144,7 +144,10 @@ def test_materials_by_tag(tmp_path):
 
 
 @pytest.mark.parametrize("mutate,code,msg", [
-    (lambda c: c["sources"].append({"type": "planewave"}), -4, "planewave"),
+    (lambda c: c["sources"].append({"type": "dipole"}), -4, "dipole"),
+    (lambda c: c["sources"].append({"type": "planewave", "polarization": [0, 0, 1], "propagation": [1, 0, 0],
+                                    "tags": [1], "magnitude": {"type": "gaussian", "spread": 0.1}}),
+     -1, "TF/SF attribute 1 lies on a boundary face"),
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "XYZ"}), -1, "incorrect"),
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [99], "type": "PEC"}), -1, "bdr_attribute"),
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "SGBC"}), -4, "SGBC"),
@@ -273,3 +276,33 @@ def test_snapshot_mesh_and_fields_round_trip(tmp_path):
         assert header == {"FiniteElementCollection": "L2_T1_3D_P2", "VDim": "1", "Ordering": "0"}
         np.testing.assert_allclose(vals, x[c * n:(c + 1) * n], rtol=1e-15, atol=0)  # 16 significant digits
     assert float((tmp_path / "time.txt").read_text()) == 0.125
+
+
+def test_tfsf_case_plane_wave_and_sides():
+    """3D_TFSF (testData/maxwellInputs): a Gaussian plane wave (E_z, +x) through
+    the TF/SF plane x = 1 (tag 2), refined once."""
+    c = D.Case(os.path.join(CASES, "3D_TFSF.json"))
+    ref = json.load(open(os.path.join(CASES, "3D_TFSF.json")))
+    (w,) = c.planewaves
+    src = ref["sources"][0]
+    assert w.kind == 0 and list(w.polarization) == src["polarization"]
+    assert list(w.propagation) == src["propagation"]
+    assert w.spread == src["magnitude"]["spread"]
+    assert w.mean == pytest.approx(np.dot(src["magnitude"]["mean"], src["propagation"]))
+    tf = c.face_bc == 7
+    pts = c.vx[c.EToV[np.where(tf)[0]]]
+    # every TF/SF face lies in the plane x = 1 (all three face vertices)
+    faces = [c.EToV[e, list(MG.FACE_VERTS[f])] for e, f in zip(*np.where(tf))]
+    assert np.allclose(c.vx[np.array(faces)][..., 0], 1.0)
+    assert tf.sum() == 2 * 4 * 14  # 14 triangles, refined into 4 each, two sides
+    assert pts.shape[0] == tf.sum()
+
+
+def test_planewave_auto_mean(tmp_path):
+    shutil.copy(os.path.join(CASES, "3D_TFSF.msh"), tmp_path / "3D_TFSF.msh")
+    case = json.load(open(os.path.join(CASES, "3D_TFSF.json")))
+    del case["sources"][0]["magnitude"]["mean"]
+    json.dump(case, open(tmp_path / "c.json", "w"))
+    (w,) = D.Case(tmp_path / "c.json").planewaves
+    # the TF/SF plane is x = 1: min phase 1, minus 5 sqrt(2) spreads (driver.cpp:575-588)
+    assert w.mean == pytest.approx(1.0 - 5.0 * 0.6 * np.sqrt(2.0))

@@ -71,7 +71,7 @@ def test_one_sided_code_applies_to_the_face():
 
 
 @pytest.mark.parametrize("bad,msg", [("pec_interior", "1-3 on an interior face"),
-                                     ("int_on_boundary", "4-6 on a boundary face"),
+                                     ("int_on_boundary", "on a boundary face"),
                                      ("mismatch", "different codes")])
 def test_inconsistent_codes_are_refused(bad, msg):
     vx, E, fbc = pec_box(1)
