@@ -105,3 +105,23 @@ def test_launcher_domain_snapshots(tmp_path):
             assert lines[1] == "FiniteElementCollection: L2_T1_3D_P2"
             vals = np.array([float(v) for v in lines[5:] if v.strip()])
             np.testing.assert_allclose(vals, want[k * n:(k + 1) * n], rtol=1e-14, atol=1e-15)
+
+
+def test_launcher_tfsf_case(tmp_path):
+    """3D_TFSF (plane wave through the TF/SF plane x = 1 of a PEC/PMC
+    waveguide, refined once): the launcher's run equals the library run, and
+    the probe at x = 0 on the scattered-field side stays at rest while the
+    pulse (peak at x = t - 1) crosses the total-field side."""
+    case = os.path.join(ROOT, "tests", "golden", "cases", "3D_TFSF.json")
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, case, "--final-time", "2.5", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    x, info = D.run_case(case, precision=64, integrator="rk4", final_time=2.5)
+    assert summary["steps"] == info["steps"] == 2500
+    assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-10)
+    assert np.linalg.norm(x) > 1.0  # the wave is in the total-field region
+    rows = np.array([[float(v) for v in ln.split()] for ln in
+                     (out / "PointProbes" / "PointProbe0.dat").read_text().splitlines()[4:]])
+    assert rows.shape[0] == 2501 and np.abs(rows[:, 1:]).max() < 1e-3
