@@ -71,7 +71,7 @@ struct MShape {
     static constexpr int S = KP / 4;             // volume k-steps
     static constexpr int NPG = (NP + 15) / 16 * 16;
     static constexpr int G = NPG / 16;           // output-node groups
-    static constexpr int US = (KP % 2 == 0) ? KP + 1 : KP;  // LDS row stride of a task's fields
+    static constexpr int US = KP + 1;            // LDS row stride of a task's fields (odd: KP % 4 == 0)
     static constexpr int NFPP = (NFP + 3) / 4 * 4;  // face nodes padded to whole k-steps
     static constexpr int SF = NFPP / 4;             // LIFT k-steps per face
 };
@@ -180,13 +180,19 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     const int e0 = task * TE;
     const int ne = min(TE, K - e0);
 
+    // Every global load and store of a task is unconditional (out-of-range
+    // lanes use a clamped, in-bounds index): with no branch around memory
+    // instructions the compiler counts outstanding operations exactly, so a
+    // wait for one load does not also wait for every store and prefetch
+    // issued before it.
+
     // ---- stage the task's fields in LDS: all loads in flight (fields and the
     // lane's element record), then the LDS writes ----
     wave_sync();  // previous task's reads of U are done
     const bool valid = el < ne;
     const int e = e0 + (valid ? el : 0);
     const T* ge = a.geo + (size_t)e * kGeoStrideDev;
-    T gm[9];
+    T gm[9], gface[4][4];  // metric terms; per face (nx, ny, nz, fscale/2)
     T scale;
     int fnb[4], fcode[4];
     {
@@ -199,27 +205,34 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 const int t = lane + 64 * k;
-                v[c][k] = (t < cnt) ? src[t] : T(0);
+                v[c][k] = src[t < cnt ? t : cnt - 1];
             }
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) gm[k] = ge[k];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) gface[f][k] = ge[9 + 4 * f + k];
         // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
-        scale = valid ? ge[25 + half] : T(0);
+        const T sc = ge[25 + half];
+        scale = valid ? sc : T(0);
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
             fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
         }
+        // rows ne..TE-1 of a partial task receive copies of the last valid
+        // value (only the rows of the padding elements read them); lanes past
+        // TE*NP write the never-read last column of row 0 (US - 1 >= KP), so
+        // the writes need no branch into which the compiler could sink loads
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 const int t = lane + 64 * k;
-                if (t < cnt) {
-                    const int r = t / NP;
-                    U[(c * TE + r) * US + (t - r * NP)] = v[c][k];
-                }
+                const int r = t / NP;
+                U[t < TE * NP ? (c * TE + r) * US + (t - r * NP) : c * TE * US + US - 1] = v[c][k];
             }
     }
     wave_sync();
@@ -239,51 +252,49 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     // lanes of an element read the same neighbour element.  A lane forms
     // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
     // takes the other field's jumps from its partner row (lane ^ 8, one
-    // DPP row rotation).  Neighbours inside the task and boundary faces
-    // read LDS (a boundary face reads the element's own trace, weighted
+    // DPP row rotation).  The exterior trace is gathered from memory one
+    // k-step ahead (a boundary face reads the element's own trace, weighted
     // by the boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
-    // 309-311); a neighbour in another task is gathered from memory for
-    // all k-steps of the face when the face starts.
+    // 309-311); the interior trace comes from the task's fields in LDS.
     const T* Ue = U + el * US;
     const int cbase = half ? 3 : 0;  // own field's first component
     // per-face setup: neighbour kind, node table, LDS source, own-trace weight
-    auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
     auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
-    auto gather = [&](int s, T (&v)[3]) {  // remote trace of k-step s (own field's 3 components)
+    // exterior trace of k-step s (own field's 3 components), always read from
+    // memory: the neighbour's node facing this lane's node, or on a boundary
+    // face the element's own node (L2 hits for neighbours inside the task)
+    auto gather = [&](int s, T (&v)[3]) {
         const int f = s / SF, b = 4 * (s % SF) + q;
-        const T* src = a.qin + (size_t)cbase * N + (size_t)fnb[f] * NP + face_tab(f)[b < NFP ? b : NFP - 1];
+        const T* src = a.qin + (size_t)cbase * N + (size_t)(fnb[f] >= 0 ? fnb[f] : e) * NP +
+                       face_tab(f)[b < NFP ? b : NFP - 1];
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = src[c * N];
     };
-    T pf[3] = {T(0), T(0), T(0)};
-    if (face_remote(0)) gather(0, pf);
+    T pf[3];
+    gather(0, pf);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const int nb = fnb[f];
         const bool interior = nb >= 0;
-        const bool remote = face_remote(f);
-        const int* tab = face_tab(f);
-        const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
         T cown;
         {
             T ce, ch;
             bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
             cown = interior ? T(-1) : (half ? ch : ce) - T(1);
         }
-        const T* gf = ge + 9 + 4 * f;
-        const T n3[3] = {gf[0], gf[1], gf[2]};
-        const T fsc = gf[3] * scale;
+        const T n3[3] = {gface[f][0], gface[f][1], gface[f][2]};
+        const T fsc = gface[f][3] * scale;
 #pragma unroll
         for (int t = 0; t < SF; ++t) {
             const int s = f * SF + t;
             const int b = 4 * t + q;
             const int bb = b < NFP ? b : NFP - 1;
             const int iM = sFm[f * NFP + bb];
-            const int ip = tab[bb];
             T cur[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[c] : Ub[c * TE * US + ip];
-            if (s + 1 < 4 * SF && face_remote((s + 1) / SF)) gather(s + 1, pf);
+            for (int c = 0; c < 3; ++c) cur[c] = pf[c];
+            if (s + 1 < 4 * SF) gather(s + 1, pf);
+            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a k-step ahead of its use
             T B3[3], A3[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -344,20 +355,26 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     constexpr int NLIN = (2 * TE * NP + 63) / 64;
     constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
     constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
-    auto lin_index = [&](int x, int t, bool& okt) {
-        const int h = t >= TE * NP;
-        const int u = t - h * TE * NP;
-        okt = t < 2 * TE * NP && u < ne * NP;
-        return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (okt ? u : 0);
+    // Linear slot t (E_x | H_x halves) clamped to the last valid node of its
+    // half: lanes past the end repeat that slot's load, arithmetic and store
+    // (identical values to the same address).
+    auto clamp_slot = [&](int t) {
+        const int tc = t < 2 * TE * NP ? t : 2 * TE * NP - 1;
+        const int h = tc >= TE * NP;
+        const int u = tc - h * TE * NP;
+        return h * TE * NP + (u < ne * NP ? u : ne * NP - 1);
+    };
+    auto lin_index = [&](int x, int tc) {
+        const int h = tc >= TE * NP;
+        return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (tc - h * TE * NP);
     };
     T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
     auto fetch = [&](int x, int slot) {
 #pragma unroll
         for (int k = 0; k < NLIN; ++k) {
-            bool okt;
-            const size_t o = lin_index(x, lane + 64 * k, okt);
-            if constexpr (kNeedRes) rl[slot][k] = okt ? a.res[o] : T(0);
-            if constexpr (kNeedX) xl[slot][k] = okt ? a.x[o] : T(0);
+            const size_t o = lin_index(x, clamp_slot(lane + 64 * k));
+            if constexpr (kNeedRes) rl[slot][k] = a.res[o];
+            if constexpr (kNeedX) xl[slot][k] = a.x[o];
         }
     };
     fetch(0, 0);
@@ -377,10 +394,8 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
         if (x + 1 < 3) fetch(x + 1, cur ^ 1);
 #pragma unroll
         for (int k = 0; k < NLIN; ++k) {
-            const int t = lane + 64 * k;
-            bool okt;
-            const size_t o = lin_index(x, t, okt);
-            if (!okt) continue;
+            const int t = clamp_slot(lane + 64 * k);
+            const size_t o = lin_index(x, t);
             const int h = t >= TE * NP;
             const int u = t - h * TE * NP;
             const int c = h ? 3 + x : x;
