@@ -77,6 +77,7 @@ struct MShape {
 };
 
 constexpr int kTaskElems = kMfmaTaskElems;  // MFMA rows: 8 elements x {E, H} outputs
+constexpr int kGatherDepth = 2;  // flux-loop k-steps whose exterior traces are in flight
 
 // Waves per block, chosen so the block's LDS fits in 160 KiB.
 template <int P, typename T>
@@ -252,30 +253,43 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     // lanes of an element read the same neighbour element.  A lane forms
     // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
     // takes the other field's jumps from its partner row (lane ^ 8, one
-    // DPP row rotation).  The exterior trace is gathered from memory one
-    // k-step ahead (a boundary face reads the element's own trace, weighted
-    // by the boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
-    // 309-311); the interior trace comes from the task's fields in LDS.
+    // DPP row rotation).  Exterior traces are gathered from memory D k-steps
+    // ahead (a boundary face reads the element's own trace, weighted by the
+    // boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
+    // 309-311).  FP32: only neighbours in another task are gathered, the
+    // others read the task's fields in LDS; FP64: every exterior trace is
+    // gathered (fewer selects, the neighbour's data is an L2 hit).
+    constexpr bool kRemoteOnly = sizeof(T) == 4;
     const T* Ue = U + el * US;
     const int cbase = half ? 3 : 0;  // own field's first component
-    // per-face setup: neighbour kind, node table, LDS source, own-trace weight
+    auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
     auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
-    // exterior trace of k-step s (own field's 3 components), always read from
-    // memory: the neighbour's node facing this lane's node, or on a boundary
-    // face the element's own node (L2 hits for neighbours inside the task)
+    // exterior trace of k-step s (own field's 3 components).  Every lane
+    // issues the load (no branch); with kRemoteOnly, lanes without a remote
+    // face all read the task's first value, one cache line, and discard it.
     auto gather = [&](int s, T (&v)[3]) {
         const int f = s / SF, b = 4 * (s % SF) + q;
-        const T* src = a.qin + (size_t)cbase * N + (size_t)(fnb[f] >= 0 ? fnb[f] : e) * NP +
-                       face_tab(f)[b < NFP ? b : NFP - 1];
+        const int idx = face_tab(f)[b < NFP ? b : NFP - 1];
+        size_t off;
+        if constexpr (kRemoteOnly)
+            off = face_remote(f) ? (size_t)fnb[f] * NP + idx : (size_t)e0 * NP;
+        else
+            off = (size_t)(fnb[f] >= 0 ? fnb[f] : e) * NP + idx;
+        const T* src = a.qin + (size_t)cbase * N + off;
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = src[c * N];
     };
-    T pf[3];
-    gather(0, pf);
+    constexpr int D = kGatherDepth < 4 * SF ? kGatherDepth : 4 * SF;
+    T pf[D][3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gather(d, pf[d]);
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const int nb = fnb[f];
         const bool interior = nb >= 0;
+        const bool remote = face_remote(f);
+        const int* tab = face_tab(f);
+        const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
         T cown;
         {
             T ce, ch;
@@ -292,9 +306,20 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
             const int iM = sFm[f * NFP + bb];
             T cur[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) cur[c] = pf[c];
-            if (s + 1 < 4 * SF) gather(s + 1, pf);
-            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch a k-step ahead of its use
+            for (int c = 0; c < 3; ++c) {
+                if constexpr (kRemoteOnly) {
+                    // the empty asm keeps the LDS value a register operand, so
+                    // the compiler cannot fold the select into one load from a
+                    // generic pointer here (which would undo the prefetch)
+                    T lv = Ub[c * TE * US + tab[bb]];
+                    asm volatile("" : "+v"(lv));
+                    cur[c] = remote ? pf[s % D][c] : lv;
+                } else {
+                    cur[c] = pf[s % D][c];
+                }
+            }
+            if (s + D < 4 * SF) gather(s + D, pf[s % D]);
+            __builtin_amdgcn_sched_barrier(0);  // the prefetch stays D k-steps ahead of its use
             T B3[3], A3[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
