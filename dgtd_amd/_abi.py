@@ -32,7 +32,7 @@ class DgtdError(RuntimeError):
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("K", c_int32), ("order", c_int32), ("vx", _dp), ("nv", c_int32),
-                ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp)]
+                ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp), ("dim", c_int32)]
 
 
 class Opts(ctypes.Structure):

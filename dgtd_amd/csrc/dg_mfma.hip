@@ -305,18 +305,21 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
             const int bb = b < NFP ? b : NFP - 1;
             const int iM = sFm[f * NFP + bb];
             T cur[3];
+            if constexpr (kRemoteOnly) {
+                // the empty asm keeps the LDS values register operands, so the
+                // compiler cannot fold the select into one load from a generic
+                // pointer here (which would undo the prefetch); it is not
+                // volatile, so it does not count as a memory side effect
+                const int ip = tab[bb];
+                T lv[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if constexpr (kRemoteOnly) {
-                    // the empty asm keeps the LDS value a register operand, so
-                    // the compiler cannot fold the select into one load from a
-                    // generic pointer here (which would undo the prefetch)
-                    T lv = Ub[c * TE * US + tab[bb]];
-                    asm volatile("" : "+v"(lv));
-                    cur[c] = remote ? pf[s % D][c] : lv;
-                } else {
-                    cur[c] = pf[s % D][c];
-                }
+                for (int c = 0; c < 3; ++c) lv[c] = Ub[c * TE * US + ip];
+                asm("" : "+v"(lv[0]), "+v"(lv[1]), "+v"(lv[2]));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[s % D][c] : lv[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cur[c] = pf[s % D][c];
             }
             if (s + D < 4 * SF) gather(s + D, pf[s % D]);
             __builtin_amdgcn_sched_barrier(0);  // the prefetch stays D k-steps ahead of its use

@@ -1,6 +1,7 @@
 // C ABI of the operator (include/dgtd_hip.h): setup on the host in FP64,
 // device-resident buffers, fused stage launches.
 #include "dgtd_hip.h"
+#include "tri.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -66,6 +67,8 @@ struct dgtd_op {
     int integrator = DGTD_LSERK4;
     double alpha = 1.0;
     dgtd::RefTet R;
+    int dim = 3;       // 2: triangles (the 2D operator, RefTri T2)
+    dgtd::RefTri T2;
     int K = 0, Np = 0, Nfp = 0;
     size_t N = 0;
     int64_t ndofs = 0;
@@ -148,6 +151,7 @@ struct dgtd_op {
     template <typename T>
     hipError_t launch_on(const dgtd::StageArgs<T>& a0, int mode, int var, hipStream_t s) const
     {
+        if (dim == 2) return dgtd::launch_tri_stage<T>(T2.p, a0, mode, grid, s);
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
             a.mats = static_cast<const T*>(d_mats_mfma);
@@ -348,6 +352,93 @@ std::vector<int> morton_task_order(const dgtd_mesh_desc* mesh)
 
 namespace {
 
+dgtd::TriMeshInput tri_input(const dgtd_mesh_desc* mesh)
+{
+    dgtd::TriMeshInput in;
+    in.K = mesh->K;
+    in.nv = mesh->nv;
+    in.vx = mesh->vx;
+    in.EToV = mesh->EToV;
+    in.face_bc = mesh->face_bc;
+    in.eps_r = mesh->eps_r;
+    in.mu_r = mesh->mu_r;
+    return in;
+}
+
+dgtd::TriMeshInput tri_input(const dgtd_op* op)
+{
+    dgtd::TriMeshInput in;
+    in.K = op->K;
+    in.nv = op->nv;
+    in.vx = op->vx.data();
+    in.EToV = op->EToV.data();
+    return in;
+}
+
+// Set-up of a triangle-mesh operator (mesh->dim == 2): reference triangle,
+// geometry and connectivity records, the per-node stage kernel only.
+void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh)
+{
+    op->dim = 2;
+    op->T2 = dgtd::build_reftri(mesh->order);
+    const dgtd::RefTri& R = op->T2;
+    op->K = mesh->K;
+    op->Np = R.Np;
+    op->Nfp = R.Nfp;
+    op->N = (size_t)op->Np * mesh->K;
+    op->ndofs = 6 * (int64_t)op->N;
+    const dgtd::TriMeshData md = dgtd::build_tri_mesh(tri_input(mesh), R, op->alpha);
+    op->max_fscale = md.max_fscale;
+    op->nv = mesh->nv;
+    op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
+    op->EToV.assign(mesh->EToV, mesh->EToV + 3 * (size_t)mesh->K);
+    hip_check(hipSetDevice(op->device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&op->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&op->ev1), "hipEventCreate");
+    std::vector<int> tabs(R.Fmask.begin(), R.Fmask.end());
+    tabs.insert(tabs.end(), R.edge_perm.begin(), R.edge_perm.end());
+    op->d_tabs = static_cast<int*>(upload(tabs));
+    op->d_conn = static_cast<int*>(upload(md.conn));
+    // Dr^T, Ds^T ([j][i]), then LIFT^T ([m][i])
+    const int Np = R.Np, NF = 3 * R.Nfp;
+    std::vector<double> m((size_t)2 * Np * Np + (size_t)NF * Np);
+    for (int i = 0; i < Np; ++i)
+        for (int j = 0; j < Np; ++j) {
+            m[(size_t)j * Np + i] = R.Dr[(size_t)i * Np + j];
+            m[(size_t)Np * Np + (size_t)j * Np + i] = R.Ds[(size_t)i * Np + j];
+        }
+    for (int i = 0; i < Np; ++i)
+        for (int k = 0; k < NF; ++k) m[2 * (size_t)Np * Np + (size_t)k * Np + i] = R.LIFT[(size_t)i * NF + k];
+    if (op->precision == 64) {
+        op->d_geo = upload(md.geo);
+        op->d_mats = upload(m);
+    } else {
+        op->d_geo = upload(dgtd::cast_vec<float>(md.geo));
+        op->d_mats = upload(dgtd::cast_vec<float>(m));
+    }
+    op->d_M = static_cast<double*>(upload(R.M));
+    op->d_jac = static_cast<double*>(upload(md.jac));
+    op->d_eps = static_cast<double*>(upload(md.eps));
+    op->d_mu = static_cast<double*>(upload(md.mu));
+    const int te = dgtd::norms_tile(op->Np);
+    op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
+    hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+    hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+    const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
+    for (int b = 0; b < nbuf; ++b) {
+        hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
+        hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
+    }
+    hip_check(hipStreamSynchronize(op->stream), "sync");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, op->device), "hipGetDeviceProperties");
+    const int epb = dgtd::tri_elements_per_block(R.p);
+    op->nblk = (op->K + epb - 1) / epb;
+    op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * 4));
+    op->variant = DGTD_KERNEL_VALU;
+}
+
 dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
 {
     dgtd::MeshInput in;
@@ -381,6 +472,9 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (!std::isfinite(opts->alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
     if (opts->kernel < DGTD_KERNEL_AUTO || opts->kernel > DGTD_KERNEL_MFMA)
         return fail(DGTD_E_ARG, "kernel must be DGTD_KERNEL_AUTO, _VALU or _MFMA");
+    if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
+    if (mesh->dim == 2 && opts->kernel == DGTD_KERNEL_MFMA)
+        return fail(DGTD_E_UNSUPPORTED, "triangles run the per-node kernel only");
     dgtd_op* op = nullptr;
     try {
         op = new dgtd_op;
@@ -388,6 +482,11 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->precision = opts->precision;
         op->integrator = opts->integrator;
         op->alpha = opts->alpha;
+        if (mesh->dim == 2) {
+            create_tri(op, mesh);
+            *out = op;
+            return DGTD_OK;
+        }
         op->R = dgtd::build_reftet(mesh->order);
         op->K = mesh->K;
         op->Np = op->R.Np;
@@ -539,6 +638,10 @@ int dgtd_op_max_fscale(const dgtd_op* op, double* out)
 int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
 {
     if (!op || !xyz) return fail(DGTD_E_ARG, "NULL argument");
+    if (op->dim == 2) {
+        dgtd::tri_node_positions(tri_input(op), op->T2, xyz);
+        return DGTD_OK;
+    }
     dgtd::MeshInput in;
     in.K = op->K;
     in.nv = op->nv;
@@ -717,6 +820,7 @@ int dgtd_op_norms(dgtd_op* op, double* out3)
 int dgtd_op_set_planewaves(dgtd_op* op, const dgtd_planewave* w, int32_t n)
 {
     if (!op || n < 0 || (n > 0 && !w)) return fail(DGTD_E_ARG, "NULL argument or negative count");
+    if (op->dim == 2 && n > 0) return fail(DGTD_E_UNSUPPORTED, "TF/SF plane waves are not available on triangles");
     std::vector<dgtd::TfsfWave> dev((size_t)n);
     for (int i = 0; i < n; ++i) {
         const dgtd_planewave& p = w[i];
@@ -762,12 +866,16 @@ int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem)
         hip_check(hipSetDevice(op->device), "hipSetDevice");
         std::vector<int32_t> el(n, -1);
         std::vector<double> w((size_t)n * op->Np, 0.0);
-        dgtd::MeshInput in;
-        in.K = op->K;
-        in.nv = op->nv;
-        in.vx = op->vx.data();
-        in.EToV = op->EToV.data();
-        dgtd::locate_points(in, op->R, n, xyz, el.data(), w.data());
+        if (op->dim == 2) {
+            dgtd::tri_locate_points(tri_input(op), op->T2, n, xyz, el.data(), w.data());
+        } else {
+            dgtd::MeshInput in;
+            in.K = op->K;
+            in.nv = op->nv;
+            in.vx = op->vx.data();
+            in.EToV = op->EToV.data();
+            dgtd::locate_points(in, op->R, n, xyz, el.data(), w.data());
+        }
         for (void* q : {(void*)op->d_probe_elem, (void*)op->d_probe_w, (void*)op->d_probe_out})
             if (q) hip_check(hipFree(q), "hipFree");
         op->d_probe_elem = nullptr;
@@ -842,6 +950,11 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops)
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     const double s = (double)op->real_size();
     const double Np = op->Np, Nfp = op->Nfp;
+    if (op->dim == 2) {  // triangles: 2 derivative directions, 3 edges, 15-value record
+        if (bytes) *bytes = 24.0 * Np * s + 15.0 * s + 24.0;
+        if (flops) *flops = 24.0 * Np * Np + 36.0 * Np * Nfp + 150.0 * Nfp + 96.0 * Np;
+        return DGTD_OK;
+    }
     if (bytes) *bytes = 24.0 * Np * s + 27.0 * s + 32.0;
     if (flops) *flops = 36.0 * Np * Np + 48.0 * Np * Nfp + 240.0 * Nfp + 96.0 * Np;
     return DGTD_OK;
@@ -876,6 +989,7 @@ int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* 
 int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP, int8_t* face_codes)
 {
     if (!mesh || !vmapM || !vmapP) return fail(DGTD_E_ARG, "NULL argument");
+    if (mesh->dim == 2) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
     try {
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
@@ -902,6 +1016,10 @@ int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, i
     if (!mesh || n < 0 || (n > 0 && (!xyz || !elem || !weights))) return fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     try {
+        if (mesh->dim == 2) {
+            dgtd::tri_locate_points(tri_input(mesh), dgtd::build_reftri(mesh->order), n, xyz, elem, weights);
+            return DGTD_OK;
+        }
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         dgtd::locate_points(mesh_input(mesh), R, n, xyz, elem, weights);
     } catch (const std::exception& e) {
@@ -913,6 +1031,7 @@ int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, i
 int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo)
 {
     if (!mesh || !geo) return fail(DGTD_E_ARG, "NULL argument");
+    if (mesh->dim == 2) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
     try {
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
@@ -935,10 +1054,12 @@ extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgt
 {
     if (!mesh || !x || (nsrc > 0 && !src)) return fail(DGTD_E_ARG, "NULL argument");
     try {
-        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
-        const size_t N = (size_t)R.Np * mesh->K;
+        if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) throw std::runtime_error("order must be in 1..5");
+        const int Np = mesh->dim == 2 ? dgtd::tri_num_nodes(mesh->order) : dgtd::num_nodes(mesh->order);
+        const size_t N = (size_t)Np * mesh->K;
         std::vector<double> xyz(3 * N);
-        dgtd::node_positions(mesh_input(mesh), R, xyz.data());
+        if (mesh->dim == 2) dgtd::tri_node_positions(tri_input(mesh), dgtd::build_reftri(mesh->order), xyz.data());
+        else dgtd::node_positions(mesh_input(mesh), dgtd::build_reftet(mesh->order), xyz.data());
         std::fill(x, x + 6 * N, 0.0);
         for (int s = 0; s < nsrc; ++s) {
             const dgtd_initial_field& f = src[s];

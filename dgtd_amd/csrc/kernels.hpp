@@ -46,6 +46,14 @@ int stage_blocks_per_cu(int order);
 
 int elements_per_block(int order);
 
+// Triangle (2D) stage kernel, dg_tri.hip: StageArgs with the triangle records
+// (geo: 15 values, conn: 6 ints per element; mats: Dr^T, Ds^T, LIFT^T; tabs:
+// Fmask[3*Nfp] then the edge table [18*Nfp]).
+template <typename T>
+hipError_t launch_tri_stage(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+int tri_elements_per_block(int order);
+size_t tri_smem_bytes(int order, int precision);
+
 // MFMA variant (dg_mfma.hip): wave-tasks of kMfmaTaskElems elements,
 // waves_per_block waves.
 constexpr int kMfmaTaskElems = 8;

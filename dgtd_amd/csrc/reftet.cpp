@@ -87,6 +87,8 @@ Dense inv(const Dense& A)
     return X;
 }
 
+}  // namespace
+
 // Orthonormal Jacobi polynomial P_n^{(a,b)} on [-1,1] (weight (1-x)^a (1+x)^b).
 double jac(double x, double a, double b, int n)
 {
@@ -149,6 +151,8 @@ std::vector<double> gll01(int p)
     for (auto& v : x) v = 0.5 * (v + 1.0);
     return x;
 }
+
+namespace {
 
 // Basis psi_{ijk}(r,s,t) and its gradient in collapsed coordinates (a,b,c).
 void pkd3(double r, double s, double t, int i, int j, int k, double& v, double& vr, double& vs,

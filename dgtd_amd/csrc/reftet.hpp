@@ -34,6 +34,12 @@ struct RefTet {
     std::vector<int> face_perm;    // 96*Nfp
 };
 
+// Orthonormal Jacobi polynomial P_n^{(a,b)} on [-1,1] and its derivative;
+// Gauss-Lobatto-Legendre points on [0,1] (p+1 points).
+double jac(double x, double a, double b, int n);
+double djac(double x, double a, double b, int n);
+std::vector<double> gll01(int p);
+
 // Builds the reference element of order p (1..kMaxOrder).  Throws
 // std::runtime_error on failure.
 RefTet build_reftet(int p);

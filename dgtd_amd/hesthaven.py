@@ -46,14 +46,21 @@ class _Mesh:
     """Keeps the numpy arrays alive while the C descriptor points at them."""
 
     def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None):
-        self.vx = np.ascontiguousarray(vx, dtype=np.float64).reshape(-1, 3)
-        self.EToV = np.ascontiguousarray(EToV, dtype=np.int32).reshape(-1, 4)
-        self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, 4)
+        E = np.asarray(EToV)
+        # triangles (K, 3): the 2D operator; vertices may carry 2 or 3 coordinates
+        self.dim = 2 if E.ndim == 2 and E.shape[1] == 3 else 3
+        nvert = self.dim + 1
+        v = np.asarray(vx, dtype=np.float64)
+        if self.dim == 2 and v.ndim == 2 and v.shape[1] == 2:
+            v = np.hstack([v, np.zeros((v.shape[0], 1))])
+        self.vx = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        self.EToV = np.ascontiguousarray(E, dtype=np.int32).reshape(-1, nvert)
+        self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, nvert)
         self.eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
         self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
         self.desc = _abi.MeshDesc(self.EToV.shape[0], order, dptr(self.vx), self.vx.shape[0],
                                   i32ptr(self.EToV), i8ptr(self.face_bc), dptr(self.eps),
-                                  dptr(self.mu))
+                                  dptr(self.mu), self.dim)
 
 
 def gaussian(field="E", center=(0.5, 0.5, 0.5), spread=0.1, polarization=(0, 0, 1), dimension=3):
@@ -88,11 +95,15 @@ def planewave(polarization, propagation, spread, mean=0.0, frequency=None):
     return w
 
 
+def _nodes(order, dim):
+    return num_nodes(order) if dim == 3 else (order + 1) * (order + 2) // 2
+
+
 def project_initial_fields(vx, EToV, order, sources):
     """Sum of the sources evaluated at the nodes: a 6*Np*K state vector."""
     m = _Mesh(vx, EToV, None, order)
     arr = (_abi.InitialField * len(sources))(*sources)
-    x = np.empty(6 * num_nodes(order) * m.EToV.shape[0])
+    x = np.empty(6 * _nodes(order, m.dim) * m.EToV.shape[0])
     check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), arr, len(sources), dptr(x)))
     return x
 
@@ -127,7 +138,7 @@ def mesh_locate(vx, EToV, order, points):
     m = _Mesh(vx, EToV, None, order)
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     elem = np.zeros(pts.shape[0], dtype=np.int32)
-    w = np.zeros((pts.shape[0], num_nodes(order)))
+    w = np.zeros((pts.shape[0], _nodes(order, m.dim)))
     check(lib().dgtd_mesh_locate(ctypes.byref(m.desc), pts.shape[0], dptr(pts), i32ptr(elem), dptr(w)))
     return elem, w
 
@@ -152,9 +163,10 @@ class HesthavenEvolution:
         self.integrator = integrator
         self.alpha = alpha
         mesh = _Mesh(vx, EToV, face_bc, order, eps, mu)
+        self.dim = mesh.dim
         self.K = mesh.EToV.shape[0]
-        self.Np = num_nodes(order)
-        self.Nfp = num_face_nodes(order)
+        self.Np = _nodes(order, mesh.dim)
+        self.Nfp = num_face_nodes(order) if mesh.dim == 3 else order + 1
         opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device),
                          KERNELS[kernel])
         h = ctypes.c_void_p()

@@ -79,6 +79,14 @@ typedef struct {
     const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (1-3 boundary, 4-7 interior faces), NULL = none */
     const double* eps_r;    /* K relative permittivity, NULL = 1 */
     const double* mu_r;     /* K relative permeability, NULL = 1 */
+    int32_t dim;            /* 0 or 3: tetrahedra.  2: triangles -- the 2D hesthaven
+                             * operator (HesthavenEvolution.cpp:450-542 on a 2D mesh,
+                             * d/dz = 0 and n_z = 0, DGOperatorFactory.h:428-437):
+                             * EToV and face_bc hold 3 entries per element, edge f
+                             * joins local vertices (f, f+1 mod 3) as in MFEM's
+                             * triangle, codes 1-6 (no TF/SF), vx keeps 3
+                             * coordinates per vertex with z ignored; the state keeps
+                             * all six components, Np = (p+1)(p+2)/2. */
 } dgtd_mesh_desc;
 
 /* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)

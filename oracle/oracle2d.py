@@ -1,0 +1,334 @@
+"""CPU FP64 restatement (numpy) of the reference's Hesthaven DGTD operator on
+triangles, the 2D variant of HesthavenEvolution::Mult
+(src/evolution/HesthavenEvolution.cpp:450-542).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ and never by the dgtd_amd
+package.
+
+The reference's operator is dimension-generic: the directional matrices are
+built for d = X, Y, Z and `buildDerivativeSubOperator` returns an empty
+(zero) operator for d >= mesh dimension (src/components/DGOperatorFactory.h:
+428-437), and 2D face normals have no z component; faces are the element's
+edges with order+1 nodes (HesthavenEvolution.cpp:178-179).  So on triangles
+the six field components evolve with the 3D flux and curl formulas with
+d/dz = 0 and n_z = 0.
+
+Reference element: MFEM's L2 Gauss-Lobatto triangle nodes, the orthonormal
+Proriol-Koornwinder-Dubiner basis (Hesthaven & Warburton 2008, ch. 6),
+Dr, Ds = V_r V^-1, V_s V^-1, M = (V V^T)^-1, LIFT = M^-1 E with E the edge
+mass matrices (reference edge length 2); fscale = (edge length / 2) / J with
+J = area / 2 (reference area 2, HesthavenEvolution.cpp:133-147).
+"""
+import numpy as np
+
+# Carpenter & Kennedy (1994) low-storage RK4(5) coefficients
+RK4A = np.array([0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                 -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0])
+RK4B = np.array([1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                 1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                 2277821191437.0 / 14882151754819.0])
+RK4C = np.array([0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                 2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0])
+
+# boundary codes (same as the 3D operator): 0 interior/none, 1 PEC, 2 PMC,
+# 3 SMA, 4/5/6 interior PEC/PMC/SMA
+EDGES = ((0, 1), (1, 2), (2, 0))  # MFEM triangle edge vertices
+
+
+def gll01(p):
+    """Gauss-Lobatto-Legendre points on [0, 1] (ends and roots of P_p')."""
+    if p == 1:
+        return np.array([0.0, 1.0])
+    c = np.zeros(p + 1)
+    c[p] = 1.0
+    inner = np.sort(np.polynomial.legendre.legroots(np.polynomial.legendre.legder(c)).real)
+    x = np.concatenate([[-1.0], inner, [1.0]])
+    x = 0.5 * (x - x[::-1])  # exact symmetry
+    return 0.5 * (x + 1.0)
+
+
+def jacobi_p(x, a, b, n):
+    """Orthonormal Jacobi polynomial P_n^{(a,b)} (Hesthaven & Warburton, JacobiP)."""
+    x = np.asarray(x, dtype=float)
+    from math import gamma
+    g0 = 2 ** (a + b + 1) / (a + b + 1) * gamma(a + 1) * gamma(b + 1) / gamma(a + b + 1)
+    p0 = np.full_like(x, 1.0 / np.sqrt(g0))
+    if n == 0:
+        return p0
+    g1 = (a + 1) * (b + 1) / (a + b + 3) * g0
+    p1 = ((a + b + 2) * x / 2 + (a - b) / 2) / np.sqrt(g1)
+    if n == 1:
+        return p1
+    aold = 2 / (2 + a + b) * np.sqrt((a + 1) * (b + 1) / (a + b + 3))
+    for i in range(1, n):
+        h1 = 2 * i + a + b
+        anew = 2 / (h1 + 2) * np.sqrt((i + 1) * (i + 1 + a + b) * (i + 1 + a) * (i + 1 + b) / (h1 + 1) / (h1 + 3))
+        bnew = -(a * a - b * b) / h1 / (h1 + 2)
+        p0, p1 = p1, 1 / anew * (-aold * p0 + (x - bnew) * p1)
+        aold = anew
+    return p1
+
+
+def grad_jacobi_p(x, a, b, n):
+    if n == 0:
+        return np.zeros_like(np.asarray(x, dtype=float))
+    return np.sqrt(n * (n + a + b + 1)) * jacobi_p(x, a + 1, b + 1, n - 1)
+
+
+def _rs_to_ab(r, s):
+    a = np.where(np.abs(s - 1) > 1e-14, 2 * (1 + r) / np.where(np.abs(s - 1) > 1e-14, 1 - s, 1) - 1, -1.0)
+    return a, s
+
+
+def simplex2d(r, s, i, j):
+    """Orthonormal basis psi_ij on the reference triangle (Simplex2DP)."""
+    a, b = _rs_to_ab(r, s)
+    return np.sqrt(2.0) * jacobi_p(a, 0, 0, i) * jacobi_p(b, 2 * i + 1, 0, j) * (1 - b) ** i
+
+
+def grad_simplex2d(r, s, i, j):
+    """d psi_ij / dr, d psi_ij / ds (GradSimplex2DP)."""
+    a, b = _rs_to_ab(r, s)
+    fa, dfa = jacobi_p(a, 0, 0, i), grad_jacobi_p(a, 0, 0, i)
+    gb, dgb = jacobi_p(b, 2 * i + 1, 0, j), grad_jacobi_p(b, 2 * i + 1, 0, j)
+    dr = dfa * gb
+    if i > 0:
+        dr = dr * (0.5 * (1 - b)) ** (i - 1)
+    ds = dfa * (gb * 0.5 * (1 + a))
+    if i > 0:
+        ds = ds * (0.5 * (1 - b)) ** (i - 1)
+    tmp = dgb * (0.5 * (1 - b)) ** i
+    if i > 0:
+        tmp = tmp - 0.5 * i * gb * (0.5 * (1 - b)) ** (i - 1)
+    ds = ds + fa * tmp
+    sc = 2 ** (i + 0.5)
+    return dr * sc, ds * sc
+
+
+class RefTri:
+    """Reference triangle of order p in MFEM's L2 Gauss-Lobatto node order:
+    j (eta) outer, i (xi) inner, node = (g_i, g_j) / (g_i + g_j + g_k),
+    k = p - i - j; r = 2 xi - 1, s = 2 eta - 1."""
+
+    def __init__(self, p):
+        self.p = p
+        g = gll01(p)
+        lat, xi = [], []
+        for j in range(p + 1):
+            for i in range(p + 1 - j):
+                k = p - i - j
+                w = g[i] + g[j] + g[k]
+                lat.append((i, j, k))
+                xi.append((g[i] / w, g[j] / w))
+        self.lattice = np.array(lat)
+        self.xi = np.array(xi)
+        self.Np = len(xi)
+        self.Nfp = p + 1
+        r, s = 2 * self.xi[:, 0] - 1, 2 * self.xi[:, 1] - 1
+        modes = [(i, j) for i in range(p + 1) for j in range(p + 1 - i)]
+        self.V = np.stack([simplex2d(r, s, i, j) for i, j in modes], axis=1)
+        gr = [grad_simplex2d(r, s, i, j) for i, j in modes]
+        Vr = np.stack([t[0] for t in gr], axis=1)
+        Vs = np.stack([t[1] for t in gr], axis=1)
+        self.modes = modes
+        self.Vinv = np.linalg.inv(self.V)
+        self.Dr = Vr @ self.Vinv
+        self.Ds = Vs @ self.Vinv
+        self.Minv = self.V @ self.V.T
+        self.M = np.linalg.inv(self.Minv)
+        # edge node lists, ordered from the edge's first vertex to its second
+        # (vertex weights: v0 <- k, v1 <- i, v2 <- j)
+        L = self.lattice
+        f0 = [n for n in np.argsort(L[:, 0]) if L[n, 1] == 0]          # v0 -> v1: j = 0, i ascending
+        f1 = [n for n in np.argsort(L[:, 1]) if L[n, 2] == 0]          # v1 -> v2: k = 0, j ascending
+        f2 = [n for n in np.argsort(-L[:, 1]) if L[n, 0] == 0]         # v2 -> v0: i = 0, j descending
+        self.Fmask = np.array([f0, f1, f2])
+        # edge mass (reference edge length 2) from the 1D orthonormal basis
+        E = np.zeros((self.Np, 3 * self.Nfp))
+        lat_v = np.stack([L[:, 2], L[:, 0], L[:, 1]], axis=1)  # lattice index of v0, v1, v2
+        for f, (va, vb) in enumerate(EDGES):
+            # on an edge the node's weight of vertex vb is the GLL point g[l_vb]:
+            # edge coordinate -1 at va, +1 at vb
+            tg = 2 * g[lat_v[self.Fmask[f], vb]] - 1
+            V1 = np.stack([jacobi_p(tg, 0, 0, n) for n in range(self.Nfp)], axis=1)
+            M1 = np.linalg.inv(V1 @ V1.T)
+            E[np.ix_(self.Fmask[f], f * self.Nfp + np.arange(self.Nfp))] = M1
+        self.E = E
+        self.LIFT = self.Minv @ E
+
+    def interp_weights(self, xi, eta):
+        """w with u(xi, eta) = sum_n w[n] u_n."""
+        r, s = np.array([2 * xi - 1.0]), np.array([2 * eta - 1.0])
+        P = np.array([simplex2d(r, s, i, j)[0] for i, j in self.modes])
+        return self.Vinv.T @ P
+
+
+class Operator2D:
+    """The 2D Hesthaven operator on a triangle mesh.
+
+    vx: (nv, 2) or (nv, 3) vertex coordinates (z ignored); EToV: (K, 3);
+    face_bc: (K, 3) codes per edge f = (v_f, v_{f+1 mod 3}); eps, mu: (K,)."""
+
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
+        self.R = R = RefTri(order)
+        vx = np.asarray(vx, dtype=float)[:, :2]
+        EToV = np.asarray(EToV, dtype=np.int64)
+        self.K = K = EToV.shape[0]
+        self.Np, self.Nfp = R.Np, R.Nfp
+        self.alpha = float(alpha)
+        self.eps = np.ones(K) if eps is None else np.asarray(eps, dtype=float)
+        self.mu = np.ones(K) if mu is None else np.asarray(mu, dtype=float)
+        v0, v1, v2 = vx[EToV[:, 0]], vx[EToV[:, 1]], vx[EToV[:, 2]]
+        xr, xs = 0.5 * (v1 - v0), 0.5 * (v2 - v0)
+        J = xr[:, 0] * xs[:, 1] - xs[:, 0] * xr[:, 1]
+        if np.any(np.abs(J) < 1e-300):
+            raise ValueError("degenerate triangle")
+        self.J = J
+        # d/dx = rx d/dr + sx d/ds, d/dy = ry d/dr + sy d/ds
+        self.rx, self.ry = xs[:, 1] / J, -xs[:, 0] / J
+        self.sx, self.sy = -xr[:, 1] / J, xr[:, 0] / J
+        self.pos = (v0[:, None, :] + R.xi[None, :, 0:1] * (v1 - v0)[:, None, :]
+                    + R.xi[None, :, 1:2] * (v2 - v0)[:, None, :])  # (K, Np, 2)
+        verts = [v0, v1, v2]
+        self.nrm = np.zeros((K, 3, 2))
+        self.fscale = np.zeros((K, 3))
+        sgn = np.sign(J)
+        for f, (a, b) in enumerate(EDGES):
+            t = verts[b] - verts[a]
+            Lf = np.hypot(t[:, 0], t[:, 1])
+            self.nrm[:, f, 0] = sgn * t[:, 1] / Lf
+            self.nrm[:, f, 1] = -sgn * t[:, 0] / Lf
+            self.fscale[:, f] = (Lf / 2) / np.abs(J)
+        # connectivity: shared edges by vertex pair, node matching by position
+        bc = np.asarray(face_bc, dtype=np.int64).reshape(K, 3).copy()
+        owner = {}
+        nbr = -np.ones((K, 3), dtype=np.int64)
+        nbrf = -np.ones((K, 3), dtype=np.int64)
+        for e in range(K):
+            for f, (a, b) in enumerate(EDGES):
+                key = tuple(sorted((int(EToV[e, a]), int(EToV[e, b]))))
+                if key in owner:
+                    e2, f2 = owner.pop(key)
+                    nbr[e, f], nbrf[e, f], nbr[e2, f2], nbrf[e2, f2] = e2, f2, e, f
+                else:
+                    owner[key] = (e, f)
+        for e in range(K):
+            for f in range(3):
+                if nbr[e, f] >= 0 and 1 <= bc[e, f] <= 3:
+                    raise ValueError("PEC/PMC/SMA codes 1-3 on an interior edge (use 4-6)")
+                if nbr[e, f] < 0 and bc[e, f] >= 4:
+                    raise ValueError("interior-boundary codes on a boundary edge")
+                if bc[e, f] == 3 and self.alpha == 0.0:
+                    raise ValueError("SMA boundaries need upwind flux (alpha != 0)")
+        Nfp = R.Nfp
+        h = np.min(np.abs(J)) ** 0.5
+        self.vmapM = np.zeros((K, 3, Nfp), dtype=np.int64)
+        self.vmapP = np.zeros((K, 3, Nfp), dtype=np.int64)
+        for e in range(K):
+            for f in range(3):
+                own = e * R.Np + R.Fmask[f]
+                self.vmapM[e, f] = own
+                e2 = nbr[e, f]
+                if e2 < 0:
+                    self.vmapP[e, f] = own
+                    continue
+                f2 = nbrf[e, f]
+                pm = self.pos[e, R.Fmask[f]]
+                pp = self.pos[e2, R.Fmask[f2]]
+                d = np.linalg.norm(pm[:, None, :] - pp[None, :, :], axis=2)
+                b2 = np.argmin(d, axis=1)
+                if np.any(d[np.arange(Nfp), b2] > 1e-7 * h):
+                    raise ValueError("non-conforming edge: node not matched")
+                self.vmapP[e, f] = e2 * R.Np + R.Fmask[f2][b2]
+        self.bc = bc
+        self.nbr = nbr
+
+    def ndofs(self):
+        return 6 * self.Np * self.K
+
+    def max_fscale(self):
+        return float(self.fscale.max())
+
+    def node_positions(self):
+        return self.pos
+
+    def mult(self, x):
+        """HesthavenEvolution::Mult with d/dz = 0 and n_z = 0."""
+        R, K, Np, Nfp = self.R, self.K, self.Np, self.Nfp
+        q = np.asarray(x, dtype=float).reshape(6, K * Np)
+        M, P = self.vmapM.reshape(K, -1), self.vmapP.reshape(K, -1)
+        dF = q[:, P] - q[:, M]  # (6, K, 3Nfp): jump = exterior - interior
+        own = q[:, M]
+        code = np.repeat(self.bc, Nfp, axis=1)  # (K, 3Nfp)
+        ce = np.select([code == 1, code == 3, code == 4, code == 6],
+                       [-2.0, -1.0 / self.alpha if self.alpha else 0.0, -1.0, -0.5], 0.0)
+        ch = np.select([code == 2, code == 3, code == 5, code == 6],
+                       [-2.0, -1.0 / self.alpha if self.alpha else 0.0, -1.0, -0.5], 0.0)
+        ovr = code > 0
+        for c in range(3):
+            dF[c] = np.where(ovr, ce * own[c], dF[c])
+            dF[3 + c] = np.where(ovr, ch * own[3 + c], dF[3 + c])
+        n = np.repeat(self.nrm, Nfp, axis=1)  # (K, 3Nfp, 2)
+        nv = [n[..., 0], n[..., 1], np.zeros_like(n[..., 0])]
+        ndE = nv[0] * dF[0] + nv[1] * dF[1] + nv[2] * dF[2]
+        ndH = nv[0] * dF[3] + nv[1] * dF[4] + nv[2] * dF[5]
+        fs = np.repeat(self.fscale, Nfp, axis=1) / 2.0
+        a = self.alpha
+        out = np.zeros((6, K, Np))
+        u = q.reshape(6, K, Np)
+        ur = np.einsum("ij,ckj->cki", R.Dr, u)
+        us = np.einsum("ij,ckj->cki", R.Ds, u)
+        dx = self.rx[None, :, None] * ur + self.sx[None, :, None] * us
+        dy = self.ry[None, :, None] * ur + self.sy[None, :, None] * us
+        D = [dx, dy, np.zeros_like(dx)]  # D[d][c]: d/dx_d of component c
+        for x in range(3):
+            y, z = (x + 1) % 3, (x + 2) % 3
+            fH = -nv[y] * dF[z] + nv[z] * dF[y] + a * (dF[3 + x] - ndH * nv[x])
+            fE = nv[y] * dF[3 + z] - nv[z] * dF[3 + y] + a * (dF[x] - ndE * nv[x])
+            lH = np.einsum("im,km->ki", R.LIFT, fH * fs)
+            lE = np.einsum("im,km->ki", R.LIFT, fE * fs)
+            hres = -D[y][z] + D[z][y] + lH
+            eres = D[y][3 + z] - D[z][3 + y] + lE
+            out[3 + x] = hres / self.mu[:, None]
+            out[x] = eres / self.eps[:, None]
+        return out.reshape(-1)
+
+    def rk4_step(self, x, dt):
+        """mfem::RK4Solver stage order (Solver.cpp:535-551)."""
+        k = self.mult(x)
+        y, z = x + dt / 2 * k, x + dt / 6 * k
+        k = self.mult(y)
+        y, z = x + dt / 2 * k, z + dt / 3 * k
+        k = self.mult(y)
+        y, z = x + dt * k, z + dt / 3 * k
+        k = self.mult(y)
+        return z + dt / 6 * k
+
+    def lserk4_step(self, q, res, dt):
+        for s in range(5):
+            k = self.mult(q)
+            res = RK4A[s] * res + dt * k
+            q = q + RK4B[s] * res
+        return q, res
+
+    def run(self, x, dt, final_time, integrator="lserk4"):
+        """Solver::run loop guard and last-step truncation (Solver.cpp:497, 537)."""
+        x = np.array(x, dtype=float)
+        res = np.zeros_like(x)
+        t, steps = 0.0, 0
+        while t <= final_time - 1e-8 * dt:
+            h = min(dt, final_time - t)
+            if integrator == "rk4":
+                x = self.rk4_step(x, h)
+            else:
+                x, res = self.lserk4_step(x, res, h)
+            t += h
+            steps += 1
+        return x, steps, t
+
+    def energy(self, x):
+        """1/2 sum_e J_e (eps E^T M E + mu H^T M H) over all six components."""
+        u = np.asarray(x, dtype=float).reshape(6, self.K, self.Np)
+        s = np.einsum("cki,ij,ckj->ck", u, self.R.M, u)
+        J = np.abs(self.J)
+        return float(0.5 * np.sum(J * (self.eps * s[:3].sum(0) + self.mu * s[3:].sum(0))))

@@ -202,3 +202,61 @@ def cut_along_plane(vx, EToV, axis, value, tol=1e-12):
     for e in np.where(far)[0]:
         E2[e] = [copy.get(int(v), int(v)) for v in EToV[e]]
     return vx2, E2
+
+
+# ---- triangles (the 2D operator) ----
+
+TRI_EDGES = ((0, 1), (1, 2), (2, 0))
+
+
+def square_mesh(n, L=1.0, bc="pec", diagonal="alternate", jitter=0.0, seed=0):
+    """n x n squares on [0, L]^2, each split into two triangles.
+
+    Returns vx (nv, 3) with z = 0, EToV (K, 3) int32 (counter-clockwise),
+    face_bc (K, 3) with edge f = (v_f, v_{f+1 mod 3}).  bc: one code for all
+    four sides or a list for the sides x=0, x=L, y=0, y=L.  diagonal:
+    "alternate" flips the split diagonal in a checkerboard; jitter moves the
+    interior vertices by up to jitter * h (a non-uniform mesh)."""
+    n1 = n + 1
+    g = np.arange(n1, dtype=np.float64) * (L / n)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    vx = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        inner = (vx[:, 0] > 0) & (vx[:, 0] < L) & (vx[:, 1] > 0) & (vx[:, 1] < L)
+        vx[inner, :2] += rng.uniform(-jitter, jitter, (inner.sum(), 2)) * (L / n)
+
+    def vid(i, j):
+        return i + n1 * j
+
+    tris = []
+    for j in range(n):
+        for i in range(n):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)
+            if diagonal == "alternate" and (i + j) % 2:
+                tris += [(a, b, d), (b, c, d)]
+            else:
+                tris += [(a, b, c), (a, c, d)]
+    EToV = np.array(tris, dtype=np.int32)
+    codes = [BC[bc]] * 4 if isinstance(bc, str) else [BC[c] if isinstance(c, str) else int(c) for c in bc]
+    face_bc = np.zeros(EToV.shape, dtype=np.int32)
+    for e in range(EToV.shape[0]):
+        for f, (p, q) in enumerate(TRI_EDGES):
+            A, B = vx[EToV[e, p], :2], vx[EToV[e, q], :2]
+            for side, (axis, val) in enumerate([(0, 0.0), (0, L), (1, 0.0), (1, L)]):
+                if abs(A[axis] - val) < 1e-12 and abs(B[axis] - val) < 1e-12:
+                    face_bc[e, f] = codes[side]
+    return vx, EToV, face_bc
+
+
+def tm_mode(pos, t, m=1, n=1, L=1.0):
+    """TM_mn mode of the PEC square [0,L]^2 (eps = mu = 1): E_z, H_x, H_y at
+    positions pos (..., 2) and time t; the other three components are zero."""
+    x, y = pos[..., 0], pos[..., 1]
+    kx, ky = m * np.pi / L, n * np.pi / L
+    w = np.hypot(kx, ky)
+    ez = np.sin(kx * x) * np.sin(ky * y) * np.cos(w * t)
+    hx = -(ky / w) * np.sin(kx * x) * np.cos(ky * y) * np.sin(w * t)
+    hy = (kx / w) * np.cos(kx * x) * np.sin(ky * y) * np.sin(w * t)
+    z = np.zeros_like(ez)
+    return np.stack([z, z, ez, hx, hy, z])

@@ -1,0 +1,201 @@
+"""Pins of the 2D (triangle) oracle, oracle/oracle2d.py, and the product's
+host set-up on triangles.
+
+Golden data from the reference's test/hesthavenComparison/Hesthaven2DTest.cpp:
+nodal positions of the order-2 space on the two triangles of the unit square
+(nodalPosition, :598-646, element 0 permuted by rotatorO2, :83-90), the order-1
+mass matrix (massMatrix, :180-205: Hesthaven's (V V^T)^-1 on the reference
+triangle of area 2) and Hesthaven's order-2 Dr and Ds (DrOperator :207-232,
+DsOperator :555-580, given in Hesthaven's node order, which is the lattice
+order used here).  The rest is pinned by closed forms: polynomial exactness of
+the derivatives and the edge mass, the divergence theorem per element, energy
+conservation with the centred flux, and convergence to the TM_11 cavity mode.
+"""
+import numpy as np
+import pytest
+
+import dgtd_amd as D
+from oracle import oracle2d as O
+
+import meshgen as G
+
+ROT_O2 = np.array([[0, 0, 0, 0, 0, 1], [0, 0, 0, 1, 0, 0], [1, 0, 0, 0, 0, 0],
+                   [0, 0, 0, 0, 1, 0], [0, 1, 0, 0, 0, 0], [0, 0, 1, 0, 0, 0]], dtype=float)
+
+
+def test_nodal_positions_match_reference_golden():
+    # MakeCartesian2D(1,1,TRIANGLE) elements (vertices as MFEM numbers them)
+    vx = np.array([[0, 0], [1, 0], [1, 1], [0, 1.0]])
+    EToV = np.array([[0, 2, 3], [2, 0, 1]])
+    op = O.Operator2D(vx, EToV, [[0, 1, 1], [0, 1, 1]], 2)
+    golden = np.array([[0.0, 1.0], [0.0, 0.5], [0.0, 0.0], [0.5, 1.0], [0.5, 0.5], [1.0, 1.0],
+                       [1.0, 1.0], [0.5, 0.5], [0.0, 0.0], [1.0, 0.5], [0.5, 0.0], [1.0, 0.0]])
+    got = np.vstack([ROT_O2 @ op.pos[0], op.pos[1]])
+    np.testing.assert_allclose(got, golden, atol=1e-14)
+    np.testing.assert_allclose(op.J, 0.25)  # area 1/2 over reference area 2 (massMatrix :187-191)
+
+
+def test_mass_order1_matches_reference_golden():
+    R = O.RefTri(1)
+    golden = np.array([[1 / 3, 1 / 6, 1 / 6], [1 / 6, 1 / 3, 1 / 6], [1 / 6, 1 / 6, 1 / 3]])
+    np.testing.assert_allclose(R.M, golden, atol=1e-14)
+
+
+def test_dr_ds_order2_match_hesthaven():
+    R = O.RefTri(2)
+    Dr = np.array([[-1.5, 2.0, -0.5, 0, 0, 0], [-0.5, 0, 0.5, 0, 0, 0], [0.5, -2, 1.5, 0, 0, 0],
+                   [-0.5, 1, -0.5, -1, 1, 0], [0.5, -1, 0.5, -1, 1, 0], [0.5, 0, -0.5, -2, 2, 0]])
+    Ds = np.array([[-1.5, 0, 0, 2, 0, -0.5], [-0.5, -1, 0, 1, 1, -0.5], [0.5, -2, 0, 0, 2, -0.5],
+                   [-0.5, 0, 0, 0, 0, 0.5], [0.5, -1, 0, -1, 1, 0.5], [0.5, 0, 0, -2, 0, 1.5]])
+    np.testing.assert_allclose(R.Dr, Dr, atol=1e-13)
+    np.testing.assert_allclose(R.Ds, Ds, atol=1e-13)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_derivatives_exact_on_polynomials(p):
+    vx, E, bc = G.square_mesh(2, bc="pec", jitter=0.2, seed=p)
+    op = O.Operator2D(vx, E, bc, p)
+    x, y = op.pos[..., 0], op.pos[..., 1]
+    R = op.R
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            u = x ** a * y ** b
+            ur = np.einsum("ij,kj->ki", R.Dr, u)
+            us = np.einsum("ij,kj->ki", R.Ds, u)
+            dx = op.rx[:, None] * ur + op.sx[:, None] * us
+            dy = op.ry[:, None] * ur + op.sy[:, None] * us
+            ex = a * x ** max(a - 1, 0) * y ** b if a else 0 * x
+            ey = b * x ** a * y ** max(b - 1, 0) if b else 0 * x
+            np.testing.assert_allclose(dx, ex, atol=1e-9)
+            np.testing.assert_allclose(dy, ey, atol=1e-9)
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_edge_mass_integrates_polynomials(p):
+    R = O.RefTri(p)
+    nodes, w = np.polynomial.legendre.leggauss(p + 2)
+    g = 2 * O.gll01(p) - 1
+    # Lagrange basis on the GLL points, evaluated at the Gauss points
+    Lg = np.array([[np.prod([(t - g[m]) / (g[k] - g[m]) for m in range(p + 1) if m != k]) for k in range(p + 1)]
+                   for t in nodes])
+    M1 = Lg.T @ (w[:, None] * Lg)
+    for f in range(3):
+        Eb = R.E[np.ix_(R.Fmask[f], f * R.Nfp + np.arange(R.Nfp))]
+        np.testing.assert_allclose(Eb, M1, atol=1e-13)
+    np.testing.assert_allclose(R.M @ R.LIFT, R.E, atol=1e-12)
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_divergence_theorem_per_element(p):
+    """int_K du/dx dA = sum_f int_f u n_x ds for u of degree p: checks the
+    metric, the mass scaling J, the normals and fscale together."""
+    vx, E, bc = G.square_mesh(3, bc="pec", jitter=0.25, seed=7)
+    op = O.Operator2D(vx, E, bc, p)
+    R = op.R
+    u = np.sin(np.arange(op.K * R.Np)).reshape(op.K, R.Np)  # any nodal field: degree p polynomial per element
+    ur = np.einsum("ij,kj->ki", R.Dr, u)
+    us = np.einsum("ij,kj->ki", R.Ds, u)
+    for comp, (gr, gs) in enumerate([(op.rx, op.sx), (op.ry, op.sy)]):
+        du = gr[:, None] * ur + gs[:, None] * us
+        vol = np.abs(op.J) * np.einsum("i,ij,kj->k", np.ones(R.Np), R.M, du)
+        surf = np.zeros(op.K)
+        for f in range(3):
+            uf = u[:, R.Fmask[f]]
+            Eb = R.E[np.ix_(R.Fmask[f], f * R.Nfp + np.arange(R.Nfp))]
+            # edge length / 2 = fscale * J
+            surf += op.fscale[:, f] * np.abs(op.J) * op.nrm[:, f, comp] * np.einsum("i,ij,kj->k", np.ones(R.Nfp), Eb, uf)
+        np.testing.assert_allclose(vol, surf, atol=1e-11)
+
+
+def test_centred_flux_conserves_energy_upwind_dissipates():
+    vx, E, bc = G.square_mesh(3, bc=["pec", "pmc", "pec", "pmc"])
+    x0 = None
+    energies = {}
+    for alpha in (0.0, 1.0):
+        op = O.Operator2D(vx, E, bc, 3, alpha=alpha)
+        if x0 is None:
+            r2 = (op.pos[..., 0] - 0.4) ** 2 + (op.pos[..., 1] - 0.55) ** 2
+            g = np.exp(-r2 / (2 * 0.15 ** 2))
+            x0 = np.zeros((6,) + g.shape)
+            x0[2], x0[3], x0[4] = g, 0.5 * g, -0.3 * g
+            x0 = x0.reshape(-1)
+        dt = 0.2 / (op.max_fscale() * 9)
+        x, _, _ = op.run(x0, dt, 0.3, "lserk4")
+        energies[alpha] = (op.energy(x0), op.energy(x))
+    e0, e1 = energies[0.0]
+    assert abs(e1 - e0) / e0 < 1e-7
+    e0, e1 = energies[1.0]
+    assert e1 < e0 * (1 - 1e-5)
+
+
+def test_tm11_mode_converges():
+    errs = []
+    for n in (2, 4):
+        vx, E, bc = G.square_mesh(n, bc="pec")
+        op = O.Operator2D(vx, E, bc, 3)
+        x0 = G.tm_mode(op.pos, 0.0).reshape(-1)
+        x, _, t = op.run(x0, 0.25 / (op.max_fscale() * 9), 0.3, "lserk4")
+        ex = G.tm_mode(op.pos, t).reshape(-1)
+        errs.append(np.linalg.norm(x - ex) / np.linalg.norm(ex))
+    assert errs[0] < 0.05 and errs[1] < errs[0] / 12  # order p + 1 = 4 in h
+
+
+def test_rk4_fourth_order_in_time():
+    vx, E, bc = G.square_mesh(2, bc="pec")
+    op = O.Operator2D(vx, E, bc, 3)
+    x0 = G.tm_mode(op.pos, 0.0).reshape(-1)
+    ref, _, _ = op.run(x0, 0.0025, 0.2, "rk4")
+    e = [np.linalg.norm(op.run(x0, dt, 0.2, "rk4")[0] - ref) for dt in (0.02, 0.01)]
+    assert 12 < e[0] / e[1] < 20
+
+
+def _cut_at_x(vx, E, x0):
+    """Duplicates the vertices on the line x = x0 for the elements right of it:
+    the same elements and nodes, with the line turned into two boundaries."""
+    vx2 = vx.copy()
+    on = np.where(np.abs(vx[:, 0] - x0) < 1e-12)[0]
+    dup = {int(v): len(vx2) + i for i, v in enumerate(on)}
+    vx2 = np.vstack([vx2, vx[on]])
+    E2 = E.copy()
+    for e in range(E.shape[0]):
+        if vx[E[e], 0].mean() > x0:
+            E2[e] = [dup.get(int(v), int(v)) for v in E[e]]
+    return vx2, E2
+
+
+@pytest.mark.parametrize("code,bcode", [(4, 1), (5, 2), (6, 3)])
+def test_interior_condition_is_mean_of_cut_sides(code, bcode):
+    """The flux is linear in the jump: with an interior condition (-1, 0),
+    (0, -1) or (-1/2, -1/2) times the own trace on the line x = 1/2, the
+    right-hand side equals the mean of the cut mesh with PEC (-2, 0), PMC
+    (0, -2) or SMA (-1, -1) there (alpha = 1) and of the cut mesh with zero jump."""
+    vx, E, bc = G.square_mesh(4, bc="pec", jitter=0.0)
+    onl = np.zeros(bc.shape, dtype=bool)
+    for e in range(E.shape[0]):
+        for f, (p, q) in enumerate(G.TRI_EDGES):
+            onl[e, f] = abs(vx[E[e, p], 0] - 0.5) < 1e-12 and abs(vx[E[e, q], 0] - 0.5) < 1e-12
+    op = O.Operator2D(vx, E, np.where(onl, code, bc), 3)
+    vx2, E2 = _cut_at_x(vx, E, 0.5)
+    opb = O.Operator2D(vx2, E2, np.where(onl, bcode, bc), 3)
+    op0 = O.Operator2D(vx2, E2, np.where(onl, 0, bc), 3)
+    x = np.random.default_rng(code).standard_normal(op.ndofs())
+    np.testing.assert_allclose(op.mult(x), 0.5 * (opb.mult(x) + op0.mult(x)), atol=1e-11)
+
+
+def test_product_projection_and_location_match_oracle_positions():
+    vx, E, bc = G.square_mesh(3, bc="pec", jitter=0.2, seed=1)
+    for p in (1, 3, 5):
+        op = O.Operator2D(vx, E, bc, p)
+        src = [D.resonant((2, 1), field="E", polarization=(0, 0, 1))]
+        x = D.project_initial_fields(vx, E, p, src).reshape(6, op.K, op.Np)
+        want = np.sin(2 * np.pi * op.pos[..., 0]) * np.sin(np.pi * op.pos[..., 1])
+        np.testing.assert_allclose(x[2], want, atol=1e-13)
+        assert np.abs(x[[0, 1, 3, 4, 5]]).max() == 0.0
+        # location: degree-p polynomial reproduced at random points
+        rng = np.random.default_rng(p)
+        pts = np.column_stack([rng.uniform(0, 1, 20), rng.uniform(0, 1, 20), np.zeros(20)])
+        elem, w = D.mesh_locate(vx, E, p, pts)
+        assert (elem >= 0).all()
+        u = op.pos[..., 0] ** p + 2 * op.pos[..., 0] * op.pos[..., 1] ** (p - 1)
+        got = np.einsum("pn,pn->p", w, u[elem])
+        np.testing.assert_allclose(got, pts[:, 0] ** p + 2 * pts[:, 0] * pts[:, 1] ** (p - 1), atol=1e-11)
