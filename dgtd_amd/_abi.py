@@ -61,7 +61,7 @@ class CaseInfo(ctypes.Structure):
                 ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
                 ("evolution_operator", c_int32), ("nsources", c_int32), ("nprobes", c_int32),
                 ("exporter_steps", c_int32), ("refinement", c_int32), ("snapshot_steps", c_int32),
-                ("snapshot_saves", c_int32), ("nplanewaves", c_int32)]
+                ("snapshot_saves", c_int32), ("nplanewaves", c_int32), ("dim", c_int32)]
 
 
 DGTD_E_CAPACITY = -6
@@ -107,6 +107,7 @@ SIGNATURES = {
     "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
     "dgtd_case_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave)]),
     "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),
+    "dgtd_case_save_fields": (c_int32, [c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, c_double]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),
     "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),

@@ -35,9 +35,11 @@ class Case:
             d = _abi.MeshDesc()
             check(lib().dgtd_case_mesh(h, ctypes.byref(d)))
             K, nv = d.K, d.nv
+            nvert = 3 if d.dim == 2 else 4  # triangles (2D) or tetrahedra
+            self.dim = 2 if d.dim == 2 else 3
             self.vx = np.ctypeslib.as_array(d.vx, shape=(3 * nv,)).reshape(nv, 3).copy()
-            self.EToV = np.ctypeslib.as_array(d.EToV, shape=(4 * K,)).reshape(K, 4).copy()
-            self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(4 * K,)).reshape(K, 4).copy()
+            self.EToV = np.ctypeslib.as_array(d.EToV, shape=(nvert * K,)).reshape(K, nvert).copy()
+            self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(nvert * K,)).reshape(K, nvert).copy()
             self.eps = None if not d.eps_r else np.ctypeslib.as_array(d.eps_r, shape=(K,)).copy()
             self.mu = None if not d.mu_r else np.ctypeslib.as_array(d.mu_r, shape=(K,)).copy()
             n = ci.nsources
@@ -78,8 +80,13 @@ class Case:
         return self.EToV.shape[0]
 
     @property
+    def Np(self):
+        p = self.order
+        return num_nodes(p) if self.dim == 3 else (p + 1) * (p + 2) // 2
+
+    @property
     def ndofs(self):
-        return 6 * num_nodes(self.order) * self.K
+        return 6 * self.Np * self.K
 
     def initial_state(self):
         from .hesthaven import _Mesh

@@ -216,7 +216,7 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
     cs.mu.assign(K, 1.0);
     bool any = false;
     for (int32_t e = 0; e < K; ++e) {
-        auto it = mat.find(cs.mesh.tet_attr[e]);
+        auto it = mat.find(cs.mesh.elem_attr()[e]);
         if (it == mat.end()) continue;
         cs.eps[e] = it->second.first;
         cs.mu[e] = it->second.second;
@@ -230,7 +230,9 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
     // boundaries by triangle attribute (driver.cpp:870-887, 994-1035)
     if (!m.contains("boundaries")) throw std::runtime_error("JSON data does not include 'boundaries'.");
     int max_attr = 0;
-    for (int a : cs.mesh.tri_attr) max_attr = std::max(max_attr, a);
+    for (int a : cs.mesh.bdr_attr()) max_attr = std::max(max_attr, a);
+    if (cs.mesh.dim == 2 && !cs.waves.empty())
+        throw Unsupported("TF/SF plane waves are not supported on triangle meshes");
     std::vector<int> attr_bc(max_attr + 1, -1);
     const dgtd::Json& bs = m.at("boundaries");
     for (size_t i = 0; i < bs.size(); ++i) {
@@ -282,9 +284,16 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
         }
         w.mean = min_phase - 5.0 * w.spread * std::sqrt(2.0);
     }
-    cs.EToV.resize(4 * (size_t)K);
-    for (int32_t e = 0; e < K; ++e)
-        for (int j = 0; j < 4; ++j) cs.EToV[4 * (size_t)e + j] = cs.mesh.tets[e][j];
+    if (cs.mesh.dim == 2) {
+        cs.EToV.resize(3 * (size_t)K);
+        for (int32_t e = 0; e < K; ++e)
+            for (int j = 0; j < 3; ++j) cs.EToV[3 * (size_t)e + j] = cs.mesh.tris[e][j];
+    } else {
+        cs.EToV.resize(4 * (size_t)K);
+        for (int32_t e = 0; e < K; ++e)
+            for (int j = 0; j < 4; ++j) cs.EToV[4 * (size_t)e + j] = cs.mesh.tets[e][j];
+    }
+    cs.info.dim = cs.mesh.dim;
     cs.info.K = K;
     cs.info.nv = cs.mesh.nv();
 }
@@ -342,6 +351,7 @@ extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
     mesh->face_bc = c->face_bc.data();
     mesh->eps_r = c->eps.empty() ? nullptr : c->eps.data();
     mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
+    mesh->dim = c->mesh.dim;
     return DGTD_OK;
 }
 
@@ -356,7 +366,9 @@ extern "C" int dgtd_case_save_mesh(const dgtd_case* c, const char* path)
     return DGTD_OK;
 }
 
-extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t)
+namespace {
+
+int save_fields(const char* dir, int dim, int32_t order, int64_t ndofs, const double* x, double t)
 {
     if (!dir || !x) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (order < 1 || order > 5) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
@@ -364,7 +376,7 @@ extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, c
     static const char* names[6] = {"Ex.gf", "Ey.gf", "Ez.gf", "Hx.gf", "Hy.gf", "Hz.gf"};
     const int64_t n = ndofs / 6;
     const std::string base(dir);
-    const std::string fec = "L2_T1_3D_P" + std::to_string(order);  // L2, GaussLobatto basis (type 1)
+    const std::string fec = std::string(dim == 2 ? "L2_T1_2D_P" : "L2_T1_3D_P") + std::to_string(order);  // L2, GaussLobatto (type 1)
     for (int c = 0; c < 6; ++c) {
         const std::string path = base + "/" + names[c];
         FILE* f = std::fopen(path.c_str(), "w");
@@ -378,6 +390,19 @@ extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, c
     std::fprintf(f, "%.17g", t);
     std::fclose(f);
     return DGTD_OK;
+}
+
+}  // namespace
+
+extern "C" int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t)
+{
+    return save_fields(dir, 3, order, ndofs, x, t);
+}
+
+extern "C" int dgtd_case_save_fields(const dgtd_case* c, const char* dir, int64_t ndofs, const double* x, double t)
+{
+    if (!c) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    return save_fields(dir, c->mesh.dim, c->info.order, ndofs, x, t);
 }
 
 extern "C" int dgtd_case_planewaves(const dgtd_case* c, dgtd_planewave* out)

@@ -165,7 +165,7 @@ int main(int argc, char** argv)
         std::filesystem::create_directories(dir);
         double tt = 0.0;
         check(dgtd_op_get_state(op, x.data(), &tt), "snapshot state");
-        check(dgtd_save_fields(dir.c_str(), info.order, ndofs, x.data(), t), "snapshot fields");
+        check(dgtd_case_save_fields(cs, dir.c_str(), ndofs, x.data(), t), "snapshot fields");
         ++nsnap;
     };
     std::vector<double> vals(6 * (size_t)np);

@@ -86,7 +86,8 @@ TetMesh read_gmsh22(const std::string& path)
                 int nn = 0;
                 if (type == 4) nn = 4;
                 else if (type == 2) nn = 3;
-                else if (type == 15 || type == 1) continue;  // points and lines dropped
+                else if (type == 1) nn = 2;
+                else if (type == 15) continue;  // points dropped
                 else throw MeshUnsupported(path + ": element type " + std::to_string(type) + " is not supported (tetrahedra only)");
                 int32_t vs[4];
                 for (int k = 0; k < nn; ++k) {
@@ -99,16 +100,25 @@ TetMesh read_gmsh22(const std::string& path)
                 if (nn == 4) {
                     m.tets.push_back({vs[0], vs[1], vs[2], vs[3]});
                     m.tet_attr.push_back(attr);
-                } else {
+                } else if (nn == 3) {
                     m.tris.push_back({vs[0], vs[1], vs[2]});
                     m.tri_attr.push_back(attr);
+                } else {
+                    m.segs.push_back({vs[0], vs[1]});
+                    m.seg_attr.push_back(attr);
                 }
             }
             have_elements = true;
         }
     }
     if (!have_format || !have_nodes || !have_elements) bad(path, "missing $MeshFormat, $Nodes or $Elements");
-    if (m.tets.empty()) bad(path, "no tetrahedra");
+    if (m.tets.empty()) {
+        if (m.tris.empty()) bad(path, "no tetrahedra or triangles");
+        m.dim = 2;  // triangles are the elements, lines the boundary
+    } else {
+        m.segs.clear();  // lines of a 3D mesh are dropped
+        m.seg_attr.clear();
+    }
     return m;
 }
 
@@ -139,43 +149,65 @@ TetMesh read_mfem_mesh(const std::string& path)
     };
     auto next_int = [&]() { return std::stol(next()); };
     TetMesh m;
-    bool have_vertices = false;
+    bool have_vertices = false, have_dim = false;
     while (p < tok.size()) {
         const std::string sec = next();
         if (sec == "dimension") {
-            if (next_int() != 3) bad(path, "only 3D meshes are supported");
+            const long d = next_int();
+            if (d != 3 && d != 2) bad(path, "only 2D and 3D meshes are supported");
+            m.dim = (int)d;
+            have_dim = true;
         } else if (sec == "elements") {
+            if (!have_dim) bad(path, "elements before dimension");
             const long n = next_int();
             for (long i = 0; i < n; ++i) {
                 const int attr = (int)next_int();
                 const int geom = (int)next_int();
-                if (geom != 4)
+                const int want = m.dim == 3 ? 4 : 2;  // tetrahedra, or triangles in 2D
+                if (geom != want)
                     throw MeshUnsupported(path + ": element geometry " + std::to_string(geom) +
-                                          " is not supported (tetrahedra only)");
-                std::array<int32_t, 4> t;
-                for (auto& v : t) v = (int32_t)next_int();
-                m.tets.push_back(t);
-                m.tet_attr.push_back(attr);
+                                          " is not supported (tetrahedra or triangles only)");
+                if (m.dim == 3) {
+                    std::array<int32_t, 4> t;
+                    for (auto& v : t) v = (int32_t)next_int();
+                    m.tets.push_back(t);
+                    m.tet_attr.push_back(attr);
+                } else {
+                    std::array<int32_t, 3> t;
+                    for (auto& v : t) v = (int32_t)next_int();
+                    m.tris.push_back(t);
+                    m.tri_attr.push_back(attr);
+                }
             }
         } else if (sec == "boundary") {
+            if (!have_dim) bad(path, "boundary before dimension");
             const long n = next_int();
             for (long i = 0; i < n; ++i) {
                 const int attr = (int)next_int();
                 const int geom = (int)next_int();
-                if (geom != 2) bad(path, "boundary geometry must be triangles");
-                std::array<int32_t, 3> t;
-                for (auto& v : t) v = (int32_t)next_int();
-                m.tris.push_back(t);
-                m.tri_attr.push_back(attr);
+                if (m.dim == 3) {
+                    if (geom != 2) bad(path, "boundary geometry must be triangles");
+                    std::array<int32_t, 3> t;
+                    for (auto& v : t) v = (int32_t)next_int();
+                    m.tris.push_back(t);
+                    m.tri_attr.push_back(attr);
+                } else {
+                    if (geom != 1) bad(path, "boundary geometry must be segments");
+                    std::array<int32_t, 2> t;
+                    for (auto& v : t) v = (int32_t)next_int();
+                    m.segs.push_back(t);
+                    m.seg_attr.push_back(attr);
+                }
             }
         } else if (sec == "vertices") {
             const long n = next_int();
             if (p < tok.size() && tok[p] == "nodes")
                 throw MeshUnsupported(path + ": curved meshes (nodes) are not supported");
             const long vdim = next_int();
-            if (vdim != 3) bad(path, "vertex dimension must be 3");
-            m.vx.resize(3 * (size_t)n);
-            for (long i = 0; i < 3 * n; ++i) m.vx[i] = std::stod(next());
+            if (vdim != 3 && vdim != 2) bad(path, "vertex dimension must be 2 or 3");
+            m.vx.assign(3 * (size_t)n, 0.0);
+            for (long i = 0; i < n; ++i)
+                for (long d = 0; d < vdim; ++d) m.vx[3 * i + d] = std::stod(next());
             have_vertices = true;
         } else if (sec == "nodes") {
             throw MeshUnsupported(path + ": curved meshes (nodes) are not supported");
@@ -185,12 +217,15 @@ TetMesh read_mfem_mesh(const std::string& path)
             bad(path, "unknown section '" + sec + "'");
         }
     }
-    if (!have_vertices || m.tets.empty()) bad(path, "missing vertices or elements");
+    if (!have_vertices || m.K() == 0) bad(path, "missing vertices or elements");
     const int32_t nv = m.nv();
     for (const auto& t : m.tets)
         for (int v : t)
             if (v < 0 || v >= nv) bad(path, "element vertex out of range");
     for (const auto& t : m.tris)
+        for (int v : t)
+            if (v < 0 || v >= nv) bad(path, "element or boundary vertex out of range");
+    for (const auto& t : m.segs)
         for (int v : t)
             if (v < 0 || v >= nv) bad(path, "boundary vertex out of range");
     return m;
@@ -201,6 +236,17 @@ void write_mfem_mesh(const TetMesh& m, const std::string& path)
     std::ofstream o(path);
     if (!o) throw std::runtime_error("cannot write " + path);
     o.precision(16);
+    if (m.dim == 2) {  // triangles (geometry 2), segments (geometry 1), 2D vertices
+        o << "MFEM mesh v1.0\n\ndimension\n2\n\nelements\n" << m.tris.size() << "\n";
+        for (size_t e = 0; e < m.tris.size(); ++e)
+            o << m.tri_attr[e] << " 2 " << m.tris[e][0] << " " << m.tris[e][1] << " " << m.tris[e][2] << "\n";
+        o << "\nboundary\n" << m.segs.size() << "\n";
+        for (size_t i = 0; i < m.segs.size(); ++i) o << m.seg_attr[i] << " 1 " << m.segs[i][0] << " " << m.segs[i][1] << "\n";
+        o << "\nvertices\n" << m.nv() << "\n2\n";
+        for (int32_t v = 0; v < m.nv(); ++v) o << m.vx[3 * v] << " " << m.vx[3 * v + 1] << "\n";
+        if (!o) throw std::runtime_error("write failed: " + path);
+        return;
+    }
     o << "MFEM mesh v1.0\n\ndimension\n3\n\nelements\n" << m.tets.size() << "\n";
     for (size_t e = 0; e < m.tets.size(); ++e)
         o << m.tet_attr[e] << " 4 " << m.tets[e][0] << " " << m.tets[e][1] << " " << m.tets[e][2] << " "
@@ -223,6 +269,15 @@ TetMesh read_mesh_file(const std::string& path)
 
 void orient_positive(TetMesh& m)
 {
+    for (auto& t : m.tris) {
+        if (m.dim != 2) break;
+        const double* a = &m.vx[3 * (size_t)t[0]];
+        const double* b = &m.vx[3 * (size_t)t[1]];
+        const double* c = &m.vx[3 * (size_t)t[2]];
+        const double a2 = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
+        if (a2 == 0.0) throw std::runtime_error("degenerate triangle in mesh");
+        if (a2 < 0) std::swap(t[0], t[1]);
+    }
     for (auto& t : m.tets) {
         const double v6 = signed_volume6(m.vx.data(), t);
         if (v6 == 0.0) throw std::runtime_error("degenerate tetrahedron in mesh");
@@ -232,6 +287,7 @@ void orient_positive(TetMesh& m)
 
 TetMesh refine_uniform(const TetMesh& m)
 {
+    if (m.dim == 2) throw MeshUnsupported("uniform refinement of triangle meshes is not supported");
     TetMesh r;
     r.vx = m.vx;
     const int64_t nv0 = m.nv();
@@ -291,8 +347,47 @@ TetMesh refine_uniform(const TetMesh& m)
     return r;
 }
 
+namespace {
+
+std::vector<int8_t> edge_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
+{
+    static const int kEdge[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    const int64_t nv = m.nv();
+    std::unordered_map<int64_t, std::pair<int64_t, int64_t>> edges;
+    edges.reserve(3 * m.tris.size());
+    for (size_t e = 0; e < m.tris.size(); ++e)
+        for (int f = 0; f < 3; ++f) {
+            const int64_t a = m.tris[e][kEdge[f][0]], b = m.tris[e][kEdge[f][1]];
+            const int64_t k = std::min(a, b) * nv + std::max(a, b);
+            auto it = edges.find(k);
+            if (it == edges.end()) edges.emplace(k, std::make_pair((int64_t)(3 * e + f), (int64_t)-1));
+            else it->second.second = (int64_t)(3 * e + f);
+        }
+    std::vector<int8_t> bc(3 * m.tris.size(), 0);
+    for (size_t i = 0; i < m.segs.size(); ++i) {
+        const int attr = m.seg_attr[i];
+        const int code = (attr >= 0 && attr < (int)attr_to_bc.size()) ? attr_to_bc[attr] : -1;
+        if (code < 0) continue;
+        if (code == 7) throw std::runtime_error("TF/SF surfaces are not supported on triangle meshes");
+        const int64_t a = m.segs[i][0], b = m.segs[i][1];
+        auto it = edges.find(std::min(a, b) * nv + std::max(a, b));
+        if (it == edges.end())
+            throw std::runtime_error("boundary segment with attribute " + std::to_string(attr) + " is no element edge");
+        if (it->second.second >= 0) {
+            bc[it->second.first] = (int8_t)(code + 3);
+            bc[it->second.second] = (int8_t)(code + 3);
+        } else {
+            bc[it->second.first] = (int8_t)code;
+        }
+    }
+    return bc;
+}
+
+}  // namespace
+
 std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
 {
+    if (m.dim == 2) return edge_codes(m, attr_to_bc);
     static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
     const int64_t nv = m.nv();
     auto key_of = [&](int32_t a, int32_t b, int32_t c) {

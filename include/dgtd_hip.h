@@ -270,6 +270,7 @@ typedef struct {
     int32_t snapshot_steps;       /* probes.domain_snapshot: "steps" (0 if absent), -1 = no snapshots */
     int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
     int32_t nplanewaves;          /* "planewave" sources (their tags are TF/SF faces) */
+    int32_t dim;                  /* 3: tetrahedral mesh; 2: triangle mesh (2D operator) */
 } dgtd_case_info;
 
 /* Parse the case file, read and refine its mesh, map tags to boundary codes
@@ -291,6 +292,8 @@ int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out);
  * the given order, 16 significant digits) plus time.txt in directory dir. */
 int dgtd_case_save_mesh(const dgtd_case* c, const char* path);
 int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t);
+/* The same with the case's dimension and order (L2_T1_2D_P<p> on triangles). */
+int dgtd_case_save_fields(const dgtd_case* c, const char* dir, int64_t ndofs, const double* x, double t);
 
 /* Point probes (buildProbes, driver.cpp:707-864): 3 * info.nprobes
  * coordinates, and per probe its "steps" and "saves" entries (0 if absent;

@@ -260,3 +260,31 @@ def tm_mode(pos, t, m=1, n=1, L=1.0):
     hy = (kx / w) * np.cos(kx * x) * np.sin(ky * y) * np.sin(w * t)
     z = np.zeros_like(ez)
     return np.stack([z, z, ez, hx, hy, z])
+
+
+def read_gmsh22_2d(path):
+    """Minimal Gmsh 2.2 reader for a 2D mesh: vertices, triangles (with the
+    elementary tag) and lines as (node, node, elementary tag)."""
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    i = lines.index("$Nodes")
+    nn = int(lines[i + 1])
+    ids, xyz = [], []
+    for ln in lines[i + 2: i + 2 + nn]:
+        t = ln.split()
+        ids.append(int(t[0]))
+        xyz.append([float(v) for v in t[1:4]])
+    idmap = {v: k for k, v in enumerate(ids)}
+    i = lines.index("$Elements")
+    ne = int(lines[i + 1])
+    tris, segs = [], []
+    for ln in lines[i + 2: i + 2 + ne]:
+        t = [int(v) for v in ln.split()]
+        etype, ntags = t[1], t[2]
+        tag = t[3 + 1] if ntags >= 2 else t[3]
+        nodes = [idmap[v] for v in t[3 + ntags:]]
+        if etype == 2:
+            tris.append(nodes + [tag])
+        elif etype == 1:
+            segs.append((nodes[0], nodes[1], tag))
+    return np.array(xyz), np.array(tris, dtype=np.int64), segs

@@ -306,3 +306,65 @@ def test_planewave_auto_mean(tmp_path):
     (w,) = D.Case(tmp_path / "c.json").planewaves
     # the TF/SF plane is x = 1: min phase 1, minus 5 sqrt(2) spreads (driver.cpp:575-588)
     assert w.mean == pytest.approx(1.0 - 5.0 * 0.6 * np.sqrt(2.0))
+
+
+PEC2D = os.path.join(CASES, "2D_PEC.json")
+
+
+def test_2d_pec_case_triangles_segments_and_sources():
+    """testData/maxwellInputs/2D_PEC: a Gmsh file without tetrahedra is a 2D
+    mesh, triangles are the elements and lines the boundary; tags 2, 4 PEC
+    and 1, 3 PMC; the Gaussian of dimension 1 varies along x only."""
+    c = D.Case(PEC2D)
+    assert c.dim == 2 and c.info["dim"] == 2
+    vx, tris, lines = MG.read_gmsh22_2d(os.path.join(CASES, "2D_PEC.msh"))
+    assert c.K == len(tris) == 246 and c.EToV.shape == (246, 3)
+    # elements positively oriented, vertex sets as in the file
+    v = c.vx[c.EToV][:, :, :2]
+    area2 = (v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - (v[:, 2, 0] - v[:, 0, 0]) * (v[:, 1, 1] - v[:, 0, 1])
+    assert (area2 > 0).all()
+    assert sorted(map(tuple, np.sort(c.EToV, axis=1))) == sorted(map(tuple, np.sort(tris[:, :3], axis=1)))
+    # boundary codes from the segment tags
+    want = {}
+    for a, b, tag in lines:
+        want[tuple(sorted((a, b)))] = {2: 1, 4: 1, 1: 2, 3: 2}[tag]
+    got = {}
+    for e in range(c.K):
+        for f, (p, q) in enumerate(MG.TRI_EDGES):
+            if c.face_bc[e, f]:
+                got[tuple(sorted((int(c.EToV[e, p]), int(c.EToV[e, q]))))] = int(c.face_bc[e, f])
+    assert got == want and len(got) == 40
+    # probes given as 2-vectors padded with z = 0
+    np.testing.assert_array_equal(c.probes, [[0.0, 0.5, 0.0], [1.0, 0.5, 0.0], [0.5, 0.5, 0.0]])
+    # initial Ez = exp(-(x - 0.5)^2 / (2 0.12^2)) at the triangle nodes
+    from oracle import oracle2d as O2
+    ref = O2.Operator2D(c.vx, c.EToV, c.face_bc, c.order)
+    x = c.initial_state().reshape(6, c.K, -1)
+    np.testing.assert_allclose(x[2], np.exp(-(ref.pos[..., 0] - 0.5) ** 2 / (2 * 0.12 ** 2)), atol=1e-14)
+    assert np.abs(x[[0, 1, 3, 4, 5]]).max() == 0.0
+
+
+def test_2d_snapshot_mesh_round_trip(tmp_path):
+    c = D.Case(PEC2D)
+    c.save_mesh(tmp_path / "mesh_rank0")
+    text = (tmp_path / "mesh_rank0").read_text()
+    assert "dimension\n2\n" in text
+    # the written mesh reads back through the MFEM reader as the same triangles
+    case = json.load(open(PEC2D))
+    case["model"]["filename"] = str(tmp_path / "mesh.mesh")
+    os.rename(tmp_path / "mesh_rank0", tmp_path / "mesh.mesh")
+    (tmp_path / "c.json").write_text(json.dumps(case))
+    c2 = D.Case(str(tmp_path / "c.json"))
+    assert c2.dim == 2
+    np.testing.assert_array_equal(c2.EToV, c.EToV)
+    np.testing.assert_allclose(c2.vx, c.vx, atol=1e-15)
+    np.testing.assert_array_equal(c2.face_bc, c.face_bc)
+
+
+def test_2d_case_refusals(tmp_path):
+    case = json.load(open(PEC2D))
+    shutil.copy(os.path.join(CASES, "2D_PEC.msh"), tmp_path / "2D_PEC.msh")
+    case["model"]["refinement"] = 1
+    (tmp_path / "r.json").write_text(json.dumps(case))
+    with pytest.raises(D.DgtdError, match="refinement of triangle meshes"):
+        D.Case(str(tmp_path / "r.json"))

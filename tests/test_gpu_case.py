@@ -5,6 +5,7 @@ mesh; point probes against the nodal interpolant of the returned state; the
 C++ launcher's PointProbe files against the same numbers."""
 import json
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -125,3 +126,81 @@ def test_launcher_tfsf_case(tmp_path):
     rows = np.array([[float(v) for v in ln.split()] for ln in
                      (out / "PointProbes" / "PointProbe0.dat").read_text().splitlines()[4:]])
     assert rows.shape[0] == 2501 and np.abs(rows[:, 1:]).max() < 1e-3
+
+
+PEC2D = os.path.join(ROOT, "tests", "golden", "cases", "2D_PEC.json")
+
+
+def _case_variant(tmp_path, src, **opts):
+    case = json.load(open(src))
+    case["solver_options"].update(opts)
+    d = os.path.dirname(src)
+    for f in os.listdir(d):
+        if f.startswith(os.path.splitext(os.path.basename(src))[0]) and f.endswith(".msh"):
+            shutil.copy(os.path.join(d, f), tmp_path / f)
+    path = tmp_path / os.path.basename(src)
+    path.write_text(json.dumps(case))
+    return str(path)
+
+
+def test_2d_pec_case_fp64_rk4_matches_oracle():
+    from oracle import oracle2d as O2
+    c = D.Case(PEC2D)
+    x, info = D.run_case(PEC2D, precision=64, integrator="rk4", final_time=0.01)
+    assert info["steps"] == 10
+    ref = O2.Operator2D(c.vx, c.EToV, c.face_bc, c.order, alpha=c.info["alpha"])
+    xr, steps, t = ref.run(c.initial_state(), info["dt"], 0.01, "rk4")
+    assert steps == 10 and t == pytest.approx(info["time"], abs=1e-14)
+    assert rel(x, xr) <= 1e-11
+
+
+@pytest.mark.parametrize("alpha,dt,norm_tol", [(0.0, 1e-3, 1e-3), (1.0, 5e-3, 5e-3)])
+def test_2d_pec_hesthaven_like_reference(tmp_path, alpha, dt, norm_tol):
+    """ExtensiveCasesTest 2D_PEC_Centered_Hesthaven / 2D_PEC_Upwind_Hesthaven
+    (test/cases/ExtensiveCasesTest.cpp:205-270, 339-404): a 1D Gaussian in Ez
+    between PEC walls at x = 0, 1 (PMC at y = 0, 1) splits, reflects and
+    returns after t = 2; the probes on the PEC walls see E ~ 0 and H_y = +-1 at
+    t = 1/2, the centre probe E_z ~ 1 at t = 0 and t = 2 (tolerance 2e-2)."""
+    path = _case_variant(tmp_path, PEC2D, upwind_alpha=alpha, order=3, evolution_operator="hesthaven",
+                         time_step=dt)
+    c = D.Case(path)
+    op = c.operator(precision=64, integrator="rk4")
+    try:
+        x0 = c.initial_state()
+        op.set_state(x0, 0.0)
+        op.probe_set(c.probes)
+        movies = [(0.0, op.probe_sample())]
+        n = int(round(2.0 / dt))
+        for _ in range(n):
+            op.advance(dt, 1)
+            movies.append((op.GetTime(), op.probe_sample()))
+        x = op.get_state()
+    finally:
+        op.close()
+    tol = 2e-2
+    nd = x.size // 6
+    assert np.linalg.norm(x[2 * nd:3 * nd] - x0[2 * nd:3 * nd]) <= tol  # eOld.DistanceTo(eNew)
+    assert abs(np.linalg.norm(x) - np.linalg.norm(x0)) <= norm_tol
+    seen = 0
+    for t, f in movies:
+        for p, sign in ((0, 1.0), (1, -1.0)):
+            assert np.abs(f[p, :3]).max() <= tol
+            if abs(t - 0.5) <= 1e-3:
+                assert f[p, 4] == pytest.approx(sign * 1.0, abs=tol)
+                seen += 1
+        if abs(t) <= 1e-3 or abs(t - 2.0) <= 1e-3:
+            assert f[2, 2] == pytest.approx(1.0, abs=tol) and abs(f[2, 4]) <= tol
+    assert seen >= 2
+
+
+def test_launcher_runs_2d_case(tmp_path):
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, PEC2D, "--final-time", "0.05", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["steps"] == 50 and summary["K"] == 246 and summary["nonfinite"] == 0
+    x, info = D.run_case(PEC2D, precision=64, integrator="rk4", final_time=0.05)
+    assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-10)  # printed with 11 digits
+    rows = (out / "PointProbes" / "PointProbe2.dat").read_text().splitlines()
+    assert rows[0] == "PointProbe ID 2" and len(rows) == 4 + 51
