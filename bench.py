@@ -159,6 +159,47 @@ def bench_c3(D, args):
     print(json.dumps(out))
 
 
+def bench_tri(D, args):
+    """The 2D operator: PEC square [0,1]^2, 512^2 squares x 2 = 524,288
+    triangles, p=4 (Np=15), Ez Gaussian pulse, dt from the Hesthaven rule
+    (cfl 0.5), 200 LSERK4 steps in FP32 and FP64."""
+    n, order = 512, 4
+    steps = args.steps if args.steps != 100 else 200
+    vx, E, fbc = D.square_mesh(n, 1.0, "pec")
+    K = E.shape[0]
+    Np = (order + 1) * (order + 2) // 2
+    c = (0.43, 0.57, 0.0)
+    x0 = D.project_initial_fields(vx, E, order, [D.gaussian("E", c, 0.05, (0, 0, 1), dimension=2)])
+    dof = 6 * Np * K
+    out = {"workload": f"PEC square [0,1]^2, {n}^2 squares x 2 = {K} triangles, p={order} (Np={Np}), "
+                       f"{dof} DOF, Ez Gaussian at ({c[0]},{c[1]}), {steps} LSERK4 steps", "dof": dof}
+    states = {}
+    for prec in (32, 64):
+        op = D.HesthavenEvolution(vx, E, fbc, order, precision=prec, integrator="lserk4")
+        dt = 0.5 / (op.max_fscale() * order ** 2)
+        op.set_state(x0)
+        e0 = op.norms()["energy"]
+        op.advance(dt, args.warmup)
+        op.set_timing(True)
+        t0 = time.perf_counter()
+        op.advance(dt, steps)
+        wall = time.perf_counter() - t0
+        us, launches = op.last_kernel_us()
+        nm = op.norms()
+        b, f = op.work_per_element()
+        states[prec] = op.get_state()
+        out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
+                            "kernel_avg_us": us, "kernel": f"dg_tri_stage<{order},{'float' if prec == 32 else 'double'},LSERK>",
+                            "algorithmic_bytes_per_launch": b * K,
+                            "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                            "achieved_tflops": f * K / (us * 1e-6) / 1e12,
+                            "energy_initial": e0, "energy_final": nm["energy"], "nonfinite": nm["nonfinite"]}
+        op.close()
+    out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(states[32] - states[64]) / np.linalg.norm(states[64]))
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    print(json.dumps(out))
+
+
 def bench_c4(D, args):
     """64 independent PEC cavities, edge n_i log-uniform over {2..24} hexes,
     order uniform over {1..5}, seeded Gaussian pulses, FP32, one persistent
@@ -236,9 +277,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "tri"],
                     help="c2 (default, the headline): 32^3 cavity p=4; c3: 48^3 SMA box with random eps, "
-                         "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream")
+                         "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream; "
+                         "tri: the 2D operator on 524,288 triangles, p=4")
     ap.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
@@ -266,6 +308,8 @@ def main():
         return bench_c4(D, args)
     if args.workload == "c5":
         return bench_c5(D, args)
+    if args.workload == "tri":
+        return bench_tri(D, args)
     n, order = args.n, args.order
     vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
     K = E.shape[0]

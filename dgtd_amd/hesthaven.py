@@ -42,6 +42,33 @@ def cube_mesh(n, L=1.0, bc="pec"):
     return vx.reshape(-1, 3), EToV.reshape(-1, 4), face_bc.reshape(-1, 4)
 
 
+def square_mesh(n, L=1.0, bc="pec"):
+    """n^2 squares on [0,L]^2, two counter-clockwise triangles each, split
+    along the (0,0)-(1,1) diagonal of every square (library generator for
+    the 2D operator).  bc: one of none/pec/pmc/sma for all sides or a list of
+    four (x=0, x=L, y=0, y=L).  Returns (vx (nv,3) with z = 0, EToV (K,3),
+    face_bc (K,3)), edge f joining local vertices f and f+1 mod 3."""
+    names = {"none": 0, "pec": 1, "pmc": 2, "sma": 3}
+    sides = [bc] * 4 if isinstance(bc, (str, int)) else list(bc)
+    side = [names[s] if isinstance(s, str) else int(s) for s in sides]
+    g = np.linspace(0.0, L, n + 1)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    vx = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    a = (i + (n + 1) * j).ravel()
+    b, c, d = a + 1, a + n + 2, a + n + 1
+    EToV = np.empty((2 * a.size, 3), dtype=np.int32)
+    EToV[0::2] = np.stack([a, b, c], axis=1)  # lower-right triangle
+    EToV[1::2] = np.stack([a, c, d], axis=1)  # upper-left triangle
+    face_bc = np.zeros(EToV.shape, dtype=np.int8)
+    ii, jj = i.ravel(), j.ravel()
+    face_bc[0::2, 0] = np.where(jj == 0, side[2], 0)          # a-b on y = 0
+    face_bc[0::2, 1] = np.where(ii == n - 1, side[1], 0)      # b-c on x = L
+    face_bc[1::2, 1] = np.where(jj == n - 1, side[3], 0)      # c-d on y = L
+    face_bc[1::2, 2] = np.where(ii == 0, side[0], 0)          # d-a on x = 0
+    return vx, EToV, face_bc
+
+
 class _Mesh:
     """Keeps the numpy arrays alive while the C descriptor points at them."""
 
