@@ -231,8 +231,6 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
     if (!m.contains("boundaries")) throw std::runtime_error("JSON data does not include 'boundaries'.");
     int max_attr = 0;
     for (int a : cs.mesh.bdr_attr()) max_attr = std::max(max_attr, a);
-    if (cs.mesh.dim == 2 && !cs.waves.empty())
-        throw Unsupported("TF/SF plane waves are not supported on triangle meshes");
     std::vector<int> attr_bc(max_attr + 1, -1);
     const dgtd::Json& bs = m.at("boundaries");
     for (size_t i = 0; i < bs.size(); ++i) {
@@ -270,14 +268,18 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
                                     w.propagation[2] * w.propagation[2]);
         double min_phase = 0.0;
         bool any = false;
-        for (size_t t = 0; t < cs.mesh.tris.size(); ++t) {
-            const int a = cs.mesh.tri_attr[t];
+        const bool two_d = cs.mesh.dim == 2;  // TF/SF boundary elements: segments in 2D, triangles in 3D
+        const size_t nb = two_d ? cs.mesh.segs.size() : cs.mesh.tris.size();
+        for (size_t t = 0; t < nb; ++t) {
+            const int a = two_d ? cs.mesh.seg_attr[t] : cs.mesh.tri_attr[t];
             if (std::find(cs.tfsf_tags.begin(), cs.tfsf_tags.end(), a) == cs.tfsf_tags.end()) continue;
+            const int nvt = two_d ? 2 : 3;
             double ph = 0.0;
             for (int d = 0; d < 3; ++d) {
                 double c = 0.0;
-                for (int k = 0; k < 3; ++k) c += cs.mesh.vx[3 * (size_t)cs.mesh.tris[t][k] + d];
-                ph += c / 3.0 * w.propagation[d] / nd;
+                for (int k = 0; k < nvt; ++k)
+                    c += cs.mesh.vx[3 * (size_t)(two_d ? cs.mesh.segs[t][k] : cs.mesh.tris[t][k]) + d];
+                ph += c / nvt * w.propagation[d] / nd;
             }
             min_phase = any ? std::min(min_phase, ph) : ph;
             any = true;

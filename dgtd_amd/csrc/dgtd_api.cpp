@@ -198,6 +198,7 @@ struct dgtd_op {
     {
         if (ntfsf == 0 || nwaves == 0) return;
         dgtd::TfsfArgs<T> a{};
+        a.dim = dim;
         a.nelem = ntfsf;
         a.elem = d_tfsf_elem;
         a.sign = d_tfsf_sign;
@@ -429,6 +430,39 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh)
     for (int b = 0; b < nbuf; ++b) {
         hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
         hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
+    }
+    if (!md.tfsf.empty()) {  // triangles owning TF/SF edges: edge signs, edge-node positions, records
+        std::vector<int> elems, signs;
+        std::vector<double> xyz, geo;
+        std::vector<int> slot(op->K, -1);
+        for (const auto& t : md.tfsf) {
+            const int e = t[0];
+            if (slot[e] < 0) {
+                slot[e] = (int)elems.size();
+                elems.push_back(e);
+                signs.insert(signs.end(), {0, 0, 0, 0});
+                geo.insert(geo.end(), md.geo.begin() + (size_t)dgtd::kTriGeoStride * e,
+                           md.geo.begin() + (size_t)dgtd::kTriGeoStride * (e + 1));
+                const int32_t* v = &mesh->EToV[3 * (size_t)e];
+                for (int m = 0; m < 3 * R.Nfp; ++m) {
+                    const int n = R.Fmask[m];
+                    const double a = R.xi[2 * n], b = R.xi[2 * n + 1];
+                    for (int d = 0; d < 3; ++d) {
+                        const double x0 = mesh->vx[3 * (size_t)v[0] + d];
+                        xyz.push_back(d == 2 ? 0.0
+                                             : x0 + (mesh->vx[3 * (size_t)v[1] + d] - x0) * a +
+                                                   (mesh->vx[3 * (size_t)v[2] + d] - x0) * b);
+                    }
+                }
+            }
+            signs[4 * (size_t)slot[e] + t[1]] = t[2];
+        }
+        op->ntfsf = (int)elems.size();
+        op->d_tfsf_elem = static_cast<int*>(upload(elems));
+        op->d_tfsf_sign = static_cast<int*>(upload(signs));
+        op->d_tfsf_xyz = static_cast<double*>(upload(xyz));
+        op->d_tfsf_geo = static_cast<double*>(upload(geo));
+        op->d_lift = static_cast<double*>(upload(R.LIFT));
     }
     hip_check(hipStreamSynchronize(op->stream), "sync");
     hipDeviceProp_t prop;
@@ -820,7 +854,6 @@ int dgtd_op_norms(dgtd_op* op, double* out3)
 int dgtd_op_set_planewaves(dgtd_op* op, const dgtd_planewave* w, int32_t n)
 {
     if (!op || n < 0 || (n > 0 && !w)) return fail(DGTD_E_ARG, "NULL argument or negative count");
-    if (op->dim == 2 && n > 0) return fail(DGTD_E_UNSUPPORTED, "TF/SF plane waves are not available on triangles");
     std::vector<dgtd::TfsfWave> dev((size_t)n);
     for (int i = 0; i < n; ++i) {
         const dgtd_planewave& p = w[i];

@@ -134,12 +134,13 @@ struct TfsfWave {
 
 template <typename T>
 struct TfsfArgs {
+    int dim;             // 3: tetrahedra (4 faces, 27-value records); 2: triangles (3 edges, 15 values)
     int nelem;           // elements owning TF/SF faces
     const int* elem;     // [nelem] element ids
     const int* sign;     // [4*nelem] per face: -1 scattered side, +1 total side, 0
-    const double* xyz;   // [nelem][4*Nfp][3] face node positions
-    const double* geo;   // [nelem][27] FP64 geometry records
-    const double* lift;  // [Np][4*Nfp]
+    const double* xyz;   // [nelem][nfaces*Nfp][3] face node positions
+    const double* geo;   // [nelem][27 or 15] FP64 geometry records
+    const double* lift;  // [Np][nfaces*Nfp]
     const TfsfWave* waves;
     int nwaves;
     int Np, Nfp;

@@ -368,12 +368,16 @@ std::vector<int8_t> edge_codes(const TetMesh& m, const std::vector<int>& attr_to
         const int attr = m.seg_attr[i];
         const int code = (attr >= 0 && attr < (int)attr_to_bc.size()) ? attr_to_bc[attr] : -1;
         if (code < 0) continue;
-        if (code == 7) throw std::runtime_error("TF/SF surfaces are not supported on triangle meshes");
         const int64_t a = m.segs[i][0], b = m.segs[i][1];
         auto it = edges.find(std::min(a, b) * nv + std::max(a, b));
         if (it == edges.end())
             throw std::runtime_error("boundary segment with attribute " + std::to_string(attr) + " is no element edge");
-        if (it->second.second >= 0) {
+        if (code == 7) {  // TF/SF line: interior edges only
+            if (it->second.second < 0)
+                throw std::runtime_error("TF/SF attribute " + std::to_string(attr) + " lies on a boundary edge");
+            bc[it->second.first] = 7;
+            bc[it->second.second] = 7;
+        } else if (it->second.second >= 0) {
             bc[it->second.first] = (int8_t)(code + 3);
             bc[it->second.second] = (int8_t)(code + 3);
         } else {

@@ -9,7 +9,8 @@
 // on the elements that own TF/SF faces; this kernel adds c_out * S to the
 // stage output and c_res * S to the residual after the fused stage kernel,
 // with the coefficients the stage applies to the right-hand side.  One
-// 64-thread block per such element, FP64 arithmetic.
+// 64-thread block per such element, FP64 arithmetic.  On triangles (dim 2)
+// the faces are the three edges and n_z = 0.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -26,8 +27,9 @@ __global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
     __shared__ double flux[6][kTfsfMaxFaceNodes];
     const int k = blockIdx.x;
     const int e = a.elem[k];
-    const int Np = a.Np, Nfp = a.Nfp, Nf = 4 * Nfp;
-    const double* g = a.geo + (size_t)k * 27;
+    const bool tri = a.dim == 2;
+    const int Np = a.Np, Nfp = a.Nfp, Nf = (tri ? 3 : 4) * Nfp;
+    const double* g = a.geo + (size_t)k * (tri ? 15 : 27);
     for (int m = threadIdx.x; m < Nf; m += 64) {
         const int f = m / Nfp;
         const int sg = a.sign[4 * k + f];
@@ -46,8 +48,11 @@ __global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
                 }
             }
         }
-        const double n[3] = {g[9 + 4 * f], g[10 + 4 * f], g[11 + 4 * f]};
-        const double fs = g[12 + 4 * f];
+        // records: tetrahedron [9+4f..12+4f] = nx ny nz fscale/2; triangle
+        // [4+3f..6+3f] = nx ny fscale/2
+        const double n[3] = {tri ? g[4 + 3 * f] : g[9 + 4 * f], tri ? g[5 + 3 * f] : g[10 + 4 * f],
+                             tri ? 0.0 : g[11 + 4 * f]};
+        const double fs = tri ? g[6 + 3 * f] : g[12 + 4 * f];
         const double ndE = n[0] * dE[0] + n[1] * dE[1] + n[2] * dE[2];
         const double ndH = n[0] * dH[0] + n[1] * dH[1] + n[2] * dH[2];
         for (int x = 0; x < 3; ++x) {
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
         for (int c = 0; c < 6; ++c) {
             double s = 0.0;
             for (int m = 0; m < Nf; ++m) s += a.lift[(size_t)i * Nf + m] * flux[c][m];
-            s *= c < 3 ? g[25] : g[26];
+            s *= c < 3 ? g[tri ? 13 : 25] : g[tri ? 14 : 26];
             const size_t idx = c * a.N + (size_t)e * Np + i;
             a.out[idx] = (T)((double)a.out[idx] + a.c_out * s);
             if (a.res) a.res[idx] = (T)((double)a.res[idx] + a.c_res * s);

@@ -252,18 +252,38 @@ TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha
             nbr[3 * (size_t)e2 + f2] = e;
             code[3 * (size_t)e2 + f2] = (f2 * 3 + f) * 2 + c;
         }
+    // effective codes: an interior condition or TF/SF code given on one side
+    // of an interior edge applies to both sides
+    std::vector<int> eff(3 * (size_t)K, 0);
+    for (size_t i = 0; i < eff.size(); ++i) {
+        eff[i] = in.face_bc ? in.face_bc[i] : 0;
+        if (eff[i] < 0 || eff[i] > 7) throw std::runtime_error("face_bc code out of range");
+    }
     for (int e = 0; e < K; ++e)
         for (int f = 0; f < 3; ++f) {
             const size_t i = 3 * (size_t)e + f;
-            const int bc = in.face_bc ? in.face_bc[i] : 0;
-            if (bc < 0 || bc > 7) throw std::runtime_error("face_bc code out of range");
-            if (bc == 7) throw std::runtime_error("TF/SF faces are not supported on triangles");
+            if (nbr[i] < 0 || eff[i] < 4) continue;
+            const size_t j = 3 * (size_t)nbr[i] + (code[i] / 2) % 3;
+            if (eff[j] != 0 && eff[j] != eff[i])
+                throw std::runtime_error("conflicting interior conditions on the two sides of an edge");
+            eff[j] = eff[i];
+        }
+    for (int e = 0; e < K; ++e)
+        for (int f = 0; f < 3; ++f) {
+            const size_t i = 3 * (size_t)e + f;
+            const int bc = eff[i];
             int32_t* cr = &md.conn[(size_t)kTriConnStride * e];
             if (nbr[i] < 0) {
-                if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-6 on a boundary edge");
+                if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-7 on a boundary edge");
                 if (bc == 3 && alpha == 0.0) throw std::runtime_error("SMA boundaries need upwind flux (alpha != 0)");
                 cr[f] = -(bc + 1);
                 cr[3 + f] = 0;
+            } else if (bc == 7) {
+                // TF/SF edge: normal neighbour gather, plus the injected plane
+                // waves; the lower-index element is the scattered-field side
+                cr[f] = nbr[i];
+                cr[3 + f] = code[i];
+                md.tfsf.push_back({e, f, e < nbr[i] ? -1 : 1});
             } else if (bc >= 4) {
                 // interior condition: both sides use the own trace (no gather)
                 cr[f] = -(bc + 1);
@@ -275,20 +295,6 @@ TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha
                 cr[3 + f] = code[i];
             }
             md.face_code[i] = (int8_t)bc;
-        }
-    // an interior condition given on one side applies to both
-    for (int e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            const size_t i = 3 * (size_t)e + f;
-            if (md.face_code[i] < 4 || nbr[i] < 0) continue;
-            const int e2 = nbr[i], f2 = (code[i] / 2) % 3;
-            const size_t j = 3 * (size_t)e2 + f2;
-            if (md.face_code[j] != 0 && md.face_code[j] != md.face_code[i])
-                throw std::runtime_error("conflicting interior conditions on the two sides of an edge");
-            md.face_code[j] = md.face_code[i];
-            int32_t* cr = &md.conn[(size_t)kTriConnStride * e2];
-            cr[f2] = -(md.face_code[i] + 1);
-            cr[3 + f2] = 0;
         }
     return md;
 }

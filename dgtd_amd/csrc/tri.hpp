@@ -6,6 +6,7 @@
 // layout; only the element, its faces and its geometry change.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -64,11 +65,12 @@ struct TriMeshData {
     std::vector<double> eps, mu;
     double max_fscale = 0.0;
     std::vector<int8_t> face_code;  // 3K
+    std::vector<std::array<int, 3>> tfsf;  // TF/SF edge sides: (element, edge, -1 scattered / +1 total)
 };
 
 // Throws std::runtime_error on invalid input (same rules as the tetrahedral
-// mesh: codes 1-3 only on boundary edges, 4-6 only on interior edges; TF/SF
-// (code 7) is not available on triangles).
+// mesh: codes 1-3 only on boundary edges, 4-7 only on interior edges; a TF/SF
+// edge keeps its neighbour and its lower-index element is the scattered side).
 TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha);
 
 // Physical node positions: 3 blocks of Np*K (z block zero).

@@ -212,6 +212,14 @@ class Operator2D:
                     nbr[e, f], nbrf[e, f], nbr[e2, f2], nbrf[e2, f2] = e2, f2, e, f
                 else:
                     owner[key] = (e, f)
+        # an interior condition or TF/SF code on one side applies to both
+        for e in range(K):
+            for f in range(3):
+                if nbr[e, f] >= 0 and bc[e, f] >= 4:
+                    e2, f2 = nbr[e, f], nbrf[e, f]
+                    if bc[e2, f2] not in (0, bc[e, f]):
+                        raise ValueError("conflicting interior conditions on the two sides of an edge")
+                    bc[e2, f2] = bc[e, f]
         for e in range(K):
             for f in range(3):
                 if nbr[e, f] >= 0 and 1 <= bc[e, f] <= 3:
@@ -240,8 +248,15 @@ class Operator2D:
                 if np.any(d[np.arange(Nfp), b2] > 1e-7 * h):
                     raise ValueError("non-conforming edge: node not matched")
                 self.vmapP[e, f] = e2 * R.Np + R.Fmask[f2][b2]
+        # TF/SF edges (code 7): normal neighbour jumps plus the incident field,
+        # -inc on the scattered side (the lower-index element, MFEM's first
+        # element of the face) and +inc on the total side
+        # (HesthavenEvolution::evaluateTFSF, HesthavenEvolution.cpp:97-124)
+        self.tfsf = np.where(bc == 7, np.where(np.arange(K)[:, None] < nbr, -1, 1), 0)
+        bc = np.where(bc == 7, 0, bc)
         self.bc = bc
         self.nbr = nbr
+        self.waves = []
 
     def ndofs(self):
         return 6 * self.Np * self.K
@@ -252,8 +267,36 @@ class Operator2D:
     def node_positions(self):
         return self.pos
 
-    def mult(self, x):
-        """HesthavenEvolution::Mult with d/dz = 0 and n_z = 0."""
+    def set_planewaves(self, waves):
+        """waves: list of dicts kind ('gaussian' | 'modulated'), polarization,
+        propagation, spread, mean (projected, k.mean), frequency."""
+        out = []
+        for w in waves:
+            pol = np.asarray(w["polarization"], dtype=float)
+            k = np.asarray(w["propagation"], dtype=float)
+            pol, k = pol / np.linalg.norm(pol), k / np.linalg.norm(k)
+            out.append(dict(pol=pol, k=k, kxp=np.cross(k, pol), spread=float(w["spread"]),
+                            mean=float(w.get("mean", 0.0)), freq=float(w.get("frequency", 0.0)),
+                            modulated=w.get("kind", "gaussian") == "modulated"))
+        self.waves = out
+
+    def incident(self, pos, t):
+        """E_inc, H_inc (each (3, ...)) of the plane waves at positions pos
+        (..., 2): g(u) pol and g(u) k x pol with u = k.x - t - mean
+        (Planewave, src/math/Function.h:331-374)."""
+        E = np.zeros((3,) + pos.shape[:-1])
+        H = np.zeros_like(E)
+        for w in self.waves:
+            u = w["k"][0] * pos[..., 0] + w["k"][1] * pos[..., 1] - t - w["mean"]
+            g = np.exp(-u * u / (2 * w["spread"] ** 2))
+            if w["modulated"]:
+                g = g * np.cos(2 * np.pi * w["freq"] * u)
+            E += np.multiply.outer(w["pol"], g)
+            H += np.multiply.outer(w["kxp"], g)
+        return E, H
+
+    def mult(self, x, t=0.0):
+        """HesthavenEvolution::Mult with d/dz = 0 and n_z = 0, at time t."""
         R, K, Np, Nfp = self.R, self.K, self.Np, self.Nfp
         q = np.asarray(x, dtype=float).reshape(6, K * Np)
         M, P = self.vmapM.reshape(K, -1), self.vmapP.reshape(K, -1)
@@ -268,6 +311,13 @@ class Operator2D:
         for c in range(3):
             dF[c] = np.where(ovr, ce * own[c], dF[c])
             dF[3 + c] = np.where(ovr, ch * own[3 + c], dF[3 + c])
+        if self.waves and np.any(self.tfsf):
+            sg = np.repeat(self.tfsf, Nfp, axis=1)  # (K, 3Nfp)
+            pM = self.pos.reshape(-1, 2)[M.reshape(-1)].reshape(K, 3 * Nfp, 2)
+            Ei, Hi = self.incident(pM, t)
+            for c in range(3):
+                dF[c] = dF[c] + sg * Ei[c]
+                dF[3 + c] = dF[3 + c] + sg * Hi[c]
         n = np.repeat(self.nrm, Nfp, axis=1)  # (K, 3Nfp, 2)
         nv = [n[..., 0], n[..., 1], np.zeros_like(n[..., 0])]
         ndE = nv[0] * dF[0] + nv[1] * dF[1] + nv[2] * dF[2]
@@ -293,20 +343,20 @@ class Operator2D:
             out[x] = eres / self.eps[:, None]
         return out.reshape(-1)
 
-    def rk4_step(self, x, dt):
-        """mfem::RK4Solver stage order (Solver.cpp:535-551)."""
-        k = self.mult(x)
+    def rk4_step(self, x, dt, t=0.0):
+        """mfem::RK4Solver stage order and stage times (Solver.cpp:535-551)."""
+        k = self.mult(x, t)
         y, z = x + dt / 2 * k, x + dt / 6 * k
-        k = self.mult(y)
+        k = self.mult(y, t + dt / 2)
         y, z = x + dt / 2 * k, z + dt / 3 * k
-        k = self.mult(y)
+        k = self.mult(y, t + dt / 2)
         y, z = x + dt * k, z + dt / 3 * k
-        k = self.mult(y)
+        k = self.mult(y, t + dt)
         return z + dt / 6 * k
 
-    def lserk4_step(self, q, res, dt):
+    def lserk4_step(self, q, res, dt, t=0.0):
         for s in range(5):
-            k = self.mult(q)
+            k = self.mult(q, t + RK4C[s] * dt)
             res = RK4A[s] * res + dt * k
             q = q + RK4B[s] * res
         return q, res
@@ -319,9 +369,9 @@ class Operator2D:
         while t <= final_time - 1e-8 * dt:
             h = min(dt, final_time - t)
             if integrator == "rk4":
-                x = self.rk4_step(x, h)
+                x = self.rk4_step(x, h, t)
             else:
-                x, res = self.lserk4_step(x, res, h)
+                x, res = self.lserk4_step(x, res, h, t)
             t += h
             steps += 1
         return x, steps, t

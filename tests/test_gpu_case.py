@@ -204,3 +204,71 @@ def test_launcher_runs_2d_case(tmp_path):
     assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-10)  # printed with 11 digits
     rows = (out / "PointProbes" / "PointProbe2.dat").read_text().splitlines()
     assert rows[0] == "PointProbe ID 2" and len(rows) == 4 + 51
+
+
+TFSF2D = os.path.join(ROOT, "tests", "golden", "cases", "2D_TFSF.json")
+TFSF2D_INT = os.path.join(ROOT, "tests", "golden", "cases", "2D_TFSF_IntBoundary.json")
+EX, EY, EZ, HX, HY, HZ = range(6)
+
+
+def _probe_movies(path, dt=None):
+    c = D.Case(path)
+    op = c.operator(precision=64, integrator="rk4")
+    try:
+        dt = c.time_step(op) if dt is None else dt
+        x0 = c.initial_state()
+        op.set_state(x0, 0.0)
+        op.probe_set(c.probes)
+        movies = [(0.0, op.probe_sample())]
+        T = c.info["final_time"]
+        while op.GetTime() <= T - 1e-8 * dt:
+            op.advance(min(dt, T - op.GetTime()), 1)
+            movies.append((op.GetTime(), op.probe_sample()))
+        x = op.get_state()
+    finally:
+        op.close()
+    return x0, x, movies
+
+
+def _check_movies(movies, zero_probes, expect, tol):
+    hits = 0
+    for t, f in movies:
+        for p in zero_probes:
+            assert np.abs(f[p, [EX, EZ, HX, HY]]).max() <= tol, (t, p, f[p])
+        for p, te, want in expect:
+            if abs(t - te) <= 1e-3:
+                for comp, v in want.items():
+                    assert f[p, comp] == pytest.approx(v, abs=tol), (t, p, comp, f[p])
+                hits += 1
+    assert hits >= len(expect)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.0])
+def test_2d_tfsf_hesthaven_like_reference(tmp_path, alpha):
+    """ExtensiveCasesTest 2D_TFSF_Centered_Hesthaven / 2D_TFSF_Upwind_Hesthaven
+    (test/cases/ExtensiveCasesTest.cpp:533-611, 691-769): a Gaussian plane
+    wave (E along y) enters through the TF/SF line and bounces in the PEC/PMC
+    guide; tolerance 1e-2."""
+    path = _case_variant(tmp_path, TFSF2D, upwind_alpha=alpha, evolution_operator="hesthaven")
+    x0, x, movies = _probe_movies(path)
+    assert np.linalg.norm(x0) == 0.0
+    assert movies[-1][0] + 1e-2 > 9.5
+    _check_movies(movies, (0, 1, 2),
+                  [(0, 9.8, {HZ: 0.0, EY: 2.0}), (1, 2.8, {HZ: 1.0, EY: 1.0}), (1, 8.8, {HZ: -1.0, EY: 1.0}),
+                   (2, 5.8, {HZ: 0.0, EY: 2.0})], 1e-2)
+
+
+def test_2d_tfsf_interior_pec_like_reference(tmp_path):
+    """ExtensiveCasesTest 2D_TFSF_InteriorPEC_Hesthaven (:85-200): the wave is
+    injected, reflected by an interior PEC line and leaves through SMA; behind
+    the interior PEC the field stays at rest; tolerance 2e-2."""
+    path = _case_variant(tmp_path, TFSF2D_INT, evolution_operator="hesthaven")
+    x0, x, movies = _probe_movies(path)
+    tol = 2e-2
+    assert np.linalg.norm(x0) <= tol
+    assert abs(np.linalg.norm(x) - np.linalg.norm(x0)) <= tol
+    _check_movies(movies, (1, 3),
+                  [(1, 0.0, {EY: 0.0, HZ: 0.0}), (3, 0.0, {EY: 0.0, HZ: 0.0}),
+                   (1, 2.1, {EY: 1.0, HZ: 1.0}), (3, 2.1, {EY: 0.0, HZ: 0.0}),
+                   (2, 2.9, {EY: 0.0, HZ: 2.0}), (3, 2.9, {EY: 0.0, HZ: 0.0}),
+                   (1, 3.7, {EY: -1.0, HZ: 1.0}), (3, 3.7, {EY: 0.0, HZ: 0.0})], tol)

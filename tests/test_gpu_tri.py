@@ -131,10 +131,58 @@ def test_tri_refusals():
     vx, E, bc = G.square_mesh(2, bc="pec")
     with pytest.raises(D.DgtdError):
         D.HesthavenEvolution(vx, E, bc, 2, kernel="mfma")
-    gpu = D.HesthavenEvolution(vx, E, bc, 2)
-    with pytest.raises(D.DgtdError):
-        gpu.set_planewaves([D.planewave((0, 0, 1), (1, 0, 0), 0.1)])
     bad = bc.copy()
     bad[bad == 0] = 1  # PEC on interior edges
     with pytest.raises(D.DgtdError):
         D.HesthavenEvolution(vx, E, bad, 2)
+    tf = bc.copy()
+    tf[bc == 1] = 7  # TF/SF on boundary edges
+    with pytest.raises(D.DgtdError):
+        D.HesthavenEvolution(vx, E, tf, 2)
+
+
+def _tfsf_case(n=8, order=3, alpha=1.0):
+    vx, E, bc = G.square_mesh(n, L=1.0, bc=["sma", "sma", "pec", "pec"], jitter=0.0)
+    vx = vx.copy()
+    vx[:, 0] *= 4.0
+    for e in range(E.shape[0]):
+        for f, (p, q) in enumerate(G.TRI_EDGES):
+            if abs(vx[E[e, p], 0] - 1.0) < 1e-12 and abs(vx[E[e, q], 0] - 1.0) < 1e-12:
+                bc[e, f] = 7
+    waves = [dict(polarization=(0, 0.6, 0.8), propagation=(1, 0.2, 0), spread=0.4, mean=-0.5),
+             dict(polarization=(0, 1, 0), propagation=(1, 0, 0), spread=0.3, mean=0.2, kind="modulated",
+                  frequency=0.8)]
+    gw = [D.planewave(w["polarization"], w["propagation"], w["spread"], w["mean"], w.get("frequency"))
+          for w in waves]
+    return vx, E, bc, waves, gw
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_tri_tfsf_mult_matches_oracle(order):
+    vx, E, bc, waves, gw = _tfsf_case(order=order)
+    ref = O.Operator2D(vx, E, bc, order, alpha=0.7)
+    ref.set_planewaves(waves)
+    gpu = D.HesthavenEvolution(vx, E, bc, order, alpha=0.7, precision=64)
+    gpu.set_planewaves(gw)
+    x = np.random.default_rng(3).standard_normal(ref.ndofs())
+    for t in (0.0, 0.8, 1.7):
+        gpu.SetTime(t)
+        y = np.empty_like(x)
+        gpu.Mult(x, y)
+        yr = ref.mult(x, t)
+        assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("integrator", ["lserk4", "rk4"])
+def test_tri_tfsf_run_matches_oracle(integrator):
+    vx, E, bc, waves, gw = _tfsf_case(order=3)
+    ref = O.Operator2D(vx, E, bc, 3)
+    ref.set_planewaves(waves)
+    dt = 0.4 / (ref.max_fscale() * 9)
+    xr, steps, t = ref.run(np.zeros(ref.ndofs()), dt, 1.0, integrator)
+    for prec, tol in ((64, 1e-11), (32, 1e-5)):
+        gpu = D.HesthavenEvolution(vx, E, bc, 3, precision=prec, integrator=integrator)
+        gpu.set_planewaves(gw)
+        gpu.set_state(np.zeros(ref.ndofs()))
+        assert gpu.run(dt, 1.0) == steps
+        assert rel(gpu.get_state(), xr) <= tol

@@ -199,3 +199,43 @@ def test_product_projection_and_location_match_oracle_positions():
         u = op.pos[..., 0] ** p + 2 * op.pos[..., 0] * op.pos[..., 1] ** (p - 1)
         got = np.einsum("pn,pn->p", w, u[elem])
         np.testing.assert_allclose(got, pts[:, 0] ** p + 2 * pts[:, 0] * pts[:, 1] ** (p - 1), atol=1e-11)
+
+
+def _waveguide_tfsf(n=16, length=4.0, x_tfsf=1.0, side_bc=("sma", "sma", "pec", "pec")):
+    vx, E, bc = G.square_mesh(n, L=1.0, bc=list(side_bc))
+    vx = vx.copy()
+    vx[:, 0] *= length
+    for e in range(E.shape[0]):
+        for f, (p, q) in enumerate(G.TRI_EDGES):
+            if abs(vx[E[e, p], 0] - x_tfsf) < 1e-12 and abs(vx[E[e, q], 0] - x_tfsf) < 1e-12:
+                bc[e, f] = 7
+    return vx, E, bc
+
+
+def test_tfsf_plane_wave_through_waveguide():
+    """A Gaussian plane wave (E along y, travelling along x) injected on the
+    TF/SF line x = 1 of a PEC-walled channel: the total-field side carries the
+    analytic incident wave, the scattered-field side stays at rest."""
+    vx, E, bc = _waveguide_tfsf()
+    op = O.Operator2D(vx, E, bc, 3)
+    op.set_planewaves([dict(polarization=(0, 1, 0), propagation=(1, 0, 0), spread=0.3, mean=-1.0)])
+    x, _, t = op.run(np.zeros(op.ndofs()), 0.4 / (op.max_fscale() * 9), 2.5, "lserk4")
+    q = x.reshape(6, op.K, op.Np)
+    Ei, Hi = op.incident(op.pos, t)
+    total = op.pos[..., 0].mean(axis=1) > 1.0
+    assert np.linalg.norm((q[1] - Ei[1])[total]) / np.linalg.norm(Ei[1][total]) < 3e-3
+    assert np.linalg.norm((q[5] - Hi[2])[total]) / np.linalg.norm(Hi[2][total]) < 3e-3
+    assert np.abs(q[:, ~total]).max() < 1e-3
+
+
+def test_tfsf_injection_is_additive():
+    """The injection is a state-independent term: L(x, t) = L0(x) + L(0, t)."""
+    vx, E, bc = _waveguide_tfsf(n=4)
+    op = O.Operator2D(vx, E, bc, 2, alpha=0.5)
+    op0 = O.Operator2D(vx, E, bc, 2, alpha=0.5)
+    op.set_planewaves([dict(polarization=(0, 0.6, 0.8), propagation=(1, 0.2, 0), spread=0.5, mean=0.3,
+                            kind="modulated", frequency=0.7)])
+    x = np.random.default_rng(0).standard_normal(op.ndofs())
+    t = 0.9
+    np.testing.assert_allclose(op.mult(x, t), op0.mult(x) + op.mult(np.zeros_like(x), t), atol=1e-12)
+    assert np.abs(op.mult(np.zeros_like(x), t)).max() > 1e-3
