@@ -107,6 +107,7 @@ SIGNATURES = {
     "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
     "dgtd_case_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave)]),
     "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),
+    "dgtd_op_set_graphs": (c_int32, [c_void_p, c_int32]),
     "dgtd_case_save_fields": (c_int32, [c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, c_double]),
     "dgtd_batch_create": (c_int32, [ctypes.POINTER(BatchOpts), ctypes.POINTER(c_void_p)]),
     "dgtd_batch_destroy": (None, [c_void_p]),

@@ -192,10 +192,26 @@ hipError_t tri_launch_one(const StageArgs<T>& a, int grid, hipStream_t s)
 {
     constexpr size_t bytes = TSmem<P, T>::bytes;
     auto kern = dg_tri_stage<P, T, MODE>;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), bytes, s, a);
     return hipGetLastError();
+}
+
+template <int P, typename T, int MODE>
+void tri_set_attr()
+{
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_tri_stage<P, T, MODE>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)TSmem<P, T>::bytes);
+}
+
+template <int P, typename T>
+void tri_prepare_order()
+{
+    tri_set_attr<P, T, kModeMult>();
+    tri_set_attr<P, T, kModeLserk>();
+    tri_set_attr<P, T, kModeRk1>();
+    tri_set_attr<P, T, kModeRk2>();
+    tri_set_attr<P, T, kModeRk3>();
+    tri_set_attr<P, T, kModeRk4>();
 }
 
 template <int P, typename T>
@@ -227,6 +243,19 @@ hipError_t launch_tri_stage(int order, const StageArgs<T>& a, int mode, int grid
     }
 }
 
+template <typename T>
+void tri_prepare(int order)
+{
+    switch (order) {
+        case 1: tri_prepare_order<1, T>(); break;
+        case 2: tri_prepare_order<2, T>(); break;
+        case 3: tri_prepare_order<3, T>(); break;
+        case 4: tri_prepare_order<4, T>(); break;
+        case 5: tri_prepare_order<5, T>(); break;
+        default: break;
+    }
+}
+
 int tri_elements_per_block(int order)
 {
     switch (order) {
@@ -255,6 +284,8 @@ size_t tri_smem_bytes(int order, int precision)
     return precision == 64 ? pick(double{}) : pick(float{});
 }
 
+template void tri_prepare<double>(int);
+template void tri_prepare<float>(int);
 template hipError_t launch_tri_stage<double>(int, const StageArgs<double>&, int, int, hipStream_t);
 template hipError_t launch_tri_stage<float>(int, const StageArgs<float>&, int, int, hipStream_t);
 

@@ -113,6 +113,15 @@ struct dgtd_op {
 
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // One time step replayed from a hipGraph (per LSERK buffer parity),
+    // captured for the current dt; not used while plane waves are set (the
+    // TF/SF correction takes the stage times as arguments).  Off by default:
+    // measured on the 8^3 and 32^3 cavities, replaying the graph is no faster
+    // than the five asynchronous launches (the stage kernels dominate).
+    bool graphs = false;
+    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+    double graph_dt = 0.0;
     double last_us = 0.0;
     int last_launches = 0;
 
@@ -129,6 +138,7 @@ struct dgtd_op {
                         (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
+        drop_graphs();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -262,7 +272,50 @@ struct dgtd_op {
         return 4;
     }
 
-    int step(double dt) { return precision == 64 ? one_step<double>(dt) : one_step<float>(dt); }
+    int step_eager(double dt) { return precision == 64 ? one_step<double>(dt) : one_step<float>(dt); }
+
+    void drop_graphs()
+    {
+        for (auto& g : step_graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    // A step: replayed from the captured graph of this dt and buffer parity
+    // (captured on first use), or launched directly.
+    int step(double dt)
+    {
+        if (!graphs || (ntfsf > 0 && nwaves > 0)) return step_eager(dt);
+        if (dt != graph_dt) {
+            drop_graphs();
+            graph_dt = dt;
+        }
+        const int par = integrator == DGTD_LSERK4 ? cur : 0;
+        int launches = integrator == DGTD_LSERK4 ? 5 : 4;
+        if (!step_graph[par]) {
+            const int cur0 = cur;
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                launches = step_eager(dt);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                cur = cur0;
+                throw;
+            }
+            hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&step_graph[par], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(e, "hipGraphInstantiate");
+            cur = cur0;  // captured, not run
+        }
+        hip_check(hipGraphLaunch(step_graph[par], stream), "hipGraphLaunch");
+        if (integrator == DGTD_LSERK4) cur ^= 1;  // five buffer flips
+        return launches;
+    }
 
     template <typename T>
     void mult_dev(const void* in, void* out, hipStream_t s, double tm)
@@ -471,6 +524,8 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh)
     op->nblk = (op->K + epb - 1) / epb;
     op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * 4));
     op->variant = DGTD_KERNEL_VALU;
+    if (op->precision == 64) dgtd::tri_prepare<double>(R.p);
+    else dgtd::tri_prepare<float>(R.p);
 }
 
 dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
@@ -960,6 +1015,20 @@ int dgtd_op_set_timing(dgtd_op* op, int32_t on)
 {
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     op->timing = on != 0;
+    return DGTD_OK;
+}
+
+int dgtd_op_set_graphs(dgtd_op* op, int32_t on)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        op->graphs = on != 0;
+        op->drop_graphs();
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
     return DGTD_OK;
 }
 

@@ -52,6 +52,10 @@ int elements_per_block(int order);
 template <typename T>
 hipError_t launch_tri_stage(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
 int tri_elements_per_block(int order);
+// Raises the dynamic-LDS limit of the triangle kernels of this order (at set-up,
+// outside any stream capture).
+template <typename T>
+void tri_prepare(int order);
 size_t tri_smem_bytes(int order, int precision);
 
 // MFMA variant (dg_mfma.hip): wave-tasks of kMfmaTaskElems elements,

@@ -319,6 +319,11 @@ class HesthavenEvolution:
         check(lib().dgtd_op_norms(self._h, dptr(out)))
         return {"norm2": out[0], "energy": out[1], "nonfinite": int(out[2])}
 
+    def set_graphs(self, on=True):
+        """Replay time steps from a captured HIP graph, or (the default)
+        launch every stage kernel directly."""
+        check(lib().dgtd_op_set_graphs(self._h, 1 if on else 0))
+
     def set_timing(self, on=True):
         check(lib().dgtd_op_set_timing(self._h, 1 if on else 0))
 

@@ -198,6 +198,12 @@ int dgtd_op_probe_sample(dgtd_op* op, double* out);
  * operator-kernel duration in microseconds over its launches, measured with
  * HIP events on the handle's stream (0 when timing is off). */
 int dgtd_op_set_timing(dgtd_op* op, int32_t on);
+
+/* on != 0: time steps of dgtd_op_advance / dgtd_op_run / dgtd_op_step are
+ * replayed from a HIP graph of one step (captured per dt on first use, not
+ * used while plane waves are set).  Default 0: every stage kernel is launched
+ * directly.  Results are identical either way. */
+int dgtd_op_set_graphs(dgtd_op* op, int32_t on);
 int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches);
 
 /* Stage-kernel variant in use (DGTD_KERNEL_VALU or DGTD_KERNEL_MFMA). */
