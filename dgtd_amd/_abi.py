@@ -21,7 +21,7 @@ DGTD_OK = 0
 DGTD_BC_NONE, DGTD_BC_PEC, DGTD_BC_PMC, DGTD_BC_SMA = 0, 1, 2, 3
 DGTD_BC_INT_PEC, DGTD_BC_INT_PMC, DGTD_BC_INT_SMA, DGTD_BC_TFSF = 4, 5, 6, 7
 DGTD_RK4, DGTD_LSERK4 = 0, 1
-DGTD_KERNEL_AUTO, DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA = 0, 1, 2
+DGTD_KERNEL_AUTO, DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA, DGTD_KERNEL_CSR = 0, 1, 2, 3
 
 
 class DgtdError(RuntimeError):
@@ -32,7 +32,8 @@ class DgtdError(RuntimeError):
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("K", c_int32), ("order", c_int32), ("vx", _dp), ("nv", c_int32),
-                ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp), ("dim", c_int32)]
+                ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp), ("dim", c_int32),
+                ("sigma", _dp)]
 
 
 class Opts(ctypes.Structure):
@@ -122,6 +123,8 @@ SIGNATURES = {
     "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),
     "dgtd_mesh_vmaps": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _i32p, _i32p, _i8p]),
     "dgtd_mesh_geometry": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _dp]),
+    "dgtd_global_assemble": (c_int32, [ctypes.POINTER(MeshDesc), c_double, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64), _i32p, _dp]),
     "dgtd_mesh_locate": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _dp, _i32p, _dp]),
 }
 

@@ -19,8 +19,9 @@ OPERATORS = {_abi.DGTD_OPERATOR_MAXWELL: "maxwell", _abi.DGTD_OPERATOR_GLOBAL: "
 
 class Case:
     """A loaded case: `info` (dict), mesh arrays `vx`, `EToV`, `face_bc`, `eps`,
-    `mu`, `sources` (InitialField list), `planewaves` (PlaneWave list, the
-    TF/SF faces carry code 7) and `probes` (n x 3)."""
+    `mu`, `sigma` (normalised conductivity, None if no material conducts),
+    `sources` (InitialField list), `planewaves` (PlaneWave list, the TF/SF
+    faces carry code 7) and `probes` (n x 3)."""
 
     def __init__(self, path):
         self.path = os.path.abspath(path)
@@ -42,6 +43,7 @@ class Case:
             self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(nvert * K,)).reshape(K, nvert).copy()
             self.eps = None if not d.eps_r else np.ctypeslib.as_array(d.eps_r, shape=(K,)).copy()
             self.mu = None if not d.mu_r else np.ctypeslib.as_array(d.mu_r, shape=(K,)).copy()
+            self.sigma = None if not d.sigma else np.ctypeslib.as_array(d.sigma, shape=(K,)).copy()
             n = ci.nsources
             arr = (_abi.InitialField * max(n, 1))()
             check(lib().dgtd_case_sources(h, arr))
@@ -98,9 +100,13 @@ class Case:
         return x
 
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
+        """The case's operator; conductive materials need the assembled one
+        ("csr"), which kernel="auto" then selects."""
+        if kernel == "auto" and self.sigma is not None:
+            kernel = "csr"
         op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
                                 eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
-                                kernel=kernel, device=device)
+                                kernel=kernel, device=device, sigma=self.sigma)
         if self.planewaves:
             op.set_planewaves(self.planewaves)
         return op

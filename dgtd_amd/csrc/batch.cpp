@@ -376,6 +376,8 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 {
     if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "batched instances are tetrahedral meshes");
+    if (mesh->sigma)
+        return dgtd::api_fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxBatchOrder)
         return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");

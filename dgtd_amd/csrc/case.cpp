@@ -2,6 +2,7 @@
 // reference's JSON launcher (driver.cpp:531-706, 940-1035, 1242-1300) that the
 // hesthaven path can run, read into plain arrays for dgtd_op_create.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,7 +24,7 @@ struct dgtd_case {
     dgtd::TetMesh mesh;
     std::vector<int32_t> EToV;
     std::vector<int8_t> face_bc;
-    std::vector<double> eps, mu;
+    std::vector<double> eps, mu, sigma;  // sigma: normalised conductivity, empty if none
     dgtd_case_info info{};
     std::vector<dgtd_initial_field> sources;
     std::vector<double> probes;  // 3 per point probe
@@ -195,9 +196,12 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
     if (cs.info.refinement < 0 || cs.info.refinement > 6) throw std::runtime_error("refinement must be in 0..6");
     for (int r = 0; r < cs.info.refinement; ++r) cs.mesh = dgtd::refine_uniform(cs.mesh);
 
-    // materials by element attribute (driver.cpp:946-966); the hesthaven
-    // operator applies eps_r, mu_r per element as 1/eps, 1/mu on the rhs
-    std::map<int, std::pair<double, double>> mat;
+    // materials by element attribute (driver.cpp:946-966); the operator
+    // applies eps_r, mu_r per element as 1/eps, 1/mu on the rhs, and the bulk
+    // conductivity, normalised sigma_SI * Z0 (driver.cpp:962-963,
+    // PhysicalConstants.h:9-15), as -sigma/eps E (assembled operator only)
+    constexpr double kZ0 = 4.0e-7 * M_PI * 299792458.0;
+    std::map<int, std::array<double, 3>> mat;
     if (!m.contains("materials")) throw std::runtime_error("JSON data does not include 'materials'.");
     const dgtd::Json& mats = m.at("materials");
     for (size_t i = 0; i < mats.size(); ++i) {
@@ -205,27 +209,31 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
         double eps = 1.0, mu = 1.0;
         if (mj.contains("relative_permittivity")) eps = mj.at("relative_permittivity").number();
         if (mj.contains("relative_permeability")) mu = mj.at("relative_permeability").number();
-        if (mj.contains("bulk_conductivity") && mj.at("bulk_conductivity").number() != 0.0)
-            throw Unsupported("conductive materials are not supported");
+        const double sig = mj.contains("bulk_conductivity") ? mj.at("bulk_conductivity").number() * kZ0 : 0.0;
         if (!(eps > 0) || !(mu > 0)) throw std::runtime_error("relative permittivity/permeability must be > 0");
+        if (!(sig >= 0.0)) throw std::runtime_error("Conductivity under 0.0 not allowed.");
         const dgtd::Json& tags = mj.at("tags");
-        for (size_t t = 0; t < tags.size(); ++t) mat[tags.at(t).integer()] = {eps, mu};
+        for (size_t t = 0; t < tags.size(); ++t) mat[tags.at(t).integer()] = {eps, mu, sig};
     }
     const int32_t K = cs.mesh.K();
     cs.eps.assign(K, 1.0);
     cs.mu.assign(K, 1.0);
-    bool any = false;
+    cs.sigma.assign(K, 0.0);
+    bool any = false, conductive = false;
     for (int32_t e = 0; e < K; ++e) {
         auto it = mat.find(cs.mesh.elem_attr()[e]);
         if (it == mat.end()) continue;
-        cs.eps[e] = it->second.first;
-        cs.mu[e] = it->second.second;
-        any = any || it->second.first != 1.0 || it->second.second != 1.0;
+        cs.eps[e] = it->second[0];
+        cs.mu[e] = it->second[1];
+        cs.sigma[e] = it->second[2];
+        any = any || it->second[0] != 1.0 || it->second[1] != 1.0;
+        conductive = conductive || it->second[2] != 0.0;
     }
     if (!any) {
         cs.eps.clear();
         cs.mu.clear();
     }
+    if (!conductive) cs.sigma.clear();
 
     // boundaries by triangle attribute (driver.cpp:870-887, 994-1035)
     if (!m.contains("boundaries")) throw std::runtime_error("JSON data does not include 'boundaries'.");
@@ -354,6 +362,7 @@ extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
     mesh->eps_r = c->eps.empty() ? nullptr : c->eps.data();
     mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
     mesh->dim = c->mesh.dim;
+    mesh->sigma = c->sigma.empty() ? nullptr : c->sigma.data();
     return DGTD_OK;
 }
 

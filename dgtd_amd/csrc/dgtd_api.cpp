@@ -1,6 +1,7 @@
 // C ABI of the operator (include/dgtd_hip.h): setup on the host in FP64,
 // device-resident buffers, fused stage launches.
 #include "dgtd_hip.h"
+#include "global.hpp"
 #include "tri.hpp"
 
 #include <hip/hip_runtime.h>
@@ -84,6 +85,11 @@ struct dgtd_op {
     int* d_order = nullptr;
     int variant = DGTD_KERNEL_VALU;
     int grid_mfma = 0;
+    // assembled operator (DGTD_KERNEL_CSR)
+    int64_t* d_rowptr = nullptr;
+    int* d_col = nullptr;
+    void* d_val = nullptr;
+    int64_t nnz = 0;
     int* d_tabs = nullptr;
     double* d_M = nullptr;
     double* d_jac = nullptr;
@@ -134,7 +140,7 @@ struct dgtd_op {
         for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
                         (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_probe_elem,
                         (void*)d_tfsf_elem, (void*)d_tfsf_sign, (void*)d_tfsf_xyz, (void*)d_tfsf_geo,
-                        (void*)d_lift, (void*)d_waves,
+                        (void*)d_lift, (void*)d_waves, (void*)d_rowptr, (void*)d_col, d_val,
                         (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
@@ -155,12 +161,17 @@ struct dgtd_op {
         a.conn = d_conn;
         a.mats = static_cast<const T*>(d_mats);
         a.tabs = d_tabs;
+        a.rowptr = d_rowptr;
+        a.col = d_col;
+        a.val = static_cast<const T*>(d_val);
+        a.nrows = ndofs;
         return a;
     }
 
     template <typename T>
     hipError_t launch_on(const dgtd::StageArgs<T>& a0, int mode, int var, hipStream_t s) const
     {
+        if (var == DGTD_KERNEL_CSR) return dgtd::launch_csr_stage<T>(a0, mode, s);
         if (dim == 2) return dgtd::launch_tri_stage<T>(T2.p, a0, mode, grid, s);
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
@@ -431,7 +442,15 @@ dgtd::TriMeshInput tri_input(const dgtd_op* op)
 
 // Set-up of a triangle-mesh operator (mesh->dim == 2): reference triangle,
 // geometry and connectivity records, the per-node stage kernel only.
-void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh)
+void upload_csr(dgtd_op* op, const dgtd::Csr& A)
+{
+    op->nnz = (int64_t)A.val.size();
+    op->d_rowptr = static_cast<int64_t*>(upload(A.rowptr));
+    op->d_col = static_cast<int*>(upload(A.col));
+    op->d_val = op->precision == 64 ? upload(A.val) : upload(dgtd::cast_vec<float>(A.val));
+}
+
+void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
 {
     op->dim = 2;
     op->T2 = dgtd::build_reftri(mesh->order);
@@ -524,6 +543,10 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh)
     op->nblk = (op->K + epb - 1) / epb;
     op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * 4));
     op->variant = DGTD_KERNEL_VALU;
+    if (kernel == DGTD_KERNEL_CSR) {
+        upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha));
+        op->variant = DGTD_KERNEL_CSR;
+    }
     if (op->precision == 64) dgtd::tri_prepare<double>(R.p);
     else dgtd::tri_prepare<float>(R.p);
 }
@@ -559,8 +582,10 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder)
         return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(opts->alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
-    if (opts->kernel < DGTD_KERNEL_AUTO || opts->kernel > DGTD_KERNEL_MFMA)
-        return fail(DGTD_E_ARG, "kernel must be DGTD_KERNEL_AUTO, _VALU or _MFMA");
+    if (opts->kernel < DGTD_KERNEL_AUTO || opts->kernel > DGTD_KERNEL_CSR)
+        return fail(DGTD_E_ARG, "kernel must be DGTD_KERNEL_AUTO, _VALU, _MFMA or _CSR");
+    if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
+        return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
     if (mesh->dim == 2 && opts->kernel == DGTD_KERNEL_MFMA)
         return fail(DGTD_E_UNSUPPORTED, "triangles run the per-node kernel only");
@@ -572,7 +597,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->integrator = opts->integrator;
         op->alpha = opts->alpha;
         if (mesh->dim == 2) {
-            create_tri(op, mesh);
+            create_tri(op, mesh, opts->kernel);
             *out = op;
             return DGTD_OK;
         }
@@ -681,7 +706,10 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         }
         if (opts->kernel == DGTD_KERNEL_MFMA && op->grid_mfma <= 0)
             throw std::runtime_error("the MFMA kernel does not fit this order and precision");
-        if (opts->kernel == DGTD_KERNEL_AUTO) {
+        if (opts->kernel == DGTD_KERNEL_CSR) {
+            upload_csr(op, dgtd::assemble_global_tet(md, op->R, mesh->sigma, op->alpha));
+            op->variant = DGTD_KERNEL_CSR;
+        } else if (opts->kernel == DGTD_KERNEL_AUTO) {
             if (op->precision == 64) op->autotune<double>();
             else op->autotune<float>();
         } else {
@@ -1052,6 +1080,12 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops)
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     const double s = (double)op->real_size();
     const double Np = op->Np, Nfp = op->Nfp;
+    if (op->variant == DGTD_KERNEL_CSR) {  // per entry value + column; per row its pointer and the LSERK vectors (state, residual in/out, out)
+        const double rows = 6.0 * Np, nnz_e = (double)op->nnz / op->K;
+        if (bytes) *bytes = nnz_e * (s + 4.0) + rows * 8.0 + rows * 4.0 * s;
+        if (flops) *flops = 2.0 * nnz_e + 4.0 * rows;
+        return DGTD_OK;
+    }
     if (op->dim == 2) {  // triangles: 2 derivative directions, 3 edges, 15-value record
         if (bytes) *bytes = 24.0 * Np * s + 15.0 * s + 24.0;
         if (flops) *flops = 24.0 * Np * Np + 36.0 * Np * Nfp + 150.0 * Nfp + 96.0 * Np;
@@ -1138,6 +1172,36 @@ int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo)
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
         std::copy(md.geo.begin(), md.geo.end(), geo);
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_global_assemble(const dgtd_mesh_desc* mesh, double alpha, int64_t* nnz, int64_t* rowptr, int32_t* col,
+                         double* val)
+{
+    if (!mesh || !nnz) return fail(DGTD_E_ARG, "NULL argument");
+    const bool fill = rowptr || col || val;
+    if (fill && !(rowptr && col && val)) return fail(DGTD_E_ARG, "rowptr, col and val: all or none");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    if (!std::isfinite(alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
+    try {
+        dgtd::Csr A;
+        if (mesh->dim == 2) {
+            const dgtd::RefTri R = dgtd::build_reftri(mesh->order);
+            A = dgtd::assemble_global_tri(dgtd::build_tri_mesh(tri_input(mesh), R, alpha), R, mesh->sigma, alpha);
+        } else {
+            const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+            A = dgtd::assemble_global_tet(dgtd::build_mesh(mesh_input(mesh), R, alpha), R, mesh->sigma, alpha);
+        }
+        if (fill && (int64_t)A.val.size() != *nnz) throw std::runtime_error("*nnz does not match the matrix");
+        *nnz = (int64_t)A.val.size();
+        if (fill) {
+            std::copy(A.rowptr.begin(), A.rowptr.end(), rowptr);
+            std::copy(A.col.begin(), A.col.end(), col);
+            std::copy(A.val.begin(), A.val.end(), val);
+        }
     } catch (const std::exception& e) {
         return fail(DGTD_E_ARG, e.what());
     }

@@ -1,0 +1,253 @@
+// The Global operator as an assembled sparse matrix (GlobalEvolution::Mult,
+// src/evolution/GlobalEvolution.cpp:628-1088: out = A [E; H] with A built
+// from DGOperatorFactory's sub-operators, src/components/DGOperatorFactory.h).
+//
+// For straight-sided elements the weak-form DG operator the reference
+// assembles with MFEM integrators and the strong-form hesthaven operator are
+// the same semi-discrete operator (the reference's 2D tests check the Global
+// sub-operators against Hesthaven's matrices, Hesthaven2DTest.cpp:234-553),
+// so A is assembled here from the same element data as the matrix-free
+// kernels, plus the bulk conductivity the Global operator adds to the E
+// equations (-sigma/eps E, collectGlobalConductiveOperator,
+// DGOperatorFactory.h:1554).  Rows follow the state layout (row c*N + e*Np + i),
+// so a stage kernel can apply its update to row r of the product directly.
+//
+// Per element: rhs = volume curl (metric-weighted reference derivatives) +
+// LIFT fscale/2 C(n) d over the face nodes, with d the jump (exterior -
+// interior, or the boundary / interior-condition weights of the own trace)
+// and C(n) = [[a(I - n n^T), [n x]], [-[n x], a(I - n n^T)]]; E rows scaled
+// by 1/eps, H rows by 1/mu.
+#include "global.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace dgtd {
+
+namespace {
+
+// BC weights of the own trace (same table as the device kernels).
+void bc_weights(int bc, double alpha, double& ce, double& ch)
+{
+    const double sma = alpha != 0.0 ? -1.0 / alpha : 0.0;
+    ce = bc == 1 ? -2.0 : bc == 3 ? sma : bc == 4 ? -1.0 : bc == 6 ? -0.5 : 0.0;
+    ch = bc == 2 ? -2.0 : bc == 3 ? sma : bc == 5 ? -1.0 : bc == 6 ? -0.5 : 0.0;
+}
+
+// Element view shared by tetrahedra and triangles.
+struct ElemView {
+    int Np, Nfp, nf, nd;                  // nodes, face nodes, faces, reference directions
+    const double* D[3];                   // Dr, Ds, Dt (row-major Np x Np)
+    const double* lift;                   // Np x nf*Nfp
+    const int* fmask;                     // nf*Nfp
+    const int* perm;                      // neighbour-node table
+    // per element e: metric G[k][d] = d r_k / d x_d, normals, fscale/2, 1/eps, 1/mu
+    double G[3][3], n[4][3], fs[4], ieps, imu;
+    int nb[4], code[4];
+};
+
+void load_tet(ElemView& v, const MeshData& md, int e)
+{
+    const double* g = &md.geo[(size_t)kGeoStride * e];
+    for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < 3; ++d) v.G[k][d] = g[3 * k + d];
+    for (int f = 0; f < 4; ++f) {
+        for (int d = 0; d < 3; ++d) v.n[f][d] = g[9 + 4 * f + d];
+        v.fs[f] = g[12 + 4 * f];
+        v.nb[f] = md.conn[(size_t)kConnStride * e + f];
+        v.code[f] = md.conn[(size_t)kConnStride * e + 4 + f];
+    }
+    v.ieps = g[25];
+    v.imu = g[26];
+}
+
+void load_tri(ElemView& v, const TriMeshData& md, int e)
+{
+    const double* g = &md.geo[(size_t)kTriGeoStride * e];
+    for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < 3; ++d) v.G[k][d] = 0.0;
+    v.G[0][0] = g[0];
+    v.G[0][1] = g[1];
+    v.G[1][0] = g[2];
+    v.G[1][1] = g[3];
+    for (int f = 0; f < 3; ++f) {
+        v.n[f][0] = g[4 + 3 * f];
+        v.n[f][1] = g[5 + 3 * f];
+        v.n[f][2] = 0.0;
+        v.fs[f] = g[6 + 3 * f];
+        v.nb[f] = md.conn[(size_t)kTriConnStride * e + f];
+        v.code[f] = md.conn[(size_t)kTriConnStride * e + 3 + f];
+    }
+    v.ieps = g[13];
+    v.imu = g[14];
+}
+
+// Dense element block: rows (c, i) of the element, columns (slot, k, j) with
+// slot 0 = own element, 1 + f = neighbour across face f.
+void element_block(const ElemView& v, double alpha, double sigma, std::vector<double>& B)
+{
+    const int Np = v.Np, Nfp = v.Nfp, nf = v.nf;
+    const int ncol = (1 + nf) * 6 * Np;
+    B.assign((size_t)6 * Np * ncol, 0.0);
+    auto at = [&](int c, int i, int slot, int k, int j) -> double& {
+        return B[((size_t)c * Np + i) * ncol + ((size_t)slot * 6 + k) * Np + j];
+    };
+    // volume: D_d = sum_k G[k][d] D_k
+    std::vector<double> Dd[3];
+    for (int d = 0; d < 3; ++d) {
+        Dd[d].assign((size_t)Np * Np, 0.0);
+        for (int k = 0; k < v.nd; ++k)
+            for (size_t q = 0; q < (size_t)Np * Np; ++q) Dd[d][q] += v.G[k][d] * v.D[k][q];
+    }
+    for (int x = 0; x < 3; ++x) {
+        const int y = (x + 1) % 3, z = (x + 2) % 3;
+        for (int i = 0; i < Np; ++i)
+            for (int j = 0; j < Np; ++j) {
+                // E_x: (D_y H_z - D_z H_y)/eps;  H_x: (-D_y E_z + D_z E_y)/mu
+                at(x, i, 0, 3 + z, j) += Dd[y][(size_t)i * Np + j] * v.ieps;
+                at(x, i, 0, 3 + y, j) -= Dd[z][(size_t)i * Np + j] * v.ieps;
+                at(3 + x, i, 0, z, j) -= Dd[y][(size_t)i * Np + j] * v.imu;
+                at(3 + x, i, 0, y, j) += Dd[z][(size_t)i * Np + j] * v.imu;
+            }
+    }
+    // surface
+    for (int f = 0; f < nf; ++f) {
+        const double* n = v.n[f];
+        double C[6][6] = {};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const double proj = alpha * ((a == b ? 1.0 : 0.0) - n[a] * n[b]);
+                C[a][b] = proj;
+                C[3 + a][3 + b] = proj;
+            }
+        for (int a = 0; a < 3; ++a) {  // [n x] v = n x v
+            const int y = (a + 1) % 3, z = (a + 2) % 3;
+            C[a][3 + z] += n[y];
+            C[a][3 + y] -= n[z];
+            C[3 + a][z] -= n[y];
+            C[3 + a][y] += n[z];
+        }
+        const bool interior = v.nb[f] >= 0;
+        double ce = 0.0, ch = 0.0;
+        if (!interior) bc_weights(-v.nb[f] - 1, alpha, ce, ch);
+        for (int b = 0; b < Nfp; ++b) {
+            const int m = f * Nfp + b;
+            const int M = v.fmask[m];
+            const int P = interior ? v.perm[(size_t)v.code[f] * Nfp + b] : -1;
+            for (int c = 0; c < 6; ++c) {
+                const double sc = (c < 3 ? v.ieps : v.imu) * v.fs[f];
+                for (int i = 0; i < Np; ++i) {
+                    const double l = v.lift[(size_t)i * nf * Nfp + m] * sc;
+                    if (l == 0.0) continue;
+                    for (int k = 0; k < 6; ++k) {
+                        const double w = l * C[c][k];
+                        if (w == 0.0) continue;
+                        if (interior) {
+                            at(c, i, 1 + f, k, P) += w;
+                            at(c, i, 0, k, M) -= w;
+                        } else {
+                            at(c, i, 0, k, M) += w * (k < 3 ? ce : ch);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // bulk conductivity on the E rows: -sigma/eps E
+    if (sigma != 0.0)
+        for (int c = 0; c < 3; ++c)
+            for (int i = 0; i < Np; ++i) at(c, i, 0, c, i) -= sigma * v.ieps;
+}
+
+template <typename View>
+void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double alpha, Csr& out)
+{
+    if (sigma)  // Material.cpp:14 refuses negative conductivity
+        for (int e = 0; e < K; ++e)
+            if (!(sigma[e] >= 0.0) || !std::isfinite(sigma[e]))
+                throw std::runtime_error("conductivity must be finite and >= 0");
+    const size_t N = (size_t)Np * K;
+    const int ncol = (1 + nf) * 6 * Np;
+    std::vector<double> B;
+    std::vector<int64_t> count(6 * N + 1, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            out.rowptr.assign(6 * N + 1, 0);
+            for (size_t r = 0; r < 6 * N; ++r) out.rowptr[r + 1] = out.rowptr[r] + count[r];
+            out.col.assign((size_t)out.rowptr[6 * N], 0);
+            out.val.assign((size_t)out.rowptr[6 * N], 0.0);
+            std::fill(count.begin(), count.end(), 0);
+        }
+        for (int e = 0; e < K; ++e) {
+            ElemView v = view_of(e);
+            element_block(v, alpha, sigma ? sigma[e] : 0.0, B);
+            for (int c = 0; c < 6; ++c)
+                for (int i = 0; i < Np; ++i) {
+                    const size_t r = c * N + (size_t)e * Np + i;
+                    const double* row = &B[((size_t)c * Np + i) * ncol];
+                    for (int q = 0; q < ncol; ++q) {
+                        if (row[q] == 0.0) continue;
+                        if (pass == 1) {
+                            const int slot = q / (6 * Np), k = (q / Np) % 6, j = q % Np;
+                            const int elem = slot == 0 ? e : v.nb[slot - 1];
+                            const size_t at = (size_t)out.rowptr[r] + count[r];
+                            out.col[at] = (int32_t)(k * N + (size_t)elem * Np + j);
+                            out.val[at] = row[q];
+                        }
+                        ++count[r];
+                    }
+                }
+        }
+    }
+}
+
+}  // namespace
+
+Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha)
+{
+    if (6.0 * R.Np * (double)md.K > 2147483647.0) throw std::runtime_error("mesh too large for 32-bit column indices");
+    Csr out;
+    auto view = [&](int e) {
+        ElemView v{};
+        v.Np = R.Np;
+        v.Nfp = R.Nfp;
+        v.nf = 4;
+        v.nd = 3;
+        v.D[0] = R.Dr.data();
+        v.D[1] = R.Ds.data();
+        v.D[2] = R.Dt.data();
+        v.lift = R.LIFT.data();
+        v.fmask = R.Fmask.data();
+        v.perm = R.face_perm.data();
+        load_tet(v, md, e);
+        return v;
+    };
+    assemble(md.K, R.Np, 4, view, sigma, alpha, out);
+    return out;
+}
+
+Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* sigma, double alpha)
+{
+    if (6.0 * R.Np * (double)md.K > 2147483647.0) throw std::runtime_error("mesh too large for 32-bit column indices");
+    Csr out;
+    auto view = [&](int e) {
+        ElemView v{};
+        v.Np = R.Np;
+        v.Nfp = R.Nfp;
+        v.nf = 3;
+        v.nd = 2;
+        v.D[0] = R.Dr.data();
+        v.D[1] = R.Ds.data();
+        v.D[2] = R.Ds.data();  // unused (nd = 2)
+        v.lift = R.LIFT.data();
+        v.fmask = R.Fmask.data();
+        v.perm = R.edge_perm.data();
+        load_tri(v, md, e);
+        return v;
+    };
+    assemble(md.K, R.Np, 3, view, sigma, alpha, out);
+    return out;
+}
+
+}  // namespace dgtd

@@ -1,0 +1,25 @@
+// Assembly of the Global operator (the same semi-discrete DG operator as the
+// matrix-free kernels, plus bulk conductivity) into CSR, rows and columns in
+// the state layout c*N + e*Np + i.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "mesh.hpp"
+#include "tri.hpp"
+
+namespace dgtd {
+
+struct Csr {
+    std::vector<int64_t> rowptr;  // 6N + 1
+    std::vector<int32_t> col;
+    std::vector<double> val;
+};
+
+// sigma: bulk conductivity per element (NULL = 0).  TF/SF faces are ordinary
+// interior faces here: their injection is a separate, state-independent term.
+Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha);
+Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* sigma, double alpha);
+
+}  // namespace dgtd

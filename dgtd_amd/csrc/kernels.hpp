@@ -35,7 +35,17 @@ struct StageArgs {
     const T* mats;       // Dr^T, Ds^T, Dt^T ([j][i] each), then LIFT^T ([m][i])
     const int* tabs;     // Fmask[4*Nfp], then face permutation table [96*Nfp]
     const int* order;    // MFMA variant: processing order of the 16-element tasks (NULL = identity)
+    // assembled (Global) operator, rows and columns in the state layout
+    const int64_t* rowptr;
+    const int* col;
+    const T* val;
+    int64_t nrows;
 };
+
+// Stage over the assembled operator (csr.hip): row r of the product is the
+// right-hand side of DOF r; the stage update as in the matrix-free kernels.
+template <typename T>
+hipError_t launch_csr_stage(const StageArgs<T>& a, int mode, hipStream_t s);
 
 template <typename T>
 hipError_t launch_stage(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);

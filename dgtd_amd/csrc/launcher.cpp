@@ -8,7 +8,9 @@
 // and DomainSnapshotProbes/rank_0/cycle_NNNNNN/{Ex,Ey,Ez,Hx,Hy,Hz}.gf, time.txt.
 //
 //   dgtd_launch CASE.json [--precision 64|32] [--integrator rk4|lserk4]
-//               [--kernel auto|valu|mfma] [--final-time T] [--out DIR]
+//               [--kernel auto|valu|mfma|csr] [--final-time T] [--out DIR]
+// A case with conductive materials runs on the assembled operator (csr) under
+// --kernel auto.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -48,7 +50,7 @@ void usage()
 {
     std::fprintf(stderr,
                  "usage: dgtd_launch CASE.json [--precision 64|32] [--integrator rk4|lserk4]\n"
-                 "                   [--kernel auto|valu|mfma] [--final-time T] [--out DIR]\n");
+                 "                   [--kernel auto|valu|mfma|csr] [--final-time T] [--out DIR]\n");
     std::exit(2);
 }
 
@@ -74,7 +76,11 @@ int main(int argc, char** argv)
             if (integrator < 0) usage();
         } else if (a == "--kernel") {
             const std::string v = val();
-            kernel = v == "auto" ? DGTD_KERNEL_AUTO : v == "valu" ? DGTD_KERNEL_VALU : v == "mfma" ? DGTD_KERNEL_MFMA : -1;
+            kernel = v == "auto"   ? DGTD_KERNEL_AUTO
+                     : v == "valu" ? DGTD_KERNEL_VALU
+                     : v == "mfma" ? DGTD_KERNEL_MFMA
+                     : v == "csr"  ? DGTD_KERNEL_CSR
+                                   : -1;
             if (kernel < 0) usage();
         } else if (a == "--final-time") final_time = std::atof(val().c_str());
         else if (a == "--out") out_dir = val();
@@ -94,7 +100,7 @@ int main(int argc, char** argv)
     o.alpha = info.alpha;
     o.precision = precision;
     o.integrator = integrator;
-    o.kernel = kernel;
+    o.kernel = kernel == DGTD_KERNEL_AUTO && mesh.sigma ? DGTD_KERNEL_CSR : kernel;
     dgtd_op* op = nullptr;
     check(dgtd_op_create(&mesh, &o, &op), "operator");
 

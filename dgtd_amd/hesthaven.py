@@ -15,7 +15,8 @@ from . import _abi
 from ._abi import check, dptr, i32ptr, i8ptr, lib
 
 INTEGRATORS = {"rk4": _abi.DGTD_RK4, "lserk4": _abi.DGTD_LSERK4}
-KERNELS = {"auto": _abi.DGTD_KERNEL_AUTO, "valu": _abi.DGTD_KERNEL_VALU, "mfma": _abi.DGTD_KERNEL_MFMA}
+KERNELS = {"auto": _abi.DGTD_KERNEL_AUTO, "valu": _abi.DGTD_KERNEL_VALU, "mfma": _abi.DGTD_KERNEL_MFMA,
+           "csr": _abi.DGTD_KERNEL_CSR}
 
 
 def num_nodes(order):
@@ -72,7 +73,7 @@ def square_mesh(n, L=1.0, bc="pec"):
 class _Mesh:
     """Keeps the numpy arrays alive while the C descriptor points at them."""
 
-    def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None):
+    def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None, sigma=None):
         E = np.asarray(EToV)
         # triangles (K, 3): the 2D operator; vertices may carry 2 or 3 coordinates
         self.dim = 2 if E.ndim == 2 and E.shape[1] == 3 else 3
@@ -85,9 +86,10 @@ class _Mesh:
         self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, nvert)
         self.eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
         self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        self.sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
         self.desc = _abi.MeshDesc(self.EToV.shape[0], order, dptr(self.vx), self.vx.shape[0],
                                   i32ptr(self.EToV), i8ptr(self.face_bc), dptr(self.eps),
-                                  dptr(self.mu), self.dim)
+                                  dptr(self.mu), self.dim, dptr(self.sigma))
 
 
 def gaussian(field="E", center=(0.5, 0.5, 0.5), spread=0.1, polarization=(0, 0, 1), dimension=3):
@@ -177,6 +179,22 @@ def mesh_geometry(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
     return geo.reshape(-1, 27)
 
 
+def global_assemble(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, sigma=None):
+    """The assembled operator of the "csr" kernel on the host, FP64:
+    (rowptr, col, val) of the ndofs x ndofs CSR matrix (the Global operator's
+    matrix, DGOperatorFactory::buildGlobalOperator, DGOperatorFactory.h:1428-1600)."""
+    m = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma)
+    nnz = ctypes.c_int64()
+    check(lib().dgtd_global_assemble(ctypes.byref(m.desc), float(alpha), ctypes.byref(nnz), None, None, None))
+    n = 6 * _nodes(order, m.dim) * m.EToV.shape[0]
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    val = np.zeros(nnz.value)
+    check(lib().dgtd_global_assemble(ctypes.byref(m.desc), float(alpha), ctypes.byref(nnz),
+                                     rowptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), i32ptr(col), dptr(val)))
+    return rowptr, col, val
+
+
 class HesthavenEvolution:
     """GPU hesthaven operator: Mult(in, out) over 6*Np*K FP64 host vectors.
 
@@ -184,12 +202,12 @@ class HesthavenEvolution:
     std::runtime_error (HesthavenEvolution.cpp:37,42,146)."""
 
     def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, precision=64,
-                 integrator="lserk4", device=0, kernel="auto"):
+                 integrator="lserk4", device=0, kernel="auto", sigma=None):
         self.order = order
         self.precision = precision
         self.integrator = integrator
         self.alpha = alpha
-        mesh = _Mesh(vx, EToV, face_bc, order, eps, mu)
+        mesh = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma)
         self.dim = mesh.dim
         self.K = mesh.EToV.shape[0]
         self.Np = _nodes(order, mesh.dim)
@@ -332,6 +350,19 @@ class HesthavenEvolution:
         n = ctypes.c_int32()
         check(lib().dgtd_op_last_kernel_us(self._h, ctypes.byref(us), ctypes.byref(n)))
         return us.value, n.value
+
+
+class GlobalEvolution(HesthavenEvolution):
+    """The reference's Global operator (maxwell::GlobalEvolution,
+    src/evolution/GlobalEvolution.cpp:628-1088) on the GPU: the operator
+    assembled into one CSR matrix, applied by a fused SpMV + stage-update
+    kernel, with per-element bulk conductivity sigma (normalised as sigma_SI
+    * Z0, Material.h:30) adding -sigma/eps_r E (DGOperatorFactory.h:1351-1361)."""
+
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, sigma=None, precision=64,
+                 integrator="lserk4", device=0):
+        super().__init__(vx, EToV, face_bc, order, alpha=alpha, eps=eps, mu=mu, precision=precision,
+                         integrator=integrator, device=device, kernel="csr", sigma=sigma)
 
 
 def estimate_time_step(op, cfl=1.0):

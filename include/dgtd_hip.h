@@ -84,9 +84,14 @@ typedef struct {
                              * d/dz = 0 and n_z = 0, DGOperatorFactory.h:428-437):
                              * EToV and face_bc hold 3 entries per element, edge f
                              * joins local vertices (f, f+1 mod 3) as in MFEM's
-                             * triangle, codes 1-6 (no TF/SF), vx keeps 3
-                             * coordinates per vertex with z ignored; the state keeps
-                             * all six components, Np = (p+1)(p+2)/2. */
+                             * triangle, codes 1-7, vx keeps 3 coordinates per
+                             * vertex with z ignored; the state keeps all six
+                             * components, Np = (p+1)(p+2)/2. */
+    const double* sigma;    /* K bulk conductivity, normalised as the reference's
+                             * Material (sigma_SI * Z0, Material.h:30, driver.cpp:962-963),
+                             * NULL = 0.  Only with DGTD_KERNEL_CSR: the Global operator's
+                             * -M_eps^-1 M_sigma E term (collectGlobalConductiveOperator,
+                             * DGOperatorFactory.h:1351-1361), -sigma/eps_r E per element. */
 } dgtd_mesh_desc;
 
 /* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)
@@ -103,6 +108,14 @@ typedef struct {
 #define DGTD_KERNEL_AUTO 0 /* time the available variants on this mesh at create, keep the fastest */
 #define DGTD_KERNEL_VALU 1 /* per-node vector FMA, reference matrices in LDS */
 #define DGTD_KERNEL_MFMA 2 /* matrix cores (v_mfma 16x16x4), 8-element wave tasks */
+/* The assembled operator: the reference's Global operator path
+ * (GlobalEvolution::Mult, GlobalEvolution.cpp:628-1088, a CSR SpMV over the
+ * matrix DGOperatorFactory::buildGlobalOperator assembles,
+ * DGOperatorFactory.h:1428-1600).  The matrix holds the same semi-discrete
+ * operator as the matrix-free variants (same boundary and interior-condition
+ * weights), plus bulk conductivity; one fused kernel does SpMV and stage
+ * update.  Chosen explicitly (never by autotune); tetrahedra and triangles. */
+#define DGTD_KERNEL_CSR 3
 
 const char* dgtd_last_error(void);
 const char* dgtd_version(void);
@@ -206,7 +219,7 @@ int dgtd_op_set_timing(dgtd_op* op, int32_t on);
 int dgtd_op_set_graphs(dgtd_op* op, int32_t on);
 int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches);
 
-/* Stage-kernel variant in use (DGTD_KERNEL_VALU or DGTD_KERNEL_MFMA). */
+/* Stage-kernel variant in use (DGTD_KERNEL_VALU, _MFMA or _CSR). */
 int dgtd_op_kernel_variant(const dgtd_op* op, int32_t* variant);
 
 /* Algorithmic HBM bytes and flops per element per stage for this operator. */
@@ -251,13 +264,14 @@ int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_f
  *                   ode_type (RK4)
  *   model: filename (Gmsh 2.2 ASCII .msh or MFEM v1.0 .mesh, relative to the
  *          case file; driver.cpp:1083-1188), refinement (uniform, 1:8),
- *          materials (relative_permittivity / _permeability per element tag),
+ *          materials (relative_permittivity / _permeability / bulk_conductivity
+ *          per element tag; a conductive case needs DGTD_KERNEL_CSR),
  *          boundaries (PEC / PMC / SMA per tag; a tag on interior faces
  *          becomes the interior condition DGTD_BC_INT_*)
  *   sources: "initial" gaussian / resonant fields; "planewave" TF/SF
  *            sources (Gaussian or modulated Gaussian, tags = TF/SF surface)
  *   probes: point positions, exporter steps, domain snapshots
- * Dipole sources, SGBC, conductivity and non-RK4 integrators are refused
+ * Dipole sources, SGBC and non-RK4 integrators are refused
  * with DGTD_E_UNSUPPORTED.
  * ------------------------------------------------------------------------ */
 typedef struct dgtd_case dgtd_case;
@@ -386,6 +400,14 @@ int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, in
 /* Per-element geometric factors, 27 doubles per element: dr/dx..dt/dz (9),
  * per face nx ny nz fscale/2 (16), 1/eps_r, 1/mu_r. */
 int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo);
+
+/* The assembled operator DGTD_KERNEL_CSR uploads, on the host in FP64 (the
+ * Global operator's matrix, DGOperatorFactory::buildGlobalOperator,
+ * DGOperatorFactory.h:1428-1600): ndofs x ndofs CSR, rows and columns in the
+ * state layout, mesh->sigma included.  Call with rowptr, col and val NULL to
+ * get *nnz, then with arrays of ndofs+1, *nnz and *nnz entries. */
+int dgtd_global_assemble(const dgtd_mesh_desc* mesh, double alpha, int64_t* nnz, int64_t* rowptr, int32_t* col,
+                         double* val);
 
 /* Point location as dgtd_op_probe_set does it: elem[p] (or -1) and Np nodal
  * interpolation weights per point (u(x_p) = sum_n weights[p*Np + n] u_n). */

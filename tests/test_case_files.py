@@ -141,6 +141,21 @@ def test_materials_by_tag(tmp_path):
     np.testing.assert_array_equal(c.eps, 4.0)
     np.testing.assert_array_equal(c.mu, 2.0)
     assert set(np.unique(c.face_bc)) == {0, 3}
+    assert c.sigma is None
+
+
+def test_bulk_conductivity_is_normalised_by_free_space_impedance(tmp_path):
+    # driver.cpp:962-963: sigma = bulk_conductivity * Z0, Z0 = mu0 c0 (PhysicalConstants.h:9-15)
+    shutil.copy(os.path.join(CASES, "3D_PEC.msh"), tmp_path / "m.msh")
+    p = write_case(str(tmp_path), "m.msh", [{"tags": [1, 2, 3, 4, 5, 6], "type": "PEC"}])
+    case = json.load(open(p))
+    case["model"]["materials"] = [{"tags": [1], "type": "conductor", "bulk_conductivity": 0.01}]
+    json.dump(case, open(p, "w"))
+    c = D.Case(p)
+    z0 = 4e-7 * np.pi * 299792458.0
+    assert z0 == pytest.approx(376.730313, rel=1e-8)
+    np.testing.assert_allclose(c.sigma, 0.01 * z0, rtol=1e-15)
+    assert c.eps is None and c.mu is None
 
 
 @pytest.mark.parametrize("mutate,code,msg", [
@@ -151,7 +166,7 @@ def test_materials_by_tag(tmp_path):
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "XYZ"}), -1, "incorrect"),
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [99], "type": "PEC"}), -1, "bdr_attribute"),
     (lambda c: c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "SGBC"}), -4, "SGBC"),
-    (lambda c: c["model"]["materials"][0].__setitem__("bulk_conductivity", 1.0), -4, "conductive"),
+    (lambda c: c["model"]["materials"][0].__setitem__("bulk_conductivity", -1.0), -1, "Conductivity under 0.0"),
     (lambda c: c["solver_options"].__setitem__("ode_type", 1), -4, "RK4"),
     (lambda c: c["solver_options"].__setitem__("evolution_operator", "foo"), -1, "Evolution Operator"),
     (lambda c: c["model"].__setitem__("filename", "missing.msh"), -1, "cannot open"),

@@ -1,0 +1,130 @@
+"""The assembled operator of the "csr" kernel (the Global operator's matrix,
+DGOperatorFactory::buildGlobalOperator, DGOperatorFactory.h:1428-1600),
+checked on the host.
+
+- Without conductivity A x equals the oracle's Mult (tetrahedra:
+  oracle/dgtd_oracle.cpp; triangles: oracle/oracle2d.py) for every face code,
+  to 1e-12 relative: only the summation order differs.
+- Bulk conductivity (collectGlobalConductiveOperator, DGOperatorFactory.h:
+  1351-1361, -M_eps^-1 M_sigma on the E blocks) is pinned through the energy
+  balance of the semi-discrete system: with the energy inner product
+  B(x, y) = sum_e J_e (eps E_x^T M E_y + mu H_x^T M H_y), the centred operator
+  on PEC walls is skew (B(x, A x) = 0) and conductivity adds exactly
+  -sum_e J_e sigma_e E^T M E (Poynting's theorem: d/dt W = -int sigma |E|^2).
+  A wrong sign, a missing 1/eps or a term on the H rows fails it.
+"""
+import numpy as np
+import pytest
+
+import meshgen as MG
+from oracle import oracle as O
+from oracle import oracle2d as O2
+
+import dgtd_amd as D
+
+
+def csr_matvec(A, x):
+    rowptr, col, val = A
+    out = np.zeros(rowptr.size - 1)
+    prod = val * x[col]
+    nz = np.diff(rowptr) > 0
+    out[nz] = np.add.reduceat(prod, rowptr[:-1][nz])
+    return out
+
+
+def bform(energy, x, y):
+    """B(x, y) from the quadratic form energy(x) = B(x, x) / 2."""
+    return 0.5 * (energy(x + y) - energy(x - y))
+
+
+def cube_case(n, seed, bc=("pec", "pmc", "sma", "pec", "pmc", "sma")):
+    rng = np.random.default_rng(seed)
+    vx, E, fbc = MG.cube_mesh(n, 1.0, list(bc))
+    K = E.shape[0]
+    return vx, E, fbc, rng.uniform(1, 4, K), rng.uniform(1, 2, K)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("alpha", [1.0, 0.5])
+def test_tet_matrix_equals_oracle_mult(order, alpha):
+    vx, E, fbc, eps, mu = cube_case(2, order)
+    A = D.global_assemble(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu)
+    ref = O.Operator(vx, E, fbc, order, alpha=alpha, eps=eps, mu=mu)
+    x = np.random.default_rng(order).standard_normal(ref.ndofs)
+    y, yr = csr_matvec(A, x), ref.mult(x)
+    assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+    rowptr, col, val = A
+    assert rowptr[0] == 0 and np.all(np.diff(rowptr) > 0)
+    assert col.min() >= 0 and col.max() < ref.ndofs and np.all(val != 0.0)
+    for r in range(0, ref.ndofs, 97):  # columns unique within a row
+        seg = col[rowptr[r]:rowptr[r + 1]]
+        assert np.unique(seg).size == seg.size
+
+
+@pytest.mark.parametrize("cond", [4, 5, 6, 7])
+def test_tet_interior_and_tfsf_faces_equal_oracle_mult(cond):
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    fbc = fbc.astype(np.int32)
+    fbc[MG.plane_faces(vx, E, 0, 0.5)] = cond
+    A = D.global_assemble(vx, E, fbc, 3, alpha=0.7)
+    ref = O.Operator(vx, E, fbc, 3, alpha=0.7)
+    x = np.random.default_rng(cond).standard_normal(ref.ndofs)
+    yr = ref.mult(x)  # TF/SF faces without plane waves: plain interior faces
+    assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_tri_matrix_equals_oracle_mult(order):
+    vx, E, bc = MG.square_mesh(4, bc=["pec", "pmc", "sma", "pec"], jitter=0.2, seed=order)
+    rng = np.random.default_rng(order)
+    eps, mu = rng.uniform(1, 4, E.shape[0]), rng.uniform(1, 2, E.shape[0])
+    for alpha in (1.0, 0.5):
+        A = D.global_assemble(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu)
+        ref = O2.Operator2D(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu)
+        x = rng.standard_normal(ref.ndofs())
+        yr = ref.mult(x)
+        assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_conductivity_energy_balance(dim):
+    rng = np.random.default_rng(dim)
+    if dim == 3:
+        vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+        order = 3
+    else:
+        vx, E, fbc = MG.square_mesh(5, bc="pec", jitter=0.2, seed=1)
+        order = 4
+    K = E.shape[0]
+    eps, mu, sigma = rng.uniform(1, 4, K), rng.uniform(1, 2, K), rng.uniform(0.5, 3.0, K)
+
+    def oracle(eps_, mu_):
+        return (O.Operator(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_) if dim == 3
+                else O2.Operator2D(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_))
+
+    W = oracle(eps, mu)
+    Ws = oracle(sigma, np.ones(K))  # energy with eps -> sigma: (1/2) sum J sigma E^T M E + ...
+    A0 = D.global_assemble(vx, E, fbc, order, alpha=0.0, eps=eps, mu=mu)
+    As = D.global_assemble(vx, E, fbc, order, alpha=0.0, eps=eps, mu=mu, sigma=sigma)
+    x = rng.standard_normal(A0[0].size - 1)
+    y0, ys = csr_matvec(A0, x), csr_matvec(As, x)
+    scale = np.linalg.norm(x) * np.linalg.norm(y0) * eps.max()
+    # centred flux with PEC walls: skew in the energy inner product
+    assert abs(bform(W.energy, x, y0)) <= 1e-12 * scale
+    # conductivity: B(x, A_sigma x) = -sum_e J_e sigma_e E^T M E
+    xe = x.copy()
+    xe[xe.size // 2:] = 0.0
+    want = -2.0 * Ws.energy(xe)
+    assert bform(W.energy, x, ys) == pytest.approx(want, rel=1e-11)
+    # and it only touches the E rows
+    half = x.size // 2
+    assert np.array_equal(ys[half:], y0[half:])
+
+
+def test_conductivity_needs_the_assembled_operator():
+    vx, E, fbc = MG.cube_mesh(1, 1.0, "pec")
+    sig = np.ones(E.shape[0])
+    with pytest.raises(D.DgtdError, match="DGTD_KERNEL_CSR"):
+        D.HesthavenEvolution(vx, E, fbc, 2, sigma=sig, kernel="valu")
+    with pytest.raises(D.DgtdError, match="conductivity"):
+        D.global_assemble(vx, E, fbc, 2, sigma=-sig)
