@@ -1,9 +1,16 @@
 // Stage kernel over the assembled Global operator (GlobalEvolution::Mult,
-// GlobalEvolution.cpp:628-1088, out = A [E; H]) in CSR: 16 lanes per row
-// stride over the row's entries (column-gathered state values), a 16-lane
-// reduction, then the lane with the sum applies the stage update of that DOF
-// (rows are in the state layout).  HBM-bound on the matrix: per entry one
-// value and one 32-bit column index.
+// GlobalEvolution.cpp:628-1088, out = A [E; H]) in CSR: a block of 256
+// threads takes 16 rows, streams their entries (value, 32-bit column, gathered
+// state value) into LDS products, 16 lanes per row sum them, and the lane with
+// the sum applies the stage update of that DOF (rows are in the state layout).
+// HBM-bound on the matrix: per entry one value and one column index.
+//
+// Block order: a block takes 16 consecutive DOFs of one field component, and
+// consecutive logical blocks cycle through the six components of the same
+// DOFs, so a run of logical blocks needs the state of one element range (and
+// its neighbours) only.  Hardware block b runs on XCD b % 8; logical block
+// (b % 8) * (grid / 8) + b / 8 gives each XCD one contiguous run, whose
+// gathered state then stays in that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -13,19 +20,57 @@ namespace dgtd {
 namespace {
 
 constexpr int kLanesPerRow = 16;
+constexpr int kRowsPerBlock = 256 / kLanesPerRow;
+constexpr int kXcds = 8;
 
-template <typename T, int MODE>
+// kUnroll: entries per thread and chunk (the chunk is 256 * kUnroll entries).
+template <typename T, int MODE, int kUnroll>
 __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
 {
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanesPerRow;
-    const int sub = threadIdx.x % kLanesPerRow;
-    const int64_t r = row < a.nrows ? row : a.nrows - 1;  // tail rows repeat the last row (no branch)
-    const int64_t b = a.rowptr[r], e = a.rowptr[r + 1];
+    constexpr int kChunk = 256 * kUnroll;
+    __shared__ T prod[kChunk];
+    const int t = threadIdx.x;
+    const int64_t per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
+    const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    const int64_t N = a.nrows / 6;
+    const int64_t dof0 = (lb / 6) * kRowsPerBlock;
+    const int64_t nrow = dof0 < N ? (N - dof0 < kRowsPerBlock ? N - dof0 : kRowsPerBlock) : 0;
+    const int64_t row0 = (lb % 6) * N + (nrow > 0 ? dof0 : 0);
+    const int64_t dof = dof0 + t / kLanesPerRow;
+    const bool valid = dof < N;  // padding blocks and the tail repeat the block's first row (no branch)
+    const int64_t r = valid ? (lb % 6) * N + dof : row0;
+    const int sub = t % kLanesPerRow;
+    // The block's rows are consecutive, so their entries are one contiguous
+    // range [b0, b1): it streams through in chunks of kChunk entries, every
+    // thread issuing kUnroll independent loads of (column, value) before their
+    // gathers; the products go to LDS and each row sums its own part.
+    const int64_t b0 = a.rowptr[row0], b1 = a.rowptr[row0 + nrow];
+    const int64_t rb = a.rowptr[r], re = a.rowptr[r + 1];
     T acc = T(0);
-    for (int64_t k = b + sub; k < e; k += kLanesPerRow) acc += a.val[k] * a.qin[a.col[k]];
+    for (int64_t c0 = b0; c0 < b1; c0 += kChunk) {  // block-uniform trip count
+        int c[kUnroll];
+        T v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int64_t k = c0 + u * 256 + t;
+            const int64_t kk = k < b1 ? k : b1 - 1;
+            c[u] = __builtin_nontemporal_load(a.col + kk);
+            v[u] = __builtin_nontemporal_load(a.val + kk);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const T pr = v[u] * a.qin[c[u]];
+            prod[u * 256 + t] = c0 + u * 256 + t < b1 ? pr : T(0);
+        }
+        __syncthreads();
+        const int64_t lo = rb > c0 ? rb : c0;
+        const int64_t hi = re < c0 + kChunk ? re : c0 + kChunk;
+        for (int64_t k = lo + sub; k < hi; k += kLanesPerRow) acc += prod[k - c0];
+        __syncthreads();
+    }
 #pragma unroll
     for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kLanesPerRow);
-    if (sub != 0 || row >= a.nrows) return;
+    if (sub != 0 || !valid) return;
     const T rhs = acc;
     if constexpr (MODE == kModeMult) {
         a.out[r] = rhs;
@@ -49,9 +94,18 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
 template <typename T, int MODE>
 hipError_t csr_launch(const StageArgs<T>& a, hipStream_t s)
 {
-    const int64_t threads = a.nrows * kLanesPerRow;
-    const int64_t grid = (threads + 255) / 256;
-    hipLaunchKernelGGL((csr_stage<T, MODE>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    // one chunk per block where the rows allow: the smallest chunk holding an
+    // average block's entries (16 rows), at most 4096 entries
+    const double per_block = (double)a.nnz / (double)a.nrows * kRowsPerBlock;
+    const int64_t N = a.nrows / 6;
+    const int64_t blocks = 6 * ((N + kRowsPerBlock - 1) / kRowsPerBlock);
+    const int64_t grid = kXcds * ((blocks + kXcds - 1) / kXcds);
+    if (per_block <= 256 * 8)
+        hipLaunchKernelGGL((csr_stage<T, MODE, 8>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else if (per_block <= 256 * 12)
+        hipLaunchKernelGGL((csr_stage<T, MODE, 12>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((csr_stage<T, MODE, 16>), dim3((unsigned)grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -60,7 +114,7 @@ hipError_t csr_launch(const StageArgs<T>& a, hipStream_t s)
 template <typename T>
 hipError_t launch_csr_stage(const StageArgs<T>& a, int mode, hipStream_t s)
 {
-    if (a.nrows <= 0) return hipErrorInvalidValue;
+    if (a.nrows <= 0 || a.nrows % 6 != 0) return hipErrorInvalidValue;
     switch (mode) {
         case kModeMult: return csr_launch<T, kModeMult>(a, s);
         case kModeLserk: return csr_launch<T, kModeLserk>(a, s);

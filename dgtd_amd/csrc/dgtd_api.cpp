@@ -165,6 +165,7 @@ struct dgtd_op {
         a.col = d_col;
         a.val = static_cast<const T*>(d_val);
         a.nrows = ndofs;
+        a.nnz = nnz;
         return a;
     }
 

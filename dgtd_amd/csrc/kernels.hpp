@@ -39,7 +39,7 @@ struct StageArgs {
     const int64_t* rowptr;
     const int* col;
     const T* val;
-    int64_t nrows;
+    int64_t nrows, nnz;
 };
 
 // Stage over the assembled operator (csr.hip): row r of the product is the
