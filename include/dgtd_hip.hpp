@@ -59,15 +59,18 @@ protected:
 // them: vertex coordinates, tetrahedra, a boundary code per face (face f
 // opposite local vertex f), optional eps_r/mu_r per element.
 struct Mesh {
+    int dim = 3;                  // 3: tetrahedra; 2: triangles (3 vertices and edges per element)
     std::vector<double> vx;       // 3 per vertex
-    std::vector<int32_t> EToV;    // 4 per element
-    std::vector<int8_t> face_bc;  // 4 per element, DGTD_BC_*
+    std::vector<int32_t> EToV;    // dim+1 per element
+    std::vector<int8_t> face_bc;  // dim+1 per element, DGTD_BC_*
     std::vector<double> eps_r, mu_r;
+    std::vector<double> sigma;    // bulk conductivity per element (DGTD_KERNEL_CSR only)
 
     dgtd_mesh_desc desc(int order) const
     {
         dgtd_mesh_desc d{};
-        d.K = (int32_t)(EToV.size() / 4);
+        d.dim = dim;
+        d.K = (int32_t)(EToV.size() / (dim + 1));
         d.order = order;
         d.vx = vx.data();
         d.nv = (int32_t)(vx.size() / 3);
@@ -75,6 +78,7 @@ struct Mesh {
         d.face_bc = face_bc.empty() ? nullptr : face_bc.data();
         d.eps_r = eps_r.empty() ? nullptr : eps_r.data();
         d.mu_r = mu_r.empty() ? nullptr : mu_r.data();
+        d.sigma = sigma.empty() ? nullptr : sigma.data();
         return d;
     }
 
