@@ -150,6 +150,10 @@ void read_sources(const dgtd::Json& c, dgtd_case& cs)
             if (modes.empty() || modes.size() > 3) throw std::runtime_error("resonant modes must have 1..3 entries");
             f.nmodes = (int32_t)modes.size();
             for (size_t k = 0; k < modes.size(); ++k) f.modes[k] = (int32_t)modes[k];
+        } else if (mt == "besselj6_2D") {  // buildBesselJ6InitialField, driver.cpp:393-400, centre 0
+            f.kind = DGTD_FIELD_BESSELJ6_2D;
+        } else if (mt == "besselj6_3D") {  // buildSphericalBesselJ6InitialField, driver.cpp:474-481, centre 0
+            f.kind = DGTD_FIELD_BESSELJ6_3D;
         } else {
             throw Unsupported("initial field magnitude '" + mt + "' is not supported");
         }

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "fields.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "packing.hpp"
@@ -1235,7 +1236,7 @@ extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgt
                 throw std::runtime_error("gaussian needs dimension 1..3 and spread > 0");
             if (f.kind == DGTD_FIELD_RESONANT && (f.nmodes < 1 || f.nmodes > 3))
                 throw std::runtime_error("resonant mode needs 1..3 modes");
-            if (f.kind != DGTD_FIELD_GAUSSIAN && f.kind != DGTD_FIELD_RESONANT)
+            if (f.kind < DGTD_FIELD_GAUSSIAN || f.kind > DGTD_FIELD_BESSELJ6_3D)
                 throw std::runtime_error("unknown initial field kind");
             for (size_t n = 0; n < N; ++n) {
                 double y[3];
@@ -1245,6 +1246,10 @@ extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgt
                     double r2 = 0.0;
                     for (int d = 0; d < f.dimension; ++d) r2 += y[d] * y[d];
                     v = std::exp(-r2 / (2.0 * f.spread * f.spread));
+                } else if (f.kind == DGTD_FIELD_BESSELJ6_2D) {
+                    v = dgtd::besselj6_2d(y[0], y[1]);
+                } else if (f.kind == DGTD_FIELD_BESSELJ6_3D) {
+                    v = dgtd::besselj6_3d(y[0], y[1], y[2]);
                 } else {
                     v = 1.0;
                     for (int d = 0; d < f.nmodes; ++d) v *= std::sin(f.modes[d] * M_PI * y[d]);

@@ -112,6 +112,17 @@ def resonant(modes=(1, 1), field="E", polarization=(0, 0, 1)):
     return f
 
 
+def bessel_j6(dim=2, field="E", polarization=(0, 0, 1)):
+    """J6 cavity mode at t = 0, centred at the origin: J_6(a6 r) cos(6 phi) in
+    2D ("besselj6_2D"), j_6(a6 r) Re Y_6^6 in 3D ("besselj6_3D"),
+    driver.cpp:393-400, 474-481; a6 = 13.589290170541217."""
+    f = _abi.InitialField()
+    f.kind = 2 if dim == 2 else 3
+    f.field = 0 if field == "E" else 1
+    f.polarization[:] = list(polarization)
+    return f
+
+
 def planewave(polarization, propagation, spread, mean=0.0, frequency=None):
     """Gaussian (or, with a frequency, modulated Gaussian) plane wave for the
     TF/SF faces; mean is the projected 1D mean k.mean (driver.cpp:483-520)."""

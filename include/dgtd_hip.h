@@ -241,6 +241,12 @@ int dgtd_cube_mesh(int32_t n, double L, const int8_t* side_bc, double* vx, int32
  * Several sources add up, as setInitialFields does. */
 #define DGTD_FIELD_GAUSSIAN 0
 #define DGTD_FIELD_RESONANT 1
+/* besselj6_2D: f = J_6(a6 r) cos(6 phi), r and phi of (y_0, y_1)        (Function.h:170-194)
+ * besselj6_3D: f = j_6(a6 r) Y_6^6(theta, phi) real part, with GSL's
+ *              normalised associated Legendre function               (Function.h:196-222)
+ * a6 = 13.589290170541217 (first zero of J_6); both at t = 0. */
+#define DGTD_FIELD_BESSELJ6_2D 2
+#define DGTD_FIELD_BESSELJ6_3D 3
 typedef struct {
     int32_t kind;            /* DGTD_FIELD_* */
     int32_t field;           /* 0 electric, 1 magnetic */
@@ -268,7 +274,7 @@ int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_f
  *          per element tag; a conductive case needs DGTD_KERNEL_CSR),
  *          boundaries (PEC / PMC / SMA per tag; a tag on interior faces
  *          becomes the interior condition DGTD_BC_INT_*)
- *   sources: "initial" gaussian / resonant fields; "planewave" TF/SF
+ *   sources: "initial" gaussian / resonant / besselj6_2D / besselj6_3D fields; "planewave" TF/SF
  *            sources (Gaussian or modulated Gaussian, tags = TF/SF surface)
  *   probes: point positions, exporter steps, domain snapshots
  * Dipole sources, SGBC and non-RK4 integrators are refused

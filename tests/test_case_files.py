@@ -81,6 +81,35 @@ def test_initial_fields_follow_their_formulas():
                                atol=1e-14)
 
 
+def test_bessel_initial_fields_match_scipy():
+    """besselj6_2D / besselj6_3D sources (Function.h:170-222) against scipy's
+    Bessel and spherical-harmonic routines at the node positions."""
+    import scipy.special as sp
+
+    a6 = 13.589290170541217
+    assert sp.jv(6, a6) == pytest.approx(0.0, abs=1e-14)  # the first zero of J_6
+    vx, E, fbc = MG.cube_mesh(3, 1.0, "pec")
+    vx = 2.0 * vx - 1.0  # [-1, 1]^3: r up to sqrt(3), a6 r up to 23.5 (series and recurrence)
+    pos = O.Operator(vx, E, fbc, 3).positions().reshape(3, -1)
+    x, y, z = pos
+    r = np.sqrt(x * x + y * y + z * z)
+    phi = np.arctan2(y, x)
+    want3 = sp.spherical_jn(6, a6 * r) * np.real(sp.sph_harm_y(6, 6, np.arccos(np.clip(z / r, -1, 1)), phi))
+    got3 = D.project_initial_fields(vx, E, 3, [D.bessel_j6(3, "H", (0, 1, 0))]).reshape(6, -1)
+    np.testing.assert_allclose(got3[4], want3, rtol=1e-11, atol=1e-14 * np.abs(want3).max())
+    assert np.abs(np.delete(got3, 4, axis=0)).max() == 0
+    vt, Et, bt = MG.square_mesh(6, L=2.0, bc="pec", jitter=0.2, seed=3)
+    vt = vt.copy()
+    vt[:, :2] -= 1.0
+    gt = D.project_initial_fields(vt, Et, 4, [D.bessel_j6(2)]).reshape(6, -1)
+    from oracle import oracle2d as O2
+
+    p2 = O2.Operator2D(vt, Et, bt, 4).pos.reshape(-1, 2)
+    want2 = sp.jv(6, a6 * np.hypot(p2[:, 0], p2[:, 1])) * np.cos(6 * np.arctan2(p2[:, 1], p2[:, 0]))
+    np.testing.assert_allclose(gt[2], want2, rtol=1e-12, atol=1e-15)
+    assert np.abs(want2).max() > 0.1
+
+
 def test_uniform_refinement_invariants():
     t = D.Case(TM55)  # "refinement": 1
     vx0, E0, tris0 = MG.read_gmsh22(os.path.join(CASES, "3D_Resonant_Box_TM55_H1_P2.msh"))
