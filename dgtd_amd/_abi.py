@@ -62,7 +62,8 @@ class CaseInfo(ctypes.Structure):
                 ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
                 ("evolution_operator", c_int32), ("nsources", c_int32), ("nprobes", c_int32),
                 ("exporter_steps", c_int32), ("refinement", c_int32), ("snapshot_steps", c_int32),
-                ("snapshot_saves", c_int32), ("nplanewaves", c_int32), ("dim", c_int32)]
+                ("snapshot_saves", c_int32), ("nplanewaves", c_int32), ("dim", c_int32),
+                ("nfield_probes", c_int32)]
 
 
 DGTD_E_CAPACITY = -6
@@ -105,6 +106,7 @@ SIGNATURES = {
     "dgtd_case_mesh": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc)]),
     "dgtd_case_sources": (c_int32, [c_void_p, ctypes.POINTER(InitialField)]),
     "dgtd_case_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p]),
+    "dgtd_case_field_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p, _i32p]),
     "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
     "dgtd_case_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave)]),
     "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),

@@ -21,7 +21,8 @@ class Case:
     """A loaded case: `info` (dict), mesh arrays `vx`, `EToV`, `face_bc`, `eps`,
     `mu`, `sigma` (normalised conductivity, None if no material conducts),
     `sources` (InitialField list), `planewaves` (PlaneWave list, the TF/SF
-    faces carry code 7) and `probes` (n x 3)."""
+    faces carry code 7), `probes` (n x 3) and `field_probes` (n x 3, with
+    `field_probe_components` 0..5 = Ex..Hz)."""
 
     def __init__(self, path):
         self.path = os.path.abspath(path)
@@ -56,6 +57,14 @@ class Case:
             self.probe_saves = np.zeros(ci.nprobes, dtype=np.int32)
             check(lib().dgtd_case_probes(h, dptr(self.probes), i32ptr(self.probe_steps),
                                          i32ptr(self.probe_saves)))
+            nf = ci.nfield_probes
+            self.field_probes = np.zeros((nf, 3))
+            self.field_probe_steps = np.zeros(nf, dtype=np.int32)
+            self.field_probe_saves = np.zeros(nf, dtype=np.int32)
+            self.field_probe_components = np.zeros(nf, dtype=np.int32)  # 0..5 = Ex..Hz
+            check(lib().dgtd_case_field_probes(h, dptr(self.field_probes), i32ptr(self.field_probe_steps),
+                                               i32ptr(self.field_probe_saves),
+                                               i32ptr(self.field_probe_components)))
         except Exception:
             self.close()
             raise

@@ -29,6 +29,8 @@ struct dgtd_case {
     std::vector<dgtd_initial_field> sources;
     std::vector<double> probes;  // 3 per point probe
     std::vector<int32_t> probe_steps, probe_saves;
+    std::vector<double> fprobes;  // field probes: 3 per probe
+    std::vector<int32_t> fprobe_steps, fprobe_saves, fprobe_comp;
     std::vector<dgtd_planewave> waves;
     std::vector<int> tfsf_tags;
     std::vector<bool> wave_auto_mean;  // mean from the TF/SF surface (no "mean" in the case)
@@ -174,6 +176,21 @@ void read_probes(const dgtd::Json& c, dgtd_case& cs)
         const dgtd::Json& d = p.at("domain_snapshot");
         cs.info.snapshot_steps = d.contains("steps") ? d.at("steps").integer() : 0;
         cs.info.snapshot_saves = d.contains("saves") ? d.at("saves").integer() : 0;
+    }
+    if (p.contains("field")) {  // driver.cpp:764-778
+        const dgtd::Json& fs = p.at("field");
+        for (size_t i = 0; i < fs.size(); ++i) {
+            const dgtd::Json& fj = fs.at(i);
+            const std::string& pol = fj.at("polarization").string();
+            const int d = pol == "X" ? 0 : pol == "Y" ? 1 : pol == "Z" ? 2 : -1;
+            if (d < 0) throw std::runtime_error("Wrong Field Polarization in Point Probe assignation.");
+            double x[3];
+            vec3(fj.at("position"), x);
+            cs.fprobes.insert(cs.fprobes.end(), x, x + 3);
+            cs.fprobe_comp.push_back(3 * field_type(fj.at("field_type")) + d);
+            cs.fprobe_steps.push_back(fj.contains("steps") ? fj.at("steps").integer() : 0);
+            cs.fprobe_saves.push_back(fj.contains("saves") ? fj.at("saves").integer() : 0);
+        }
     }
     if (p.contains("point")) {
         const dgtd::Json& pts = p.at("point");
@@ -329,6 +346,7 @@ extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
         cs->info.nsources = (int32_t)cs->sources.size();
         cs->info.nplanewaves = (int32_t)cs->waves.size();
         cs->info.nprobes = (int32_t)(cs->probes.size() / 3);
+        cs->info.nfield_probes = (int32_t)cs->fprobe_comp.size();
         if (cs->info.alpha == 0.0)
             for (int8_t b : cs->face_bc)
                 if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598
@@ -431,6 +449,20 @@ extern "C" int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out)
 {
     if (!c || (!out && !c->sources.empty())) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     for (size_t i = 0; i < c->sources.size(); ++i) out[i] = c->sources[i];
+    return DGTD_OK;
+}
+
+extern "C" int dgtd_case_field_probes(const dgtd_case* c, double* xyz, int32_t* steps, int32_t* saves,
+                                      int32_t* component)
+{
+    if (!c) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    for (size_t i = 0; i < c->fprobe_comp.size(); ++i) {
+        if (xyz)
+            for (int d = 0; d < 3; ++d) xyz[3 * i + d] = c->fprobes[3 * i + d];
+        if (steps) steps[i] = c->fprobe_steps[i];
+        if (saves) saves[i] = c->fprobe_saves[i];
+        if (component) component[i] = c->fprobe_comp[i];
+    }
     return DGTD_OK;
 }
 

@@ -3,7 +3,8 @@
 // operator.  Mirrors Solver::run (src/solver/Solver.cpp:483-551) with point
 // probes written as ProbesManager writes them (ProbesManager.cpp:183-218,
 // 375-406): Exports/<case>/PointProbes/PointProbe<ID>.dat, one line per
-// sample with time / c (seconds) and Ex Ey Ez Hx Hy Hz; and domain snapshots
+// sample with time / c (seconds) and Ex Ey Ez Hx Hy Hz; field probes as
+// FieldProbes/FieldProbe<ID>.dat with one component (:222-259, 344-371); and domain snapshots
 // as ProbesManager writes them (:457-495): DomainSnapshotProbes/meshes/mesh_rank0
 // and DomainSnapshotProbes/rank_0/cycle_NNNNNN/{Ex,Ey,Ez,Hx,Hy,Hz}.gf, time.txt.
 //
@@ -125,11 +126,15 @@ int main(int argc, char** argv)
         dt = info.cfl / (fs * info.order * info.order);
     }
 
-    // point probes and their sampling intervals (driver.cpp:717-733)
-    const int np = info.nprobes;
+    // point probes, then field probes, in one device probe set, and their
+    // sampling intervals (driver.cpp:717-733, 764-778)
+    const int npp = info.nprobes, nfp = info.nfield_probes, np = npp + nfp;
     std::vector<double> pxyz(3 * (size_t)np);
-    std::vector<int32_t> psteps(np), psaves(np), pelem(np), interval(np, 1);
+    std::vector<int32_t> psteps(np), psaves(np), pelem(np), interval(np, 1), comp(np, -1);
     check(dgtd_case_probes(cs, pxyz.data(), psteps.data(), psaves.data()), "probes");
+    check(dgtd_case_field_probes(cs, pxyz.data() + 3 * (size_t)npp, psteps.data() + npp, psaves.data() + npp,
+                                 comp.data() + npp),
+          "field probes");
     if (np > 0) check(dgtd_op_probe_set(op, np, pxyz.data(), pelem.data()), "probe set");
     const int total_steps = (int)std::ceil(final_time / dt);
     for (int p = 0; p < np; ++p)
@@ -137,19 +142,25 @@ int main(int argc, char** argv)
 
     if (out_dir.empty()) out_dir = "Exports/" + stem(case_path);
     std::vector<std::ofstream> files(np);
-    if (np > 0) {
-        const std::string base = out_dir + "/PointProbes/";
+    static const char* const kComp[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    for (int kind = 0; kind < 2; ++kind) {  // PointProbes (:186-218), FieldProbes (:222-259)
+        const int first = kind == 0 ? 0 : npp, count = kind == 0 ? npp : nfp;
+        if (count == 0) continue;
+        const std::string name = kind == 0 ? "PointProbe" : "FieldProbe";
+        const std::string base = out_dir + "/" + name + "s/";
         std::filesystem::create_directories(base);
-        for (int p = 0; p < np; ++p) {
+        for (int p = first; p < first + count; ++p) {
             if (pelem[p] < 0) continue;  // outside the mesh: no file, as elementId -2
-            files[p].open(base + "PointProbe" + std::to_string(p) + ".dat", std::ios::trunc);
-            files[p] << "PointProbe ID " << p << "\n";
+            const int id = p - first;
+            files[p].open(base + name + std::to_string(id) + ".dat", std::ios::trunc);
+            files[p] << name << " ID " << id << "\n";
             files[p] << "Spatial Position (X, Y, Z) \n";
             files[p] << std::scientific << std::setprecision(5);
             files[p] << std::to_string(pxyz[3 * p]) + " " + std::to_string(pxyz[3 * p + 1]) + " " +
                             std::to_string(pxyz[3 * p + 2])
                      << "\n";
-            files[p] << "Time (s) // Ex // Ey // Ez // Hx // Hy // Hz \n";
+            if (kind == 0) files[p] << "Time (s) // Ex // Ey // Ez // Hx // Hy // Hz \n";
+            else files[p] << "Time (s) // " << kComp[comp[p]] << "\n";
         }
     }
     // domain snapshots (ProbesManager.cpp:457-495): the mesh once, then
@@ -183,7 +194,10 @@ int main(int argc, char** argv)
         for (int p = 0; p < np; ++p) {
             if (pelem[p] < 0 || !(final || cycle % interval[p] == 0)) continue;
             files[p] << t / kSpeedOfLightSI;
-            for (int c = 0; c < 6; ++c) files[p] << " " << vals[6 * p + c];
+            if (comp[p] < 0)
+                for (int c = 0; c < 6; ++c) files[p] << " " << vals[6 * p + c];
+            else
+                files[p] << " " << vals[6 * p + comp[p]];
             files[p] << "\n";
         }
     };

@@ -297,6 +297,7 @@ typedef struct {
     int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
     int32_t nplanewaves;          /* "planewave" sources (their tags are TF/SF faces) */
     int32_t dim;                  /* 3: tetrahedral mesh; 2: triangle mesh (2D operator) */
+    int32_t nfield_probes;        /* probes.field entries */
 } dgtd_case_info;
 
 /* Parse the case file, read and refine its mesh, map tags to boundary codes
@@ -326,6 +327,13 @@ int dgtd_case_save_fields(const dgtd_case* c, const char* dir, int64_t ndofs, co
  * either array may be NULL).  The sampling interval is steps, else
  * max(1, ceil(final_time/dt)/saves), else 1 (driver.cpp:717-733). */
 int dgtd_case_probes(const dgtd_case* c, double* xyz, int32_t* steps, int32_t* saves);
+
+/* Field probes (FieldProbe, Probes.h:75-130; driver.cpp:764-778): one field
+ * component at a point, written as FieldProbes/FieldProbe<ID>.dat
+ * (ProbesManager.cpp:222-259, 344-371).  3 * info.nfield_probes coordinates,
+ * "steps" and "saves" as for point probes, and the component 3*field +
+ * direction (0..5 = Ex, Ey, Ez, Hx, Hy, Hz); any array may be NULL. */
+int dgtd_case_field_probes(const dgtd_case* c, double* xyz, int32_t* steps, int32_t* saves, int32_t* component);
 
 /* ------------------------------------------------------------------------
  * Batched instances: many independent meshes of uneven size and order

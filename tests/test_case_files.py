@@ -53,6 +53,27 @@ def test_pec_case_options_sources_probes():
     assert list(s.center) == src["center"] and list(s.polarization) == src["polarization"]
 
 
+def test_field_probes_parse(tmp_path):
+    case = json.load(open(PEC))
+    case["probes"]["field"] = [{"field_type": "electric", "polarization": "Z", "position": [0.1, 0.2, 0.3]},
+                               {"field_type": "magnetic", "polarization": "X", "position": [0.5, 0.5, 0.5],
+                                "steps": 3, "saves": 7}]
+    shutil.copy(os.path.join(CASES, "3D_PEC.msh"), tmp_path / "3D_PEC.msh")
+    p = tmp_path / "3D_PEC.json"
+    p.write_text(json.dumps(case))
+    c = D.Case(str(p))
+    assert c.info["nfield_probes"] == 2
+    np.testing.assert_array_equal(c.field_probes, [[0.1, 0.2, 0.3], [0.5, 0.5, 0.5]])
+    np.testing.assert_array_equal(c.field_probe_components, [2, 3])
+    np.testing.assert_array_equal(c.field_probe_steps, [0, 3])
+    np.testing.assert_array_equal(c.field_probe_saves, [0, 7])
+    assert c.probes.shape == (3, 3)  # point probes unchanged
+    case["probes"]["field"][0]["polarization"] = "W"
+    p.write_text(json.dumps(case))
+    with pytest.raises(D.DgtdError, match="Wrong Field Polarization"):
+        D.Case(str(p))
+
+
 def test_gmsh_reader_matches_independent_reader():
     c = D.Case(PEC)
     vx, E, tris = MG.read_gmsh22(os.path.join(CASES, "3D_PEC.msh"))

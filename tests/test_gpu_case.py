@@ -83,6 +83,36 @@ def test_launcher_writes_reference_probe_files(tmp_path):
         np.testing.assert_allclose(rows[-1, 1:], last[p], rtol=1e-4, atol=1e-9)
 
 
+def test_launcher_field_probes_match_point_probe_columns(tmp_path):
+    """FieldProbe files (ProbesManager.cpp:222-259, 344-371): one component of
+    the point probe at the same position, same times, every second step."""
+    case = json.load(open(PEC))
+    pts = [p["position"] for p in case["probes"]["point"]]
+    case["probes"]["field"] = [{"field_type": "electric", "polarization": "Z", "position": pts[0]},
+                               {"field_type": "magnetic", "polarization": "Y", "position": pts[1], "steps": 2}]
+    shutil.copy(os.path.join(os.path.dirname(PEC), "3D_PEC.msh"), tmp_path / "3D_PEC.msh")
+    path = tmp_path / "3D_PEC.json"
+    path.write_text(json.dumps(case))
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, str(path), "--final-time", "0.05", "--out", str(out)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+
+    def rows(f, skip=4):
+        return np.array([[float(v) for v in ln.split()] for ln in f.read_text().splitlines()[skip:]])
+
+    for fid, (pid, comp, every) in enumerate([(0, 2, 1), (1, 4, 2)]):
+        f = out / "FieldProbes" / f"FieldProbe{fid}.dat"
+        head = f.read_text().splitlines()[:4]
+        assert head[0] == f"FieldProbe ID {fid}" and head[3] == "Time (s) // " + ["Ez", "Hy"][fid]
+        fr, pr = rows(f), rows(out / "PointProbes" / f"PointProbe{pid}.dat")
+        sel = np.r_[np.arange(0, 6, every), 5] if every > 1 else np.arange(6)  # the final time is always sampled
+        sel = np.unique(sel)
+        assert fr.shape == (sel.size, 2)
+        np.testing.assert_array_equal(fr[:, 0], pr[sel, 0])
+        np.testing.assert_array_equal(fr[:, 1], pr[sel, 1 + comp])
+
+
 def test_launcher_domain_snapshots(tmp_path):
     """3D_Resonant_Box_TM55_H1_P2 (refinement 1, domain_snapshot every 2828
     steps): 3 steps give snapshots at cycle 0 and at the final time."""
