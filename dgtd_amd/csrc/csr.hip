@@ -3,7 +3,8 @@
 // threads takes 16 rows, streams their entries (value, 32-bit column, gathered
 // state value) into LDS products, 16 lanes per row sum them, and the lane with
 // the sum applies the stage update of that DOF (rows are in the state layout).
-// HBM-bound on the matrix: per entry one value and one column index.
+// HBM-bound on the matrix: per entry one value and a 16-bit element-local
+// column code (decoded through the block's element table in LDS).
 //
 // Block order: a block takes 16 consecutive DOFs of one field component, and
 // consecutive logical blocks cycle through the six components of the same
@@ -21,6 +22,7 @@ namespace {
 
 constexpr int kLanesPerRow = 16;
 constexpr int kRowsPerBlock = 256 / kLanesPerRow;
+static_assert(kRowsPerBlock == kCsrRowsPerBlock, "the column codes are per block of 16 DOFs");
 constexpr int kXcds = 8;
 
 // kUnroll: entries per thread and chunk (the chunk is 256 * kUnroll entries).
@@ -29,6 +31,7 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
 {
     constexpr int kChunk = 256 * kUnroll;
     __shared__ T prod[kChunk];
+    __shared__ int tab[kCsrTab];
     const int t = threadIdx.x;
     const int64_t per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
     const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -40,6 +43,11 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
     const bool valid = dof < N;  // padding blocks and the tail repeat the block's first row (no branch)
     const int64_t r = valid ? (lb % 6) * N + dof : row0;
     const int sub = t % kLanesPerRow;
+    const int64_t nb = (N + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int64_t db = lb / 6 < nb ? lb / 6 : nb - 1;
+    if (t < kCsrTab) tab[t] = a.etab[db * kCsrTab + t];
+    __syncthreads();
+    const int n32 = (int)N;  // 6 N < 2^31 (checked at assembly)
     // The block's rows are consecutive, so their entries are one contiguous
     // range [b0, b1): it streams through in chunks of kChunk entries, every
     // thread issuing kUnroll independent loads of (column, value) before their
@@ -48,18 +56,19 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
     const int64_t rb = a.rowptr[r], re = a.rowptr[r + 1];
     T acc = T(0);
     for (int64_t c0 = b0; c0 < b1; c0 += kChunk) {  // block-uniform trip count
-        int c[kUnroll];
+        unsigned c[kUnroll];
         T v[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             const int64_t k = c0 + u * 256 + t;
             const int64_t kk = k < b1 ? k : b1 - 1;
-            c[u] = __builtin_nontemporal_load(a.col + kk);
+            c[u] = __builtin_nontemporal_load(a.col16 + kk);
             v[u] = __builtin_nontemporal_load(a.val + kk);
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            const T pr = v[u] * a.qin[c[u]];
+            const int col = (int)(c[u] >> 6 & 7) * n32 + tab[c[u] >> 9] * a.np + (int)(c[u] & 63);
+            const T pr = v[u] * a.qin[col];
             prod[u * 256 + t] = c0 + u * 256 + t < b1 ? pr : T(0);
         }
         __syncthreads();

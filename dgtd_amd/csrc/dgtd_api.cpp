@@ -88,7 +88,8 @@ struct dgtd_op {
     int grid_mfma = 0;
     // assembled operator (DGTD_KERNEL_CSR)
     int64_t* d_rowptr = nullptr;
-    int* d_col = nullptr;
+    uint16_t* d_col = nullptr;
+    int* d_etab = nullptr;
     void* d_val = nullptr;
     int64_t nnz = 0;
     int* d_tabs = nullptr;
@@ -141,7 +142,7 @@ struct dgtd_op {
         for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
                         (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_probe_elem,
                         (void*)d_tfsf_elem, (void*)d_tfsf_sign, (void*)d_tfsf_xyz, (void*)d_tfsf_geo,
-                        (void*)d_lift, (void*)d_waves, (void*)d_rowptr, (void*)d_col, d_val,
+                        (void*)d_lift, (void*)d_waves, (void*)d_rowptr, (void*)d_col, (void*)d_etab, d_val,
                         (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
                         d_buf[1], d_buf[2], d_buf[3]})
             if (p) (void)hipFree(p);
@@ -163,7 +164,9 @@ struct dgtd_op {
         a.mats = static_cast<const T*>(d_mats);
         a.tabs = d_tabs;
         a.rowptr = d_rowptr;
-        a.col = d_col;
+        a.col16 = d_col;
+        a.etab = d_etab;
+        a.np = Np;
         a.val = static_cast<const T*>(d_val);
         a.nrows = ndofs;
         a.nnz = nnz;
@@ -447,8 +450,10 @@ dgtd::TriMeshInput tri_input(const dgtd_op* op)
 void upload_csr(dgtd_op* op, const dgtd::Csr& A)
 {
     op->nnz = (int64_t)A.val.size();
+    const dgtd::CsrLocal16 L = dgtd::csr_local16(A, (int64_t)op->N, op->Np);
     op->d_rowptr = static_cast<int64_t*>(upload(A.rowptr));
-    op->d_col = static_cast<int*>(upload(A.col));
+    op->d_col = static_cast<uint16_t*>(upload(L.col));
+    op->d_etab = static_cast<int*>(upload(L.etab));
     op->d_val = op->precision == 64 ? upload(A.val) : upload(dgtd::cast_vec<float>(A.val));
 }
 
@@ -1082,9 +1087,12 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops)
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     const double s = (double)op->real_size();
     const double Np = op->Np, Nfp = op->Nfp;
-    if (op->variant == DGTD_KERNEL_CSR) {  // per entry value + column; per row its pointer and the LSERK vectors (state, residual in/out, out)
+    if (op->variant == DGTD_KERNEL_CSR) {
+        // per entry its value and 16-bit column code; per row its pointer,
+        // its share of the block's element table (32 ints per 16 rows) and
+        // the LSERK vectors (state, residual in/out, out)
         const double rows = 6.0 * Np, nnz_e = (double)op->nnz / op->K;
-        if (bytes) *bytes = nnz_e * (s + 4.0) + rows * 8.0 + rows * 4.0 * s;
+        if (bytes) *bytes = nnz_e * (s + 2.0) + rows * (8.0 + 8.0) + rows * 4.0 * s;
         if (flops) *flops = 2.0 * nnz_e + 4.0 * rows;
         return DGTD_OK;
     }

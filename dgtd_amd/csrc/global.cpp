@@ -19,6 +19,8 @@
 // by 1/eps, H rows by 1/mu.
 #include "global.hpp"
 
+#include "kernels.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -203,6 +205,39 @@ void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double
 }
 
 }  // namespace
+
+CsrLocal16 csr_local16(const Csr& A, int64_t N, int Np)
+{
+    if (Np > 63) throw std::runtime_error("element-local columns need Np < 64");
+    const int64_t nb = (N + kCsrRowsPerBlock - 1) / kCsrRowsPerBlock;
+    CsrLocal16 out;
+    out.col.resize(A.col.size());
+    out.etab.assign((size_t)nb * kCsrTab, 0);
+    for (int64_t b = 0; b < nb; ++b) {
+        int32_t* tab = &out.etab[(size_t)b * kCsrTab];
+        int used = 0;
+        const int64_t d1 = std::min<int64_t>(N, (b + 1) * kCsrRowsPerBlock);
+        for (int c = 0; c < 6; ++c)
+            for (int64_t dof = b * kCsrRowsPerBlock; dof < d1; ++dof) {
+                const int64_t r = c * N + dof;
+                for (int64_t k = A.rowptr[r]; k < A.rowptr[r + 1]; ++k) {
+                    const int64_t col = A.col[k];
+                    const int comp = (int)(col / N);
+                    const int64_t rem = col % N;
+                    const int32_t elem = (int32_t)(rem / Np);
+                    const int j = (int)(rem % Np);
+                    int slot = 0;
+                    while (slot < used && tab[slot] != elem) ++slot;
+                    if (slot == used) {
+                        if (used == kCsrTab) throw std::runtime_error("a DOF block references too many elements");
+                        tab[used++] = elem;
+                    }
+                    out.col[k] = (uint16_t)(slot << 9 | comp << 6 | j);
+                }
+            }
+    }
+    return out;
+}
 
 Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha)
 {

@@ -22,4 +22,12 @@ struct Csr {
 Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha);
 Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* sigma, double alpha);
 
+// Element-local 16-bit column codes of A for the device kernel (layout in
+// kernels.hpp: kCsrRowsPerBlock, kCsrTab); N = Np * K.
+struct CsrLocal16 {
+    std::vector<uint16_t> col;   // one per entry of A
+    std::vector<int32_t> etab;   // kCsrTab per block of kCsrRowsPerBlock DOFs
+};
+CsrLocal16 csr_local16(const Csr& A, int64_t N, int Np);
+
 }  // namespace dgtd

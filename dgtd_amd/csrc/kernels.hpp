@@ -37,10 +37,19 @@ struct StageArgs {
     const int* order;    // MFMA variant: processing order of the 16-element tasks (NULL = identity)
     // assembled (Global) operator, rows and columns in the state layout
     const int64_t* rowptr;
-    const int* col;
+    const uint16_t* col16;  // element-local column codes (csr_local16)
+    const int* etab;        // kCsrTab element ids per 16-DOF block
     const T* val;
     int64_t nrows, nnz;
+    int np;  // nodes per element
 };
+
+// Device form of the assembled operator: the rows of DOFs [16 b, 16 b + 16)
+// (all six components) reference at most kCsrTab elements, listed in
+// etab[b * kCsrTab ...]; a column (component k, element etab[slot], node j)
+// is stored as the 16-bit code slot << 9 | k << 6 | j (nodes < 64).
+constexpr int kCsrRowsPerBlock = 16;
+constexpr int kCsrTab = 32;
 
 // Stage over the assembled operator (csr.hip): row r of the product is the
 // right-hand side of DOF r; the stage update as in the matrix-free kernels.
