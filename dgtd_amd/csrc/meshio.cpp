@@ -88,7 +88,11 @@ TetMesh read_gmsh22(const std::string& path)
                 else if (type == 2) nn = 3;
                 else if (type == 1) nn = 2;
                 else if (type == 15) continue;  // points dropped
-                else throw MeshUnsupported(path + ": element type " + std::to_string(type) + " is not supported (tetrahedra only)");
+                else if (type == 3 || type == 5)
+                    throw MeshUnsupported(path + ": quadrilateral and hexahedral elements are not supported");
+                else
+                    throw MeshUnsupported(path + ": element type " + std::to_string(type) +
+                                          " is not supported (straight-sided tetrahedra and triangles only)");
                 int32_t vs[4];
                 for (int k = 0; k < nn; ++k) {
                     long long nid;

@@ -433,3 +433,14 @@ def test_2d_case_refusals(tmp_path):
     (tmp_path / "r.json").write_text(json.dumps(case))
     with pytest.raises(D.DgtdError, match="refinement of triangle meshes"):
         D.Case(str(tmp_path / "r.json"))
+    # quadrilateral meshes (Gmsh element type 3) and curved (second-order)
+    # triangles (type 9) are refused by name
+    for etype, nodes, msg in ((3, "1 2 3 4", "quadrilateral"), (9, "1 2 3 4 5 6", "element type 9")):
+        msh = ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n6\n1 0 0 0\n2 1 0 0\n3 1 1 0\n4 0 1 0\n"
+               "5 0.5 0 0\n6 0.5 0.5 0\n$EndNodes\n$Elements\n1\n1 %d 2 1 1 %s\n$EndElements\n" % (etype, nodes))
+        (tmp_path / "q.msh").write_text(msh)
+        case["model"]["refinement"] = 0
+        case["model"]["filename"] = "q.msh"
+        (tmp_path / "q.json").write_text(json.dumps(case))
+        with pytest.raises(D.DgtdError, match=msg):
+            D.Case(str(tmp_path / "q.json"))
