@@ -91,12 +91,14 @@ struct dgtd_batch {
 
     size_t rs() const { return precision == 64 ? 8 : 4; }
 
-    // Orders 2-5 run on the matrix-core stage where it exists for the
-    // precision; order 1 (4 nodes, 16-column tiles mostly padding) and FP64
-    // order 5 run on the per-node stage.
+    // Orders 2-5 run on the matrix-core stage; order 1 (4 nodes, 16-column
+    // tiles mostly padding) and FP64 order 5 run on the per-node stage (the
+    // FP64 order-5 matrix-core stage reads its matrices from global memory
+    // and spills registers: 699 us against 730 us per stage on 48,000 tets,
+    // not worth a separate launch in a batch).
     bool use_mfma(int order) const
     {
-        if (order < 2) return false;
+        if (order < 2 || (precision == 64 && order == 5)) return false;
         return precision == 64 ? dgtd::mfma_available<double>(order) : dgtd::mfma_available<float>(order);
     }
 

@@ -79,19 +79,26 @@ struct MShape {
 constexpr int kTaskElems = kMfmaTaskElems;  // MFMA rows: 8 elements x {E, H} outputs
 constexpr int kGatherDepth = 2;  // flux-loop k-steps whose exterior traces are in flight
 
-// Waves per block, chosen so the block's LDS fits in 160 KiB.
+// Waves per block, chosen so the block's LDS fits in 160 KiB.  When the
+// reference matrices and kMinLdsWaves waves' tiles do not fit together (FP64
+// at order 5: 135 KiB of matrices), the matrices stay in global memory (read
+// through the caches; 135 KiB is L2-resident) and LDS holds the node tables
+// and the tiles only.
 template <int P, typename T>
 struct MWaves {
     static constexpr int kMaxWaves = sizeof(T) == 4 ? 12 : 8;
+    static constexpr int kMinLdsWaves = 4;
     using S = MShape<P>;
-    static constexpr size_t mats = sizeof(T) * ((size_t)S::KP * 3 * S::NPG + (size_t)4 * S::NFPP * S::NPG) +
-                                   sizeof(int) * (size_t)(4 * S::NFP + 96 * S::NFP);
+    static constexpr size_t bmats = sizeof(T) * ((size_t)S::KP * 3 * S::NPG + (size_t)4 * S::NFPP * S::NPG);
+    static constexpr size_t tables = sizeof(int) * (size_t)(4 * S::NFP + 96 * S::NFP);
     static constexpr size_t per_wave = sizeof(T) * (6 * kTaskElems * (size_t)S::US + 2 * kTaskElems * (size_t)S::NP);
     static constexpr size_t budget = 160 * 1024;
+    static constexpr bool bglobal = bmats + tables + kMinLdsWaves * per_wave > budget;
+    static constexpr size_t mats = (bglobal ? 0 : bmats) + tables;
     static constexpr int fit = (int)((budget - mats) / per_wave);
     static constexpr int W = fit >= kMaxWaves ? kMaxWaves : (fit < 1 ? 1 : fit);
     static constexpr size_t bytes = mats + per_wave * W;
-    static constexpr bool ok = bytes <= budget;  // FP64 at order 5 does not fit
+    static constexpr bool ok = bytes <= budget;
 };
 
 // Value of the lane 8 away within the 16-lane row (DPP row_ror:8 swaps the
@@ -120,8 +127,8 @@ __device__ __forceinline__ void wave_sync()
 // block, per-wave field and staging tiles).
 template <int P, typename T>
 struct MfmaLds {
-    T* sBv;   // [KP][3*NPG]
-    T* sBl;   // [4*NFPP][NPG], row f*NFPP + b
+    const T* sBv;  // [KP][3*NPG] (LDS, or global memory when MWaves::bglobal)
+    const T* sBl;  // [4*NFPP][NPG], row f*NFPP + b
     T* sU;    // [WAVES][6][TE][US]
     T* sS;    // [WAVES][2][TE*NP] output staging
     int* sFm; // [4*NFP]
@@ -137,16 +144,22 @@ __device__ MfmaLds<P, T> mfma_load_ref(const T* mats, const int* tabs, unsigned 
     constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, US = S::US, NFPP = S::NFPP;
     constexpr int TE = kTaskElems;
     constexpr int WAVES = MWaves<P, T>::W;
+    constexpr bool kBGlobal = MWaves<P, T>::bglobal;
     MfmaLds<P, T> L;
-    L.sBv = reinterpret_cast<T*>(smem_raw);
-    L.sBl = L.sBv + KP * 3 * NPG;
-    L.sU = L.sBl + 4 * NFPP * NPG;
+    T* sB = reinterpret_cast<T*>(smem_raw);
+    L.sU = kBGlobal ? sB : sB + KP * 3 * NPG + 4 * NFPP * NPG;
     L.sS = L.sU + WAVES * 6 * TE * US;
     L.sFm = reinterpret_cast<int*>(L.sS + WAVES * 2 * TE * NP);
     L.sPt = L.sFm + 4 * NFP;
     const int tid = threadIdx.x;
     const int nthreads = WAVES * 64;
-    for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) L.sBv[i] = mats[i];
+    if constexpr (kBGlobal) {
+        L.sBv = mats;
+    } else {
+        for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) sB[i] = mats[i];
+        L.sBv = sB;
+    }
+    L.sBl = L.sBv + KP * 3 * NPG;
     for (int i = tid; i < 4 * NFP + 96 * NFP; i += nthreads) L.sFm[i] = tabs[i];
     for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) L.sU[i] = T(0);
     __syncthreads();

@@ -33,11 +33,6 @@ KERNELS = ["valu", "mfma"]
 
 
 def make_gpu(*args, kernel="valu", **kw):
-    """The MFMA variant has no FP64 build at order 5 (its LDS tiles exceed 160 KiB)."""
-    if kernel == "mfma" and kw.get("precision") == 64 and args[3] == 5:
-        with pytest.raises(D.DgtdError):
-            D.HesthavenEvolution(*args, kernel=kernel, **kw)
-        pytest.skip("no FP64 MFMA variant at order 5 (creation refused, as checked)")
     return D.HesthavenEvolution(*args, kernel=kernel, **kw)
 
 
