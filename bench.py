@@ -237,7 +237,7 @@ def bench_tri(D, args):
                        f"{dof} DOF, Ez Gaussian at ({c[0]},{c[1]}), {steps} LSERK4 steps", "dof": dof}
     states = {}
     for prec in (32, 64):
-        op = D.HesthavenEvolution(vx, E, fbc, order, precision=prec, integrator="lserk4")
+        op = D.HesthavenEvolution(vx, E, fbc, order, precision=prec, integrator="lserk4", kernel=args.kernel)
         dt = 0.5 / (op.max_fscale() * order ** 2)
         op.set_state(x0)
         e0 = op.norms()["energy"]
@@ -251,7 +251,10 @@ def bench_tri(D, args):
         b, f = op.work_per_element()
         states[prec] = op.get_state()
         out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
-                            "kernel_avg_us": us, "kernel": f"dg_tri_stage<{order},{'float' if prec == 32 else 'double'},LSERK>",
+                            "kernel_avg_us": us, "kernel_variant": op.kernel_variant(),
+                            "kernel": (f"dg_stage_mfma<{order},{'float' if prec == 32 else 'double'},LSERK,2>"
+                                       if op.kernel_variant() == "mfma" else
+                                       f"dg_tri_stage<{order},{'float' if prec == 32 else 'double'},LSERK>"),
                             "algorithmic_bytes_per_launch": b * K,
                             "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
                             "achieved_tflops": f * K / (us * 1e-6) / 1e12,

@@ -62,11 +62,18 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
 };
 
-template <int P>
+// Element shape: tetrahedra (DIM 3) or triangles (DIM 2, the 2D operator:
+// d/dz = 0, n_z = 0, edges as faces; records as tri.hpp lays them out).
+template <int P, int DIM = 3>
 struct MShape {
-    static constexpr int NP = (P + 1) * (P + 2) * (P + 3) / 6;
-    static constexpr int NFP = (P + 1) * (P + 2) / 2;
-    static constexpr int NF = 4 * NFP;
+    static constexpr int NFACE = DIM == 3 ? 4 : 3;
+    static constexpr int NDIR = DIM;  // reference derivative directions
+    static constexpr int NP = DIM == 3 ? (P + 1) * (P + 2) * (P + 3) / 6 : (P + 1) * (P + 2) / 2;
+    static constexpr int NFP = DIM == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
+    static constexpr int NF = NFACE * NFP;
+    static constexpr int NPERM = DIM == 3 ? 96 : 18;  // rows of the neighbour-node table
+    static constexpr int GEO = DIM == 3 ? kGeoStrideDev : kTriGeoStrideDev;
+    static constexpr int CONN = DIM == 3 ? kConnStrideDev : kTriConnStrideDev;
     static constexpr int KP = (NP + 3) / 4 * 4;  // K padding of the volume product
     static constexpr int S = KP / 4;             // volume k-steps
     static constexpr int NPG = (NP + 15) / 16 * 16;
@@ -84,13 +91,14 @@ constexpr int kGatherDepth = 2;  // flux-loop k-steps whose exterior traces are 
 // at order 5: 135 KiB of matrices), the matrices stay in global memory (read
 // through the caches; 135 KiB is L2-resident) and LDS holds the node tables
 // and the tiles only.
-template <int P, typename T>
+template <int P, typename T, int DIM = 3>
 struct MWaves {
     static constexpr int kMaxWaves = sizeof(T) == 4 ? 12 : 8;
     static constexpr int kMinLdsWaves = 4;
-    using S = MShape<P>;
-    static constexpr size_t bmats = sizeof(T) * ((size_t)S::KP * 3 * S::NPG + (size_t)4 * S::NFPP * S::NPG);
-    static constexpr size_t tables = sizeof(int) * (size_t)(4 * S::NFP + 96 * S::NFP);
+    using S = MShape<P, DIM>;
+    static constexpr size_t bmats =
+        sizeof(T) * ((size_t)S::KP * S::NDIR * S::NPG + (size_t)S::NFACE * S::NFPP * S::NPG);
+    static constexpr size_t tables = sizeof(int) * (size_t)(S::NFACE * S::NFP + S::NPERM * S::NFP);
     static constexpr size_t per_wave = sizeof(T) * (6 * kTaskElems * (size_t)S::US + 2 * kTaskElems * (size_t)S::NP);
     static constexpr size_t budget = 160 * 1024;
     static constexpr bool bglobal = bmats + tables + kMinLdsWaves * per_wave > budget;
@@ -125,7 +133,7 @@ __device__ __forceinline__ void wave_sync()
 
 // Per-block LDS carve-up of the MFMA stage (reference matrices shared by the
 // block, per-wave field and staging tiles).
-template <int P, typename T>
+template <int P, typename T, int DIM = 3>
 struct MfmaLds {
     const T* sBv;  // [KP][3*NPG] (LDS, or global memory when MWaves::bglobal)
     const T* sBl;  // [4*NFPP][NPG], row f*NFPP + b
@@ -137,30 +145,31 @@ struct MfmaLds {
 
 // Loads the reference matrices and node tables of order P into LDS and zeroes
 // the field tiles (their padding columns j >= NP stay 0); block-wide.
-template <int P, typename T>
-__device__ MfmaLds<P, T> mfma_load_ref(const T* mats, const int* tabs, unsigned char* smem_raw)
+template <int P, typename T, int DIM = 3>
+__device__ MfmaLds<P, T, DIM> mfma_load_ref(const T* mats, const int* tabs, unsigned char* smem_raw)
 {
-    using S = MShape<P>;
+    using S = MShape<P, DIM>;
     constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, US = S::US, NFPP = S::NFPP;
+    constexpr int NDIR = S::NDIR, NFACE = S::NFACE;
     constexpr int TE = kTaskElems;
-    constexpr int WAVES = MWaves<P, T>::W;
-    constexpr bool kBGlobal = MWaves<P, T>::bglobal;
-    MfmaLds<P, T> L;
+    constexpr int WAVES = MWaves<P, T, DIM>::W;
+    constexpr bool kBGlobal = MWaves<P, T, DIM>::bglobal;
+    MfmaLds<P, T, DIM> L;
     T* sB = reinterpret_cast<T*>(smem_raw);
-    L.sU = kBGlobal ? sB : sB + KP * 3 * NPG + 4 * NFPP * NPG;
+    L.sU = kBGlobal ? sB : sB + KP * NDIR * NPG + NFACE * NFPP * NPG;
     L.sS = L.sU + WAVES * 6 * TE * US;
     L.sFm = reinterpret_cast<int*>(L.sS + WAVES * 2 * TE * NP);
-    L.sPt = L.sFm + 4 * NFP;
+    L.sPt = L.sFm + NFACE * NFP;
     const int tid = threadIdx.x;
     const int nthreads = WAVES * 64;
     if constexpr (kBGlobal) {
         L.sBv = mats;
     } else {
-        for (int i = tid; i < KP * 3 * NPG + 4 * NFPP * NPG; i += nthreads) sB[i] = mats[i];
+        for (int i = tid; i < KP * NDIR * NPG + NFACE * NFPP * NPG; i += nthreads) sB[i] = mats[i];
         L.sBv = sB;
     }
-    L.sBl = L.sBv + KP * 3 * NPG;
-    for (int i = tid; i < 4 * NFP + 96 * NFP; i += nthreads) L.sFm[i] = tabs[i];
+    L.sBl = L.sBv + KP * NDIR * NPG;
+    for (int i = tid; i < NFACE * NFP + S::NPERM * NFP; i += nthreads) L.sFm[i] = tabs[i];
     for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) L.sU[i] = T(0);
     __syncthreads();
     return L;
@@ -168,10 +177,11 @@ __device__ MfmaLds<P, T> mfma_load_ref(const T* mats, const int* tabs, unsigned 
 
 // One wave-task: the fused stage of elements [task*8, task*8 + 8) of the mesh
 // described by `a`.  Only wave-level synchronisation inside.
-template <int P, typename T, int MODE>
-__device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P, T>& L, const int task)
+template <int P, typename T, int MODE, int DIM = 3>
+__device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P, T, DIM>& L, const int task)
 {
-    using S = MShape<P>;
+    using S = MShape<P, DIM>;
+    constexpr int NFACE = S::NFACE, NDIR = S::NDIR;
     using MM = Mfma<T>;
     using acc_t = typename MM::acc_t;
     constexpr int NP = S::NP, NFP = S::NFP, NPG = S::NPG, G = S::G, US = S::US;
@@ -205,10 +215,10 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     wave_sync();  // previous task's reads of U are done
     const bool valid = el < ne;
     const int e = e0 + (valid ? el : 0);
-    const T* ge = a.geo + (size_t)e * kGeoStrideDev;
-    T gm[9], gface[4][4];  // metric terms; per face (nx, ny, nz, fscale/2)
+    const T* ge = a.geo + (size_t)e * S::GEO;
+    T gm[9], gface[NFACE][4];  // metric terms dr_k/dx_d [k][d]; per face (nx, ny, nz, fscale/2)
     T scale;
-    int fnb[4], fcode[4];
+    int fnb[NFACE], fcode[NFACE];
     {
         constexpr int NL = (TE * NP + 63) / 64;
         const int cnt = ne * NP;
@@ -222,19 +232,35 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
                 v[c][k] = src[t < cnt ? t : cnt - 1];
             }
         }
+        if constexpr (DIM == 3) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) gm[k] = ge[k];
+            for (int k = 0; k < 9; ++k) gm[k] = ge[k];
 #pragma unroll
-        for (int f = 0; f < 4; ++f)
+            for (int f = 0; f < 4; ++f)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) gface[f][k] = ge[9 + 4 * f + k];
+                for (int k = 0; k < 4; ++k) gface[f][k] = ge[9 + 4 * f + k];
+        } else {  // [0..3] dr/dx dr/dy ds/dx ds/dy; edge f: [4+3f..6+3f] nx ny fscale/2
+#pragma unroll
+            for (int k = 0; k < 9; ++k) gm[k] = T(0);
+            gm[0] = ge[0];
+            gm[1] = ge[1];
+            gm[3] = ge[2];
+            gm[4] = ge[3];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                gface[f][0] = ge[4 + 3 * f];
+                gface[f][1] = ge[5 + 3 * f];
+                gface[f][2] = T(0);
+                gface[f][3] = ge[6 + 3 * f];
+            }
+        }
         // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
-        const T sc = ge[25 + half];
+        const T sc = ge[S::GEO - 2 + half];
         scale = valid ? sc : T(0);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            fnb[f] = a.conn[(size_t)e * kConnStrideDev + f];
-            fcode[f] = a.conn[(size_t)e * kConnStrideDev + 4 + f];
+        for (int f = 0; f < NFACE; ++f) {
+            fnb[f] = a.conn[(size_t)e * S::CONN + f];
+            fcode[f] = a.conn[(size_t)e * S::CONN + NFACE + f];
         }
         // rows ne..TE-1 of a partial task receive copies of the last valid
         // value (only the rows of the padding elements read them); lanes past
@@ -292,12 +318,12 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = src[c * N];
     };
-    constexpr int D = kGatherDepth < 4 * SF ? kGatherDepth : 4 * SF;
+    constexpr int D = kGatherDepth < NFACE * SF ? kGatherDepth : NFACE * SF;
     T pf[D][3];
 #pragma unroll
     for (int d = 0; d < D; ++d) gather(d, pf[d]);
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < NFACE; ++f) {
         const int nb = fnb[f];
         const bool interior = nb >= 0;
         const bool remote = face_remote(f);
@@ -334,7 +360,7 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
 #pragma unroll
                 for (int c = 0; c < 3; ++c) cur[c] = pf[s % D][c];
             }
-            if (s + D < 4 * SF) gather(s + D, pf[s % D]);
+            if (s + D < NFACE * SF) gather(s + D, pf[s % D]);
             __builtin_amdgcn_sched_barrier(0);  // the prefetch stays D k-steps ahead of its use
             T B3[3], A3[3];
 #pragma unroll
@@ -369,9 +395,9 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
         T src[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
-        const T* bv = sBv + j * 3 * NPG + li;
+        const T* bv = sBv + j * NDIR * NPG + li;
 #pragma unroll
-        for (int kd = 0; kd < 3; ++kd) {
+        for (int kd = 0; kd < NDIR; ++kd) {
             const T* gk = gm + 3 * kd;
             T w[3];
 #pragma unroll
@@ -463,13 +489,13 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     }
 }
 
-template <int P, typename T, int MODE>
-__global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArgs<T> a)
+template <int P, typename T, int MODE, int DIM>
+__global__ __launch_bounds__((MWaves<P, T, DIM>::W * 64)) void dg_stage_mfma(StageArgs<T> a)
 {
     constexpr int TE = kTaskElems;
-    constexpr int WAVES = MWaves<P, T>::W;
+    constexpr int WAVES = MWaves<P, T, DIM>::W;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const MfmaLds<P, T> L = mfma_load_ref<P, T>(a.mats, a.tabs, smem_raw);
+    const MfmaLds<P, T, DIM> L = mfma_load_ref<P, T, DIM>(a.mats, a.tabs, smem_raw);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int ntasks = (a.K + TE - 1) / TE;
     // XCD-contiguous task ranges: block b runs on XCD b % 8.
@@ -481,7 +507,7 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_stage_mfma(StageArg
     const int t_begin = xcd * per_xcd;
     const int t_end = min(ntasks, t_begin + per_xcd);
     for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES)
-        mfma_task<P, T, MODE>(a, L, a.order ? a.order[tix] : tix);
+        mfma_task<P, T, MODE, DIM>(a, L, a.order ? a.order[tix] : tix);
 }
 
 // Batch variant: the running instances of order P, one (instance, task) work
@@ -515,46 +541,51 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_batch_stage_mfma(Ba
     }
 }
 
-template <int P, typename T, int MODE>
+template <int P, typename T, int MODE, int DIM>
 void mfma_set_attr()
 {
-    if constexpr (!MWaves<P, T>::ok) return;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_stage_mfma<P, T, MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)MWaves<P, T>::bytes);
+    if constexpr (!MWaves<P, T, DIM>::ok) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_stage_mfma<P, T, MODE, DIM>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)MWaves<P, T, DIM>::bytes);
 }
 
-template <int P, typename T>
+// Sets the LDS attribute of every mode (set-up time, outside any capture)
+// and returns the resident blocks per CU (0: no variant for this order).
+template <int P, typename T, int DIM>
 int mfma_prepare()
 {
-    if constexpr (!MWaves<P, T>::ok) return 0;
-    mfma_set_attr<P, T, kModeMult>();
-    mfma_set_attr<P, T, kModeLserk>();
-    mfma_set_attr<P, T, kModeRk1>();
-    mfma_set_attr<P, T, kModeRk2>();
-    mfma_set_attr<P, T, kModeRk3>();
-    mfma_set_attr<P, T, kModeRk4>();
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)MWaves<P, T>::bytes);
+    using W = MWaves<P, T, DIM>;
+    if constexpr (!W::ok) return 0;
+    mfma_set_attr<P, T, kModeMult, DIM>();
+    mfma_set_attr<P, T, kModeLserk, DIM>();
+    mfma_set_attr<P, T, kModeRk1, DIM>();
+    mfma_set_attr<P, T, kModeRk2, DIM>();
+    mfma_set_attr<P, T, kModeRk3, DIM>();
+    mfma_set_attr<P, T, kModeRk4, DIM>();
+    if constexpr (DIM == 3)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)W::bytes);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_stage_mfma<P, T, kModeLserk>,
-                                                     MWaves<P, T>::W * 64, MWaves<P, T>::bytes) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_stage_mfma<P, T, kModeLserk, DIM>, W::W * 64,
+                                                     W::bytes) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
-template <int P, typename T>
+template <int P, typename T, int DIM>
 hipError_t mfma_launch(const StageArgs<T>& a, int mode, int grid, hipStream_t s)
 {
-    constexpr int threads = MWaves<P, T>::W * 64;
-    constexpr size_t bytes = MWaves<P, T>::bytes;
-    if constexpr (!MWaves<P, T>::ok) return hipErrorInvalidConfiguration;
+    using W = MWaves<P, T, DIM>;
+    constexpr int threads = W::W * 64;
+    constexpr size_t bytes = W::bytes;
+    if constexpr (!W::ok) return hipErrorInvalidConfiguration;
     switch (mode) {
-        case kModeMult: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeMult>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeLserk: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeLserk>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk1: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk1>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk2: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk2>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk3: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk3>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk4: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk4>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeMult: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeMult, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeLserk: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeLserk, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeRk1: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk1, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeRk2: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk2, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeRk3: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk3, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
+        case kModeRk4: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk4, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -605,11 +636,11 @@ template <typename T>
 hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s)
 {
     switch (order) {
-        case 1: return mfma_launch<1, T>(a, mode, grid, s);
-        case 2: return mfma_launch<2, T>(a, mode, grid, s);
-        case 3: return mfma_launch<3, T>(a, mode, grid, s);
-        case 4: return mfma_launch<4, T>(a, mode, grid, s);
-        case 5: return mfma_launch<5, T>(a, mode, grid, s);
+        case 1: return mfma_launch<1, T, 3>(a, mode, grid, s);
+        case 2: return mfma_launch<2, T, 3>(a, mode, grid, s);
+        case 3: return mfma_launch<3, T, 3>(a, mode, grid, s);
+        case 4: return mfma_launch<4, T, 3>(a, mode, grid, s);
+        case 5: return mfma_launch<5, T, 3>(a, mode, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -618,11 +649,11 @@ template <typename T>
 int mfma_blocks_per_cu(int order)
 {
     switch (order) {
-        case 1: return mfma_prepare<1, T>();
-        case 2: return mfma_prepare<2, T>();
-        case 3: return mfma_prepare<3, T>();
-        case 4: return mfma_prepare<4, T>();
-        case 5: return mfma_prepare<5, T>();
+        case 1: return mfma_prepare<1, T, 3>();
+        case 2: return mfma_prepare<2, T, 3>();
+        case 3: return mfma_prepare<3, T, 3>();
+        case 4: return mfma_prepare<4, T, 3>();
+        case 5: return mfma_prepare<5, T, 3>();
         default: return 1;
     }
 }
@@ -640,6 +671,51 @@ int mfma_waves_per_block(int order)
     }
 }
 
+template <typename T>
+hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s)
+{
+    switch (order) {
+        case 1: return mfma_launch<1, T, 2>(a, mode, grid, s);
+        case 2: return mfma_launch<2, T, 2>(a, mode, grid, s);
+        case 3: return mfma_launch<3, T, 2>(a, mode, grid, s);
+        case 4: return mfma_launch<4, T, 2>(a, mode, grid, s);
+        case 5: return mfma_launch<5, T, 2>(a, mode, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+int tri_mfma_blocks_per_cu(int order)
+{
+    switch (order) {
+        case 1: return mfma_prepare<1, T, 2>();
+        case 2: return mfma_prepare<2, T, 2>();
+        case 3: return mfma_prepare<3, T, 2>();
+        case 4: return mfma_prepare<4, T, 2>();
+        case 5: return mfma_prepare<5, T, 2>();
+        default: return 0;
+    }
+}
+
+template <typename T>
+int tri_mfma_waves_per_block(int order)
+{
+    switch (order) {
+        case 1: return MWaves<1, T, 2>::W;
+        case 2: return MWaves<2, T, 2>::W;
+        case 3: return MWaves<3, T, 2>::W;
+        case 4: return MWaves<4, T, 2>::W;
+        case 5: return MWaves<5, T, 2>::W;
+        default: return 1;
+    }
+}
+
+template hipError_t launch_tri_stage_mfma<float>(int, const StageArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_tri_stage_mfma<double>(int, const StageArgs<double>&, int, int, hipStream_t);
+template int tri_mfma_blocks_per_cu<float>(int);
+template int tri_mfma_blocks_per_cu<double>(int);
+template int tri_mfma_waves_per_block<float>(int);
+template int tri_mfma_waves_per_block<double>(int);
 template hipError_t launch_stage_mfma<float>(int, const StageArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_stage_mfma<double>(int, const StageArgs<double>&, int, int, hipStream_t);
 template int mfma_blocks_per_cu<float>(int);

@@ -177,13 +177,14 @@ struct dgtd_op {
     hipError_t launch_on(const dgtd::StageArgs<T>& a0, int mode, int var, hipStream_t s) const
     {
         if (var == DGTD_KERNEL_CSR) return dgtd::launch_csr_stage<T>(a0, mode, s);
-        if (dim == 2) return dgtd::launch_tri_stage<T>(T2.p, a0, mode, grid, s);
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
             a.mats = static_cast<const T*>(d_mats_mfma);
             a.order = d_order;
+            if (dim == 2) return dgtd::launch_tri_stage_mfma<T>(T2.p, a, mode, grid_mfma, s);
             return dgtd::launch_stage_mfma<T>(R.p, a, mode, grid_mfma, s);
         }
+        if (dim == 2) return dgtd::launch_tri_stage<T>(T2.p, a0, mode, grid, s);
         return dgtd::launch_stage<T>(R.p, a0, mode, grid, s);
     }
 
@@ -379,16 +380,18 @@ namespace {
 std::vector<int> morton_task_order(const dgtd_mesh_desc* mesh)
 {
     const int TE = dgtd::kMfmaTaskElems;
+    const int nvert = mesh->dim == 2 ? 3 : 4;
     const int K = mesh->K, nt = (K + TE - 1) / TE;
     std::vector<double> c(3 * (size_t)nt, 0.0);
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int t = 0; t < nt; ++t) {
         const int e0 = TE * t, e1 = std::min(K, e0 + TE);
         for (int e = e0; e < e1; ++e)
-            for (int v = 0; v < 4; ++v)
-                for (int d = 0; d < 3; ++d) c[3 * (size_t)t + d] += mesh->vx[3 * (size_t)mesh->EToV[4 * (size_t)e + v] + d];
+            for (int v = 0; v < nvert; ++v)
+                for (int d = 0; d < 3; ++d)
+                    c[3 * (size_t)t + d] += mesh->vx[3 * (size_t)mesh->EToV[nvert * (size_t)e + v] + d];
         for (int d = 0; d < 3; ++d) {
-            c[3 * (size_t)t + d] /= 4.0 * (e1 - e0);
+            c[3 * (size_t)t + d] /= (double)nvert * (e1 - e0);
             lo[d] = std::min(lo[d], c[3 * (size_t)t + d]);
             hi[d] = std::max(hi[d], c[3 * (size_t)t + d]);
         }
@@ -550,12 +553,33 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     op->nblk = (op->K + epb - 1) / epb;
     op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * 4));
     op->variant = DGTD_KERNEL_VALU;
+    if (op->precision == 64) dgtd::tri_prepare<double>(R.p);
+    else dgtd::tri_prepare<float>(R.p);
+    // matrix-core variant: 8-triangle wave tasks in Morton order
+    const int mbpc = op->precision == 64 ? dgtd::tri_mfma_blocks_per_cu<double>(R.p)
+                                         : dgtd::tri_mfma_blocks_per_cu<float>(R.p);
+    if (mbpc > 0) {
+        const int mwaves = op->precision == 64 ? dgtd::tri_mfma_waves_per_block<double>(R.p)
+                                               : dgtd::tri_mfma_waves_per_block<float>(R.p);
+        op->d_mats_mfma = op->precision == 64 ? upload(dgtd::pack_mats_mfma<double>(R))
+                                              : upload(dgtd::pack_mats_mfma<float>(R));
+        op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
+        const int ntasks = (op->K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
+        int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
+        if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)
+        op->grid_mfma = std::max(1, g);
+    }
+    if (kernel == DGTD_KERNEL_MFMA && op->grid_mfma <= 0)
+        throw std::runtime_error("the MFMA kernel does not fit this order and precision");
     if (kernel == DGTD_KERNEL_CSR) {
         upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha));
         op->variant = DGTD_KERNEL_CSR;
+    } else if (kernel == DGTD_KERNEL_AUTO) {
+        if (op->precision == 64) op->autotune<double>();
+        else op->autotune<float>();
+    } else {
+        op->variant = kernel;
     }
-    if (op->precision == 64) dgtd::tri_prepare<double>(R.p);
-    else dgtd::tri_prepare<float>(R.p);
 }
 
 dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
@@ -594,8 +618,6 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
         return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
-    if (mesh->dim == 2 && opts->kernel == DGTD_KERNEL_MFMA)
-        return fail(DGTD_E_UNSUPPORTED, "triangles run the per-node kernel only");
     dgtd_op* op = nullptr;
     try {
         op = new dgtd_op;

@@ -10,6 +10,8 @@ namespace dgtd {
 
 constexpr int kGeoStrideDev = 27;  // matches mesh.hpp kGeoStride
 constexpr int kConnStrideDev = 8;  // matches mesh.hpp kConnStride
+constexpr int kTriGeoStrideDev = 15;  // matches tri.hpp kTriGeoStride
+constexpr int kTriConnStrideDev = 6;  // matches tri.hpp kTriConnStride
 
 // Stage variants of the fused kernel.
 enum StageMode : int {
@@ -88,6 +90,14 @@ template <typename T>
 int mfma_blocks_per_cu(int order);
 template <typename T>
 int mfma_waves_per_block(int order);
+// The same matrix-core stage on triangles (the 2D operator; records as in
+// tri.hpp, mats Bv [KP][2*NPG] then Bl [3*NFPP][NPG]).
+template <typename T>
+hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+template <typename T>
+int tri_mfma_blocks_per_cu(int order);
+template <typename T>
+int tri_mfma_waves_per_block(int order);
 
 // ---------------------------------------------------------------------------
 // Multi-instance batch (dg_batch.hip): many meshes of different size and order

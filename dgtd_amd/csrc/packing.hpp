@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "reftet.hpp"
+#include "tri.hpp"
 
 namespace dgtd {
 
@@ -39,6 +40,26 @@ inline std::vector<T> pack_mats_mfma(const RefTet& R)
             for (int i = 0; i < Np; ++i) m[(size_t)j * 3 * NPG + kd * NPG + i] = (T)(*D[kd])[(size_t)i * Np + j];
     T* L = m.data() + (size_t)KP * 3 * NPG;
     for (int f = 0; f < 4; ++f)
+        for (int b = 0; b < Nfp; ++b)
+            for (int i = 0; i < Np; ++i)
+                L[(size_t)(f * NFPP + b) * NPG + i] = (T)R.LIFT[(size_t)i * NF + f * Nfp + b];
+    return m;
+}
+
+// Matrix-core layout on triangles: Bv [KP][2*NPG] (Dr, Ds transposed per input
+// node j), then Bl [3*NFPP][NPG] (LIFT columns, each edge padded to whole k-steps).
+template <typename T>
+inline std::vector<T> pack_mats_mfma(const RefTri& R)
+{
+    const int Np = R.Np, Nfp = R.Nfp, NF = 3 * Nfp;
+    const int KP = (Np + 3) / 4 * 4, NPG = (Np + 15) / 16 * 16, NFPP = (Nfp + 3) / 4 * 4;
+    std::vector<T> m((size_t)KP * 2 * NPG + (size_t)3 * NFPP * NPG, T(0));
+    const std::vector<double>* D[2] = {&R.Dr, &R.Ds};
+    for (int j = 0; j < Np; ++j)
+        for (int kd = 0; kd < 2; ++kd)
+            for (int i = 0; i < Np; ++i) m[(size_t)j * 2 * NPG + kd * NPG + i] = (T)(*D[kd])[(size_t)i * Np + j];
+    T* L = m.data() + (size_t)KP * 2 * NPG;
+    for (int f = 0; f < 3; ++f)
         for (int b = 0; b < Nfp; ++b)
             for (int i = 0; i < Np; ++i)
                 L[(size_t)(f * NFPP + b) * NPG + i] = (T)R.LIFT[(size_t)i * NF + f * Nfp + b];

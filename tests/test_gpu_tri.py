@@ -1,7 +1,8 @@
 """Parity of the HIP triangle (2D) operator, through the C ABI, with the 2D
 oracle (oracle/oracle2d.py).
 
-Bars as for tetrahedra: FP64 Mult and RK steps agree with the oracle to
+Both stage variants (per-node VALU and matrix-core MFMA, 8-triangle wave
+tasks).  Bars as for tetrahedra: FP64 Mult and RK steps agree with the oracle to
 1e-12 relative (summation order only), FP64 runs to 1e-11 relative L2, FP32
 agrees with FP64 to 1e-5 relative L2.
 """
@@ -37,12 +38,17 @@ def mixed_case(n, seed=0, media=True, interior=None):
     return vx, E, bc, eps, mu
 
 
+KERNELS = ["valu", "mfma"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("alpha", [1.0, 0.5])
-def test_tri_mult_fp64_matches_oracle(order, alpha):
+def test_tri_mult_fp64_matches_oracle(order, alpha, kernel):
     vx, E, bc, eps, mu = mixed_case(4, seed=order)
     ref = O.Operator2D(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu)
-    gpu = D.HesthavenEvolution(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, precision=64)
+    gpu = D.HesthavenEvolution(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, precision=64, kernel=kernel)
+    assert gpu.kernel_variant() == kernel
     assert gpu.Np == ref.Np and gpu.ndofs == ref.ndofs()
     x = np.random.default_rng(7).standard_normal(ref.ndofs())
     y = gpu.mult(x)
@@ -55,34 +61,37 @@ def test_tri_mult_fp64_matches_oracle(order, alpha):
     assert np.abs(xyz[2]).max() == 0.0
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("code", [4, 5, 6])
-def test_tri_interior_conditions_match_oracle(code):
+def test_tri_interior_conditions_match_oracle(code, kernel):
     vx, E, bc, eps, mu = mixed_case(4, seed=code, interior=code)
     ref = O.Operator2D(vx, E, bc, 3, alpha=1.0, eps=eps, mu=mu)
-    gpu = D.HesthavenEvolution(vx, E, bc, 3, alpha=1.0, eps=eps, mu=mu, precision=64)
+    gpu = D.HesthavenEvolution(vx, E, bc, 3, alpha=1.0, eps=eps, mu=mu, precision=64, kernel=kernel)
     x = np.random.default_rng(1).standard_normal(ref.ndofs())
     yr = ref.mult(x)
     assert np.abs(gpu.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
-@pytest.mark.parametrize("order", [2, 4])
-def test_tri_mult_fp32_close_to_fp64(order):
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_tri_mult_fp32_close_to_fp64(order, kernel):
     vx, E, bc, eps, mu = mixed_case(6, seed=3)
     x = np.random.default_rng(2).standard_normal(6 * (order + 1) * (order + 2) // 2 * E.shape[0])
-    y64 = D.HesthavenEvolution(vx, E, bc, order, eps=eps, mu=mu, precision=64).mult(x)
-    y32 = D.HesthavenEvolution(vx, E, bc, order, eps=eps, mu=mu, precision=32).mult(x)
+    y64 = D.HesthavenEvolution(vx, E, bc, order, eps=eps, mu=mu, precision=64, kernel="valu").mult(x)
+    y32 = D.HesthavenEvolution(vx, E, bc, order, eps=eps, mu=mu, precision=32, kernel=kernel).mult(x)
     assert rel(y32, y64) <= 1e-5
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("integrator", ["lserk4", "rk4"])
-def test_tri_run_matches_oracle(integrator):
+def test_tri_run_matches_oracle(integrator, kernel):
     vx, E, bc = G.square_mesh(6, bc="pec")
     ref = O.Operator2D(vx, E, bc, 3)
     x0 = G.tm_mode(ref.pos, 0.0).reshape(-1)
     dt = 0.5 / (ref.max_fscale() * 9)
     T = 40.5 * dt  # 41 steps, the last truncated (Solver::run)
     xr, steps, t = ref.run(x0, dt, T, integrator)
-    gpu = D.HesthavenEvolution(vx, E, bc, 3, precision=64, integrator=integrator)
+    gpu = D.HesthavenEvolution(vx, E, bc, 3, precision=64, integrator=integrator, kernel=kernel)
     gpu.set_state(x0)
     n = gpu.run(dt, T)
     assert n == steps == 41
@@ -129,8 +138,6 @@ def test_tri_probes_match_interpolant():
 
 def test_tri_refusals():
     vx, E, bc = G.square_mesh(2, bc="pec")
-    with pytest.raises(D.DgtdError):
-        D.HesthavenEvolution(vx, E, bc, 2, kernel="mfma")
     bad = bc.copy()
     bad[bad == 0] = 1  # PEC on interior edges
     with pytest.raises(D.DgtdError):
@@ -157,12 +164,13 @@ def _tfsf_case(n=8, order=3, alpha=1.0):
     return vx, E, bc, waves, gw
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", [2, 4])
-def test_tri_tfsf_mult_matches_oracle(order):
+def test_tri_tfsf_mult_matches_oracle(order, kernel):
     vx, E, bc, waves, gw = _tfsf_case(order=order)
     ref = O.Operator2D(vx, E, bc, order, alpha=0.7)
     ref.set_planewaves(waves)
-    gpu = D.HesthavenEvolution(vx, E, bc, order, alpha=0.7, precision=64)
+    gpu = D.HesthavenEvolution(vx, E, bc, order, alpha=0.7, precision=64, kernel=kernel)
     gpu.set_planewaves(gw)
     x = np.random.default_rng(3).standard_normal(ref.ndofs())
     for t in (0.0, 0.8, 1.7):
