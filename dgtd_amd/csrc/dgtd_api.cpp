@@ -194,8 +194,10 @@ struct dgtd_op {
         hip_check(launch_on<T>(a, mode, variant, stream), "stage kernel launch");
     }
 
-    // Times each available variant (operator application on the zeroed state
-    // buffers) and keeps the fastest.
+    // Times each available variant on the zeroed state buffers and keeps the
+    // fastest: the fused LSERK stage for LSERK4 operators (the launch that
+    // steps run; with ca = cb = dt = 0 the buffers stay zero), the operator
+    // application otherwise.
     template <typename T>
     void autotune()
     {
@@ -203,12 +205,20 @@ struct dgtd_op {
         auto a = base_args<T>();
         a.qin = static_cast<const T*>(d_buf[0]);
         a.out = static_cast<T*>(d_buf[1]);
+        int mode = dgtd::kModeMult;
+        if (integrator == DGTD_LSERK4) {
+            mode = dgtd::kModeLserk;
+            a.res = static_cast<T*>(d_buf[2]);
+            a.ca = T(0);
+            a.cb = T(0);
+            a.dt = T(0);
+        }
         float best = 1e30f;
         int best_var = DGTD_KERNEL_VALU;
         for (int var : {DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA}) {
-            hip_check(launch_on<T>(a, dgtd::kModeMult, var, stream), "autotune launch");
+            hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");
             hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
-            for (int r = 0; r < 3; ++r) hip_check(launch_on<T>(a, dgtd::kModeMult, var, stream), "autotune launch");
+            for (int r = 0; r < 3; ++r) hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");
             hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
             hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
             float ms = 0.f;
