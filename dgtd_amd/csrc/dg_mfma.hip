@@ -414,6 +414,9 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
         }
     }
 
+    // The residual and base-state reads and the stage's stores are used once
+    // per stage: they stream non-temporally, so L2 keeps the fields that
+    // neighbouring tasks gather.
     // ---- stage update: per x, the E_x and H_x right-hand sides go through
     // the wave's LDS staging buffer and the update runs as a linear pass
     // over the task's contiguous chunk (whole sectors, coalesced).  The
@@ -440,8 +443,8 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
 #pragma unroll
         for (int k = 0; k < NLIN; ++k) {
             const size_t o = lin_index(x, clamp_slot(lane + 64 * k));
-            if constexpr (kNeedRes) rl[slot][k] = a.res[o];
-            if constexpr (kNeedX) xl[slot][k] = a.x[o];
+            if constexpr (kNeedRes) rl[slot][k] = __builtin_nontemporal_load(a.res + o);
+            if constexpr (kNeedX) xl[slot][k] = __builtin_nontemporal_load(a.x + o);
         }
     };
     fetch(0, 0);
@@ -470,19 +473,19 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
             const T rhs = Sg[t];
             const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
             if constexpr (MODE == kModeMult) {
-                a.out[o] = rhs;
+                __builtin_nontemporal_store(rhs, a.out + o);
             } else if constexpr (MODE == kModeLserk) {
                 const T rr = a.ca * rl[cur][k] + a.dt * rhs;
-                a.res[o] = rr;
-                a.out[o] = qv + a.cb * rr;
+                __builtin_nontemporal_store(rr, a.res + o);
+                __builtin_nontemporal_store(qv + a.cb * rr, a.out + o);
             } else if constexpr (MODE == kModeRk1) {
-                a.out[o] = qv + a.ca * rhs;
-                a.res[o] = qv + a.cb * rhs;
+                __builtin_nontemporal_store(qv + a.ca * rhs, a.out + o);
+                __builtin_nontemporal_store(qv + a.cb * rhs, a.res + o);
             } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                a.out[o] = xl[cur][k] + a.ca * rhs;
-                a.res[o] = rl[cur][k] + a.cb * rhs;
+                __builtin_nontemporal_store(xl[cur][k] + a.ca * rhs, a.out + o);
+                __builtin_nontemporal_store(rl[cur][k] + a.cb * rhs, a.res + o);
             } else {
-                a.out[o] = rl[cur][k] + a.ca * rhs;
+                __builtin_nontemporal_store(rl[cur][k] + a.ca * rhs, a.out + o);
             }
         }
         wave_sync();  // staging buffer free for the next x
