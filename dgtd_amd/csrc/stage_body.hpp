@@ -163,7 +163,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
                 if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
-                    rv[c] = a.res[c * N + idx];
+                    rv[c] = __builtin_nontemporal_load(a.res + c * N + idx);
                 if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
             }
 #pragma unroll
@@ -171,19 +171,19 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
                 const size_t o = c * N + idx;
                 const T qv = sQ[c * EB * NP + el * NP + i];
                 if constexpr (MODE == kModeMult) {
-                    a.out[o] = rhs[c];
+                    __builtin_nontemporal_store(rhs[c], a.out + o);
                 } else if constexpr (MODE == kModeLserk) {
                     const T r = a.ca * rv[c] + a.dt * rhs[c];
-                    a.res[o] = r;
-                    a.out[o] = qv + a.cb * r;
+                    __builtin_nontemporal_store(r, a.res + o);
+                    __builtin_nontemporal_store(qv + a.cb * r, a.out + o);
                 } else if constexpr (MODE == kModeRk1) {
-                    a.out[o] = qv + a.ca * rhs[c];
-                    a.res[o] = qv + a.cb * rhs[c];
+                    __builtin_nontemporal_store(qv + a.ca * rhs[c], a.out + o);
+                    __builtin_nontemporal_store(qv + a.cb * rhs[c], a.res + o);
                 } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                    a.out[o] = xv[c] + a.ca * rhs[c];
-                    a.res[o] = rv[c] + a.cb * rhs[c];
+                    __builtin_nontemporal_store(xv[c] + a.ca * rhs[c], a.out + o);
+                    __builtin_nontemporal_store(rv[c] + a.cb * rhs[c], a.res + o);
                 } else {  // kModeRk4
-                    a.out[o] = rv[c] + a.ca * rhs[c];
+                    __builtin_nontemporal_store(rv[c] + a.ca * rhs[c], a.out + o);
                 }
             }
         }
