@@ -16,7 +16,9 @@
 // LIFT fscale/2 C(n) d over the face nodes, with d the jump (exterior -
 // interior, or the boundary / interior-condition weights of the own trace)
 // and C(n) = [[a(I - n n^T), [n x]], [-[n x], a(I - n n^T)]]; E rows scaled
-// by 1/eps, H rows by 1/mu.
+// by 1/eps, H rows by 1/mu.  One difference from the hesthaven operator: on
+// SMA boundary faces a = 1 whatever alpha, as the Global operator assembles
+// them (with alpha = 1 the two agree).
 #include "global.hpp"
 
 #include "kernels.hpp"
@@ -116,10 +118,15 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
     // surface
     for (int f = 0; f < nf; ++f) {
         const double* n = v.n[f];
+        const bool interior = v.nb[f] >= 0;
+        // The Global operator weights the upwind terms of an SMA boundary face
+        // by 1 whatever alpha (DGOperatorFactory.h:485-495, 521-531, 557-567),
+        // i.e. the face behaves as the hesthaven face with alpha = 1.
+        const double af = !interior && -v.nb[f] - 1 == 3 ? 1.0 : alpha;
         double C[6][6] = {};
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {
-                const double proj = alpha * ((a == b ? 1.0 : 0.0) - n[a] * n[b]);
+                const double proj = af * ((a == b ? 1.0 : 0.0) - n[a] * n[b]);
                 C[a][b] = proj;
                 C[3 + a][3 + b] = proj;
             }
@@ -130,9 +137,8 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
             C[3 + a][z] -= n[y];
             C[3 + a][y] += n[z];
         }
-        const bool interior = v.nb[f] >= 0;
         double ce = 0.0, ch = 0.0;
-        if (!interior) bc_weights(-v.nb[f] - 1, alpha, ce, ch);
+        if (!interior) bc_weights(-v.nb[f] - 1, af, ce, ch);
         for (int b = 0; b < Nfp; ++b) {
             const int m = f * Nfp + b;
             const int M = v.fmask[m];

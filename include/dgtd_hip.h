@@ -112,9 +112,11 @@ typedef struct {
  * (GlobalEvolution::Mult, GlobalEvolution.cpp:628-1088, a CSR SpMV over the
  * matrix DGOperatorFactory::buildGlobalOperator assembles,
  * DGOperatorFactory.h:1428-1600).  The matrix holds the same semi-discrete
- * operator as the matrix-free variants (same boundary and interior-condition
- * weights), plus bulk conductivity; one fused kernel does SpMV and stage
- * update.  Chosen explicitly (never by autotune); tetrahedra and triangles. */
+ * operator as the matrix-free variants, except that SMA boundary faces weight
+ * their upwind terms by 1 whatever alpha, as the Global operator does
+ * (DGOperatorFactory.h:485-495, 521-531, 557-567; identical at alpha = 1),
+ * plus bulk conductivity; one fused kernel does SpMV and stage update.
+ * Chosen explicitly (never by autotune); tetrahedra and triangles. */
 #define DGTD_KERNEL_CSR 3
 
 const char* dgtd_last_error(void);

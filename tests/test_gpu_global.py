@@ -2,7 +2,8 @@
 reference's Global path (GlobalEvolution::Mult, GlobalEvolution.cpp:628-1088)
 as one fused CSR SpMV + stage-update kernel.
 
-Bars: FP64 Mult equals the oracle's Mult (no conductivity) and the host CSR
+Bars: FP64 Mult equals the oracle's Mult (no conductivity; SMA faces as the
+Global operator weights them, see tests/test_global_assembly.py) and the host CSR
 product (with conductivity) to 1e-12 relative; FP64 RK4 / LSERK4 runs equal
 the oracle's runs to 1e-11 relative L2 (TF/SF plane waves included); FP32
 agrees with FP64 to 1e-5; a conductive cavity mode decays as the damped
@@ -14,7 +15,7 @@ import pytest
 import meshgen as MG
 from oracle import oracle as O
 from oracle import oracle2d as O2
-from test_global_assembly import csr_matvec
+from test_global_assembly import csr_matvec, global_expected
 from test_gpu_tfsf import WAVE
 from test_tfsf import MEAN, SPREAD, waveguide
 
@@ -37,11 +38,10 @@ def cube_case(n, seed):
 @pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
 def test_tet_mult_matches_oracle(order):
     vx, E, fbc, eps, mu = cube_case(2, order)
-    ref = O.Operator(vx, E, fbc, order, alpha=0.5, eps=eps, mu=mu)
     gpu = D.GlobalEvolution(vx, E, fbc, order, alpha=0.5, eps=eps, mu=mu, precision=64)
     assert gpu.kernel_variant() == "csr"
-    x = np.random.default_rng(order).standard_normal(ref.ndofs)
-    yr = ref.mult(x)
+    x = np.random.default_rng(order).standard_normal(gpu.ndofs)
+    yr = global_expected(lambda b, a: O.Operator(vx, E, b, order, alpha=a, eps=eps, mu=mu), fbc, x, 0.5)
     assert np.abs(gpu.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
