@@ -109,9 +109,13 @@ class Case:
         return x
 
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
-        """The case's operator; conductive materials need the assembled one
-        ("csr"), which kernel="auto" then selects."""
-        if kernel == "auto" and self.sigma is not None:
+        """The case's operator; conductive materials, and SMA faces with
+        alpha != 1 under the Global or Maxwell operator (which weight SMA faces
+        as at alpha = 1), need the assembled one ("csr"), which kernel="auto"
+        then selects."""
+        global_sma = (self.info["evolution_operator"] != "hesthaven" and self.info["alpha"] != 1.0
+                      and bool((self.face_bc == 3).any()))
+        if kernel == "auto" and (self.sigma is not None or global_sma):
             kernel = "csr"
         op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
                                 eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,

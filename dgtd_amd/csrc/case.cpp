@@ -347,9 +347,9 @@ extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
         cs->info.nplanewaves = (int32_t)cs->waves.size();
         cs->info.nprobes = (int32_t)(cs->probes.size() / 3);
         cs->info.nfield_probes = (int32_t)cs->fprobe_comp.size();
-        if (cs->info.alpha == 0.0)
+        if (cs->info.alpha == 0.0 && cs->info.evolution_operator == DGTD_OPERATOR_HESTHAVEN)
             for (int8_t b : cs->face_bc)
-                if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598
+                if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598 (the Global operator runs it)
                     throw Unsupported("Centered SMA with Hesthaven Evolution Operator not supported yet.");
         *out = cs.release();
     } catch (const Unsupported& e) {

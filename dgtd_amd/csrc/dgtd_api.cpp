@@ -480,7 +480,10 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     op->Nfp = R.Nfp;
     op->N = (size_t)op->Np * mesh->K;
     op->ndofs = 6 * (int64_t)op->N;
-    const dgtd::TriMeshData md = dgtd::build_tri_mesh(tri_input(mesh), R, op->alpha);
+    // (alpha only guards SMA faces against the centred flux, which the
+    // assembled operator handles: its SMA faces are weighted as at alpha = 1)
+    const dgtd::TriMeshData md =
+        dgtd::build_tri_mesh(tri_input(mesh), R, kernel == DGTD_KERNEL_CSR ? 1.0 : op->alpha);
     op->max_fscale = md.max_fscale;
     op->nv = mesh->nv;
     op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
@@ -654,7 +657,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         in.face_bc = mesh->face_bc;
         in.eps_r = mesh->eps_r;
         in.mu_r = mesh->mu_r;
-        dgtd::MeshData md = dgtd::build_mesh(in, op->R, op->alpha);
+        dgtd::MeshData md = dgtd::build_mesh(in, op->R, opts->kernel == DGTD_KERNEL_CSR ? 1.0 : op->alpha);
         op->max_fscale = md.max_fscale;
         op->nv = mesh->nv;
         op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
@@ -1232,10 +1235,10 @@ int dgtd_global_assemble(const dgtd_mesh_desc* mesh, double alpha, int64_t* nnz,
         dgtd::Csr A;
         if (mesh->dim == 2) {
             const dgtd::RefTri R = dgtd::build_reftri(mesh->order);
-            A = dgtd::assemble_global_tri(dgtd::build_tri_mesh(tri_input(mesh), R, alpha), R, mesh->sigma, alpha);
+            A = dgtd::assemble_global_tri(dgtd::build_tri_mesh(tri_input(mesh), R, 1.0), R, mesh->sigma, alpha);
         } else {
             const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
-            A = dgtd::assemble_global_tet(dgtd::build_mesh(mesh_input(mesh), R, alpha), R, mesh->sigma, alpha);
+            A = dgtd::assemble_global_tet(dgtd::build_mesh(mesh_input(mesh), R, 1.0), R, mesh->sigma, alpha);
         }
         if (fill && (int64_t)A.val.size() != *nnz) throw std::runtime_error("*nnz does not match the matrix");
         *nnz = (int64_t)A.val.size();

@@ -222,6 +222,7 @@ def test_bulk_conductivity_is_normalised_by_free_space_impedance(tmp_path):
     (lambda c: c["model"].__setitem__("filename", "missing.msh"), -1, "cannot open"),
     (lambda c: c["model"].__setitem__("filename", "m.vtk"), -1, "name.msh or name.mesh"),
     (lambda c: (c["solver_options"].__setitem__("upwind_alpha", 0.0),
+                c["solver_options"].__setitem__("evolution_operator", "hesthaven"),
                 c["model"]["boundaries"].__setitem__(0, {"tags": [1], "type": "SMA"})), -4, "Centered SMA"),
 ])
 def test_case_refusals(tmp_path, mutate, code, msg):

@@ -45,6 +45,20 @@ def test_tet_mult_matches_oracle(order):
     assert np.abs(gpu.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
+@pytest.mark.parametrize("alpha", [0.0, 0.5])
+def test_centred_and_partial_upwind_sma_follow_the_global_operator(alpha):
+    """SMA faces weighted as at alpha = 1 (centred SMA runs, as in the
+    reference's Global operator; the hesthaven operator refuses it)."""
+    vx, E, fbc, eps, mu = cube_case(2, 11)
+    gpu = D.GlobalEvolution(vx, E, fbc, 3, alpha=alpha, eps=eps, mu=mu, precision=64)
+    x = np.random.default_rng(5).standard_normal(gpu.ndofs)
+    yr = global_expected(lambda b, a: O.Operator(vx, E, b, 3, alpha=a, eps=eps, mu=mu), fbc, x, alpha)
+    assert np.abs(gpu.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
+    if alpha == 0.0:  # the matrix-free hesthaven operator refuses centred SMA
+        with pytest.raises(D.DgtdError, match="SMA"):
+            D.HesthavenEvolution(vx, E, fbc, 3, alpha=0.0, precision=64)
+
+
 @pytest.mark.parametrize("order", [2, 4])
 def test_tri_mult_matches_oracle(order):
     vx, E, bc = MG.square_mesh(5, bc=["pec", "pmc", "sma", "pec"], jitter=0.2, seed=order)
