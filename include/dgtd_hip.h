@@ -69,7 +69,8 @@ typedef struct dgtd_op dgtd_op;
 
 /* Mesh description; host arrays borrowed for the duration of dgtd_op_create.
  * Replaces the (ParFiniteElementSpace, Model) pair of the reference
- * constructor (HesthavenEvolution.cpp:315): affine tetrahedra only. */
+ * constructor (HesthavenEvolution.cpp:315): affine tetrahedra, or affine
+ * triangles with dim = 2. */
 typedef struct {
     int32_t K;              /* number of tetrahedra */
     int32_t order;          /* polynomial order p, 1..5 */
@@ -141,7 +142,8 @@ int dgtd_op_node_positions(const dgtd_op* op, double* xyz);
 
 /* out = L(in) -- HesthavenEvolution::Mult (HesthavenEvolution.cpp:450-542).
  * Host pointers, 6*Np*K doubles each; `t` mirrors TimeDependentOperator::SetTime
- * (used by the reference only for TF/SF sources, which are not supported). */
+ * (used by the reference only for TF/SF sources: the plane waves of
+ * dgtd_op_set_planewaves are evaluated at t). */
 int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out);
 
 /* Same on device pointers in the operator's precision, on `stream` (a
