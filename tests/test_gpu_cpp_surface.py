@@ -1,6 +1,8 @@
 """The C++ operator surface (include/dgtd_hip.hpp) on the GPU: a program
 linked against libdgtd_hip.so and the oracle checks Mult and an RK4
-Solver::run against the oracle (tests/cpp/test_cpp_surface.cpp)."""
+Solver::run against the oracle, the assembled variant (with and without
+conductivity), and the three variants on a triangle mesh against each other
+(tests/cpp/test_cpp_surface.cpp)."""
 import os
 import subprocess
 
