@@ -378,6 +378,8 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 {
     if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "batched instances are tetrahedral meshes");
+    if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
+        return dgtd::api_fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     if (mesh->sigma)
         return dgtd::api_fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxBatchOrder)

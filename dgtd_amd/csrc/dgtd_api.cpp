@@ -631,6 +631,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
         return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
+    if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
+        return fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     dgtd_op* op = nullptr;
     try {
         op = new dgtd_op;

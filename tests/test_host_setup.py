@@ -142,3 +142,16 @@ def test_initial_field_projection_matches_reference_sources():
     g = np.exp(-(pos[0] - c[0]) ** 2 / (2 * 0.04))
     np.testing.assert_allclose(x.reshape(6, -1)[4], g.ravel(), atol=1e-14)
     assert np.abs(np.delete(x.reshape(6, -1), 4, axis=0)).max() == 0
+
+
+def test_empty_mesh_is_refused_before_any_device_call():
+    import ctypes
+    from dgtd_amd import _abi
+
+    desc = _abi.MeshDesc()
+    desc.K, desc.order, desc.nv, desc.dim = 0, 2, 0, 3
+    opts = _abi.Opts(1.0, 64, _abi.DGTD_LSERK4, 0, _abi.DGTD_KERNEL_AUTO)
+    h = ctypes.c_void_p()
+    rc = _abi.lib().dgtd_op_create(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert b"at least one element" in _abi.lib().dgtd_last_error()
