@@ -91,14 +91,13 @@ struct dgtd_batch {
 
     size_t rs() const { return precision == 64 ? 8 : 4; }
 
-    // Orders 2-5 run on the matrix-core stage; order 1 (4 nodes, 16-column
-    // tiles mostly padding) and FP64 order 5 run on the per-node stage (the
-    // FP64 order-5 matrix-core stage reads its matrices from global memory
-    // and spills registers: 699 us against 730 us per stage on 48,000 tets,
-    // not worth a separate launch in a batch).
+    // Orders 2-5 run on the matrix-core stage (FP64 order 5 too, with its
+    // matrices in global memory: the 64-cavity batch's FP64 rate rose from
+    // 2.03e10 to 2.68e10 DOF*stages/s against the per-node stage); order 1
+    // (4 nodes, 16-column tiles mostly padding) runs on the per-node stage.
     bool use_mfma(int order) const
     {
-        if (order < 2 || (precision == 64 && order == 5)) return false;
+        if (order < 2) return false;
         return precision == 64 ? dgtd::mfma_available<double>(order) : dgtd::mfma_available<float>(order);
     }
 
