@@ -119,10 +119,13 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
     for (int f = 0; f < nf; ++f) {
         const double* n = v.n[f];
         const bool interior = v.nb[f] >= 0;
-        // The Global operator weights the upwind terms of an SMA boundary face
-        // by 1 whatever alpha (DGOperatorFactory.h:485-495, 521-531, 557-567),
-        // i.e. the face behaves as the hesthaven face with alpha = 1.
-        const double af = !interior && -v.nb[f] - 1 == 3 ? 1.0 : alpha;
+        // The Global operator weights the upwind terms of SMA faces by 1
+        // whatever alpha: boundary SMA (DGOperatorFactory.h:485-495, 521-531,
+        // 557-567) and interior SMA (coeff 1.0 against c*alpha for PEC/PMC,
+        // :587-591, 655-659), i.e. the face behaves as the hesthaven face with
+        // alpha = 1.
+        const int bcode = interior ? 0 : -v.nb[f] - 1;
+        const double af = bcode == 3 || bcode == 6 ? 1.0 : alpha;
         double C[6][6] = {};
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) {

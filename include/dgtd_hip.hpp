@@ -207,9 +207,15 @@ public:
     {
         height = width = (int)impl_.Height();
     }
+    // HostRead/HostWrite: under an MFEM device backend (the reference
+    // launcher's -d option) the valid copy of `in` may be on the device, so
+    // it is brought to the host first, and `out` is marked valid on the host
+    // (GetData() would read a stale host copy and leave the result unseen).
     void Mult(const mfem::Vector& in, mfem::Vector& out) const override
     {
-        dgtd::check(dgtd_op_mult(impl_.handle(), GetTime(), in.GetData(), out.GetData()));
+        const double* x = in.HostRead();
+        double* y = out.HostWrite();
+        dgtd::check(dgtd_op_mult(impl_.handle(), GetTime(), x, y));
     }
     double MaxFscale() const { return impl_.MaxFscale(); }
 

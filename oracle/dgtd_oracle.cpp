@@ -388,6 +388,7 @@ const double rk4c[5] = {0.0, 1432997174477.0 / 9575080441755.0,
 struct oracle_op {
     int K = 0, p = 0, Np = 0, Nfp = 0;
     double alpha = 1.0;
+    bool global_sma = false; /* SMA faces at upwind weight 1 (Global operator) */
     RefElem R;
     std::vector<double> metric;   /* 9K: rx ry rz sx sy sz tx ty tz */
     std::vector<double> vol;      /* K */
@@ -469,8 +470,10 @@ void oracle_ref_element(int order, double* rst, double* M, double* Dr, double* D
     (void)Nfp;
 }
 
-oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* EToV,
-                         const int* face_bc, const double* eps, const double* mu, double alpha)
+namespace {
+oracle_op* create_op(int K, int order, const double* vx, int nv, const int* EToV,
+                     const int* face_bc, const double* eps, const double* mu, double alpha,
+                     bool global_sma)
 {
     try {
         if (K <= 0 || order < 1 || order > 8 || !vx || !EToV || !face_bc)
@@ -479,6 +482,7 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
         op->K = K;
         op->p = order;
         op->alpha = alpha;
+        op->global_sma = global_sma;
         op->R = build_ref(order);
         const int Np = op->R.Np, Nfp = op->R.Nfp;
         op->Np = Np;
@@ -608,7 +612,7 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
                     op->tfsf[4 * (size_t)e + f] = e < e2 ? -1 : 1;
                     code = 0;
                 }
-                if (code == 3 && alpha == 0.0)
+                if (code == 3 && alpha == 0.0 && !global_sma)
                     throw std::runtime_error("SMA boundaries need upwind flux (alpha != 0)");
                 op->bc[4 * (size_t)e + f] = code;
                 for (int b = 0; b < Nfp; ++b) {
@@ -636,6 +640,19 @@ oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* 
         g_err = ex.what();
         return nullptr;
     }
+}
+}  // namespace
+
+oracle_op* oracle_create(int K, int order, const double* vx, int nv, const int* EToV,
+                         const int* face_bc, const double* eps, const double* mu, double alpha)
+{
+    return create_op(K, order, vx, nv, EToV, face_bc, eps, mu, alpha, false);
+}
+
+oracle_op* oracle_create_global(int K, int order, const double* vx, int nv, const int* EToV,
+                                const int* face_bc, const double* eps, const double* mu, double alpha)
+{
+    return create_op(K, order, vx, nv, EToV, face_bc, eps, mu, alpha, true);
 }
 
 void oracle_destroy(oracle_op* op) { delete op; }
@@ -722,6 +739,7 @@ void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t)
             /* jumps dQ = Q[P] - Q[M] (:474-479), boundary overrides (:275-313) */
             for (int f = 0; f < 4; ++f) {
                 const int code = op->bc[4 * (size_t)e + f];
+                const double af = op->global_sma && (code == 3 || code == 6) ? 1.0 : alpha;
                 for (int b = 0; b < Nfp; ++b) {
                     const size_t n = (size_t)e * Nf + (size_t)f * Nfp + b;
                     const size_t M = op->vmapM[n], P = op->vmapP[n];
@@ -733,7 +751,7 @@ void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t)
                         for (int c = 0; c < 3; ++c) dF[c][f * Nfp + b] = 0.0 * in[c * N + M];
                         for (int c = 3; c < 6; ++c) dF[c][f * Nfp + b] = -2.0 * in[c * N + M];
                     } else if (code == 3) { /* SMA: dE = -E/alpha, dH = -H/alpha */
-                        for (int c = 0; c < 6; ++c) dF[c][f * Nfp + b] = (-1.0 / alpha) * in[c * N + M];
+                        for (int c = 0; c < 6; ++c) dF[c][f * Nfp + b] = (-1.0 / af) * in[c * N + M];
                     } else if (code >= 4) { /* interior PEC (-1, 0), PMC (0, -1), SMA (-1/2, -1/2) (:309-311) */
                         const double ce = code == 4 ? -1.0 : (code == 6 ? -0.5 : 0.0);
                         const double ch = code == 5 ? -1.0 : (code == 6 ? -0.5 : 0.0);
@@ -762,6 +780,8 @@ void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t)
             for (int f = 0; f < 4; ++f) {
                 const double* nv = &op->nrm[12 * (size_t)e + 3 * f];
                 const double fs = op->fscale[4 * (size_t)e + f] / 2.0;
+                const int code = op->bc[4 * (size_t)e + f];
+                const double af = op->global_sma && (code == 3 || code == 6) ? 1.0 : alpha;
                 for (int b = 0; b < Nfp; ++b) {
                     const int m = f * Nfp + b;
                     double ndotdH = 0.0, ndotdE = 0.0;
@@ -772,9 +792,9 @@ void oracle_mult_t(const oracle_op* op, const double* in, double* out, double t)
                     for (int x = 0; x < 3; ++x) {
                         const int y = (x + 1) % 3, z = (x + 2) % 3;
                         const double fH = -1.0 * nv[y] * dF[z][m] + nv[z] * dF[y][m] +
-                                          alpha * (dF[3 + x][m] - ndotdH * nv[x]);
+                                          af * (dF[3 + x][m] - ndotdH * nv[x]);
                         const double fE = nv[y] * dF[3 + z][m] - 1.0 * nv[z] * dF[3 + y][m] +
-                                          alpha * (dF[x][m] - ndotdE * nv[x]);
+                                          af * (dF[x][m] - ndotdE * nv[x]);
                         flux[3 + x][m] = fH * fs;
                         flux[x][m] = fE * fs;
                     }

@@ -71,6 +71,18 @@ const char* oracle_last_error(void);
 /* Plane waves injected on the TF/SF faces (replaces the previous set). */
 int oracle_set_planewaves(oracle_op* op, const oracle_planewave* w, int n);
 
+/* oracle_create with global_sma != 0: SMA faces, boundary (code 3) and
+ * interior (code 6), take upwind weight 1 whatever alpha, as the reference's
+ * Global operator assembles them (SMA coefficient 1 in bdrCentCoeff /
+ * bdrUpwindCoeff, DGOperatorFactory.h:21-35; boundary sub-operators
+ * :485-567; interior-boundary sub-operators :575-665, where SMA gets coeff
+ * 1.0 and PEC/PMC c*alpha): such a face behaves as a hesthaven face with
+ * alpha = 1 (boundary SMA jump -1, flux weight 1), so alpha = 0 with SMA
+ * faces is allowed. */
+oracle_op* oracle_create_global(int K, int order, const double* vx, int nv,
+                                const int* EToV, const int* face_bc,
+                                const double* eps, const double* mu, double alpha);
+
 int64_t oracle_ndofs(const oracle_op* op); /* 6*Np*K */
 double oracle_max_fscale(const oracle_op* op);
 

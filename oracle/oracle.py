@@ -33,6 +33,8 @@ def lib():
         L.oracle_create.restype = ctypes.c_void_p
         L.oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
                                     _dp, _dp, ctypes.c_double]
+        L.oracle_create_global.restype = ctypes.c_void_p
+        L.oracle_create_global.argtypes = L.oracle_create.argtypes
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_ndofs.restype = ctypes.c_int64
@@ -106,7 +108,10 @@ class PlaneWave(ctypes.Structure):
 class Operator:
     """The oracle's HesthavenEvolution over one tetrahedral mesh."""
 
-    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, global_sma=False):
+        """global_sma: SMA faces (codes 3 and 6) at upwind weight 1 whatever
+        alpha, as the reference's Global operator assembles them
+        (DGOperatorFactory.h:21-35, 485-567, 575-665)."""
         self.vx = np.ascontiguousarray(vx, dtype=np.float64).reshape(-1, 3)
         self.EToV = np.ascontiguousarray(EToV, dtype=np.int32).reshape(-1, 4)
         self.face_bc = np.ascontiguousarray(face_bc, dtype=np.int32).reshape(-1, 4)
@@ -115,7 +120,8 @@ class Operator:
         self.Np, self.Nfp = np_of(order), nfp_of(order)
         self.eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
         self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
-        h = lib().oracle_create(self.K, order, _d(self.vx), self.vx.shape[0], _i(self.EToV),
+        create = lib().oracle_create_global if global_sma else lib().oracle_create
+        h = create(self.K, order, _d(self.vx), self.vx.shape[0], _i(self.EToV),
                                 _i(self.face_bc), _d(self.eps), _d(self.mu), float(alpha))
         if not h:
             raise RuntimeError("oracle_create: " + lib().oracle_last_error().decode())

@@ -169,7 +169,11 @@ class Operator2D:
     vx: (nv, 2) or (nv, 3) vertex coordinates (z ignored); EToV: (K, 3);
     face_bc: (K, 3) codes per edge f = (v_f, v_{f+1 mod 3}); eps, mu: (K,)."""
 
-    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, global_sma=False):
+        # global_sma: SMA edges (codes 3 and 6) at upwind weight 1 whatever
+        # alpha, as the reference's Global operator assembles them
+        # (DGOperatorFactory.h:21-35, 485-567, 575-665)
+        self.global_sma = bool(global_sma)
         self.R = R = RefTri(order)
         vx = np.asarray(vx, dtype=float)[:, :2]
         EToV = np.asarray(EToV, dtype=np.int64)
@@ -226,7 +230,7 @@ class Operator2D:
                     raise ValueError("PEC/PMC/SMA codes 1-3 on an interior edge (use 4-6)")
                 if nbr[e, f] < 0 and bc[e, f] >= 4:
                     raise ValueError("interior-boundary codes on a boundary edge")
-                if bc[e, f] == 3 and self.alpha == 0.0:
+                if bc[e, f] == 3 and self.alpha == 0.0 and not self.global_sma:
                     raise ValueError("SMA boundaries need upwind flux (alpha != 0)")
         Nfp = R.Nfp
         h = np.min(np.abs(J)) ** 0.5
@@ -303,10 +307,11 @@ class Operator2D:
         dF = q[:, P] - q[:, M]  # (6, K, 3Nfp): jump = exterior - interior
         own = q[:, M]
         code = np.repeat(self.bc, Nfp, axis=1)  # (K, 3Nfp)
-        ce = np.select([code == 1, code == 3, code == 4, code == 6],
-                       [-2.0, -1.0 / self.alpha if self.alpha else 0.0, -1.0, -0.5], 0.0)
-        ch = np.select([code == 2, code == 3, code == 5, code == 6],
-                       [-2.0, -1.0 / self.alpha if self.alpha else 0.0, -1.0, -0.5], 0.0)
+        # per edge-node upwind weight: alpha, or 1 on SMA edges (global_sma)
+        a = np.where((code == 3) | (code == 6), 1.0, self.alpha) if self.global_sma else np.full(code.shape, self.alpha)
+        sma = np.divide(-1.0, a, out=np.zeros_like(a), where=a != 0.0)
+        ce = np.select([code == 1, code == 3, code == 4, code == 6], [-2.0, sma, -1.0, -0.5], 0.0)
+        ch = np.select([code == 2, code == 3, code == 5, code == 6], [-2.0, sma, -1.0, -0.5], 0.0)
         ovr = code > 0
         for c in range(3):
             dF[c] = np.where(ovr, ce * own[c], dF[c])
@@ -323,7 +328,6 @@ class Operator2D:
         ndE = nv[0] * dF[0] + nv[1] * dF[1] + nv[2] * dF[2]
         ndH = nv[0] * dF[3] + nv[1] * dF[4] + nv[2] * dF[5]
         fs = np.repeat(self.fscale, Nfp, axis=1) / 2.0
-        a = self.alpha
         out = np.zeros((6, K, Np))
         u = q.reshape(6, K, Np)
         ur = np.einsum("ij,ckj->cki", R.Dr, u)

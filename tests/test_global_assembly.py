@@ -43,7 +43,9 @@ def bform(energy, x, y):
 
 def global_expected(make, fbc, x, alpha):
     """Global-operator product from the hesthaven oracle: make(fbc, alpha)
-    builds an oracle operator; SMA faces (code 3) are taken at alpha = 1."""
+    builds an oracle operator; SMA boundary faces (code 3) are taken at
+    alpha = 1 by linearity (the face's contribution isolated as the
+    difference of the operator with and without it, both at alpha = 1)."""
     fbc = np.asarray(fbc)
     zero = fbc.copy()
     zero[fbc == 3] = 0
@@ -51,6 +53,27 @@ def global_expected(make, fbc, x, alpha):
     if (fbc == 3).any():
         y = y + make(fbc, 1.0).mult(x) - make(zero, 1.0).mult(x)
     return y
+
+
+def test_oracle_global_sma_switch_equals_linearity():
+    """The oracle's Global-operator SMA weighting (global_sma) against the
+    linearity construction on boundary SMA faces, and against the plain
+    oracle when no SMA face is present or alpha = 1."""
+    vx, E, fbc, eps, mu = cube_case(2, 7)
+    x = np.random.default_rng(7).standard_normal(6 * O.np_of(3) * E.shape[0])
+    for alpha in (0.5, 0.0):
+        mk = lambda b, a, g=False: O.Operator(vx, E, b, 3, alpha=a, eps=eps, mu=mu, global_sma=g)
+        yl = global_expected(lambda b, a: mk(b, a), fbc, x, alpha)
+        yg = mk(fbc, alpha, True).mult(x)
+        assert np.abs(yg - yl).max() <= 1e-12 * np.abs(yl).max()
+    nosma = np.where(np.asarray(fbc) == 3, 1, fbc)
+    for g in (False, True):
+        y = O.Operator(vx, E, nosma, 3, alpha=0.5, global_sma=g).mult(x)
+        if not g:
+            y0 = y
+    assert np.array_equal(y, y0)
+    y1 = O.Operator(vx, E, fbc, 3, alpha=1.0, eps=eps, mu=mu).mult(x)
+    assert np.array_equal(O.Operator(vx, E, fbc, 3, alpha=1.0, eps=eps, mu=mu, global_sma=True).mult(x), y1)
 
 
 def cube_case(n, seed, bc=("pec", "pmc", "sma", "pec", "pmc", "sma")):
@@ -88,9 +111,63 @@ def test_tet_interior_and_tfsf_faces_equal_oracle_mult(cond):
     fbc = fbc.astype(np.int32)
     fbc[MG.plane_faces(vx, E, 0, 0.5)] = cond
     A = D.global_assemble(vx, E, fbc, 3, alpha=0.7)
-    ref = O.Operator(vx, E, fbc, 3, alpha=0.7)
+    ref = O.Operator(vx, E, fbc, 3, alpha=0.7, global_sma=True)
     x = np.random.default_rng(cond).standard_normal(ref.ndofs)
     yr = ref.mult(x)  # TF/SF faces without plane waves: plain interior faces
+    assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
+    if cond == 6:  # interior SMA at upwind weight 1: differs from the hesthaven weighting
+        yh = O.Operator(vx, E, fbc, 3, alpha=0.7).mult(x)
+        assert np.abs(csr_matvec(A, x) - yh).max() > 1e-6 * np.abs(yr).max()
+
+
+def _tri_line_edges(vx, E, x0):
+    on = np.zeros(E.shape, dtype=bool)
+    for f, (p, q) in enumerate(MG.TRI_EDGES):
+        on[:, f] = (np.abs(vx[E[:, p], 0] - x0) < 1e-12) & (np.abs(vx[E[:, q], 0] - x0) < 1e-12)
+    return on
+
+
+def _tri_cut_at_x(vx, E, x0):
+    on = np.where(np.abs(vx[:, 0] - x0) < 1e-12)[0]
+    dup = {int(v): vx.shape[0] + i for i, v in enumerate(on)}
+    vx2 = np.vstack([vx, vx[on]])
+    E2 = E.copy()
+    for e in range(E.shape[0]):
+        if vx[E[e], 0].mean() > x0:
+            E2[e] = [dup.get(int(v), int(v)) for v in E[e]]
+    return vx2, E2
+
+
+@pytest.mark.parametrize("alpha", [0.5, 0.0])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_interior_sma_weighted_as_global_operator(dim, alpha):
+    """Interior SMA faces (code 6) take upwind weight 1 in the Global
+    operator whatever alpha (DGOperatorFactory.h:587-591, 655-659), centred
+    flux included; SMA boundary faces as well.  The oracle's weighting is
+    pinned first: the flux is linear in the jump and in the face's upwind
+    weight, so an interior SMA face at weight 1 equals the mean of the mesh
+    cut along that surface with SMA boundaries (jump -1, weight 1) and of the
+    cut mesh with zero jump there."""
+    if dim == 3:
+        vx, E, fbc = MG.cube_mesh(2, 1.0, ["pec", "sma", "pmc", "sma", "pec", "pec"])
+        fbc = np.asarray(fbc, dtype=np.int32)
+        on = MG.plane_faces(vx, E, 0, 0.5)
+        vx2, E2 = MG.cut_along_plane(vx, E, 0, 0.5)
+        make = lambda v, e, b: O.Operator(v, e, b, 3, alpha=alpha, global_sma=True)
+    else:
+        vx, E, fbc = MG.square_mesh(4, bc=["pec", "sma", "pmc", "pec"], jitter=0.0, seed=1)
+        fbc = np.asarray(fbc, dtype=np.int32)
+        on = _tri_line_edges(vx, E, 0.5)
+        vx2, E2 = _tri_cut_at_x(vx, E, 0.5)
+        make = lambda v, e, b: O2.Operator2D(v, e, b, 3, alpha=alpha, global_sma=True)
+    assert on.any()
+    fbc6 = np.where(on, 6, fbc)
+    ref = make(vx, E, fbc6)
+    A = D.global_assemble(vx, E, fbc6, 3, alpha=alpha)
+    x = np.random.default_rng(11).standard_normal(A[0].size - 1)
+    yr = ref.mult(x)
+    ycut = 0.5 * (make(vx2, E2, np.where(on, 3, fbc)).mult(x) + make(vx2, E2, np.where(on, 0, fbc)).mult(x))
+    assert np.abs(yr - ycut).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
