@@ -68,7 +68,7 @@ class CaseInfo(ctypes.Structure):
 
 DGTD_E_CAPACITY = -6
 DGTD_OPERATOR_MAXWELL, DGTD_OPERATOR_GLOBAL, DGTD_OPERATOR_HESTHAVEN = 0, 1, 2
-DGTD_INST_FREE, DGTD_INST_RUNNING, DGTD_INST_DONE = 0, 1, 2
+DGTD_INST_FREE, DGTD_INST_RUNNING, DGTD_INST_DONE, DGTD_INST_FAILED, DGTD_INST_CANCELLED = 0, 1, 2, 3, 4
 
 # name -> (restype, argtypes); every symbol declared in include/dgtd_hip.h.
 SIGNATURES = {
@@ -90,6 +90,7 @@ SIGNATURES = {
     "dgtd_op_time": (c_int32, [c_void_p, _dp]),
     "dgtd_op_step": (c_int32, [c_void_p, _dp, _dp, c_double, c_int32]),
     "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_unstable": (c_int32, [c_void_p, _i32p]),
     "dgtd_op_set_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave), c_int32]),
     "dgtd_op_probe_set": (c_int32, [c_void_p, c_int32, _dp, _i32p]),
     "dgtd_op_probe_sample": (c_int32, [c_void_p, _dp]),
@@ -121,6 +122,7 @@ SIGNATURES = {
     "dgtd_batch_info": (c_int32, [c_void_p, c_int32, _dp, _i32p, _i32p, _dp]),
     "dgtd_batch_get_state": (c_int32, [c_void_p, c_int32, _dp]),
     "dgtd_batch_remove": (c_int32, [c_void_p, c_int32]),
+    "dgtd_batch_cancel": (c_int32, [c_void_p, c_int32]),
     "dgtd_batch_stats": (c_int32, [c_void_p, _i64p, _i32p, _i32p, _i32p]),
     "dgtd_ref_element": (c_int32, [c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _i32p]),
     "dgtd_mesh_vmaps": (c_int32, [ctypes.POINTER(MeshDesc), c_double, _i32p, _i32p, _i8p]),

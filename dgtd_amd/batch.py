@@ -10,7 +10,8 @@ from . import _abi
 from ._abi import check, dptr, lib
 from .hesthaven import _Mesh, num_nodes
 
-STATES = {_abi.DGTD_INST_FREE: "free", _abi.DGTD_INST_RUNNING: "running", _abi.DGTD_INST_DONE: "done"}
+STATES = {_abi.DGTD_INST_FREE: "free", _abi.DGTD_INST_RUNNING: "running", _abi.DGTD_INST_DONE: "done",
+          _abi.DGTD_INST_FAILED: "failed", _abi.DGTD_INST_CANCELLED: "cancelled"}
 
 
 def footprint(vx, EToV, face_bc, order, precision=32):
@@ -60,6 +61,8 @@ class Batch:
         check(lib().dgtd_batch_step(self._h, int(nsteps)))
 
     def poll(self, cap=4096):
+        """(ids that ended since the last poll -- done, failed or cancelled --,
+        number still running)."""
         ids = np.zeros(cap, dtype=np.int32)
         nd, nr = ctypes.c_int32(), ctypes.c_int32()
         check(lib().dgtd_batch_poll(self._h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
@@ -77,6 +80,10 @@ class Batch:
         x = np.empty(self._sizes[i])
         check(lib().dgtd_batch_get_state(self._h, int(i), dptr(x)))
         return x
+
+    def cancel(self, i):
+        """Stop a running instance; poll reports it once as ended ("cancelled")."""
+        check(lib().dgtd_batch_cancel(self._h, int(i)))
 
     def remove(self, i):
         check(lib().dgtd_batch_remove(self._h, int(i)))

@@ -477,7 +477,9 @@ int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndon
         for (int i = 0; i < b->max_slots; ++i) {
             auto& s = b->slots[i];
             if (s.state == dgtd::kSlotRunning) ++nr;
-            if (s.state == dgtd::kSlotDone && !s.reported && nd < cap) {
+            const bool ended = s.state == dgtd::kSlotDone || s.state == dgtd::kSlotFailed ||
+                               s.state == dgtd::kSlotCancelled;
+            if (ended && !s.reported && nd < cap) {
                 if (done_ids) done_ids[nd] = i;
                 s.reported = true;
                 ++nd;
@@ -528,6 +530,27 @@ int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x)
             std::vector<float> tmp(ndofs);
             hcheck(hipMemcpy(tmp.data(), s.buf[s.cur], sizeof(float) * ndofs, hipMemcpyDeviceToHost), "D2H state");
             std::copy(tmp.begin(), tmp.end(), x);
+        }
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
+    return DGTD_OK;
+}
+
+int dgtd_batch_cancel(dgtd_batch* b, int32_t id)
+{
+    if (!b || id < 0 || id >= b->max_slots) return dgtd::api_fail(DGTD_E_ARG, "bad instance id");
+    std::lock_guard<std::mutex> lk(b->mu);
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        if (!b->dirty) b->sync_pull();  // device values are authoritative: fetch before editing
+        else hcheck(hipStreamSynchronize(b->stream), "sync");
+        auto& s = b->slots[id];
+        if (s.state == dgtd::kSlotFree || s.state == dgtd_batch::kSlotReserved)
+            return dgtd::api_fail(DGTD_E_STATE, "instance slot is free or still being set up");
+        if (s.state == dgtd::kSlotRunning) {
+            s.state = dgtd::kSlotCancelled;
+            b->dirty = true;  // leaves the work list at the next step
         }
     } catch (const std::exception& e) {
         return batch_error(e);

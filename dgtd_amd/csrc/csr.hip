@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "stability.hpp"
 
 namespace dgtd {
 
@@ -81,12 +82,18 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
     for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kLanesPerRow);
     if (sub != 0 || !valid) return;
     const T rhs = acc;
+    // stability test on the step's last stage (rare atomic; unstable_entry)
+    auto check = [&](T qn) {
+        if (a.unstable != nullptr && unstable_entry(qn)) atomicOr(a.unstable, 1);
+    };
     if constexpr (MODE == kModeMult) {
         a.out[r] = rhs;
     } else if constexpr (MODE == kModeLserk) {
         const T rr = a.ca * a.res[r] + a.dt * rhs;
         a.res[r] = rr;
-        a.out[r] = a.qin[r] + a.cb * rr;
+        const T qn = a.qin[r] + a.cb * rr;
+        a.out[r] = qn;
+        check(qn);
     } else if constexpr (MODE == kModeRk1) {
         const T q = a.qin[r];
         a.out[r] = q + a.ca * rhs;
@@ -96,7 +103,9 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
         a.out[r] = xv + a.ca * rhs;
         a.res[r] = rv + a.cb * rhs;
     } else {
-        a.out[r] = a.res[r] + a.ca * rhs;
+        const T qn = a.res[r] + a.ca * rhs;
+        a.out[r] = qn;
+        check(qn);
     }
 }
 

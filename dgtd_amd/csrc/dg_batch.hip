@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void dg_batch_stage(BatchArgs<T> b)
         a.alpha = (T)I.alpha;
         a.geo = I.geo;
         a.conn = I.conn;
+        a.unstable = b.stage == 4 ? &b.inst[w.inst].unstable : nullptr;  // the step's last stage
         switch (order) {
             case 1: stage_block<1, T, kModeLserk>(a, smem_raw, w.blk); break;
             case 2: stage_block<2, T, kModeLserk>(a, smem_raw, w.blk); break;
@@ -66,6 +67,9 @@ __global__ __launch_bounds__(256) void dg_batch_stage(BatchArgs<T> b)
 // After the 5 stages: the state is in the other buffer; time advances and an
 // instance retires once t > final_time - 1e-8 dt (Solver::run's loop guard,
 // Solver.cpp:497), its last step having been truncated to final_time - t (:537).
+// An instance whose step stored an unstable entry (the per-step stability test
+// of Solver::run, Solver.cpp:499-516, which the reference only reports) is
+// retired as failed, so its slot and HBM are freed for others.
 template <typename T>
 __global__ void dg_batch_advance(BatchInst<T>* inst, int nslots)
 {
@@ -76,7 +80,9 @@ __global__ void dg_batch_advance(BatchInst<T>* inst, int nslots)
     I.cur ^= 1;
     I.t += I.dt_step;
     I.steps += 1;
-    if (I.t <= I.final_time - 1e-8 * I.dt) {
+    if (I.unstable) {
+        I.state = kSlotFailed;
+    } else if (I.t <= I.final_time - 1e-8 * I.dt) {
         I.dt_step = fmin(I.dt, I.final_time - I.t);
     } else {
         I.state = kSlotDone;

@@ -29,6 +29,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "stability.hpp"
 
 namespace dgtd {
 
@@ -450,6 +451,7 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
         }
     };
     fetch(0, 0);
+    bool bad = false;  // an unstable state entry stored by this lane (a.unstable set)
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         const int cur = x & 1;
@@ -478,8 +480,10 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
                 __builtin_nontemporal_store(rhs, a.out + o);
             } else if constexpr (MODE == kModeLserk) {
                 const T rr = a.ca * rl[cur][k] + a.dt * rhs;
+                const T qn = qv + a.cb * rr;
                 __builtin_nontemporal_store(rr, a.res + o);
-                __builtin_nontemporal_store(qv + a.cb * rr, a.out + o);
+                __builtin_nontemporal_store(qn, a.out + o);
+                bad = bad || unstable_entry(qn);
             } else if constexpr (MODE == kModeRk1) {
                 __builtin_nontemporal_store(qv + a.ca * rhs, a.out + o);
                 __builtin_nontemporal_store(qv + a.cb * rhs, a.res + o);
@@ -487,11 +491,14 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
                 __builtin_nontemporal_store(xl[cur][k] + a.ca * rhs, a.out + o);
                 __builtin_nontemporal_store(rl[cur][k] + a.cb * rhs, a.res + o);
             } else {
-                __builtin_nontemporal_store(rl[cur][k] + a.ca * rhs, a.out + o);
+                const T qn = rl[cur][k] + a.ca * rhs;
+                __builtin_nontemporal_store(qn, a.out + o);
+                bad = bad || unstable_entry(qn);
             }
         }
         wave_sync();  // staging buffer free for the next x
     }
+    if constexpr (MODE == kModeLserk || MODE == kModeRk4) report_unstable(a.unstable, bad);
 }
 
 template <int P, typename T, int MODE, int DIM>
@@ -542,6 +549,7 @@ __global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_batch_stage_mfma(Ba
         a.alpha = (T)I.alpha;
         a.geo = I.geo;
         a.conn = I.conn;
+        a.unstable = b.stage == 4 ? &b.inst[w.inst].unstable : nullptr;  // the step's last stage
         mfma_task<P, T, kModeLserk>(a, L, w.blk);
     }
 }

@@ -12,6 +12,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "stability.hpp"
 
 namespace dgtd {
 
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
         __syncthreads();
 
         // ---- volume: curl with d/dz = 0, LIFT, stage update ----
+        bool bad = false;  // an unstable state entry stored by this thread (a.unstable set)
         for (int t = tid; t < ne * NP; t += blockDim.x) {
             const int el = t / NP;
             const int i = t - el * NP;
@@ -171,8 +173,10 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
                     a.out[o] = rhs[c];
                 } else if constexpr (MODE == kModeLserk) {
                     const T r = a.ca * rv[c] + a.dt * rhs[c];
+                    const T qn = qv + a.cb * r;
                     a.res[o] = r;
-                    a.out[o] = qv + a.cb * r;
+                    a.out[o] = qn;
+                    bad = bad || unstable_entry(qn);
                 } else if constexpr (MODE == kModeRk1) {
                     a.out[o] = qv + a.ca * rhs[c];
                     a.res[o] = qv + a.cb * rhs[c];
@@ -180,10 +184,13 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
                     a.out[o] = xv[c] + a.ca * rhs[c];
                     a.res[o] = rv[c] + a.cb * rhs[c];
                 } else {
-                    a.out[o] = rv[c] + a.ca * rhs[c];
+                    const T qn = rv[c] + a.ca * rhs[c];
+                    a.out[o] = qn;
+                    bad = bad || unstable_entry(qn);
                 }
             }
         }
+        if constexpr (MODE == kModeLserk || MODE == kModeRk4) report_unstable(a.unstable, bad);
     }
 }
 

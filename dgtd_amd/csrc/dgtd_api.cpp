@@ -99,6 +99,7 @@ struct dgtd_op {
     double* d_mu = nullptr;
     double* d_partials = nullptr;
     double* d_out3 = nullptr;
+    int* d_unstable = nullptr;  // per-step stability test (report_unstable), since set_state
     int nparts = 0;
     void* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
     int cur = 0;  // LSERK: buffer holding the state
@@ -140,7 +141,7 @@ struct dgtd_op {
     {
         (void)hipSetDevice(device);
         for (void* p : {d_geo, (void*)d_conn, d_mats, d_mats_mfma, (void*)d_order, (void*)d_tabs, (void*)d_M, (void*)d_jac,
-                        (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_probe_elem,
+                        (void*)d_eps, (void*)d_mu, (void*)d_partials, (void*)d_out3, (void*)d_unstable, (void*)d_probe_elem,
                         (void*)d_tfsf_elem, (void*)d_tfsf_sign, (void*)d_tfsf_xyz, (void*)d_tfsf_geo,
                         (void*)d_lift, (void*)d_waves, (void*)d_rowptr, (void*)d_col, (void*)d_etab, d_val,
                         (void*)d_probe_w, (void*)d_probe_out, d_buf[0],
@@ -270,6 +271,7 @@ struct dgtd_op {
                 a.res = static_cast<T*>(d_buf[2]);
                 a.ca = (T)kA[s];
                 a.cb = (T)kB[s];
+                a.unstable = s == 4 ? d_unstable : nullptr;  // the step's new state
                 launch<T>(a, dgtd::kModeLserk);
                 // res' = res + dt S, q' = q + b res'  =>  q' += b dt S
                 tfsf<T>(t + kC[s] * dt, a.out, kB[s] * dt, a.res, dt, stream);
@@ -294,6 +296,7 @@ struct dgtd_op {
         launch<T>(a, dgtd::kModeRk3);
         tfsf<T>(t + dt / 2, y1, dt, z, dt / 3, stream);
         a.qin = y1; a.out = x; a.ca = (T)(dt / 6);
+        a.unstable = d_unstable;  // the step's new state
         launch<T>(a, dgtd::kModeRk4);
         tfsf<T>(t + dt, x, dt / 6, nullptr, 0.0, stream);
         return 4;
@@ -521,6 +524,8 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
     hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
     hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+    hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
+    hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
     const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
     for (int b = 0; b < nbuf; ++b) {
         hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
@@ -697,6 +702,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
         hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
         hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+        hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
+        hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
         const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
         for (int b = 0; b < nbuf; ++b) {
             hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
@@ -859,6 +866,7 @@ int dgtd_op_set_state(dgtd_op* op, const double* x, double t)
         // LSERK residual starts at zero (rk4a[0] = 0 makes the first stage ignore it)
         if (op->integrator == DGTD_LSERK4)
             hip_check(hipMemsetAsync(op->d_buf[2], 0, op->real_size() * op->ndofs, op->stream), "memset");
+        hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
         hip_check(hipStreamSynchronize(op->stream), "sync");
         op->t = t;
     } catch (const std::exception& e) {
@@ -953,6 +961,21 @@ int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps)
     rc = dgtd_op_advance(op, dt, nsteps);
     if (rc) return rc;
     return dgtd_op_get_state(op, x, t);
+}
+
+int dgtd_op_unstable(dgtd_op* op, int32_t* flag)
+{
+    if (!op || !flag) return fail(DGTD_E_ARG, "NULL argument");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        int h = 0;
+        hip_check(hipMemcpyAsync(&h, op->d_unstable, sizeof(int), hipMemcpyDeviceToHost, op->stream), "D2H");
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        *flag = h ? 1 : 0;
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
 }
 
 int dgtd_op_norms(dgtd_op* op, double* out3)

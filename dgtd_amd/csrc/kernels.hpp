@@ -44,7 +44,13 @@ struct StageArgs {
     const T* val;
     int64_t nrows, nnz;
     int np;  // nodes per element
+    int* unstable;  // non-null: set to 1 when a stored state entry is unstable (unstable_entry)
 };
+
+// Stability test of the reference's time loop (Solver::run, Solver.cpp:499-516,
+// flags a non-finite state norm or one above 1e20), applied to the entries a
+// step's last stage stores (stability.hpp).
+constexpr double kUnstableBound = 1e20;
 
 // Device form of the assembled operator: the rows of DOFs [16 b, 16 b + 16)
 // (all six components) reference at most kCsrTab elements, listed in
@@ -105,12 +111,16 @@ int tri_mfma_waves_per_block(int order);
 // of (instance, element block) items sorted by order.
 constexpr int kMaxBatchOrder = 5;
 
-enum BatchState : int { kSlotFree = 0, kSlotRunning = 1, kSlotDone = 2 };
+// kSlotFailed: retired by the stability test (unstable_entry) of a step's
+// last stage; kSlotCancelled: stopped by the host (dgtd_batch_cancel).
+enum BatchState : int { kSlotFree = 0, kSlotRunning = 1, kSlotDone = 2, kSlotFailed = 3, kSlotCancelled = 4 };
 
 template <typename T>
 struct BatchInst {
     int order, K, nblk, state;  // state: BatchState
-    int cur, steps, pad0, pad1; // cur: which q buffer holds the state
+    int cur, steps;             // cur: which q buffer holds the state
+    int unstable;               // set by the last stage of a step (report_unstable)
+    int pad1;
     T* q[2];
     T* res;
     const T* geo;

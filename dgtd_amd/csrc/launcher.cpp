@@ -236,14 +236,22 @@ int main(int argc, char** argv)
     }
     double norms[3] = {0, 0, 0};
     check(dgtd_op_norms(op, norms), "norms");
+    int32_t unstable = 0;
+    check(dgtd_op_unstable(op, &unstable), "stability");
+    if (unstable)  // the warning of Solver::run (Solver.cpp:499-516); the run went on, as there
+        std::fprintf(stderr,
+                     "WARNING: Simulation is potentially unstable (a step stored a non-finite field value or one "
+                     "above 1e20; final field norm = %g).\n  Time: %g / %g,  dt: %g\n"
+                     "  Verify your setup and consider lowering the time step.\n",
+                     norms[0], t, final_time, dt);
     const double run_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_run).count();
     const int stages = integrator == DGTD_LSERK4 ? 5 : 4;
     std::printf("{\"case\": \"%s\", \"K\": %d, \"order\": %d, \"ndofs\": %lld, \"precision\": %d, "
                 "\"integrator\": \"%s\", \"dt\": %.10e, \"steps\": %ld, \"time\": %.10e, \"norm2\": %.10e, "
-                "\"energy\": %.10e, \"nonfinite\": %d, \"probes\": %d, \"setup_s\": %.3f, \"run_s\": %.3f, "
+                "\"energy\": %.10e, \"nonfinite\": %d, \"unstable\": %d, \"probes\": %d, \"setup_s\": %.3f, \"run_s\": %.3f, "
                 "\"dof_stages_per_s\": %.4e, \"snapshots\": %ld, \"out\": \"%s\"}\n",
                 stem(case_path).c_str(), info.K, info.order, (long long)ndofs, precision,
-                integrator == DGTD_LSERK4 ? "lserk4" : "rk4", dt, steps, t, norms[0], norms[1], (int)norms[2], np,
+                integrator == DGTD_LSERK4 ? "lserk4" : "rk4", dt, steps, t, norms[0], norms[1], (int)norms[2], (int)unstable, np,
                 setup_s, run_s, run_s > 0 ? (double)ndofs * stages * steps / run_s : 0.0, nsnap, out_dir.c_str());
     for (auto& f : files)
         if (f.is_open()) f.close();

@@ -7,6 +7,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "stability.hpp"
 
 namespace dgtd {
 namespace valu {
@@ -124,6 +125,7 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
         __syncthreads();
 
         // ---- volume: curl + LIFT, stage update ----
+        bool bad = false;  // an unstable state entry stored by this thread (a.unstable set)
         for (int t = tid; t < ne * NP; t += blockDim.x) {
             const int el = t / NP;
             const int i = t - el * NP;
@@ -174,8 +176,10 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
                     __builtin_nontemporal_store(rhs[c], a.out + o);
                 } else if constexpr (MODE == kModeLserk) {
                     const T r = a.ca * rv[c] + a.dt * rhs[c];
+                    const T qn = qv + a.cb * r;
                     __builtin_nontemporal_store(r, a.res + o);
-                    __builtin_nontemporal_store(qv + a.cb * r, a.out + o);
+                    __builtin_nontemporal_store(qn, a.out + o);
+                    bad = bad || unstable_entry(qn);
                 } else if constexpr (MODE == kModeRk1) {
                     __builtin_nontemporal_store(qv + a.ca * rhs[c], a.out + o);
                     __builtin_nontemporal_store(qv + a.cb * rhs[c], a.res + o);
@@ -183,10 +187,13 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
                     __builtin_nontemporal_store(xv[c] + a.ca * rhs[c], a.out + o);
                     __builtin_nontemporal_store(rv[c] + a.cb * rhs[c], a.res + o);
                 } else {  // kModeRk4
-                    __builtin_nontemporal_store(rv[c] + a.ca * rhs[c], a.out + o);
+                    const T qn = rv[c] + a.ca * rhs[c];
+                    __builtin_nontemporal_store(qn, a.out + o);
+                    bad = bad || unstable_entry(qn);
                 }
             }
         }
+        if constexpr (MODE == kModeLserk || MODE == kModeRk4) report_unstable(a.unstable, bad);
     }
     __syncthreads();  // LDS tiles free for the next block
 }

@@ -348,6 +348,14 @@ class HesthavenEvolution:
         check(lib().dgtd_op_norms(self._h, dptr(out)))
         return {"norm2": out[0], "energy": out[1], "nonfinite": int(out[2])}
 
+    def unstable(self):
+        """True if a step since the last set_state stored a non-finite state
+        entry or one above 1e20 (Solver::run's per-step stability test,
+        Solver.cpp:499-516, which reports and goes on)."""
+        f = ctypes.c_int32()
+        check(lib().dgtd_op_unstable(self._h, ctypes.byref(f)))
+        return bool(f.value)
+
     def set_graphs(self, on=True):
         """Replay time steps from a captured HIP graph, or (the default)
         launch every stage kernel directly."""

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _abi
-from .batch import Batch, CapacityError
+from .batch import Batch, CapacityError, footprint
 from .hesthaven import cube_mesh, gaussian, num_nodes, project_initial_fields
 
 
@@ -58,6 +58,7 @@ class Record:
     dof: int = 0
     order: int = 0
     K: int = 0
+    status: str = ""        # "done", "failed" (stability test), "cancelled", "rejected" (never fits)
     extra: dict = field(default_factory=dict)
 
     @property
@@ -94,9 +95,13 @@ def build_request(req):
 
 class Server:
     def __init__(self, precision=32, hbm_budget=int(32e9), max_instances=256, poll_every=4, workers=8,
-                 device=0):
-        self.batch = Batch(precision=precision, max_instances=max_instances, hbm_budget=hbm_budget,
-                           device=device)
+                 device=0, batch=None):
+        """batch: an object with Batch's add/step/poll/info/get_state/remove
+        methods to serve on (default: a new Batch on `device`)."""
+        self.batch = batch if batch is not None else Batch(precision=precision, max_instances=max_instances,
+                                                           hbm_budget=hbm_budget, device=device)
+        self.precision = precision
+        self.hbm_budget = hbm_budget
         self.poll_every = poll_every
         self.pool = ThreadPoolExecutor(max_workers=workers)
         self.keep_fields = False
@@ -126,8 +131,15 @@ class Server:
                 nxt += 1
             while setups and setups[0][1].done():
                 r, fut = setups.popleft()
-                records[r.rid].setup_done = time.perf_counter() - t0
-                waiting.append((r, fut.result()))
+                rec = records[r.rid]
+                rec.setup_done = time.perf_counter() - t0
+                built = fut.result()
+                if footprint(built[0], built[1], built[2], r.order, self.precision) > self.hbm_budget:
+                    # could never be admitted: refuse it now rather than block the queue behind it
+                    rec.status = "rejected"
+                    rec.finished = rec.setup_done
+                    continue
+                waiting.append((r, built))
             if running:
                 b.step(self.poll_every)  # asynchronous: admission below overlaps these steps
             admitted = 0
@@ -148,7 +160,9 @@ class Server:
                     rid = inst_of.pop(i)
                     rec = records[rid]
                     rec.finished = tdone
-                    rec.steps = b.info(i)["steps"]
+                    info = b.info(i)
+                    rec.steps = info["steps"]
+                    rec.status = info["state"]  # done, or failed by the stability test
                     if self.keep_fields:
                         rec.extra["x"] = b.get_state(i)
                     b.remove(i)
@@ -164,13 +178,17 @@ class Server:
 
 
 def summarize(records, wall):
-    done = [r for r in records.values() if r.finished > 0]
+    """Latency percentiles over the solves that completed (status "done");
+    failed and rejected requests are counted apart."""
+    done = [r for r in records.values() if r.finished > 0 and r.status == "done"]
     lat = np.array([r.latency for r in done]) if done else np.zeros(1)
     work = sum(r.dof * 5 * r.steps for r in done)
     executed = sum(r.dof * 5 * r.steps for r in records.values())
     return {
         "requests": len(records),
         "completed": len(done),
+        "failed": sum(r.status == "failed" for r in records.values()),
+        "rejected": sum(r.status == "rejected" for r in records.values()),
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()),

@@ -171,6 +171,14 @@ int dgtd_op_time(const dgtd_op* op, double* t);
 /* Host convenience: x (host FP64) advanced by nsteps steps; t updated. */
 int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps);
 
+/* *flag = 1 if a time step since the last dgtd_op_set_state stored a
+ * non-finite state entry or one above 1e20 in magnitude, else 0: the per-step
+ * stability test of Solver::run (Solver.cpp:499-516; the reference tests the
+ * state norm, non-finite or above 1e20, and prints a warning while the run
+ * goes on, as dgtd_op_run does).  Tested in the fused last stage of every
+ * step, at no extra pass over the state. */
+int dgtd_op_unstable(dgtd_op* op, int32_t* flag);
+
 /* FP64 reductions on the resident state (fixed-order, no atomics):
  * out[0] = DOF 2-norm (Fields::getNorml2, Fields.h:21),
  * out[1] = energy 0.5*sum_e J_e (eps E^T M E + mu H^T M H),
@@ -359,10 +367,16 @@ typedef struct {
     int64_t hbm_budget;    /* bytes of instance storage admitted at once (0 = 80% of free HBM) */
 } dgtd_batch_opts;
 
-/* Instance states reported by dgtd_batch_info. */
+/* Instance states reported by dgtd_batch_info.  FAILED: a step stored a
+ * non-finite entry or one above 1e20 in magnitude -- the per-step stability
+ * test of Solver::run (Solver.cpp:499-516), which the reference only reports;
+ * here the instance stops at that step and keeps its fields for inspection.
+ * CANCELLED: stopped by dgtd_batch_cancel. */
 #define DGTD_INST_FREE 0
 #define DGTD_INST_RUNNING 1
 #define DGTD_INST_DONE 2
+#define DGTD_INST_FAILED 3
+#define DGTD_INST_CANCELLED 4
 
 int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out);
 void dgtd_batch_destroy(dgtd_batch* b);
@@ -380,8 +394,8 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 /* Launch nsteps time steps of all running instances (asynchronous). */
 int dgtd_batch_step(dgtd_batch* b, int32_t nsteps);
 
-/* Wait for the device, then report instances that finished since the last
- * poll (up to cap ids) and the number still running. */
+/* Wait for the device, then report instances that ended since the last poll
+ * (done, failed or cancelled; up to cap ids) and the number still running. */
 int dgtd_batch_poll(dgtd_batch* b, int32_t* done_ids, int32_t cap, int32_t* ndone, int32_t* nrunning);
 
 /* Per instance: time, steps taken, DGTD_INST_* state, dt in use. */
@@ -389,6 +403,13 @@ int dgtd_batch_info(dgtd_batch* b, int32_t id, double* t, int32_t* steps, int32_
 
 /* Fields of an instance (host FP64, reference layout). */
 int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x);
+
+/* Stop a running instance before its final time (no counterpart in the
+ * reference, which runs one case per process): it leaves the work list at the
+ * next step, keeps its fields as of the last completed step and is reported
+ * once by dgtd_batch_poll as DGTD_INST_CANCELLED.  An instance that already
+ * ended is left as it is. */
+int dgtd_batch_cancel(dgtd_batch* b, int32_t id);
 
 /* Release an instance's device storage and slot. */
 int dgtd_batch_remove(dgtd_batch* b, int32_t id);

@@ -113,3 +113,96 @@ def test_admission_by_footprint():
     b.remove(i1)
     b.add(vx, E, fbc, 3, final_time=0.1)
     assert b.stats()["bytes_in_use"] == 2 * fp
+
+
+def test_unstable_instances_fail_and_the_rest_finish_as_alone():
+    """The per-step stability test (Solver::run, Solver.cpp:499-516): a NaN
+    in one request's initial fields (order 1: the per-node stage) and an
+    over-CFL dt in another (order 3, 10x the Hesthaven rule: the matrix-core
+    stage) retire those instances as failed at the step whose stored state
+    first holds a non-finite entry or one above 1e20; the other instances
+    run to their final time bitwise as alone, and the slots free up."""
+    good = [(3, 2, 0.25, "pec"), (2, 4, 0.15, "sma")]
+    alone = []
+    for k, (n, order, T, bc) in enumerate(good):
+        vx, E, fbc, ref, q0 = case_input(n, order, bc, 200 + k)
+        b1 = D.Batch(precision=32, max_instances=1)
+        i = b1.add(vx, E, fbc, order, x0=q0, final_time=T)
+        b1.run(poll_every=3)
+        alone.append((b1.get_state(i), b1.info(i)["steps"]))
+        b1.close()
+    b = D.Batch(precision=32, max_instances=8)
+    ids = []
+    for k, (n, order, T, bc) in enumerate(good):
+        vx, E, fbc, ref, q0 = case_input(n, order, bc, 200 + k)
+        ids.append(b.add(vx, E, fbc, order, x0=q0, final_time=T))
+    vx, E, fbc, ref, q0 = case_input(2, 1, "pec", 300)
+    qn = q0.copy()
+    qn[7] = np.nan
+    nan_id = b.add(vx, E, fbc, 1, x0=qn, final_time=0.3)
+    vx, E, fbc, ref, q0 = case_input(2, 3, "pec", 301)
+    dt_rule = 1.0 / (ref.max_fscale() * 9)
+    # the oracle's own run crosses 1e20 at step 6 with this dt (the state's largest entry)
+    x, res = q0.copy(), np.zeros_like(q0)
+    first = None
+    for s in range(40):
+        ref.lserk4_step(x, res, s * 10 * dt_rule, 10 * dt_rule)
+        if not np.abs(x).max() <= 1e20:
+            first = s + 1
+            break
+    assert first is not None
+    cfl_id = b.add(vx, E, fbc, 3, x0=q0, dt=10 * dt_rule, final_time=400 * dt_rule)
+    ended = {}
+    while True:
+        done, running = b.poll()
+        for i in done:
+            ended[i] = b.info(i)
+        if running == 0:
+            break
+        b.step(2)
+    assert ended[nan_id]["state"] == "failed" and ended[nan_id]["steps"] == 1
+    assert ended[cfl_id]["state"] == "failed"
+    assert abs(ended[cfl_id]["steps"] - first) <= 1  # FP32 vs FP64 growth
+    for i, (xa, steps) in zip(ids, alone):
+        assert ended[i]["state"] == "done" and ended[i]["steps"] == steps
+        np.testing.assert_array_equal(b.get_state(i), xa)
+    for i in (nan_id, cfl_id):
+        b.remove(i)
+    assert b.stats()["running"] == 0
+
+
+def test_cancel_stops_one_instance():
+    vx, E, fbc, ref, q0 = case_input(2, 2, "pec", 8)
+    b = D.Batch(precision=32, max_instances=4)
+    keep = b.add(vx, E, fbc, 2, x0=q0, final_time=0.3)
+    stop = b.add(vx, E, fbc, 2, x0=q0, final_time=0.3)
+    b.step(3)
+    b.cancel(stop)
+    done, running = b.poll()
+    assert done == [stop] and running == 1
+    info = b.info(stop)
+    assert info["state"] == "cancelled" and info["steps"] == 3
+    x3 = b.get_state(stop)
+    b.run(poll_every=2)
+    np.testing.assert_array_equal(b.get_state(stop), x3)  # did not move after the cancel
+    assert b.info(keep)["state"] == "done" and b.info(keep)["steps"] > 3
+    b.cancel(keep)  # already ended: left as it is
+    assert b.info(keep)["state"] == "done"
+
+
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+def test_operator_stability_flag(kernel):
+    vx, E, fbc, ref, q0 = case_input(2, 3, "pec", 9)
+    op = D.HesthavenEvolution(vx, E, fbc, 3, precision=32, integrator="lserk4", kernel=kernel)
+    dt = 1.0 / (op.max_fscale() * 9)
+    op.set_state(q0)
+    op.advance(dt, 5)
+    assert not op.unstable()
+    bad = q0.copy()
+    bad[3] = np.inf
+    op.set_state(bad)
+    op.advance(dt, 1)
+    assert op.unstable()
+    op.set_state(q0)  # cleared by a new state
+    assert not op.unstable()
+    op.close()
