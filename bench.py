@@ -62,13 +62,13 @@ def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup, kernel="au
                 bytes_el=bytes_el, flops_el=flops_el, variant=variant)
 
 
-def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0, threads=None):
+def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0, threads=None, eps=None):
     """LSERK4 steps (5 operator applications each) of the oracle on the host,
     as many as fit in about budget_s seconds (at least one), on `threads`
     OpenMP threads (default: all)."""
     from oracle import oracle as O
 
-    ref = O.Operator(vx, E.reshape(-1, 4), fbc.reshape(-1, 4).astype(np.int32), order)
+    ref = O.Operator(vx, E.reshape(-1, 4), fbc.reshape(-1, 4).astype(np.int32), order, eps=eps)
     all_threads = O.threads()
     if threads:
         O.set_threads(threads)
@@ -120,12 +120,91 @@ def mass_norm(D, vx, E, fbc, order, v):
         op.close()
 
 
+def work_per_element(order, precision):
+    """Algorithmic bytes and flops per tetrahedron per stage (the library's
+    dgtd_op_work_per_element formula): 24 Np s + 27 s + 32 bytes and
+    36 Np^2 + 48 Np Nfp + 240 Nfp + 96 Np flops."""
+    s = 4 if precision == 32 else 8
+    Np = (order + 1) * (order + 2) * (order + 3) // 6
+    Nfp = (order + 1) * (order + 2) // 2
+    return 24 * Np * s + 27 * s + 32, 36 * Np * Np + 48 * Np * Nfp + 240 * Nfp + 96 * Np
+
+
+def roofline_fracs(bytes_, flops, seconds, precision):
+    """HBM and matrix-core fractions of `bytes_` and `flops` done in `seconds`,
+    the attainable (max of the two ideal times) fraction, and the bound."""
+    peak_tf = FP32_PEAK_TFLOPS if precision == 32 else FP64_PEAK_TFLOPS
+    gbs = bytes_ / seconds / 1e9
+    tf = flops / seconds / 1e12
+    t_hbm = bytes_ / (HBM_PEAK_GBS * 1e9)
+    t_mfma = flops / (peak_tf * 1e12)
+    return {"bound": "mfma" if t_mfma > t_hbm else "hbm",
+            "hbm_achieved_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+            "achieved_tflops": tf, "mfma_frac": tf / peak_tf, "peak_tflops": peak_tf,
+            "ideal_us": {"hbm": t_hbm * 1e6, "mfma": t_mfma * 1e6},
+            "attainable_frac": max(t_hbm, t_mfma) / seconds}
+
+
+def te110_exact(pos, t):
+    """Analytic TE110 mode of the PEC unit cube (eps = mu = 1) at the nodes."""
+    x, y = pos[0], pos[1]
+    w = np.pi * np.sqrt(2.0)
+    q = np.zeros((6,) + x.shape)
+    q[2] = np.sin(np.pi * x) * np.sin(np.pi * y) * np.cos(w * t)
+    q[3] = -(np.pi / w) * np.sin(np.pi * x) * np.cos(np.pi * y) * np.sin(w * t)
+    q[4] = (np.pi / w) * np.cos(np.pi * x) * np.sin(np.pi * y) * np.sin(w * t)
+    return q.ravel()
+
+
+def bench_c1(D, args, cpu=True):
+    """8^3 hexes x 6 = 3,072 tets, p=3, PEC unit cube, TE110 initial mode
+    (Ez = sin(pi x) sin(pi y)), upwind, LSERK4 dt = 2e-3 to T = 1: FP64 on
+    the GPU end to end against the analytic mode, and the same run of the
+    CPU FP64 oracle end to end (all host threads)."""
+    n, order, dt, T = 8, 3, 2e-3, 1.0
+    vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
+    K = E.shape[0]
+    x0 = D.project_initial_fields(vx, E, order, [D.resonant((1, 1), "E", (0, 0, 1))])
+    dof = 6 * D.num_nodes(order) * K
+    op = D.HesthavenEvolution(vx, E, fbc, order, precision=64, integrator="lserk4")
+    pos = op.node_positions()
+    op.set_state(x0)
+    op.advance(dt, 2)
+    op.set_state(x0)
+    t0 = time.perf_counter()
+    steps = op.run(dt, T)
+    wall = time.perf_counter() - t0
+    xg = op.get_state()
+    variant = op.kernel_variant()
+    op.close()
+    ex = te110_exact(pos, T)
+    out = {"workload": f"PEC cavity [0,1]^3, {n}^3 hexes x 6 = {K} tets, p={order}, TE110, LSERK4 dt={dt}, T={T}",
+           "dof": dof, "steps": steps,
+           "fp64": {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
+                    "kernel_variant": variant, "us_per_step": 1e6 * wall / steps},
+           "rel_l2_vs_analytic": float(np.linalg.norm(xg - ex) / np.linalg.norm(ex))}
+    if cpu:
+        from oracle import oracle as O
+
+        ref = O.Operator(vx, E.reshape(-1, 4), fbc.reshape(-1, 4).astype(np.int32), order)
+        x = x0.copy()
+        t0 = time.perf_counter()
+        csteps, _ = ref.run(x, dt, T, "lserk4")
+        cwall = time.perf_counter() - t0
+        out["rel_l2_gpu_fp64_vs_oracle"] = float(np.linalg.norm(xg - x) / np.linalg.norm(x))
+        out["cpu_baseline"] = {"value": dof * 5 * csteps / cwall, "unit": "DOF*RK-stages/s",
+                               "cores": O.threads(), "kind": "port", "wall_s": cwall,
+                               "sample": f"the whole run ({csteps} LSERK4 steps) of the FP64 CPU oracle, "
+                                         f"OpenMP {O.threads()} threads"}
+    return out
+
+
 def bench_c3(D, args):
     """48^3 hexes x 6 = 663,552 tets on [0,2]^3, p=3, eps_r ~ U[1,4] per element
     (seeded), mu_r = 1, SMA on all sides, Ez Gaussian pulse, dt = 5e-4,
     FP32 state with FP64 energy reductions, 2000 LSERK4 steps (FP64 beside it)."""
     n, L, order, dt = 48, 2.0, 3, 5e-4
-    steps = args.steps if args.steps != 100 else 2000
+    steps = args.steps if args.workload == "c3" and args.steps != 100 else 2000
     vx, E, fbc = D.cube_mesh(n, L, "sma")
     K = E.shape[0]
     rng = np.random.default_rng(args.seed)
@@ -151,14 +230,29 @@ def bench_c3(D, args):
         nm = op.norms()
         b, f = op.work_per_element()
         states[prec] = op.get_state()
+        var = op.kernel_variant()
         out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
-                            "kernel_avg_us": us, "kernel_variant": op.kernel_variant(),
+                            "kernel_avg_us": us, "kernel_variant": var,
+                            "kernel": ("dg_stage_mfma" if var == "mfma" else "dg_stage_kernel")
+                                      + f"<{order},{'float' if prec == 32 else 'double'},LSERK>",
+                            "algorithmic_bytes_per_launch": b * K, "algorithmic_flops_per_launch": f * K,
+                            "roofline": roofline_fracs(b * K, f * K, us * 1e-6, prec),
                             "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
                             "energy_initial": e0, "energy_final": nm["energy"], "nonfinite": nm["nonfinite"]}
+        if prec == 32:
+            traffic, src = profiled_traffic(out["fp32"]["kernel"], f"smabox{n}_p{order}_fp32_lserk4")
+            out["fp32"]["roofline"]["traffic"] = traffic
+            out["fp32"]["roofline"]["traffic_source"] = src
         op.close()
     out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(states[32] - states[64]) / np.linalg.norm(states[64]))
     out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
-    print(json.dumps(out))
+    if not args.no_cpu:
+        wall, csteps, threads = cpu_baseline(vx, E, fbc, order, x0, dt, budget_s=8.0, eps=eps)
+        out["cpu_baseline"] = {"value": dof * 5 * csteps / wall, "unit": "DOF*RK-stages/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{csteps} LSERK4 step(s) of the same mesh (SMA, random eps), FP64 CPU "
+                                         f"oracle, OpenMP {threads} threads, {wall:.2f} s"}
+    return out
 
 
 def bench_global(D, args):
@@ -269,16 +363,10 @@ def bench_c4(D, args):
     """64 independent PEC cavities, edge n_i log-uniform over {2..24} hexes,
     order uniform over {1..5}, seeded Gaussian pulses, FP32, one persistent
     launch per stage, hipGraph-replayed steps; all run to final time 0.1."""
-    rng = np.random.default_rng(args.seed)
+    from dgtd_amd import workloads as W
+
     T = 0.1
-    insts = []
-    for k in range(64):
-        n = int(round(np.exp(rng.uniform(np.log(2), np.log(24)))))
-        order = int(rng.integers(1, 6))
-        vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
-        c = rng.uniform(0.3, 0.7, 3)
-        x0 = D.project_initial_fields(vx, E, order, [D.gaussian("E", c, 0.1, (0, 0, 1))])
-        insts.append((n, order, vx, E, fbc, x0))
+    insts = W.cavity_batch(args.seed, 64)
     res = {}
     for prec in (32, 64):
         b = D.Batch(precision=prec, max_instances=64)
@@ -289,13 +377,20 @@ def bench_c4(D, args):
         launched = b.run(poll_every=16)
         wall = time.perf_counter() - t0
         work = 0
+        bytes_, flops = 0.0, 0.0
         states = []
         for (n, order, vx, E, fbc, x0), i in zip(insts, ids):
             info = b.info(i)
-            work += 6 * D.num_nodes(order) * E.shape[0] * 5 * info["steps"]
+            K = E.shape[0]
+            stages = 5 * info["steps"]
+            work += 6 * D.num_nodes(order) * K * stages
+            be, fe = work_per_element(order, prec)
+            bytes_ += be * K * stages
+            flops += fe * K * stages
             states.append(b.get_state(i))
         res[prec] = dict(value=work / wall, wall_s=wall, graph_steps=launched, items=st["items"],
-                         grid=st["grid"], states=states)
+                         grid=st["grid"], states=states, roofline=roofline_fracs(bytes_, flops, wall, prec),
+                         bytes=bytes_, flops=flops)
         b.close()
     errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(res[32]["states"], res[64]["states"])]
     out = {"workload": "64 PEC cavities, n~logU{2..24}, p~U{1..5}, Gaussian pulses, T=0.1, LSERK4",
@@ -304,10 +399,12 @@ def bench_c4(D, args):
     for prec in (32, 64):
         r = res[prec]
         out[f"fp{prec}"] = {"value": r["value"], "unit": "DOF*RK-stages/s", "wall_s": r["wall_s"],
-                            "graph_steps": r["graph_steps"], "work_items": r["items"], "grid": r["grid"]}
+                            "graph_steps": r["graph_steps"], "work_items": r["items"], "grid": r["grid"],
+                            "algorithmic_bytes": r["bytes"], "algorithmic_flops": r["flops"],
+                            "roofline": r["roofline"]}
     out["max_rel_l2_fp32_vs_fp64"] = max(errs)
     out["fp32_over_fp64"] = res[32]["value"] / res[64]["value"]
-    print(json.dumps(out))
+    return out
 
 
 def bench_c5(D, args):
@@ -342,7 +439,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison path")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma", "csr"])
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "tri", "global"],
+    ap.add_argument("--headline-only", action="store_true",
+                    help="c2: skip the c1, c3 and c4 sections of the line")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "tri", "global"],
                     help="c2 (default, the headline): 32^3 cavity p=4; c3: 48^3 SMA box with random eps, "
                          "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream; "
                          "tri: the 2D operator on 524,288 triangles, p=4")
@@ -367,10 +466,12 @@ def main():
 
     import dgtd_amd as D
 
+    if args.workload == "c1":
+        return print(json.dumps(bench_c1(D, args, cpu=not args.no_cpu)))
     if args.workload == "c3":
-        return bench_c3(D, args)
+        return print(json.dumps(bench_c3(D, args)))
     if args.workload == "c4":
-        return bench_c4(D, args)
+        return print(json.dumps(bench_c4(D, args)))
     if args.workload == "c5":
         return bench_c5(D, args)
     if args.workload == "tri":
@@ -475,6 +576,10 @@ def main():
             "sample_1_thread": f"{csteps1} LSERK4 step(s) of the same mesh on 1 thread (the reference's "
                                f"serial Mult), {wall1:.2f} s",
         }
+    if not args.headline_only:
+        # the other configurations of the metric, measured in the same run
+        out["configs"] = {"c1": bench_c1(D, args, cpu=not args.no_cpu), "c3": bench_c3(D, args),
+                          "c4": bench_c4(D, args)}
     print(json.dumps(out))
 
 
