@@ -70,7 +70,8 @@ struct dgtd_batch {
     int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
 
-    static constexpr int kSlotReserved = 3;  // host only: being set up by dgtd_batch_add
+    static constexpr int kSlotReserved = 100;  // host only: being set up by dgtd_batch_add (not a BatchState)
+    static_assert(kSlotReserved > dgtd::kSlotCancelled, "the host-only state must differ from every BatchState");
 
     struct Slot {
         int state = dgtd::kSlotFree;
