@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libdgtd_hip.so")
+# DGTD_LIB: another build of the same library (the sanitizer build of the host
+# code, make -C dgtd_amd/csrc asan); by default the in-tree build.
+LIB_PATH = os.environ.get("DGTD_LIB") or os.path.join(_HERE, "lib", "libdgtd_hip.so")
 
 c_int32 = ctypes.c_int32
 c_double = ctypes.c_double

@@ -14,6 +14,7 @@
 
 #include "dgtd_hip.h"
 #include "kernels.hpp"
+#include "trace.hpp"
 #include "mesh.hpp"
 #include "packing.hpp"
 #include "reftet.hpp"
@@ -211,6 +212,7 @@ struct dgtd_batch {
     template <typename T>
     void capture(int max_order)
     {
+        dgtd::TraceRange tr("dgtd batch capture");
         if (graph) {
             hcheck(hipGraphExecDestroy(graph), "hipGraphExecDestroy");
             graph = nullptr;
@@ -376,6 +378,7 @@ void dgtd_batch_destroy(dgtd_batch* b) { delete b; }
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt, double final_time,
                    int32_t* id)
 {
+    dgtd::TraceRange tr("dgtd_batch_add");
     if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "batched instances are tetrahedral meshes");
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
@@ -454,6 +457,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 
 int dgtd_batch_step(dgtd_batch* b, int32_t nsteps)
 {
+    dgtd::TraceRange tr("dgtd_batch_step");
     if (!b || nsteps < 0) return dgtd::api_fail(DGTD_E_ARG, "bad argument");
     std::lock_guard<std::mutex> lk(b->mu);
     try {

@@ -16,6 +16,7 @@
 
 #include "fields.hpp"
 #include "kernels.hpp"
+#include "trace.hpp"
 #include "mesh.hpp"
 #include "packing.hpp"
 #include "reftet.hpp"
@@ -192,6 +193,7 @@ struct dgtd_op {
     template <typename T>
     void launch(const dgtd::StageArgs<T>& a, int mode)
     {
+        dgtd::TraceRange tr("dgtd stage launch");
         hip_check(launch_on<T>(a, mode, variant, stream), "stage kernel launch");
     }
 
@@ -202,6 +204,7 @@ struct dgtd_op {
     template <typename T>
     void autotune()
     {
+        dgtd::TraceRange tr("dgtd autotune");
         if (grid_mfma <= 0) { variant = DGTD_KERNEL_VALU; return; }
         auto a = base_args<T>();
         a.qin = static_cast<const T*>(d_buf[0]);
@@ -262,6 +265,7 @@ struct dgtd_op {
     template <typename T>
     int one_step(double dt)
     {
+        dgtd::TraceRange tr("dgtd step");
         auto a = base_args<T>();
         a.dt = (T)dt;
         if (integrator == DGTD_LSERK4) {
@@ -622,6 +626,7 @@ const char* dgtd_version(void) { return "dgtd_hip 0.1 (gfx950)"; }
 
 int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** out)
 {
+    dgtd::TraceRange tr("dgtd_op_create");
     if (!mesh || !opts || !out) return fail(DGTD_E_ARG, "NULL argument");
     *out = nullptr;
     if (opts->precision != 64 && opts->precision != 32)
@@ -821,6 +826,7 @@ int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
 
 int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, void* stream)
 {
+    dgtd::TraceRange tr("dgtd_op_mult_device");
     if (!op || !d_in || !d_out) return fail(DGTD_E_ARG, "NULL argument");
     if (d_in == d_out) return fail(DGTD_E_ARG, "in and out must not alias");
     try {
@@ -836,6 +842,7 @@ int dgtd_op_mult_device(dgtd_op* op, double t, const void* d_in, void* d_out, vo
 
 int dgtd_op_mult(dgtd_op* op, double t, const double* in, double* out)
 {
+    dgtd::TraceRange tr("dgtd_op_mult");
     if (!op || !in || !out) return fail(DGTD_E_ARG, "NULL argument");
     void *d_in = nullptr, *d_out = nullptr;
     try {
@@ -890,6 +897,7 @@ int dgtd_op_get_state(dgtd_op* op, double* x, double* t)
 
 int dgtd_op_advance(dgtd_op* op, double dt, int32_t nsteps)
 {
+    dgtd::TraceRange tr("dgtd_op_advance");
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     if (nsteps < 0 || !std::isfinite(dt)) return fail(DGTD_E_ARG, "bad dt or nsteps");
     try {
@@ -917,6 +925,7 @@ int dgtd_op_advance(dgtd_op* op, double dt, int32_t nsteps)
 
 int dgtd_op_run(dgtd_op* op, double dt, double final_time, int32_t* steps)
 {
+    dgtd::TraceRange tr("dgtd_op_run");
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     if (!(dt > 0.0) || !std::isfinite(final_time)) return fail(DGTD_E_ARG, "bad dt or final_time");
     int n = 0;
