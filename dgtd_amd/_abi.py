@@ -93,6 +93,7 @@ SIGNATURES = {
     "dgtd_op_step": (c_int32, [c_void_p, _dp, _dp, c_double, c_int32]),
     "dgtd_op_norms": (c_int32, [c_void_p, _dp]),
     "dgtd_op_unstable": (c_int32, [c_void_p, _i32p]),
+    "dgtd_op_norms_noalloc": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dgtd_op_set_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave), c_int32]),
     "dgtd_op_probe_set": (c_int32, [c_void_p, c_int32, _dp, _i32p]),
     "dgtd_op_probe_sample": (c_int32, [c_void_p, _dp]),

@@ -525,8 +525,10 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     op->d_eps = static_cast<double*>(upload(md.eps));
     op->d_mu = static_cast<double*>(upload(md.mu));
     const int te = dgtd::norms_tile(op->Np);
-    op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
+    op->nparts = std::max(1, std::min(dgtd::kNormsMaxParts, (op->K + te - 1) / te));
     hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+    double* scratch_dev = nullptr;  // resolves the allocation-free entry point's scratch on this device
+    hip_check(dgtd::norms_scratch(op->device, &scratch_dev), "norms scratch");
     hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
     hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
     hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
@@ -704,8 +706,10 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->d_eps = static_cast<double*>(upload(md.eps));
         op->d_mu = static_cast<double*>(upload(md.mu));
         const int te = dgtd::norms_tile(op->Np);
-        op->nparts = std::max(1, std::min(1024, (op->K + te - 1) / te));
+        op->nparts = std::max(1, std::min(dgtd::kNormsMaxParts, (op->K + te - 1) / te));
         hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+        double* scratch_dev = nullptr;  // resolves the allocation-free entry point's scratch on this device
+        hip_check(dgtd::norms_scratch(op->device, &scratch_dev), "norms scratch");
         hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
         hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
         hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
@@ -981,6 +985,32 @@ int dgtd_op_unstable(dgtd_op* op, int32_t* flag)
         hip_check(hipMemcpyAsync(&h, op->d_unstable, sizeof(int), hipMemcpyDeviceToHost, op->stream), "D2H");
         hip_check(hipStreamSynchronize(op->stream), "sync");
         *flag = h ? 1 : 0;
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_HIP, e.what());
+    }
+    return DGTD_OK;
+}
+
+int dgtd_op_norms_noalloc(dgtd_op* op, const void* d_state, double* d_out3, void* stream)
+{
+    if (!op || !d_out3) return fail(DGTD_E_ARG, "NULL argument");
+    if (op->nparts > dgtd::kNormsMaxParts)
+        return fail(DGTD_E_ARG, "reduction needs " + std::to_string(op->nparts) + " partial blocks, the scratch holds " +
+                                    std::to_string(dgtd::kNormsMaxParts));
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        double* part = nullptr;
+        hip_check(dgtd::norms_scratch(op->device, &part), "norms scratch");
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : op->stream;
+        const void* q = d_state ? d_state : op->state();
+        hipError_t e;
+        if (op->precision == 64)
+            e = dgtd::launch_norms<double>(static_cast<const double*>(q), op->K, op->Np, op->d_M, op->d_jac,
+                                           op->d_eps, op->d_mu, part, op->nparts, d_out3, s);
+        else
+            e = dgtd::launch_norms<float>(static_cast<const float*>(q), op->K, op->Np, op->d_M, op->d_jac,
+                                          op->d_eps, op->d_mu, part, op->nparts, d_out3, s);
+        hip_check(e, "norms kernel launch");
     } catch (const std::exception& e) {
         return fail(DGTD_E_HIP, e.what());
     }

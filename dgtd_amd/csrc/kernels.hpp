@@ -206,6 +206,13 @@ inline int norms_tile(int Np)
     return te < 1 ? 1 : (te > 32 ? 32 : te);
 }
 
+// Partial-sum blocks of the norms reduction (the operators use at most this
+// many), and the device address of the allocation-free entry point's static
+// partials array on `device` (resolved once per device, with that device
+// current).
+constexpr int kNormsMaxParts = 1024;
+hipError_t norms_scratch(int device, double** ptr);
+
 template <typename T>
 hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const double* jac,
                         const double* eps, const double* mu, double* partials, int nparts,

@@ -4,6 +4,8 @@
 // Fixed-order two-pass tree reduction, no atomics: bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace dgtd {
@@ -11,6 +13,18 @@ namespace dgtd {
 namespace {
 
 constexpr int kThreads = 256;
+
+// Scratch of the allocation-free norms entry point (dgtd_op_norms_noalloc):
+// the per-block partials of norms_partial, at most kNormsMaxParts blocks x 3
+// doubles (24 KiB on every GPU the library runs on, held until the process
+// ends).  In host code `scratch` names a host-side stand-in, never the device
+// array: its device address is resolved per device by norms_scratch.  Every
+// call on one GPU shares that GPU's copy, so calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[3 * kNormsMaxParts];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
 
 __device__ void block_reduce3(double v[3], double* sh)
 {
@@ -105,6 +119,24 @@ hipError_t launch_norms(const T* q, int K, int Np, const double* Mref, const dou
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(norms_final, dim3(1), dim3(kThreads), 0, s, partials, nparts, out3);
     return hipGetLastError();
+}
+
+hipError_t norms_scratch(int device, double** ptr)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // resolved with the device current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *ptr = g_scratch[device];
+    return hipSuccess;
 }
 
 template hipError_t launch_norms<double>(const double*, int, int, const double*, const double*,

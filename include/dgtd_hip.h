@@ -179,6 +179,20 @@ int dgtd_op_step(dgtd_op* op, double* x, double* t, double dt, int32_t nsteps);
  * step, at no extra pass over the state. */
 int dgtd_op_unstable(dgtd_op* op, int32_t* flag);
 
+/* Allocation-free variant of dgtd_op_norms.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same three FP64 values of a device state in the operator's precision
+ * (d_state; NULL = the resident state) are written to d_out3 (3 device
+ * doubles), asynchronously on `stream` (NULL = the handle's stream).  The
+ * reduction's partial sums live in a static device array of the library
+ * (3 x 1024 doubles per GPU, resolved at dgtd_op_create); an operator whose
+ * reduction needs more than 1024 partial blocks is refused with DGTD_E_ARG
+ * (none does: dgtd_op_create caps the count at 1024).  No hipMalloc, hipFree
+ * or allocating library call is made.  Calls that overlap on one GPU share
+ * that array and race: serialise them (one stream, or events). */
+int dgtd_op_norms_noalloc(dgtd_op* op, const void* d_state, double* d_out3, void* stream);
+
 /* FP64 reductions on the resident state (fixed-order, no atomics):
  * out[0] = DOF 2-norm (Fields::getNorml2, Fields.h:21),
  * out[1] = energy 0.5*sum_e J_e (eps E^T M E + mu H^T M H),
