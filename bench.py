@@ -46,6 +46,7 @@ def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup, kernel="au
     op = D.HesthavenEvolution(vx, E, fbc, order, precision=precision, integrator="lserk4",
                               kernel=kernel)
     variant = op.kernel_variant()
+    kname, tuning = stage_kernel(op, order, precision)
     op.set_state(q0, 0.0)
     if warmup:
         op.advance(dt, warmup)
@@ -59,7 +60,7 @@ def time_path(D, vx, E, fbc, order, precision, q0, dt, steps, warmup, kernel="au
     bytes_el, flops_el = op.work_per_element()
     op.close()
     return dict(wall=wall, kern_us=kern_us, launches=launches, state=state, norms=nm,
-                bytes_el=bytes_el, flops_el=flops_el, variant=variant)
+                bytes_el=bytes_el, flops_el=flops_el, variant=variant, kernel=kname, tuning=tuning)
 
 
 def cpu_baseline(vx, E, fbc, order, q0, dt, budget_s=10.0, threads=None, eps=None):
@@ -128,6 +129,21 @@ def work_per_element(order, precision):
     Np = (order + 1) * (order + 2) * (order + 3) // 6
     Nfp = (order + 1) * (order + 2) // 2
     return 24 * Np * s + 27 * s + 32, 36 * Np * Np + 48 * Np * Nfp + 240 * Nfp + 96 * Np
+
+
+def stage_kernel(op, order, precision, dim=3):
+    """The stage kernel an operator runs, named as in the rocprof summaries
+    (matrix-core kernels with their tuning configuration), and the
+    autotuner's record for it."""
+    t = op.tuning()
+    ty = "float" if precision == 32 else "double"
+    if t["variant"] == "mfma":
+        name = f"dg_stage_mfma<{order},{ty},LSERK,{dim}D,cfg{t['config']}>"
+    elif dim == 2:
+        name = f"dg_tri_stage<{order},{ty},LSERK>"
+    else:
+        name = f"dg_stage_kernel<{order},{ty},LSERK>"
+    return name, t
 
 
 def roofline_fracs(bytes_, flops, seconds, precision):
@@ -231,10 +247,9 @@ def bench_c3(D, args):
         b, f = op.work_per_element()
         states[prec] = op.get_state()
         var = op.kernel_variant()
+        kname, tuning = stage_kernel(op, order, prec)
         out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
-                            "kernel_avg_us": us, "kernel_variant": var,
-                            "kernel": ("dg_stage_mfma" if var == "mfma" else "dg_stage_kernel")
-                                      + f"<{order},{'float' if prec == 32 else 'double'},LSERK>",
+                            "kernel_avg_us": us, "kernel_variant": var, "kernel": kname, "tuning": tuning,
                             "algorithmic_bytes_per_launch": b * K, "algorithmic_flops_per_launch": f * K,
                             "roofline": roofline_fracs(b * K, f * K, us * 1e-6, prec),
                             "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
@@ -344,11 +359,10 @@ def bench_tri(D, args):
         nm = op.norms()
         b, f = op.work_per_element()
         states[prec] = op.get_state()
+        kname, tuning = stage_kernel(op, order, prec, dim=2)
         out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
                             "kernel_avg_us": us, "kernel_variant": op.kernel_variant(),
-                            "kernel": (f"dg_stage_mfma<{order},{'float' if prec == 32 else 'double'},LSERK,2>"
-                                       if op.kernel_variant() == "mfma" else
-                                       f"dg_tri_stage<{order},{'float' if prec == 32 else 'double'},LSERK>"),
+                            "kernel": kname, "tuning": tuning,
                             "algorithmic_bytes_per_launch": b * K,
                             "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
                             "achieved_tflops": f * K / (us * 1e-6) / 1e12,
@@ -483,7 +497,7 @@ def main():
     K = E.shape[0]
     Np = D.num_nodes(order)
     dof = 6 * Np * K
-    probe = D.HesthavenEvolution(vx, E, fbc, order, precision=64)
+    probe = D.HesthavenEvolution(vx, E, fbc, order, precision=64, kernel="valu")  # positions and dt only: no tuning
     pos = probe.node_positions()
     dt = D.estimate_time_step(probe)
     probe.close()
@@ -500,7 +514,7 @@ def main():
     t_star = max(t_hbm, t_mfma)
     # the bound is the resource whose ideal time is the longer one
     mfma_bound = t_mfma > t_hbm
-    kname = ("dg_stage_mfma" if r32["variant"] == "mfma" else "dg_stage_kernel") + f"<{order},float,LSERK>"
+    kname = r32["kernel"]
     traffic, traffic_src = profiled_traffic(kname, f"cavity{n}_p{order}_fp32_lserk4")
     out = {
         "metric": "DOF*RK-stages/s (FP32 fused LSERK4 stage, hesthaven DGTD operator)",
@@ -541,6 +555,7 @@ def main():
             "achieved_tflops": achieved_tf,
             "fp32_peak_tflops": FP32_PEAK_TFLOPS,
             "attainable_frac": t_star / kern_s,
+            "tuning": r32["tuning"],
         },
     }
     if not args.no_fp64:
@@ -552,6 +567,8 @@ def main():
             "ms_per_step": 1e3 * r64["wall"] / args.steps,
             "kernel_avg_us": r64["kern_us"],
             "kernel_variant": r64["variant"],
+            "kernel": r64["kernel"],
+            "tuning": r64["tuning"],
             "hbm_frac": r64["bytes_el"] * K / k64 / 1e9 / HBM_PEAK_GBS,
             "achieved_tflops": r64["flops_el"] * K / k64 / 1e12,
         }

@@ -59,6 +59,12 @@ class BatchOpts(ctypes.Structure):
                 ("max_instances", c_int32), ("reserved", c_int32), ("hbm_budget", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("variant", c_int32), ("config", c_int32), ("waves_per_block", c_int32),
+                ("gather_depth", c_int32), ("early_residual", c_int32), ("task_order", c_int32),
+                ("from_cache", c_int32), ("ncandidates", c_int32), ("candidate_us", c_double * 8)]
+
+
 class CaseInfo(ctypes.Structure):
     _fields_ = [("K", c_int32), ("nv", c_int32), ("order", c_int32), ("alpha", c_double),
                 ("time_step", c_double), ("final_time", c_double), ("cfl", c_double),
@@ -100,6 +106,8 @@ SIGNATURES = {
     "dgtd_op_set_timing": (c_int32, [c_void_p, c_int32]),
     "dgtd_op_last_kernel_us": (c_int32, [c_void_p, _dp, _i32p]),
     "dgtd_op_kernel_variant": (c_int32, [c_void_p, _i32p]),
+    "dgtd_op_tuning": (c_int32, [c_void_p, ctypes.POINTER(Tuning)]),
+    "dgtd_op_set_candidate": (c_int32, [c_void_p, c_int32]),
     "dgtd_op_work_per_element": (c_int32, [c_void_p, _dp, _dp]),
     "dgtd_cube_mesh": (c_int32, [c_int32, c_double, _i8p, _dp, _i32p, _i8p]),
     "dgtd_project_initial_fields": (c_int32, [ctypes.POINTER(MeshDesc), ctypes.POINTER(InitialField),

@@ -21,6 +21,7 @@
 
 namespace dgtd {
 int api_fail(int code, const std::string& msg);  // dgtd_api.cpp
+bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg);  // dgtd_api.cpp
 }
 
 namespace {
@@ -68,6 +69,12 @@ struct dgtd_batch {
     void* d_mats[dgtd::kMaxBatchOrder + 1] = {};
     void* d_mats_mfma[dgtd::kMaxBatchOrder + 1] = {};
     int grid_mfma[dgtd::kMaxBatchOrder + 1] = {};
+    // per order, decided by the autotuner when its first instance is admitted
+    // (the process-wide cache, else a tuning run on a cube of that size):
+    // matrix-core stage or per-node stage, and the matrix-core configuration
+    bool tuned[dgtd::kMaxBatchOrder + 1] = {};
+    bool mfma[dgtd::kMaxBatchOrder + 1] = {};
+    int cfg[dgtd::kMaxBatchOrder + 1] = {};
     int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
 
@@ -97,11 +104,41 @@ struct dgtd_batch {
     // matrices in global memory: the 64-cavity batch's FP64 rate rose from
     // 2.03e10 to 2.68e10 DOF*stages/s against the per-node stage); order 1
     // (4 nodes, 16-column tiles mostly padding) runs on the per-node stage.
-    bool use_mfma(int order) const
+    // The matrix-core configuration comes from the autotuner (the same
+    // arithmetic in every configuration, so results do not depend on it).
+    bool use_mfma(int order) const { return tuned[order] && mfma[order]; }
+
+    // Tunes the order once per batch.  Called without `mu` (it may build and
+    // time an operator); `tune_mu` serialises tuning, `mu` guards the result.
+    std::mutex tune_mu;
+    void tune(int order, int64_t K)
     {
-        if (order < 2) return false;
-        return precision == 64 ? dgtd::mfma_available<double>(order) : dgtd::mfma_available<float>(order);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (tuned[order]) return;
+        }
+        std::lock_guard<std::mutex> tl(tune_mu);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (tuned[order]) return;
+        }
+        int c = 0;
+        const bool can = order >= 2 && (precision == 64 ? dgtd::mfma_available<double>(order, 0)
+                                                         : dgtd::mfma_available<float>(order, 0));
+        if (can && !dgtd::tuned_mfma_config(order, precision, K, &c)) c = 0;
+        const bool fits = precision == 64 ? dgtd::mfma_available<double>(order, c) : dgtd::mfma_available<float>(order, c);
+        // Leaves `dirty` alone: marking the host table dirty without pulling
+        // the device table first would push stale step counters over steps in
+        // flight.  The admission that follows pulls, publishes and marks it;
+        // the flush after that sees the order's new stage choice and
+        // recaptures the step.
+        std::lock_guard<std::mutex> lk(mu);
+        mfma[order] = can;
+        cfg[order] = fits ? c : 0;
+        tuned[order] = true;
     }
+    bool cap_mfma[dgtd::kMaxBatchOrder + 1] = {};
+    int cap_cfg[dgtd::kMaxBatchOrder + 1] = {};
 
     ~dgtd_batch()
     {
@@ -206,7 +243,10 @@ struct dgtd_batch {
         hcheck(hipMemcpyAsync(d_count, &nvalu, sizeof(int), hipMemcpyHostToDevice, stream), "H2D count");
         hcheck(hipMemcpyAsync(d_ranges, ranges, sizeof(ranges), hipMemcpyHostToDevice, stream), "H2D ranges");
         hcheck(hipStreamSynchronize(stream), "sync");  // host vectors die here
-        if (max_order != graph_order || !graph) capture<T>(max_order);
+        bool recapture = max_order != graph_order || !graph;
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o)
+            recapture = recapture || use_mfma(o) != cap_mfma[o] || cfg[o] != cap_cfg[o];
+        if (recapture) capture<T>(max_order);
     }
 
     template <typename T>
@@ -228,8 +268,8 @@ struct dgtd_batch {
             a.mats[o] = static_cast<const T*>(d_mats[o]);
             a.mats_mfma[o] = static_cast<const T*>(d_mats_mfma[o]);
             a.tabs[o] = d_tabs[o];
-            if (use_mfma(o) && !grid_mfma[o]) {
-                const int bpc = dgtd::mfma_blocks_per_cu<T>(o);
+            if (use_mfma(o)) {
+                const int bpc = dgtd::mfma_blocks_per_cu<T>(o, cfg[o]);
                 grid_mfma[o] = std::max(1, bpc) * num_cu;
             }
         }
@@ -242,7 +282,8 @@ struct dgtd_batch {
             a.cb = (T)kB[s];
             err = dgtd::launch_batch_stage<T>(a, grid, smem, stream);
             for (int o = 1; o <= dgtd::kMaxBatchOrder && err == hipSuccess; ++o)
-                if (use_mfma(o)) err = dgtd::launch_batch_stage_mfma<T>(o, a, d_ranges + 2 * o, grid_mfma[o], stream);
+                if (use_mfma(o))
+                    err = dgtd::launch_batch_stage_mfma<T>(o, a, d_ranges + 2 * o, grid_mfma[o], stream, cfg[o]);
         }
         if (err == hipSuccess) err = dgtd::launch_batch_advance<T>(a.inst, max_slots, stream);
         hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -252,6 +293,10 @@ struct dgtd_batch {
         (void)hipGraphDestroy(g);
         hcheck(e3, "hipGraphInstantiate");
         graph_order = max_order;
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+            cap_mfma[o] = use_mfma(o);
+            cap_cfg[o] = cfg[o];
+        }
     }
 
     void flush()
@@ -304,7 +349,7 @@ void upload_refs(dgtd_batch* b)
         tabs.insert(tabs.end(), R.face_perm.begin(), R.face_perm.end());
         hcheck(hipMalloc(&b->d_mats[o], sizeof(T) * m.size()), "hipMalloc mats");
         hcheck(hipMemcpy(b->d_mats[o], m.data(), sizeof(T) * m.size(), hipMemcpyHostToDevice), "H2D mats");
-        if (b->use_mfma(o)) {
+        if (dgtd::mfma_available<T>(o, 0) || dgtd::mfma_available<T>(o, 1)) {
             const std::vector<T> mm = dgtd::pack_mats_mfma<T>(R);
             hcheck(hipMalloc(&b->d_mats_mfma[o], sizeof(T) * mm.size()), "hipMalloc mats");
             hcheck(hipMemcpy(b->d_mats_mfma[o], mm.data(), sizeof(T) * mm.size(), hipMemcpyHostToDevice), "H2D mats");
@@ -393,6 +438,12 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
             if (mesh->face_bc[i] == DGTD_BC_TFSF)
                 return dgtd::api_fail(DGTD_E_UNSUPPORTED, "TF/SF faces are not supported by batched instances");
     const int64_t bytes = footprint(mesh->K, mesh->order, b->precision);
+    try {
+        hcheck(hipSetDevice(b->device), "hipSetDevice");
+        b->tune(mesh->order, mesh->K);
+    } catch (const std::exception& e) {
+        return batch_error(e);
+    }
     int slot = -1;
     {  // reserve a slot and the bytes
         std::lock_guard<std::mutex> lk(b->mu);

@@ -1,743 +1,79 @@
-// MFMA (matrix-core) variant of the fused DGTD stage kernel for gfx950.
-//
-// Per element the stage needs, for the six right-hand sides r_c,
-//     r_c = sum_k D_k w_{c,k}  +  LIFT f_c
-// where D_k (k = r,s,t) are the reference derivative matrices, w_{c,k} the
-// metric-weighted curl combinations of two field components
-// (e.g. r_Hx: w_k = (g_kz E_y - g_ky E_z)/mu, g_kd = dr_k/dx_d) and f_c the
-// fscale/2-scaled face flux (with 1/eps or 1/mu folded in).  Both terms are
-// products with matrices shared by all elements, so a wave-task of 8 elements
-// evaluates, for x = 0..2,
-//     [r_Ex ; r_Hx]^T (16 x Np) = [W_x F_x]^T (16 x (3Np + 4Nfp)) * [D_r D_s D_t LIFT]^T
-// with v_mfma_{f32,f64}_16x16x4: A rows 0-7 carry the E_x operands and rows
-// 8-15 the H_x operands of the same 8 elements, columns = output nodes in
-// groups of 16, K = input nodes (for D_k) and face nodes (for LIFT).
-//   - A operands (W, F) are formed in registers from the LDS-staged fields:
-//     lane (row l%16, slot l/16); for the LIFT a k-step covers four
-//     consecutive nodes of one face, each lane gathering the neighbour trace
-//     of its node and forming the flux there; fluxes never touch LDS or HBM.
-//   - B operands (the reference matrices, zero-padded to 16-column groups and
-//     to whole k-steps per face) stay in LDS for the life of the block.
-//   - The accumulators are the finished right-hand sides; the stage update is
-//     applied straight from them.
-// Waves run independent tasks (only the initial matrix load is block-wide);
-// tasks run in Morton order of their centroids, in contiguous ranges per XCD
-// so neighbour re-reads stay in that XCD's L2.
+// Run-time dispatch of the matrix-core stage over its tuning configurations
+// (device code in mfma_stage.hpp, one translation unit per configuration).
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
-
-#include "bc_coeffs.hpp"
 #include "kernels.hpp"
-#include "stability.hpp"
 
 namespace dgtd {
 
-namespace {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
 template <typename T>
-struct Mfma;
-
-template <>
-struct Mfma<float> {
-    using acc_t = f32x4;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // element row held by (slot q, register r): C[4q + r][l%16]
-    static __device__ __forceinline__ int row(int q, int r) { return 4 * q + r; }
-};
-
-template <>
-struct Mfma<double> {
-    using acc_t = f64x4;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // C[q + 4r][l%16]
-    static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
-};
-
-// Element shape: tetrahedra (DIM 3) or triangles (DIM 2, the 2D operator:
-// d/dz = 0, n_z = 0, edges as faces; records as tri.hpp lays them out).
-template <int P, int DIM = 3>
-struct MShape {
-    static constexpr int NFACE = DIM == 3 ? 4 : 3;
-    static constexpr int NDIR = DIM;  // reference derivative directions
-    static constexpr int NP = DIM == 3 ? (P + 1) * (P + 2) * (P + 3) / 6 : (P + 1) * (P + 2) / 2;
-    static constexpr int NFP = DIM == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
-    static constexpr int NF = NFACE * NFP;
-    static constexpr int NPERM = DIM == 3 ? 96 : 18;  // rows of the neighbour-node table
-    static constexpr int GEO = DIM == 3 ? kGeoStrideDev : kTriGeoStrideDev;
-    static constexpr int CONN = DIM == 3 ? kConnStrideDev : kTriConnStrideDev;
-    static constexpr int KP = (NP + 3) / 4 * 4;  // K padding of the volume product
-    static constexpr int S = KP / 4;             // volume k-steps
-    static constexpr int NPG = (NP + 15) / 16 * 16;
-    static constexpr int G = NPG / 16;           // output-node groups
-    static constexpr int US = KP + 1;            // LDS row stride of a task's fields (odd: KP % 4 == 0)
-    static constexpr int NFPP = (NFP + 3) / 4 * 4;  // face nodes padded to whole k-steps
-    static constexpr int SF = NFPP / 4;             // LIFT k-steps per face
-};
-
-constexpr int kTaskElems = kMfmaTaskElems;  // MFMA rows: 8 elements x {E, H} outputs
-constexpr int kGatherDepth = 2;  // flux-loop k-steps whose exterior traces are in flight
-
-// Waves per block, chosen so the block's LDS fits in 160 KiB.  When the
-// reference matrices and kMinLdsWaves waves' tiles do not fit together (FP64
-// at order 5: 135 KiB of matrices), the matrices stay in global memory (read
-// through the caches; 135 KiB is L2-resident) and LDS holds the node tables
-// and the tiles only.
-template <int P, typename T, int DIM = 3>
-struct MWaves {
-    static constexpr int kMaxWaves = sizeof(T) == 4 ? 12 : 8;
-    static constexpr int kMinLdsWaves = 4;
-    using S = MShape<P, DIM>;
-    static constexpr size_t bmats =
-        sizeof(T) * ((size_t)S::KP * S::NDIR * S::NPG + (size_t)S::NFACE * S::NFPP * S::NPG);
-    static constexpr size_t tables = sizeof(int) * (size_t)(S::NFACE * S::NFP + S::NPERM * S::NFP);
-    static constexpr size_t per_wave = sizeof(T) * (6 * kTaskElems * (size_t)S::US + 2 * kTaskElems * (size_t)S::NP);
-    static constexpr size_t budget = 160 * 1024;
-    static constexpr bool bglobal = bmats + tables + kMinLdsWaves * per_wave > budget;
-    static constexpr size_t mats = (bglobal ? 0 : bmats) + tables;
-    static constexpr int fit = (int)((budget - mats) / per_wave);
-    static constexpr int W = fit >= kMaxWaves ? kMaxWaves : (fit < 1 ? 1 : fit);
-    static constexpr size_t bytes = mats + per_wave * W;
-    static constexpr bool ok = bytes <= budget;
-};
-
-// Value of the lane 8 away within the 16-lane row (DPP row_ror:8 swaps the
-// two halves of every row: rows l%16 < 8 and >= 8 of an MFMA A operand).
-__device__ __forceinline__ float dpp_swap8(float v)
+MfmaInfo mfma_info(int order, int dim, int cfg)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
-}
-
-__device__ __forceinline__ double dpp_swap8(double v)
-{
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffLL), 0x128, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), 0x128, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Per-block LDS carve-up of the MFMA stage (reference matrices shared by the
-// block, per-wave field and staging tiles).
-template <int P, typename T, int DIM = 3>
-struct MfmaLds {
-    const T* sBv;  // [KP][3*NPG] (LDS, or global memory when MWaves::bglobal)
-    const T* sBl;  // [4*NFPP][NPG], row f*NFPP + b
-    T* sU;    // [WAVES][6][TE][US]
-    T* sS;    // [WAVES][2][TE*NP] output staging
-    int* sFm; // [4*NFP]
-    int* sPt; // [96*NFP]
-};
-
-// Loads the reference matrices and node tables of order P into LDS and zeroes
-// the field tiles (their padding columns j >= NP stay 0); block-wide.
-template <int P, typename T, int DIM = 3>
-__device__ MfmaLds<P, T, DIM> mfma_load_ref(const T* mats, const int* tabs, unsigned char* smem_raw)
-{
-    using S = MShape<P, DIM>;
-    constexpr int NP = S::NP, NFP = S::NFP, KP = S::KP, NPG = S::NPG, US = S::US, NFPP = S::NFPP;
-    constexpr int NDIR = S::NDIR, NFACE = S::NFACE;
-    constexpr int TE = kTaskElems;
-    constexpr int WAVES = MWaves<P, T, DIM>::W;
-    constexpr bool kBGlobal = MWaves<P, T, DIM>::bglobal;
-    MfmaLds<P, T, DIM> L;
-    T* sB = reinterpret_cast<T*>(smem_raw);
-    L.sU = kBGlobal ? sB : sB + KP * NDIR * NPG + NFACE * NFPP * NPG;
-    L.sS = L.sU + WAVES * 6 * TE * US;
-    L.sFm = reinterpret_cast<int*>(L.sS + WAVES * 2 * TE * NP);
-    L.sPt = L.sFm + NFACE * NFP;
-    const int tid = threadIdx.x;
-    const int nthreads = WAVES * 64;
-    if constexpr (kBGlobal) {
-        L.sBv = mats;
-    } else {
-        for (int i = tid; i < KP * NDIR * NPG + NFACE * NFPP * NPG; i += nthreads) sB[i] = mats[i];
-        L.sBv = sB;
-    }
-    L.sBl = L.sBv + KP * NDIR * NPG;
-    for (int i = tid; i < NFACE * NFP + S::NPERM * NFP; i += nthreads) L.sFm[i] = tabs[i];
-    for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) L.sU[i] = T(0);
-    __syncthreads();
-    return L;
-}
-
-// One wave-task: the fused stage of elements [task*8, task*8 + 8) of the mesh
-// described by `a`.  Only wave-level synchronisation inside.
-template <int P, typename T, int MODE, int DIM = 3>
-__device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P, T, DIM>& L, const int task)
-{
-    using S = MShape<P, DIM>;
-    constexpr int NFACE = S::NFACE, NDIR = S::NDIR;
-    using MM = Mfma<T>;
-    using acc_t = typename MM::acc_t;
-    constexpr int NP = S::NP, NFP = S::NFP, NPG = S::NPG, G = S::G, US = S::US;
-    constexpr int NFPP = S::NFPP, SF = S::SF;
-    constexpr int TE = kTaskElems;
-    const T* sBv = L.sBv;
-    const T* sBl = L.sBl;
-    const int* sFm = L.sFm;
-    const int* sPt = L.sPt;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int lane = threadIdx.x & 63;
-    const int q = lane >> 4;    // k slot within a step
-    const int li = lane & 15;   // A row (= output column within a node group)
-    const int el = li & 7;      // element of the A row
-    const int half = li >> 3;   // A rows 0-7: E outputs, 8-15: H outputs of the same elements
-    T* U = L.sU + wave * 6 * TE * US;
-    T* Sg = L.sS + wave * 2 * TE * NP;
-    const int K = a.K;
-    const size_t N = (size_t)NP * K;
-    const int e0 = task * TE;
-    const int ne = min(TE, K - e0);
-
-    // Every global load and store of a task is unconditional (out-of-range
-    // lanes use a clamped, in-bounds index): with no branch around memory
-    // instructions the compiler counts outstanding operations exactly, so a
-    // wait for one load does not also wait for every store and prefetch
-    // issued before it.
-
-    // ---- stage the task's fields in LDS: all loads in flight (fields and the
-    // lane's element record), then the LDS writes ----
-    wave_sync();  // previous task's reads of U are done
-    const bool valid = el < ne;
-    const int e = e0 + (valid ? el : 0);
-    const T* ge = a.geo + (size_t)e * S::GEO;
-    T gm[9], gface[NFACE][4];  // metric terms dr_k/dx_d [k][d]; per face (nx, ny, nz, fscale/2)
-    T scale;
-    int fnb[NFACE], fcode[NFACE];
-    {
-        constexpr int NL = (TE * NP + 63) / 64;
-        const int cnt = ne * NP;
-        T v[6][NL];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const T* src = a.qin + c * N + (size_t)e0 * NP;
-#pragma unroll
-            for (int k = 0; k < NL; ++k) {
-                const int t = lane + 64 * k;
-                v[c][k] = src[t < cnt ? t : cnt - 1];
-            }
-        }
-        if constexpr (DIM == 3) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) gm[k] = ge[k];
-#pragma unroll
-            for (int f = 0; f < 4; ++f)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) gface[f][k] = ge[9 + 4 * f + k];
-        } else {  // [0..3] dr/dx dr/dy ds/dx ds/dy; edge f: [4+3f..6+3f] nx ny fscale/2
-#pragma unroll
-            for (int k = 0; k < 9; ++k) gm[k] = T(0);
-            gm[0] = ge[0];
-            gm[1] = ge[1];
-            gm[3] = ge[2];
-            gm[4] = ge[3];
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                gface[f][0] = ge[4 + 3 * f];
-                gface[f][1] = ge[5 + 3 * f];
-                gface[f][2] = T(0);
-                gface[f][3] = ge[6 + 3 * f];
-            }
-        }
-        // r_E = (curl H + LIFT f_E)/eps, r_H = (-curl E + LIFT f_H)/mu
-        const T sc = ge[S::GEO - 2 + half];
-        scale = valid ? sc : T(0);
-#pragma unroll
-        for (int f = 0; f < NFACE; ++f) {
-            fnb[f] = a.conn[(size_t)e * S::CONN + f];
-            fcode[f] = a.conn[(size_t)e * S::CONN + NFACE + f];
-        }
-        // rows ne..TE-1 of a partial task receive copies of the last valid
-        // value (only the rows of the padding elements read them); lanes past
-        // TE*NP write the never-read last column of row 0 (US - 1 >= KP), so
-        // the writes need no branch into which the compiler could sink loads
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int k = 0; k < NL; ++k) {
-                const int t = lane + 64 * k;
-                const int r = t / NP;
-                U[t < TE * NP ? (c * TE + r) * US + (t - r * NP) : c * TE * US + US - 1] = v[c][k];
-            }
-    }
-    wave_sync();
-
-    // half 0 forms E-output operands, half 1 H-output
-    const T sign = half ? T(-1) : T(1);
-    const T sscale = sign * scale;
-
-    acc_t R[3][G];
-#pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
-
-    // ---- surface -> LIFT ----
-    // Face by face; k-step t of face f covers nodes b = 4t + q, so the
-    // lanes of an element read the same neighbour element.  A lane forms
-    // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
-    // takes the other field's jumps from its partner row (lane ^ 8, one
-    // DPP row rotation).  Exterior traces are gathered from memory D k-steps
-    // ahead (a boundary face reads the element's own trace, weighted by the
-    // boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
-    // 309-311).  Only neighbours in another task are gathered, the others
-    // read the task's fields in LDS; except for FP64 tetrahedra at orders 4
-    // and 5, where every exterior trace is gathered (fewer selects; measured
-    // 1.5 % faster at order 4, while at order 3 and on triangles the
-    // remote-only form is 9-10 % faster in FP64).
-    constexpr bool kRemoteOnly = sizeof(T) == 4 || DIM == 2 || P <= 3;
-    const T* Ue = U + el * US;
-    const int cbase = half ? 3 : 0;  // own field's first component
-    auto face_remote = [&](int f) { return fnb[f] >= 0 && (unsigned)(fnb[f] - e0) >= (unsigned)ne; };
-    auto face_tab = [&](int f) { return fnb[f] >= 0 ? sPt + fcode[f] * NFP : sFm + f * NFP; };
-    // exterior trace of k-step s (own field's 3 components).  Every lane
-    // issues the load (no branch); with kRemoteOnly, lanes without a remote
-    // face all read the task's first value, one cache line, and discard it.
-    auto gather = [&](int s, T (&v)[3]) {
-        const int f = s / SF, b = 4 * (s % SF) + q;
-        const int idx = face_tab(f)[b < NFP ? b : NFP - 1];
-        size_t off;
-        if constexpr (kRemoteOnly)
-            off = face_remote(f) ? (size_t)fnb[f] * NP + idx : (size_t)e0 * NP;
-        else
-            off = (size_t)(fnb[f] >= 0 ? fnb[f] : e) * NP + idx;
-        const T* src = a.qin + (size_t)cbase * N + off;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = src[c * N];
-    };
-    constexpr int D = kGatherDepth < NFACE * SF ? kGatherDepth : NFACE * SF;
-    T pf[D][3];
-#pragma unroll
-    for (int d = 0; d < D; ++d) gather(d, pf[d]);
-#pragma unroll
-    for (int f = 0; f < NFACE; ++f) {
-        const int nb = fnb[f];
-        const bool interior = nb >= 0;
-        const bool remote = face_remote(f);
-        const int* tab = face_tab(f);
-        const T* Ub = U + (interior && !remote ? nb - e0 : el) * US + cbase * TE * US;
-        T cown;
-        {
-            T ce, ch;
-            bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
-            cown = interior ? T(-1) : (half ? ch : ce) - T(1);
-        }
-        const T n3[3] = {gface[f][0], gface[f][1], gface[f][2]};
-        const T fsc = gface[f][3] * scale;
-#pragma unroll
-        for (int t = 0; t < SF; ++t) {
-            const int s = f * SF + t;
-            const int b = 4 * t + q;
-            const int bb = b < NFP ? b : NFP - 1;
-            const int iM = sFm[f * NFP + bb];
-            T cur[3];
-            if constexpr (kRemoteOnly) {
-                // the empty asm keeps the LDS values register operands, so the
-                // compiler cannot fold the select into one load from a generic
-                // pointer here (which would undo the prefetch); it is not
-                // volatile, so it does not count as a memory side effect
-                const int ip = tab[bb];
-                T lv[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) lv[c] = Ub[c * TE * US + ip];
-                asm("" : "+v"(lv[0]), "+v"(lv[1]), "+v"(lv[2]));
-#pragma unroll
-                for (int c = 0; c < 3; ++c) cur[c] = remote ? pf[s % D][c] : lv[c];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) cur[c] = pf[s % D][c];
-            }
-            if (s + D < NFACE * SF) gather(s + D, pf[s % D]);
-            __builtin_amdgcn_sched_barrier(0);  // the prefetch stays D k-steps ahead of its use
-            T B3[3], A3[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                B3[c] = cur[c] + cown * Ue[(cbase + c) * TE * US + iM];
-                A3[c] = dpp_swap8(B3[c]);  // partner row's jump: the other field
-            }
-            const T nB = n3[0] * B3[0] + n3[1] * B3[1] + n3[2] * B3[2];
-            const T w = (b < NFP) ? fsc : T(0);  // padded face slots add nothing
-            // E out: n x dH + alpha (dE - n (n.dE));  H out: -n x dE + alpha (dH - n (n.dH))
-            T F[3];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const int y = (x + 1) % 3, z = (x + 2) % 3;
-                F[x] = (sign * (n3[y] * A3[z] - n3[z] * A3[y]) + a.alpha * (B3[x] - nB * n3[x])) * w;
-            }
-            const T* bl = sBl + (f * NFPP + b) * NPG + li;
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const T bv = bl[16 * g];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
-            }
-        }
-    }
-
-    // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
-    // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
-#pragma unroll 3
-    for (int ks = 0; ks < S::S; ++ks) {
-        const int j = q * S::S + ks;
-        T src[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
-        const T* bv = sBv + j * NDIR * NPG + li;
-#pragma unroll
-        for (int kd = 0; kd < NDIR; ++kd) {
-            const T* gk = gm + 3 * kd;
-            T w[3];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const int y = (x + 1) % 3, z = (x + 2) % 3;
-                w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
-            }
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const T bval = bv[kd * NPG + 16 * g];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
-            }
-        }
-    }
-
-    // The residual and base-state reads and the stage's stores are used once
-    // per stage: they stream non-temporally, so L2 keeps the fields that
-    // neighbouring tasks gather.
-    // ---- stage update: per x, the E_x and H_x right-hand sides go through
-    // the wave's LDS staging buffer and the update runs as a linear pass
-    // over the task's contiguous chunk (whole sectors, coalesced).  The
-    // residual / base state of the pair x+1 (linear index t over
-    // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
-    constexpr int NLIN = (2 * TE * NP + 63) / 64;
-    constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
-    constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
-    // Linear slot t (E_x | H_x halves) clamped to the last valid node of its
-    // half: lanes past the end repeat that slot's load, arithmetic and store
-    // (identical values to the same address).
-    auto clamp_slot = [&](int t) {
-        const int tc = t < 2 * TE * NP ? t : 2 * TE * NP - 1;
-        const int h = tc >= TE * NP;
-        const int u = tc - h * TE * NP;
-        return h * TE * NP + (u < ne * NP ? u : ne * NP - 1);
-    };
-    auto lin_index = [&](int x, int tc) {
-        const int h = tc >= TE * NP;
-        return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (tc - h * TE * NP);
-    };
-    T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
-    auto fetch = [&](int x, int slot) {
-#pragma unroll
-        for (int k = 0; k < NLIN; ++k) {
-            const size_t o = lin_index(x, clamp_slot(lane + 64 * k));
-            if constexpr (kNeedRes) rl[slot][k] = __builtin_nontemporal_load(a.res + o);
-            if constexpr (kNeedX) xl[slot][k] = __builtin_nontemporal_load(a.x + o);
-        }
-    };
-    fetch(0, 0);
-    bool bad = false;  // an unstable state entry stored by this lane (a.unstable set)
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-        const int cur = x & 1;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int i = 16 * g + li;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = MM::row(q, r);
-                if (i < NP) Sg[(row >> 3) * TE * NP + (row & 7) * NP + i] = R[x][g][r];
-            }
-        }
-        wave_sync();
-        if (x + 1 < 3) fetch(x + 1, cur ^ 1);
-#pragma unroll
-        for (int k = 0; k < NLIN; ++k) {
-            const int t = clamp_slot(lane + 64 * k);
-            const size_t o = lin_index(x, t);
-            const int h = t >= TE * NP;
-            const int u = t - h * TE * NP;
-            const int c = h ? 3 + x : x;
-            const int r_el = u / NP;
-            const T rhs = Sg[t];
-            const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
-            if constexpr (MODE == kModeMult) {
-                __builtin_nontemporal_store(rhs, a.out + o);
-            } else if constexpr (MODE == kModeLserk) {
-                const T rr = a.ca * rl[cur][k] + a.dt * rhs;
-                const T qn = qv + a.cb * rr;
-                __builtin_nontemporal_store(rr, a.res + o);
-                __builtin_nontemporal_store(qn, a.out + o);
-                bad = bad || unstable_entry(qn);
-            } else if constexpr (MODE == kModeRk1) {
-                __builtin_nontemporal_store(qv + a.ca * rhs, a.out + o);
-                __builtin_nontemporal_store(qv + a.cb * rhs, a.res + o);
-            } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                __builtin_nontemporal_store(xl[cur][k] + a.ca * rhs, a.out + o);
-                __builtin_nontemporal_store(rl[cur][k] + a.cb * rhs, a.res + o);
-            } else {
-                const T qn = rl[cur][k] + a.ca * rhs;
-                __builtin_nontemporal_store(qn, a.out + o);
-                bad = bad || unstable_entry(qn);
-            }
-        }
-        wave_sync();  // staging buffer free for the next x
-    }
-    if constexpr (MODE == kModeLserk || MODE == kModeRk4) report_unstable(a.unstable, bad);
-}
-
-template <int P, typename T, int MODE, int DIM>
-__global__ __launch_bounds__((MWaves<P, T, DIM>::W * 64)) void dg_stage_mfma(StageArgs<T> a)
-{
-    constexpr int TE = kTaskElems;
-    constexpr int WAVES = MWaves<P, T, DIM>::W;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const MfmaLds<P, T, DIM> L = mfma_load_ref<P, T, DIM>(a.mats, a.tabs, smem_raw);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int ntasks = (a.K + TE - 1) / TE;
-    // XCD-contiguous task ranges: block b runs on XCD b % 8.
-    const int nxcd = (gridDim.x >= 8 && gridDim.x % 8 == 0) ? 8 : 1;
-    const int xcd = blockIdx.x % nxcd;
-    const int bpx = gridDim.x / nxcd;
-    const int bin = blockIdx.x / nxcd;
-    const int per_xcd = (ntasks + nxcd - 1) / nxcd;
-    const int t_begin = xcd * per_xcd;
-    const int t_end = min(ntasks, t_begin + per_xcd);
-    for (int tix = t_begin + bin * WAVES + wave; tix < t_end; tix += bpx * WAVES)
-        mfma_task<P, T, MODE, DIM>(a, L, a.order ? a.order[tix] : tix);
-}
-
-// Batch variant: the running instances of order P, one (instance, task) work
-// item per wave-task, items [range[0], range[1]) of the list; stage `b.stage`
-// of LSERK4 with the instance's own dt, buffers and geometry.
-template <int P, typename T>
-__global__ __launch_bounds__((MWaves<P, T>::W * 64)) void dg_batch_stage_mfma(BatchArgs<T> b, const int* range)
-{
-    constexpr int WAVES = MWaves<P, T>::W;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const MfmaLds<P, T> L = mfma_load_ref<P, T>(b.mats_mfma[P], b.tabs[P], smem_raw);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int begin = range[0], end = range[1];
-    for (int it = begin + blockIdx.x * WAVES + wave; it < end; it += gridDim.x * WAVES) {
-        const BatchItem w = b.items[it];
-        const BatchInst<T> I = b.inst[w.inst];
-        if (I.state != kSlotRunning) continue;  // uniform across the wave
-        StageArgs<T> a{};
-        a.K = I.K;
-        const int src = I.cur ^ (b.stage & 1);
-        a.qin = I.q[src];
-        a.out = I.q[src ^ 1];
-        a.res = I.res;
-        a.ca = b.ca;
-        a.cb = b.cb;
-        a.dt = (T)I.dt_step;
-        a.alpha = (T)I.alpha;
-        a.geo = I.geo;
-        a.conn = I.conn;
-        a.unstable = b.stage == 4 ? &b.inst[w.inst].unstable : nullptr;  // the step's last stage
-        mfma_task<P, T, kModeLserk>(a, L, w.blk);
-    }
-}
-
-template <int P, typename T, int MODE, int DIM>
-void mfma_set_attr()
-{
-    if constexpr (!MWaves<P, T, DIM>::ok) return;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_stage_mfma<P, T, MODE, DIM>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)MWaves<P, T, DIM>::bytes);
-}
-
-// Sets the LDS attribute of every mode (set-up time, outside any capture)
-// and returns the resident blocks per CU (0: no variant for this order).
-template <int P, typename T, int DIM>
-int mfma_prepare()
-{
-    using W = MWaves<P, T, DIM>;
-    if constexpr (!W::ok) return 0;
-    mfma_set_attr<P, T, kModeMult, DIM>();
-    mfma_set_attr<P, T, kModeLserk, DIM>();
-    mfma_set_attr<P, T, kModeRk1, DIM>();
-    mfma_set_attr<P, T, kModeRk2, DIM>();
-    mfma_set_attr<P, T, kModeRk3, DIM>();
-    mfma_set_attr<P, T, kModeRk4, DIM>();
-    if constexpr (DIM == 3)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)W::bytes);
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_stage_mfma<P, T, kModeLserk, DIM>, W::W * 64,
-                                                     W::bytes) != hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
-}
-
-template <int P, typename T, int DIM>
-hipError_t mfma_launch(const StageArgs<T>& a, int mode, int grid, hipStream_t s)
-{
-    using W = MWaves<P, T, DIM>;
-    constexpr int threads = W::W * 64;
-    constexpr size_t bytes = W::bytes;
-    if constexpr (!W::ok) return hipErrorInvalidConfiguration;
-    switch (mode) {
-        case kModeMult: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeMult, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeLserk: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeLserk, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk1: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk1, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk2: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk2, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk3: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk3, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        case kModeRk4: hipLaunchKernelGGL((dg_stage_mfma<P, T, kModeRk4, DIM>), dim3(grid), dim3(threads), bytes, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-template <int P, typename T>
-hipError_t batch_mfma_launch(const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
-{
-    constexpr int threads = MWaves<P, T>::W * 64;
-    constexpr size_t bytes = MWaves<P, T>::bytes;
-    if constexpr (!MWaves<P, T>::ok) {
-        return hipErrorInvalidConfiguration;
-    } else {
-        hipLaunchKernelGGL((dg_batch_stage_mfma<P, T>), dim3(grid), dim3(threads), bytes, s, b, range);
-        return hipGetLastError();
-    }
-}
-
-}  // namespace
-
-template <typename T>
-bool mfma_available(int order)
-{
-    switch (order) {
-        case 1: return MWaves<1, T>::ok;
-        case 2: return MWaves<2, T>::ok;
-        case 3: return MWaves<3, T>::ok;
-        case 4: return MWaves<4, T>::ok;
-        case 5: return MWaves<5, T>::ok;
-        default: return false;
-    }
+    return cfg == 1 ? mfma_info_cfg<1, T>(order, dim) : mfma_info_cfg<0, T>(order, dim);
 }
 
 template <typename T>
-hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
+bool mfma_available(int order, int cfg)
 {
-    switch (order) {
-        case 1: return batch_mfma_launch<1, T>(b, range, grid, s);
-        case 2: return batch_mfma_launch<2, T>(b, range, grid, s);
-        case 3: return batch_mfma_launch<3, T>(b, range, grid, s);
-        case 4: return batch_mfma_launch<4, T>(b, range, grid, s);
-        case 5: return batch_mfma_launch<5, T>(b, range, grid, s);
-        default: return hipErrorInvalidValue;
-    }
+    return mfma_info<T>(order, 3, cfg).ok;
 }
 
 template <typename T>
-hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s)
+hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s, int cfg)
 {
-    switch (order) {
-        case 1: return mfma_launch<1, T, 3>(a, mode, grid, s);
-        case 2: return mfma_launch<2, T, 3>(a, mode, grid, s);
-        case 3: return mfma_launch<3, T, 3>(a, mode, grid, s);
-        case 4: return mfma_launch<4, T, 3>(a, mode, grid, s);
-        case 5: return mfma_launch<5, T, 3>(a, mode, grid, s);
-        default: return hipErrorInvalidValue;
-    }
+    return cfg == 1 ? mfma_launch_cfg<1, T>(order, 3, a, mode, grid, s) : mfma_launch_cfg<0, T>(order, 3, a, mode, grid, s);
 }
 
 template <typename T>
-int mfma_blocks_per_cu(int order)
+hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s, int cfg)
 {
-    switch (order) {
-        case 1: return mfma_prepare<1, T, 3>();
-        case 2: return mfma_prepare<2, T, 3>();
-        case 3: return mfma_prepare<3, T, 3>();
-        case 4: return mfma_prepare<4, T, 3>();
-        case 5: return mfma_prepare<5, T, 3>();
-        default: return 1;
-    }
+    return cfg == 1 ? mfma_launch_cfg<1, T>(order, 2, a, mode, grid, s) : mfma_launch_cfg<0, T>(order, 2, a, mode, grid, s);
 }
 
 template <typename T>
-int mfma_waves_per_block(int order)
+hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s,
+                                   int cfg)
 {
-    switch (order) {
-        case 1: return MWaves<1, T>::W;
-        case 2: return MWaves<2, T>::W;
-        case 3: return MWaves<3, T>::W;
-        case 4: return MWaves<4, T>::W;
-        case 5: return MWaves<5, T>::W;
-        default: return 1;
-    }
+    return cfg == 1 ? batch_mfma_launch_cfg<1, T>(order, b, range, grid, s)
+                    : batch_mfma_launch_cfg<0, T>(order, b, range, grid, s);
 }
 
 template <typename T>
-hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s)
+int mfma_blocks_per_cu(int order, int cfg)
 {
-    switch (order) {
-        case 1: return mfma_launch<1, T, 2>(a, mode, grid, s);
-        case 2: return mfma_launch<2, T, 2>(a, mode, grid, s);
-        case 3: return mfma_launch<3, T, 2>(a, mode, grid, s);
-        case 4: return mfma_launch<4, T, 2>(a, mode, grid, s);
-        case 5: return mfma_launch<5, T, 2>(a, mode, grid, s);
-        default: return hipErrorInvalidValue;
-    }
+    return cfg == 1 ? mfma_prepare_cfg<1, T>(order, 3) : mfma_prepare_cfg<0, T>(order, 3);
 }
 
 template <typename T>
-int tri_mfma_blocks_per_cu(int order)
+int tri_mfma_blocks_per_cu(int order, int cfg)
 {
-    switch (order) {
-        case 1: return mfma_prepare<1, T, 2>();
-        case 2: return mfma_prepare<2, T, 2>();
-        case 3: return mfma_prepare<3, T, 2>();
-        case 4: return mfma_prepare<4, T, 2>();
-        case 5: return mfma_prepare<5, T, 2>();
-        default: return 0;
-    }
+    return cfg == 1 ? mfma_prepare_cfg<1, T>(order, 2) : mfma_prepare_cfg<0, T>(order, 2);
 }
 
 template <typename T>
-int tri_mfma_waves_per_block(int order)
+int mfma_waves_per_block(int order, int cfg)
 {
-    switch (order) {
-        case 1: return MWaves<1, T, 2>::W;
-        case 2: return MWaves<2, T, 2>::W;
-        case 3: return MWaves<3, T, 2>::W;
-        case 4: return MWaves<4, T, 2>::W;
-        case 5: return MWaves<5, T, 2>::W;
-        default: return 1;
-    }
+    return mfma_info<T>(order, 3, cfg).waves;
 }
 
-template hipError_t launch_tri_stage_mfma<float>(int, const StageArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_tri_stage_mfma<double>(int, const StageArgs<double>&, int, int, hipStream_t);
-template int tri_mfma_blocks_per_cu<float>(int);
-template int tri_mfma_blocks_per_cu<double>(int);
-template int tri_mfma_waves_per_block<float>(int);
-template int tri_mfma_waves_per_block<double>(int);
-template hipError_t launch_stage_mfma<float>(int, const StageArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_stage_mfma<double>(int, const StageArgs<double>&, int, int, hipStream_t);
-template int mfma_blocks_per_cu<float>(int);
-template int mfma_blocks_per_cu<double>(int);
-template int mfma_waves_per_block<float>(int);
-template int mfma_waves_per_block<double>(int);
-template bool mfma_available<float>(int);
-template bool mfma_available<double>(int);
-template hipError_t launch_batch_stage_mfma<float>(int, const BatchArgs<float>&, const int*, int, hipStream_t);
-template hipError_t launch_batch_stage_mfma<double>(int, const BatchArgs<double>&, const int*, int, hipStream_t);
+template <typename T>
+int tri_mfma_waves_per_block(int order, int cfg)
+{
+    return mfma_info<T>(order, 2, cfg).waves;
+}
+
+#define DGTD_DISPATCH(T)                                                                                   \
+    template MfmaInfo mfma_info<T>(int, int, int);                                                        \
+    template bool mfma_available<T>(int, int);                                                            \
+    template hipError_t launch_stage_mfma<T>(int, const StageArgs<T>&, int, int, hipStream_t, int);       \
+    template hipError_t launch_tri_stage_mfma<T>(int, const StageArgs<T>&, int, int, hipStream_t, int);   \
+    template hipError_t launch_batch_stage_mfma<T>(int, const BatchArgs<T>&, const int*, int, hipStream_t, \
+                                                   int);                                                  \
+    template int mfma_blocks_per_cu<T>(int, int);                                                         \
+    template int tri_mfma_blocks_per_cu<T>(int, int);                                                     \
+    template int mfma_waves_per_block<T>(int, int);                                                       \
+    template int tri_mfma_waves_per_block<T>(int, int);
+DGTD_DISPATCH(float)
+DGTD_DISPATCH(double)
 
 }  // namespace dgtd

@@ -9,9 +9,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "fields.hpp"
@@ -63,6 +66,51 @@ void* upload(const std::vector<T>& h)
 
 }  // namespace
 
+namespace {
+// Process-wide autotuning cache: the chosen candidate per (element shape,
+// order, precision, stage mode, size class = floor(log2 K)).
+struct TuneKey {
+    int dim, order, precision, mode, size_class;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(dim, order, precision, mode, size_class) < std::tie(o.dim, o.order, o.precision, o.mode, o.size_class);
+    }
+};
+// The fastest candidate overall, the fastest matrix-core candidate (-1:
+// none) and every candidate's time in microseconds (0: did not fit).
+struct TuneChoice {
+    int best = 0, best_mfma = -1;
+    double us[8] = {};
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice> g_tune;
+
+int size_class(int64_t K)
+{
+    int c = 0;
+    while (K > 1) {
+        K >>= 1;
+        ++c;
+    }
+    return c;
+}
+
+bool tune_cache_get(int dim, int order, int precision, int mode, int64_t K, TuneChoice& choice)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tune.find(TuneKey{dim, order, precision, mode, size_class(K)});
+    if (it == g_tune.end()) return false;
+    choice = it->second;
+    return true;
+}
+
+void tune_cache_put(int dim, int order, int precision, int mode, int64_t K, TuneChoice choice)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[TuneKey{dim, order, precision, mode, size_class(K)}] = choice;
+}
+}  // namespace
+
 struct dgtd_op {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -86,7 +134,12 @@ struct dgtd_op {
     void* d_mats_mfma = nullptr;
     int* d_order = nullptr;
     int variant = DGTD_KERNEL_VALU;
-    int grid_mfma = 0;
+    int grid_mfma = 0;                            // grid of the configuration in use
+    int mfma_cfg = 0;                             // matrix-core tuning configuration in use
+    bool morton = true;                           // matrix-core tasks in Morton order (else element order)
+    int grid_cfg[dgtd::kMfmaConfigs] = {};        // grid per configuration, 0 = does not fit
+    // autotuning record: candidates timed at creation (or taken from the cache)
+    dgtd_tuning tuning{};
     // assembled operator (DGTD_KERNEL_CSR)
     int64_t* d_rowptr = nullptr;
     uint16_t* d_col = nullptr;
@@ -182,9 +235,9 @@ struct dgtd_op {
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
             a.mats = static_cast<const T*>(d_mats_mfma);
-            a.order = d_order;
-            if (dim == 2) return dgtd::launch_tri_stage_mfma<T>(T2.p, a, mode, grid_mfma, s);
-            return dgtd::launch_stage_mfma<T>(R.p, a, mode, grid_mfma, s);
+            a.order = morton ? d_order : nullptr;
+            if (dim == 2) return dgtd::launch_tri_stage_mfma<T>(T2.p, a, mode, grid_cfg[mfma_cfg], s, mfma_cfg);
+            return dgtd::launch_stage_mfma<T>(R.p, a, mode, grid_cfg[mfma_cfg], s, mfma_cfg);
         }
         if (dim == 2) return dgtd::launch_tri_stage<T>(T2.p, a0, mode, grid, s);
         return dgtd::launch_stage<T>(R.p, a0, mode, grid, s);
@@ -197,15 +250,62 @@ struct dgtd_op {
         hip_check(launch_on<T>(a, mode, variant, stream), "stage kernel launch");
     }
 
-    // Times each available variant on the zeroed state buffers and keeps the
-    // fastest: the fused LSERK stage for LSERK4 operators (the launch that
-    // steps run; with ca = cb = dt = 0 the buffers stay zero), the operator
-    // application otherwise.
+    // Candidate stage kernels of the autotuner: the per-node variant, then
+    // the matrix-core variant in each tuning configuration that fits (waves
+    // per block, gather depth, residual prefetch), with its tasks in Morton
+    // order or in element order.
+    static constexpr int kCandidates = 1 + 2 * dgtd::kMfmaConfigs;
+    static_assert(kCandidates <= (int)(sizeof(dgtd_tuning::candidate_us) / sizeof(double)),
+                  "dgtd_tuning holds every candidate's time");
+    bool candidate(int c, int& var, int& cfg, bool& mort) const
+    {
+        var = c == 0 ? DGTD_KERNEL_VALU : DGTD_KERNEL_MFMA;
+        cfg = c == 0 ? 0 : (c - 1) / 2;
+        mort = c == 0 || (c - 1) % 2 == 0;
+        return c == 0 || (grid_cfg[cfg] > 0 && (mort || d_order));
+    }
+
+    void apply_candidate(int c)
+    {
+        int var, cfg;
+        bool mort;
+        candidate(c, var, cfg, mort);
+        variant = var;
+        if (var == DGTD_KERNEL_MFMA) {
+            mfma_cfg = cfg;
+            morton = mort;
+            grid_mfma = grid_cfg[cfg];
+        }
+        const dgtd::MfmaInfo mi = precision == 64 ? dgtd::mfma_info<double>(dim == 2 ? T2.p : R.p, dim, mfma_cfg)
+                                                  : dgtd::mfma_info<float>(dim == 2 ? T2.p : R.p, dim, mfma_cfg);
+        tuning.variant = variant;
+        tuning.config = mfma_cfg;
+        tuning.waves_per_block = variant == DGTD_KERNEL_MFMA ? mi.waves : 4;
+        tuning.gather_depth = variant == DGTD_KERNEL_MFMA ? mi.gather_depth : 0;
+        tuning.early_residual = variant == DGTD_KERNEL_MFMA && mi.early_residual;
+        tuning.task_order = variant == DGTD_KERNEL_MFMA && morton ? 1 : 0;
+    }
+
+    // Times every candidate on the zeroed state buffers (the fused LSERK
+    // stage for LSERK4 operators -- the launch that steps run; with
+    // ca = cb = dt = 0 the buffers stay zero -- the operator application
+    // otherwise) and keeps the fastest.  The choice is cached per (element
+    // shape, order, precision, stage mode, size class) for the process, so
+    // later operators of the same kind skip the timing.
     template <typename T>
     void autotune()
     {
         dgtd::TraceRange tr("dgtd autotune");
-        if (grid_mfma <= 0) { variant = DGTD_KERNEL_VALU; return; }
+        const int order = dim == 2 ? T2.p : R.p;
+        const int mode_key = integrator == DGTD_LSERK4 ? dgtd::kModeLserk : dgtd::kModeMult;
+        TuneChoice cached;
+        if (tune_cache_get(dim, order, precision, mode_key, K, cached)) {
+            apply_candidate(cached.best);
+            tuning.from_cache = 1;
+            tuning.ncandidates = kCandidates;
+            for (int c = 0; c < kCandidates; ++c) tuning.candidate_us[c] = cached.us[c];
+            return;
+        }
         auto a = base_args<T>();
         a.qin = static_cast<const T*>(d_buf[0]);
         a.out = static_cast<T*>(d_buf[1]);
@@ -217,19 +317,47 @@ struct dgtd_op {
             a.cb = T(0);
             a.dt = T(0);
         }
-        float best = 1e30f;
-        int best_var = DGTD_KERNEL_VALU;
-        for (int var : {DGTD_KERNEL_VALU, DGTD_KERNEL_MFMA}) {
-            hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");
+        // Two rounds over the candidates: three launches each, then enough
+        // launches for about 1 ms of kernel time (3..64); a candidate's time
+        // is the lower of its two per-launch means (interference only adds).
+        auto time_launches = [&](int var, int reps) {
+            hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");  // warm-up
             hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
-            for (int r = 0; r < 3; ++r) hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");
+            for (int r = 0; r < reps; ++r) hip_check(launch_on<T>(a, mode, var, stream), "autotune launch");
             hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
             hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
             float ms = 0.f;
             hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
-            if (ms < best) { best = ms; best_var = var; }
+            return 1000.0 * ms / reps;
+        };
+        for (int c = 0; c < kCandidates; ++c) tuning.candidate_us[c] = 0.0;
+        for (int round = 0; round < 2; ++round)
+            for (int c = 0; c < kCandidates; ++c) {
+                int var, cfg;
+                bool mort;
+                if (!candidate(c, var, cfg, mort)) continue;
+                apply_candidate(c);
+                double& us = tuning.candidate_us[c];
+                const int reps = round == 0 ? 3 : std::max(3, std::min(64, (int)std::ceil(1000.0 / std::max(us, 1.0))));
+                const double t = time_launches(var, reps);
+                us = round == 0 ? t : std::min(us, t);
+            }
+        double best = 1e30, best_m = 1e30;
+        int best_c = 0, best_mc = -1;
+        for (int c = 0; c < kCandidates; ++c) {
+            int var, cfg;
+            bool mort;
+            if (!candidate(c, var, cfg, mort)) continue;
+            const double us = tuning.candidate_us[c];
+            if (us < best) { best = us; best_c = c; }
+            if (var == DGTD_KERNEL_MFMA && us < best_m) { best_m = us; best_mc = c; }
         }
-        variant = best_var;
+        apply_candidate(best_c);
+        tuning.from_cache = 0;
+        tuning.ncandidates = kCandidates;
+        TuneChoice made{best_c, best_mc, {}};
+        for (int c = 0; c < kCandidates; ++c) made.us[c] = tuning.candidate_us[c];
+        tune_cache_put(dim, order, precision, mode_key, K, made);
     }
 
     // Adds the TF/SF right-hand side at time ts: c_out * S to out and
@@ -477,6 +605,34 @@ void upload_csr(dgtd_op* op, const dgtd::Csr& A)
     op->d_val = op->precision == 64 ? upload(A.val) : upload(dgtd::cast_vec<float>(A.val));
 }
 
+// Persistent grid of the matrix-core stage: enough blocks for the tasks, at
+// most the resident ones, in whole XCD groups (the kernel splits the tasks
+// per XCD).
+int mfma_grid(int64_t K, int waves, int resident)
+{
+    const int64_t ntasks = (K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
+    int g = (int)std::min<int64_t>((ntasks + waves - 1) / waves, resident);
+    if (g >= 8) g -= g % 8;
+    return std::max(1, g);
+}
+
+// The stage variant: assembled (CSR), autotuned, or as requested (matrix
+// cores in configuration 0 with Morton-ordered tasks).
+template <typename CsrSetup>
+void select_variant(dgtd_op* op, int kernel, CsrSetup csr_setup)
+{
+    if (kernel == DGTD_KERNEL_CSR) {
+        csr_setup();
+        op->variant = DGTD_KERNEL_CSR;
+        op->tuning.variant = DGTD_KERNEL_CSR;
+    } else if (kernel == DGTD_KERNEL_AUTO) {
+        if (op->precision == 64) op->autotune<double>();
+        else op->autotune<float>();
+    } else {
+        op->apply_candidate(kernel == DGTD_KERNEL_MFMA ? 1 : 0);
+    }
+}
+
 void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
 {
     op->dim = 2;
@@ -579,31 +735,25 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     op->variant = DGTD_KERNEL_VALU;
     if (op->precision == 64) dgtd::tri_prepare<double>(R.p);
     else dgtd::tri_prepare<float>(R.p);
-    // matrix-core variant: 8-triangle wave tasks in Morton order
-    const int mbpc = op->precision == 64 ? dgtd::tri_mfma_blocks_per_cu<double>(R.p)
-                                         : dgtd::tri_mfma_blocks_per_cu<float>(R.p);
-    if (mbpc > 0) {
-        const int mwaves = op->precision == 64 ? dgtd::tri_mfma_waves_per_block<double>(R.p)
-                                               : dgtd::tri_mfma_waves_per_block<float>(R.p);
+    // matrix-core variant: 8-triangle wave tasks (Morton or element order), per configuration
+    bool any_mfma = false;
+    for (int c = 0; c < dgtd::kMfmaConfigs; ++c) {
+        const int mbpc = op->precision == 64 ? dgtd::tri_mfma_blocks_per_cu<double>(R.p, c)
+                                             : dgtd::tri_mfma_blocks_per_cu<float>(R.p, c);
+        if (mbpc <= 0) continue;
+        const int mwaves = op->precision == 64 ? dgtd::tri_mfma_waves_per_block<double>(R.p, c)
+                                               : dgtd::tri_mfma_waves_per_block<float>(R.p, c);
+        op->grid_cfg[c] = mfma_grid(op->K, mwaves, prop.multiProcessorCount * mbpc);
+        any_mfma = true;
+    }
+    if (any_mfma) {
         op->d_mats_mfma = op->precision == 64 ? upload(dgtd::pack_mats_mfma<double>(R))
                                               : upload(dgtd::pack_mats_mfma<float>(R));
         op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
-        const int ntasks = (op->K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
-        int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
-        if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)
-        op->grid_mfma = std::max(1, g);
     }
-    if (kernel == DGTD_KERNEL_MFMA && op->grid_mfma <= 0)
+    if (kernel == DGTD_KERNEL_MFMA && op->grid_cfg[0] <= 0)
         throw std::runtime_error("the MFMA kernel does not fit this order and precision");
-    if (kernel == DGTD_KERNEL_CSR) {
-        upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha));
-        op->variant = DGTD_KERNEL_CSR;
-    } else if (kernel == DGTD_KERNEL_AUTO) {
-        if (op->precision == 64) op->autotune<double>();
-        else op->autotune<float>();
-    } else {
-        op->variant = kernel;
-    }
+    select_variant(op, kernel, [&] { upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha)); });
 }
 
 dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
@@ -685,21 +835,17 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         tabs.insert(tabs.end(), op->R.face_perm.begin(), op->R.face_perm.end());
         op->d_tabs = static_cast<int*>(upload(tabs));
         op->d_conn = static_cast<int*>(upload(md.conn));
-        int bpc, mbpc, mwaves;
+        int bpc;
         if (op->precision == 64) {
             op->d_geo = upload(md.geo);
             op->d_mats = upload(dgtd::pack_mats<double>(op->R));
             op->d_mats_mfma = upload(dgtd::pack_mats_mfma<double>(op->R));
             bpc = dgtd::stage_blocks_per_cu<double>(op->R.p);
-            mbpc = dgtd::mfma_blocks_per_cu<double>(op->R.p);
-            mwaves = dgtd::mfma_waves_per_block<double>(op->R.p);
         } else {
             op->d_geo = upload(dgtd::cast_vec<float>(md.geo));
             op->d_mats = upload(dgtd::pack_mats<float>(op->R));
             op->d_mats_mfma = upload(dgtd::pack_mats_mfma<float>(op->R));
             bpc = dgtd::stage_blocks_per_cu<float>(op->R.p);
-            mbpc = dgtd::mfma_blocks_per_cu<float>(op->R.p);
-            mwaves = dgtd::mfma_waves_per_block<float>(op->R.p);
         }
         op->d_M = static_cast<double*>(upload(op->R.M));
         op->d_jac = static_cast<double*>(upload(md.jac));
@@ -757,24 +903,21 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         const int epb = dgtd::elements_per_block(op->R.p);
         op->nblk = (op->K + epb - 1) / epb;
         op->grid = std::max(1, std::min(op->nblk, prop.multiProcessorCount * bpc));
-        if (mbpc > 0) {
-            const int ntasks = (op->K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
-            op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
-            int g = std::min((ntasks + mwaves - 1) / mwaves, prop.multiProcessorCount * mbpc);
-            if (g >= 8) g -= g % 8;  // whole XCD groups (the kernel splits tasks per XCD)
-            op->grid_mfma = std::max(1, g);
+        bool any_mfma = false;
+        for (int c = 0; c < dgtd::kMfmaConfigs; ++c) {
+            const int mbpc = op->precision == 64 ? dgtd::mfma_blocks_per_cu<double>(op->R.p, c)
+                                                 : dgtd::mfma_blocks_per_cu<float>(op->R.p, c);
+            if (mbpc <= 0) continue;
+            const int mwaves = op->precision == 64 ? dgtd::mfma_waves_per_block<double>(op->R.p, c)
+                                                   : dgtd::mfma_waves_per_block<float>(op->R.p, c);
+            op->grid_cfg[c] = mfma_grid(op->K, mwaves, prop.multiProcessorCount * mbpc);
+            any_mfma = true;
         }
-        if (opts->kernel == DGTD_KERNEL_MFMA && op->grid_mfma <= 0)
+        if (any_mfma) op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
+        if (opts->kernel == DGTD_KERNEL_MFMA && op->grid_cfg[0] <= 0)
             throw std::runtime_error("the MFMA kernel does not fit this order and precision");
-        if (opts->kernel == DGTD_KERNEL_CSR) {
-            upload_csr(op, dgtd::assemble_global_tet(md, op->R, mesh->sigma, op->alpha));
-            op->variant = DGTD_KERNEL_CSR;
-        } else if (opts->kernel == DGTD_KERNEL_AUTO) {
-            if (op->precision == 64) op->autotune<double>();
-            else op->autotune<float>();
-        } else {
-            op->variant = opts->kernel;
-        }
+        select_variant(op, opts->kernel,
+                       [&] { upload_csr(op, dgtd::assemble_global_tet(md, op->R, mesh->sigma, op->alpha)); });
     } catch (const HipError& e) {
         delete op;
         return fail(DGTD_E_HIP, e.what());
@@ -1171,6 +1314,86 @@ int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches)
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     if (avg_us) *avg_us = op->last_us;
     if (launches) *launches = op->last_launches;
+    return DGTD_OK;
+}
+
+}  // extern "C"
+
+namespace dgtd {
+// The fastest matrix-core configuration of the LSERK stage for tetrahedra of
+// this order and precision at about K elements: from the process cache, else
+// by tuning an operator on a PEC cube of about K tetrahedra (n^3 x 6, n in
+// 4..32), which fills the cache.  Used by the batch engine, whose instances
+// cannot be timed in place; it keeps its own rule for the per-node or
+// matrix-core choice (that choice changes the arithmetic, the configuration
+// does not), so an instance's result is the same alone and in any mix.
+// Returns false when no matrix-core configuration was timed.
+bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg)
+{
+    TuneChoice choice;
+    if (!tune_cache_get(3, order, precision, kModeLserk, K, choice)) {
+        int n = (int)std::lround(std::cbrt((double)K / 6.0));
+        n = std::min(32, std::max(4, n));
+        const int8_t pec[6] = {DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC};
+        const size_t nk = 6 * (size_t)n * n * n;
+        std::vector<double> vx(3 * (size_t)(n + 1) * (n + 1) * (n + 1));
+        std::vector<int32_t> EToV(4 * nk);
+        std::vector<int8_t> fbc(4 * nk);
+        if (dgtd_cube_mesh(n, 1.0, pec, vx.data(), EToV.data(), fbc.data()) != DGTD_OK) return false;
+        dgtd_mesh_desc m{};
+        m.K = (int32_t)nk;
+        m.order = order;
+        m.vx = vx.data();
+        m.nv = (int32_t)(vx.size() / 3);
+        m.EToV = EToV.data();
+        m.face_bc = fbc.data();
+        dgtd_opts o{};
+        o.alpha = 1.0;
+        o.precision = precision;
+        o.integrator = DGTD_LSERK4;
+        o.kernel = DGTD_KERNEL_AUTO;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        o.device = dev;
+        dgtd_op* op = nullptr;
+        if (dgtd_op_create(&m, &o, &op) != DGTD_OK) return false;
+        const bool got = tune_cache_get(3, order, precision, kModeLserk, (int64_t)nk, choice);
+        dgtd_op_destroy(op);
+        if (!got) return false;
+        tune_cache_put(3, order, precision, kModeLserk, K, choice);
+    }
+    if (choice.best_mfma < 1) return false;
+    *cfg = (choice.best_mfma - 1) / 2;
+    return true;
+}
+}  // namespace dgtd
+
+extern "C" {
+
+int dgtd_op_tuning(const dgtd_op* op, dgtd_tuning* out)
+{
+    if (!op || !out) return fail(DGTD_E_ARG, "NULL argument");
+    *out = op->tuning;
+    return DGTD_OK;
+}
+
+int dgtd_op_set_candidate(dgtd_op* op, int32_t candidate)
+{
+    if (!op) return fail(DGTD_E_ARG, "NULL argument");
+    if (op->variant == DGTD_KERNEL_CSR) return fail(DGTD_E_ARG, "assembled operator: no stage candidates");
+    if (candidate < 0 || candidate >= dgtd_op::kCandidates) return fail(DGTD_E_ARG, "candidate out of range");
+    int var, cfg;
+    bool mort;
+    if (!op->candidate(candidate, var, cfg, mort)) return fail(DGTD_E_ARG, "candidate does not fit this operator");
+    try {
+        hip_check(hipSetDevice(op->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(op->stream), "hipStreamSynchronize");
+        op->apply_candidate(candidate);
+        op->tuning.from_cache = 0;
+        op->drop_graphs();  // captured with the previous kernel
+    } catch (const std::exception& ex) {
+        return fail(DGTD_E_HIP, ex.what());
+    }
     return DGTD_OK;
 }
 

@@ -85,25 +85,46 @@ template <typename T>
 void tri_prepare(int order);
 size_t tri_smem_bytes(int order, int precision);
 
-// MFMA variant (dg_mfma.hip): wave-tasks of kMfmaTaskElems elements,
-// waves_per_block waves.
+// MFMA variant (mfma_stage.hpp, dg_mfma*.hip): wave-tasks of kMfmaTaskElems
+// elements, waves_per_block waves, in one of kMfmaConfigs tuning
+// configurations (waves per block, gather depth, residual prefetch).
 constexpr int kMfmaTaskElems = 8;
+constexpr int kMfmaConfigs = 2;
+struct MfmaInfo {
+    bool ok;              // the configuration's LDS tiles fit
+    int waves;            // waves per block
+    int gather_depth;     // flux k-steps of exterior traces in flight
+    bool early_residual;  // first residual pass read before the volume products
+};
+template <int CFG, typename T>
+MfmaInfo mfma_info_cfg(int order, int dim);
+template <int CFG, typename T>
+int mfma_prepare_cfg(int order, int dim);
+template <int CFG, typename T>
+hipError_t mfma_launch_cfg(int order, int dim, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+template <typename T>
+struct BatchArgs;
+template <int CFG, typename T>
+hipError_t batch_mfma_launch_cfg(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s);
+
+template <typename T>
+MfmaInfo mfma_info(int order, int dim, int cfg);
 // mats layout: Bv [KP][3*NPG] then Bl [4*Nfp][NPG] (see pack in dgtd_api.cpp).
 template <typename T>
-hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+hipError_t launch_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s, int cfg);
 // Sets LDS limits; returns resident blocks per CU, 0 if the variant does not fit.
 template <typename T>
-int mfma_blocks_per_cu(int order);
+int mfma_blocks_per_cu(int order, int cfg);
 template <typename T>
-int mfma_waves_per_block(int order);
+int mfma_waves_per_block(int order, int cfg);
 // The same matrix-core stage on triangles (the 2D operator; records as in
 // tri.hpp, mats Bv [KP][2*NPG] then Bl [3*NFPP][NPG]).
 template <typename T>
-hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
+hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s, int cfg);
 template <typename T>
-int tri_mfma_blocks_per_cu(int order);
+int tri_mfma_blocks_per_cu(int order, int cfg);
 template <typename T>
-int tri_mfma_waves_per_block(int order);
+int tri_mfma_waves_per_block(int order, int cfg);
 
 // ---------------------------------------------------------------------------
 // Multi-instance batch (dg_batch.hip): many meshes of different size and order
@@ -149,11 +170,12 @@ hipError_t launch_batch_stage(const BatchArgs<T>& b, int grid, size_t smem, hipS
 // Matrix-core batch stage for the instances of one order: items (instance,
 // 8-element task) in [range[0], range[1]) of b.items (range in device memory).
 template <typename T>
-hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s);
-// Whether the matrix-core variant exists for this order and precision (its
-// LDS tiles fit; FP64 at order 5 does not).
+hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s,
+                                   int cfg);
+// Whether the matrix-core variant exists for this order, precision and
+// configuration (its LDS tiles fit).
 template <typename T>
-bool mfma_available(int order);
+bool mfma_available(int order, int cfg);
 template <typename T>
 hipError_t launch_batch_advance(BatchInst<T>* inst, int nslots, hipStream_t s);
 // LDS bytes of the batch kernel when orders up to max_order are present,

@@ -15,6 +15,8 @@ from . import _abi
 from ._abi import check, dptr, i32ptr, i8ptr, lib
 
 INTEGRATORS = {"rk4": _abi.DGTD_RK4, "lserk4": _abi.DGTD_LSERK4}
+# the autotuner's candidates, numbered as in dgtd_tuning
+CANDIDATES = ["valu"] + [f"mfma_c{c}_{o}" for c in range(2) for o in ("morton", "element")]
 KERNELS = {"auto": _abi.DGTD_KERNEL_AUTO, "valu": _abi.DGTD_KERNEL_VALU, "mfma": _abi.DGTD_KERNEL_MFMA,
            "csr": _abi.DGTD_KERNEL_CSR}
 
@@ -287,6 +289,29 @@ class HesthavenEvolution:
         v = ctypes.c_int32()
         check(lib().dgtd_op_kernel_variant(self._h, ctypes.byref(v)))
         return {v_: k for k, v_ in KERNELS.items()}[v.value]
+
+    def tuning(self):
+        """The autotuner's choice (dgtd_op_tuning): variant, matrix-core
+        configuration, waves per block, gather depth, residual prefetch, task
+        order, whether it came from the process cache, and the timed
+        candidates in microseconds per launch."""
+        t = _abi.Tuning()
+        check(lib().dgtd_op_tuning(self._h, ctypes.byref(t)))
+        names = CANDIDATES
+        return {"variant": {v_: k for k, v_ in KERNELS.items()}[t.variant], "config": t.config,
+                "waves_per_block": t.waves_per_block, "gather_depth": t.gather_depth,
+                "early_residual": bool(t.early_residual), "task_order": "morton" if t.task_order else "element",
+                "from_cache": bool(t.from_cache),
+                "candidates_us": {names[i]: round(t.candidate_us[i], 2) for i in range(min(t.ncandidates, 5))
+                                  if t.candidate_us[i] > 0}}
+
+    def set_candidate(self, candidate):
+        """Pin the stage kernel to one autotuner candidate (dgtd_op_set_candidate):
+        0 per-node, 1 + 2c + o matrix-core configuration c, Morton (o = 0) or
+        element (o = 1) task order; the names of tuning()['candidates_us']."""
+        if isinstance(candidate, str):
+            candidate = CANDIDATES.index(candidate)
+        check(lib().dgtd_op_set_candidate(self._h, int(candidate)))
 
     def work_per_element(self):
         b, f = ctypes.c_double(), ctypes.c_double()

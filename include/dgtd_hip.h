@@ -248,6 +248,41 @@ int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches)
 /* Stage-kernel variant in use (DGTD_KERNEL_VALU, _MFMA or _CSR). */
 int dgtd_op_kernel_variant(const dgtd_op* op, int32_t* variant);
 
+/* The stage kernel the autotuner chose (no counterpart in the reference).
+ * With DGTD_KERNEL_AUTO every candidate is timed at create on the zeroed
+ * state (the fused LSERK stage for LSERK4 operators, Mult otherwise; two
+ * rounds, three launches then about 1 ms of launches each, the lower mean
+ * kept) and the fastest kept; the choice is
+ * cached for the process per (element shape, order, precision, stage mode,
+ * floor(log2 K)), and later operators of that kind take it without timing.
+ * Candidates: 0 the per-node stage; 1 + 2c + o the matrix-core stage in
+ * configuration c (0: up to 12 FP32 / 8 FP64 waves per block, exterior
+ * traces gathered 2 k-steps ahead, residual read at the update; 1: up to 8 /
+ * 4 waves with twice the registers per wave, gathered 4 k-steps ahead, the
+ * first residual pass read during the volume products) with its 8-element
+ * tasks in Morton order (o = 0) or element order (o = 1). */
+typedef struct {
+    int32_t variant;          /* DGTD_KERNEL_VALU, _MFMA or _CSR */
+    int32_t config;           /* matrix-core configuration */
+    int32_t waves_per_block;
+    int32_t gather_depth;
+    int32_t early_residual;
+    int32_t task_order;       /* 1 Morton, 0 element order */
+    int32_t from_cache;       /* 1: taken from the process cache, not timed */
+    int32_t ncandidates;
+    double candidate_us[8];   /* per candidate, microseconds per launch (0: not timed or does not fit) */
+} dgtd_tuning;
+int dgtd_op_tuning(const dgtd_op* op, dgtd_tuning* out);
+
+/* Pins the stage kernel to one candidate of the autotuner (no counterpart in
+ * the reference): the candidate numbered as above (0 .. ncandidates-1).
+ * DGTD_E_ARG when it does not fit this operator (no matrix-core variant at
+ * this order and precision, element order without a Morton permutation) or
+ * the operator is assembled (DGTD_KERNEL_CSR).  Every candidate computes the
+ * same per-node arithmetic in the same order, so the result does not depend
+ * on the choice; callers pin one for reproducible timing. */
+int dgtd_op_set_candidate(dgtd_op* op, int32_t candidate);
+
 /* Algorithmic HBM bytes and flops per element per stage for this operator. */
 int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops);
 

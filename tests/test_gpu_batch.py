@@ -102,6 +102,27 @@ def test_continuous_admission_and_removal():
     assert b.stats()["bytes_in_use"] == 0
 
 
+def test_new_order_admitted_while_steps_are_in_flight():
+    """A request of an order the batch has not run yet (its stage choice is
+    tuned at admission) joins while queued steps of another instance are in
+    flight: the running instance keeps its progress and ends bitwise as
+    alone."""
+    vx, E, fbc, ref, q0 = case_input(3, 2, "pec", 11)
+    alone = D.Batch(precision=32, max_instances=1)
+    a = alone.add(vx, E, fbc, 2, x0=q0, final_time=0.3)
+    alone.run(poll_every=3)
+    xa, sa = alone.get_state(a), alone.info(a)["steps"]
+    alone.close()
+    b = D.Batch(precision=32, max_instances=4)
+    first = b.add(vx, E, fbc, 2, x0=q0, final_time=0.3)
+    b.step(6)  # queued, not waited for
+    vx3, E3, fbc3, _, q3 = case_input(2, 3, "sma", 12)
+    b.add(vx3, E3, fbc3, 3, x0=q3, final_time=0.2)
+    b.run(poll_every=3)
+    assert b.info(first)["steps"] == sa
+    np.testing.assert_array_equal(b.get_state(first), xa)
+
+
 def test_admission_by_footprint():
     vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
     fp = D.footprint(vx, E, fbc, 3, precision=32)

@@ -174,6 +174,46 @@ def test_autotune_picks_a_variant():
     vx, E, fbc = MG.cube_mesh(4, 1.0, "pec")
     g = D.HesthavenEvolution(vx, E, fbc, 4, precision=32, kernel="auto")
     assert g.kernel_variant() in ("valu", "mfma")
+    t = g.tuning()
+    assert t["variant"] == g.kernel_variant() and t["candidates_us"]
+    g2 = D.HesthavenEvolution(vx, E, fbc, 4, precision=32, kernel="auto")
+    assert g2.tuning()["from_cache"] and g2.kernel_variant() == g.kernel_variant()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_every_candidate_same_bits(order, precision):
+    """Every matrix-core candidate of the autotuner (configuration x task
+    order) stores the same bits after LSERK4 steps (the same per-node
+    arithmetic in the same order); the per-node candidate agrees to the
+    precision's bar, and FP64 to the oracle at 1e-12.  750 tetrahedra: the
+    last 8-element task is partial."""
+    vx, E, fbc, eps, mu = mixed_case(5, order, seed=20 + order)
+    ref = O.Operator(vx, E, fbc, order, eps=eps, mu=mu)
+    g = D.HesthavenEvolution(vx, E, fbc, order, eps=eps, mu=mu, precision=precision, integrator="lserk4",
+                             kernel="mfma")
+    x0 = np.random.default_rng(order).standard_normal(ref.ndofs)
+    dt = 0.5 / (ref.max_fscale() * order ** 2)
+    out = {}
+    for c, name in enumerate(D.CANDIDATES):
+        try:
+            g.set_candidate(c)
+        except D.DgtdError:
+            continue
+        g.set_state(x0)
+        g.advance(dt, 4)
+        out[name] = g.get_state()
+    mf = [n for n in out if n.startswith("mfma")]
+    assert "valu" in out and "mfma_c0_morton" in out and "mfma_c0_element" in out
+    for n in mf:
+        np.testing.assert_array_equal(out[n], out["mfma_c0_morton"], err_msg=n)
+    xr = x0.copy()
+    res = np.zeros_like(xr)
+    for s in range(4):
+        ref.lserk4_step(xr, res, s * dt, dt)
+    tol = 1e-12 if precision == 64 else 1e-5
+    for n in out:
+        assert rel(out[n], xr) <= tol, n
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
