@@ -444,6 +444,14 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
     } catch (const std::exception& e) {
         return batch_error(e);
     }
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        if (b->use_mfma(mesh->order) &&
+            !dgtd::mfma_fits_buffers(6 * (int64_t)dgtd::num_nodes(mesh->order) * mesh->K, b->precision / 8))
+            return dgtd::api_fail(DGTD_E_UNSUPPORTED,
+                                  "instance state reaches 4 GiB per buffer: too large for the batched "
+                                  "matrix-core stage (run it as a single operator)");
+    }
     int slot = -1;
     {  // reserve a slot and the bytes
         std::lock_guard<std::mutex> lk(b->mu);

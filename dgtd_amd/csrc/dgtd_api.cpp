@@ -740,7 +740,7 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     for (int c = 0; c < dgtd::kMfmaConfigs; ++c) {
         const int mbpc = op->precision == 64 ? dgtd::tri_mfma_blocks_per_cu<double>(R.p, c)
                                              : dgtd::tri_mfma_blocks_per_cu<float>(R.p, c);
-        if (mbpc <= 0) continue;
+        if (mbpc <= 0 || !dgtd::mfma_fits_buffers(6 * (int64_t)op->Np * op->K, op->precision / 8)) continue;
         const int mwaves = op->precision == 64 ? dgtd::tri_mfma_waves_per_block<double>(R.p, c)
                                                : dgtd::tri_mfma_waves_per_block<float>(R.p, c);
         op->grid_cfg[c] = mfma_grid(op->K, mwaves, prop.multiProcessorCount * mbpc);
@@ -752,7 +752,7 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
         op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
     }
     if (kernel == DGTD_KERNEL_MFMA && op->grid_cfg[0] <= 0)
-        throw std::runtime_error("the MFMA kernel does not fit this order and precision");
+        throw std::runtime_error("the MFMA kernel does not fit this order and precision, or a state buffer reaches 4 GiB");
     select_variant(op, kernel, [&] { upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha)); });
 }
 
@@ -907,7 +907,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         for (int c = 0; c < dgtd::kMfmaConfigs; ++c) {
             const int mbpc = op->precision == 64 ? dgtd::mfma_blocks_per_cu<double>(op->R.p, c)
                                                  : dgtd::mfma_blocks_per_cu<float>(op->R.p, c);
-            if (mbpc <= 0) continue;
+            if (mbpc <= 0 || !dgtd::mfma_fits_buffers(6 * (int64_t)op->Np * op->K, op->precision / 8)) continue;
             const int mwaves = op->precision == 64 ? dgtd::mfma_waves_per_block<double>(op->R.p, c)
                                                    : dgtd::mfma_waves_per_block<float>(op->R.p, c);
             op->grid_cfg[c] = mfma_grid(op->K, mwaves, prop.multiProcessorCount * mbpc);
@@ -915,7 +915,7 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         }
         if (any_mfma) op->d_order = static_cast<int*>(upload(morton_task_order(mesh)));
         if (opts->kernel == DGTD_KERNEL_MFMA && op->grid_cfg[0] <= 0)
-            throw std::runtime_error("the MFMA kernel does not fit this order and precision");
+            throw std::runtime_error("the MFMA kernel does not fit this order and precision, or a state buffer reaches 4 GiB");
         select_variant(op, opts->kernel,
                        [&] { upload_csr(op, dgtd::assemble_global_tet(md, op->R, mesh->sigma, op->alpha)); });
     } catch (const HipError& e) {

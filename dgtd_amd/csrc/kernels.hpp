@@ -89,6 +89,13 @@ size_t tri_smem_bytes(int order, int precision);
 // elements, waves_per_block waves, in one of kMfmaConfigs tuning
 // configurations (waves per block, gather depth, residual prefetch).
 constexpr int kMfmaTaskElems = 8;
+// The matrix-core stage addresses each state buffer (6 N values) through
+// 32-bit byte offsets: it runs while a buffer stays below 4 GiB (FP32: up to
+// 178,956,970 nodes, e.g. 5.1 million tetrahedra at order 4).
+inline bool mfma_fits_buffers(int64_t ndofs, int value_bytes)
+{
+    return ndofs * value_bytes < (int64_t(1) << 32);
+}
 constexpr int kMfmaConfigs = 2;
 struct MfmaInfo {
     bool ok;              // the configuration's LDS tiles fit

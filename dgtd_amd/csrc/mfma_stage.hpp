@@ -68,6 +68,47 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
 };
 
+// Field accesses go through buffer descriptors with 32-bit byte offsets
+// (one descriptor per state buffer, uniform across the wave): the per-lane
+// address is a 32-bit offset, and the component and task bases ride in the
+// scalar offset, so no per-lane 64-bit address arithmetic is issued.  Each
+// state buffer (6 N values) must stay below 4 GiB (mfma_fits_buffers).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kBufferFlags = 0x00020000;  // raw (untyped) 32-bit dword access on gfx9
+constexpr int kAuxNontemporal = 2;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferFlags);
+}
+
+// AUX: the cache-policy bits (0, or kAuxNontemporal for data used once)
+template <typename T, int AUX = 0>
+__device__ __forceinline__ T bload(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+    } else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX);
+        return __longlong_as_double(((long long)v.y << 32) | v.x);
+    }
+}
+
+template <int AUX>
+__device__ __forceinline__ void bstore(float v, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, AUX);
+}
+template <int AUX>
+__device__ __forceinline__ void bstore(double v, rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const u32x2 w = {(unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)voff, (int)soff, AUX);
+}
+
 // Element shape: tetrahedra (DIM 3) or triangles (DIM 2, the 2D operator:
 // d/dz = 0, n_z = 0, edges as faces; records as tri.hpp lays them out).
 template <int P, int DIM = 3>
@@ -91,19 +132,38 @@ struct MShape {
 
 constexpr int kTaskElems = kMfmaTaskElems;  // MFMA rows: 8 elements x {E, H} outputs
 
-// Tuning configurations of the stage (chosen per operator by the autotuner):
+// Tuning configurations of the stage (chosen per operator by the autotuner).
+// A task issues the exterior-trace gathers of its first flux k-steps, runs
+// the volume products (which hide their latency), then the flux loop, which
+// keeps the same number of k-steps in flight:
 //   0 "wide": as many waves as LDS allows (at most 12 FP32 / 8 FP64, i.e. 3
-//     or 2 waves per SIMD sharing its registers), exterior traces gathered 2
-//     flux k-steps ahead, the residual read when the update starts;
+//     or 2 waves per SIMD sharing its registers), 2 k-steps of traces in
+//     flight, the residual read when the update starts;
 //   1 "deep": at most 8 FP32 / 4 FP64 waves (twice the registers per wave),
-//     traces gathered 4 k-steps ahead, the first residual pass read before
-//     the volume products.
+//     16 FP32 / 4 FP64 k-steps in flight (all of a face set up to order 4 in
+//     FP32), the first residual pass read during the flux loop.
 template <int CFG>
 struct MCfg {
     static constexpr int kMaxWaves32 = CFG == 0 ? 12 : 8;
     static constexpr int kMaxWaves64 = CFG == 0 ? 8 : 4;
-    static constexpr int kGatherDepth = CFG == 0 ? 2 : 4;  // flux-loop k-steps whose exterior traces are in flight
+    static constexpr int kGatherDepth32 = CFG == 0 ? 2 : 16;  // flux k-steps whose exterior traces are in flight
+    static constexpr int kGatherDepth64 = CFG == 0 ? 2 : 4;
     static constexpr bool kEarlyResidual = CFG == 1;
+    template <typename T>
+    static constexpr int gather_depth() { return sizeof(T) == 4 ? kGatherDepth32 : kGatherDepth64; }
+};
+
+// Output nodes on the VALU instead of a mostly padded last MFMA column group:
+// when the last group of 16 would hold only NT < 8 valid columns (order 3: 4,
+// order 4: 3), summing those nodes per lane on the VALU costs 3 NT FMAs per
+// lane and k-step against three MFMAs of 32 (FP32) or 64 (FP64) cycles.
+// Not for FP64 at order 3, where the partial sums' registers make the stage
+// spill (measured 840 -> 912 us on the 663,552-element order-3 box).
+template <int P, typename T, int DIM>
+struct MTail {
+    using S = MShape<P, DIM>;
+    static constexpr int NT = S::NP - 16 * (S::G - 1);
+    static constexpr bool on = S::G >= 2 && NT < 8 && !(sizeof(T) == 8 && P == 3);
 };
 
 // Waves per block, chosen so the block's LDS fits in 160 KiB.  When the
@@ -113,7 +173,10 @@ struct MCfg {
 // and the tiles only.
 template <int P, typename T, int DIM = 3, int CFG = 0>
 struct MWaves {
-    static constexpr int kMaxWaves = sizeof(T) == 4 ? MCfg<CFG>::kMaxWaves32 : MCfg<CFG>::kMaxWaves64;
+    // the VALU tail's partial sums need the registers of at most 2 waves per
+    // SIMD (at 3 the FP32 order 3 and 4 stages spill)
+    static constexpr int kMaxWaves = sizeof(T) == 4 ? (MTail<P, T, DIM>::on ? 8 : MCfg<CFG>::kMaxWaves32)
+                                                    : MCfg<CFG>::kMaxWaves64;
     static constexpr int kMinLdsWaves = 4;
     using S = MShape<P, DIM>;
     static constexpr size_t bmats =
@@ -225,6 +288,12 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     const size_t N = (size_t)NP * K;
     const int e0 = task * TE;
     const int ne = min(TE, K - e0);
+    // byte offsets below 4 GiB (mfma_fits_buffers): component c of node n at
+    // (c*N + n) * sizeof(T)
+    constexpr uint32_t SZ = sizeof(T);
+    const uint32_t bytes = (uint32_t)(6 * N * SZ);
+    const uint32_t nb_bytes = (uint32_t)(N * SZ);  // one component
+    const rsrc_t rq = make_rsrc(a.qin, bytes);
 
     // Every global load and store of a task is unconditional (out-of-range
     // lanes use a clamped, in-bounds index): with no branch around memory
@@ -247,11 +316,11 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
         T v[6][NL];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            const T* src = a.qin + c * N + (size_t)e0 * NP;
+            const uint32_t so = c * nb_bytes + (uint32_t)e0 * NP * SZ;
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 const int t = lane + 64 * k;
-                v[c][k] = src[t < cnt ? t : cnt - 1];
+                v[c][k] = bload<T>(rq, (uint32_t)(t < cnt ? t : cnt - 1) * SZ, so);
             }
         }
         if constexpr (DIM == 3) {
@@ -303,19 +372,36 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     const T sign = half ? T(-1) : T(1);
     const T sscale = sign * scale;
 
-    acc_t R[3][G];
+    // Output nodes: GM groups of 16 on the matrix cores and, with MTail, the
+    // last NT nodes summed per lane on the VALU (each lane over its own
+    // k-slots, the four slots added at the end).
+    constexpr bool kVTail = MTail<P, T, DIM>::on;
+    constexpr int GM = kVTail ? G - 1 : G;
+    constexpr int NT = kVTail ? MTail<P, T, DIM>::NT : 1;
+    acc_t R[3][GM];
+    T Rt[3][NT];
 #pragma unroll
-    for (int x = 0; x < 3; ++x)
+    for (int x = 0; x < 3; ++x) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) R[x][g] = acc_t{0, 0, 0, 0};
+        for (int g = 0; g < GM; ++g) R[x][g] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < NT; ++n) Rt[x][n] = T(0);
+    }
+    // B values of the tail nodes in row `row` of a reference matrix whose
+    // rows are NPG long (the same for the lanes of a k-slot: a broadcast read)
+    auto tail_b = [&](const T* row, T (&bt)[NT]) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) bt[n] = row[16 * GM + n];
+    };
 
-    // ---- surface -> LIFT ----
+    // ---- surface -> LIFT (after the volume products, below) ----
     // Face by face; k-step t of face f covers nodes b = 4t + q, so the
     // lanes of an element read the same neighbour element.  A lane forms
     // the jumps of its own field only (E in rows 0-7, H in rows 8-15) and
     // takes the other field's jumps from its partner row (lane ^ 8, one
     // DPP row rotation).  Exterior traces are gathered from memory D k-steps
-    // ahead (a boundary face reads the element's own trace, weighted by the
+    // ahead, the first D before the volume products (a boundary face reads
+    // the element's own trace, weighted by the
     // boundary coefficient minus one; HesthavenEvolution.cpp:290-292,
     // 309-311).  Only neighbours in another task are gathered, the others
     // read the task's fields in LDS; except for FP64 tetrahedra at orders 4
@@ -333,20 +419,102 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     auto gather = [&](int s, T (&v)[3]) {
         const int f = s / SF, b = 4 * (s % SF) + q;
         const int idx = face_tab(f)[b < NFP ? b : NFP - 1];
-        size_t off;
+        uint32_t off;
         if constexpr (kRemoteOnly)
-            off = face_remote(f) ? (size_t)fnb[f] * NP + idx : (size_t)e0 * NP;
+            off = face_remote(f) ? (uint32_t)fnb[f] * NP + idx : (uint32_t)e0 * NP;
         else
-            off = (size_t)(fnb[f] >= 0 ? fnb[f] : e) * NP + idx;
-        const T* src = a.qin + (size_t)cbase * N + off;
+            off = (uint32_t)(fnb[f] >= 0 ? fnb[f] : e) * NP + idx;
+        const uint32_t vo = (half ? 3 * nb_bytes : 0u) + off * SZ;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = src[c * N];
+        for (int c = 0; c < 3; ++c) v[c] = bload<T>(rq, vo, c * nb_bytes);
     };
-    constexpr int kGatherDepth = MCfg<CFG>::kGatherDepth;
+    constexpr int kGatherDepth = MCfg<CFG>::template gather_depth<T>();
     constexpr int D = kGatherDepth < NFACE * SF ? kGatherDepth : NFACE * SF;
     T pf[D][3];
 #pragma unroll
     for (int d = 0; d < D; ++d) gather(d, pf[d]);
+    // The residual and base-state reads and the stage's stores are used once
+    // per stage: they stream non-temporally, so L2 keeps the fields that
+    // neighbouring tasks gather.
+    // ---- stage update: per x, the E_x and H_x right-hand sides go through
+    // the wave's LDS staging buffer and the update runs as a linear pass
+    // over the task's contiguous chunk (whole sectors, coalesced).  The
+    // residual / base state of the pair x+1 (linear index t over
+    // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
+    constexpr int NLIN = (2 * TE * NP + 63) / 64;
+    constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
+    constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
+    // Linear slot t (E_x | H_x halves) clamped to the last valid node of its
+    // half: lanes past the end repeat that slot's load, arithmetic and store
+    // (identical values to the same address).
+    auto clamp_slot = [&](int t) {
+        const int tc = t < 2 * TE * NP ? t : 2 * TE * NP - 1;
+        const int h = tc >= TE * NP;
+        const int u = tc - h * TE * NP;
+        return h * TE * NP + (u < ne * NP ? u : ne * NP - 1);
+    };
+    // byte offset of linear slot tc within component x's pass: the lane part
+    // (E_x or H_x half, node within the task) and the wave-uniform part
+    auto lin_voff = [&](int tc) {
+        const int h = tc >= TE * NP;
+        return (h ? 3 * nb_bytes : 0u) + (uint32_t)(tc - h * TE * NP) * SZ;
+    };
+    auto lin_soff = [&](int x) { return x * nb_bytes + (uint32_t)e0 * NP * SZ; };
+    const rsrc_t rres = make_rsrc(a.res, bytes);
+    const rsrc_t rout = make_rsrc(a.out, bytes);
+    const rsrc_t rx = make_rsrc(a.x, bytes);
+    // the lane's linear slots, the same for every x: byte offsets (lane part)
+    uint32_t lvo[NLIN];
+#pragma unroll
+    for (int k = 0; k < NLIN; ++k) lvo[k] = lin_voff(clamp_slot(lane + 64 * k));
+    T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
+    auto fetch = [&](int x, int slot) {
+#pragma unroll
+        for (int k = 0; k < NLIN; ++k) {
+            if constexpr (kNeedRes) rl[slot][k] = bload<T, kAuxNontemporal>(rres, lvo[k], lin_soff(x));
+            if constexpr (kNeedX) xl[slot][k] = bload<T, kAuxNontemporal>(rx, lvo[k], lin_soff(x));
+        }
+    };
+    // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
+    // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
+    // three k-steps per iteration, one with the VALU tail at order 3 (its
+    // extra LDS reads hoisted over three k-steps spill)
+    constexpr int kVolUnroll = kVTail && P <= 3 ? 1 : 3;
+#pragma unroll kVolUnroll
+    for (int ks = 0; ks < S::S; ++ks) {
+        const int j = q * S::S + ks;
+        T src[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
+        const T* bv = sBv + j * NDIR * NPG + li;
+#pragma unroll
+        for (int kd = 0; kd < NDIR; ++kd) {
+            const T* gk = gm + 3 * kd;
+            T w[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int y = (x + 1) % 3, z = (x + 2) % 3;
+                w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
+            }
+#pragma unroll
+            for (int g = 0; g < GM; ++g) {
+                const T bval = bv[kd * NPG + 16 * g];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
+            }
+            if constexpr (kVTail) {
+                T bt[NT];
+                tail_b(sBv + j * NDIR * NPG + kd * NPG, bt);
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                    for (int n = 0; n < NT; ++n) Rt[x][n] += w[x] * bt[n];
+            }
+        }
+    }
+
+    if constexpr (MCfg<CFG>::kEarlyResidual) fetch(0, 0);  // in flight during the flux loop
+
 #pragma unroll
     for (int f = 0; f < NFACE; ++f) {
         const int nb = fnb[f];
@@ -414,83 +582,41 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
             }
             const T* bl = sBl + (f * NFPP + b) * NPG + li;
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
+            for (int g = 0; g < GM; ++g) {
                 const T bv = bl[16 * g];
 #pragma unroll
                 for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(F[x], bv, R[x][g]);
             }
-        }
-    }
-
-    // The residual and base-state reads and the stage's stores are used once
-    // per stage: they stream non-temporally, so L2 keeps the fields that
-    // neighbouring tasks gather.
-    // ---- stage update: per x, the E_x and H_x right-hand sides go through
-    // the wave's LDS staging buffer and the update runs as a linear pass
-    // over the task's contiguous chunk (whole sectors, coalesced).  The
-    // residual / base state of the pair x+1 (linear index t over
-    // E_x|H_x, t < 2*TE*NP) is read while pass x runs. ----
-    constexpr int NLIN = (2 * TE * NP + 63) / 64;
-    constexpr bool kNeedRes = MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4;
-    constexpr bool kNeedX = MODE == kModeRk2 || MODE == kModeRk3;
-    // Linear slot t (E_x | H_x halves) clamped to the last valid node of its
-    // half: lanes past the end repeat that slot's load, arithmetic and store
-    // (identical values to the same address).
-    auto clamp_slot = [&](int t) {
-        const int tc = t < 2 * TE * NP ? t : 2 * TE * NP - 1;
-        const int h = tc >= TE * NP;
-        const int u = tc - h * TE * NP;
-        return h * TE * NP + (u < ne * NP ? u : ne * NP - 1);
-    };
-    auto lin_index = [&](int x, int tc) {
-        const int h = tc >= TE * NP;
-        return (size_t)(h ? 3 + x : x) * N + (size_t)e0 * NP + (tc - h * TE * NP);
-    };
-    T rl[2][NLIN], xl[2][kNeedX ? NLIN : 1];
-    auto fetch = [&](int x, int slot) {
+            if constexpr (kVTail) {
+                T bt[NT];
+                tail_b(sBl + (f * NFPP + b) * NPG, bt);
 #pragma unroll
-        for (int k = 0; k < NLIN; ++k) {
-            const size_t o = lin_index(x, clamp_slot(lane + 64 * k));
-            if constexpr (kNeedRes) rl[slot][k] = __builtin_nontemporal_load(a.res + o);
-            if constexpr (kNeedX) xl[slot][k] = __builtin_nontemporal_load(a.x + o);
-        }
-    };
-    if constexpr (MCfg<CFG>::kEarlyResidual) fetch(0, 0);  // in flight during the volume products
-
-    // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
-    // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
-#pragma unroll 3
-    for (int ks = 0; ks < S::S; ++ks) {
-        const int j = q * S::S + ks;
-        T src[3];
+                for (int x = 0; x < 3; ++x)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
-        const T* bv = sBv + j * NDIR * NPG + li;
-#pragma unroll
-        for (int kd = 0; kd < NDIR; ++kd) {
-            const T* gk = gm + 3 * kd;
-            T w[3];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const int y = (x + 1) % 3, z = (x + 2) % 3;
-                w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
-            }
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const T bval = bv[kd * NPG + 16 * g];
-#pragma unroll
-                for (int x = 0; x < 3; ++x) R[x][g] = MM::mma(w[x], bval, R[x][g]);
+                    for (int n = 0; n < NT; ++n) Rt[x][n] += F[x] * bt[n];
             }
         }
     }
 
     if constexpr (!MCfg<CFG>::kEarlyResidual) fetch(0, 0);
+    // and their staging-buffer index and field-tile index of component x = 0
+    // (E_x or H_x row of the slot's element; component x adds x * TE * US)
+    int lst[NLIN], lqv[NLIN];
+#pragma unroll
+    for (int k = 0; k < NLIN; ++k) {
+        const int t = clamp_slot(lane + 64 * k);
+        const int h = t >= TE * NP;
+        const unsigned u = (unsigned)(t - h * TE * NP);
+        const unsigned r_el = u / NP;
+        lst[k] = t;
+        lqv[k] = (int)(((h ? 3u : 0u) * TE + r_el) * US + (u - r_el * NP));
+    }
     bool bad = false;  // an unstable state entry stored by this lane (a.unstable set)
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         const int cur = x & 1;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < GM; ++g) {
             const int i = 16 * g + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -498,35 +624,43 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
                 if (i < NP) Sg[(row >> 3) * TE * NP + (row & 7) * NP + i] = R[x][g][r];
             }
         }
+        if constexpr (kVTail) {
+            // row li's tail nodes: the four k-slot partial sums added (the
+            // lanes li, li+16, li+32, li+48), node n stored by slot n % 4
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                T v = Rt[x][n];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (n % 4 == q) Sg[(li >> 3) * TE * NP + (li & 7) * NP + 16 * GM + n] = v;
+            }
+        }
         wave_sync();
         if (x + 1 < 3) fetch(x + 1, cur ^ 1);
 #pragma unroll
         for (int k = 0; k < NLIN; ++k) {
-            const int t = clamp_slot(lane + 64 * k);
-            const size_t o = lin_index(x, t);
-            const int h = t >= TE * NP;
-            const int u = t - h * TE * NP;
-            const int c = h ? 3 + x : x;
-            const int r_el = u / NP;
-            const T rhs = Sg[t];
-            const T qv = U[(c * TE + r_el) * US + (u - r_el * NP)];
+            const uint32_t vo = lvo[k];
+            const uint32_t so = lin_soff(x);
+            const T rhs = Sg[lst[k]];
+            const T qv = U[lqv[k] + x * TE * US];
+            constexpr int nt = kAuxNontemporal;
             if constexpr (MODE == kModeMult) {
-                __builtin_nontemporal_store(rhs, a.out + o);
+                bstore<nt>(rhs, rout, vo, so);
             } else if constexpr (MODE == kModeLserk) {
                 const T rr = a.ca * rl[cur][k] + a.dt * rhs;
                 const T qn = qv + a.cb * rr;
-                __builtin_nontemporal_store(rr, a.res + o);
-                __builtin_nontemporal_store(qn, a.out + o);
+                bstore<nt>(rr, rres, vo, so);
+                bstore<nt>(qn, rout, vo, so);
                 bad = bad || unstable_entry(qn);
             } else if constexpr (MODE == kModeRk1) {
-                __builtin_nontemporal_store(qv + a.ca * rhs, a.out + o);
-                __builtin_nontemporal_store(qv + a.cb * rhs, a.res + o);
+                bstore<nt>(qv + a.ca * rhs, rout, vo, so);
+                bstore<nt>(qv + a.cb * rhs, rres, vo, so);
             } else if constexpr (MODE == kModeRk2 || MODE == kModeRk3) {
-                __builtin_nontemporal_store(xl[cur][k] + a.ca * rhs, a.out + o);
-                __builtin_nontemporal_store(rl[cur][k] + a.cb * rhs, a.res + o);
+                bstore<nt>(xl[cur][k] + a.ca * rhs, rout, vo, so);
+                bstore<nt>(rl[cur][k] + a.cb * rhs, rres, vo, so);
             } else {
                 const T qn = rl[cur][k] + a.ca * rhs;
-                __builtin_nontemporal_store(qn, a.out + o);
+                bstore<nt>(qn, rout, vo, so);
                 bad = bad || unstable_entry(qn);
             }
         }
@@ -668,9 +802,9 @@ template <int CFG, typename T>
 MfmaInfo mfma_info_cfg(int order, int dim)
 {
 #define DGTD_INFO(P)                                                                                       \
-    (dim == 2 ? MfmaInfo{MWaves<P, T, 2, CFG>::ok, MWaves<P, T, 2, CFG>::W, MCfg<CFG>::kGatherDepth,        \
+    (dim == 2 ? MfmaInfo{MWaves<P, T, 2, CFG>::ok, MWaves<P, T, 2, CFG>::W, MCfg<CFG>::template gather_depth<T>(),        \
                          MCfg<CFG>::kEarlyResidual}                                                          \
-              : MfmaInfo{MWaves<P, T, 3, CFG>::ok, MWaves<P, T, 3, CFG>::W, MCfg<CFG>::kGatherDepth,        \
+              : MfmaInfo{MWaves<P, T, 3, CFG>::ok, MWaves<P, T, 3, CFG>::W, MCfg<CFG>::template gather_depth<T>(),        \
                          MCfg<CFG>::kEarlyResidual})
     DGTD_ORDER_SWITCH(order, DGTD_INFO, (MfmaInfo{false, 1, 0, false}))
 #undef DGTD_INFO
