@@ -111,6 +111,7 @@ struct dgtd_batch {
     // Tunes the order once per batch.  Called without `mu` (it may build and
     // time an operator); `tune_mu` serialises tuning, `mu` guards the result.
     std::mutex tune_mu;
+    static constexpr int64_t kTuneElements = 131072;
     void tune(int order, int64_t K)
     {
         {
@@ -125,7 +126,9 @@ struct dgtd_batch {
         int c = 0;
         const bool can = order >= 2 && (precision == 64 ? dgtd::mfma_available<double>(order, 0)
                                                          : dgtd::mfma_available<float>(order, 0));
-        if (can && !dgtd::tuned_mfma_config(order, precision, K, &c)) c = 0;
+        // tuned at the size of a launch that fills the GPU (the order's
+        // launch carries every instance of the order, not this one alone)
+        if (can && !dgtd::tuned_mfma_config(order, precision, std::max<int64_t>(K, kTuneElements), &c)) c = 0;
         const bool fits = precision == 64 ? dgtd::mfma_available<double>(order, c) : dgtd::mfma_available<float>(order, c);
         // Leaves `dirty` alone: marking the host table dirty without pulling
         // the device table first would push stale step counters over steps in

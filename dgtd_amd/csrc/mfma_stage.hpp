@@ -109,6 +109,25 @@ __device__ __forceinline__ void bstore(double v, rsrc_t r, uint32_t voff, uint32
     __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)voff, (int)soff, AUX);
 }
 
+// A value known to be the same in every lane of the wave, moved to a scalar
+// register (the first active lane's copy).
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename P>
+__device__ __forceinline__ P* uniform(P* p)
+{
+    const unsigned long long b = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
+}
+
 // Element shape: tetrahedra (DIM 3) or triangles (DIM 2, the 2D operator:
 // d/dz = 0, n_z = 0, edges as faces; records as tri.hpp lays them out).
 template <int P, int DIM = 3>
@@ -702,23 +721,29 @@ __global__ __launch_bounds__((MWaves<P, T, 3, CFG>::W * 64)) void dg_batch_stage
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int begin = range[0], end = range[1];
     for (int it = begin + blockIdx.x * WAVES + wave; it < end; it += gridDim.x * WAVES) {
+        // The item and the instance record are the same for the whole wave
+        // but come through vector loads (the records are written on the
+        // device); made wave-uniform (scalar registers) so that the task's
+        // buffer descriptors and offsets are scalar, without waterfall loops.
         const BatchItem w = b.items[it];
-        const BatchInst<T> I = b.inst[w.inst];
-        if (I.state != kSlotRunning) continue;  // uniform across the wave
+        const int inst = uniform(w.inst);
+        const int blk = uniform(w.blk);
+        const BatchInst<T>& I = b.inst[inst];
+        if (uniform(I.state) != kSlotRunning) continue;
         StageArgs<T> a{};
-        a.K = I.K;
-        const int src = I.cur ^ (b.stage & 1);
-        a.qin = I.q[src];
-        a.out = I.q[src ^ 1];
-        a.res = I.res;
+        a.K = uniform(I.K);
+        const int src = uniform(I.cur) ^ (b.stage & 1);
+        a.qin = uniform(I.q[src]);
+        a.out = uniform(I.q[src ^ 1]);
+        a.res = uniform(I.res);
         a.ca = b.ca;
         a.cb = b.cb;
-        a.dt = (T)I.dt_step;
-        a.alpha = (T)I.alpha;
-        a.geo = I.geo;
-        a.conn = I.conn;
-        a.unstable = b.stage == 4 ? &b.inst[w.inst].unstable : nullptr;  // the step's last stage
-        mfma_task<P, T, kModeLserk, 3, CFG>(a, L, w.blk);
+        a.dt = (T)uniform(I.dt_step);
+        a.alpha = (T)uniform(I.alpha);
+        a.geo = uniform(I.geo);
+        a.conn = uniform(I.conn);
+        a.unstable = b.stage == 4 ? &b.inst[inst].unstable : nullptr;  // the step's last stage
+        mfma_task<P, T, kModeLserk, 3, CFG>(a, L, blk);
     }
 }
 
