@@ -100,13 +100,17 @@ def profiled_traffic(kernel, workload):
     for name in sorted(os.listdir(pdir)):
         if not name.endswith("_hbm_counters.md"):
             continue
+        # records of "kernel = ...", "workload = ...", "traffic_bytes_per_launch = ..." lines
         kv = {}
         for ln in open(os.path.join(pdir, name)):
-            if " = " in ln:
-                k, v = ln.strip().split(" = ", 1)
-                kv[k] = v
-        if kv.get("kernel") == kernel and kv.get("workload") == workload and "traffic_bytes_per_launch" in kv:
-            return float(kv["traffic_bytes_per_launch"]), "profiles/" + name
+            if " = " not in ln:
+                continue
+            k, v = ln.strip().split(" = ", 1)
+            if k == "kernel":
+                kv = {}
+            kv[k] = v
+            if kv.get("kernel") == kernel and kv.get("workload") == workload and "traffic_bytes_per_launch" in kv:
+                return float(kv["traffic_bytes_per_launch"]), "profiles/" + name
     return None, None
 
 
