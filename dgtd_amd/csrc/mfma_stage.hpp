@@ -390,6 +390,9 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     // half 0 forms E-output operands, half 1 H-output
     const T sign = half ? T(-1) : T(1);
     const T sscale = sign * scale;
+    T gms[9];  // the metric terms times the lane's sign and 1/eps or 1/mu
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gms[k] = gm[k] * sscale;
 
     // Output nodes: GM groups of 16 on the matrix cores and, with MTail, the
     // last NT nodes summed per lane on the VALU (each lane over its own
@@ -496,24 +499,28 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
     };
     // ---- volume: sum_k D_k w_k with w_k the metric-weighted curl terms ----
     // E_x out: (g_ky H_z - g_kz H_y)/eps;  H_x out: -(g_ky E_z - g_kz E_y)/mu
-    // three k-steps per iteration, one with the VALU tail at order 3 (its
-    // extra LDS reads hoisted over three k-steps spill)
-    constexpr int kVolUnroll = kVTail && P <= 3 ? 1 : 3;
+    // three k-steps per iteration; one with the VALU tail at order 3 and with
+    // the matrices in global memory (loads hoisted over three k-steps spill)
+    constexpr int kVolUnroll = (kVTail && P <= 3) || MWaves<P, T, DIM, CFG>::bglobal ? 1 : 3;
+    // the lane's k-slot q covers input nodes j = q*S .. q*S + S-1 of the
+    // other field (H for E outputs, E for H outputs)
+    const T* usrc = Ue + (half ? 0 : 3) * TE * US + q * S::S;
+    const T* bvq = sBv + q * S::S * NDIR * NPG;
 #pragma unroll kVolUnroll
     for (int ks = 0; ks < S::S; ++ks) {
-        const int j = q * S::S + ks;
         T src[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) src[c] = half ? Ue[c * TE * US + j] : Ue[(3 + c) * TE * US + j];
-        const T* bv = sBv + j * NDIR * NPG + li;
+        for (int c = 0; c < 3; ++c) src[c] = usrc[c * TE * US + ks];
+        const T* brow = bvq + ks * NDIR * NPG;
+        const T* bv = brow + li;
 #pragma unroll
         for (int kd = 0; kd < NDIR; ++kd) {
-            const T* gk = gm + 3 * kd;
+            const T* gk = gms + 3 * kd;
             T w[3];
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
                 const int y = (x + 1) % 3, z = (x + 2) % 3;
-                w[x] = (gk[y] * src[z] - gk[z] * src[y]) * sscale;
+                w[x] = gk[y] * src[z] - gk[z] * src[y];
             }
 #pragma unroll
             for (int g = 0; g < GM; ++g) {
@@ -523,7 +530,7 @@ __device__ __forceinline__ void mfma_task(const StageArgs<T>& a, const MfmaLds<P
             }
             if constexpr (kVTail) {
                 T bt[NT];
-                tail_b(sBv + j * NDIR * NPG + kd * NPG, bt);
+                tail_b(brow + kd * NPG, bt);
 #pragma unroll
                 for (int x = 0; x < 3; ++x)
 #pragma unroll
