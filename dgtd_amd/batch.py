@@ -54,7 +54,8 @@ class Batch:
         if rc == _abi.DGTD_E_CAPACITY:
             raise CapacityError(rc, lib().dgtd_last_error().decode())
         check(rc)
-        self._sizes[i.value] = 6 * num_nodes(order) * m.EToV.shape[0]
+        npe = num_nodes(order) if m.dim == 3 else (order + 1) * (order + 2) // 2
+        self._sizes[i.value] = 6 * npe * m.EToV.shape[0]
         return i.value
 
     def step(self, nsteps=1):

@@ -18,10 +18,11 @@
 #include "mesh.hpp"
 #include "packing.hpp"
 #include "reftet.hpp"
+#include "tri.hpp"
 
 namespace dgtd {
 int api_fail(int code, const std::string& msg);  // dgtd_api.cpp
-bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg);  // dgtd_api.cpp
+bool tuned_mfma_config(int dim, int order, int precision, int64_t K, int* cfg);  // dgtd_api.cpp
 }
 
 namespace {
@@ -42,11 +43,16 @@ const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612
                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                       2277821191437.0 / 14882151754819.0};
 
-int64_t footprint(int K, int order, int precision)
+int nodes(int dim, int order) { return dim == 2 ? dgtd::tri_num_nodes(order) : dgtd::num_nodes(order); }
+
+// Device bytes of an instance: three state buffers, geometry and connectivity records.
+int64_t footprint(int dim, int K, int order, int precision)
 {
     const int64_t s = precision == 64 ? 8 : 4;
-    const int64_t Np = dgtd::num_nodes(order);
-    return 3 * 6 * Np * K * s + (int64_t)K * dgtd::kGeoStride * s + (int64_t)K * dgtd::kConnStride * 4;
+    const int64_t Np = nodes(dim, order);
+    const int64_t geo = dim == 2 ? dgtd::kTriGeoStride : dgtd::kGeoStride;
+    const int64_t conn = dim == 2 ? dgtd::kTriConnStride : dgtd::kConnStride;
+    return 3 * 6 * Np * K * s + (int64_t)K * geo * s + (int64_t)K * conn * 4;
 }
 
 }  // namespace
@@ -65,7 +71,10 @@ struct dgtd_batch {
     dgtd::BatchItem* d_items = nullptr;
     int item_cap = 0;
     int* d_count = nullptr;
-    int* d_ranges = nullptr;  // per order o: [2o, 2o+1] = matrix-core item range
+    // matrix-core item ranges: tetrahedra of order o at [2o, 2o+1], triangles
+    // of order o at [kTriRanges + 2o, kTriRanges + 2o + 1]
+    static constexpr int kTriRanges = 2 * (dgtd::kMaxBatchOrder + 1);
+    int* d_ranges = nullptr;
     void* d_mats[dgtd::kMaxBatchOrder + 1] = {};
     void* d_mats_mfma[dgtd::kMaxBatchOrder + 1] = {};
     int grid_mfma[dgtd::kMaxBatchOrder + 1] = {};
@@ -76,6 +85,13 @@ struct dgtd_batch {
     bool mfma[dgtd::kMaxBatchOrder + 1] = {};
     int cfg[dgtd::kMaxBatchOrder + 1] = {};
     int* d_tabs[dgtd::kMaxBatchOrder + 1] = {};
+    // triangles (every order on the matrix-core stage), per order: matrices,
+    // node tables, launch grid, tuned configuration
+    void* d_mats_tri[dgtd::kMaxBatchOrder + 1] = {};
+    int* d_tabs_tri[dgtd::kMaxBatchOrder + 1] = {};
+    int grid_tri[dgtd::kMaxBatchOrder + 1] = {};
+    bool tuned_tri[dgtd::kMaxBatchOrder + 1] = {};
+    int cfg_tri[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
 
     static constexpr int kSlotReserved = 100;  // host only: being set up by dgtd_batch_add (not a BatchState)
@@ -83,7 +99,7 @@ struct dgtd_batch {
 
     struct Slot {
         int state = dgtd::kSlotFree;
-        int order = 0, K = 0, nblk = 0;
+        int dim = 3, order = 0, K = 0, nblk = 0;
         int64_t bytes = 0;
         void* buf[5] = {};  // q0, q1, res, geo, conn
         bool reported = false;
@@ -112,24 +128,36 @@ struct dgtd_batch {
     // time an operator); `tune_mu` serialises tuning, `mu` guards the result.
     std::mutex tune_mu;
     static constexpr int64_t kTuneElements = 131072;
-    void tune(int order, int64_t K)
+    void tune(int dim, int order, int64_t K)
     {
+        bool* done = dim == 2 ? tuned_tri : tuned;
         {
             std::lock_guard<std::mutex> lk(mu);
-            if (tuned[order]) return;
+            if (done[order]) return;
         }
         std::lock_guard<std::mutex> tl(tune_mu);
         {
             std::lock_guard<std::mutex> lk(mu);
-            if (tuned[order]) return;
+            if (done[order]) return;
         }
         int c = 0;
-        const bool can = order >= 2 && (precision == 64 ? dgtd::mfma_available<double>(order, 0)
-                                                         : dgtd::mfma_available<float>(order, 0));
+        auto available = [&](int cf) {
+            if (dim == 2)
+                return (precision == 64 ? dgtd::tri_mfma_blocks_per_cu<double>(order, cf)
+                                        : dgtd::tri_mfma_blocks_per_cu<float>(order, cf)) > 0;
+            return precision == 64 ? dgtd::mfma_available<double>(order, cf) : dgtd::mfma_available<float>(order, cf);
+        };
+        const bool can = (dim == 2 || order >= 2) && available(0);
         // tuned at the size of a launch that fills the GPU (the order's
         // launch carries every instance of the order, not this one alone)
-        if (can && !dgtd::tuned_mfma_config(order, precision, std::max<int64_t>(K, kTuneElements), &c)) c = 0;
-        const bool fits = precision == 64 ? dgtd::mfma_available<double>(order, c) : dgtd::mfma_available<float>(order, c);
+        if (can && !dgtd::tuned_mfma_config(dim, order, precision, std::max<int64_t>(K, kTuneElements), &c)) c = 0;
+        const bool fits = available(c);
+        if (dim == 2) {
+            std::lock_guard<std::mutex> lk(mu);
+            cfg_tri[order] = fits ? c : 0;
+            tuned_tri[order] = true;
+            return;
+        }
         // Leaves `dirty` alone: marking the host table dirty without pulling
         // the device table first would push stale step counters over steps in
         // flight.  The admission that follows pulls, publishes and marks it;
@@ -142,6 +170,8 @@ struct dgtd_batch {
     }
     bool cap_mfma[dgtd::kMaxBatchOrder + 1] = {};
     int cap_cfg[dgtd::kMaxBatchOrder + 1] = {};
+    bool cap_tri[dgtd::kMaxBatchOrder + 1] = {};
+    int cap_cfg_tri[dgtd::kMaxBatchOrder + 1] = {};
 
     ~dgtd_batch()
     {
@@ -156,6 +186,8 @@ struct dgtd_batch {
             if (d_mats[o]) (void)hipFree(d_mats[o]);
             if (d_mats_mfma[o]) (void)hipFree(d_mats_mfma[o]);
             if (d_tabs[o]) (void)hipFree(d_tabs[o]);
+            if (d_mats_tri[o]) (void)hipFree(d_mats_tri[o]);
+            if (d_tabs_tri[o]) (void)hipFree(d_tabs_tri[o]);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -213,22 +245,29 @@ struct dgtd_batch {
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
             if (use_mfma(o)) continue;
             for (int i = 0; i < max_slots; ++i)
-                if (slots[i].state == dgtd::kSlotRunning && slots[i].order == o) {
+                if (slots[i].state == dgtd::kSlotRunning && slots[i].dim == 3 && slots[i].order == o) {
                     max_order = std::max(max_order, o);
                     for (int blk = 0; blk < slots[i].nblk; ++blk) items.push_back({i, blk});
                 }
         }
         const int nvalu = (int)items.size();
-        int ranges[2 * (dgtd::kMaxBatchOrder + 1)] = {};
+        int ranges[2 * kTriRanges] = {};
+        auto add_tasks = [&](int dim, int o) {
+            for (int i = 0; i < max_slots; ++i)
+                if (slots[i].state == dgtd::kSlotRunning && slots[i].dim == dim && slots[i].order == o) {
+                    const int ntask = (slots[i].K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
+                    for (int t = 0; t < ntask; ++t) items.push_back({i, t});
+                }
+        };
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
             ranges[2 * o] = (int)items.size();
-            if (use_mfma(o))
-                for (int i = 0; i < max_slots; ++i)
-                    if (slots[i].state == dgtd::kSlotRunning && slots[i].order == o) {
-                        const int ntask = (slots[i].K + dgtd::kMfmaTaskElems - 1) / dgtd::kMfmaTaskElems;
-                        for (int t = 0; t < ntask; ++t) items.push_back({i, t});
-                    }
+            if (use_mfma(o)) add_tasks(3, o);
             ranges[2 * o + 1] = (int)items.size();
+        }
+        for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+            ranges[kTriRanges + 2 * o] = (int)items.size();
+            if (tuned_tri[o]) add_tasks(2, o);
+            ranges[kTriRanges + 2 * o + 1] = (int)items.size();
         }
         if ((int)items.size() > item_cap) {
             if (d_items) hcheck(hipFree(d_items), "hipFree");
@@ -248,7 +287,8 @@ struct dgtd_batch {
         hcheck(hipStreamSynchronize(stream), "sync");  // host vectors die here
         bool recapture = max_order != graph_order || !graph;
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o)
-            recapture = recapture || use_mfma(o) != cap_mfma[o] || cfg[o] != cap_cfg[o];
+            recapture = recapture || use_mfma(o) != cap_mfma[o] || cfg[o] != cap_cfg[o] ||
+                        tuned_tri[o] != cap_tri[o] || cfg_tri[o] != cap_cfg_tri[o];
         if (recapture) capture<T>(max_order);
     }
 
@@ -271,9 +311,15 @@ struct dgtd_batch {
             a.mats[o] = static_cast<const T*>(d_mats[o]);
             a.mats_mfma[o] = static_cast<const T*>(d_mats_mfma[o]);
             a.tabs[o] = d_tabs[o];
+            a.mats_mfma_tri[o] = static_cast<const T*>(d_mats_tri[o]);
+            a.tabs_tri[o] = d_tabs_tri[o];
             if (use_mfma(o)) {
                 const int bpc = dgtd::mfma_blocks_per_cu<T>(o, cfg[o]);
                 grid_mfma[o] = std::max(1, bpc) * num_cu;
+            }
+            if (tuned_tri[o]) {
+                const int bpc = dgtd::tri_mfma_blocks_per_cu<T>(o, cfg_tri[o]);
+                grid_tri[o] = std::max(1, bpc) * num_cu;
             }
         }
         hipGraph_t g = nullptr;
@@ -286,7 +332,11 @@ struct dgtd_batch {
             err = dgtd::launch_batch_stage<T>(a, grid, smem, stream);
             for (int o = 1; o <= dgtd::kMaxBatchOrder && err == hipSuccess; ++o)
                 if (use_mfma(o))
-                    err = dgtd::launch_batch_stage_mfma<T>(o, a, d_ranges + 2 * o, grid_mfma[o], stream, cfg[o]);
+                    err = dgtd::launch_batch_stage_mfma<T>(o, 3, a, d_ranges + 2 * o, grid_mfma[o], stream, cfg[o]);
+            for (int o = 1; o <= dgtd::kMaxBatchOrder && err == hipSuccess; ++o)
+                if (tuned_tri[o])
+                    err = dgtd::launch_batch_stage_mfma<T>(o, 2, a, d_ranges + kTriRanges + 2 * o, grid_tri[o], stream,
+                                                           cfg_tri[o]);
         }
         if (err == hipSuccess) err = dgtd::launch_batch_advance<T>(a.inst, max_slots, stream);
         hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -299,6 +349,8 @@ struct dgtd_batch {
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
             cap_mfma[o] = use_mfma(o);
             cap_cfg[o] = cfg[o];
+            cap_tri[o] = tuned_tri[o];
+            cap_cfg_tri[o] = cfg_tri[o];
         }
     }
 
@@ -322,9 +374,8 @@ namespace {
 
 // Runs without the batch lock: synchronous copies on the null stream (the
 // batch stream is non-blocking, so a step in flight is not waited for).
-template <typename T>
-void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const dgtd::MeshData& md, const double* x0,
-                     size_t ndofs)
+template <typename T, typename MD>
+void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const MD& md, const double* x0, size_t ndofs)
 {
     (void)b;
     const size_t bytes_q = sizeof(T) * ndofs;
@@ -359,6 +410,16 @@ void upload_refs(dgtd_batch* b)
         }
         hcheck(hipMalloc(&b->d_tabs[o], sizeof(int) * tabs.size()), "hipMalloc tabs");
         hcheck(hipMemcpy(b->d_tabs[o], tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice), "H2D tabs");
+        if (dgtd::tri_mfma_blocks_per_cu<T>(o, 0) > 0) {  // triangles
+            const dgtd::RefTri Rt = dgtd::build_reftri(o);
+            const std::vector<T> mt = dgtd::pack_mats_mfma<T>(Rt);
+            std::vector<int> tt(Rt.Fmask.begin(), Rt.Fmask.end());
+            tt.insert(tt.end(), Rt.edge_perm.begin(), Rt.edge_perm.end());
+            hcheck(hipMalloc(&b->d_mats_tri[o], sizeof(T) * mt.size()), "hipMalloc mats");
+            hcheck(hipMemcpy(b->d_mats_tri[o], mt.data(), sizeof(T) * mt.size(), hipMemcpyHostToDevice), "H2D mats");
+            hcheck(hipMalloc(&b->d_tabs_tri[o], sizeof(int) * tt.size()), "hipMalloc tabs");
+            hcheck(hipMemcpy(b->d_tabs_tri[o], tt.data(), sizeof(int) * tt.size(), hipMemcpyHostToDevice), "H2D tabs");
+        }
     }
 }
 
@@ -376,7 +437,7 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
 {
     if (!mesh || !bytes) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
-    *bytes = footprint(mesh->K, mesh->order, precision);
+    *bytes = footprint(mesh->dim == 2 ? 2 : 3, mesh->K, mesh->order, precision);
     return DGTD_OK;
 }
 
@@ -407,7 +468,7 @@ int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out)
         const size_t rec = b->precision == 64 ? sizeof(dgtd::BatchInst<double>) : sizeof(dgtd::BatchInst<float>);
         hcheck(hipMalloc(&b->d_inst, rec * b->max_slots), "hipMalloc table");
         hcheck(hipMalloc(&b->d_count, sizeof(int)), "hipMalloc count");
-        hcheck(hipMalloc(&b->d_ranges, sizeof(int) * 2 * (dgtd::kMaxBatchOrder + 1)), "hipMalloc ranges");
+        hcheck(hipMalloc(&b->d_ranges, sizeof(int) * 2 * dgtd_batch::kTriRanges), "hipMalloc ranges");
         b->item_cap = 1 << 16;
         hcheck(hipMalloc(&b->d_items, sizeof(dgtd::BatchItem) * b->item_cap), "hipMalloc items");
         if (b->precision == 64) upload_refs<double>(b);
@@ -428,7 +489,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
 {
     dgtd::TraceRange tr("dgtd_batch_add");
     if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
-    if (mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "batched instances are tetrahedral meshes");
+    const int dim = mesh->dim == 2 ? 2 : 3;
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return dgtd::api_fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     if (mesh->sigma)
@@ -437,20 +498,22 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
         return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");
     if (mesh->face_bc)
-        for (size_t i = 0; i < 4 * (size_t)std::max(0, mesh->K); ++i)
+        for (size_t i = 0; i < (size_t)(dim + 1) * std::max(0, mesh->K); ++i)
             if (mesh->face_bc[i] == DGTD_BC_TFSF)
                 return dgtd::api_fail(DGTD_E_UNSUPPORTED, "TF/SF faces are not supported by batched instances");
-    const int64_t bytes = footprint(mesh->K, mesh->order, b->precision);
+    const int64_t bytes = footprint(dim, mesh->K, mesh->order, b->precision);
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
-        b->tune(mesh->order, mesh->K);
+        b->tune(dim, mesh->order, mesh->K);
     } catch (const std::exception& e) {
         return batch_error(e);
     }
     {
         std::lock_guard<std::mutex> lk(b->mu);
-        if (b->use_mfma(mesh->order) &&
-            !dgtd::mfma_fits_buffers(6 * (int64_t)dgtd::num_nodes(mesh->order) * mesh->K, b->precision / 8))
+        if (dim == 2 && !b->tuned_tri[mesh->order])
+            return dgtd::api_fail(DGTD_E_UNSUPPORTED, "no matrix-core stage for triangles of this order and precision");
+        if ((dim == 2 || b->use_mfma(mesh->order)) &&
+            !dgtd::mfma_fits_buffers(6 * (int64_t)nodes(dim, mesh->order) * mesh->K, b->precision / 8))
             return dgtd::api_fail(DGTD_E_UNSUPPORTED,
                                   "instance state reaches 4 GiB per buffer: too large for the batched "
                                   "matrix-core stage (run it as a single operator)");
@@ -470,24 +533,44 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
     dgtd_batch::Slot s;  // built outside the lock
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
-        const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
-        dgtd::MeshInput in;
-        in.K = mesh->K;
-        in.nv = mesh->nv;
-        in.vx = mesh->vx;
-        in.EToV = mesh->EToV;
-        in.face_bc = mesh->face_bc;
-        in.eps_r = mesh->eps_r;
-        in.mu_r = mesh->mu_r;
-        const dgtd::MeshData md = dgtd::build_mesh(in, R, b->alpha);
-        const size_t ndofs = 6 * (size_t)R.Np * mesh->K;
-        if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
-        else upload_instance<float>(b, s, md, x0, ndofs);
+        const size_t ndofs = 6 * (size_t)nodes(dim, mesh->order) * mesh->K;
+        double max_fscale = 0.0;
+        if (dim == 2) {
+            const dgtd::RefTri R = dgtd::build_reftri(mesh->order);
+            dgtd::TriMeshInput in;
+            in.K = mesh->K;
+            in.nv = mesh->nv;
+            in.vx = mesh->vx;
+            in.EToV = mesh->EToV;
+            in.face_bc = mesh->face_bc;
+            in.eps_r = mesh->eps_r;
+            in.mu_r = mesh->mu_r;
+            const dgtd::TriMeshData md = dgtd::build_tri_mesh(in, R, b->alpha);
+            max_fscale = md.max_fscale;
+            if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
+            else upload_instance<float>(b, s, md, x0, ndofs);
+        } else {
+            const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
+            dgtd::MeshInput in;
+            in.K = mesh->K;
+            in.nv = mesh->nv;
+            in.vx = mesh->vx;
+            in.EToV = mesh->EToV;
+            in.face_bc = mesh->face_bc;
+            in.eps_r = mesh->eps_r;
+            in.mu_r = mesh->mu_r;
+            const dgtd::MeshData md = dgtd::build_mesh(in, R, b->alpha);
+            max_fscale = md.max_fscale;
+            if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
+            else upload_instance<float>(b, s, md, x0, ndofs);
+        }
+        s.dim = dim;
         s.order = mesh->order;
         s.K = mesh->K;
-        s.nblk = (mesh->K + dgtd::elements_per_block(mesh->order) - 1) / dgtd::elements_per_block(mesh->order);
+        s.nblk = dim == 2 ? 0
+                          : (mesh->K + dgtd::elements_per_block(mesh->order) - 1) / dgtd::elements_per_block(mesh->order);
         s.bytes = bytes;
-        s.dt = dt > 0.0 ? dt : 1.0 / (md.max_fscale * mesh->order * mesh->order);
+        s.dt = dt > 0.0 ? dt : 1.0 / (max_fscale * mesh->order * mesh->order);
         s.final_time = final_time;
         if (0.0 <= final_time - 1e-8 * s.dt) {
             s.state = dgtd::kSlotRunning;
@@ -590,7 +673,7 @@ int dgtd_batch_get_state(dgtd_batch* b, int32_t id, double* x)
         const auto& s = b->slots[id];
         if (s.state == dgtd::kSlotFree || s.state == dgtd_batch::kSlotReserved)
             return dgtd::api_fail(DGTD_E_STATE, "instance slot is free or still being set up");
-        const size_t ndofs = 6 * (size_t)dgtd::num_nodes(s.order) * s.K;
+        const size_t ndofs = 6 * (size_t)nodes(s.dim, s.order) * s.K;
         if (b->precision == 64) {
             hcheck(hipMemcpy(x, s.buf[s.cur], sizeof(double) * ndofs, hipMemcpyDeviceToHost), "D2H state");
         } else {

@@ -31,11 +31,11 @@ hipError_t launch_tri_stage_mfma(int order, const StageArgs<T>& a, int mode, int
 }
 
 template <typename T>
-hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s,
-                                   int cfg)
+hipError_t launch_batch_stage_mfma(int order, int dim, const BatchArgs<T>& b, const int* range, int grid,
+                                   hipStream_t s, int cfg)
 {
-    return cfg == 1 ? batch_mfma_launch_cfg<1, T>(order, b, range, grid, s)
-                    : batch_mfma_launch_cfg<0, T>(order, b, range, grid, s);
+    return cfg == 1 ? batch_mfma_launch_cfg<1, T>(order, dim, b, range, grid, s)
+                    : batch_mfma_launch_cfg<0, T>(order, dim, b, range, grid, s);
 }
 
 template <typename T>
@@ -67,8 +67,8 @@ int tri_mfma_waves_per_block(int order, int cfg)
     template bool mfma_available<T>(int, int);                                                            \
     template hipError_t launch_stage_mfma<T>(int, const StageArgs<T>&, int, int, hipStream_t, int);       \
     template hipError_t launch_tri_stage_mfma<T>(int, const StageArgs<T>&, int, int, hipStream_t, int);   \
-    template hipError_t launch_batch_stage_mfma<T>(int, const BatchArgs<T>&, const int*, int, hipStream_t, \
-                                                   int);                                                  \
+    template hipError_t launch_batch_stage_mfma<T>(int, int, const BatchArgs<T>&, const int*, int,         \
+                                                   hipStream_t, int);                                     \
     template int mfma_blocks_per_cu<T>(int, int);                                                         \
     template int tri_mfma_blocks_per_cu<T>(int, int);                                                     \
     template int mfma_waves_per_block<T>(int, int);                                                       \

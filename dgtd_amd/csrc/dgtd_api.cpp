@@ -1320,26 +1320,60 @@ int dgtd_op_last_kernel_us(const dgtd_op* op, double* avg_us, int32_t* launches)
 }  // extern "C"
 
 namespace dgtd {
-// The fastest matrix-core configuration of the LSERK stage for tetrahedra of
-// this order and precision at about K elements: from the process cache, else
-// by tuning an operator on a PEC cube of about K tetrahedra (n^3 x 6, n in
-// 4..32), which fills the cache.  Used by the batch engine, whose instances
+// The fastest matrix-core configuration of the LSERK stage for elements of
+// this shape (dim 3 tetrahedra, 2 triangles), order and precision at about K
+// elements: from the process cache, else by tuning an operator on a PEC cube
+// or square of about K elements, which fills the cache.  Used by the batch engine, whose instances
 // cannot be timed in place; it keeps its own rule for the per-node or
 // matrix-core choice (that choice changes the arithmetic, the configuration
 // does not), so an instance's result is the same alone and in any mix.
 // Returns false when no matrix-core configuration was timed.
-bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg)
+bool tuned_mfma_config(int dim, int order, int precision, int64_t K, int* cfg)
 {
     TuneChoice choice;
-    if (!tune_cache_get(3, order, precision, kModeLserk, K, choice)) {
-        int n = (int)std::lround(std::cbrt((double)K / 6.0));
-        n = std::min(32, std::max(4, n));
-        const int8_t pec[6] = {DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC};
-        const size_t nk = 6 * (size_t)n * n * n;
-        std::vector<double> vx(3 * (size_t)(n + 1) * (n + 1) * (n + 1));
-        std::vector<int32_t> EToV(4 * nk);
-        std::vector<int8_t> fbc(4 * nk);
-        if (dgtd_cube_mesh(n, 1.0, pec, vx.data(), EToV.data(), fbc.data()) != DGTD_OK) return false;
+    if (!tune_cache_get(dim, order, precision, kModeLserk, K, choice)) {
+        // a PEC cube of n^3 x 6 tetrahedra (n in 4..32), or a PEC square of
+        // n^2 x 2 triangles (n in 8..724)
+        std::vector<double> vx;
+        std::vector<int32_t> EToV;
+        std::vector<int8_t> fbc;
+        size_t nk = 0;
+        if (dim == 2) {
+            const int n = std::min(724, std::max(8, (int)std::lround(std::sqrt((double)K / 2.0))));
+            nk = 2 * (size_t)n * n;
+            vx.resize(3 * (size_t)(n + 1) * (n + 1));
+            for (int j = 0; j <= n; ++j)
+                for (int i = 0; i <= n; ++i) {
+                    const size_t v = (size_t)j * (n + 1) + i;
+                    vx[3 * v] = (double)i / n;
+                    vx[3 * v + 1] = (double)j / n;
+                    vx[3 * v + 2] = 0.0;
+                }
+            EToV.resize(3 * nk);
+            fbc.assign(3 * nk, 0);
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    const int32_t a = j * (n + 1) + i, b = a + 1, c = a + n + 2, d = a + n + 1;
+                    const size_t t = 2 * ((size_t)j * n + i);  // two counter-clockwise triangles
+                    const int32_t lo[3] = {a, b, c}, up[3] = {a, c, d};
+                    for (int k = 0; k < 3; ++k) {
+                        EToV[3 * t + k] = lo[k];
+                        EToV[3 * t + 3 + k] = up[k];
+                    }
+                    if (j == 0) fbc[3 * t] = DGTD_BC_PEC;              // a-b on y = 0
+                    if (i == n - 1) fbc[3 * t + 1] = DGTD_BC_PEC;      // b-c on x = 1
+                    if (j == n - 1) fbc[3 * t + 3 + 1] = DGTD_BC_PEC;  // c-d on y = 1
+                    if (i == 0) fbc[3 * t + 3 + 2] = DGTD_BC_PEC;      // d-a on x = 0
+                }
+        } else {
+            const int n = std::min(32, std::max(4, (int)std::lround(std::cbrt((double)K / 6.0))));
+            const int8_t pec[6] = {DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC, DGTD_BC_PEC};
+            nk = 6 * (size_t)n * n * n;
+            vx.resize(3 * (size_t)(n + 1) * (n + 1) * (n + 1));
+            EToV.resize(4 * nk);
+            fbc.resize(4 * nk);
+            if (dgtd_cube_mesh(n, 1.0, pec, vx.data(), EToV.data(), fbc.data()) != DGTD_OK) return false;
+        }
         dgtd_mesh_desc m{};
         m.K = (int32_t)nk;
         m.order = order;
@@ -1347,6 +1381,7 @@ bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg)
         m.nv = (int32_t)(vx.size() / 3);
         m.EToV = EToV.data();
         m.face_bc = fbc.data();
+        m.dim = dim;
         dgtd_opts o{};
         o.alpha = 1.0;
         o.precision = precision;
@@ -1357,10 +1392,10 @@ bool tuned_mfma_config(int order, int precision, int64_t K, int* cfg)
         o.device = dev;
         dgtd_op* op = nullptr;
         if (dgtd_op_create(&m, &o, &op) != DGTD_OK) return false;
-        const bool got = tune_cache_get(3, order, precision, kModeLserk, (int64_t)nk, choice);
+        const bool got = tune_cache_get(dim, order, precision, kModeLserk, (int64_t)nk, choice);
         dgtd_op_destroy(op);
         if (!got) return false;
-        tune_cache_put(3, order, precision, kModeLserk, K, choice);
+        tune_cache_put(dim, order, precision, kModeLserk, K, choice);
     }
     if (choice.best_mfma < 1) return false;
     *cfg = (choice.best_mfma - 1) / 2;

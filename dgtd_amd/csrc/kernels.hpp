@@ -112,7 +112,8 @@ hipError_t mfma_launch_cfg(int order, int dim, const StageArgs<T>& a, int mode, 
 template <typename T>
 struct BatchArgs;
 template <int CFG, typename T>
-hipError_t batch_mfma_launch_cfg(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s);
+hipError_t batch_mfma_launch_cfg(int order, int dim, const BatchArgs<T>& b, const int* range, int grid,
+                                 hipStream_t s);
 
 template <typename T>
 MfmaInfo mfma_info(int order, int dim, int cfg);
@@ -168,17 +169,20 @@ struct BatchArgs {
     const T* mats[kMaxBatchOrder + 1];       // per-node kernel layout
     const T* mats_mfma[kMaxBatchOrder + 1];  // matrix-core kernel layout
     const int* tabs[kMaxBatchOrder + 1];
+    const T* mats_mfma_tri[kMaxBatchOrder + 1];  // the same for triangles
+    const int* tabs_tri[kMaxBatchOrder + 1];
     int stage;
     T ca, cb;
 };
 
 template <typename T>
 hipError_t launch_batch_stage(const BatchArgs<T>& b, int grid, size_t smem, hipStream_t s);
-// Matrix-core batch stage for the instances of one order: items (instance,
-// 8-element task) in [range[0], range[1]) of b.items (range in device memory).
+// Matrix-core batch stage for the instances of one element shape (dim 3
+// tetrahedra, 2 triangles) and order: items (instance, 8-element task) in
+// [range[0], range[1]) of b.items (range in device memory).
 template <typename T>
-hipError_t launch_batch_stage_mfma(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s,
-                                   int cfg);
+hipError_t launch_batch_stage_mfma(int order, int dim, const BatchArgs<T>& b, const int* range, int grid,
+                                   hipStream_t s, int cfg);
 // Whether the matrix-core variant exists for this order, precision and
 // configuration (its LDS tiles fit).
 template <typename T>

@@ -716,15 +716,18 @@ __global__ __launch_bounds__((MWaves<P, T, DIM, CFG>::W * 64)) void dg_stage_mfm
         mfma_task<P, T, MODE, DIM, CFG>(a, L, a.order ? a.order[tix] : tix);
 }
 
-// Batch variant: the running instances of order P, one (instance, task) work
-// item per wave-task, items [range[0], range[1]) of the list; stage `b.stage`
-// of LSERK4 with the instance's own dt, buffers and geometry.
-template <int P, typename T, int CFG>
-__global__ __launch_bounds__((MWaves<P, T, 3, CFG>::W * 64)) void dg_batch_stage_mfma(BatchArgs<T> b, const int* range)
+// Batch variant: the running instances of element shape DIM and order P, one
+// (instance, task) work item per wave-task, items [range[0], range[1]) of the
+// list; stage `b.stage` of LSERK4 with the instance's own dt, buffers and
+// geometry.
+template <int P, typename T, int DIM, int CFG>
+__global__ __launch_bounds__((MWaves<P, T, DIM, CFG>::W * 64)) void dg_batch_stage_mfma(BatchArgs<T> b,
+                                                                                        const int* range)
 {
-    constexpr int WAVES = MWaves<P, T, 3, CFG>::W;
+    constexpr int WAVES = MWaves<P, T, DIM, CFG>::W;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const MfmaLds<P, T> L = mfma_load_ref<P, T, 3, CFG>(b.mats_mfma[P], b.tabs[P], smem_raw);
+    const MfmaLds<P, T, DIM> L = mfma_load_ref<P, T, DIM, CFG>(DIM == 3 ? b.mats_mfma[P] : b.mats_mfma_tri[P],
+                                                               DIM == 3 ? b.tabs[P] : b.tabs_tri[P], smem_raw);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int begin = range[0], end = range[1];
     for (int it = begin + blockIdx.x * WAVES + wave; it < end; it += gridDim.x * WAVES) {
@@ -750,7 +753,7 @@ __global__ __launch_bounds__((MWaves<P, T, 3, CFG>::W * 64)) void dg_batch_stage
         a.geo = uniform(I.geo);
         a.conn = uniform(I.conn);
         a.unstable = b.stage == 4 ? &b.inst[inst].unstable : nullptr;  // the step's last stage
-        mfma_task<P, T, kModeLserk, 3, CFG>(a, L, blk);
+        mfma_task<P, T, kModeLserk, DIM, CFG>(a, L, blk);
     }
 }
 
@@ -775,9 +778,8 @@ int mfma_prepare()
     mfma_set_attr<P, T, kModeRk2, DIM, CFG>();
     mfma_set_attr<P, T, kModeRk3, DIM, CFG>();
     mfma_set_attr<P, T, kModeRk4, DIM, CFG>();
-    if constexpr (DIM == 3)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T, CFG>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)W::bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dg_batch_stage_mfma<P, T, DIM, CFG>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)W::bytes);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dg_stage_mfma<P, T, kModeLserk, DIM, CFG>, W::W * 64,
                                                      W::bytes) != hipSuccess)
@@ -804,14 +806,15 @@ hipError_t mfma_launch(const StageArgs<T>& a, int mode, int grid, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int P, typename T, int CFG>
+template <int P, typename T, int DIM, int CFG>
 hipError_t batch_mfma_launch(const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
 {
-    using W = MWaves<P, T, 3, CFG>;
+    using W = MWaves<P, T, DIM, CFG>;
     if constexpr (!W::ok) {
         return hipErrorInvalidConfiguration;
     } else {
-        hipLaunchKernelGGL((dg_batch_stage_mfma<P, T, CFG>), dim3(grid), dim3(W::W * 64), W::bytes, s, b, range);
+        hipLaunchKernelGGL((dg_batch_stage_mfma<P, T, DIM, CFG>), dim3(grid), dim3(W::W * 64), W::bytes, s, b,
+                           range);
         return hipGetLastError();
     }
 }
@@ -859,9 +862,11 @@ hipError_t mfma_launch_cfg(int order, int dim, const StageArgs<T>& a, int mode, 
 }
 
 template <int CFG, typename T>
-hipError_t batch_mfma_launch_cfg(int order, const BatchArgs<T>& b, const int* range, int grid, hipStream_t s)
+hipError_t batch_mfma_launch_cfg(int order, int dim, const BatchArgs<T>& b, const int* range, int grid,
+                                 hipStream_t s)
 {
-#define DGTD_BLAUNCH(P) batch_mfma_launch<P, T, CFG>(b, range, grid, s)
+#define DGTD_BLAUNCH(P) \
+    (dim == 2 ? batch_mfma_launch<P, T, 2, CFG>(b, range, grid, s) : batch_mfma_launch<P, T, 3, CFG>(b, range, grid, s))
     DGTD_ORDER_SWITCH(order, DGTD_BLAUNCH, hipErrorInvalidValue)
 #undef DGTD_BLAUNCH
 }
@@ -870,6 +875,6 @@ hipError_t batch_mfma_launch_cfg(int order, const BatchArgs<T>& b, const int* ra
     template MfmaInfo mfma_info_cfg<CFG, T>(int, int);                                                       \
     template int mfma_prepare_cfg<CFG, T>(int, int);                                                         \
     template hipError_t mfma_launch_cfg<CFG, T>(int, int, const StageArgs<T>&, int, int, hipStream_t);       \
-    template hipError_t batch_mfma_launch_cfg<CFG, T>(int, const BatchArgs<T>&, const int*, int, hipStream_t);
+    template hipError_t batch_mfma_launch_cfg<CFG, T>(int, int, const BatchArgs<T>&, const int*, int, hipStream_t);
 
 }  // namespace dgtd

@@ -436,7 +436,12 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
 
 /* Admit a mesh with initial fields x0 (host FP64, reference layout; NULL =
  * zero), time step dt (<= 0: the Hesthaven rule 1/(max fscale p^2)) and final
- * time.  Returns DGTD_E_CAPACITY when the HBM budget or the slots are full. */
+ * time.  Returns DGTD_E_CAPACITY when the HBM budget or the slots are full.
+ * Tetrahedral (dim 0 or 3) and triangle (dim 2) meshes may share a batch:
+ * triangles of every order and tetrahedra of orders 2-5 run on the
+ * matrix-core stage, order-1 tetrahedra on the per-node stage.  TF/SF faces,
+ * conductivity and state buffers of 4 GiB or more are refused
+ * (DGTD_E_UNSUPPORTED). */
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
                    double final_time, int32_t* id);
 

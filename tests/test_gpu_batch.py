@@ -12,6 +12,7 @@ import pytest
 import dgtd_amd as D
 import meshgen as MG
 from oracle import oracle as O
+from oracle import oracle2d as O2
 
 pytestmark = pytest.mark.gpu
 
@@ -227,3 +228,75 @@ def test_operator_stability_flag(kernel):
     op.set_state(q0)  # cleared by a new state
     assert not op.unstable()
     op.close()
+
+
+TRI_CASES = [  # n, order, final_time, bc (x=0, x=L, y=0, y=L)
+    (3, 1, 0.30, "pec"),
+    (4, 2, 0.25, ["pec", "pmc", "sma", "pec"]),
+    (3, 3, 0.20, "sma"),
+    (5, 4, 0.15, "pmc"),
+    (2, 5, 0.10, "pec"),
+]
+
+
+def tri_case(n, order, bc, seed):
+    vx, E, fbc = MG.square_mesh(n, bc=bc, jitter=0.15, seed=seed)
+    rng = np.random.default_rng(seed)
+    ref = O2.Operator2D(vx, E, fbc, order)
+    c = rng.uniform(0.3, 0.7, 2)
+    r2 = (ref.pos[..., 0] - c[0]) ** 2 + (ref.pos[..., 1] - c[1]) ** 2
+    q0 = np.zeros((6,) + r2.shape)
+    for k in range(6):
+        q0[k] = rng.uniform(0.5, 1.0) * np.exp(-r2 / (2 * 0.15 ** 2))
+    return vx, E, fbc, ref, q0.reshape(-1)
+
+
+def test_triangles_and_tetrahedra_fp64_match_oracles():
+    """Triangle meshes of orders 1-5 (the matrix-core stage, DIM = 2) in one
+    batch with tetrahedral meshes: each instance takes the oracle's number of
+    steps and ends within 1e-11 of its oracle run (HesthavenEvolution::Mult on
+    triangles through the 2D oracle)."""
+    b = D.Batch(precision=64, max_instances=16)
+    runs = []
+    for k, (n, order, T, bc) in enumerate(TRI_CASES):
+        vx, E, fbc, ref, q0 = tri_case(n, order, bc, 50 + k)
+        i = b.add(vx, E, fbc, order, x0=q0, final_time=T)
+        dt = b.info(i)["dt"]
+        assert dt == pytest.approx(1.0 / (ref.max_fscale() * order ** 2), rel=1e-14)
+        x, steps, t = ref.run(q0, dt, T, "lserk4")
+        runs.append((i, x, steps))
+    for k, (n, order, T, bc) in enumerate(CASES[1:3]):
+        vx, E, fbc, ref, q0 = case_input(n, order, bc, 60 + k)
+        i = b.add(vx, E, fbc, order, x0=q0, final_time=T)
+        x = q0.copy()
+        steps, t = ref.run(x, b.info(i)["dt"], T, "lserk4")
+        runs.append((i, x, steps))
+    b.run(poll_every=4)
+    for i, x, steps in runs:
+        info = b.info(i)
+        assert info["state"] == "done" and info["steps"] == steps
+        assert rel(b.get_state(i), x) <= 1e-11, i
+
+
+def test_triangle_alone_equals_in_mix_fp32():
+    n, order, T, bc = 6, 4, 0.2, "pec"
+    vx, E, fbc, ref, q0 = tri_case(n, order, bc, 70)
+    alone = D.Batch(precision=32, max_instances=1)
+    a = alone.add(vx, E, fbc, order, x0=q0, final_time=T)
+    alone.run(poll_every=3)
+    xa = alone.get_state(a)
+    alone.close()
+    mix = D.Batch(precision=32)
+    for k, (n2, o2, T2, bc2) in enumerate(TRI_CASES):
+        vx2, E2, fbc2, _, q2 = tri_case(n2, o2, bc2, 80 + k)
+        mix.add(vx2, E2, fbc2, o2, x0=q2, final_time=T2)
+    for k, (n2, o2, T2, bc2) in enumerate(CASES):
+        vx2, E2, fbc2, _, q2 = case_input(n2, o2, bc2, 90 + k)
+        mix.add(vx2, E2, fbc2, o2, x0=q2, final_time=T2)
+    m = mix.add(vx, E, fbc, order, x0=q0, final_time=T)
+    mix.run(poll_every=5)
+    np.testing.assert_array_equal(mix.get_state(m), xa)
+    g64 = D.Batch(precision=64)
+    r = g64.add(vx, E, fbc, order, x0=q0, final_time=T)
+    g64.run()
+    assert rel(xa, g64.get_state(r)) <= 1e-5

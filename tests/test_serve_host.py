@@ -85,3 +85,23 @@ def test_failed_solve_is_retired_and_counted(monkeypatch):
     assert [recs[i].status for i in range(3)] == ["done", "failed", "done"]
     summ = S.summarize(recs, srv.wall)
     assert summ["completed"] == 2 and summ["failed"] == 1
+
+
+def test_footprint_counts_the_element_shape():
+    """Admission measure per instance: three state buffers of 6 Np K values
+    plus the geometry and connectivity records, for tetrahedra (27 values and
+    8 ints per element) and triangles (15 values and 6 ints)."""
+    import meshgen as MG
+
+    vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
+    K = E.shape[0]
+    for order, prec in ((3, 32), (5, 64)):
+        s = prec // 8
+        Np = (order + 1) * (order + 2) * (order + 3) // 6
+        assert footprint(vx, E, fbc, order, prec) == 3 * 6 * Np * K * s + K * 27 * s + K * 8 * 4
+    vx, E, fbc = MG.square_mesh(3, bc="pec")
+    K = E.shape[0]
+    for order, prec in ((1, 32), (4, 64)):
+        s = prec // 8
+        Np = (order + 1) * (order + 2) // 2
+        assert footprint(vx, E, fbc, order, prec) == 3 * 6 * Np * K * s + K * 15 * s + K * 6 * 4
