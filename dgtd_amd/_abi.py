@@ -128,6 +128,8 @@ SIGNATURES = {
     "dgtd_batch_destroy": (None, [c_void_p]),
     "dgtd_batch_footprint": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, _i64p]),
     "dgtd_batch_add": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc), _dp, c_double, c_double, _i32p]),
+    "dgtd_batch_add_tfsf": (c_int32, [c_void_p, ctypes.POINTER(MeshDesc), _dp, c_double, c_double,
+                                      ctypes.POINTER(PlaneWave), c_int32, _i32p]),
     "dgtd_batch_step": (c_int32, [c_void_p, c_int32]),
     "dgtd_batch_poll": (c_int32, [c_void_p, _i32p, c_int32, _i32p, _i32p]),
     "dgtd_batch_info": (c_int32, [c_void_p, c_int32, _dp, _i32p, _i32p, _dp]),

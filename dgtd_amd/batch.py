@@ -44,13 +44,17 @@ class Batch:
     def __del__(self):
         self.close()
 
-    def add(self, vx, EToV, face_bc, order, x0=None, dt=0.0, final_time=1.0, eps=None, mu=None):
-        """Admit one mesh; raises CapacityError when the HBM budget or the slots are full."""
+    def add(self, vx, EToV, face_bc, order, x0=None, dt=0.0, final_time=1.0, eps=None, mu=None, planewaves=None):
+        """Admit one mesh (tetrahedra or triangles); planewaves: dgtd_amd.planewave
+        sources injected on its TF/SF faces (code 7).  Raises CapacityError when
+        the HBM budget or the slots are full."""
         m = _Mesh(vx, EToV, face_bc, order, eps, mu)
         x = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         i = ctypes.c_int32()
-        rc = lib().dgtd_batch_add(self._h, ctypes.byref(m.desc), dptr(x), float(dt), float(final_time),
-                                  ctypes.byref(i))
+        waves = list(planewaves or [])
+        arr = (_abi.PlaneWave * max(1, len(waves)))(*waves)
+        rc = lib().dgtd_batch_add_tfsf(self._h, ctypes.byref(m.desc), dptr(x), float(dt), float(final_time),
+                                       arr, len(waves), ctypes.byref(i))
         if rc == _abi.DGTD_E_CAPACITY:
             raise CapacityError(rc, lib().dgtd_last_error().decode())
         check(rc)

@@ -19,6 +19,7 @@
 #include "packing.hpp"
 #include "reftet.hpp"
 #include "tri.hpp"
+#include "tfsf_host.hpp"
 
 namespace dgtd {
 int api_fail(int code, const std::string& msg);  // dgtd_api.cpp
@@ -42,17 +43,37 @@ const double kA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 /
 const double kB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                       2277821191437.0 / 14882151754819.0};
+const double kC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                      2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
 int nodes(int dim, int order) { return dim == 2 ? dgtd::tri_num_nodes(order) : dgtd::num_nodes(order); }
 
-// Device bytes of an instance: three state buffers, geometry and connectivity records.
-int64_t footprint(int dim, int K, int order, int precision)
+// Device bytes of an instance: three state buffers, geometry and connectivity
+// records, and the records of its ntf elements owning TF/SF faces (id, four
+// face signs, face-node positions, FP64 geometry).
+int64_t footprint(int dim, int K, int order, int precision, int64_t ntf = 0)
 {
     const int64_t s = precision == 64 ? 8 : 4;
     const int64_t Np = nodes(dim, order);
     const int64_t geo = dim == 2 ? dgtd::kTriGeoStride : dgtd::kGeoStride;
     const int64_t conn = dim == 2 ? dgtd::kTriConnStride : dgtd::kConnStride;
-    return 3 * 6 * Np * K * s + (int64_t)K * geo * s + (int64_t)K * conn * 4;
+    const int64_t nfn = dim == 2 ? 3 * (order + 1) : 4 * (order + 1) * (order + 2) / 2;
+    return 3 * 6 * Np * K * s + (int64_t)K * geo * s + (int64_t)K * conn * 4 +
+           ntf * (5 * 4 + nfn * 3 * 8 + geo * 8);
+}
+
+// Elements with a TF/SF face code.
+int64_t tfsf_elements(const dgtd_mesh_desc* mesh, int dim)
+{
+    int64_t n = 0;
+    if (mesh->face_bc)
+        for (int e = 0; e < mesh->K; ++e)
+            for (int f = 0; f <= dim; ++f)
+                if (mesh->face_bc[(size_t)(dim + 1) * e + f] == DGTD_BC_TFSF) {
+                    ++n;
+                    break;
+                }
+    return n;
 }
 
 }  // namespace
@@ -90,6 +111,14 @@ struct dgtd_batch {
     void* d_mats_tri[dgtd::kMaxBatchOrder + 1] = {};
     int* d_tabs_tri[dgtd::kMaxBatchOrder + 1] = {};
     int grid_tri[dgtd::kMaxBatchOrder + 1] = {};
+    // TF/SF injection: LIFT per shape and order (FP64), the work list of
+    // (instance, TF/SF element record) items and its device count
+    double* d_lift[dgtd::kMaxBatchOrder + 1] = {};
+    double* d_lift_tri[dgtd::kMaxBatchOrder + 1] = {};
+    dgtd::BatchItem* d_tf_items = nullptr;
+    int tf_cap = 0;
+    int* d_tf_count = nullptr;
+    bool cap_tf = false;  // the captured step launches the injection
     bool tuned_tri[dgtd::kMaxBatchOrder + 1] = {};
     int cfg_tri[dgtd::kMaxBatchOrder + 1] = {};
     int num_cu = 1;
@@ -102,6 +131,8 @@ struct dgtd_batch {
         int dim = 3, order = 0, K = 0, nblk = 0;
         int64_t bytes = 0;
         void* buf[5] = {};  // q0, q1, res, geo, conn
+        void* tf_buf[5] = {};  // TF/SF element ids, face signs, node positions, geometry, waves
+        int tf_n = 0, tf_nwaves = 0;
         bool reported = false;
         // host copy of the device record (authoritative only while dirty)
         double t = 0, dt = 0, dt_step = 0, final_time = 0;
@@ -178,9 +209,13 @@ struct dgtd_batch {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (auto& s : slots)
+        for (auto& s : slots) {
             for (void* p : s.buf)
                 if (p) (void)hipFree(p);
+            for (void* p : s.tf_buf)
+                if (p) (void)hipFree(p);
+        }
+        for (void* p : {(void*)d_tf_items, (void*)d_tf_count}) if (p) (void)hipFree(p);
         for (void* p : {d_inst, (void*)d_items, (void*)d_count, (void*)d_ranges}) if (p) (void)hipFree(p);
         for (int o = 0; o <= dgtd::kMaxBatchOrder; ++o) {
             if (d_mats[o]) (void)hipFree(d_mats[o]);
@@ -188,6 +223,8 @@ struct dgtd_batch {
             if (d_tabs[o]) (void)hipFree(d_tabs[o]);
             if (d_mats_tri[o]) (void)hipFree(d_mats_tri[o]);
             if (d_tabs_tri[o]) (void)hipFree(d_tabs_tri[o]);
+            if (d_lift[o]) (void)hipFree(d_lift[o]);
+            if (d_lift_tri[o]) (void)hipFree(d_lift_tri[o]);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -236,6 +273,14 @@ struct dgtd_batch {
             r.dt = s.dt;
             r.dt_step = s.dt_step;
             r.final_time = s.final_time;
+            r.dim = s.dim;
+            r.tf_elem = static_cast<const int*>(s.tf_buf[0]);
+            r.tf_sign = static_cast<const int*>(s.tf_buf[1]);
+            r.tf_xyz = static_cast<const double*>(s.tf_buf[2]);
+            r.tf_geo = static_cast<const double*>(s.tf_buf[3]);
+            r.tf_waves = static_cast<const dgtd::TfsfWave*>(s.tf_buf[4]);
+            r.tf_n = s.tf_n;
+            r.tf_nwaves = s.tf_nwaves;
         }
         hcheck(hipMemcpyAsync(d_inst, h.data(), sizeof(h[0]) * max_slots, hipMemcpyHostToDevice, stream), "H2D table");
         // work list: running instances' element blocks for the per-node stage
@@ -284,16 +329,35 @@ struct dgtd_batch {
                    "H2D items");
         hcheck(hipMemcpyAsync(d_count, &nvalu, sizeof(int), hipMemcpyHostToDevice, stream), "H2D count");
         hcheck(hipMemcpyAsync(d_ranges, ranges, sizeof(ranges), hipMemcpyHostToDevice, stream), "H2D ranges");
+        // TF/SF work list: the running instances' elements owning TF/SF faces
+        std::vector<dgtd::BatchItem> tf;
+        for (int i = 0; i < max_slots; ++i)
+            if (slots[i].state == dgtd::kSlotRunning && slots[i].tf_nwaves > 0)
+                for (int k = 0; k < slots[i].tf_n; ++k) tf.push_back({i, k});
+        const int ntf = (int)tf.size();
+        if (ntf > tf_cap) {
+            if (d_tf_items) hcheck(hipFree(d_tf_items), "hipFree");
+            tf_cap = std::max(ntf, 2 * tf_cap);
+            hcheck(hipMalloc(&d_tf_items, sizeof(dgtd::BatchItem) * tf_cap), "hipMalloc TF/SF items");
+            if (graph) {
+                hcheck(hipGraphExecDestroy(graph), "hipGraphExecDestroy");
+                graph = nullptr;
+            }
+        }
+        if (ntf)
+            hcheck(hipMemcpyAsync(d_tf_items, tf.data(), sizeof(tf[0]) * ntf, hipMemcpyHostToDevice, stream),
+                   "H2D TF/SF items");
+        hcheck(hipMemcpyAsync(d_tf_count, &ntf, sizeof(int), hipMemcpyHostToDevice, stream), "H2D TF/SF count");
         hcheck(hipStreamSynchronize(stream), "sync");  // host vectors die here
-        bool recapture = max_order != graph_order || !graph;
+        bool recapture = max_order != graph_order || !graph || (ntf > 0) != cap_tf;
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o)
             recapture = recapture || use_mfma(o) != cap_mfma[o] || cfg[o] != cap_cfg[o] ||
                         tuned_tri[o] != cap_tri[o] || cfg_tri[o] != cap_cfg_tri[o];
-        if (recapture) capture<T>(max_order);
+        if (recapture) capture<T>(max_order, ntf > 0);
     }
 
     template <typename T>
-    void capture(int max_order)
+    void capture(int max_order, bool with_tf)
     {
         dgtd::TraceRange tr("dgtd batch capture");
         if (graph) {
@@ -337,6 +401,21 @@ struct dgtd_batch {
                 if (tuned_tri[o])
                     err = dgtd::launch_batch_stage_mfma<T>(o, 2, a, d_ranges + kTriRanges + 2 * o, grid_tri[o], stream,
                                                            cfg_tri[o]);
+            if (with_tf && err == hipSuccess) {
+                // res += dt S, q += b dt S at the stage time t + c dt (every instance its own t, dt)
+                dgtd::BatchTfsfArgs<T> ta{};
+                ta.inst = a.inst;
+                ta.items = d_tf_items;
+                ta.count = d_tf_count;
+                for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
+                    ta.lift[o] = d_lift[o];
+                    ta.lift_tri[o] = d_lift_tri[o];
+                }
+                ta.stage = s;
+                ta.c = kC[s];
+                ta.b = kB[s];
+                err = dgtd::launch_batch_tfsf<T>(ta, 4 * num_cu, stream);
+            }
         }
         if (err == hipSuccess) err = dgtd::launch_batch_advance<T>(a.inst, max_slots, stream);
         hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -346,6 +425,7 @@ struct dgtd_batch {
         (void)hipGraphDestroy(g);
         hcheck(e3, "hipGraphInstantiate");
         graph_order = max_order;
+        cap_tf = with_tf;
         for (int o = 1; o <= dgtd::kMaxBatchOrder; ++o) {
             cap_mfma[o] = use_mfma(o);
             cap_cfg[o] = cfg[o];
@@ -393,6 +473,22 @@ void upload_instance(dgtd_batch* b, dgtd_batch::Slot& s, const MD& md, const dou
     hcheck(hipMemcpy(s.buf[4], md.conn.data(), sizeof(int) * md.conn.size(), hipMemcpyHostToDevice), "H2D conn");
 }
 
+// The instance's TF/SF element records and plane waves (device copies).
+void upload_tfsf(dgtd_batch::Slot& s, const dgtd::TfsfRecords& r, const std::vector<dgtd::TfsfWave>& waves)
+{
+    auto put = [](void*& dst, const void* src, size_t bytes) {
+        hcheck(hipMalloc(&dst, bytes), "hipMalloc TF/SF");
+        hcheck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "H2D TF/SF");
+    };
+    put(s.tf_buf[0], r.elems.data(), sizeof(int) * r.elems.size());
+    put(s.tf_buf[1], r.signs.data(), sizeof(int) * r.signs.size());
+    put(s.tf_buf[2], r.xyz.data(), sizeof(double) * r.xyz.size());
+    put(s.tf_buf[3], r.geo.data(), sizeof(double) * r.geo.size());
+    put(s.tf_buf[4], waves.data(), sizeof(dgtd::TfsfWave) * waves.size());
+    s.tf_n = (int)r.elems.size();
+    s.tf_nwaves = (int)waves.size();
+}
+
 template <typename T>
 void upload_refs(dgtd_batch* b)
 {
@@ -410,6 +506,9 @@ void upload_refs(dgtd_batch* b)
         }
         hcheck(hipMalloc(&b->d_tabs[o], sizeof(int) * tabs.size()), "hipMalloc tabs");
         hcheck(hipMemcpy(b->d_tabs[o], tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice), "H2D tabs");
+        hcheck(hipMalloc(&b->d_lift[o], sizeof(double) * R.LIFT.size()), "hipMalloc LIFT");
+        hcheck(hipMemcpy(b->d_lift[o], R.LIFT.data(), sizeof(double) * R.LIFT.size(), hipMemcpyHostToDevice),
+               "H2D LIFT");
         if (dgtd::tri_mfma_blocks_per_cu<T>(o, 0) > 0) {  // triangles
             const dgtd::RefTri Rt = dgtd::build_reftri(o);
             const std::vector<T> mt = dgtd::pack_mats_mfma<T>(Rt);
@@ -419,6 +518,10 @@ void upload_refs(dgtd_batch* b)
             hcheck(hipMemcpy(b->d_mats_tri[o], mt.data(), sizeof(T) * mt.size(), hipMemcpyHostToDevice), "H2D mats");
             hcheck(hipMalloc(&b->d_tabs_tri[o], sizeof(int) * tt.size()), "hipMalloc tabs");
             hcheck(hipMemcpy(b->d_tabs_tri[o], tt.data(), sizeof(int) * tt.size(), hipMemcpyHostToDevice), "H2D tabs");
+            hcheck(hipMalloc(&b->d_lift_tri[o], sizeof(double) * Rt.LIFT.size()), "hipMalloc LIFT");
+            hcheck(hipMemcpy(b->d_lift_tri[o], Rt.LIFT.data(), sizeof(double) * Rt.LIFT.size(),
+                             hipMemcpyHostToDevice),
+                   "H2D LIFT");
         }
     }
 }
@@ -437,7 +540,8 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
 {
     if (!mesh || !bytes) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
-    *bytes = footprint(mesh->dim == 2 ? 2 : 3, mesh->K, mesh->order, precision);
+    const int dim = mesh->dim == 2 ? 2 : 3;
+    *bytes = footprint(dim, mesh->K, mesh->order, precision, tfsf_elements(mesh, dim));
     return DGTD_OK;
 }
 
@@ -468,6 +572,7 @@ int dgtd_batch_create(const dgtd_batch_opts* opts, dgtd_batch** out)
         const size_t rec = b->precision == 64 ? sizeof(dgtd::BatchInst<double>) : sizeof(dgtd::BatchInst<float>);
         hcheck(hipMalloc(&b->d_inst, rec * b->max_slots), "hipMalloc table");
         hcheck(hipMalloc(&b->d_count, sizeof(int)), "hipMalloc count");
+        hcheck(hipMalloc(&b->d_tf_count, sizeof(int)), "hipMalloc TF/SF count");
         hcheck(hipMalloc(&b->d_ranges, sizeof(int) * 2 * dgtd_batch::kTriRanges), "hipMalloc ranges");
         b->item_cap = 1 << 16;
         hcheck(hipMalloc(&b->d_items, sizeof(dgtd::BatchItem) * b->item_cap), "hipMalloc items");
@@ -487,8 +592,20 @@ void dgtd_batch_destroy(dgtd_batch* b) { delete b; }
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt, double final_time,
                    int32_t* id)
 {
+    return dgtd_batch_add_tfsf(b, mesh, x0, dt, final_time, nullptr, 0, id);
+}
+
+int dgtd_batch_add_tfsf(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt, double final_time,
+                        const dgtd_planewave* waves, int32_t nwaves, int32_t* id)
+{
     dgtd::TraceRange tr("dgtd_batch_add");
-    if (!b || !mesh || !id) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    if (!b || !mesh || !id || nwaves < 0 || (nwaves > 0 && !waves))
+        return dgtd::api_fail(DGTD_E_ARG, "NULL argument or negative wave count");
+    std::vector<dgtd::TfsfWave> dwaves;
+    {
+        std::string err;
+        if (!dgtd::planewaves_device(waves, nwaves, dwaves, err)) return dgtd::api_fail(DGTD_E_ARG, err);
+    }
     const int dim = mesh->dim == 2 ? 2 : 3;
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return dgtd::api_fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
@@ -497,11 +614,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
     if (mesh->order < 1 || mesh->order > dgtd::kMaxBatchOrder)
         return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     if (!std::isfinite(final_time) || !std::isfinite(dt)) return dgtd::api_fail(DGTD_E_ARG, "bad dt or final_time");
-    if (mesh->face_bc)
-        for (size_t i = 0; i < (size_t)(dim + 1) * std::max(0, mesh->K); ++i)
-            if (mesh->face_bc[i] == DGTD_BC_TFSF)
-                return dgtd::api_fail(DGTD_E_UNSUPPORTED, "TF/SF faces are not supported by batched instances");
-    const int64_t bytes = footprint(dim, mesh->K, mesh->order, b->precision);
+    const int64_t bytes = footprint(dim, mesh->K, mesh->order, b->precision, tfsf_elements(mesh, dim));
     try {
         hcheck(hipSetDevice(b->device), "hipSetDevice");
         b->tune(dim, mesh->order, mesh->K);
@@ -549,6 +662,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
             max_fscale = md.max_fscale;
             if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
             else upload_instance<float>(b, s, md, x0, ndofs);
+            if (!md.tfsf.empty() && !dwaves.empty()) upload_tfsf(s, dgtd::tfsf_records(mesh, R, md.tfsf, md.geo), dwaves);
         } else {
             const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
             dgtd::MeshInput in;
@@ -563,6 +677,7 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
             max_fscale = md.max_fscale;
             if (b->precision == 64) upload_instance<double>(b, s, md, x0, ndofs);
             else upload_instance<float>(b, s, md, x0, ndofs);
+            if (!md.tfsf.empty() && !dwaves.empty()) upload_tfsf(s, dgtd::tfsf_records(mesh, R, md.tfsf, md.geo), dwaves);
         }
         s.dim = dim;
         s.order = mesh->order;
@@ -580,6 +695,8 @@ int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, 
         }
     } catch (const std::exception& e) {
         for (void*& p : s.buf)
+            if (p) { (void)hipFree(p); p = nullptr; }
+        for (void*& p : s.tf_buf)
             if (p) { (void)hipFree(p); p = nullptr; }
         std::lock_guard<std::mutex> lk(b->mu);
         b->slots[slot] = dgtd_batch::Slot{};
@@ -720,6 +837,8 @@ int dgtd_batch_remove(dgtd_batch* b, int32_t id)
         if (s.state == dgtd::kSlotFree || s.state == dgtd_batch::kSlotReserved)
             return dgtd::api_fail(DGTD_E_STATE, "instance slot is free or still being set up");
         for (void*& p : s.buf)
+            if (p) { hcheck(hipFree(p), "hipFree"); p = nullptr; }
+        for (void*& p : s.tf_buf)
             if (p) { hcheck(hipFree(p), "hipFree"); p = nullptr; }
         b->used -= s.bytes;
         s = dgtd_batch::Slot{};

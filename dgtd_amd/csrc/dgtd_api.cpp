@@ -20,6 +20,7 @@
 #include "fields.hpp"
 #include "kernels.hpp"
 #include "trace.hpp"
+#include "tfsf_host.hpp"
 #include "mesh.hpp"
 #include "packing.hpp"
 #include "reftet.hpp"
@@ -694,36 +695,12 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
         hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
     }
     if (!md.tfsf.empty()) {  // triangles owning TF/SF edges: edge signs, edge-node positions, records
-        std::vector<int> elems, signs;
-        std::vector<double> xyz, geo;
-        std::vector<int> slot(op->K, -1);
-        for (const auto& t : md.tfsf) {
-            const int e = t[0];
-            if (slot[e] < 0) {
-                slot[e] = (int)elems.size();
-                elems.push_back(e);
-                signs.insert(signs.end(), {0, 0, 0, 0});
-                geo.insert(geo.end(), md.geo.begin() + (size_t)dgtd::kTriGeoStride * e,
-                           md.geo.begin() + (size_t)dgtd::kTriGeoStride * (e + 1));
-                const int32_t* v = &mesh->EToV[3 * (size_t)e];
-                for (int m = 0; m < 3 * R.Nfp; ++m) {
-                    const int n = R.Fmask[m];
-                    const double a = R.xi[2 * n], b = R.xi[2 * n + 1];
-                    for (int d = 0; d < 3; ++d) {
-                        const double x0 = mesh->vx[3 * (size_t)v[0] + d];
-                        xyz.push_back(d == 2 ? 0.0
-                                             : x0 + (mesh->vx[3 * (size_t)v[1] + d] - x0) * a +
-                                                   (mesh->vx[3 * (size_t)v[2] + d] - x0) * b);
-                    }
-                }
-            }
-            signs[4 * (size_t)slot[e] + t[1]] = t[2];
-        }
-        op->ntfsf = (int)elems.size();
-        op->d_tfsf_elem = static_cast<int*>(upload(elems));
-        op->d_tfsf_sign = static_cast<int*>(upload(signs));
-        op->d_tfsf_xyz = static_cast<double*>(upload(xyz));
-        op->d_tfsf_geo = static_cast<double*>(upload(geo));
+        const dgtd::TfsfRecords r = dgtd::tfsf_records(mesh, R, md.tfsf, md.geo);
+        op->ntfsf = (int)r.elems.size();
+        op->d_tfsf_elem = static_cast<int*>(upload(r.elems));
+        op->d_tfsf_sign = static_cast<int*>(upload(r.signs));
+        op->d_tfsf_xyz = static_cast<double*>(upload(r.xyz));
+        op->d_tfsf_geo = static_cast<double*>(upload(r.geo));
         op->d_lift = static_cast<double*>(upload(R.LIFT));
     }
     hip_check(hipStreamSynchronize(op->stream), "sync");
@@ -865,36 +842,12 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
             hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
         }
         if (!md.tfsf.empty()) {  // elements owning TF/SF faces, their face signs, nodes and records
-            std::vector<int> elems, signs;
-            std::vector<double> xyz, geo;
-            std::vector<int> slot(op->K, -1);
-            for (const auto& t : md.tfsf) {
-                const int e = t[0];
-                if (slot[e] < 0) {
-                    slot[e] = (int)elems.size();
-                    elems.push_back(e);
-                    signs.insert(signs.end(), {0, 0, 0, 0});
-                    geo.insert(geo.end(), md.geo.begin() + (size_t)dgtd::kGeoStride * e,
-                               md.geo.begin() + (size_t)dgtd::kGeoStride * (e + 1));
-                    const int32_t* v = &mesh->EToV[4 * (size_t)e];
-                    for (int m = 0; m < 4 * op->Nfp; ++m) {
-                        const int n = op->R.Fmask[m];
-                        const double a = op->R.xi[3 * n], b = op->R.xi[3 * n + 1], c = op->R.xi[3 * n + 2];
-                        for (int d = 0; d < 3; ++d) {
-                            const double x0 = mesh->vx[3 * (size_t)v[0] + d];
-                            xyz.push_back(x0 + (mesh->vx[3 * (size_t)v[1] + d] - x0) * a +
-                                          (mesh->vx[3 * (size_t)v[2] + d] - x0) * b +
-                                          (mesh->vx[3 * (size_t)v[3] + d] - x0) * c);
-                        }
-                    }
-                }
-                signs[4 * (size_t)slot[e] + t[1]] = t[2];
-            }
-            op->ntfsf = (int)elems.size();
-            op->d_tfsf_elem = static_cast<int*>(upload(elems));
-            op->d_tfsf_sign = static_cast<int*>(upload(signs));
-            op->d_tfsf_xyz = static_cast<double*>(upload(xyz));
-            op->d_tfsf_geo = static_cast<double*>(upload(geo));
+            const dgtd::TfsfRecords r = dgtd::tfsf_records(mesh, op->R, md.tfsf, md.geo);
+            op->ntfsf = (int)r.elems.size();
+            op->d_tfsf_elem = static_cast<int*>(upload(r.elems));
+            op->d_tfsf_sign = static_cast<int*>(upload(r.signs));
+            op->d_tfsf_xyz = static_cast<double*>(upload(r.xyz));
+            op->d_tfsf_geo = static_cast<double*>(upload(r.geo));
             op->d_lift = static_cast<double*>(upload(op->R.LIFT));
         }
         hip_check(hipStreamSynchronize(op->stream), "sync");
@@ -1186,31 +1139,9 @@ int dgtd_op_norms(dgtd_op* op, double* out3)
 int dgtd_op_set_planewaves(dgtd_op* op, const dgtd_planewave* w, int32_t n)
 {
     if (!op || n < 0 || (n > 0 && !w)) return fail(DGTD_E_ARG, "NULL argument or negative count");
-    std::vector<dgtd::TfsfWave> dev((size_t)n);
-    for (int i = 0; i < n; ++i) {
-        const dgtd_planewave& p = w[i];
-        if (p.kind != DGTD_PLANEWAVE_GAUSSIAN && p.kind != DGTD_PLANEWAVE_MODULATED)
-            return fail(DGTD_E_ARG, "unknown plane-wave kind");
-        const double np = std::sqrt(p.polarization[0] * p.polarization[0] + p.polarization[1] * p.polarization[1] +
-                                    p.polarization[2] * p.polarization[2]);
-        const double nd = std::sqrt(p.propagation[0] * p.propagation[0] + p.propagation[1] * p.propagation[1] +
-                                    p.propagation[2] * p.propagation[2]);
-        if (!(np > 0) || !(nd > 0) || !(p.spread > 0) || !std::isfinite(p.mean) || !std::isfinite(p.frequency))
-            return fail(DGTD_E_ARG, "plane wave needs nonzero polarization and propagation and spread > 0");
-        dgtd::TfsfWave& d = dev[i];
-        for (int c = 0; c < 3; ++c) {  // Planewave normalises both (Function.h:331-340)
-            d.pol[c] = p.polarization[c] / np;
-            d.dir[c] = p.propagation[c] / nd;
-        }
-        for (int c = 0; c < 3; ++c) {  // H = k x pol for an electric-field plane wave (:365-374)
-            const int y = (c + 1) % 3, z = (c + 2) % 3;
-            d.kxp[c] = d.dir[y] * d.pol[z] - d.dir[z] * d.pol[y];
-        }
-        d.spread = p.spread;
-        d.mean = p.mean;
-        d.freq = p.frequency;
-        d.kind = p.kind;
-    }
+    std::vector<dgtd::TfsfWave> dev;
+    std::string err;
+    if (!dgtd::planewaves_device(w, n, dev, err)) return fail(DGTD_E_ARG, err);
     try {
         hip_check(hipSetDevice(op->device), "hipSetDevice");
         hip_check(hipStreamSynchronize(op->stream), "sync");

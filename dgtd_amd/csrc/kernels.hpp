@@ -144,17 +144,26 @@ constexpr int kMaxBatchOrder = 5;
 // last stage; kSlotCancelled: stopped by the host (dgtd_batch_cancel).
 enum BatchState : int { kSlotFree = 0, kSlotRunning = 1, kSlotDone = 2, kSlotFailed = 3, kSlotCancelled = 4 };
 
+struct TfsfWave;
+
 template <typename T>
 struct BatchInst {
     int order, K, nblk, state;  // state: BatchState
     int cur, steps;             // cur: which q buffer holds the state
     int unstable;               // set by the last stage of a step (report_unstable)
-    int pad1;
+    int dim;                    // 3 tetrahedra, 2 triangles
     T* q[2];
     T* res;
     const T* geo;
     const int* conn;
     double alpha, t, dt, dt_step, final_time;
+    // elements owning TF/SF faces (TfsfArgs layout) and the plane waves
+    const int* tf_elem;
+    const int* tf_sign;
+    const double* tf_xyz;
+    const double* tf_geo;
+    const TfsfWave* tf_waves;
+    int tf_n, tf_nwaves;
 };
 
 struct BatchItem {
@@ -230,6 +239,21 @@ struct TfsfArgs {
 
 template <typename T>
 hipError_t launch_tfsf(const TfsfArgs<T>& a, hipStream_t s);
+
+// TF/SF injection of a batch stage: items (instance, TF/SF element record)
+// of the running instances, *count of them (device).
+template <typename T>
+struct BatchTfsfArgs {
+    BatchInst<T>* inst;
+    const BatchItem* items;
+    const int* count;
+    const double* lift[kMaxBatchOrder + 1];      // tetrahedra: LIFT [Np][4 Nfp]
+    const double* lift_tri[kMaxBatchOrder + 1];  // triangles: LIFT [Np][3 Nfp]
+    int stage;
+    double c, b;  // the stage's time fraction and b coefficient
+};
+template <typename T>
+hipError_t launch_batch_tfsf(const BatchTfsfArgs<T>& a, int grid, hipStream_t s);
 
 // Elements per tile of the norms reduction: mass (Np^2), tile (6*TE*Np) and
 // reduction scratch (3*256) doubles stay within 60 KiB of LDS.

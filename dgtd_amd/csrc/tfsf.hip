@@ -21,24 +21,27 @@ namespace {
 
 constexpr int kTfsfMaxFaceNodes = 4 * 21;  // order 5
 
+// The injection of one element owning TF/SF faces (block-wide, 64 threads):
+// record k of the element lists, element e of a mesh with Np nodes per
+// element and N nodes per component.
 template <typename T>
-__global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
+__device__ void tfsf_element(int dim, int k, int e, const int* sign, const double* xyz, const double* geo,
+                             const double* lift, const TfsfWave* waves, int nwaves, int Np, int Nfp, size_t N,
+                             double alpha, double t, T* out, double c_out, T* res, double c_res,
+                             double (&flux)[6][kTfsfMaxFaceNodes])
 {
-    __shared__ double flux[6][kTfsfMaxFaceNodes];
-    const int k = blockIdx.x;
-    const int e = a.elem[k];
-    const bool tri = a.dim == 2;
-    const int Np = a.Np, Nfp = a.Nfp, Nf = (tri ? 3 : 4) * Nfp;
-    const double* g = a.geo + (size_t)k * (tri ? 15 : 27);
+    const bool tri = dim == 2;
+    const int Nf = (tri ? 3 : 4) * Nfp;
+    const double* g = geo + (size_t)k * (tri ? 15 : 27);
     for (int m = threadIdx.x; m < Nf; m += 64) {
         const int f = m / Nfp;
-        const int sg = a.sign[4 * k + f];
+        const int sg = sign[4 * k + f];
         double dE[3] = {0.0, 0.0, 0.0}, dH[3] = {0.0, 0.0, 0.0};
         if (sg != 0) {
-            const double* x = a.xyz + ((size_t)k * Nf + m) * 3;
-            for (int w = 0; w < a.nwaves; ++w) {
-                const TfsfWave& pw = a.waves[w];
-                const double u = pw.dir[0] * x[0] + pw.dir[1] * x[1] + pw.dir[2] * x[2] - a.t;
+            const double* x = xyz + ((size_t)k * Nf + m) * 3;
+            for (int w = 0; w < nwaves; ++w) {
+                const TfsfWave& pw = waves[w];
+                const double u = pw.dir[0] * x[0] + pw.dir[1] * x[1] + pw.dir[2] * x[2] - t;
                 const double d = u - pw.mean;
                 double mag = exp(-d * d / (2.0 * pw.spread * pw.spread));
                 if (pw.kind == 1) mag *= cos(2.0 * M_PI * pw.freq * d);
@@ -57,20 +60,54 @@ __global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
         const double ndH = n[0] * dH[0] + n[1] * dH[1] + n[2] * dH[2];
         for (int x = 0; x < 3; ++x) {
             const int y = (x + 1) % 3, z = (x + 2) % 3;
-            flux[x][m] = fs * (n[y] * dH[z] - n[z] * dH[y] + a.alpha * (dE[x] - ndE * n[x]));
-            flux[3 + x][m] = fs * (-n[y] * dE[z] + n[z] * dE[y] + a.alpha * (dH[x] - ndH * n[x]));
+            flux[x][m] = fs * (n[y] * dH[z] - n[z] * dH[y] + alpha * (dE[x] - ndE * n[x]));
+            flux[3 + x][m] = fs * (-n[y] * dE[z] + n[z] * dE[y] + alpha * (dH[x] - ndH * n[x]));
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < Np; i += 64) {
         for (int c = 0; c < 6; ++c) {
             double s = 0.0;
-            for (int m = 0; m < Nf; ++m) s += a.lift[(size_t)i * Nf + m] * flux[c][m];
+            for (int m = 0; m < Nf; ++m) s += lift[(size_t)i * Nf + m] * flux[c][m];
             s *= c < 3 ? g[tri ? 13 : 25] : g[tri ? 14 : 26];
-            const size_t idx = c * a.N + (size_t)e * Np + i;
-            a.out[idx] = (T)((double)a.out[idx] + a.c_out * s);
-            if (a.res) a.res[idx] = (T)((double)a.res[idx] + a.c_res * s);
+            const size_t idx = c * N + (size_t)e * Np + i;
+            out[idx] = (T)((double)out[idx] + c_out * s);
+            if (res) res[idx] = (T)((double)res[idx] + c_res * s);
         }
+    }
+    __syncthreads();  // flux is reused by the block's next element
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void tfsf_apply(TfsfArgs<T> a)
+{
+    __shared__ double flux[6][kTfsfMaxFaceNodes];
+    const int k = blockIdx.x;
+    tfsf_element<T>(a.dim, k, a.elem[k], a.sign, a.xyz, a.geo, a.lift, a.waves, a.nwaves, a.Np, a.Nfp, a.N,
+                    a.alpha, a.t, a.out, a.c_out, a.res, a.c_res, flux);
+}
+
+// Batch variant: items (instance, record k) over the running instances'
+// elements owning TF/SF faces, *count of them; stage `stage` of LSERK4 at the
+// instance's own time t + c dt_step, adding dt_step b S to the stage output
+// and dt_step S to the residual (as the single operator does).
+template <typename T>
+__global__ __launch_bounds__(64) void batch_tfsf_apply(BatchTfsfArgs<T> a)
+{
+    __shared__ double flux[6][kTfsfMaxFaceNodes];
+    const int n = *a.count;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const BatchItem w = a.items[it];
+        const BatchInst<T>& I = a.inst[w.inst];
+        if (I.state != kSlotRunning) continue;  // uniform across the block
+        const int dim = I.dim;
+        const int Np = dim == 2 ? (I.order + 1) * (I.order + 2) / 2 : (I.order + 1) * (I.order + 2) * (I.order + 3) / 6;
+        const int Nfp = dim == 2 ? I.order + 1 : (I.order + 1) * (I.order + 2) / 2;
+        const int src = I.cur ^ (a.stage & 1);
+        const double dt = I.dt_step;
+        tfsf_element<T>(dim, w.blk, I.tf_elem[w.blk], I.tf_sign, I.tf_xyz, I.tf_geo,
+                        dim == 2 ? a.lift_tri[I.order] : a.lift[I.order], I.tf_waves, I.tf_nwaves, Np, Nfp,
+                        (size_t)Np * I.K, I.alpha, I.t + a.c * dt, I.q[src ^ 1], a.b * dt, I.res, dt, flux);
     }
 }
 
@@ -85,7 +122,16 @@ hipError_t launch_tfsf(const TfsfArgs<T>& a, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_batch_tfsf(const BatchTfsfArgs<T>& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_tfsf_apply<T>, dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
 template hipError_t launch_tfsf<float>(const TfsfArgs<float>&, hipStream_t);
 template hipError_t launch_tfsf<double>(const TfsfArgs<double>&, hipStream_t);
+template hipError_t launch_batch_tfsf<float>(const BatchTfsfArgs<float>&, int, hipStream_t);
+template hipError_t launch_batch_tfsf<double>(const BatchTfsfArgs<double>&, int, hipStream_t);
 
 }  // namespace dgtd

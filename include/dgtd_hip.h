@@ -439,11 +439,18 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
  * time.  Returns DGTD_E_CAPACITY when the HBM budget or the slots are full.
  * Tetrahedral (dim 0 or 3) and triangle (dim 2) meshes may share a batch:
  * triangles of every order and tetrahedra of orders 2-5 run on the
- * matrix-core stage, order-1 tetrahedra on the per-node stage.  TF/SF faces,
- * conductivity and state buffers of 4 GiB or more are refused
- * (DGTD_E_UNSUPPORTED). */
+ * matrix-core stage, order-1 tetrahedra on the per-node stage.  Conductivity
+ * and state buffers of 4 GiB or more are refused (DGTD_E_UNSUPPORTED); TF/SF
+ * faces (code 7) inject nothing without plane waves (dgtd_batch_add_tfsf). */
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
                    double final_time, int32_t* id);
+
+/* dgtd_batch_add with plane waves injected on the mesh's TF/SF faces
+ * (HesthavenEvolution::evaluateTFSF, HesthavenEvolution.cpp:97-124; the same
+ * waves as dgtd_op_set_planewaves takes), at every stage of the instance's own
+ * LSERK4 step times. */
+int dgtd_batch_add_tfsf(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
+                        double final_time, const dgtd_planewave* waves, int32_t nwaves, int32_t* id);
 
 /* Launch nsteps time steps of all running instances (asynchronous). */
 int dgtd_batch_step(dgtd_batch* b, int32_t nsteps);

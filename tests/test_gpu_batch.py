@@ -300,3 +300,66 @@ def test_triangle_alone_equals_in_mix_fp32():
     r = g64.add(vx, E, fbc, order, x0=q0, final_time=T)
     g64.run()
     assert rel(xa, g64.get_state(r)) <= 1e-5
+
+
+def _tfsf_instances():
+    """A tetrahedral PEC/PMC waveguide with SMA ends and a TF/SF plane, and a
+    triangle strip with a TF/SF line (the single-operator TF/SF cases), each
+    with its oracle run over its own dt and final time."""
+    from test_gpu_tri import _tfsf_case
+    from test_tfsf import MEAN, SPREAD, waveguide
+
+    vx, E, fbc = waveguide(3)
+    ref = O.Operator(vx, E, fbc, 3)
+    ref.set_planewaves([dict(pol=(0, 0, 1), dir=(1, 0, 0), spread=SPREAD, mean=MEAN)])
+    dt = 0.5 / (ref.max_fscale() * 9)
+    x = np.zeros(ref.ndofs)
+    steps, _ = ref.run(x, dt, 1.2, "lserk4")
+    tet = (vx, E, fbc, 3, dt, 1.2, [D.planewave((0, 0, 1), (1, 0, 0), SPREAD, MEAN)], x, steps)
+    vx2, E2, bc2, waves, gw = _tfsf_case(order=3)
+    ref2 = O2.Operator2D(vx2, E2, bc2, 3)
+    ref2.set_planewaves(waves)
+    dt2 = 0.4 / (ref2.max_fscale() * 9)
+    x2, steps2, _ = ref2.run(np.zeros(ref2.ndofs()), dt2, 1.0, "lserk4")
+    tri = (vx2, E2, bc2, 3, dt2, 1.0, gw, x2, steps2)
+    return [tet, tri]
+
+
+def test_tfsf_instances_fp64_match_oracles():
+    """Plane waves injected on TF/SF faces of batched instances (tetrahedra
+    and triangles, each at its own stage times t + c_s dt), beside plain
+    instances: every instance matches its oracle run to 1e-11."""
+    b = D.Batch(precision=64, max_instances=8)
+    runs = []
+    for vx, E, fbc, order, dt, T, gw, x, steps in _tfsf_instances():
+        i = b.add(vx, E, fbc, order, x0=np.zeros(x.size), dt=dt, final_time=T, planewaves=gw)
+        runs.append((i, x, steps))
+    vx, E, fbc, ref, q0 = case_input(3, 2, "sma", 5)
+    i = b.add(vx, E, fbc, 2, x0=q0, final_time=0.2)
+    x = q0.copy()
+    steps, _ = ref.run(x, b.info(i)["dt"], 0.2, "lserk4")
+    runs.append((i, x, steps))
+    b.run(poll_every=8)
+    for i, x, steps in runs:
+        info = b.info(i)
+        assert info["state"] == "done" and info["steps"] == steps
+        assert rel(b.get_state(i), x) <= 1e-11, i
+
+
+def test_tfsf_instance_alone_equals_in_mix_fp32():
+    tet, tri = _tfsf_instances()
+    out = []
+    for mixed in (False, True):
+        b = D.Batch(precision=32, max_instances=16)
+        if mixed:
+            for k, (n, order, T, bc) in enumerate(CASES):
+                vx, E, fbc, _, q0 = case_input(n, order, bc, 110 + k)
+                b.add(vx, E, fbc, order, x0=q0, final_time=T)
+        ids = [b.add(vx, E, fbc, order, x0=np.zeros(x.size), dt=dt, final_time=T, planewaves=gw)
+               for vx, E, fbc, order, dt, T, gw, x, steps in (tet, tri)]
+        b.run(poll_every=6)
+        out.append([b.get_state(i) for i in ids])
+        b.close()
+    for a, m, inst in zip(out[0], out[1], (tet, tri)):
+        np.testing.assert_array_equal(a, m)
+        assert rel(a, inst[7]) <= 1e-5
