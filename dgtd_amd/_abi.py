@@ -35,7 +35,7 @@ class DgtdError(RuntimeError):
 class MeshDesc(ctypes.Structure):
     _fields_ = [("K", c_int32), ("order", c_int32), ("vx", _dp), ("nv", c_int32),
                 ("EToV", _i32p), ("face_bc", _i8p), ("eps_r", _dp), ("mu_r", _dp), ("dim", c_int32),
-                ("sigma", _dp)]
+                ("sigma", _dp), ("nvert", c_int32)]
 
 
 class Opts(ctypes.Structure):
@@ -120,6 +120,9 @@ SIGNATURES = {
     "dgtd_case_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p]),
     "dgtd_case_field_probes": (c_int32, [c_void_p, _dp, _i32p, _i32p, _i32p]),
     "dgtd_case_save_mesh": (c_int32, [c_void_p, ctypes.c_char_p]),
+    "dgtd_case_time_step": (c_int32, [c_void_p, _dp]),
+    "dgtd_op_estimate_time_step": (c_int32, [c_void_p, c_int32, c_double, _dp]),
+    "dgtd_mesh_estimate_time_step": (c_int32, [ctypes.POINTER(MeshDesc), c_int32, c_double, _dp]),
     "dgtd_case_planewaves": (c_int32, [c_void_p, ctypes.POINTER(PlaneWave)]),
     "dgtd_save_fields": (c_int32, [ctypes.c_char_p, c_int32, ctypes.c_int64, _dp, c_double]),
     "dgtd_op_set_graphs": (c_int32, [c_void_p, c_int32]),

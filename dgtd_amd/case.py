@@ -37,7 +37,8 @@ class Case:
             d = _abi.MeshDesc()
             check(lib().dgtd_case_mesh(h, ctypes.byref(d)))
             K, nv = d.K, d.nv
-            nvert = 3 if d.dim == 2 else 4  # triangles (2D) or tetrahedra
+            self.quads = d.dim == 2 and d.nvert == 4
+            nvert = 3 if d.dim == 2 and not self.quads else 4  # triangles, quadrilaterals or tetrahedra
             self.dim = 2 if d.dim == 2 else 3
             self.vx = np.ctypeslib.as_array(d.vx, shape=(3 * nv,)).reshape(nv, 3).copy()
             self.EToV = np.ctypeslib.as_array(d.EToV, shape=(nvert * K,)).reshape(K, nvert).copy()
@@ -93,6 +94,8 @@ class Case:
     @property
     def Np(self):
         p = self.order
+        if self.quads:
+            return (p + 1) ** 2
         return num_nodes(p) if self.dim == 3 else (p + 1) * (p + 2) // 2
 
     @property
@@ -102,7 +105,7 @@ class Case:
     def initial_state(self):
         from .hesthaven import _Mesh
 
-        m = _Mesh(self.vx, self.EToV, self.face_bc, self.order)
+        m = _Mesh(self.vx, self.EToV, self.face_bc, self.order, quads=self.quads)
         x = np.zeros(self.ndofs)
         srcs = (_abi.InitialField * max(len(self.sources), 1))(*self.sources)
         check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), srcs, len(self.sources), dptr(x)))
@@ -111,25 +114,29 @@ class Case:
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
         """The case's operator; conductive materials, and SMA faces with
         alpha != 1 under the Global or Maxwell operator (which weight SMA faces
-        as at alpha = 1), need the assembled one ("csr"), which kernel="auto"
-        then selects."""
+        as at alpha = 1), and quadrilateral meshes need the assembled one
+        ("csr"), which kernel="auto" then selects."""
         global_sma = (self.info["evolution_operator"] != "hesthaven" and self.info["alpha"] != 1.0
                       and bool((self.face_bc == 3).any()))
-        if kernel == "auto" and (self.sigma is not None or global_sma):
+        if kernel == "auto" and (self.sigma is not None or global_sma or self.quads):
             kernel = "csr"
         op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
                                 eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
-                                kernel=kernel, device=device, sigma=self.sigma)
+                                kernel=kernel, device=device, sigma=self.sigma, quads=self.quads)
         if self.planewaves:
             op.set_planewaves(self.planewaves)
         return op
 
-    def time_step(self, op):
-        """time_step, or the Hesthaven rule cfl / (max fscale * p^2) when it is 0
-        (Solver.cpp:110-115, 335-345)."""
-        if self.info["time_step"] > 0:
-            return self.info["time_step"]
-        return self.info["cfl"] / (op.max_fscale() * self.order ** 2)
+    def time_step(self, op=None):
+        """time_step, or when it is 0 the reference's estimate
+        (estimateTimeStep, Solver.cpp:110-115, 302-353): cfl / (max fscale *
+        p^2) for the hesthaven operator on a 3D mesh, else the mesh rule
+        0.75 * (2/3) * rmin * min vol/(surface/2) * cfl, halved on
+        quadrilaterals, / 0.8 for the Global operator in 3D.  (op is
+        accepted for compatibility and not needed.)"""
+        dt = ctypes.c_double()
+        check(lib().dgtd_case_time_step(self._h, ctypes.byref(dt)))
+        return dt.value
 
 
 def save_fields(directory, order, x, t):

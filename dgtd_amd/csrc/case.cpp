@@ -315,15 +315,10 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
         }
         w.mean = min_phase - 5.0 * w.spread * std::sqrt(2.0);
     }
-    if (cs.mesh.dim == 2) {
-        cs.EToV.resize(3 * (size_t)K);
-        for (int32_t e = 0; e < K; ++e)
-            for (int j = 0; j < 3; ++j) cs.EToV[3 * (size_t)e + j] = cs.mesh.tris[e][j];
-    } else {
-        cs.EToV.resize(4 * (size_t)K);
-        for (int32_t e = 0; e < K; ++e)
-            for (int j = 0; j < 4; ++j) cs.EToV[4 * (size_t)e + j] = cs.mesh.tets[e][j];
-    }
+    const int nvert = cs.mesh.nvert();
+    cs.EToV.resize((size_t)nvert * K);
+    for (int32_t e = 0; e < K; ++e)
+        for (int j = 0; j < nvert; ++j) cs.EToV[(size_t)nvert * e + j] = cs.mesh.vertex(e, j);
     cs.info.dim = cs.mesh.dim;
     cs.info.K = K;
     cs.info.nv = cs.mesh.nv();
@@ -347,6 +342,10 @@ extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
         cs->info.nplanewaves = (int32_t)cs->waves.size();
         cs->info.nprobes = (int32_t)(cs->probes.size() / 3);
         cs->info.nfield_probes = (int32_t)cs->fprobe_comp.size();
+        if (cs->mesh.quad() && cs->info.evolution_operator == DGTD_OPERATOR_HESTHAVEN)
+            // the reference's hesthaven quadrilateral cases are disabled
+            // (ExtensiveCasesTest.cpp:1159 DISABLED_2D_TFSF_Centered_Quads_Hesthaven)
+            throw Unsupported("quadrilateral meshes run on the Global operator only, not on hesthaven");
         if (cs->info.alpha == 0.0 && cs->info.evolution_operator == DGTD_OPERATOR_HESTHAVEN)
             for (int8_t b : cs->face_bc)
                 if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598 (the Global operator runs it)
@@ -385,7 +384,20 @@ extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
     mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
     mesh->dim = c->mesh.dim;
     mesh->sigma = c->sigma.empty() ? nullptr : c->sigma.data();
+    mesh->nvert = c->mesh.quad() ? 4 : 0;
     return DGTD_OK;
+}
+
+extern "C" int dgtd_case_time_step(const dgtd_case* c, double* dt)
+{
+    if (!c || !dt) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
+    if (c->info.time_step > 0.0) {
+        *dt = c->info.time_step;
+        return DGTD_OK;
+    }
+    dgtd_mesh_desc mesh{};
+    if (int rc = dgtd_case_mesh(c, &mesh)) return rc;
+    return dgtd_mesh_estimate_time_step(&mesh, c->info.evolution_operator, c->info.cfl, dt);
 }
 
 extern "C" int dgtd_case_save_mesh(const dgtd_case* c, const char* path)

@@ -38,6 +38,10 @@ int fail(int code, const std::string& msg)
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// a request the implementation does not cover (DGTD_E_UNSUPPORTED)
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -119,8 +123,10 @@ struct dgtd_op {
     int integrator = DGTD_LSERK4;
     double alpha = 1.0;
     dgtd::RefTet R;
-    int dim = 3;       // 2: triangles (the 2D operator, RefTri T2)
+    int dim = 3;       // 2: triangles (the 2D operator, RefTri T2) or quadrilaterals (quad, RefQuad Q2)
     dgtd::RefTri T2;
+    bool quad = false;
+    dgtd::RefQuad Q2;
     int K = 0, Np = 0, Nfp = 0;
     size_t N = 0;
     int64_t ndofs = 0;
@@ -368,7 +374,7 @@ struct dgtd_op {
     {
         if (ntfsf == 0 || nwaves == 0) return;
         dgtd::TfsfArgs<T> a{};
-        a.dim = dim;
+        a.dim = quad ? 4 : dim;
         a.nelem = ntfsf;
         a.elem = d_tfsf_elem;
         a.sign = d_tfsf_sign;
@@ -733,6 +739,119 @@ void create_tri(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
     select_variant(op, kernel, [&] { upload_csr(op, dgtd::assemble_global_tri(md, R, mesh->sigma, op->alpha)); });
 }
 
+// Set-up of a quadrilateral-mesh operator (dim = 2, nvert = 4): the
+// assembled operator only.
+void create_quad(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
+{
+    if (kernel != DGTD_KERNEL_CSR && kernel != DGTD_KERNEL_AUTO)
+        throw UnsupportedError("quadrilaterals run on the assembled operator (DGTD_KERNEL_CSR)");
+    op->dim = 2;
+    op->quad = true;
+    op->Q2 = dgtd::build_refquad(mesh->order);
+    const dgtd::RefQuad& R = op->Q2;
+    op->K = mesh->K;
+    op->Np = R.Np;
+    op->Nfp = R.Nfp;
+    op->N = (size_t)op->Np * mesh->K;
+    op->ndofs = 6 * (int64_t)op->N;
+    const dgtd::TriMeshData md = dgtd::build_quad_mesh(tri_input(mesh), R, 1.0);
+    op->max_fscale = md.max_fscale;
+    op->nv = mesh->nv;
+    op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
+    op->EToV.assign(mesh->EToV, mesh->EToV + 4 * (size_t)mesh->K);
+    hip_check(hipSetDevice(op->device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&op->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&op->ev1), "hipEventCreate");
+    op->d_M = static_cast<double*>(upload(R.M));
+    op->d_jac = static_cast<double*>(upload(md.jac));
+    op->d_eps = static_cast<double*>(upload(md.eps));
+    op->d_mu = static_cast<double*>(upload(md.mu));
+    const int te = dgtd::norms_tile(op->Np);
+    op->nparts = std::max(1, std::min(dgtd::kNormsMaxParts, (op->K + te - 1) / te));
+    hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+    double* scratch_dev = nullptr;  // resolves the allocation-free entry point's scratch on this device
+    hip_check(dgtd::norms_scratch(op->device, &scratch_dev), "norms scratch");
+    hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+    hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
+    hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
+    const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
+    for (int b = 0; b < nbuf; ++b) {
+        hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
+        hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
+    }
+    if (!md.tfsf.empty()) {  // quadrilaterals owning TF/SF edges: edge signs, edge-node positions, records
+        const dgtd::TfsfRecords r = dgtd::tfsf_records(mesh, R, md.tfsf, md.geo);
+        op->ntfsf = (int)r.elems.size();
+        op->d_tfsf_elem = static_cast<int*>(upload(r.elems));
+        op->d_tfsf_sign = static_cast<int*>(upload(r.signs));
+        op->d_tfsf_xyz = static_cast<double*>(upload(r.xyz));
+        op->d_tfsf_geo = static_cast<double*>(upload(r.geo));
+        op->d_lift = static_cast<double*>(upload(R.LIFT));
+    }
+    hip_check(hipStreamSynchronize(op->stream), "sync");
+    select_variant(op, DGTD_KERNEL_CSR, [&] { upload_csr(op, dgtd::assemble_global_quad(md, R, mesh->sigma, op->alpha)); });
+}
+
+// Solver::estimateTimeStep (Solver.cpp:302-353).  On tetrahedra under
+// hesthaven: cfl / (max fscale * p^2).  Otherwise calcMeshTimeStep
+// (:290-300): cfl * 0.75 * (2/3) * rmin * min_e vol_e / (surface_e / 2)
+// (getTimeStepScale, :207-252: areas and edge lengths in 2D, volumes and
+// face areas in 3D; getJacobiGQ_RMin, :254-263: the first Gauss-Lobatto
+// spacing of order p on [-1, 1]; speed of light 1), halved when the mesh holds
+// quadrilaterals, divided by 0.8 on tetrahedra under the Global operator.
+int estimate_time_step(int dim, bool quad, int order, int K, const double* vx, const int32_t* EToV, double max_fscale,
+                       int evolution_operator, double cfl, double* dt)
+{
+    if (evolution_operator < DGTD_OPERATOR_MAXWELL || evolution_operator > DGTD_OPERATOR_HESTHAVEN)
+        return fail(DGTD_E_ARG, "evolution_operator must be DGTD_OPERATOR_MAXWELL, _GLOBAL or _HESTHAVEN");
+    if (!(cfl > 0.0)) return fail(DGTD_E_ARG, "cfl must be > 0");
+    if (dim == 3 && evolution_operator == DGTD_OPERATOR_HESTHAVEN) {
+        *dt = cfl / (max_fscale * order * order);
+        return DGTD_OK;
+    }
+    const int nvert = dim == 3 || quad ? 4 : 3;
+    auto P = [&](int e, int j) { return vx + 3 * (size_t)EToV[(size_t)nvert * e + j]; };
+    double scale = INFINITY;
+    for (int e = 0; e < K; ++e) {
+        double vol = 0.0, surf = 0.0;
+        if (dim == 2) {
+            for (int j = 0; j < nvert; ++j) {  // shoelace area, edge lengths
+                const double *a = P(e, j), *b = P(e, (j + 1) % nvert);
+                vol += 0.5 * (a[0] * b[1] - b[0] * a[1]);
+                surf += std::hypot(b[0] - a[0], b[1] - a[1]);
+            }
+            vol = std::fabs(vol);
+        } else {
+            static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+            double u[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int d = 0; d < 3; ++d) u[k][d] = P(e, k + 1)[d] - P(e, 0)[d];
+            vol = std::fabs(u[0][0] * (u[1][1] * u[2][2] - u[1][2] * u[2][1]) -
+                            u[0][1] * (u[1][0] * u[2][2] - u[1][2] * u[2][0]) +
+                            u[0][2] * (u[1][0] * u[2][1] - u[1][1] * u[2][0])) / 6.0;
+            for (const auto& f : kFace) {
+                double a[3], b[3];
+                for (int d = 0; d < 3; ++d) {
+                    a[d] = P(e, f[1])[d] - P(e, f[0])[d];
+                    b[d] = P(e, f[2])[d] - P(e, f[0])[d];
+                }
+                const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2],
+                             cz = a[0] * b[1] - a[1] * b[0];
+                surf += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            }
+        }
+        scale = std::min(scale, vol / (surf / 2.0));
+    }
+    const std::vector<double> g = dgtd::gll01(order);  // on [0, 1]
+    double base = scale * (2.0 * (g[1] - g[0])) * 2.0 / 3.0 * 0.75;
+    if (quad) base /= 2.0;
+    base *= cfl;
+    if (dim == 3 && evolution_operator == DGTD_OPERATOR_GLOBAL) base /= 0.8;
+    *dt = base;
+    return DGTD_OK;
+}
+
 dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
 {
     dgtd::MeshInput in;
@@ -770,6 +889,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
         return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
+    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)))
+        return fail(DGTD_E_ARG, "nvert must be 0, or 3 or 4 with dim = 2");
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     dgtd_op* op = nullptr;
@@ -779,6 +900,11 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->precision = opts->precision;
         op->integrator = opts->integrator;
         op->alpha = opts->alpha;
+        if (mesh->dim == 2 && mesh->nvert == 4) {
+            create_quad(op, mesh, opts->kernel);
+            *out = op;
+            return DGTD_OK;
+        }
         if (mesh->dim == 2) {
             create_tri(op, mesh, opts->kernel);
             *out = op;
@@ -874,6 +1000,9 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     } catch (const HipError& e) {
         delete op;
         return fail(DGTD_E_HIP, e.what());
+    } catch (const UnsupportedError& e) {
+        delete op;
+        return fail(DGTD_E_UNSUPPORTED, e.what());
     } catch (const std::bad_alloc&) {
         delete op;
         return fail(DGTD_E_NOMEM, "host allocation failed");
@@ -908,9 +1037,37 @@ int dgtd_op_max_fscale(const dgtd_op* op, double* out)
     return DGTD_OK;
 }
 
+int dgtd_op_estimate_time_step(const dgtd_op* op, int32_t evolution_operator, double cfl, double* dt)
+{
+    if (!op || !dt) return fail(DGTD_E_ARG, "NULL argument");
+    const int order = op->dim == 3 ? op->R.p : op->quad ? op->Q2.p : op->T2.p;
+    return estimate_time_step(op->dim, op->quad, order, op->K, op->vx.data(), op->EToV.data(), op->max_fscale,
+                              evolution_operator, cfl, dt);
+}
+
+int dgtd_mesh_estimate_time_step(const dgtd_mesh_desc* mesh, int32_t evolution_operator, double cfl, double* dt)
+{
+    if (!mesh || !dt || !mesh->vx || !mesh->EToV || mesh->K < 1) return fail(DGTD_E_ARG, "NULL argument or empty mesh");
+    if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    try {
+        const bool quad = mesh->dim == 2 && mesh->nvert == 4;
+        double max_fscale = 0.0;
+        if (mesh->dim != 2 && evolution_operator == DGTD_OPERATOR_HESTHAVEN)
+            max_fscale = dgtd::build_mesh(mesh_input(mesh), dgtd::build_reftet(mesh->order), 1.0).max_fscale;
+        return estimate_time_step(mesh->dim == 2 ? 2 : 3, quad, mesh->order, mesh->K, mesh->vx, mesh->EToV, max_fscale,
+                                  evolution_operator, cfl, dt);
+    } catch (const std::exception& e) {
+        return fail(DGTD_E_ARG, e.what());
+    }
+}
+
 int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
 {
     if (!op || !xyz) return fail(DGTD_E_ARG, "NULL argument");
+    if (op->quad) {
+        dgtd::quad_node_positions(tri_input(op), op->Q2, xyz);
+        return DGTD_OK;
+    }
     if (op->dim == 2) {
         dgtd::tri_node_positions(tri_input(op), op->T2, xyz);
         return DGTD_OK;
@@ -1162,7 +1319,9 @@ int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem)
         hip_check(hipSetDevice(op->device), "hipSetDevice");
         std::vector<int32_t> el(n, -1);
         std::vector<double> w((size_t)n * op->Np, 0.0);
-        if (op->dim == 2) {
+        if (op->quad) {
+            dgtd::quad_locate_points(tri_input(op), op->Q2, n, xyz, el.data(), w.data());
+        } else if (op->dim == 2) {
             dgtd::tri_locate_points(tri_input(op), op->T2, n, xyz, el.data(), w.data());
         } else {
             dgtd::MeshInput in;
@@ -1450,6 +1609,10 @@ int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, i
     if (!mesh || n < 0 || (n > 0 && (!xyz || !elem || !weights))) return fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     try {
+        if (mesh->dim == 2 && mesh->nvert == 4) {
+            dgtd::quad_locate_points(tri_input(mesh), dgtd::build_refquad(mesh->order), n, xyz, elem, weights);
+            return DGTD_OK;
+        }
         if (mesh->dim == 2) {
             dgtd::tri_locate_points(tri_input(mesh), dgtd::build_reftri(mesh->order), n, xyz, elem, weights);
             return DGTD_OK;
@@ -1486,7 +1649,10 @@ int dgtd_global_assemble(const dgtd_mesh_desc* mesh, double alpha, int64_t* nnz,
     if (!std::isfinite(alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
     try {
         dgtd::Csr A;
-        if (mesh->dim == 2) {
+        if (mesh->dim == 2 && mesh->nvert == 4) {
+            const dgtd::RefQuad R = dgtd::build_refquad(mesh->order);
+            A = dgtd::assemble_global_quad(dgtd::build_quad_mesh(tri_input(mesh), R, 1.0), R, mesh->sigma, alpha);
+        } else if (mesh->dim == 2) {
             const dgtd::RefTri R = dgtd::build_reftri(mesh->order);
             A = dgtd::assemble_global_tri(dgtd::build_tri_mesh(tri_input(mesh), R, 1.0), R, mesh->sigma, alpha);
         } else {
@@ -1519,10 +1685,13 @@ extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgt
     if (!mesh || !x || (nsrc > 0 && !src)) return fail(DGTD_E_ARG, "NULL argument");
     try {
         if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) throw std::runtime_error("order must be in 1..5");
-        const int Np = mesh->dim == 2 ? dgtd::tri_num_nodes(mesh->order) : dgtd::num_nodes(mesh->order);
+        const bool quad = mesh->dim == 2 && mesh->nvert == 4;
+        const int Np = quad ? dgtd::quad_num_nodes(mesh->order)
+                            : mesh->dim == 2 ? dgtd::tri_num_nodes(mesh->order) : dgtd::num_nodes(mesh->order);
         const size_t N = (size_t)Np * mesh->K;
         std::vector<double> xyz(3 * N);
-        if (mesh->dim == 2) dgtd::tri_node_positions(tri_input(mesh), dgtd::build_reftri(mesh->order), xyz.data());
+        if (quad) dgtd::quad_node_positions(tri_input(mesh), dgtd::build_refquad(mesh->order), xyz.data());
+        else if (mesh->dim == 2) dgtd::tri_node_positions(tri_input(mesh), dgtd::build_reftri(mesh->order), xyz.data());
         else dgtd::node_positions(mesh_input(mesh), dgtd::build_reftet(mesh->order), xyz.data());
         std::fill(x, x + 6 * N, 0.0);
         for (int s = 0; s < nsrc; ++s) {

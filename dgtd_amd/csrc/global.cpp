@@ -22,6 +22,7 @@
 #include "global.hpp"
 
 #include "kernels.hpp"
+#include "quad.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -66,25 +67,27 @@ void load_tet(ElemView& v, const MeshData& md, int e)
     v.imu = g[26];
 }
 
-void load_tri(ElemView& v, const TriMeshData& md, int e)
+// triangles (nf = 3, 15-value records) and quadrilaterals (nf = 4, 18 values)
+void load_2d(ElemView& v, const TriMeshData& md, int e, int nf)
 {
-    const double* g = &md.geo[(size_t)kTriGeoStride * e];
+    const int stride = nf == 4 ? kQuadGeoStride : kTriGeoStride, cstride = nf == 4 ? kQuadConnStride : kTriConnStride;
+    const double* g = &md.geo[(size_t)stride * e];
     for (int k = 0; k < 3; ++k)
         for (int d = 0; d < 3; ++d) v.G[k][d] = 0.0;
     v.G[0][0] = g[0];
     v.G[0][1] = g[1];
     v.G[1][0] = g[2];
     v.G[1][1] = g[3];
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < nf; ++f) {
         v.n[f][0] = g[4 + 3 * f];
         v.n[f][1] = g[5 + 3 * f];
         v.n[f][2] = 0.0;
         v.fs[f] = g[6 + 3 * f];
-        v.nb[f] = md.conn[(size_t)kTriConnStride * e + f];
-        v.code[f] = md.conn[(size_t)kTriConnStride * e + 3 + f];
+        v.nb[f] = md.conn[(size_t)cstride * e + f];
+        v.code[f] = md.conn[(size_t)cstride * e + nf + f];
     }
-    v.ieps = g[13];
-    v.imu = g[14];
+    v.ieps = g[4 + 3 * nf];
+    v.imu = g[5 + 3 * nf];
 }
 
 // Dense element block: rows (c, i) of the element, columns (slot, k, j) with
@@ -287,10 +290,37 @@ Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* si
         v.lift = R.LIFT.data();
         v.fmask = R.Fmask.data();
         v.perm = R.edge_perm.data();
-        load_tri(v, md, e);
+        load_2d(v, md, e, 3);
         return v;
     };
     assemble(md.K, R.Np, 3, view, sigma, alpha, out);
+    return out;
+}
+
+}  // namespace dgtd
+
+namespace dgtd {
+
+Csr assemble_global_quad(const TriMeshData& md, const RefQuad& R, const double* sigma, double alpha)
+{
+    if (6.0 * R.Np * (double)md.K > 2147483647.0) throw std::runtime_error("mesh too large for 32-bit column indices");
+    Csr out;
+    auto view = [&](int e) {
+        ElemView v{};
+        v.Np = R.Np;
+        v.Nfp = R.Nfp;
+        v.nf = 4;
+        v.nd = 2;
+        v.D[0] = R.Dr.data();
+        v.D[1] = R.Ds.data();
+        v.D[2] = R.Ds.data();  // unused (nd = 2)
+        v.lift = R.LIFT.data();
+        v.fmask = R.Fmask.data();
+        v.perm = R.edge_perm.data();
+        load_2d(v, md, e, 4);
+        return v;
+    };
+    assemble(md.K, R.Np, 4, view, sigma, alpha, out);
     return out;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "mesh.hpp"
+#include "quad.hpp"
 #include "tri.hpp"
 
 namespace dgtd {
@@ -21,6 +22,7 @@ struct Csr {
 // interior faces here: their injection is a separate, state-independent term.
 Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha);
 Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* sigma, double alpha);
+Csr assemble_global_quad(const TriMeshData& md, const RefQuad& R, const double* sigma, double alpha);
 
 // Element-local 16-bit column codes of A for the device kernel (layout in
 // kernels.hpp: kCsrRowsPerBlock, kCsrTab); N = Np * K.

@@ -219,7 +219,8 @@ struct TfsfWave {
 
 template <typename T>
 struct TfsfArgs {
-    int dim;             // 3: tetrahedra (4 faces, 27-value records); 2: triangles (3 edges, 15 values)
+    int dim;             // 3: tetrahedra (4 faces, 27-value records); 2: triangles (3 edges, 15 values);
+                         // 4: quadrilaterals (4 edges, 18 values)
     int nelem;           // elements owning TF/SF faces
     const int* elem;     // [nelem] element ids
     const int* sign;     // [4*nelem] per face: -1 scattered side, +1 total side, 0

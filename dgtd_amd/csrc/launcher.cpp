@@ -104,7 +104,7 @@ int main(int argc, char** argv)
     // conductivity, or the Global operator's own SMA weighting (alpha != 1
     // with SMA faces), needs the assembled operator
     bool sma = false;
-    for (int64_t i = 0; i < (int64_t)mesh.K * (mesh.dim == 2 ? 3 : 4); ++i)
+    for (int64_t i = 0; i < (int64_t)mesh.K * (mesh.dim == 2 && mesh.nvert != 4 ? 3 : 4); ++i)
         sma = sma || mesh.face_bc[i] == DGTD_BC_SMA || mesh.face_bc[i] == DGTD_BC_INT_SMA;
     const bool global_sma = info.evolution_operator != DGTD_OPERATOR_HESTHAVEN && sma && info.alpha != 1.0;
     o.kernel = kernel == DGTD_KERNEL_AUTO && (mesh.sigma || global_sma) ? DGTD_KERNEL_CSR : kernel;
@@ -124,13 +124,9 @@ int main(int argc, char** argv)
         check(dgtd_op_set_planewaves(op, waves.data(), info.nplanewaves), "set plane waves");
     }
 
-    // dt: time_step, or the Hesthaven rule (Solver.cpp:110-115, 335-345)
-    double dt = info.time_step;
-    if (dt == 0.0) {
-        double fs = 0.0;
-        check(dgtd_op_max_fscale(op, &fs), "max fscale");
-        dt = info.cfl / (fs * info.order * info.order);
-    }
+    // dt: time_step, or the reference's automatic estimate (Solver.cpp:110-115, 302-353)
+    double dt = 0.0;
+    check(dgtd_case_time_step(cs, &dt), "time step");
 
     // point probes, then field probes, in one device probe set, and their
     // sampling intervals (driver.cpp:717-733, 764-778)

@@ -84,15 +84,16 @@ TetMesh read_gmsh22(const std::string& path)
                 for (auto& t : tags) is >> t;
                 const int attr = ntags >= 2 ? (int)tags[1] : (ntags == 1 ? (int)tags[0] : 1);
                 int nn = 0;
-                if (type == 4) nn = 4;
+                if (type == 4 || type == 3) nn = 4;
                 else if (type == 2) nn = 3;
                 else if (type == 1) nn = 2;
                 else if (type == 15) continue;  // points dropped
-                else if (type == 3 || type == 5)
-                    throw MeshUnsupported(path + ": quadrilateral and hexahedral elements are not supported");
+                else if (type == 5)
+                    throw MeshUnsupported(path + ": hexahedral elements are not supported");
                 else
                     throw MeshUnsupported(path + ": element type " + std::to_string(type) +
-                                          " is not supported (straight-sided tetrahedra and triangles only)");
+                                          " is not supported (straight-sided tetrahedra, triangles and "
+                                          "quadrilaterals only)");
                 int32_t vs[4];
                 for (int k = 0; k < nn; ++k) {
                     long long nid;
@@ -101,7 +102,10 @@ TetMesh read_gmsh22(const std::string& path)
                     if (it == node_index.end()) bad(path, "element references unknown node");
                     vs[k] = it->second;
                 }
-                if (nn == 4) {
+                if (type == 3) {
+                    m.quads.push_back({vs[0], vs[1], vs[2], vs[3]});
+                    m.quad_attr.push_back(attr);
+                } else if (nn == 4) {
                     m.tets.push_back({vs[0], vs[1], vs[2], vs[3]});
                     m.tet_attr.push_back(attr);
                 } else if (nn == 3) {
@@ -117,9 +121,13 @@ TetMesh read_gmsh22(const std::string& path)
     }
     if (!have_format || !have_nodes || !have_elements) bad(path, "missing $MeshFormat, $Nodes or $Elements");
     if (m.tets.empty()) {
-        if (m.tris.empty()) bad(path, "no tetrahedra or triangles");
-        m.dim = 2;  // triangles are the elements, lines the boundary
+        if (m.tris.empty() && m.quads.empty()) bad(path, "no tetrahedra, triangles or quadrilaterals");
+        if (!m.tris.empty() && !m.quads.empty())
+            throw MeshUnsupported(path + ": meshes mixing triangles and quadrilaterals are not supported");
+        m.dim = 2;  // triangles or quadrilaterals are the elements, lines the boundary
     } else {
+        if (!m.quads.empty())
+            throw MeshUnsupported(path + ": quadrilateral faces (hexahedra, prisms) are not supported");
         m.segs.clear();  // lines of a 3D mesh are dropped
         m.seg_attr.clear();
     }
@@ -167,11 +175,16 @@ TetMesh read_mfem_mesh(const std::string& path)
             for (long i = 0; i < n; ++i) {
                 const int attr = (int)next_int();
                 const int geom = (int)next_int();
-                const int want = m.dim == 3 ? 4 : 2;  // tetrahedra, or triangles in 2D
-                if (geom != want)
+                // tetrahedra, or in 2D triangles or squares (quadrilaterals)
+                if (m.dim == 3 ? geom != 4 : geom != 2 && geom != 3)
                     throw MeshUnsupported(path + ": element geometry " + std::to_string(geom) +
-                                          " is not supported (tetrahedra or triangles only)");
-                if (m.dim == 3) {
+                                          " is not supported (tetrahedra, triangles or quadrilaterals only)");
+                if (geom == 3) {
+                    std::array<int32_t, 4> q;
+                    for (auto& v : q) v = (int32_t)next_int();
+                    m.quads.push_back(q);
+                    m.quad_attr.push_back(attr);
+                } else if (m.dim == 3) {
                     std::array<int32_t, 4> t;
                     for (auto& v : t) v = (int32_t)next_int();
                     m.tets.push_back(t);
@@ -221,8 +234,13 @@ TetMesh read_mfem_mesh(const std::string& path)
             bad(path, "unknown section '" + sec + "'");
         }
     }
+    if (!m.quads.empty() && !m.tris.empty())
+        throw MeshUnsupported(path + ": meshes mixing triangles and quadrilaterals are not supported");
     if (!have_vertices || m.K() == 0) bad(path, "missing vertices or elements");
     const int32_t nv = m.nv();
+    for (const auto& q : m.quads)
+        for (int v : q)
+            if (v < 0 || v >= nv) bad(path, "element vertex out of range");
     for (const auto& t : m.tets)
         for (int v : t)
             if (v < 0 || v >= nv) bad(path, "element vertex out of range");
@@ -240,10 +258,13 @@ void write_mfem_mesh(const TetMesh& m, const std::string& path)
     std::ofstream o(path);
     if (!o) throw std::runtime_error("cannot write " + path);
     o.precision(16);
-    if (m.dim == 2) {  // triangles (geometry 2), segments (geometry 1), 2D vertices
-        o << "MFEM mesh v1.0\n\ndimension\n2\n\nelements\n" << m.tris.size() << "\n";
-        for (size_t e = 0; e < m.tris.size(); ++e)
-            o << m.tri_attr[e] << " 2 " << m.tris[e][0] << " " << m.tris[e][1] << " " << m.tris[e][2] << "\n";
+    if (m.dim == 2) {  // triangles (geometry 2) or squares (3), segments (geometry 1), 2D vertices
+        o << "MFEM mesh v1.0\n\ndimension\n2\n\nelements\n" << m.K() << "\n";
+        for (int32_t e = 0; e < m.K(); ++e) {
+            o << m.elem_attr()[e] << (m.quad() ? " 3" : " 2");
+            for (int j = 0; j < m.nvert(); ++j) o << " " << m.vertex(e, j);
+            o << "\n";
+        }
         o << "\nboundary\n" << m.segs.size() << "\n";
         for (size_t i = 0; i < m.segs.size(); ++i) o << m.seg_attr[i] << " 1 " << m.segs[i][0] << " " << m.segs[i][1] << "\n";
         o << "\nvertices\n" << m.nv() << "\n2\n";
@@ -282,6 +303,16 @@ void orient_positive(TetMesh& m)
         if (a2 == 0.0) throw std::runtime_error("degenerate triangle in mesh");
         if (a2 < 0) std::swap(t[0], t[1]);
     }
+    for (auto& q : m.quads) {  // shoelace area; a reversed cycle keeps vertex 0
+        double a2 = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            const double* a = &m.vx[3 * (size_t)q[j]];
+            const double* b = &m.vx[3 * (size_t)q[(j + 1) % 4]];
+            a2 += a[0] * b[1] - b[0] * a[1];
+        }
+        if (a2 == 0.0) throw std::runtime_error("degenerate quadrilateral in mesh");
+        if (a2 < 0) std::swap(q[1], q[3]);
+    }
     for (auto& t : m.tets) {
         const double v6 = signed_volume6(m.vx.data(), t);
         if (v6 == 0.0) throw std::runtime_error("degenerate tetrahedron in mesh");
@@ -291,7 +322,7 @@ void orient_positive(TetMesh& m)
 
 TetMesh refine_uniform(const TetMesh& m)
 {
-    if (m.dim == 2) throw MeshUnsupported("uniform refinement of triangle meshes is not supported");
+    if (m.dim == 2) throw MeshUnsupported("uniform refinement of 2D meshes is not supported");
     TetMesh r;
     r.vx = m.vx;
     const int64_t nv0 = m.nv();
@@ -355,19 +386,20 @@ namespace {
 
 std::vector<int8_t> edge_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
 {
-    static const int kEdge[3][2] = {{0, 1}, {1, 2}, {2, 0}};
     const int64_t nv = m.nv();
+    const int n = m.nvert();  // edge f joins local vertices f and f+1 mod n
+    const int32_t K = m.K();
     std::unordered_map<int64_t, std::pair<int64_t, int64_t>> edges;
-    edges.reserve(3 * m.tris.size());
-    for (size_t e = 0; e < m.tris.size(); ++e)
-        for (int f = 0; f < 3; ++f) {
-            const int64_t a = m.tris[e][kEdge[f][0]], b = m.tris[e][kEdge[f][1]];
+    edges.reserve((size_t)n * K);
+    for (int32_t e = 0; e < K; ++e)
+        for (int f = 0; f < n; ++f) {
+            const int64_t a = m.vertex(e, f), b = m.vertex(e, (f + 1) % n);
             const int64_t k = std::min(a, b) * nv + std::max(a, b);
             auto it = edges.find(k);
-            if (it == edges.end()) edges.emplace(k, std::make_pair((int64_t)(3 * e + f), (int64_t)-1));
-            else it->second.second = (int64_t)(3 * e + f);
+            if (it == edges.end()) edges.emplace(k, std::make_pair((int64_t)n * e + f, (int64_t)-1));
+            else it->second.second = (int64_t)n * e + f;
         }
-    std::vector<int8_t> bc(3 * m.tris.size(), 0);
+    std::vector<int8_t> bc((size_t)n * K, 0);
     for (size_t i = 0; i < m.segs.size(); ++i) {
         const int attr = m.seg_attr[i];
         const int code = (attr >= 0 && attr < (int)attr_to_bc.size()) ? attr_to_bc[attr] : -1;

@@ -10,7 +10,7 @@
 // stage output and c_res * S to the residual after the fused stage kernel,
 // with the coefficients the stage applies to the right-hand side.  One
 // 64-thread block per such element, FP64 arithmetic.  On triangles (dim 2)
-// the faces are the three edges and n_z = 0.
+// and quadrilaterals (dim 4 here) the faces are the edges and n_z = 0.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -30,9 +30,11 @@ __device__ void tfsf_element(int dim, int k, int e, const int* sign, const doubl
                              double alpha, double t, T* out, double c_out, T* res, double c_res,
                              double (&flux)[6][kTfsfMaxFaceNodes])
 {
-    const bool tri = dim == 2;
-    const int Nf = (tri ? 3 : 4) * Nfp;
-    const double* g = geo + (size_t)k * (tri ? 15 : 27);
+    // dim 3 tetrahedra (27-value records), 2 triangles (15), 4 quadrilaterals (18)
+    const bool planar = dim != 3;
+    const int stride = dim == 3 ? 27 : dim == 2 ? 15 : 18;
+    const int Nf = (dim == 2 ? 3 : 4) * Nfp;
+    const double* g = geo + (size_t)k * stride;
     for (int m = threadIdx.x; m < Nf; m += 64) {
         const int f = m / Nfp;
         const int sg = sign[4 * k + f];
@@ -51,11 +53,11 @@ __device__ void tfsf_element(int dim, int k, int e, const int* sign, const doubl
                 }
             }
         }
-        // records: tetrahedron [9+4f..12+4f] = nx ny nz fscale/2; triangle
-        // [4+3f..6+3f] = nx ny fscale/2
-        const double n[3] = {tri ? g[4 + 3 * f] : g[9 + 4 * f], tri ? g[5 + 3 * f] : g[10 + 4 * f],
-                             tri ? 0.0 : g[11 + 4 * f]};
-        const double fs = tri ? g[6 + 3 * f] : g[12 + 4 * f];
+        // records: tetrahedron [9+4f..12+4f] = nx ny nz fscale/2; triangle and
+        // quadrilateral [4+3f..6+3f] = nx ny fscale/2
+        const double n[3] = {planar ? g[4 + 3 * f] : g[9 + 4 * f], planar ? g[5 + 3 * f] : g[10 + 4 * f],
+                             planar ? 0.0 : g[11 + 4 * f]};
+        const double fs = planar ? g[6 + 3 * f] : g[12 + 4 * f];
         const double ndE = n[0] * dE[0] + n[1] * dE[1] + n[2] * dE[2];
         const double ndH = n[0] * dH[0] + n[1] * dH[1] + n[2] * dH[2];
         for (int x = 0; x < 3; ++x) {
@@ -69,7 +71,7 @@ __device__ void tfsf_element(int dim, int k, int e, const int* sign, const doubl
         for (int c = 0; c < 6; ++c) {
             double s = 0.0;
             for (int m = 0; m < Nf; ++m) s += lift[(size_t)i * Nf + m] * flux[c][m];
-            s *= c < 3 ? g[tri ? 13 : 25] : g[tri ? 14 : 26];
+            s *= c < 3 ? g[stride - 2] : g[stride - 1];  // 1/eps or 1/mu
             const size_t idx = c * N + (size_t)e * Np + i;
             out[idx] = (T)((double)out[idx] + c_out * s);
             if (res) res[idx] = (T)((double)res[idx] + c_res * s);

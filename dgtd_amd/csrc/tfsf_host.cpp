@@ -47,10 +47,12 @@ namespace {
 // Records of the elements owning TF/SF faces for an element of NV vertices
 // (4 tetrahedron, 3 triangle); face node n sits at x = v0 + sum_k (v_{k+1} -
 // v0) xi_k with the reference coordinates xi[(NV-1) n + k].
+// (A parallelogram, NV = 4 with quad set, maps its unit square through
+// vertices 1 and 3: x = v0 + (v1 - v0) xi_0 + (v3 - v0) xi_1.)
 template <int NV>
 TfsfRecords records(const dgtd_mesh_desc* mesh, const std::vector<int>& Fmask, const std::vector<double>& xi,
                     int nfaces, int Nfp, int stride, const std::vector<std::array<int, 3>>& tfsf,
-                    const std::vector<double>& geo)
+                    const std::vector<double>& geo, bool quad = false)
 {
     TfsfRecords r;
     std::vector<int> slot(mesh->K, -1);
@@ -65,14 +67,19 @@ TfsfRecords records(const dgtd_mesh_desc* mesh, const std::vector<int>& Fmask, c
             for (int m = 0; m < nfaces * Nfp; ++m) {
                 const int n = Fmask[m];
                 for (int d = 0; d < 3; ++d) {
-                    if (NV == 3 && d == 2) {  // triangles lie in z = 0
+                    if ((NV < 4 || quad) && d == 2) {  // triangles and quadrilaterals lie in z = 0
                         r.xyz.push_back(0.0);
                         continue;
                     }
                     const double x0 = mesh->vx[3 * (size_t)v[0] + d];
                     double x = x0;
-                    for (int k = 0; k < NV - 1; ++k)
-                        x += (mesh->vx[3 * (size_t)v[k + 1] + d] - x0) * xi[(size_t)(NV - 1) * n + k];
+                    if (quad) {
+                        x += (mesh->vx[3 * (size_t)v[1] + d] - x0) * xi[2 * (size_t)n] +
+                             (mesh->vx[3 * (size_t)v[3] + d] - x0) * xi[2 * (size_t)n + 1];
+                    } else {
+                        for (int k = 0; k < NV - 1; ++k)
+                            x += (mesh->vx[3 * (size_t)v[k + 1] + d] - x0) * xi[(size_t)(NV - 1) * n + k];
+                    }
                     r.xyz.push_back(x);
                 }
             }
@@ -94,6 +101,12 @@ TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefTri& R, const std:
                          const std::vector<double>& geo)
 {
     return records<3>(mesh, R.Fmask, R.xi, 3, R.Nfp, kTriGeoStride, tfsf, geo);
+}
+
+TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefQuad& R, const std::vector<std::array<int, 3>>& tfsf,
+                         const std::vector<double>& geo)
+{
+    return records<4>(mesh, R.Fmask, R.xi, 4, R.Nfp, kQuadGeoStride, tfsf, geo, true);
 }
 
 }  // namespace dgtd

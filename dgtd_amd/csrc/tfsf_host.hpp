@@ -9,6 +9,7 @@
 #include "dgtd_hip.h"
 #include "kernels.hpp"
 #include "reftet.hpp"
+#include "quad.hpp"
 #include "tri.hpp"
 
 namespace dgtd {
@@ -30,6 +31,8 @@ struct TfsfRecords {
 TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefTet& R, const std::vector<std::array<int, 3>>& tfsf,
                          const std::vector<double>& geo);
 TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefTri& R, const std::vector<std::array<int, 3>>& tfsf,
+                         const std::vector<double>& geo);
+TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefQuad& R, const std::vector<std::array<int, 3>>& tfsf,
                          const std::vector<double>& geo);
 
 }  // namespace dgtd

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <map>
 #include <stdexcept>
+#include <string>
 #include <utility>
 
 #include "reftet.hpp"
@@ -15,6 +16,7 @@
 namespace dgtd {
 
 const int kTriEdges[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+const int kQuadEdgeTable[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
 
 namespace {
 
@@ -186,38 +188,58 @@ void tri_interp_weights(const RefTri& R, double xi, double eta, double* w)
 
 TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha)
 {
+    (void)R;
+    return build_2d_mesh(in, 3, alpha);
+}
+
+TriMeshData build_2d_mesh(const TriMeshInput& in, int NV, double alpha)
+{
     const int K = in.K;
-    if (K <= 0 || in.nv <= 0 || !in.vx || !in.EToV) throw std::runtime_error("empty or incomplete triangle mesh");
-    for (size_t i = 0; i < 3 * (size_t)K; ++i)
+    const bool quad = NV == 4;
+    const int (*edges)[2] = quad ? kQuadEdgeTable : kTriEdges;
+    const int stride = quad ? 18 : kTriGeoStride, cstride = quad ? 8 : kTriConnStride;
+    const char* what = quad ? "quadrilateral" : "triangle";
+    if (K <= 0 || in.nv <= 0 || !in.vx || !in.EToV)
+        throw std::runtime_error(std::string("empty or incomplete ") + what + " mesh");
+    for (size_t i = 0; i < (size_t)NV * K; ++i)
         if (in.EToV[i] < 0 || in.EToV[i] >= in.nv) throw std::runtime_error("EToV vertex id out of range");
     TriMeshData md;
     md.K = K;
-    md.geo.assign((size_t)kTriGeoStride * K, 0.0);
-    md.conn.assign((size_t)kTriConnStride * K, 0);
+    md.geo.assign((size_t)stride * K, 0.0);
+    md.conn.assign((size_t)cstride * K, 0);
     md.jac.resize(K);
     md.eps.assign(K, 1.0);
     md.mu.assign(K, 1.0);
-    md.face_code.assign(3 * (size_t)K, 0);
+    md.face_code.assign((size_t)NV * K, 0);
     for (int e = 0; e < K; ++e) {
         if (in.eps_r) md.eps[e] = in.eps_r[e];
         if (in.mu_r) md.mu[e] = in.mu_r[e];
         if (!(md.eps[e] > 0) || !(md.mu[e] > 0)) throw std::runtime_error("eps_r and mu_r must be positive");
-        const double* v[3];
-        for (int k = 0; k < 3; ++k) v[k] = in.vx + 3 * (size_t)in.EToV[3 * (size_t)e + k];
+        const double* v[4];
+        for (int k = 0; k < NV; ++k) v[k] = in.vx + 3 * (size_t)in.EToV[(size_t)NV * e + k];
+        // affine map from [-1,1]^2 (quadrilateral, vertices 1 and 3 on the r and
+        // s axes) or from the reference triangle (vertices 1 and 2)
+        const double* vs = quad ? v[3] : v[2];
         const double xr = 0.5 * (v[1][0] - v[0][0]), yr = 0.5 * (v[1][1] - v[0][1]);
-        const double xs = 0.5 * (v[2][0] - v[0][0]), ys = 0.5 * (v[2][1] - v[0][1]);
+        const double xs = 0.5 * (vs[0] - v[0][0]), ys = 0.5 * (vs[1] - v[0][1]);
         const double J = xr * ys - xs * yr;
-        if (!(std::fabs(J) > 0)) throw std::runtime_error("degenerate triangle");
+        if (!(std::fabs(J) > 0)) throw std::runtime_error(std::string("degenerate ") + what);
+        if (quad) {  // a parallelogram: v2 = v1 + v3 - v0
+            const double scale = std::fabs(xr) + std::fabs(yr) + std::fabs(xs) + std::fabs(ys);
+            for (int d = 0; d < 2; ++d)
+                if (std::fabs(v[2][d] - (v[1][d] + v[3][d] - v[0][d])) > 1e-12 * scale)
+                    throw std::runtime_error("quadrilaterals must be parallelograms (affine elements)");
+        }
         md.jac[e] = std::fabs(J);
-        double* g = &md.geo[(size_t)kTriGeoStride * e];
+        double* g = &md.geo[(size_t)stride * e];
         g[0] = ys / J;   // dr/dx
         g[1] = -xs / J;  // dr/dy
         g[2] = -yr / J;  // ds/dx
         g[3] = xr / J;   // ds/dy
         const double sg = J > 0 ? 1.0 : -1.0;
-        for (int f = 0; f < 3; ++f) {
-            const double* a = v[kTriEdges[f][0]];
-            const double* b = v[kTriEdges[f][1]];
+        for (int f = 0; f < NV; ++f) {
+            const double* a = v[edges[f][0]];
+            const double* b = v[edges[f][1]];
             const double tx = b[0] - a[0], ty = b[1] - a[1];
             const double L = std::hypot(tx, ty);
             g[4 + 3 * f] = sg * ty / L;
@@ -226,16 +248,16 @@ TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha
             g[6 + 3 * f] = 0.5 * fs;
             md.max_fscale = std::max(md.max_fscale, fs);
         }
-        g[13] = 1.0 / md.eps[e];
-        g[14] = 1.0 / md.mu[e];
+        g[4 + 3 * NV] = 1.0 / md.eps[e];
+        g[5 + 3 * NV] = 1.0 / md.mu[e];
     }
     // edges by sorted vertex pair
     std::map<std::pair<int, int>, std::pair<int, int>> open;
-    std::vector<int> nbr(3 * (size_t)K, -1), code(3 * (size_t)K, 0);
+    std::vector<int> nbr((size_t)NV * K, -1), code((size_t)NV * K, 0);
     for (int e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            const int a = in.EToV[3 * (size_t)e + kTriEdges[f][0]], b = in.EToV[3 * (size_t)e + kTriEdges[f][1]];
-            if (a == b) throw std::runtime_error("triangle with repeated vertex");
+        for (int f = 0; f < NV; ++f) {
+            const int a = in.EToV[(size_t)NV * e + edges[f][0]], b = in.EToV[(size_t)NV * e + edges[f][1]];
+            if (a == b) throw std::runtime_error(std::string(what) + " with repeated vertex");
             const auto key = std::make_pair(std::min(a, b), std::max(a, b));
             auto it = open.find(key);
             if (it == open.end()) {
@@ -244,55 +266,55 @@ TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha
             }
             const int e2 = it->second.first, f2 = it->second.second;
             open.erase(it);
-            if (e2 == e) throw std::runtime_error("triangle shares an edge with itself");
-            const int a2 = in.EToV[3 * (size_t)e2 + kTriEdges[f2][0]];
+            if (e2 == e) throw std::runtime_error(std::string(what) + " shares an edge with itself");
+            const int a2 = in.EToV[(size_t)NV * e2 + edges[f2][0]];
             const int c = (a2 == a) ? 0 : 1;  // same or opposite traversal
-            nbr[3 * (size_t)e + f] = e2;
-            code[3 * (size_t)e + f] = (f * 3 + f2) * 2 + c;
-            nbr[3 * (size_t)e2 + f2] = e;
-            code[3 * (size_t)e2 + f2] = (f2 * 3 + f) * 2 + c;
+            nbr[(size_t)NV * e + f] = e2;
+            code[(size_t)NV * e + f] = (f * NV + f2) * 2 + c;
+            nbr[(size_t)NV * e2 + f2] = e;
+            code[(size_t)NV * e2 + f2] = (f2 * NV + f) * 2 + c;
         }
     // effective codes: an interior condition or TF/SF code given on one side
     // of an interior edge applies to both sides
-    std::vector<int> eff(3 * (size_t)K, 0);
+    std::vector<int> eff((size_t)NV * K, 0);
     for (size_t i = 0; i < eff.size(); ++i) {
         eff[i] = in.face_bc ? in.face_bc[i] : 0;
         if (eff[i] < 0 || eff[i] > 7) throw std::runtime_error("face_bc code out of range");
     }
     for (int e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            const size_t i = 3 * (size_t)e + f;
+        for (int f = 0; f < NV; ++f) {
+            const size_t i = (size_t)NV * e + f;
             if (nbr[i] < 0 || eff[i] < 4) continue;
-            const size_t j = 3 * (size_t)nbr[i] + (code[i] / 2) % 3;
+            const size_t j = (size_t)NV * nbr[i] + (code[i] / 2) % NV;
             if (eff[j] != 0 && eff[j] != eff[i])
                 throw std::runtime_error("conflicting interior conditions on the two sides of an edge");
             eff[j] = eff[i];
         }
     for (int e = 0; e < K; ++e)
-        for (int f = 0; f < 3; ++f) {
-            const size_t i = 3 * (size_t)e + f;
+        for (int f = 0; f < NV; ++f) {
+            const size_t i = (size_t)NV * e + f;
             const int bc = eff[i];
-            int32_t* cr = &md.conn[(size_t)kTriConnStride * e];
+            int32_t* cr = &md.conn[(size_t)cstride * e];
             if (nbr[i] < 0) {
                 if (bc >= 4) throw std::runtime_error("interior-boundary codes 4-7 on a boundary edge");
                 if (bc == 3 && alpha == 0.0) throw std::runtime_error("SMA boundaries need upwind flux (alpha != 0)");
                 cr[f] = -(bc + 1);
-                cr[3 + f] = 0;
+                cr[NV + f] = 0;
             } else if (bc == 7) {
                 // TF/SF edge: normal neighbour gather, plus the injected plane
                 // waves; the lower-index element is the scattered-field side
                 cr[f] = nbr[i];
-                cr[3 + f] = code[i];
+                cr[NV + f] = code[i];
                 md.tfsf.push_back({e, f, e < nbr[i] ? -1 : 1});
             } else if (bc >= 4) {
                 // interior condition: both sides use the own trace (no gather)
                 cr[f] = -(bc + 1);
-                cr[3 + f] = 0;
+                cr[NV + f] = 0;
             } else if (bc >= 1) {
                 throw std::runtime_error("PEC/PMC/SMA codes 1-3 on an interior edge (use 4-6)");
             } else {
                 cr[f] = nbr[i];
-                cr[3 + f] = code[i];
+                cr[NV + f] = code[i];
             }
             md.face_code[i] = (int8_t)bc;
         }

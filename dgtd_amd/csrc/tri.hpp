@@ -14,8 +14,10 @@ namespace dgtd {
 
 inline int tri_num_nodes(int p) { return (p + 1) * (p + 2) / 2; }
 
-// Local vertices of edge f, MFEM's triangle edge table: (0,1), (1,2), (2,0).
+// Local vertices of edge f, MFEM's triangle edge table: (0,1), (1,2), (2,0),
+// and its quadrilateral table: (0,1), (1,2), (2,3), (3,0).
 extern const int kTriEdges[3][2];
+extern const int kQuadEdgeTable[4][2];
 
 struct RefTri {
     int p = 0, Np = 0, Nfp = 0;
@@ -72,6 +74,9 @@ struct TriMeshData {
 // mesh: codes 1-3 only on boundary edges, 4-7 only on interior edges; a TF/SF
 // edge keeps its neighbour and its lower-index element is the scattered side).
 TriMeshData build_tri_mesh(const TriMeshInput& in, const RefTri& R, double alpha);
+// The same for NV vertices per element: 3 triangles, 4 parallelograms (records
+// in quad.hpp's layout: 18 geometry values, 8 connectivity ints).
+TriMeshData build_2d_mesh(const TriMeshInput& in, int NV, double alpha);
 
 // Physical node positions: 3 blocks of Np*K (z block zero).
 void tri_node_positions(const TriMeshInput& in, const RefTri& R, double* xyz);

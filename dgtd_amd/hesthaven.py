@@ -72,15 +72,59 @@ def square_mesh(n, L=1.0, bc="pec"):
     return vx, EToV, face_bc
 
 
-class _Mesh:
-    """Keeps the numpy arrays alive while the C descriptor points at them."""
+def cartesian_2d(nx, ny, shape="triangle", sx=1.0, sy=1.0, bc="pec"):
+    """nx x ny cells on [0,sx] x [0,sy], as mfem::Mesh::MakeCartesian2D: per
+    cell (a, b, c, d) = (i,j), (i+1,j), (i+1,j+1), (i,j+1) either one
+    counter-clockwise quadrilateral (shape "quad") or the triangles (a, b, c)
+    and (a, c, d) (shape "triangle").  bc: one code for all sides or four
+    (bottom y=0, right x=sx, top y=sy, left x=0 -- MakeCartesian2D's boundary
+    attributes 1..4).  Returns (vx (nv,3) with z = 0, EToV, face_bc), edge f
+    joining local vertices f and f+1."""
+    names = {"none": 0, "pec": 1, "pmc": 2, "sma": 3}
+    sides = [bc] * 4 if isinstance(bc, (str, int)) else list(bc)
+    side = [names[b] if isinstance(b, str) else int(b) for b in sides]
+    X, Y = np.meshgrid(np.linspace(0.0, sx, nx + 1), np.linspace(0.0, sy, ny + 1), indexing="xy")
+    vx = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    i, j = (g.ravel() for g in np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy"))
+    a = i + (nx + 1) * j
+    b, c, d = a + 1, a + nx + 2, a + nx + 1
+    bottom = np.where(j == 0, side[0], 0)
+    right = np.where(i == nx - 1, side[1], 0)
+    top = np.where(j == ny - 1, side[2], 0)
+    left = np.where(i == 0, side[3], 0)
+    if shape == "quad":
+        EToV = np.stack([a, b, c, d], axis=1).astype(np.int32)
+        face_bc = np.stack([bottom, right, top, left], axis=1).astype(np.int8)
+        return vx, EToV, face_bc
+    if shape != "triangle":
+        raise ValueError("shape must be 'triangle' or 'quad'")
+    EToV = np.empty((2 * a.size, 3), dtype=np.int32)
+    EToV[0::2] = np.stack([a, b, c], axis=1)
+    EToV[1::2] = np.stack([a, c, d], axis=1)
+    face_bc = np.zeros(EToV.shape, dtype=np.int8)
+    face_bc[0::2, 0], face_bc[0::2, 1] = bottom, right  # a-b, b-c
+    face_bc[1::2, 1], face_bc[1::2, 2] = top, left      # c-d, d-a
+    return vx, EToV, face_bc
 
-    def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None, sigma=None):
+
+class _Mesh:
+    """Keeps the numpy arrays alive while the C descriptor points at them.
+
+    Elements: triangles (EToV (K, 3)), tetrahedra (EToV (K, 4)), or
+    quadrilaterals: EToV (K, 4) with quads=True, or by default when the
+    vertices are planar (vx (nv, 2), or z = 0 throughout, which no valid
+    tetrahedron has)."""
+
+    def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None, sigma=None, quads=None):
         E = np.asarray(EToV)
-        # triangles (K, 3): the 2D operator; vertices may carry 2 or 3 coordinates
-        self.dim = 2 if E.ndim == 2 and E.shape[1] == 3 else 3
-        nvert = self.dim + 1
         v = np.asarray(vx, dtype=np.float64)
+        if quads is None:
+            planar = v.ndim == 2 and (v.shape[1] == 2 or (v.shape[1] == 3 and not v[:, 2].any()))
+            quads = E.ndim == 2 and E.shape[1] == 4 and planar
+        self.quads = bool(quads)
+        # triangles (K, 3) or quadrilaterals: the 2D operator; vertices may carry 2 or 3 coordinates
+        self.dim = 2 if self.quads or (E.ndim == 2 and E.shape[1] == 3) else 3
+        nvert = 4 if self.quads else self.dim + 1
         if self.dim == 2 and v.ndim == 2 and v.shape[1] == 2:
             v = np.hstack([v, np.zeros((v.shape[0], 1))])
         self.vx = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
@@ -91,7 +135,8 @@ class _Mesh:
         self.sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
         self.desc = _abi.MeshDesc(self.EToV.shape[0], order, dptr(self.vx), self.vx.shape[0],
                                   i32ptr(self.EToV), i8ptr(self.face_bc), dptr(self.eps),
-                                  dptr(self.mu), self.dim, dptr(self.sigma))
+                                  dptr(self.mu), self.dim, dptr(self.sigma), 4 if self.quads else 0)
+        self.Np = _nodes(order, self.dim, self.quads)
 
 
 def gaussian(field="E", center=(0.5, 0.5, 0.5), spread=0.1, polarization=(0, 0, 1), dimension=3):
@@ -137,15 +182,17 @@ def planewave(polarization, propagation, spread, mean=0.0, frequency=None):
     return w
 
 
-def _nodes(order, dim):
+def _nodes(order, dim, quads=False):
+    if quads:
+        return (order + 1) ** 2
     return num_nodes(order) if dim == 3 else (order + 1) * (order + 2) // 2
 
 
-def project_initial_fields(vx, EToV, order, sources):
+def project_initial_fields(vx, EToV, order, sources, quads=None):
     """Sum of the sources evaluated at the nodes: a 6*Np*K state vector."""
-    m = _Mesh(vx, EToV, None, order)
+    m = _Mesh(vx, EToV, None, order, quads=quads)
     arr = (_abi.InitialField * len(sources))(*sources)
-    x = np.empty(6 * _nodes(order, m.dim) * m.EToV.shape[0])
+    x = np.empty(6 * m.Np * m.EToV.shape[0])
     check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), arr, len(sources), dptr(x)))
     return x
 
@@ -175,12 +222,12 @@ def mesh_vmaps(vx, EToV, face_bc, order, alpha=1.0):
     return M, P, codes.reshape(K, 4)
 
 
-def mesh_locate(vx, EToV, order, points):
+def mesh_locate(vx, EToV, order, points, quads=None):
     """Host point location: (elem (n,), weights (n, Np)) as the probes use them."""
-    m = _Mesh(vx, EToV, None, order)
+    m = _Mesh(vx, EToV, None, order, quads=quads)
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     elem = np.zeros(pts.shape[0], dtype=np.int32)
-    w = np.zeros((pts.shape[0], _nodes(order, m.dim)))
+    w = np.zeros((pts.shape[0], m.Np))
     check(lib().dgtd_mesh_locate(ctypes.byref(m.desc), pts.shape[0], dptr(pts), i32ptr(elem), dptr(w)))
     return elem, w
 
@@ -192,14 +239,14 @@ def mesh_geometry(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None):
     return geo.reshape(-1, 27)
 
 
-def global_assemble(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, sigma=None):
+def global_assemble(vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, sigma=None, quads=None):
     """The assembled operator of the "csr" kernel on the host, FP64:
     (rowptr, col, val) of the ndofs x ndofs CSR matrix (the Global operator's
     matrix, DGOperatorFactory::buildGlobalOperator, DGOperatorFactory.h:1428-1600)."""
-    m = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma)
+    m = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma, quads=quads)
     nnz = ctypes.c_int64()
     check(lib().dgtd_global_assemble(ctypes.byref(m.desc), float(alpha), ctypes.byref(nnz), None, None, None))
-    n = 6 * _nodes(order, m.dim) * m.EToV.shape[0]
+    n = 6 * m.Np * m.EToV.shape[0]
     rowptr = np.zeros(n + 1, dtype=np.int64)
     col = np.zeros(nnz.value, dtype=np.int32)
     val = np.zeros(nnz.value)
@@ -215,15 +262,16 @@ class HesthavenEvolution:
     std::runtime_error (HesthavenEvolution.cpp:37,42,146)."""
 
     def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, precision=64,
-                 integrator="lserk4", device=0, kernel="auto", sigma=None):
+                 integrator="lserk4", device=0, kernel="auto", sigma=None, quads=None):
         self.order = order
         self.precision = precision
         self.integrator = integrator
         self.alpha = alpha
-        mesh = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma)
+        mesh = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma, quads=quads)
         self.dim = mesh.dim
+        self.quads = mesh.quads
         self.K = mesh.EToV.shape[0]
-        self.Np = _nodes(order, mesh.dim)
+        self.Np = mesh.Np
         self.Nfp = num_face_nodes(order) if mesh.dim == 3 else order + 1
         opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device),
                          KERNELS[kernel])
@@ -404,15 +452,36 @@ class GlobalEvolution(HesthavenEvolution):
     * Z0, Material.h:30) adding -sigma/eps_r E (DGOperatorFactory.h:1351-1361)."""
 
     def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, sigma=None, precision=64,
-                 integrator="lserk4", device=0):
+                 integrator="lserk4", device=0, quads=None):
         super().__init__(vx, EToV, face_bc, order, alpha=alpha, eps=eps, mu=mu, precision=precision,
-                         integrator=integrator, device=device, kernel="csr", sigma=sigma)
+                         integrator=integrator, device=device, kernel="csr", sigma=sigma, quads=quads)
 
 
-def estimate_time_step(op, cfl=1.0):
-    """Hesthaven 3D rule of Solver::estimateTimeStep (Solver.cpp:334-346):
-    dt = cfl / (max fscale * p^2)."""
-    return 1.0 * cfl / (op.max_fscale() * op.order * op.order)
+OPERATOR_TYPES = {"maxwell": _abi.DGTD_OPERATOR_MAXWELL, "global": _abi.DGTD_OPERATOR_GLOBAL,
+                  "hesthaven": _abi.DGTD_OPERATOR_HESTHAVEN}
+
+
+def mesh_time_step(vx, EToV, order, evolution_operator="global", cfl=1.0, quads=None):
+    """estimate_time_step from the mesh alone, on the host."""
+    m = _Mesh(vx, EToV, None, order, quads=quads)
+    dt = ctypes.c_double()
+    check(lib().dgtd_mesh_estimate_time_step(ctypes.byref(m.desc), OPERATOR_TYPES[evolution_operator], float(cfl),
+                                             ctypes.byref(dt)))
+    return dt.value
+
+
+def estimate_time_step(op, cfl=1.0, evolution_operator=None):
+    """Solver::estimateTimeStep (Solver.cpp:302-353) for op's mesh under the
+    named evolution operator (default: "global" for a GlobalEvolution, else
+    "hesthaven"): cfl / (max fscale * p^2) on tetrahedra under hesthaven,
+    otherwise the mesh rule cfl * 0.75 * (2/3) * rmin * min vol/(surface/2),
+    halved on quadrilaterals, / 0.8 on tetrahedra under Global."""
+    if evolution_operator is None:
+        evolution_operator = "global" if isinstance(op, GlobalEvolution) else "hesthaven"
+    dt = ctypes.c_double()
+    check(lib().dgtd_op_estimate_time_step(op._h, OPERATOR_TYPES[evolution_operator], float(cfl),
+                                           ctypes.byref(dt)))
+    return dt.value
 
 
 class Solver:
@@ -421,10 +490,10 @@ class Solver:
     time_step = 0 selects the automatic rule, as SolverOptions::time_step = 0
     does (Solver.cpp:111-116)."""
 
-    def __init__(self, op, initial_state, final_time=2.0, time_step=0.0, cfl=1.0):
+    def __init__(self, op, initial_state, final_time=2.0, time_step=0.0, cfl=1.0, evolution_operator=None):
         self.op = op
         self.final_time = final_time
-        self.dt = estimate_time_step(op, cfl) if time_step == 0.0 else time_step
+        self.dt = estimate_time_step(op, cfl, evolution_operator) if time_step == 0.0 else time_step
         self.time = 0.0
         op.set_state(initial_state, 0.0)
 

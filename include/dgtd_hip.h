@@ -93,6 +93,14 @@ typedef struct {
                              * NULL = 0.  Only with DGTD_KERNEL_CSR: the Global operator's
                              * -M_eps^-1 M_sigma E term (collectGlobalConductiveOperator,
                              * DGOperatorFactory.h:1351-1361), -sigma/eps_r E per element. */
+    int32_t nvert;          /* vertices per element with dim = 2: 0 or 3 triangles, 4
+                             * quadrilaterals -- parallelograms, on the assembled (Global)
+                             * operator only (DGTD_KERNEL_CSR; AUTO selects it): the
+                             * reference's default operator on its quadrilateral cases
+                             * (ExtensiveCasesTest.cpp:1003,1082).  EToV and face_bc hold 4
+                             * entries per element, edge f joins local vertices (f, f+1 mod
+                             * 4), nodes are the (p+1)^2 Gauss-Lobatto tensor points in
+                             * MFEM's lexicographic order. */
 } dgtd_mesh_desc;
 
 /* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)
@@ -136,6 +144,17 @@ int dgtd_op_np(const dgtd_op* op, int32_t* Np);
 /* Largest fscale over all element faces; Solver::estimateTimeStep reads it
  * through getHesthavenElement(e).fscale (Solver.cpp:337-340). */
 int dgtd_op_max_fscale(const dgtd_op* op, double* out);
+
+/* The reference's automatic time step (estimateTimeStep, Solver.cpp:302-353)
+ * for this operator's mesh under the named evolution operator
+ * (DGTD_OPERATOR_*): on tetrahedra under hesthaven cfl / (max fscale * p^2);
+ * otherwise cfl * 0.75 * (2/3) * rmin * min_e vol_e / (surface_e / 2)
+ * (calcMeshTimeStep and getTimeStepScale, Solver.cpp:207-300; rmin is the
+ * first Gauss-Lobatto spacing of order p on [-1, 1], speed of light 1),
+ * halved on quadrilaterals and divided by 0.8 on tetrahedra under Global. */
+int dgtd_op_estimate_time_step(const dgtd_op* op, int32_t evolution_operator, double cfl, double* dt);
+/* The same from a mesh description, on the host (no device work). */
+int dgtd_mesh_estimate_time_step(const dgtd_mesh_desc* mesh, int32_t evolution_operator, double cfl, double* dt);
 
 /* Physical node positions, 3*Np*K doubles: x block, y block, z block. */
 int dgtd_op_node_positions(const dgtd_op* op, double* xyz);
@@ -357,7 +376,10 @@ typedef struct {
     int32_t snapshot_steps;       /* probes.domain_snapshot: "steps" (0 if absent), -1 = no snapshots */
     int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
     int32_t nplanewaves;          /* "planewave" sources (their tags are TF/SF faces) */
-    int32_t dim;                  /* 3: tetrahedral mesh; 2: triangle mesh (2D operator) */
+    int32_t dim;                  /* 3: tetrahedral mesh; 2: triangle or quadrilateral mesh
+                                   * (dgtd_case_mesh sets nvert = 4 for quadrilaterals, which
+                                   * run on the assembled operator; a quadrilateral case naming
+                                   * the hesthaven operator is refused) */
     int32_t nfield_probes;        /* probes.field entries */
 } dgtd_case_info;
 
@@ -379,6 +401,10 @@ int dgtd_case_sources(const dgtd_case* c, dgtd_initial_field* out);
  * a state as MFEM GridFunction files Ex.gf ... Hz.gf (L2 GaussLobatto space of
  * the given order, 16 significant digits) plus time.txt in directory dir. */
 int dgtd_case_save_mesh(const dgtd_case* c, const char* path);
+/* The case's time step: solver_options.time_step when > 0, else
+ * dgtd_mesh_estimate_time_step on the case's mesh under its evolution
+ * operator and cfl (Solver.cpp:110-115). */
+int dgtd_case_time_step(const dgtd_case* c, double* dt);
 int dgtd_save_fields(const char* dir, int32_t order, int64_t ndofs, const double* x, double t);
 /* The same with the case's dimension and order (L2_T1_2D_P<p> on triangles). */
 int dgtd_case_save_fields(const dgtd_case* c, const char* dir, int64_t ndofs, const double* x, double t);

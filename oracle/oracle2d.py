@@ -18,6 +18,16 @@ Proriol-Koornwinder-Dubiner basis (Hesthaven & Warburton 2008, ch. 6),
 Dr, Ds = V_r V^-1, V_s V^-1, M = (V V^T)^-1, LIFT = M^-1 E with E the edge
 mass matrices (reference edge length 2); fscale = (edge length / 2) / J with
 J = area / 2 (reference area 2, HesthavenEvolution.cpp:133-147).
+
+Quadrilaterals (RefQuad, Operator2D(quads=True)): the reference runs them on
+its Global operator, which assembles the same semi-discrete DG operator over
+the L2 Gauss-Lobatto space of the quadrilateral (DGOperatorFactory.h:
+1428-1600, ExtensiveCasesTest.cpp:1003,1082).  The element is the tensor
+product of the p+1 Gauss-Lobatto points in MFEM's lexicographic order with
+the orthonormal Legendre tensor basis; elements are parallelograms (affine
+maps, J = area / 4), edges (v_f, v_{f+1 mod 4}).  Pinned in
+tests/test_oracle_quad.py by closed forms (no quadrilateral matrices exist
+in the reference's tests).
 """
 import numpy as np
 
@@ -163,53 +173,114 @@ class RefTri:
         return self.Vinv.T @ P
 
 
+class RefQuad:
+    """Reference quadrilateral [-1, 1]^2 of order p: the tensor product of the
+    p+1 Gauss-Lobatto-Legendre points in MFEM's lexicographic L2 order (node
+    i + (p+1) j at (x_i, x_j)), the basis of products of orthonormal Legendre
+    polynomials, Dr, Ds = V_r V^-1, V_s V^-1, M = (V V^T)^-1 (exact mass),
+    edges (v0,v1), (v1,v2), (v2,v3), (v3,v0) of MFEM's quadrilateral with
+    vertices (-1,-1), (1,-1), (1,1), (-1,1), LIFT = M^-1 E with E the edge
+    masses (reference edge length 2).  xi = (r + 1) / 2, (s + 1) / 2."""
+
+    EDGES = ((0, 1), (1, 2), (2, 3), (3, 0))
+    AXES = (1, 3)  # x = v0 + xi (v1 - v0) + eta (v3 - v0)
+
+    def __init__(self, p):
+        self.p = p
+        n1 = p + 1
+        g = gll01(p)
+        self.xi = np.array([(g[i], g[j]) for j in range(n1) for i in range(n1)])
+        self.Np, self.Nfp = n1 * n1, n1
+        r, s = 2 * self.xi[:, 0] - 1, 2 * self.xi[:, 1] - 1
+        self.modes = [(a, b) for b in range(n1) for a in range(n1)]
+        self.V = np.stack([jacobi_p(r, 0, 0, a) * jacobi_p(s, 0, 0, b) for a, b in self.modes], axis=1)
+        Vr = np.stack([grad_jacobi_p(r, 0, 0, a) * jacobi_p(s, 0, 0, b) for a, b in self.modes], axis=1)
+        Vs = np.stack([jacobi_p(r, 0, 0, a) * grad_jacobi_p(s, 0, 0, b) for a, b in self.modes], axis=1)
+        self.Vinv = np.linalg.inv(self.V)
+        self.Dr = Vr @ self.Vinv
+        self.Ds = Vs @ self.Vinv
+        self.Minv = self.V @ self.V.T
+        self.M = np.linalg.inv(self.Minv)
+        idx = lambda i, j: i + n1 * j
+        self.Fmask = np.array([[idx(i, 0) for i in range(n1)],              # v0 -> v1: s = -1
+                               [idx(p, j) for j in range(n1)],              # v1 -> v2: r = 1
+                               [idx(i, p) for i in reversed(range(n1))],    # v2 -> v3: s = 1
+                               [idx(0, j) for j in reversed(range(n1))]])   # v3 -> v0: r = -1
+        tg = 2 * g - 1  # edge coordinate of the k-th node from the edge's first vertex
+        V1 = np.stack([jacobi_p(tg, 0, 0, n) for n in range(n1)], axis=1)
+        M1 = np.linalg.inv(V1 @ V1.T)
+        E = np.zeros((self.Np, 4 * n1))
+        for f in range(4):
+            E[np.ix_(self.Fmask[f], f * n1 + np.arange(n1))] = M1
+        self.E = E
+        self.LIFT = self.Minv @ E
+
+    def interp_weights(self, xi, eta):
+        """w with u(xi, eta) = sum_n w[n] u_n."""
+        r, s = np.array([2 * xi - 1.0]), np.array([2 * eta - 1.0])
+        P = np.array([(jacobi_p(r, 0, 0, a) * jacobi_p(s, 0, 0, b))[0] for a, b in self.modes])
+        return self.Vinv.T @ P
+
+
+RefTri.EDGES = EDGES
+RefTri.AXES = (1, 2)  # x = v0 + xi (v1 - v0) + eta (v2 - v0)
+
+
 class Operator2D:
-    """The 2D Hesthaven operator on a triangle mesh.
+    """The 2D operator on a triangle mesh, or with quads=True on a mesh of
+    parallelograms (the Global operator's quadrilateral cases: the same
+    semi-discrete operator on the tensor-product element).
 
-    vx: (nv, 2) or (nv, 3) vertex coordinates (z ignored); EToV: (K, 3);
-    face_bc: (K, 3) codes per edge f = (v_f, v_{f+1 mod 3}); eps, mu: (K,)."""
+    vx: (nv, 2) or (nv, 3) vertex coordinates (z ignored); EToV: (K, n);
+    face_bc: (K, n) codes per edge f = (v_f, v_{f+1 mod n}), n = 3 or 4;
+    eps, mu: (K,)."""
 
-    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, global_sma=False):
+    def __init__(self, vx, EToV, face_bc, order, alpha=1.0, eps=None, mu=None, global_sma=False, quads=False):
         # global_sma: SMA edges (codes 3 and 6) at upwind weight 1 whatever
         # alpha, as the reference's Global operator assembles them
         # (DGOperatorFactory.h:21-35, 485-567, 575-665)
         self.global_sma = bool(global_sma)
-        self.R = R = RefTri(order)
+        self.R = R = RefQuad(order) if quads else RefTri(order)
+        self.nf = nf = len(R.EDGES)
         vx = np.asarray(vx, dtype=float)[:, :2]
-        EToV = np.asarray(EToV, dtype=np.int64)
+        EToV = np.asarray(EToV, dtype=np.int64).reshape(-1, nf)
         self.K = K = EToV.shape[0]
         self.Np, self.Nfp = R.Np, R.Nfp
         self.alpha = float(alpha)
         self.eps = np.ones(K) if eps is None else np.asarray(eps, dtype=float)
         self.mu = np.ones(K) if mu is None else np.asarray(mu, dtype=float)
-        v0, v1, v2 = vx[EToV[:, 0]], vx[EToV[:, 1]], vx[EToV[:, 2]]
-        xr, xs = 0.5 * (v1 - v0), 0.5 * (v2 - v0)
+        verts = [vx[EToV[:, j]] for j in range(nf)]
+        v0, va, vb = verts[0], verts[R.AXES[0]], verts[R.AXES[1]]
+        xr, xs = 0.5 * (va - v0), 0.5 * (vb - v0)
         J = xr[:, 0] * xs[:, 1] - xs[:, 0] * xr[:, 1]
         if np.any(np.abs(J) < 1e-300):
-            raise ValueError("degenerate triangle")
+            raise ValueError("degenerate element")
+        if quads:
+            gap = np.abs(verts[2] - (va + vb - v0)).max(axis=1)
+            if np.any(gap > 1e-10 * np.sqrt(np.abs(J))):
+                raise ValueError("quadrilateral is not a parallelogram")
         self.J = J
         # d/dx = rx d/dr + sx d/ds, d/dy = ry d/dr + sy d/ds
         self.rx, self.ry = xs[:, 1] / J, -xs[:, 0] / J
         self.sx, self.sy = -xr[:, 1] / J, xr[:, 0] / J
-        self.pos = (v0[:, None, :] + R.xi[None, :, 0:1] * (v1 - v0)[:, None, :]
-                    + R.xi[None, :, 1:2] * (v2 - v0)[:, None, :])  # (K, Np, 2)
-        verts = [v0, v1, v2]
-        self.nrm = np.zeros((K, 3, 2))
-        self.fscale = np.zeros((K, 3))
+        self.pos = (v0[:, None, :] + R.xi[None, :, 0:1] * (va - v0)[:, None, :]
+                    + R.xi[None, :, 1:2] * (vb - v0)[:, None, :])  # (K, Np, 2)
+        self.nrm = np.zeros((K, nf, 2))
+        self.fscale = np.zeros((K, nf))
         sgn = np.sign(J)
-        for f, (a, b) in enumerate(EDGES):
+        for f, (a, b) in enumerate(R.EDGES):
             t = verts[b] - verts[a]
             Lf = np.hypot(t[:, 0], t[:, 1])
             self.nrm[:, f, 0] = sgn * t[:, 1] / Lf
             self.nrm[:, f, 1] = -sgn * t[:, 0] / Lf
             self.fscale[:, f] = (Lf / 2) / np.abs(J)
         # connectivity: shared edges by vertex pair, node matching by position
-        bc = np.asarray(face_bc, dtype=np.int64).reshape(K, 3).copy()
+        bc = np.asarray(face_bc, dtype=np.int64).reshape(K, nf).copy()
         owner = {}
-        nbr = -np.ones((K, 3), dtype=np.int64)
-        nbrf = -np.ones((K, 3), dtype=np.int64)
+        nbr = -np.ones((K, nf), dtype=np.int64)
+        nbrf = -np.ones((K, nf), dtype=np.int64)
         for e in range(K):
-            for f, (a, b) in enumerate(EDGES):
+            for f, (a, b) in enumerate(R.EDGES):
                 key = tuple(sorted((int(EToV[e, a]), int(EToV[e, b]))))
                 if key in owner:
                     e2, f2 = owner.pop(key)
@@ -218,14 +289,14 @@ class Operator2D:
                     owner[key] = (e, f)
         # an interior condition or TF/SF code on one side applies to both
         for e in range(K):
-            for f in range(3):
+            for f in range(nf):
                 if nbr[e, f] >= 0 and bc[e, f] >= 4:
                     e2, f2 = nbr[e, f], nbrf[e, f]
                     if bc[e2, f2] not in (0, bc[e, f]):
                         raise ValueError("conflicting interior conditions on the two sides of an edge")
                     bc[e2, f2] = bc[e, f]
         for e in range(K):
-            for f in range(3):
+            for f in range(nf):
                 if nbr[e, f] >= 0 and 1 <= bc[e, f] <= 3:
                     raise ValueError("PEC/PMC/SMA codes 1-3 on an interior edge (use 4-6)")
                 if nbr[e, f] < 0 and bc[e, f] >= 4:
@@ -234,10 +305,10 @@ class Operator2D:
                     raise ValueError("SMA boundaries need upwind flux (alpha != 0)")
         Nfp = R.Nfp
         h = np.min(np.abs(J)) ** 0.5
-        self.vmapM = np.zeros((K, 3, Nfp), dtype=np.int64)
-        self.vmapP = np.zeros((K, 3, Nfp), dtype=np.int64)
+        self.vmapM = np.zeros((K, nf, Nfp), dtype=np.int64)
+        self.vmapP = np.zeros((K, nf, Nfp), dtype=np.int64)
         for e in range(K):
-            for f in range(3):
+            for f in range(nf):
                 own = e * R.Np + R.Fmask[f]
                 self.vmapM[e, f] = own
                 e2 = nbr[e, f]
@@ -304,9 +375,10 @@ class Operator2D:
         R, K, Np, Nfp = self.R, self.K, self.Np, self.Nfp
         q = np.asarray(x, dtype=float).reshape(6, K * Np)
         M, P = self.vmapM.reshape(K, -1), self.vmapP.reshape(K, -1)
-        dF = q[:, P] - q[:, M]  # (6, K, 3Nfp): jump = exterior - interior
+        nf = self.nf
+        dF = q[:, P] - q[:, M]  # (6, K, nf Nfp): jump = exterior - interior
         own = q[:, M]
-        code = np.repeat(self.bc, Nfp, axis=1)  # (K, 3Nfp)
+        code = np.repeat(self.bc, Nfp, axis=1)  # (K, nf Nfp)
         # per edge-node upwind weight: alpha, or 1 on SMA edges (global_sma)
         a = np.where((code == 3) | (code == 6), 1.0, self.alpha) if self.global_sma else np.full(code.shape, self.alpha)
         sma = np.divide(-1.0, a, out=np.zeros_like(a), where=a != 0.0)
@@ -317,13 +389,13 @@ class Operator2D:
             dF[c] = np.where(ovr, ce * own[c], dF[c])
             dF[3 + c] = np.where(ovr, ch * own[3 + c], dF[3 + c])
         if self.waves and np.any(self.tfsf):
-            sg = np.repeat(self.tfsf, Nfp, axis=1)  # (K, 3Nfp)
-            pM = self.pos.reshape(-1, 2)[M.reshape(-1)].reshape(K, 3 * Nfp, 2)
+            sg = np.repeat(self.tfsf, Nfp, axis=1)  # (K, nf Nfp)
+            pM = self.pos.reshape(-1, 2)[M.reshape(-1)].reshape(K, nf * Nfp, 2)
             Ei, Hi = self.incident(pM, t)
             for c in range(3):
                 dF[c] = dF[c] + sg * Ei[c]
                 dF[3 + c] = dF[3 + c] + sg * Hi[c]
-        n = np.repeat(self.nrm, Nfp, axis=1)  # (K, 3Nfp, 2)
+        n = np.repeat(self.nrm, Nfp, axis=1)  # (K, nf Nfp, 2)
         nv = [n[..., 0], n[..., 1], np.zeros_like(n[..., 0])]
         ndE = nv[0] * dF[0] + nv[1] * dF[1] + nv[2] * dF[2]
         ndH = nv[0] * dF[3] + nv[1] * dF[4] + nv[2] * dF[5]

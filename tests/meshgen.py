@@ -288,3 +288,36 @@ def read_gmsh22_2d(path):
         elif etype == 1:
             segs.append((nodes[0], nodes[1], tag))
     return np.array(xyz), np.array(tris, dtype=np.int64), segs
+
+
+# ---- quadrilaterals (parallelograms, the assembled operator) ----
+
+QUAD_EDGES = ((0, 1), (1, 2), (2, 3), (3, 0))
+
+
+def quad_mesh(nx, ny, Lx=1.0, Ly=1.0, bc="pec", stretch=0.0, shear=0.0, seed=0):
+    """nx x ny counter-clockwise quadrilaterals (a, b, c, d) = (i,j), (i+1,j),
+    (i+1,j+1), (i,j+1) on [0,Lx] x [0,Ly]; stretch varies the column widths
+    and row heights by up to that fraction (rectangles of unequal sizes), and
+    shear maps x -> x + shear * y afterwards, so every element stays a
+    parallelogram.  bc: one code for all sides or a list for x=0, x=Lx, y=0,
+    y=Ly (the sides of the mesh before the shear).  Returns vx (nv, 3) with
+    z = 0, EToV (K, 4), face_bc (K, 4), edge f = (v_f, v_{f+1 mod 4})."""
+    rng = np.random.default_rng(seed)
+
+    def ticks(n, L):
+        w = 1.0 + stretch * rng.uniform(-1, 1, n)
+        return np.concatenate([[0.0], np.cumsum(w) / w.sum() * L])
+
+    gx, gy = ticks(nx, Lx), ticks(ny, Ly)
+    X, Y = np.meshgrid(gx, gy, indexing="xy")
+    vx = np.stack([X.ravel() + shear * Y.ravel(), Y.ravel(), np.zeros(X.size)], axis=1)
+    codes = [BC[bc]] * 4 if isinstance(bc, str) else [BC[c] if isinstance(c, str) else int(c) for c in bc]
+    EToV, face_bc = [], []
+    for j in range(ny):
+        for i in range(nx):
+            a = i + (nx + 1) * j
+            EToV.append((a, a + 1, a + nx + 2, a + nx + 1))
+            face_bc.append((codes[2] if j == 0 else 0, codes[1] if i == nx - 1 else 0,
+                            codes[3] if j == ny - 1 else 0, codes[0] if i == 0 else 0))
+    return vx, np.array(EToV, dtype=np.int32), np.array(face_bc, dtype=np.int32)

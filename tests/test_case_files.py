@@ -432,16 +432,147 @@ def test_2d_case_refusals(tmp_path):
     shutil.copy(os.path.join(CASES, "2D_PEC.msh"), tmp_path / "2D_PEC.msh")
     case["model"]["refinement"] = 1
     (tmp_path / "r.json").write_text(json.dumps(case))
-    with pytest.raises(D.DgtdError, match="refinement of triangle meshes"):
+    with pytest.raises(D.DgtdError, match="refinement of 2D meshes"):
         D.Case(str(tmp_path / "r.json"))
-    # quadrilateral meshes (Gmsh element type 3) and curved (second-order)
-    # triangles (type 9) are refused by name
-    for etype, nodes, msg in ((3, "1 2 3 4", "quadrilateral"), (9, "1 2 3 4 5 6", "element type 9")):
+    # curved (second-order) triangles (type 9), hexahedra (type 5) and meshes
+    # mixing triangles and quadrilaterals are refused by name
+    for elems, msg in (("1\n1 9 2 1 1 1 2 3 4 5 6\n", "element type 9"),
+                       ("1\n1 5 2 1 1 1 2 3 4 5 6 1 2\n", "hexahedral"),
+                       ("2\n1 2 2 1 1 1 2 3\n2 3 2 1 1 1 2 3 4\n", "mixing triangles and quadrilaterals")):
         msh = ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n6\n1 0 0 0\n2 1 0 0\n3 1 1 0\n4 0 1 0\n"
-               "5 0.5 0 0\n6 0.5 0.5 0\n$EndNodes\n$Elements\n1\n1 %d 2 1 1 %s\n$EndElements\n" % (etype, nodes))
+               "5 0.5 0 0\n6 0.5 0.5 0\n$EndNodes\n$Elements\n%s$EndElements\n" % elems)
         (tmp_path / "q.msh").write_text(msh)
         case["model"]["refinement"] = 0
         case["model"]["filename"] = "q.msh"
         (tmp_path / "q.json").write_text(json.dumps(case))
         with pytest.raises(D.DgtdError, match=msg):
             D.Case(str(tmp_path / "q.json"))
+
+
+QUADS2D = os.path.join(CASES, "2D_TFSF_Quads.json")
+
+
+def test_2d_quad_case_elements_codes_and_time_step(tmp_path):
+    """testData/maxwellInputs/2D_TFSF_Quads: Gmsh quadrilaterals (type 3) are
+    the elements, lines the boundary; tags 1, 3, 5, 7 PEC, 4, 6 PMC and the
+    plane wave's tag 2 the TF/SF line x = 1 (code 7 on both sides); the
+    case's time step is taken as given."""
+    c = D.Case(QUADS2D)
+    assert c.dim == 2 and c.quads and c.K == 60 and c.EToV.shape == (60, 4) and c.Np == 16
+    assert c.info["evolution_operator"] == "global" and c.time_step() == 0.01
+    v = c.vx[c.EToV][:, :, :2]
+    area2 = np.sum(v[:, :, 0] * np.roll(v[:, :, 1], -1, 1) - np.roll(v[:, :, 0], -1, 1) * v[:, :, 1], 1)
+    assert (area2 > 0).all()
+    np.testing.assert_allclose(area2.sum() / 2, 4.0, rtol=1e-14)  # [0,4] x [0,1]
+    # element vertex sets as in the file
+    with open(os.path.join(CASES, "2D_TFSF_Quads.msh")) as fh:
+        lines = [ln.split() for ln in fh]
+    i = next(k for k, ln in enumerate(lines) if ln == ["$Elements"])
+    rows = [list(map(int, ln)) for ln in lines[i + 2: i + 2 + int(lines[i + 1][0])]]
+    quads = sorted(tuple(sorted(r[5:9])) for r in rows if r[1] == 3)
+    assert sorted(tuple(sorted(q + 1)) for q in c.EToV.tolist() for q in [np.array(q)]) == quads
+    segs = {tuple(sorted((r[5] - 1, r[6] - 1))): r[4] for r in rows if r[1] == 1}
+    code_of = {1: 1, 3: 1, 5: 1, 7: 1, 4: 2, 6: 2, 2: 7}
+    seen = {}
+    for e in range(c.K):
+        for f in range(4):
+            key = tuple(sorted((int(c.EToV[e, f]), int(c.EToV[e, (f + 1) % 4]))))
+            if key in segs:
+                assert c.face_bc[e, f] == code_of[segs[key]]
+                seen[key] = seen.get(key, 0) + 1
+            else:
+                assert c.face_bc[e, f] == 0
+    assert len(seen) == len(segs) == 42
+    assert sum(n for k, n in seen.items() if segs[k] == 2) == 8  # 4 TF/SF edges, both sides
+    assert np.linalg.norm(c.initial_state()) == 0.0
+    w = c.planewaves[0]
+    assert w.mean == pytest.approx(1.0 - 5 * 0.4 * np.sqrt(2), rel=1e-14)
+    # a hesthaven variant of the case is refused (the reference disables it,
+    # ExtensiveCasesTest.cpp:1159)
+    case = json.load(open(QUADS2D))
+    shutil.copy(os.path.join(CASES, "2D_TFSF_Quads.msh"), tmp_path / "2D_TFSF_Quads.msh")
+    case["solver_options"]["evolution_operator"] = "hesthaven"
+    (tmp_path / "h.json").write_text(json.dumps(case))
+    with pytest.raises(D.DgtdError, match="Global operator only") as ei:
+        D.Case(str(tmp_path / "h.json"))
+    assert ei.value.code == -4
+
+
+def test_2d_quad_snapshot_mesh_round_trip(tmp_path):
+    c = D.Case(QUADS2D)
+    c.save_mesh(tmp_path / "mesh.mesh")
+    text = (tmp_path / "mesh.mesh").read_text()
+    assert "dimension\n2\n" in text and "\n1 3 " in text  # attribute 1, geometry 3 (square)
+    case = json.load(open(QUADS2D))
+    case["model"]["filename"] = str(tmp_path / "mesh.mesh")
+    (tmp_path / "c.json").write_text(json.dumps(case))
+    c2 = D.Case(str(tmp_path / "c.json"))
+    assert c2.quads
+    np.testing.assert_array_equal(c2.EToV, c.EToV)
+    np.testing.assert_allclose(c2.vx, c.vx, atol=1e-15)
+    np.testing.assert_array_equal(c2.face_bc, c.face_bc)
+
+
+def _mesh_rule(vx, EToV, order, cfl=1.0):
+    """calcMeshTimeStep (Solver.cpp:290-300) restated: 0.75 * (2/3) * rmin *
+    min vol / (surface / 2), rmin from the order's Gauss-Lobatto points."""
+    from oracle import oracle2d as O2
+    v = np.asarray(vx)[np.asarray(EToV)]
+    if v.shape[1] == 4 and not np.asarray(vx)[:, 2].any():  # quadrilaterals
+        d = v[:, :, :2]
+        vol = 0.5 * np.abs(np.sum(d[:, :, 0] * np.roll(d[:, :, 1], -1, 1) - np.roll(d[:, :, 0], -1, 1) * d[:, :, 1], 1))
+        surf = np.linalg.norm(np.roll(d, -1, 1) - d, axis=2).sum(1)
+    elif v.shape[1] == 3:
+        d = v[:, :, :2]
+        e1, e2 = d[:, 1] - d[:, 0], d[:, 2] - d[:, 0]
+        vol = 0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+        surf = np.linalg.norm(np.roll(d, -1, 1) - d, axis=2).sum(1)
+    else:
+        vol = np.abs(volumes(np.asarray(vx), np.asarray(EToV)))
+        surf = sum(0.5 * np.linalg.norm(np.cross(v[:, b] - v[:, a], v[:, c] - v[:, a]), axis=1)
+                   for a, b, c in MG.FACE_VERTS)
+    g = O2.gll01(order)
+    return cfl * 0.75 * (2 / 3) * 2 * (g[1] - g[0]) * np.min(vol / (surf / 2))
+
+
+def test_automatic_time_step_follows_the_reference_rule():
+    """estimateTimeStep (Solver.cpp:302-353): the mesh rule in 2D under any
+    operator (halved on quadrilaterals) and in 3D under Maxwell (/0.8 under
+    Global); the fscale rule only for hesthaven on tetrahedra."""
+    vx, E, _ = D.cartesian_2d(14, 1, "quad")
+    want = _mesh_rule(vx, E, 3) / 2
+    for op in ("global", "hesthaven", "maxwell"):
+        assert D.mesh_time_step(vx, E, 3, op) == pytest.approx(want, rel=1e-14)
+    # hand check: cells 1/14 x 1, p = 3: rmin = 1 - 1/sqrt(5)
+    h = (1 / 14) / ((2 / 14 + 2) / 2)
+    assert want == pytest.approx(0.75 * (2 / 3) * (1 - 1 / np.sqrt(5)) * h / 2, rel=1e-14)
+    vx, E, _ = D.cartesian_2d(5, 3, "triangle", sx=2.0)
+    assert D.mesh_time_step(vx, E, 4, "global", cfl=0.5) == pytest.approx(_mesh_rule(vx, E, 4, 0.5), rel=1e-14)
+    vx, E, _ = D.cube_mesh(2)
+    base = _mesh_rule(vx, E, 2)
+    assert D.mesh_time_step(vx, E, 2, "maxwell") == pytest.approx(base, rel=1e-14)
+    assert D.mesh_time_step(vx, E, 2, "global") == pytest.approx(base / 0.8, rel=1e-14)
+    # hesthaven: 1 / (max fscale p^2), fscale = (face area / 2) / (volume * 3 / 4)
+    v = np.asarray(vx)[np.asarray(E)]
+    fs = max(np.max(0.5 * np.linalg.norm(np.cross(v[:, b] - v[:, a], v[:, c] - v[:, a]), axis=1) / 2
+                    / (np.abs(volumes(np.asarray(vx), np.asarray(E))) * 3 / 4)) for a, b, c in MG.FACE_VERTS)
+    assert D.mesh_time_step(vx, E, 2, "hesthaven") == pytest.approx(1.0 / (fs * 4), rel=1e-13)
+    # a case with time_step 0 takes the estimate of its mesh and operator
+    c = D.Case(QUADS2D)
+    assert c.time_step() == 0.01
+
+
+def test_case_without_time_step_takes_the_estimate(tmp_path):
+    for name in ("2D_TFSF_Quads", "2D_PEC", "3D_PEC"):
+        case = json.load(open(os.path.join(CASES, name + ".json")))
+        shutil.copy(os.path.join(CASES, name + ".msh"), tmp_path / (name + ".msh"))
+        case["solver_options"]["time_step"] = 0.0
+        case["solver_options"]["cfl"] = 0.7
+        for op in ("global", "hesthaven"):
+            if name.startswith("2D_TFSF_Quads") and op == "hesthaven":
+                continue
+            case["solver_options"]["evolution_operator"] = op
+            (tmp_path / "c.json").write_text(json.dumps(case))
+            c = D.Case(str(tmp_path / "c.json"))
+            want = D.mesh_time_step(c.vx, c.EToV, c.order, op, cfl=0.7, quads=c.quads)
+            assert c.time_step() == want > 0

@@ -183,21 +183,46 @@ def test_tri_matrix_equals_oracle_mult(order):
         assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
-@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_quad_matrix_equals_oracle_mult(order):
+    """Quadrilaterals (parallelograms of unequal sizes, sheared): boundary
+    PEC/PMC/SMA, interior PEC/PMC/SMA and TF/SF edges (without plane waves a
+    TF/SF edge is a plain interior edge), SMA at upwind weight 1 whatever
+    alpha, as the Global operator assembles it."""
+    vx, E, bc = MG.quad_mesh(4, 3, Lx=2.0, bc=["pec", "pmc", "sma", "pec"], stretch=0.3, shear=0.3, seed=order)
+    bc = bc.copy()
+    bc[5, 1], bc[6, 3] = 4, 4   # interior PEC edge between elements 5 and 6
+    bc[1, 2], bc[5, 0] = 5, 5   # interior PMC edge between elements 1 and 5
+    bc[6, 2], bc[10, 0] = 6, 6  # interior SMA edge between elements 6 and 10
+    bc[2, 1], bc[3, 3] = 7, 7   # TF/SF edge between elements 2 and 3
+    rng = np.random.default_rng(order)
+    eps, mu = rng.uniform(1, 4, E.shape[0]), rng.uniform(1, 2, E.shape[0])
+    for alpha in (1.0, 0.5, 0.0):
+        A = D.global_assemble(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, quads=True)
+        x = rng.standard_normal(A[0].size - 1)
+        yr = O2.Operator2D(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, global_sma=True, quads=True).mult(x)
+        assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+@pytest.mark.parametrize("dim", [3, 2, 4])
 def test_conductivity_energy_balance(dim):
+    """dim 4: quadrilaterals."""
     rng = np.random.default_rng(dim)
     if dim == 3:
         vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
         order = 3
-    else:
+    elif dim == 2:
         vx, E, fbc = MG.square_mesh(5, bc="pec", jitter=0.2, seed=1)
         order = 4
+    else:
+        vx, E, fbc = MG.quad_mesh(3, 4, bc="pec", stretch=0.3, shear=-0.2, seed=2)
+        order = 3
     K = E.shape[0]
     eps, mu, sigma = rng.uniform(1, 4, K), rng.uniform(1, 2, K), rng.uniform(0.5, 3.0, K)
 
     def oracle(eps_, mu_):
         return (O.Operator(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_) if dim == 3
-                else O2.Operator2D(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_))
+                else O2.Operator2D(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_, quads=dim == 4))
 
     W = oracle(eps, mu)
     Ws = oracle(sigma, np.ones(K))  # energy with eps -> sigma: (1/2) sum J sigma E^T M E + ...

@@ -302,3 +302,43 @@ def test_2d_tfsf_interior_pec_like_reference(tmp_path):
                    (1, 2.1, {EY: 1.0, HZ: 1.0}), (3, 2.1, {EY: 0.0, HZ: 0.0}),
                    (2, 2.9, {EY: 0.0, HZ: 2.0}), (3, 2.9, {EY: 0.0, HZ: 0.0}),
                    (1, 3.7, {EY: -1.0, HZ: 1.0}), (3, 3.7, {EY: 0.0, HZ: 0.0})], tol)
+
+
+QUADS2D = os.path.join(ROOT, "tests", "golden", "cases", "2D_TFSF_Quads.json")
+
+
+@pytest.mark.parametrize("alpha,dt", [(0.0, 5e-3), (1.0, None)])
+def test_2d_tfsf_quads_like_reference(tmp_path, alpha, dt):
+    """ExtensiveCasesTest 2D_TFSF_Centered_Quads (alpha 0, time step 5e-3) and
+    2D_TFSF_Upwind_Quads (the case as written: alpha 1, time step 0.01)
+    (test/cases/ExtensiveCasesTest.cpp:1003-1157): quadrilaterals on the
+    Global (assembled) operator; the Gaussian plane wave enters through the
+    TF/SF line x = 1 and bounces in the PEC/PMC guide; tolerance 1e-2."""
+    opts = dict(upwind_alpha=alpha)
+    if dt is not None:
+        opts["time_step"] = dt
+    path = _case_variant(tmp_path, QUADS2D, **opts)
+    c = D.Case(path)
+    assert c.quads and c.info["evolution_operator"] == "global"
+    x0, x, movies = _probe_movies(path)
+    assert np.linalg.norm(x0) == 0.0
+    assert movies[-1][0] + 1e-2 > 9.5
+    _check_movies(movies, (0, 1, 2),
+                  [(0, 9.8, {HZ: 0.0, EY: 2.0}), (1, 2.8, {HZ: 1.0, EY: 1.0}), (1, 8.8, {HZ: -1.0, EY: 1.0}),
+                   (2, 5.8, {HZ: 0.0, EY: 2.0})], 1e-2)
+
+
+def test_launcher_runs_quad_case(tmp_path):
+    """The C++ launcher takes the quadrilateral case to the assembled
+    operator and matches the library run."""
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, QUADS2D, "--final-time", "0.5", "--out", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["steps"] == 50 and summary["K"] == 60 and summary["nonfinite"] == 0
+    x, info = D.run_case(QUADS2D, precision=64, integrator="rk4", final_time=0.5)
+    assert info["steps"] == 50
+    assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-10)
+    rows = (out / "PointProbes" / "PointProbe0.dat").read_text().splitlines()
+    assert rows[0] == "PointProbe ID 0"
