@@ -277,9 +277,11 @@ int dgtd_op_kernel_variant(const dgtd_op* op, int32_t* variant);
  * Candidates: 0 the per-node stage; 1 + 2c + o the matrix-core stage in
  * configuration c (0: up to 12 FP32 / 8 FP64 waves per block, exterior
  * traces gathered 2 k-steps ahead, residual read at the update; 1: up to 8 /
- * 4 waves with twice the registers per wave, gathered 4 k-steps ahead, the
- * first residual pass read during the volume products) with its 8-element
- * tasks in Morton order (o = 0) or element order (o = 1). */
+ * 4 waves with twice the registers per wave, gathered 16 FP32 / 4 FP64
+ * k-steps ahead, the first residual pass read during the flux loop; FP32
+ * stages with the per-lane tail of output nodes, orders 3 and 4, run at most
+ * 8 waves in either configuration) with its 8-element tasks in Morton order
+ * (o = 0) or element order (o = 1). */
 typedef struct {
     int32_t variant;          /* DGTD_KERNEL_VALU, _MFMA or _CSR */
     int32_t config;           /* matrix-core configuration */
@@ -297,9 +299,11 @@ int dgtd_op_tuning(const dgtd_op* op, dgtd_tuning* out);
  * the reference): the candidate numbered as above (0 .. ncandidates-1).
  * DGTD_E_ARG when it does not fit this operator (no matrix-core variant at
  * this order and precision, element order without a Morton permutation) or
- * the operator is assembled (DGTD_KERNEL_CSR).  Every candidate computes the
- * same per-node arithmetic in the same order, so the result does not depend
- * on the choice; callers pin one for reproducible timing. */
+ * the operator is assembled (DGTD_KERNEL_CSR).  The matrix-core candidates
+ * compute the same arithmetic in the same order, so they store identical
+ * results; the per-node candidate sums in another order and agrees to
+ * rounding (FP64 to 1e-12 of the oracle).  Callers pin one for reproducible
+ * timing or bits. */
 int dgtd_op_set_candidate(dgtd_op* op, int32_t candidate);
 
 /* Algorithmic HBM bytes and flops per element per stage for this operator. */
