@@ -33,8 +33,13 @@ struct Smem {
     static constexpr size_t bytes = sizeof(T) * (size_t)nReal + sizeof(int) * (size_t)nInt;
 };
 
+// Threads per block of the per-node kernels (single-mesh and batch).
+constexpr int kThreads = 256;
+
 // Loads the order-P reference matrices and tables into LDS (block-wide;
-// the caller synchronises before use).
+// the caller synchronises before use).  Each thread keeps kChunk loads in
+// flight before their LDS writes, so the latencies overlap instead of
+// adding up load by load (up to 55 values per thread at order 5).
 template <int P, typename T>
 __device__ __forceinline__ void stage_load_ref(const T* __restrict__ mats, const int* __restrict__ tabs,
                                                unsigned char* smem_raw)
@@ -42,10 +47,37 @@ __device__ __forceinline__ void stage_load_ref(const T* __restrict__ mats, const
     using S = Shape<P>;
     using L = Smem<P, T>;
     constexpr int NFP = S::NFP, EB = S::EB;
+    constexpr int NM = L::nD + L::nL, NTAB = 4 * NFP + 96 * NFP;
+    constexpr int kChunk = 8;
     T* sD = reinterpret_cast<T*>(smem_raw);
     int* sFm = reinterpret_cast<int*>(sD + L::nReal) + EB * kConnStrideDev;
-    for (int i = threadIdx.x; i < L::nD + L::nL; i += blockDim.x) sD[i] = mats[i];
-    for (int i = threadIdx.x; i < 4 * NFP + 96 * NFP; i += blockDim.x) sFm[i] = tabs[i];
+    const int tid = threadIdx.x;
+    for (int base = 0; base < NM; base += kChunk * kThreads) {
+        T v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = base + k * kThreads + tid;
+            v[k] = mats[i < NM ? i : NM - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = base + k * kThreads + tid;
+            if (i < NM) sD[i] = v[k];
+        }
+    }
+    for (int base = 0; base < NTAB; base += kChunk * kThreads) {
+        int v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = base + k * kThreads + tid;
+            v[k] = tabs[i < NTAB ? i : NTAB - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = base + k * kThreads + tid;
+            if (i < NTAB) sFm[i] = v[k];
+        }
+    }
 }
 
 // One element block (EB elements from e0 = blk*EB) of one stage: gather,
@@ -69,55 +101,98 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
     const int tid = threadIdx.x;
     const int K = a.K;
     const size_t N = (size_t)NP * K;
+    static_assert(EB * NP <= kThreads, "one volume node per thread");
     {
         const int e0 = blk * EB;
         const int ne = min(EB, K - e0);
         __syncthreads();  // previous iteration finished with sQ/sF/sG/sC (and tables loaded)
         {
+            // every load of the block's fields and records in flight, then the
+            // LDS writes (out-of-range threads load a clamped, valid address)
+            constexpr int NG = (EB * kGeoStrideDev + kThreads - 1) / kThreads;
+            constexpr int NC = (EB * kConnStrideDev + kThreads - 1) / kThreads;
             const size_t base = (size_t)e0 * NP;
-            const int cnt = ne * NP;
+            const int cnt = ne * NP, ng = ne * kGeoStrideDev, nc = ne * kConnStrideDev;
+            const int iq = tid < cnt ? tid : cnt - 1;
+            T q[6], gv[NG];
+            int cv[NC];
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                for (int i = tid; i < cnt; i += blockDim.x) sQ[c * EB * NP + i] = a.qin[c * N + base + i];
-            for (int i = tid; i < ne * kGeoStrideDev; i += blockDim.x)
-                sG[i] = a.geo[(size_t)e0 * kGeoStrideDev + i];
-            for (int i = tid; i < ne * kConnStrideDev; i += blockDim.x)
-                sC[i] = a.conn[(size_t)e0 * kConnStrideDev + i];
+            for (int c = 0; c < 6; ++c) q[c] = a.qin[c * N + base + iq];
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const int i = tid + k * kThreads;
+                gv[k] = a.geo[(size_t)e0 * kGeoStrideDev + (i < ng ? i : ng - 1)];
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const int i = tid + k * kThreads;
+                cv[k] = a.conn[(size_t)e0 * kConnStrideDev + (i < nc ? i : nc - 1)];
+            }
+            if (tid < cnt) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sQ[c * EB * NP + tid] = q[c];
+            }
+#pragma unroll
+            for (int k = 0; k < NG; ++k)
+                if (tid + k * kThreads < ng) sG[tid + k * kThreads] = gv[k];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                if (tid + k * kThreads < nc) sC[tid + k * kThreads] = cv[k];
         }
         __syncthreads();
 
         // ---- surface: jumps, boundary conditions, flux, fscale/2 ----
-        for (int s = tid; s < ne * NF; s += blockDim.x) {
+        // A thread's face nodes (at most NS): first every exterior trace is
+        // gathered, then the fluxes are formed.
+        constexpr int NS = (EB * NF + kThreads - 1) / kThreads;
+        const int nsf = ne * NF;
+        T uM[NS][6], d[NS][6];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int s = tid + k * kThreads < nsf ? tid + k * kThreads : nsf - 1;
             const int el = s / NF;
             const int m = s - el * NF;
             const int f = m / NFP;
             const int b = m - f * NFP;
-            const T* g = sG + el * kGeoStrideDev;
             const int nb = sC[el * kConnStrideDev + f];
             const int iM = sFm[f * NFP + b];
-            T uM[6], d[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) uM[c] = sQ[c * EB * NP + el * NP + iM];
+            for (int c = 0; c < 6; ++c) uM[k][c] = sQ[c * EB * NP + el * NP + iM];
+            // a boundary face reads the element's own node (its value is
+            // replaced below)
+            const int code = sC[el * kConnStrideDev + 4 + f];
+            const size_t gP = nb >= 0 ? (size_t)nb * NP + sPt[code * NFP + b] : (size_t)(e0 + el) * NP + iM;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) d[k][c] = a.qin[c * N + gP];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int s = tid + k * kThreads;
+            if (s >= nsf) break;
+            const int el = s / NF;
+            const int m = s - el * NF;
+            const int f = m / NFP;
+            const T* g = sG + el * kGeoStrideDev;
+            const int nb = sC[el * kConnStrideDev + f];
+            T* dk = d[k];
             if (nb >= 0) {
-                const int code = sC[el * kConnStrideDev + 4 + f];
-                const size_t gP = (size_t)nb * NP + sPt[code * NFP + b];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) d[c] = a.qin[c * N + gP] - uM[c];
+                for (int c = 0; c < 6; ++c) dk[c] -= uM[k][c];
             } else {
                 T ce, ch;
                 bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) { d[c] = ce * uM[c]; d[3 + c] = ch * uM[3 + c]; }
+                for (int c = 0; c < 3; ++c) { dk[c] = ce * uM[k][c]; dk[3 + c] = ch * uM[k][3 + c]; }
             }
             const T nx = g[9 + 4 * f], ny = g[10 + 4 * f], nz = g[11 + 4 * f], fs = g[12 + 4 * f];
             const T n3[3] = {nx, ny, nz};
-            const T ndH = nx * d[3] + ny * d[4] + nz * d[5];
-            const T ndE = nx * d[0] + ny * d[1] + nz * d[2];
+            const T ndH = nx * dk[3] + ny * dk[4] + nz * dk[5];
+            const T ndE = nx * dk[0] + ny * dk[1] + nz * dk[2];
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
                 const int y = (x + 1) % 3, z = (x + 2) % 3;
-                const T fH = -n3[y] * d[z] + n3[z] * d[y] + a.alpha * (d[3 + x] - ndH * n3[x]);
-                const T fE = n3[y] * d[3 + z] - n3[z] * d[3 + y] + a.alpha * (d[x] - ndE * n3[x]);
+                const T fH = -n3[y] * dk[z] + n3[z] * dk[y] + a.alpha * (dk[3 + x] - ndH * n3[x]);
+                const T fE = n3[y] * dk[3 + z] - n3[z] * dk[3 + y] + a.alpha * (dk[x] - ndE * n3[x]);
                 sF[(3 + x) * EB * NF + el * NF + m] = fH * fs;
                 sF[x * EB * NF + el * NF + m] = fE * fs;
             }
@@ -126,9 +201,20 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
 
         // ---- volume: curl + LIFT, stage update ----
         bool bad = false;  // an unstable state entry stored by this thread (a.unstable set)
-        for (int t = tid; t < ne * NP; t += blockDim.x) {
+        if (tid < ne * NP) {
+            const int t = tid;
             const int el = t / NP;
             const int i = t - el * NP;
+            const size_t idx = (size_t)(e0 + el) * NP + i;
+            // the stage's reads of the residual / base state, issued before
+            // the products (all reads before the first store, see the MFMA variant)
+            T rv[6], xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
+                    rv[c] = __builtin_nontemporal_load(a.res + c * N + idx);
+                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
+            }
             T ar[6], as[6], at[6], lf[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) { ar[c] = T(0); as[c] = T(0); at[c] = T(0); lf[c] = T(0); }
@@ -158,15 +244,6 @@ __device__ __forceinline__ void stage_block(const StageArgs<T>& a, unsigned char
                 const int y = (x + 1) % 3, z = (x + 2) % 3;
                 rhs[3 + x] = (-Dd(y, z) + Dd(z, y) + lf[3 + x]) * g[26];
                 rhs[x] = (Dd(y, 3 + z) - Dd(z, 3 + y) + lf[x]) * g[25];
-            }
-            const size_t idx = (size_t)(e0 + el) * NP + i;
-            // all reads before the first store (see the MFMA variant)
-            T rv[6], xv[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
-                    rv[c] = __builtin_nontemporal_load(a.res + c * N + idx);
-                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
             }
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
