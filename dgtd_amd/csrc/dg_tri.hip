@@ -12,6 +12,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "lds_copy.hpp"
 #include "stability.hpp"
 
 namespace dgtd {
@@ -57,9 +58,11 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
     int* sC = reinterpret_cast<int*>(sG + L::nG);  // [el][kTriConn]
     int* sFm = sC + EB * kTriConn;                 // [3*NFP]
     int* sPt = sFm + 3 * NFP;                      // [18*NFP]
+    constexpr int kThreads = 256;  // the launch's block size
+    static_assert(EB * NP <= kThreads, "one volume node per thread");
     const int tid = threadIdx.x;
-    for (int i = tid; i < L::nD + L::nL; i += blockDim.x) sD[i] = a.mats[i];
-    for (int i = tid; i < 21 * NFP; i += blockDim.x) sFm[i] = a.tabs[i];
+    lds_copy<8>(sD, a.mats, L::nD + L::nL, tid, kThreads);
+    lds_copy<8>(sFm, a.tabs, 21 * NFP, tid, kThreads);
     const int K = a.K;
     const size_t N = (size_t)NP * K;
 
@@ -68,33 +71,77 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
         const int ne = min(EB, K - e0);
         __syncthreads();  // tables loaded; previous block done with the tiles
         {
+            // every load of the block's fields and records in flight, then the
+            // LDS writes (out-of-range threads load a clamped, valid address)
+            constexpr int NG = (EB * kTriGeo + kThreads - 1) / kThreads;
+            constexpr int NC = (EB * kTriConn + kThreads - 1) / kThreads;
             const size_t base = (size_t)e0 * NP;
-            const int cnt = ne * NP;
+            const int cnt = ne * NP, ng = ne * kTriGeo, nc = ne * kTriConn;
+            const int iq = tid < cnt ? tid : cnt - 1;
+            T q[6], gv[NG];
+            int cv[NC];
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                for (int i = tid; i < cnt; i += blockDim.x) sQ[c * EB * NP + i] = a.qin[c * N + base + i];
-            for (int i = tid; i < ne * kTriGeo; i += blockDim.x) sG[i] = a.geo[(size_t)e0 * kTriGeo + i];
-            for (int i = tid; i < ne * kTriConn; i += blockDim.x) sC[i] = a.conn[(size_t)e0 * kTriConn + i];
+            for (int c = 0; c < 6; ++c) q[c] = a.qin[c * N + base + iq];
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                const int i = tid + k * kThreads;
+                gv[k] = a.geo[(size_t)e0 * kTriGeo + (i < ng ? i : ng - 1)];
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const int i = tid + k * kThreads;
+                cv[k] = a.conn[(size_t)e0 * kTriConn + (i < nc ? i : nc - 1)];
+            }
+            if (tid < cnt) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sQ[c * EB * NP + tid] = q[c];
+            }
+#pragma unroll
+            for (int k = 0; k < NG; ++k)
+                if (tid + k * kThreads < ng) sG[tid + k * kThreads] = gv[k];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                if (tid + k * kThreads < nc) sC[tid + k * kThreads] = cv[k];
         }
         __syncthreads();
 
         // ---- edges: jumps (exterior - interior), boundary conditions, flux ----
-        for (int s = tid; s < ne * NF; s += blockDim.x) {
+        // A thread's edge nodes (at most NS): every exterior trace gathered
+        // first, then the fluxes formed.
+        constexpr int NS = (EB * NF + kThreads - 1) / kThreads;
+        const int nsf = ne * NF;
+        T uMs[NS][6], ds[NS][6];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int s = tid + k * kThreads < nsf ? tid + k * kThreads : nsf - 1;
             const int el = s / NF;
             const int m = s - el * NF;
             const int f = m / NFP;
             const int b = m - f * NFP;
-            const T* g = sG + el * kTriGeo;
             const int nb = sC[el * kTriConn + f];
             const int iM = sFm[f * NFP + b];
-            T uM[6], d[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) uM[c] = sQ[c * EB * NP + el * NP + iM];
+            for (int c = 0; c < 6; ++c) uMs[k][c] = sQ[c * EB * NP + el * NP + iM];
+            // a boundary edge reads the element's own node (replaced below)
+            const int code = sC[el * kTriConn + 3 + f];
+            const size_t gP = nb >= 0 ? (size_t)nb * NP + sPt[code * NFP + b] : (size_t)(e0 + el) * NP + iM;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ds[k][c] = a.qin[c * N + gP];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int s = tid + k * kThreads;
+            if (s >= nsf) break;
+            const int el = s / NF;
+            const int m = s - el * NF;
+            const int f = m / NFP;
+            const T* g = sG + el * kTriGeo;
+            const int nb = sC[el * kTriConn + f];
+            const T* uM = uMs[k];
+            T* d = ds[k];
             if (nb >= 0) {
-                const int code = sC[el * kTriConn + 3 + f];
-                const size_t gP = (size_t)nb * NP + sPt[code * NFP + b];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) d[c] = a.qin[c * N + gP] - uM[c];
+                for (int c = 0; c < 6; ++c) d[c] -= uM[c];
             } else {
                 T ce, ch;
                 bc_coeffs<T>(-nb - 1, a.alpha, ce, ch);
@@ -126,9 +173,19 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
 
         // ---- volume: curl with d/dz = 0, LIFT, stage update ----
         bool bad = false;  // an unstable state entry stored by this thread (a.unstable set)
-        for (int t = tid; t < ne * NP; t += blockDim.x) {
+        if (tid < ne * NP) {
+            const int t = tid;
             const int el = t / NP;
             const int i = t - el * NP;
+            const size_t idx = (size_t)(e0 + el) * NP + i;
+            // the stage's reads of the residual / base state, before the products
+            T rv[6], xv[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
+                    rv[c] = a.res[c * N + idx];
+                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
+            }
             T ar[6], as[6], lf[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) { ar[c] = T(0); as[c] = T(0); lf[c] = T(0); }
@@ -157,14 +214,6 @@ __global__ __launch_bounds__(256) void dg_tri_stage(StageArgs<T> a)
             rhs[3] = (-Dy(2) + lf[3]) * g[14];         // H_x: -dEz/dy
             rhs[4] = (Dx(2) + lf[4]) * g[14];          // H_y:  dEz/dx
             rhs[5] = (-Dx(1) + Dy(0) + lf[5]) * g[14]; // H_z: -(dEy/dx - dEx/dy)
-            const size_t idx = (size_t)(e0 + el) * NP + i;
-            T rv[6], xv[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                if constexpr (MODE == kModeLserk || MODE == kModeRk2 || MODE == kModeRk3 || MODE == kModeRk4)
-                    rv[c] = a.res[c * N + idx];
-                if constexpr (MODE == kModeRk2 || MODE == kModeRk3) xv[c] = a.x[c * N + idx];
-            }
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
                 const size_t o = c * N + idx;

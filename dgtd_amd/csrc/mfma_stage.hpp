@@ -34,6 +34,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "lds_copy.hpp"
 #include "stability.hpp"
 
 namespace dgtd {
@@ -269,11 +270,11 @@ __device__ MfmaLds<P, T, DIM> mfma_load_ref(const T* mats, const int* tabs, unsi
     if constexpr (kBGlobal) {
         L.sBv = mats;
     } else {
-        for (int i = tid; i < KP * NDIR * NPG + NFACE * NFPP * NPG; i += nthreads) sB[i] = mats[i];
+        lds_copy<8>(sB, mats, KP * NDIR * NPG + NFACE * NFPP * NPG, tid, nthreads);
         L.sBv = sB;
     }
     L.sBl = L.sBv + KP * NDIR * NPG;
-    for (int i = tid; i < NFACE * NFP + S::NPERM * NFP; i += nthreads) L.sFm[i] = tabs[i];
+    lds_copy<8>(L.sFm, tabs, NFACE * NFP + S::NPERM * NFP, tid, nthreads);
     for (int i = tid; i < WAVES * 6 * TE * US; i += nthreads) L.sU[i] = T(0);
     __syncthreads();
     return L;

@@ -7,6 +7,7 @@
 
 #include "bc_coeffs.hpp"
 #include "kernels.hpp"
+#include "lds_copy.hpp"
 #include "stability.hpp"
 
 namespace dgtd {
@@ -37,9 +38,8 @@ struct Smem {
 constexpr int kThreads = 256;
 
 // Loads the order-P reference matrices and tables into LDS (block-wide;
-// the caller synchronises before use).  Each thread keeps kChunk loads in
-// flight before their LDS writes, so the latencies overlap instead of
-// adding up load by load (up to 55 values per thread at order 5).
+// the caller synchronises before use), several loads in flight per thread
+// (up to 55 values per thread at order 5).
 template <int P, typename T>
 __device__ __forceinline__ void stage_load_ref(const T* __restrict__ mats, const int* __restrict__ tabs,
                                                unsigned char* smem_raw)
@@ -47,37 +47,10 @@ __device__ __forceinline__ void stage_load_ref(const T* __restrict__ mats, const
     using S = Shape<P>;
     using L = Smem<P, T>;
     constexpr int NFP = S::NFP, EB = S::EB;
-    constexpr int NM = L::nD + L::nL, NTAB = 4 * NFP + 96 * NFP;
-    constexpr int kChunk = 8;
     T* sD = reinterpret_cast<T*>(smem_raw);
     int* sFm = reinterpret_cast<int*>(sD + L::nReal) + EB * kConnStrideDev;
-    const int tid = threadIdx.x;
-    for (int base = 0; base < NM; base += kChunk * kThreads) {
-        T v[kChunk];
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const int i = base + k * kThreads + tid;
-            v[k] = mats[i < NM ? i : NM - 1];
-        }
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const int i = base + k * kThreads + tid;
-            if (i < NM) sD[i] = v[k];
-        }
-    }
-    for (int base = 0; base < NTAB; base += kChunk * kThreads) {
-        int v[kChunk];
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const int i = base + k * kThreads + tid;
-            v[k] = tabs[i < NTAB ? i : NTAB - 1];
-        }
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const int i = base + k * kThreads + tid;
-            if (i < NTAB) sFm[i] = v[k];
-        }
-    }
+    lds_copy<8>(sD, mats, L::nD + L::nL, (int)threadIdx.x, kThreads);
+    lds_copy<8>(sFm, tabs, 4 * NFP + 96 * NFP, (int)threadIdx.x, kThreads);
 }
 
 // One element block (EB elements from e0 = blk*EB) of one stage: gather,
