@@ -14,8 +14,8 @@ STATES = {_abi.DGTD_INST_FREE: "free", _abi.DGTD_INST_RUNNING: "running", _abi.D
           _abi.DGTD_INST_FAILED: "failed", _abi.DGTD_INST_CANCELLED: "cancelled"}
 
 
-def footprint(vx, EToV, face_bc, order, precision=32):
-    m = _Mesh(vx, EToV, face_bc, order)
+def footprint(vx, EToV, face_bc, order, precision=32, quads=None):
+    m = _Mesh(vx, EToV, face_bc, order, quads=quads)
     b = ctypes.c_int64()
     check(lib().dgtd_batch_footprint(ctypes.byref(m.desc), int(precision), ctypes.byref(b)))
     return b.value

@@ -534,12 +534,18 @@ int batch_error(const std::exception& e)
 
 }  // namespace
 
+// Quadrilaterals run on the assembled (Global) operator only; the batch
+// advances hesthaven-operator instances.
+constexpr const char* kBatchNoQuads =
+    "quadrilaterals run on the assembled operator only (dgtd_op_create with DGTD_KERNEL_CSR), not in a batch";
+
 extern "C" {
 
 int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t* bytes)
 {
     if (!mesh || !bytes) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
+    if (mesh->nvert == 4 && mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoQuads);
     const int dim = mesh->dim == 2 ? 2 : 3;
     *bytes = footprint(dim, mesh->K, mesh->order, precision, tfsf_elements(mesh, dim));
     return DGTD_OK;
@@ -607,6 +613,9 @@ int dgtd_batch_add_tfsf(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double*
         if (!dgtd::planewaves_device(waves, nwaves, dwaves, err)) return dgtd::api_fail(DGTD_E_ARG, err);
     }
     const int dim = mesh->dim == 2 ? 2 : 3;
+    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)))
+        return dgtd::api_fail(DGTD_E_ARG, "nvert must be 0, or 3 or 4 with dim = 2");
+    if (mesh->nvert == 4) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoQuads);
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return dgtd::api_fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     if (mesh->sigma)

@@ -470,7 +470,8 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
  * Tetrahedral (dim 0 or 3) and triangle (dim 2) meshes may share a batch:
  * triangles of every order and tetrahedra of orders 2-5 run on the
  * matrix-core stage, order-1 tetrahedra on the per-node stage.  Conductivity
- * and state buffers of 4 GiB or more are refused (DGTD_E_UNSUPPORTED); TF/SF
+ * and state buffers of 4 GiB or more are refused (DGTD_E_UNSUPPORTED), and so
+ * are quadrilateral meshes (nvert = 4: the assembled operator only); TF/SF
  * faces (code 7) inject nothing without plane waves (dgtd_batch_add_tfsf). */
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
                    double final_time, int32_t* id);

@@ -2,7 +2,9 @@
 needs no GPU: admission by footprint, refusal of a request that can never
 fit, and retirement of failed solves."""
 import numpy as np
+import pytest
 
+import dgtd_amd as D
 from dgtd_amd import serve as S
 from dgtd_amd.batch import CapacityError, footprint
 
@@ -105,3 +107,13 @@ def test_footprint_counts_the_element_shape():
         s = prec // 8
         Np = (order + 1) * (order + 2) // 2
         assert footprint(vx, E, fbc, order, prec) == 3 * 6 * Np * K * s + K * 15 * s + K * 6 * 4
+
+
+def test_batch_refuses_quadrilaterals():
+    """The batch advances hesthaven-operator instances; quadrilaterals run
+    on the assembled operator only, so the admission measure refuses them
+    (the same check guards dgtd_batch_add)."""
+    vx, E, fbc = D.cartesian_2d(3, 2, "quad")
+    with pytest.raises(D.DgtdError, match="assembled operator only") as ei:
+        D.footprint(vx, E, fbc, 2)
+    assert ei.value.code == -4
