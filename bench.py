@@ -274,24 +274,37 @@ def bench_c3(D, args):
     return out
 
 
-def bench_global(D, args):
+def bench_global(D, args, quads=False):
     """The assembled (Global) operator with conductivity: PEC cavity [0,1]^3,
     12^3 hexes x 6 = 10,368 tets, p=3, eps_r ~ U[1,4], a conducting slab
     (sigma = 2 for x < 1/2), Ez Gaussian pulse, 100 LSERK4 steps in FP32 and
-    FP64 through the fused CSR SpMV stage kernel.  CPU leg: the same FP64 CSR
+    FP64 through the fused CSR SpMV stage kernel.  With quads: the same on a
+    PEC/PMC square [0,1]^2 of 128^2 quadrilaterals (the reference runs its
+    quadrilateral meshes on this operator), p=3.  CPU leg: the same FP64 CSR
     product on one host core (scipy), the Global operator's
     SparseMatrix::Mult."""
-    n, order = 12, 3
+    order = 3
     steps = args.steps
-    vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
-    K = E.shape[0]
     rng = np.random.default_rng(args.seed)
+    if quads:
+        n = 128
+        vx, E, fbc = D.cartesian_2d(n, n, "quad", bc=("pmc", "pec", "pmc", "pec"))
+        np_ = (order + 1) ** 2
+        src = [D.gaussian("E", (0.6, 0.45, 0.0), 0.1, (0, 0, 1), dimension=2)]
+        mesh = f"PEC/PMC square [0,1]^2, {n}^2 quadrilaterals"
+    else:
+        n = 12
+        vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
+        np_ = D.num_nodes(order)
+        src = [D.gaussian("E", (0.6, 0.45, 0.55), 0.1, (0, 0, 1))]
+        mesh = f"PEC cavity [0,1]^3, {n}^3 hexes x 6 = {E.shape[0]} tets"
+    K = E.shape[0]
     eps = rng.uniform(1.0, 4.0, K)
     cx = vx.reshape(-1, 3)[E].mean(axis=1)[:, 0]
     sigma = np.where(cx < 0.5, 2.0, 0.0)
-    x0 = D.project_initial_fields(vx, E, order, [D.gaussian("E", (0.6, 0.45, 0.55), 0.1, (0, 0, 1))])
-    dof = 6 * D.num_nodes(order) * K
-    out = {"workload": f"PEC cavity [0,1]^3, {n}^3 hexes x 6 = {K} tets, p={order}, eps_r~U[1,4], "
+    x0 = D.project_initial_fields(vx, E, order, src)
+    dof = 6 * np_ * K
+    out = {"workload": f"{mesh}, p={order}, eps_r~U[1,4], "
                        f"sigma=2 for x<1/2, Ez Gaussian, {steps} LSERK4 steps, assembled CSR operator", "dof": dof}
     states = {}
     for prec in (32, 64):
@@ -318,7 +331,7 @@ def bench_global(D, args):
     if not args.no_cpu:
         import scipy.sparse as sp
 
-        rowptr, col, val = D.global_assemble(vx, E, fbc, order, eps=eps, sigma=sigma)
+        rowptr, col, val = D.global_assemble(vx, E, fbc, order, eps=eps, sigma=sigma, quads=quads or None)
         A = sp.csr_matrix((val, col, rowptr), shape=(dof, dof))
         out["nnz"] = int(val.size)
         x = x0.copy()
@@ -459,10 +472,11 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma", "csr"])
     ap.add_argument("--headline-only", action="store_true",
                     help="c2: skip the c1, c3 and c4 sections of the line")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "tri", "global"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "tri", "global", "quad"],
                     help="c2 (default, the headline): 32^3 cavity p=4; c3: 48^3 SMA box with random eps, "
                          "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream; "
-                         "tri: the 2D operator on 524,288 triangles, p=4")
+                         "tri: the 2D operator on 524,288 triangles, p=4; global: the assembled CSR operator "
+                         "on 10,368 tets with conductivity; quad: the same on 128^2 quadrilaterals")
     ap.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
@@ -496,6 +510,8 @@ def main():
         return bench_tri(D, args)
     if args.workload == "global":
         return bench_global(D, args)
+    if args.workload == "quad":
+        return bench_global(D, args, quads=True)
     n, order = args.n, args.order
     vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
     K = E.shape[0]
