@@ -113,12 +113,18 @@ template <typename T, int MODE>
 hipError_t csr_launch(const StageArgs<T>& a, hipStream_t s)
 {
     // one chunk per block where the rows allow: the smallest chunk holding an
-    // average block's entries (16 rows), at most 4096 entries
+    // average block's entries (16 rows), at most 4096 entries; short rows (2D
+    // meshes: 38 entries per row on order-3 quadrilaterals) take small chunks,
+    // so fewer lanes repeat the range's last entry
     const double per_block = (double)a.nnz / (double)a.nrows * kRowsPerBlock;
     const int64_t N = a.nrows / 6;
     const int64_t blocks = 6 * ((N + kRowsPerBlock - 1) / kRowsPerBlock);
     const int64_t grid = kXcds * ((blocks + kXcds - 1) / kXcds);
-    if (per_block <= 256 * 8)
+    if (per_block <= 256 * 3)
+        hipLaunchKernelGGL((csr_stage<T, MODE, 3>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else if (per_block <= 256 * 5)
+        hipLaunchKernelGGL((csr_stage<T, MODE, 5>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    else if (per_block <= 256 * 8)
         hipLaunchKernelGGL((csr_stage<T, MODE, 8>), dim3((unsigned)grid), dim3(256), 0, s, a);
     else if (sizeof(T) == 4 && per_block <= 256 * 10)  // FP64 measured faster with the 3072-entry chunk
         hipLaunchKernelGGL((csr_stage<T, MODE, 10>), dim3((unsigned)grid), dim3(256), 0, s, a);
