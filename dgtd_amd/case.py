@@ -38,11 +38,14 @@ class Case:
             check(lib().dgtd_case_mesh(h, ctypes.byref(d)))
             K, nv = d.K, d.nv
             self.quads = d.dim == 2 and d.nvert == 4
-            nvert = 3 if d.dim == 2 and not self.quads else 4  # triangles, quadrilaterals or tetrahedra
+            self.hexes = d.dim != 2 and d.nvert == 8
+            # triangles, quadrilaterals, tetrahedra or hexahedra
+            nvert = 8 if self.hexes else 3 if d.dim == 2 and not self.quads else 4
+            nfaces = 6 if self.hexes else nvert
             self.dim = 2 if d.dim == 2 else 3
             self.vx = np.ctypeslib.as_array(d.vx, shape=(3 * nv,)).reshape(nv, 3).copy()
             self.EToV = np.ctypeslib.as_array(d.EToV, shape=(nvert * K,)).reshape(K, nvert).copy()
-            self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(nvert * K,)).reshape(K, nvert).copy()
+            self.face_bc = np.ctypeslib.as_array(d.face_bc, shape=(nfaces * K,)).reshape(K, nfaces).copy()
             self.eps = None if not d.eps_r else np.ctypeslib.as_array(d.eps_r, shape=(K,)).copy()
             self.mu = None if not d.mu_r else np.ctypeslib.as_array(d.mu_r, shape=(K,)).copy()
             self.sigma = None if not d.sigma else np.ctypeslib.as_array(d.sigma, shape=(K,)).copy()
@@ -96,6 +99,8 @@ class Case:
         p = self.order
         if self.quads:
             return (p + 1) ** 2
+        if self.hexes:
+            return (p + 1) ** 3
         return num_nodes(p) if self.dim == 3 else (p + 1) * (p + 2) // 2
 
     @property
@@ -114,11 +119,11 @@ class Case:
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
         """The case's operator; conductive materials, and SMA faces with
         alpha != 1 under the Global or Maxwell operator (which weight SMA faces
-        as at alpha = 1), and quadrilateral meshes need the assembled one
-        ("csr"), which kernel="auto" then selects."""
+        as at alpha = 1), and quadrilateral and hexahedral meshes need the
+        assembled one ("csr"), which kernel="auto" then selects."""
         global_sma = (self.info["evolution_operator"] != "hesthaven" and self.info["alpha"] != 1.0
                       and bool((self.face_bc == 3).any()))
-        if kernel == "auto" and (self.sigma is not None or global_sma or self.quads):
+        if kernel == "auto" and (self.sigma is not None or global_sma or self.quads or self.hexes):
             kernel = "csr"
         op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
                                 eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,

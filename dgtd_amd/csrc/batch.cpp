@@ -59,7 +59,7 @@ int64_t footprint(int dim, int K, int order, int precision, int64_t ntf = 0)
     const int64_t conn = dim == 2 ? dgtd::kTriConnStride : dgtd::kConnStride;
     const int64_t nfn = dim == 2 ? 3 * (order + 1) : 4 * (order + 1) * (order + 2) / 2;
     return 3 * 6 * Np * K * s + (int64_t)K * geo * s + (int64_t)K * conn * 4 +
-           ntf * (5 * 4 + nfn * 3 * 8 + geo * 8);
+           ntf * ((1 + dgtd::kTfsfSignStride) * 4 + nfn * 3 * 8 + geo * 8);
 }
 
 // Elements with a TF/SF face code.

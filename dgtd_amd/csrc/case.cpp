@@ -297,17 +297,21 @@ void read_model(const dgtd::Json& c, const std::string& case_path, dgtd_case& cs
                                     w.propagation[2] * w.propagation[2]);
         double min_phase = 0.0;
         bool any = false;
-        const bool two_d = cs.mesh.dim == 2;  // TF/SF boundary elements: segments in 2D, triangles in 3D
-        const size_t nb = two_d ? cs.mesh.segs.size() : cs.mesh.tris.size();
+        // TF/SF boundary elements: segments in 2D, triangles on tetrahedra,
+        // quadrilaterals on hexahedra
+        const bool two_d = cs.mesh.dim == 2, hexm = cs.mesh.hex();
+        const size_t nb = two_d ? cs.mesh.segs.size() : hexm ? cs.mesh.quads.size() : cs.mesh.tris.size();
         for (size_t t = 0; t < nb; ++t) {
-            const int a = two_d ? cs.mesh.seg_attr[t] : cs.mesh.tri_attr[t];
+            const int a = two_d ? cs.mesh.seg_attr[t] : hexm ? cs.mesh.quad_attr[t] : cs.mesh.tri_attr[t];
             if (std::find(cs.tfsf_tags.begin(), cs.tfsf_tags.end(), a) == cs.tfsf_tags.end()) continue;
-            const int nvt = two_d ? 2 : 3;
+            const int nvt = two_d ? 2 : hexm ? 4 : 3;
+            auto vert = [&](int k) {
+                return two_d ? cs.mesh.segs[t][k] : hexm ? cs.mesh.quads[t][k] : cs.mesh.tris[t][k];
+            };
             double ph = 0.0;
             for (int d = 0; d < 3; ++d) {
                 double c = 0.0;
-                for (int k = 0; k < nvt; ++k)
-                    c += cs.mesh.vx[3 * (size_t)(two_d ? cs.mesh.segs[t][k] : cs.mesh.tris[t][k]) + d];
+                for (int k = 0; k < nvt; ++k) c += cs.mesh.vx[3 * (size_t)vert(k) + d];
                 ph += c / nvt * w.propagation[d] / nd;
             }
             min_phase = any ? std::min(min_phase, ph) : ph;
@@ -346,6 +350,10 @@ extern "C" int dgtd_case_load(const char* json_path, dgtd_case** out)
             // the reference's hesthaven quadrilateral cases are disabled
             // (ExtensiveCasesTest.cpp:1159 DISABLED_2D_TFSF_Centered_Quads_Hesthaven)
             throw Unsupported("quadrilateral meshes run on the Global operator only, not on hesthaven");
+        if (cs->mesh.hex() && cs->info.evolution_operator == DGTD_OPERATOR_HESTHAVEN)
+            // the hesthaven operator builds a tetrahedral reference element in 3D
+            // (HesthavenEvolution.cpp:39-40)
+            throw Unsupported("hexahedral meshes run on the Global operator only, not on hesthaven");
         if (cs->info.alpha == 0.0 && cs->info.evolution_operator == DGTD_OPERATOR_HESTHAVEN)
             for (int8_t b : cs->face_bc)
                 if (b == DGTD_BC_SMA)  // driver.cpp:1596-1598 (the Global operator runs it)
@@ -384,7 +392,7 @@ extern "C" int dgtd_case_mesh(const dgtd_case* c, dgtd_mesh_desc* mesh)
     mesh->mu_r = c->mu.empty() ? nullptr : c->mu.data();
     mesh->dim = c->mesh.dim;
     mesh->sigma = c->sigma.empty() ? nullptr : c->sigma.data();
-    mesh->nvert = c->mesh.quad() ? 4 : 0;
+    mesh->nvert = c->mesh.quad() ? 4 : c->mesh.hex() ? 8 : 0;
     return DGTD_OK;
 }
 

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            const int col = (int)(c[u] >> 6 & 7) * n32 + tab[c[u] >> 9] * a.np + (int)(c[u] & 63);
+            const int col = (int)(c[u] >> kCsrCompShift & 7) * n32 + tab[c[u] >> kCsrSlotShift] * a.np +
+                            (int)(c[u] & ((1u << kCsrCompShift) - 1));
             const T pr = v[u] * a.qin[col];
             prod[u * 256 + t] = c0 + u * 256 + t < b1 ? pr : T(0);
         }

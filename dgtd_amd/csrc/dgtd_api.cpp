@@ -127,6 +127,8 @@ struct dgtd_op {
     dgtd::RefTri T2;
     bool quad = false;
     dgtd::RefQuad Q2;
+    bool hex = false;  // hexahedra (dim 3, RefHex H3; the assembled operator only)
+    dgtd::RefHex H3;
     int K = 0, Np = 0, Nfp = 0;
     size_t N = 0;
     int64_t ndofs = 0;
@@ -374,7 +376,7 @@ struct dgtd_op {
     {
         if (ntfsf == 0 || nwaves == 0) return;
         dgtd::TfsfArgs<T> a{};
-        a.dim = quad ? 4 : dim;
+        a.dim = hex ? 5 : quad ? 4 : dim;
         a.nelem = ntfsf;
         a.elem = d_tfsf_elem;
         a.sign = d_tfsf_sign;
@@ -800,17 +802,17 @@ void create_quad(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
 // face areas in 3D; getJacobiGQ_RMin, :254-263: the first Gauss-Lobatto
 // spacing of order p on [-1, 1]; speed of light 1), halved when the mesh holds
 // quadrilaterals, divided by 0.8 on tetrahedra under the Global operator.
-int estimate_time_step(int dim, bool quad, int order, int K, const double* vx, const int32_t* EToV, double max_fscale,
-                       int evolution_operator, double cfl, double* dt)
+int estimate_time_step(int dim, bool quad, bool hex, int order, int K, const double* vx, const int32_t* EToV,
+                       double max_fscale, int evolution_operator, double cfl, double* dt)
 {
     if (evolution_operator < DGTD_OPERATOR_MAXWELL || evolution_operator > DGTD_OPERATOR_HESTHAVEN)
         return fail(DGTD_E_ARG, "evolution_operator must be DGTD_OPERATOR_MAXWELL, _GLOBAL or _HESTHAVEN");
     if (!(cfl > 0.0)) return fail(DGTD_E_ARG, "cfl must be > 0");
-    if (dim == 3 && evolution_operator == DGTD_OPERATOR_HESTHAVEN) {
+    if (dim == 3 && !hex && evolution_operator == DGTD_OPERATOR_HESTHAVEN) {
         *dt = cfl / (max_fscale * order * order);
         return DGTD_OK;
     }
-    const int nvert = dim == 3 || quad ? 4 : 3;
+    const int nvert = hex ? 8 : dim == 3 || quad ? 4 : 3;
     auto P = [&](int e, int j) { return vx + 3 * (size_t)EToV[(size_t)nvert * e + j]; };
     double scale = INFINITY;
     for (int e = 0; e < K; ++e) {
@@ -822,6 +824,24 @@ int estimate_time_step(int dim, bool quad, int order, int K, const double* vx, c
                 surf += std::hypot(b[0] - a[0], b[1] - a[1]);
             }
             vol = std::fabs(vol);
+        } else if (hex) {  // parallelepiped: |det| of the edge vectors, parallelogram faces
+            double u[3][3];
+            const int axis_vertex[3] = {1, 3, 4};
+            for (int k = 0; k < 3; ++k)
+                for (int d = 0; d < 3; ++d) u[k][d] = P(e, axis_vertex[k])[d] - P(e, 0)[d];
+            vol = std::fabs(u[0][0] * (u[1][1] * u[2][2] - u[1][2] * u[2][1]) -
+                            u[0][1] * (u[1][0] * u[2][2] - u[1][2] * u[2][0]) +
+                            u[0][2] * (u[1][0] * u[2][1] - u[1][1] * u[2][0]));
+            for (const auto& f : dgtd::kHexFaces) {
+                double a[3], b[3];
+                for (int d = 0; d < 3; ++d) {
+                    a[d] = P(e, f[1])[d] - P(e, f[0])[d];
+                    b[d] = P(e, f[3])[d] - P(e, f[0])[d];
+                }
+                const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2],
+                             cz = a[0] * b[1] - a[1] * b[0];
+                surf += std::sqrt(cx * cx + cy * cy + cz * cz);
+            }
         } else {
             static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
             double u[3][3];
@@ -845,7 +865,8 @@ int estimate_time_step(int dim, bool quad, int order, int K, const double* vx, c
     }
     const std::vector<double> g = dgtd::gll01(order);  // on [0, 1]
     double base = scale * (2.0 * (g[1] - g[0])) * 2.0 / 3.0 * 0.75;
-    if (quad) base /= 2.0;
+    if (quad) base /= 2.0;  // the reference's divisors for meshes holding quadrilaterals
+    if (hex) base /= 6.0;   // or hexahedra (Solver.cpp:324-328)
     base *= cfl;
     if (dim == 3 && evolution_operator == DGTD_OPERATOR_GLOBAL) base /= 0.8;
     *dt = base;
@@ -864,6 +885,61 @@ dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
     in.mu_r = mesh->mu_r;
     return in;
 }
+
+// Set-up of a hexahedral-mesh operator (dim 3, nvert = 8): the assembled
+// operator only.
+void create_hex(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
+{
+    if (kernel != DGTD_KERNEL_CSR && kernel != DGTD_KERNEL_AUTO)
+        throw UnsupportedError("hexahedra run on the assembled operator (DGTD_KERNEL_CSR)");
+    op->dim = 3;
+    op->hex = true;
+    op->H3 = dgtd::build_refhex(mesh->order);
+    const dgtd::RefHex& R = op->H3;
+    op->K = mesh->K;
+    op->Np = R.Np;
+    op->Nfp = R.Nfp;
+    op->N = (size_t)op->Np * mesh->K;
+    op->ndofs = 6 * (int64_t)op->N;
+    const dgtd::HexMeshData md = dgtd::build_hex_mesh(mesh_input(mesh), R, 1.0);
+    op->max_fscale = md.max_fscale;
+    op->nv = mesh->nv;
+    op->vx.assign(mesh->vx, mesh->vx + 3 * (size_t)mesh->nv);
+    op->EToV.assign(mesh->EToV, mesh->EToV + 8 * (size_t)mesh->K);
+    hip_check(hipSetDevice(op->device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&op->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&op->ev1), "hipEventCreate");
+    op->d_M = static_cast<double*>(upload(R.M));
+    op->d_jac = static_cast<double*>(upload(md.jac));
+    op->d_eps = static_cast<double*>(upload(md.eps));
+    op->d_mu = static_cast<double*>(upload(md.mu));
+    const int te = dgtd::norms_tile(op->Np);
+    op->nparts = std::max(1, std::min(dgtd::kNormsMaxParts, (op->K + te - 1) / te));
+    hip_check(hipMalloc(&op->d_partials, sizeof(double) * 3 * op->nparts), "hipMalloc");
+    double* scratch_dev = nullptr;  // resolves the allocation-free entry point's scratch on this device
+    hip_check(dgtd::norms_scratch(op->device, &scratch_dev), "norms scratch");
+    hip_check(hipMalloc(&op->d_out3, sizeof(double) * 3), "hipMalloc");
+    hip_check(hipMalloc(&op->d_unstable, sizeof(int)), "hipMalloc");
+    hip_check(hipMemsetAsync(op->d_unstable, 0, sizeof(int), op->stream), "memset");
+    const int nbuf = op->integrator == DGTD_LSERK4 ? 3 : 4;
+    for (int b = 0; b < nbuf; ++b) {
+        hip_check(hipMalloc(&op->d_buf[b], op->real_size() * op->ndofs), "hipMalloc state");
+        hip_check(hipMemsetAsync(op->d_buf[b], 0, op->real_size() * op->ndofs, op->stream), "memset");
+    }
+    if (!md.tfsf.empty()) {  // hexahedra owning TF/SF faces: face signs, face-node positions, records
+        const dgtd::TfsfRecords r = dgtd::tfsf_records(mesh, R, md.tfsf, md.geo);
+        op->ntfsf = (int)r.elems.size();
+        op->d_tfsf_elem = static_cast<int*>(upload(r.elems));
+        op->d_tfsf_sign = static_cast<int*>(upload(r.signs));
+        op->d_tfsf_xyz = static_cast<double*>(upload(r.xyz));
+        op->d_tfsf_geo = static_cast<double*>(upload(r.geo));
+        op->d_lift = static_cast<double*>(upload(R.LIFT));
+    }
+    hip_check(hipStreamSynchronize(op->stream), "sync");
+    select_variant(op, DGTD_KERNEL_CSR, [&] { upload_csr(op, dgtd::assemble_global_hex(md, R, mesh->sigma, op->alpha)); });
+}
+
 
 }  // namespace
 
@@ -889,8 +965,9 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
         return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
-    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)))
-        return fail(DGTD_E_ARG, "nvert must be 0, or 3 or 4 with dim = 2");
+    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)) &&
+        !(mesh->dim != 2 && (mesh->nvert == 4 || mesh->nvert == 8)))
+        return fail(DGTD_E_ARG, "nvert must be 0, 3 or 4 with dim = 2, or 4 or 8 with dim = 0 or 3");
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     dgtd_op* op = nullptr;
@@ -902,6 +979,11 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
         op->alpha = opts->alpha;
         if (mesh->dim == 2 && mesh->nvert == 4) {
             create_quad(op, mesh, opts->kernel);
+            *out = op;
+            return DGTD_OK;
+        }
+        if (mesh->dim != 2 && mesh->nvert == 8) {
+            create_hex(op, mesh, opts->kernel);
             *out = op;
             return DGTD_OK;
         }
@@ -1040,9 +1122,9 @@ int dgtd_op_max_fscale(const dgtd_op* op, double* out)
 int dgtd_op_estimate_time_step(const dgtd_op* op, int32_t evolution_operator, double cfl, double* dt)
 {
     if (!op || !dt) return fail(DGTD_E_ARG, "NULL argument");
-    const int order = op->dim == 3 ? op->R.p : op->quad ? op->Q2.p : op->T2.p;
-    return estimate_time_step(op->dim, op->quad, order, op->K, op->vx.data(), op->EToV.data(), op->max_fscale,
-                              evolution_operator, cfl, dt);
+    const int order = op->hex ? op->H3.p : op->dim == 3 ? op->R.p : op->quad ? op->Q2.p : op->T2.p;
+    return estimate_time_step(op->dim, op->quad, op->hex, order, op->K, op->vx.data(), op->EToV.data(),
+                              op->max_fscale, evolution_operator, cfl, dt);
 }
 
 int dgtd_mesh_estimate_time_step(const dgtd_mesh_desc* mesh, int32_t evolution_operator, double cfl, double* dt)
@@ -1051,11 +1133,12 @@ int dgtd_mesh_estimate_time_step(const dgtd_mesh_desc* mesh, int32_t evolution_o
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
     try {
         const bool quad = mesh->dim == 2 && mesh->nvert == 4;
+        const bool hex = mesh->dim != 2 && mesh->nvert == 8;
         double max_fscale = 0.0;
-        if (mesh->dim != 2 && evolution_operator == DGTD_OPERATOR_HESTHAVEN)
+        if (mesh->dim != 2 && !hex && evolution_operator == DGTD_OPERATOR_HESTHAVEN)
             max_fscale = dgtd::build_mesh(mesh_input(mesh), dgtd::build_reftet(mesh->order), 1.0).max_fscale;
-        return estimate_time_step(mesh->dim == 2 ? 2 : 3, quad, mesh->order, mesh->K, mesh->vx, mesh->EToV, max_fscale,
-                                  evolution_operator, cfl, dt);
+        return estimate_time_step(mesh->dim == 2 ? 2 : 3, quad, hex, mesh->order, mesh->K, mesh->vx, mesh->EToV,
+                                  max_fscale, evolution_operator, cfl, dt);
     } catch (const std::exception& e) {
         return fail(DGTD_E_ARG, e.what());
     }
@@ -1066,6 +1149,15 @@ int dgtd_op_node_positions(const dgtd_op* op, double* xyz)
     if (!op || !xyz) return fail(DGTD_E_ARG, "NULL argument");
     if (op->quad) {
         dgtd::quad_node_positions(tri_input(op), op->Q2, xyz);
+        return DGTD_OK;
+    }
+    if (op->hex) {
+        dgtd::MeshInput in;
+        in.K = op->K;
+        in.nv = op->nv;
+        in.vx = op->vx.data();
+        in.EToV = op->EToV.data();
+        dgtd::hex_node_positions(in, op->H3, xyz);
         return DGTD_OK;
     }
     if (op->dim == 2) {
@@ -1321,6 +1413,13 @@ int dgtd_op_probe_set(dgtd_op* op, int32_t n, const double* xyz, int32_t* elem)
         std::vector<double> w((size_t)n * op->Np, 0.0);
         if (op->quad) {
             dgtd::quad_locate_points(tri_input(op), op->Q2, n, xyz, el.data(), w.data());
+        } else if (op->hex) {
+            dgtd::MeshInput in;
+            in.K = op->K;
+            in.nv = op->nv;
+            in.vx = op->vx.data();
+            in.EToV = op->EToV.data();
+            dgtd::hex_locate_points(in, op->H3, n, xyz, el.data(), w.data());
         } else if (op->dim == 2) {
             dgtd::tri_locate_points(tri_input(op), op->T2, n, xyz, el.data(), w.data());
         } else {
@@ -1582,7 +1681,7 @@ int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* 
 int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP, int8_t* face_codes)
 {
     if (!mesh || !vmapM || !vmapP) return fail(DGTD_E_ARG, "NULL argument");
-    if (mesh->dim == 2) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
+    if (mesh->dim == 2 || mesh->nvert == 8) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
     try {
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
@@ -1617,6 +1716,10 @@ int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, i
             dgtd::tri_locate_points(tri_input(mesh), dgtd::build_reftri(mesh->order), n, xyz, elem, weights);
             return DGTD_OK;
         }
+        if (mesh->nvert == 8) {
+            dgtd::hex_locate_points(mesh_input(mesh), dgtd::build_refhex(mesh->order), n, xyz, elem, weights);
+            return DGTD_OK;
+        }
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         dgtd::locate_points(mesh_input(mesh), R, n, xyz, elem, weights);
     } catch (const std::exception& e) {
@@ -1628,7 +1731,7 @@ int dgtd_mesh_locate(const dgtd_mesh_desc* mesh, int32_t n, const double* xyz, i
 int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo)
 {
     if (!mesh || !geo) return fail(DGTD_E_ARG, "NULL argument");
-    if (mesh->dim == 2) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
+    if (mesh->dim == 2 || mesh->nvert == 8) return fail(DGTD_E_UNSUPPORTED, "tetrahedral meshes only");
     try {
         const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
         const dgtd::MeshData md = dgtd::build_mesh(mesh_input(mesh), R, alpha);
@@ -1655,6 +1758,9 @@ int dgtd_global_assemble(const dgtd_mesh_desc* mesh, double alpha, int64_t* nnz,
         } else if (mesh->dim == 2) {
             const dgtd::RefTri R = dgtd::build_reftri(mesh->order);
             A = dgtd::assemble_global_tri(dgtd::build_tri_mesh(tri_input(mesh), R, 1.0), R, mesh->sigma, alpha);
+        } else if (mesh->nvert == 8) {
+            const dgtd::RefHex R = dgtd::build_refhex(mesh->order);
+            A = dgtd::assemble_global_hex(dgtd::build_hex_mesh(mesh_input(mesh), R, 1.0), R, mesh->sigma, alpha);
         } else {
             const dgtd::RefTet R = dgtd::build_reftet(mesh->order);
             A = dgtd::assemble_global_tet(dgtd::build_mesh(mesh_input(mesh), R, 1.0), R, mesh->sigma, alpha);
@@ -1686,11 +1792,14 @@ extern "C" int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgt
     try {
         if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) throw std::runtime_error("order must be in 1..5");
         const bool quad = mesh->dim == 2 && mesh->nvert == 4;
-        const int Np = quad ? dgtd::quad_num_nodes(mesh->order)
-                            : mesh->dim == 2 ? dgtd::tri_num_nodes(mesh->order) : dgtd::num_nodes(mesh->order);
+        const bool hex = mesh->dim != 2 && mesh->nvert == 8;
+        const int Np = hex    ? dgtd::hex_num_nodes(mesh->order)
+                       : quad ? dgtd::quad_num_nodes(mesh->order)
+                              : mesh->dim == 2 ? dgtd::tri_num_nodes(mesh->order) : dgtd::num_nodes(mesh->order);
         const size_t N = (size_t)Np * mesh->K;
         std::vector<double> xyz(3 * N);
-        if (quad) dgtd::quad_node_positions(tri_input(mesh), dgtd::build_refquad(mesh->order), xyz.data());
+        if (hex) dgtd::hex_node_positions(mesh_input(mesh), dgtd::build_refhex(mesh->order), xyz.data());
+        else if (quad) dgtd::quad_node_positions(tri_input(mesh), dgtd::build_refquad(mesh->order), xyz.data());
         else if (mesh->dim == 2) dgtd::tri_node_positions(tri_input(mesh), dgtd::build_reftri(mesh->order), xyz.data());
         else dgtd::node_positions(mesh_input(mesh), dgtd::build_reftet(mesh->order), xyz.data());
         std::fill(x, x + 6 * N, 0.0);

@@ -22,6 +22,7 @@
 #include "global.hpp"
 
 #include "kernels.hpp"
+#include "hex.hpp"
 #include "quad.hpp"
 
 #include <algorithm>
@@ -48,8 +49,8 @@ struct ElemView {
     const int* fmask;                     // nf*Nfp
     const int* perm;                      // neighbour-node table
     // per element e: metric G[k][d] = d r_k / d x_d, normals, fscale/2, 1/eps, 1/mu
-    double G[3][3], n[4][3], fs[4], ieps, imu;
-    int nb[4], code[4];
+    double G[3][3], n[6][3], fs[6], ieps, imu;
+    int nb[6], code[6];
 };
 
 void load_tet(ElemView& v, const MeshData& md, int e)
@@ -65,6 +66,21 @@ void load_tet(ElemView& v, const MeshData& md, int e)
     }
     v.ieps = g[25];
     v.imu = g[26];
+}
+
+void load_hex(ElemView& v, const HexMeshData& md, int e)
+{
+    const double* g = &md.geo[(size_t)kHexGeoStride * e];
+    for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < 3; ++d) v.G[k][d] = g[3 * k + d];
+    for (int f = 0; f < 6; ++f) {
+        for (int d = 0; d < 3; ++d) v.n[f][d] = g[9 + 4 * f + d];
+        v.fs[f] = g[12 + 4 * f];
+        v.nb[f] = md.conn[(size_t)kHexConnStride * e + f];
+        v.code[f] = md.conn[(size_t)kHexConnStride * e + 6 + f];
+    }
+    v.ieps = g[33];
+    v.imu = g[34];
 }
 
 // triangles (nf = 3, 15-value records) and quadrilaterals (nf = 4, 18 values)
@@ -220,7 +236,8 @@ void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double
 
 CsrLocal16 csr_local16(const Csr& A, int64_t N, int Np)
 {
-    if (Np > 63) throw std::runtime_error("element-local columns need Np < 64");
+    static_assert(kCsrTab <= (1 << (16 - kCsrSlotShift)), "slots fit the code");
+    if (Np >= (1 << kCsrCompShift)) throw std::runtime_error("element-local columns need Np < 256");
     const int64_t nb = (N + kCsrRowsPerBlock - 1) / kCsrRowsPerBlock;
     CsrLocal16 out;
     out.col.resize(A.col.size());
@@ -244,7 +261,7 @@ CsrLocal16 csr_local16(const Csr& A, int64_t N, int Np)
                         if (used == kCsrTab) throw std::runtime_error("a DOF block references too many elements");
                         tab[used++] = elem;
                     }
-                    out.col[k] = (uint16_t)(slot << 9 | comp << 6 | j);
+                    out.col[k] = (uint16_t)(slot << kCsrSlotShift | comp << kCsrCompShift | j);
                 }
             }
     }
@@ -321,6 +338,33 @@ Csr assemble_global_quad(const TriMeshData& md, const RefQuad& R, const double* 
         return v;
     };
     assemble(md.K, R.Np, 4, view, sigma, alpha, out);
+    return out;
+}
+
+}  // namespace dgtd
+
+namespace dgtd {
+
+Csr assemble_global_hex(const HexMeshData& md, const RefHex& R, const double* sigma, double alpha)
+{
+    if (6.0 * R.Np * (double)md.K > 2147483647.0) throw std::runtime_error("mesh too large for 32-bit column indices");
+    Csr out;
+    auto view = [&](int e) {
+        ElemView v{};
+        v.Np = R.Np;
+        v.Nfp = R.Nfp;
+        v.nf = 6;
+        v.nd = 3;
+        v.D[0] = R.Dr.data();
+        v.D[1] = R.Ds.data();
+        v.D[2] = R.Dt.data();
+        v.lift = R.LIFT.data();
+        v.fmask = R.Fmask.data();
+        v.perm = R.face_perm.data();
+        load_hex(v, md, e);
+        return v;
+    };
+    assemble(md.K, R.Np, 6, view, sigma, alpha, out);
     return out;
 }
 

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "hex.hpp"
 #include "mesh.hpp"
 #include "quad.hpp"
 #include "tri.hpp"
@@ -23,6 +24,7 @@ struct Csr {
 Csr assemble_global_tet(const MeshData& md, const RefTet& R, const double* sigma, double alpha);
 Csr assemble_global_tri(const TriMeshData& md, const RefTri& R, const double* sigma, double alpha);
 Csr assemble_global_quad(const TriMeshData& md, const RefQuad& R, const double* sigma, double alpha);
+Csr assemble_global_hex(const HexMeshData& md, const RefHex& R, const double* sigma, double alpha);
 
 // Element-local 16-bit column codes of A for the device kernel (layout in
 // kernels.hpp: kCsrRowsPerBlock, kCsrTab); N = Np * K.

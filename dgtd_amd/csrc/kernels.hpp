@@ -55,9 +55,12 @@ constexpr double kUnstableBound = 1e20;
 // Device form of the assembled operator: the rows of DOFs [16 b, 16 b + 16)
 // (all six components) reference at most kCsrTab elements, listed in
 // etab[b * kCsrTab ...]; a column (component k, element etab[slot], node j)
-// is stored as the 16-bit code slot << 9 | k << 6 | j (nodes < 64).
+// is stored as the 16-bit code slot << kCsrSlotShift | k << kCsrCompShift | j
+// (slot < 32, nodes < 256: hexahedra up to order 5).
 constexpr int kCsrRowsPerBlock = 16;
 constexpr int kCsrTab = 32;
+constexpr int kCsrCompShift = 8;
+constexpr int kCsrSlotShift = 11;
 
 // Stage over the assembled operator (csr.hip): row r of the product is the
 // right-hand side of DOF r; the stage update as in the matrix-free kernels.
@@ -217,15 +220,18 @@ struct TfsfWave {
     int kind;
 };
 
+// Face-side slots per element in the TF/SF sign arrays (the hexahedron's six faces).
+constexpr int kTfsfSignStride = 6;
+
 template <typename T>
 struct TfsfArgs {
     int dim;             // 3: tetrahedra (4 faces, 27-value records); 2: triangles (3 edges, 15 values);
-                         // 4: quadrilaterals (4 edges, 18 values)
+                         // 4: quadrilaterals (4 edges, 18 values); 5: hexahedra (6 faces, 35 values)
     int nelem;           // elements owning TF/SF faces
     const int* elem;     // [nelem] element ids
-    const int* sign;     // [4*nelem] per face: -1 scattered side, +1 total side, 0
+    const int* sign;     // [kTfsfSignStride*nelem] per face: -1 scattered side, +1 total side, 0
     const double* xyz;   // [nelem][nfaces*Nfp][3] face node positions
-    const double* geo;   // [nelem][27 or 15] FP64 geometry records
+    const double* geo;   // [nelem][record] FP64 geometry records
     const double* lift;  // [Np][nfaces*Nfp]
     const TfsfWave* waves;
     int nwaves;

@@ -104,7 +104,8 @@ int main(int argc, char** argv)
     // conductivity, or the Global operator's own SMA weighting (alpha != 1
     // with SMA faces), needs the assembled operator
     bool sma = false;
-    for (int64_t i = 0; i < (int64_t)mesh.K * (mesh.dim == 2 && mesh.nvert != 4 ? 3 : 4); ++i)
+    const int nfaces = mesh.nvert == 8 ? 6 : mesh.dim == 2 && mesh.nvert != 4 ? 3 : 4;
+    for (int64_t i = 0; i < (int64_t)mesh.K * nfaces; ++i)
         sma = sma || mesh.face_bc[i] == DGTD_BC_SMA || mesh.face_bc[i] == DGTD_BC_INT_SMA;
     const bool global_sma = info.evolution_operator != DGTD_OPERATOR_HESTHAVEN && sma && info.alpha != 1.0;
     o.kernel = kernel == DGTD_KERNEL_AUTO && (mesh.sigma || global_sma) ? DGTD_KERNEL_CSR : kernel;

@@ -1,9 +1,12 @@
 // Mesh file readers and uniform refinement (see meshio.hpp).
 #include "meshio.hpp"
 
+#include "hex.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
@@ -84,17 +87,16 @@ TetMesh read_gmsh22(const std::string& path)
                 for (auto& t : tags) is >> t;
                 const int attr = ntags >= 2 ? (int)tags[1] : (ntags == 1 ? (int)tags[0] : 1);
                 int nn = 0;
-                if (type == 4 || type == 3) nn = 4;
+                if (type == 5) nn = 8;
+                else if (type == 4 || type == 3) nn = 4;
                 else if (type == 2) nn = 3;
                 else if (type == 1) nn = 2;
                 else if (type == 15) continue;  // points dropped
-                else if (type == 5)
-                    throw MeshUnsupported(path + ": hexahedral elements are not supported");
                 else
                     throw MeshUnsupported(path + ": element type " + std::to_string(type) +
-                                          " is not supported (straight-sided tetrahedra, triangles and "
-                                          "quadrilaterals only)");
-                int32_t vs[4];
+                                          " is not supported (straight-sided tetrahedra, hexahedra, triangles "
+                                          "and quadrilaterals only)");
+                int32_t vs[8];
                 for (int k = 0; k < nn; ++k) {
                     long long nid;
                     if (!(is >> nid)) bad(path, "bad element line");
@@ -102,7 +104,10 @@ TetMesh read_gmsh22(const std::string& path)
                     if (it == node_index.end()) bad(path, "element references unknown node");
                     vs[k] = it->second;
                 }
-                if (type == 3) {
+                if (type == 5) {  // Gmsh's hexahedron vertex order is MFEM's
+                    m.hexes.push_back({vs[0], vs[1], vs[2], vs[3], vs[4], vs[5], vs[6], vs[7]});
+                    m.hex_attr.push_back(attr);
+                } else if (type == 3) {
                     m.quads.push_back({vs[0], vs[1], vs[2], vs[3]});
                     m.quad_attr.push_back(attr);
                 } else if (nn == 4) {
@@ -120,14 +125,22 @@ TetMesh read_gmsh22(const std::string& path)
         }
     }
     if (!have_format || !have_nodes || !have_elements) bad(path, "missing $MeshFormat, $Nodes or $Elements");
-    if (m.tets.empty()) {
-        if (m.tris.empty() && m.quads.empty()) bad(path, "no tetrahedra, triangles or quadrilaterals");
+    if (!m.hexes.empty()) {
+        if (!m.tets.empty())
+            throw MeshUnsupported(path + ": meshes mixing tetrahedra and hexahedra are not supported");
+        if (!m.tris.empty())
+            throw MeshUnsupported(path + ": triangle faces beside hexahedra (prisms, pyramids) are not supported");
+        m.dim = 3;  // hexahedra are the elements, quadrilaterals the boundary
+        m.segs.clear();
+        m.seg_attr.clear();
+    } else if (m.tets.empty()) {
+        if (m.tris.empty() && m.quads.empty()) bad(path, "no tetrahedra, hexahedra, triangles or quadrilaterals");
         if (!m.tris.empty() && !m.quads.empty())
             throw MeshUnsupported(path + ": meshes mixing triangles and quadrilaterals are not supported");
         m.dim = 2;  // triangles or quadrilaterals are the elements, lines the boundary
     } else {
         if (!m.quads.empty())
-            throw MeshUnsupported(path + ": quadrilateral faces (hexahedra, prisms) are not supported");
+            throw MeshUnsupported(path + ": quadrilateral faces beside tetrahedra (prisms, pyramids) are not supported");
         m.segs.clear();  // lines of a 3D mesh are dropped
         m.seg_attr.clear();
     }
@@ -175,11 +188,17 @@ TetMesh read_mfem_mesh(const std::string& path)
             for (long i = 0; i < n; ++i) {
                 const int attr = (int)next_int();
                 const int geom = (int)next_int();
-                // tetrahedra, or in 2D triangles or squares (quadrilaterals)
-                if (m.dim == 3 ? geom != 4 : geom != 2 && geom != 3)
+                // tetrahedra or cubes (hexahedra), or in 2D triangles or squares
+                if (m.dim == 3 ? geom != 4 && geom != 5 : geom != 2 && geom != 3)
                     throw MeshUnsupported(path + ": element geometry " + std::to_string(geom) +
-                                          " is not supported (tetrahedra, triangles or quadrilaterals only)");
-                if (geom == 3) {
+                                          " is not supported (tetrahedra, hexahedra, triangles or quadrilaterals "
+                                          "only)");
+                if (geom == 5) {
+                    std::array<int32_t, 8> h;
+                    for (auto& v : h) v = (int32_t)next_int();
+                    m.hexes.push_back(h);
+                    m.hex_attr.push_back(attr);
+                } else if (geom == 3) {
                     std::array<int32_t, 4> q;
                     for (auto& v : q) v = (int32_t)next_int();
                     m.quads.push_back(q);
@@ -202,8 +221,13 @@ TetMesh read_mfem_mesh(const std::string& path)
             for (long i = 0; i < n; ++i) {
                 const int attr = (int)next_int();
                 const int geom = (int)next_int();
-                if (m.dim == 3) {
-                    if (geom != 2) bad(path, "boundary geometry must be triangles");
+                if (m.dim == 3 && geom == 3) {  // squares: the boundary of hexahedra
+                    std::array<int32_t, 4> q;
+                    for (auto& v : q) v = (int32_t)next_int();
+                    m.quads.push_back(q);
+                    m.quad_attr.push_back(attr);
+                } else if (m.dim == 3) {
+                    if (geom != 2) bad(path, "boundary geometry must be triangles or squares");
                     std::array<int32_t, 3> t;
                     for (auto& v : t) v = (int32_t)next_int();
                     m.tris.push_back(t);
@@ -234,9 +258,16 @@ TetMesh read_mfem_mesh(const std::string& path)
             bad(path, "unknown section '" + sec + "'");
         }
     }
-    if (!m.quads.empty() && !m.tris.empty())
+    if (m.dim == 2 && !m.quads.empty() && !m.tris.empty())
         throw MeshUnsupported(path + ": meshes mixing triangles and quadrilaterals are not supported");
+    if (m.dim == 3 && !m.hexes.empty() && (!m.tets.empty() || !m.tris.empty()))
+        throw MeshUnsupported(path + ": meshes mixing hexahedra with tetrahedra or triangle faces are not supported");
+    if (m.dim == 3 && m.hexes.empty() && !m.quads.empty())
+        throw MeshUnsupported(path + ": square boundary elements need hexahedra");
     if (!have_vertices || m.K() == 0) bad(path, "missing vertices or elements");
+    for (const auto& h : m.hexes)
+        for (int v : h)
+            if (v < 0 || v >= m.nv()) bad(path, "element vertex out of range");
     const int32_t nv = m.nv();
     for (const auto& q : m.quads)
         for (int v : q)
@@ -272,6 +303,23 @@ void write_mfem_mesh(const TetMesh& m, const std::string& path)
         if (!o) throw std::runtime_error("write failed: " + path);
         return;
     }
+    if (m.hex()) {  // cubes (geometry 5), squares (geometry 3)
+        o << "MFEM mesh v1.0\n\ndimension\n3\n\nelements\n" << m.K() << "\n";
+        for (int32_t e = 0; e < m.K(); ++e) {
+            o << m.hex_attr[e] << " 5";
+            for (int j = 0; j < 8; ++j) o << " " << m.hexes[e][j];
+            o << "\n";
+        }
+        o << "\nboundary\n" << m.quads.size() << "\n";
+        for (size_t i = 0; i < m.quads.size(); ++i)
+            o << m.quad_attr[i] << " 3 " << m.quads[i][0] << " " << m.quads[i][1] << " " << m.quads[i][2] << " "
+              << m.quads[i][3] << "\n";
+        o << "\nvertices\n" << m.nv() << "\n3\n";
+        for (int32_t v = 0; v < m.nv(); ++v)
+            o << m.vx[3 * v] << " " << m.vx[3 * v + 1] << " " << m.vx[3 * v + 2] << "\n";
+        if (!o) throw std::runtime_error("write failed: " + path);
+        return;
+    }
     o << "MFEM mesh v1.0\n\ndimension\n3\n\nelements\n" << m.tets.size() << "\n";
     for (size_t e = 0; e < m.tets.size(); ++e)
         o << m.tet_attr[e] << " 4 " << m.tets[e][0] << " " << m.tets[e][1] << " " << m.tets[e][2] << " "
@@ -303,7 +351,22 @@ void orient_positive(TetMesh& m)
         if (a2 == 0.0) throw std::runtime_error("degenerate triangle in mesh");
         if (a2 < 0) std::swap(t[0], t[1]);
     }
+    for (auto& h : m.hexes) {  // triple product of the edges from vertex 0 along x, y, z
+        double a[3][3];
+        const int axis[3] = {1, 3, 4};
+        for (int k = 0; k < 3; ++k)
+            for (int d = 0; d < 3; ++d) a[k][d] = m.vx[3 * (size_t)h[axis[k]] + d] - m.vx[3 * (size_t)h[0] + d];
+        const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                           a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        if (det == 0.0) throw std::runtime_error("degenerate hexahedron in mesh");
+        if (det < 0) {  // mirror across the plane of vertices 0, 2, 4, 6
+            std::swap(h[1], h[3]);
+            std::swap(h[5], h[7]);
+        }
+    }
     for (auto& q : m.quads) {  // shoelace area; a reversed cycle keeps vertex 0
+        if (m.dim != 2) break;
         double a2 = 0.0;
         for (int j = 0; j < 4; ++j) {
             const double* a = &m.vx[3 * (size_t)q[j]];
@@ -425,9 +488,58 @@ std::vector<int8_t> edge_codes(const TetMesh& m, const std::vector<int>& attr_to
 
 }  // namespace
 
+namespace {
+
+// Hexahedra: faces of MFEM's hexahedron keyed by their sorted vertices,
+// boundary quadrilaterals matched by the same key.
+std::vector<int8_t> hex_face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
+{
+    std::map<std::array<int32_t, 4>, std::pair<int64_t, int64_t>> faces;  // key -> (slot, slot or -1)
+    auto key_of = [](std::array<int32_t, 4> k) {
+        std::sort(k.begin(), k.end());
+        return k;
+    };
+    for (size_t e = 0; e < m.hexes.size(); ++e)
+        for (int f = 0; f < 6; ++f) {
+            std::array<int32_t, 4> k;
+            for (int j = 0; j < 4; ++j) k[j] = m.hexes[e][kHexFaces[f][j]];
+            const auto key = key_of(k);
+            auto it = faces.find(key);
+            if (it == faces.end())
+                faces.emplace(key, std::make_pair((int64_t)(6 * e + f), (int64_t)-1));
+            else
+                it->second.second = (int64_t)(6 * e + f);
+        }
+    std::vector<int8_t> bc(6 * m.hexes.size(), 0);
+    for (size_t i = 0; i < m.quads.size(); ++i) {
+        const int attr = m.quad_attr[i];
+        const int code = (attr >= 0 && attr < (int)attr_to_bc.size()) ? attr_to_bc[attr] : -1;
+        if (code < 0) continue;
+        auto it = faces.find(key_of(m.quads[i]));
+        if (it == faces.end())
+            throw std::runtime_error("boundary quadrilateral with attribute " + std::to_string(attr) +
+                                     " is no element face");
+        if (code == 7) {
+            if (it->second.second < 0)
+                throw std::runtime_error("TF/SF attribute " + std::to_string(attr) + " lies on a boundary face");
+            bc[it->second.first] = 7;
+            bc[it->second.second] = 7;
+        } else if (it->second.second >= 0) {
+            bc[it->second.first] = (int8_t)(code + 3);
+            bc[it->second.second] = (int8_t)(code + 3);
+        } else {
+            bc[it->second.first] = (int8_t)code;
+        }
+    }
+    return bc;
+}
+
+}  // namespace
+
 std::vector<int8_t> face_codes(const TetMesh& m, const std::vector<int>& attr_to_bc)
 {
     if (m.dim == 2) return edge_codes(m, attr_to_bc);
+    if (m.hex()) return hex_face_codes(m, attr_to_bc);
     static const int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
     const int64_t nv = m.nv();
     auto key_of = [&](int32_t a, int32_t b, int32_t c) {

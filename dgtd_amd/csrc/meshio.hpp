@@ -17,9 +17,10 @@ struct MeshUnsupported : std::runtime_error {
 };
 
 // A mesh of the launcher.  dim 3: tetrahedra are the elements, triangles the
-// boundary (and interior-boundary) elements.  dim 2: triangles, or
+// boundary (and interior-boundary) elements, or hexahedra with
+// quadrilateral boundary elements (never both).  dim 2: triangles, or
 // quadrilaterals (never both), are the elements, segments the boundary
-// elements (no tetrahedra).
+// elements.
 struct TetMesh {
     int dim = 3;
     std::vector<double> vx;                 // 3 per vertex (z = 0 in 2D)
@@ -27,25 +28,36 @@ struct TetMesh {
     std::vector<int32_t> tet_attr;
     std::vector<std::array<int32_t, 3>> tris;
     std::vector<int32_t> tri_attr;
-    std::vector<std::array<int32_t, 4>> quads;  // 2D quadrilateral elements, counter-clockwise
+    std::vector<std::array<int32_t, 4>> quads;  // 2D elements (counter-clockwise), or 3D boundary of hexahedra
     std::vector<int32_t> quad_attr;
+    std::vector<std::array<int32_t, 8>> hexes;  // 3D elements in MFEM's vertex order
+    std::vector<int32_t> hex_attr;
     std::vector<std::array<int32_t, 2>> segs;  // 2D boundary elements
     std::vector<int32_t> seg_attr;
 
     bool quad() const { return dim == 2 && !quads.empty(); }
-    int nvert() const { return dim == 2 && !quad() ? 3 : 4; }  // vertices (and faces) per element
+    bool hex() const { return dim == 3 && !hexes.empty(); }
+    int nvert() const { return hex() ? 8 : dim == 2 && !quad() ? 3 : 4; }  // vertices per element
+    int nfaces() const { return hex() ? 6 : nvert(); }
     int32_t nv() const { return (int32_t)(vx.size() / 3); }
-    int32_t K() const { return dim == 2 ? (int32_t)(quad() ? quads.size() : tris.size()) : (int32_t)tets.size(); }
-    const std::vector<int32_t>& elem_attr() const { return dim == 2 ? (quad() ? quad_attr : tri_attr) : tet_attr; }
-    const std::vector<int32_t>& bdr_attr() const { return dim == 2 ? seg_attr : tri_attr; }
+    int32_t K() const
+    {
+        return dim == 2 ? (int32_t)(quad() ? quads.size() : tris.size()) : (int32_t)(hex() ? hexes.size() : tets.size());
+    }
+    const std::vector<int32_t>& elem_attr() const
+    {
+        return dim == 2 ? (quad() ? quad_attr : tri_attr) : hex() ? hex_attr : tet_attr;
+    }
+    const std::vector<int32_t>& bdr_attr() const { return dim == 2 ? seg_attr : hex() ? quad_attr : tri_attr; }
     // local vertex j of element e (j < nvert())
     int32_t vertex(int32_t e, int j) const
     {
-        return dim == 3 ? tets[e][j] : quad() ? quads[e][j] : tris[e][j];
+        return dim == 3 ? (hex() ? hexes[e][j] : tets[e][j]) : quad() ? quads[e][j] : tris[e][j];
     }
 };
 
 // Gmsh 2.2 ASCII.  Tetrahedra (type 4) are elements, triangles (type 2)
+// boundary elements, or hexahedra (type 5) with quadrilateral (type 3)
 // boundary elements; points and lines are dropped, as the reference's
 // fixGmshMesh does for 3D meshes (driver.cpp:1076-1080).  A file without
 // tetrahedra is a 2D mesh: triangles, or quadrilaterals (type 3), are the
@@ -55,7 +67,8 @@ struct TetMesh {
 TetMesh read_gmsh22(const std::string& path);
 
 // MFEM mesh v1.0 with tetrahedra (geometry 4) and triangle boundary elements
-// (geometry 2), or in 2D triangles or quadrilaterals (geometry 3) with
+// (geometry 2), or hexahedra (geometry 5) and square boundary elements
+// (geometry 3), or in 2D triangles or quadrilaterals (geometry 3) with
 // segment boundary elements (geometry 1); straight-sided ("vertices" section; curved "nodes" refused).
 TetMesh read_mfem_mesh(const std::string& path);
 
@@ -69,8 +82,9 @@ void write_mfem_mesh(const TetMesh& m, const std::string& path);
 TetMesh read_mesh_file(const std::string& path);
 
 // Make every tetrahedron positively oriented (swap local vertices 2 and 3);
-// in 2D every triangle (swap local vertices 0 and 1) and every
-// quadrilateral (reverse the cycle: swap local vertices 1 and 3).
+// every hexahedron (mirror: swap local vertices 1 and 3, 5 and 7); in 2D
+// every triangle (swap local vertices 0 and 1) and every quadrilateral
+// (reverse the cycle: swap local vertices 1 and 3).
 void orient_positive(TetMesh& m);
 
 // One level of uniform (1:8) refinement (tetrahedra only): edge midpoints, four corner
@@ -85,6 +99,8 @@ TetMesh refine_uniform(const TetMesh& m);
 // code (value + 3) to both sides, as the reference marks such tags interior
 // (driver.cpp:1024-1035); TF/SF triangles must be interior.  Throws if a
 // triangle matches no element face.
+// On hexahedra the same rules apply to boundary quadrilaterals and the six
+// faces of MFEM's hexahedron, face_bc[6e+f].
 // In 2D the same rules apply to segments and element edges, face_bc[n e+f]
 // with edge f = (v_f, v_{f+1 mod n}), n = 3 for triangles, 4 for
 // quadrilaterals.

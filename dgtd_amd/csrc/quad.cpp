@@ -114,6 +114,42 @@ std::vector<double> inverse(std::vector<double> A, int n)
 
 }  // namespace
 
+Gll1D build_gll1d(int p)
+{
+    Gll1D g;
+    g.p = p;
+    const int n1 = p + 1;
+    g.x = gll_points(p);
+    // differentiation D[i][j] = l_j'(x_i) (barycentric weights) and the exact
+    // mass M[i][j] = int l_i l_j over [-1, 1] (Gauss-Legendre, p+2 points)
+    std::vector<double> bw(n1, 1.0);
+    for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n1; ++k)
+            if (k != j) bw[j] /= (g.x[j] - g.x[k]);
+    g.D.assign((size_t)n1 * n1, 0.0);
+    g.M.assign((size_t)n1 * n1, 0.0);
+    for (int i = 0; i < n1; ++i) {
+        double diag = 0.0;
+        for (int j = 0; j < n1; ++j)
+            if (j != i) {
+                g.D[(size_t)i * n1 + j] = (bw[j] / bw[i]) / (g.x[i] - g.x[j]);
+                diag -= g.D[(size_t)i * n1 + j];
+            }
+        g.D[(size_t)i * n1 + i] = diag;
+    }
+    std::vector<double> gx, gw;
+    gauss_legendre(p + 2, gx, gw);
+    for (size_t q = 0; q < gx.size(); ++q) {
+        const std::vector<double> l = lagrange_values(g.x, gx[q]);
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n1; ++j) g.M[(size_t)i * n1 + j] += gw[q] * l[i] * l[j];
+    }
+    g.Minv = inverse(g.M, n1);
+    return g;
+}
+
+std::vector<double> gll_lagrange(const std::vector<double>& x, double y) { return lagrange_values(x, y); }
+
 RefQuad build_refquad(int p)
 {
     if (p < 1 || p > 5) throw std::runtime_error("quadrilateral order must be in 1..5");
@@ -123,31 +159,11 @@ RefQuad build_refquad(int p)
     R.Nfp = n1;
     R.Np = n1 * n1;
     const int Np = R.Np, Nfp = R.Nfp;
-    R.x1 = gll_points(p);
-    // 1D: differentiation D1[i][j] = l_j'(x_i) (barycentric weights) and the
-    // exact mass M1[i][j] = int l_i l_j over [-1, 1] (Gauss-Legendre, p+2 points)
-    std::vector<double> bw(n1, 1.0);
-    for (int j = 0; j < n1; ++j)
-        for (int k = 0; k < n1; ++k)
-            if (k != j) bw[j] /= (R.x1[j] - R.x1[k]);
-    std::vector<double> D1((size_t)n1 * n1, 0.0), M1((size_t)n1 * n1, 0.0);
-    for (int i = 0; i < n1; ++i) {
-        double diag = 0.0;
-        for (int j = 0; j < n1; ++j)
-            if (j != i) {
-                D1[(size_t)i * n1 + j] = (bw[j] / bw[i]) / (R.x1[i] - R.x1[j]);
-                diag -= D1[(size_t)i * n1 + j];
-            }
-        D1[(size_t)i * n1 + i] = diag;
-    }
-    std::vector<double> gx, gw;
-    gauss_legendre(p + 2, gx, gw);
-    for (size_t q = 0; q < gx.size(); ++q) {
-        const std::vector<double> l = lagrange_values(R.x1, gx[q]);
-        for (int i = 0; i < n1; ++i)
-            for (int j = 0; j < n1; ++j) M1[(size_t)i * n1 + j] += gw[q] * l[i] * l[j];
-    }
-    const std::vector<double> M1inv = inverse(M1, n1);
+    const Gll1D g1 = build_gll1d(p);
+    R.x1 = g1.x;
+    const std::vector<double>& D1 = g1.D;
+    const std::vector<double>& M1 = g1.M;
+    const std::vector<double>& M1inv = g1.Minv;
     // 2D tensor products, node a = i + n1 j
     R.xi.resize(2 * (size_t)Np);
     R.M.assign((size_t)Np * Np, 0.0);

@@ -42,6 +42,18 @@ struct RefQuad {
 
 RefQuad build_refquad(int p);
 
+// The 1D Gauss-Lobatto-Legendre element of order p on [-1, 1]: points,
+// differentiation D[i][j] = l_j'(x_i), exact mass and its inverse (row-major
+// (p+1) x (p+1)); the factors of the tensor-product quadrilateral and
+// hexahedron.
+struct Gll1D {
+    int p = 0;
+    std::vector<double> x, D, M, Minv;
+};
+Gll1D build_gll1d(int p);
+// Lagrange basis on the points x, evaluated at y.
+std::vector<double> gll_lagrange(const std::vector<double>& x, double y);
+
 // Nodal interpolation weights at unit-square point (xi, eta).
 void quad_interp_weights(const RefQuad& R, double xi, double eta, double* w);
 

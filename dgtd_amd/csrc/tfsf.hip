@@ -10,7 +10,8 @@
 // stage output and c_res * S to the residual after the fused stage kernel,
 // with the coefficients the stage applies to the right-hand side.  One
 // 64-thread block per such element, FP64 arithmetic.  On triangles (dim 2)
-// and quadrilaterals (dim 4 here) the faces are the edges and n_z = 0.
+// and quadrilaterals (dim 4 here) the faces are the edges and n_z = 0;
+// hexahedra are dim 5 here (six faces).
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -19,7 +20,7 @@ namespace dgtd {
 
 namespace {
 
-constexpr int kTfsfMaxFaceNodes = 4 * 21;  // order 5
+constexpr int kTfsfMaxFaceNodes = 6 * 36;  // hexahedra of order 5
 
 // The injection of one element owning TF/SF faces (block-wide, 64 threads):
 // record k of the element lists, element e of a mesh with Np nodes per
@@ -30,14 +31,15 @@ __device__ void tfsf_element(int dim, int k, int e, const int* sign, const doubl
                              double alpha, double t, T* out, double c_out, T* res, double c_res,
                              double (&flux)[6][kTfsfMaxFaceNodes])
 {
-    // dim 3 tetrahedra (27-value records), 2 triangles (15), 4 quadrilaterals (18)
-    const bool planar = dim != 3;
-    const int stride = dim == 3 ? 27 : dim == 2 ? 15 : 18;
-    const int Nf = (dim == 2 ? 3 : 4) * Nfp;
+    // dim 3 tetrahedra (27-value records), 2 triangles (15), 4 quadrilaterals
+    // (18), 5 hexahedra (35)
+    const bool planar = dim == 2 || dim == 4;
+    const int stride = dim == 3 ? 27 : dim == 2 ? 15 : dim == 4 ? 18 : 35;
+    const int Nf = (dim == 2 ? 3 : dim == 5 ? 6 : 4) * Nfp;
     const double* g = geo + (size_t)k * stride;
     for (int m = threadIdx.x; m < Nf; m += 64) {
         const int f = m / Nfp;
-        const int sg = sign[4 * k + f];
+        const int sg = sign[kTfsfSignStride * k + f];
         double dE[3] = {0.0, 0.0, 0.0}, dH[3] = {0.0, 0.0, 0.0};
         if (sg != 0) {
             const double* x = xyz + ((size_t)k * Nf + m) * 3;
@@ -53,8 +55,8 @@ __device__ void tfsf_element(int dim, int k, int e, const int* sign, const doubl
                 }
             }
         }
-        // records: tetrahedron [9+4f..12+4f] = nx ny nz fscale/2; triangle and
-        // quadrilateral [4+3f..6+3f] = nx ny fscale/2
+        // records: tetrahedron and hexahedron [9+4f..12+4f] = nx ny nz
+        // fscale/2; triangle and quadrilateral [4+3f..6+3f] = nx ny fscale/2
         const double n[3] = {planar ? g[4 + 3 * f] : g[9 + 4 * f], planar ? g[5 + 3 * f] : g[10 + 4 * f],
                              planar ? 0.0 : g[11 + 4 * f]};
         const double fs = planar ? g[6 + 3 * f] : g[12 + 4 * f];
@@ -119,7 +121,7 @@ template <typename T>
 hipError_t launch_tfsf(const TfsfArgs<T>& a, hipStream_t s)
 {
     if (a.nelem <= 0 || a.nwaves <= 0) return hipSuccess;
-    if (4 * a.Nfp > kTfsfMaxFaceNodes) return hipErrorInvalidValue;
+    if ((a.dim == 5 ? 6 : 4) * a.Nfp > kTfsfMaxFaceNodes) return hipErrorInvalidValue;
     hipLaunchKernelGGL(tfsf_apply<T>, dim3(a.nelem), dim3(64), 0, s, a);
     return hipGetLastError();
 }

@@ -61,7 +61,7 @@ TfsfRecords records(const dgtd_mesh_desc* mesh, const std::vector<int>& Fmask, c
         if (slot[e] < 0) {
             slot[e] = (int)r.elems.size();
             r.elems.push_back(e);
-            r.signs.insert(r.signs.end(), {0, 0, 0, 0});
+            r.signs.insert(r.signs.end(), kTfsfSignStride, 0);
             r.geo.insert(r.geo.end(), geo.begin() + (size_t)stride * e, geo.begin() + (size_t)stride * (e + 1));
             const int32_t* v = &mesh->EToV[(size_t)NV * e];
             for (int m = 0; m < nfaces * Nfp; ++m) {
@@ -76,6 +76,10 @@ TfsfRecords records(const dgtd_mesh_desc* mesh, const std::vector<int>& Fmask, c
                     if (quad) {
                         x += (mesh->vx[3 * (size_t)v[1] + d] - x0) * xi[2 * (size_t)n] +
                              (mesh->vx[3 * (size_t)v[3] + d] - x0) * xi[2 * (size_t)n + 1];
+                    } else if (NV == 8) {  // hexahedra: unit-cube axes along vertices 1, 3, 4
+                        x += (mesh->vx[3 * (size_t)v[1] + d] - x0) * xi[3 * (size_t)n] +
+                             (mesh->vx[3 * (size_t)v[3] + d] - x0) * xi[3 * (size_t)n + 1] +
+                             (mesh->vx[3 * (size_t)v[4] + d] - x0) * xi[3 * (size_t)n + 2];
                     } else {
                         for (int k = 0; k < NV - 1; ++k)
                             x += (mesh->vx[3 * (size_t)v[k + 1] + d] - x0) * xi[(size_t)(NV - 1) * n + k];
@@ -84,7 +88,7 @@ TfsfRecords records(const dgtd_mesh_desc* mesh, const std::vector<int>& Fmask, c
                 }
             }
         }
-        r.signs[4 * (size_t)slot[e] + t[1]] = t[2];
+        r.signs[kTfsfSignStride * (size_t)slot[e] + t[1]] = t[2];
     }
     return r;
 }
@@ -107,6 +111,16 @@ TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefQuad& R, const std
                          const std::vector<double>& geo)
 {
     return records<4>(mesh, R.Fmask, R.xi, 4, R.Nfp, kQuadGeoStride, tfsf, geo, true);
+}
+
+}  // namespace dgtd
+
+namespace dgtd {
+
+TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefHex& R, const std::vector<std::array<int, 3>>& tfsf,
+                         const std::vector<double>& geo)
+{
+    return records<8>(mesh, R.Fmask, R.xi, 6, R.Nfp, kHexGeoStride, tfsf, geo);
 }
 
 }  // namespace dgtd

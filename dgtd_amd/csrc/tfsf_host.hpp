@@ -8,6 +8,7 @@
 
 #include "dgtd_hip.h"
 #include "kernels.hpp"
+#include "hex.hpp"
 #include "reftet.hpp"
 #include "quad.hpp"
 #include "tri.hpp"
@@ -20,7 +21,8 @@ namespace dgtd {
 bool planewaves_device(const dgtd_planewave* w, int n, std::vector<TfsfWave>& out, std::string& err);
 
 // The elements owning TF/SF faces, in TfsfArgs layout: element ids, per face
-// -1 (scattered side), +1 (total side) or 0, the face-node positions
+// (kTfsfSignStride slots per element) -1 (scattered side), +1 (total side)
+// or 0, the face-node positions
 // [nfaces*Nfp][3] and the element's FP64 geometry record.
 struct TfsfRecords {
     std::vector<int> elems, signs;
@@ -33,6 +35,8 @@ TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefTet& R, const std:
 TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefTri& R, const std::vector<std::array<int, 3>>& tfsf,
                          const std::vector<double>& geo);
 TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefQuad& R, const std::vector<std::array<int, 3>>& tfsf,
+                         const std::vector<double>& geo);
+TfsfRecords tfsf_records(const dgtd_mesh_desc* mesh, const RefHex& R, const std::vector<std::array<int, 3>>& tfsf,
                          const std::vector<double>& geo);
 
 }  // namespace dgtd

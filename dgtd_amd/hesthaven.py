@@ -107,13 +107,56 @@ def cartesian_2d(nx, ny, shape="triangle", sx=1.0, sy=1.0, bc="pec"):
     return vx, EToV, face_bc
 
 
+def cartesian_3d(nx, ny, nz, shape="hex", sx=1.0, sy=1.0, sz=1.0, bc="pec"):
+    """nx x ny x nz cells on [0,sx] x [0,sy] x [0,sz], as
+    mfem::Mesh::MakeCartesian3D: per cell one hexahedron with MFEM's vertex
+    order (shape "hex"), or six positively oriented tetrahedra around the
+    cell's (0,0,0)-(1,1,1) diagonal (shape "tet", a conforming split).  bc:
+    one code for all sides or six for MakeCartesian3D's boundary attributes
+    1..6: z=0, y=0, x=sx, y=sy, x=0, z=sz.  Returns (vx (nv,3), EToV,
+    face_bc): hexahedron faces in MFEM's order (the same six sides), a
+    tetrahedron's face f opposite its vertex f."""
+    names = {"none": 0, "pec": 1, "pmc": 2, "sma": 3}
+    sides = [bc] * 6 if isinstance(bc, (str, int)) else list(bc)
+    side = [names[b] if isinstance(b, str) else int(b) for b in sides]
+    X, Y, Z = np.meshgrid(np.linspace(0.0, sx, nx + 1), np.linspace(0.0, sy, ny + 1), np.linspace(0.0, sz, nz + 1),
+                          indexing="ij")
+    vx = np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(), Z.transpose(2, 1, 0).ravel()], axis=1)
+    i, j, k = (g.ravel() for g in np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"))
+    order = np.lexsort((i, j, k))  # x fastest, then y, then z
+    i, j, k = i[order], j[order], k[order]
+    vid = lambda a, b, c: a + (nx + 1) * (b + (ny + 1) * c)
+    hexv = np.stack([vid(i, j, k), vid(i + 1, j, k), vid(i + 1, j + 1, k), vid(i, j + 1, k),
+                     vid(i, j, k + 1), vid(i + 1, j, k + 1), vid(i + 1, j + 1, k + 1), vid(i, j + 1, k + 1)], axis=1)
+    planes = [(2, 0.0), (1, 0.0), (0, sx), (1, sy), (0, 0.0), (2, sz)]  # the six sides, attributes 1..6
+    if shape == "hex":
+        on = [k == 0, j == 0, i == nx - 1, j == ny - 1, i == 0, k == nz - 1]
+        face_bc = np.stack([np.where(on[f], side[f], 0) for f in range(6)], axis=1).astype(np.int8)
+        return vx, hexv.astype(np.int32), face_bc
+    if shape != "tet":
+        raise ValueError("shape must be 'hex' or 'tet'")
+    kuhn = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    tets = np.concatenate([hexv[:, list(t)] for t in kuhn], axis=0)
+    v = vx[tets]
+    vol = np.einsum("ij,ij->i", np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), v[:, 3] - v[:, 0])
+    neg = vol < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    face_bc = np.zeros(tets.shape, dtype=np.int8)
+    for f in range(4):
+        fv = vx[np.delete(tets, f, axis=1)]  # (K, 3, 3): the face opposite vertex f
+        for s_, (ax, val) in enumerate(planes):
+            hit = np.all(np.abs(fv[:, :, ax] - val) < 1e-12, axis=1)
+            face_bc[hit, f] = side[s_]
+    return vx, tets.astype(np.int32), face_bc
+
+
 class _Mesh:
     """Keeps the numpy arrays alive while the C descriptor points at them.
 
-    Elements: triangles (EToV (K, 3)), tetrahedra (EToV (K, 4)), or
-    quadrilaterals: EToV (K, 4) with quads=True, or by default when the
-    vertices are planar (vx (nv, 2), or z = 0 throughout, which no valid
-    tetrahedron has)."""
+    Elements: triangles (EToV (K, 3)), tetrahedra (EToV (K, 4)),
+    hexahedra (EToV (K, 8), MFEM's vertex order), or quadrilaterals: EToV
+    (K, 4) with quads=True, or by default when the vertices are planar (vx
+    (nv, 2), or z = 0 throughout, which no valid tetrahedron has)."""
 
     def __init__(self, vx, EToV, face_bc, order, eps=None, mu=None, sigma=None, quads=None):
         E = np.asarray(EToV)
@@ -122,21 +165,24 @@ class _Mesh:
             planar = v.ndim == 2 and (v.shape[1] == 2 or (v.shape[1] == 3 and not v[:, 2].any()))
             quads = E.ndim == 2 and E.shape[1] == 4 and planar
         self.quads = bool(quads)
+        self.hexes = E.ndim == 2 and E.shape[1] == 8
         # triangles (K, 3) or quadrilaterals: the 2D operator; vertices may carry 2 or 3 coordinates
         self.dim = 2 if self.quads or (E.ndim == 2 and E.shape[1] == 3) else 3
-        nvert = 4 if self.quads else self.dim + 1
+        nvert = 8 if self.hexes else 4 if self.quads else self.dim + 1
         if self.dim == 2 and v.ndim == 2 and v.shape[1] == 2:
             v = np.hstack([v, np.zeros((v.shape[0], 1))])
         self.vx = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
         self.EToV = np.ascontiguousarray(E, dtype=np.int32).reshape(-1, nvert)
-        self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, nvert)
+        nfaces = 6 if self.hexes else nvert
+        self.face_bc = None if face_bc is None else np.ascontiguousarray(face_bc, dtype=np.int8).reshape(-1, nfaces)
         self.eps = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
         self.mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
         self.sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
         self.desc = _abi.MeshDesc(self.EToV.shape[0], order, dptr(self.vx), self.vx.shape[0],
                                   i32ptr(self.EToV), i8ptr(self.face_bc), dptr(self.eps),
-                                  dptr(self.mu), self.dim, dptr(self.sigma), 4 if self.quads else 0)
-        self.Np = _nodes(order, self.dim, self.quads)
+                                  dptr(self.mu), self.dim, dptr(self.sigma),
+                                  8 if self.hexes else 4 if self.quads else 0)
+        self.Np = (order + 1) ** 3 if self.hexes else _nodes(order, self.dim, self.quads)
 
 
 def gaussian(field="E", center=(0.5, 0.5, 0.5), spread=0.1, polarization=(0, 0, 1), dimension=3):
@@ -270,9 +316,10 @@ class HesthavenEvolution:
         mesh = _Mesh(vx, EToV, face_bc, order, eps, mu, sigma, quads=quads)
         self.dim = mesh.dim
         self.quads = mesh.quads
+        self.hexes = mesh.hexes
         self.K = mesh.EToV.shape[0]
         self.Np = mesh.Np
-        self.Nfp = num_face_nodes(order) if mesh.dim == 3 else order + 1
+        self.Nfp = (order + 1) ** 2 if mesh.hexes else num_face_nodes(order) if mesh.dim == 3 else order + 1
         opts = _abi.Opts(float(alpha), int(precision), INTEGRATORS[integrator], int(device),
                          KERNELS[kernel])
         h = ctypes.c_void_p()

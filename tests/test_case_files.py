@@ -434,10 +434,10 @@ def test_2d_case_refusals(tmp_path):
     (tmp_path / "r.json").write_text(json.dumps(case))
     with pytest.raises(D.DgtdError, match="refinement of 2D meshes"):
         D.Case(str(tmp_path / "r.json"))
-    # curved (second-order) triangles (type 9), hexahedra (type 5) and meshes
+    # curved (second-order) triangles (type 9), prisms (type 6) and meshes
     # mixing triangles and quadrilaterals are refused by name
     for elems, msg in (("1\n1 9 2 1 1 1 2 3 4 5 6\n", "element type 9"),
-                       ("1\n1 5 2 1 1 1 2 3 4 5 6 1 2\n", "hexahedral"),
+                       ("1\n1 6 2 1 1 1 2 3 4 5 6\n", "element type 6"),
                        ("2\n1 2 2 1 1 1 2 3\n2 3 2 1 1 1 2 3 4\n", "mixing triangles and quadrilaterals")):
         msh = ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n6\n1 0 0 0\n2 1 0 0\n3 1 1 0\n4 0 1 0\n"
                "5 0.5 0 0\n6 0.5 0.5 0\n$EndNodes\n$Elements\n%s$EndElements\n" % elems)
@@ -576,3 +576,66 @@ def test_case_without_time_step_takes_the_estimate(tmp_path):
             c = D.Case(str(tmp_path / "c.json"))
             want = D.mesh_time_step(c.vx, c.EToV, c.order, op, cfl=0.7, quads=c.quads)
             assert c.time_step() == want > 0
+
+
+def write_gmsh_hexes(path, nx, ny, nz, sx=1.0, sy=1.0, sz=1.0, mirror=()):
+    """A MakeCartesian3D-style hexahedral Gmsh 2.2 file: hexahedra (type 5,
+    tag 1) and boundary quadrilaterals (type 3) tagged 1..6 as
+    MakeCartesian3D's attributes (z=0, y=0, x=sx, y=sy, x=0, z=sz).
+    Elements listed in `mirror` are written with a reversed (negative)
+    orientation, which the reader must correct."""
+    vx, E, fbc = D.cartesian_3d(nx, ny, nz, "hex", sx=sx, sy=sy, sz=sz, bc=(1, 2, 3, 4, 5, 6))
+    from oracle import oracle_hex as OH
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(vx))]
+    lines += [f"{i + 1} {x:.17g} {y:.17g} {z:.17g}" for i, (x, y, z) in enumerate(vx)]
+    lines += ["$EndNodes"]
+    elems = []
+    for e in range(E.shape[0]):
+        for f in range(6):
+            if fbc[e, f]:
+                q = [int(E[e, j]) + 1 for j in OH.HEX_FACES[f]]
+                elems.append(f"3 2 {fbc[e, f]} {fbc[e, f]} " + " ".join(map(str, q)))
+    for e in range(E.shape[0]):
+        v = [int(x) + 1 for x in E[e]]
+        if e in mirror:
+            v = [v[i] for i in (0, 3, 2, 1, 4, 7, 6, 5)]
+        elems.append("5 2 1 1 " + " ".join(map(str, v)))
+    lines += ["$Elements", str(len(elems))] + [f"{i + 1} {s}" for i, s in enumerate(elems)] + ["$EndElements"]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return vx, E, fbc
+
+
+def test_hexahedral_gmsh_case_and_mfem_round_trip(tmp_path):
+    """Gmsh hexahedra (type 5) with quadrilateral boundary elements: elements
+    in MFEM's vertex order (an inverted element is mirrored), face codes from
+    the boundary quadrilaterals' tags on MFEM's hexahedron faces, the case's
+    automatic time step (the mesh rule / 6 for hexahedra, / 0.8 for the
+    Global operator), and the MFEM v1.0 snapshot mesh read back."""
+    vx, E, fbc = write_gmsh_hexes(str(tmp_path / "h.msh"), 4, 2, 1, sx=2.0, mirror=(3,))
+    bnd = [{"tags": [1, 6], "type": "PEC"}, {"tags": [2, 4], "type": "PMC"}, {"tags": [3, 5], "type": "SMA"}]
+    p = write_case(str(tmp_path), "h.msh", bnd)
+    c = D.Case(p)
+    assert c.hexes and c.dim == 3 and c.K == 8 and c.EToV.shape == (8, 8) and c.Np == 27
+    np.testing.assert_array_equal(c.EToV, E)  # the mirrored element restored to MFEM's order
+    code = {1: 1, 6: 1, 2: 2, 4: 2, 3: 3, 5: 3}
+    np.testing.assert_array_equal(c.face_bc, np.vectorize(lambda a: code.get(a, 0))(fbc))
+    dt = c.time_step()
+    assert dt == pytest.approx(D.mesh_time_step(c.vx, c.EToV, 2, "global", quads=False), rel=1e-14)
+    # hand check: cells 0.5 x 0.5 x 1, p = 2 (rmin = 1): vol / (surface / 2) = 0.25 / 1.25
+    assert dt == pytest.approx(0.75 * (2 / 3) * 1.0 * (0.25 / 1.25) / 6 / 0.8, rel=1e-14)
+    c.save_mesh(tmp_path / "m.mesh")
+    text = (tmp_path / "m.mesh").read_text()
+    assert "\n1 5 " in text and "\n1 3 " in text  # cubes (geometry 5), squares (geometry 3)
+    case = json.load(open(p))
+    case["model"]["filename"] = str(tmp_path / "m.mesh")
+    (tmp_path / "c2.json").write_text(json.dumps(case))
+    c2 = D.Case(str(tmp_path / "c2.json"))
+    np.testing.assert_array_equal(c2.EToV, c.EToV)
+    np.testing.assert_array_equal(c2.face_bc, c.face_bc)
+    # the hesthaven operator refuses hexahedra
+    case["solver_options"]["evolution_operator"] = "hesthaven"
+    (tmp_path / "c3.json").write_text(json.dumps(case))
+    with pytest.raises(D.DgtdError, match="Global operator only") as ei:
+        D.Case(str(tmp_path / "c3.json"))
+    assert ei.value.code == -4

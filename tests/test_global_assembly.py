@@ -204,9 +204,96 @@ def test_quad_matrix_equals_oracle_mult(order):
         assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
 
 
-@pytest.mark.parametrize("dim", [3, 2, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_hex_matrix_equals_oracle_mult(order):
+    """Hexahedra (sheared parallelepipeds): boundary PEC/PMC/SMA, interior
+    PEC/PMC/SMA and TF/SF faces, SMA at upwind weight 1 whatever alpha, every
+    neighbour orientation the shear leaves; against oracle/oracle_hex.py."""
+    from oracle import oracle_hex as OH
+    from test_oracle_hex import sheared_hexes
+
+    vx, E, bc = sheared_hexes(3, 2, 2, bc=("pec", "pmc", "sma", "pec", "pmc", "sma"), sx=1.5)
+    bc = bc.copy()
+    bc[0, 2], bc[1, 4] = 4, 4   # interior PEC between elements 0 and 1 (x faces)
+    bc[1, 3], bc[4, 1] = 7, 7   # TF/SF between elements 1 and 4 (y faces)
+    bc[4, 5], bc[10, 0] = 5, 5  # interior PMC between elements 4 and 10 (z faces)
+    bc[2, 3], bc[5, 1] = 6, 6   # interior SMA between elements 2 and 5
+    rng = np.random.default_rng(order)
+    eps, mu = rng.uniform(1, 4, E.shape[0]), rng.uniform(1, 2, E.shape[0])
+    for alpha in (1.0, 0.5, 0.0):
+        A = D.global_assemble(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu)
+        x = rng.standard_normal(A[0].size - 1)
+        yr = OH.OperatorHex(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu).mult(x)
+        assert np.abs(csr_matvec(A, x) - yr).max() <= 1e-12 * np.abs(yr).max()
+
+
+def test_hex_neighbour_orientations():
+    """The second hexahedron of a 2 x 1 x 1 pair takes each of the cube's 24
+    rotations of its vertex list (still a positively oriented parallelepiped
+    in MFEM order): the shared face is then each of its six faces, entered at
+    each of the four corners (two positively oriented neighbours always
+    traverse a shared face in opposite directions).  The host matrix and the
+    oracle agree, and the product is unchanged up to the node relabelling
+    (the physics does not depend on the local vertex order)."""
+    from oracle import oracle_hex as OH
+
+    vx, E, bc = D.cartesian_3d(2, 1, 1, "hex", bc="pec")
+    x = np.random.default_rng(3).standard_normal(6 * 27 * 2)
+    ref = OH.OperatorHex(vx, E, bc, 2)
+    y0 = ref.mult(x)
+    # the cube's 24 proper rotations as vertex permutations of MFEM's hexahedron
+    C = OH.HEX_CORNERS
+    lookup = {tuple(c): i for i, c in enumerate(C)}
+    perms = set()
+    for mat in _rotations():
+        img = [lookup[tuple(np.rint(mat @ (c - 0.5) + 0.5).astype(int))] for c in C]
+        perms.add(tuple(img))
+    assert len(perms) == 24
+    R = OH.RefHex(2)
+    for perm in sorted(perms):
+        E2 = E.copy()
+        E2[1] = E[1][list(perm)]
+        # face codes follow the faces: boundary everywhere except the shared face
+        op2 = OH.OperatorHex(vx, E2, np.where(_shared(E2), 0, 1), 2)
+        A = D.global_assemble(vx, E2, np.where(_shared(E2), 0, 1), 2)
+        # same node set per element: map the oracle's state by node position
+        pos1, pos2 = ref.pos.reshape(-1, 3), op2.pos.reshape(-1, 3)
+        key = lambda p: np.round(p, 9).view([("", p.dtype)] * 3).ravel()
+        i1, i2 = np.argsort(key(pos1), kind="stable"), np.argsort(key(pos2), kind="stable")
+        m = np.empty(pos1.shape[0], dtype=np.int64)
+        m[i2] = i1
+        x2 = x.reshape(6, -1)[:, m].reshape(-1)
+        y2 = csr_matvec(A, x2)
+        np.testing.assert_allclose(y2, op2.mult(x2), atol=1e-12 * np.abs(y0).max())
+        np.testing.assert_allclose(y2.reshape(6, -1), y0.reshape(6, -1)[:, m], atol=1e-11 * np.abs(y0).max())
+    assert R.Np == 27
+
+
+def _rotations():
+    import itertools
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            M = np.zeros((3, 3))
+            for r, (c, s) in enumerate(zip(perm, signs)):
+                M[r, c] = s
+            if np.linalg.det(M) > 0:
+                out.append(M)
+    return out
+
+
+def _shared(E2):
+    """(2, 6) mask of the faces the two hexahedra share."""
+    from oracle import oracle_hex as OH
+
+    keys = [[tuple(sorted(E2[e][list(f)])) for f in OH.HEX_FACES] for e in range(2)]
+    common = set(keys[0]) & set(keys[1])
+    return np.array([[k in common for k in keys[e]] for e in range(2)])
+
+
+@pytest.mark.parametrize("dim", [3, 2, 4, 8])
 def test_conductivity_energy_balance(dim):
-    """dim 4: quadrilaterals."""
+    """dim 4: quadrilaterals, 8: hexahedra."""
     rng = np.random.default_rng(dim)
     if dim == 3:
         vx, E, fbc = MG.cube_mesh(2, 1.0, "pec")
@@ -214,13 +301,22 @@ def test_conductivity_energy_balance(dim):
     elif dim == 2:
         vx, E, fbc = MG.square_mesh(5, bc="pec", jitter=0.2, seed=1)
         order = 4
-    else:
+    elif dim == 4:
         vx, E, fbc = MG.quad_mesh(3, 4, bc="pec", stretch=0.3, shear=-0.2, seed=2)
         order = 3
+    else:
+        from test_oracle_hex import sheared_hexes
+
+        vx, E, fbc = sheared_hexes(2, 2, 2, bc="pec")
+        order = 2
     K = E.shape[0]
     eps, mu, sigma = rng.uniform(1, 4, K), rng.uniform(1, 2, K), rng.uniform(0.5, 3.0, K)
 
     def oracle(eps_, mu_):
+        if dim == 8:
+            from oracle import oracle_hex as OH
+
+            return OH.OperatorHex(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_)
         return (O.Operator(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_) if dim == 3
                 else O2.Operator2D(vx, E, fbc, order, alpha=0.0, eps=eps_, mu=mu_, quads=dim == 4))
 
