@@ -274,26 +274,34 @@ def bench_c3(D, args):
     return out
 
 
-def bench_global(D, args, quads=False):
+def bench_global(D, args, shape="tet"):
     """The assembled (Global) operator with conductivity: PEC cavity [0,1]^3,
     12^3 hexes x 6 = 10,368 tets, p=3, eps_r ~ U[1,4], a conducting slab
     (sigma = 2 for x < 1/2), Ez Gaussian pulse, 100 LSERK4 steps in FP32 and
-    FP64 through the fused CSR SpMV stage kernel.  With quads: the same on a
-    PEC/PMC square [0,1]^2 of 128^2 quadrilaterals (the reference runs its
-    quadrilateral meshes on this operator), p=3.  CPU leg: the same FP64 CSR
-    product on one host core (scipy), the Global operator's
-    SparseMatrix::Mult."""
+    FP64 through the fused CSR SpMV stage kernel.  shape "quad": the same on
+    a PEC/PMC square [0,1]^2 of 256^2 quadrilaterals; "hex": on a PEC cavity
+    [0,1]^3 of 32^3 hexahedra (the reference runs its quadrilateral and
+    hexahedral meshes on this operator), p=3; --size overrides the cells per
+    edge.  CPU leg: the same FP64 CSR product on one host core (scipy), the
+    Global operator's SparseMatrix::Mult."""
     order = 3
     steps = args.steps
     rng = np.random.default_rng(args.seed)
+    quads = shape == "quad"
     if quads:
-        n = 128
+        n = args.size or 256
         vx, E, fbc = D.cartesian_2d(n, n, "quad", bc=("pmc", "pec", "pmc", "pec"))
         np_ = (order + 1) ** 2
         src = [D.gaussian("E", (0.6, 0.45, 0.0), 0.1, (0, 0, 1), dimension=2)]
         mesh = f"PEC/PMC square [0,1]^2, {n}^2 quadrilaterals"
+    elif shape == "hex":
+        n = args.size or 32
+        vx, E, fbc = D.cartesian_3d(n, n, n, "hex", bc="pec")
+        np_ = (order + 1) ** 3
+        src = [D.gaussian("E", (0.6, 0.45, 0.55), 0.1, (0, 0, 1))]
+        mesh = f"PEC cavity [0,1]^3, {n}^3 hexahedra"
     else:
-        n = 12
+        n = args.size or 12
         vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
         np_ = D.num_nodes(order)
         src = [D.gaussian("E", (0.6, 0.45, 0.55), 0.1, (0, 0, 1))]
@@ -472,11 +480,14 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma", "csr"])
     ap.add_argument("--headline-only", action="store_true",
                     help="c2: skip the c1, c3 and c4 sections of the line")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "tri", "global", "quad"],
+    ap.add_argument("--workload", default="c2",
+                    choices=["c1", "c2", "c3", "c4", "c5", "tri", "global", "quad", "hex"],
                     help="c2 (default, the headline): 32^3 cavity p=4; c3: 48^3 SMA box with random eps, "
                          "p=3, 2000 steps; c4: 64 uneven cavities batched; c5: Poisson serving stream; "
                          "tri: the 2D operator on 524,288 triangles, p=4; global: the assembled CSR operator "
-                         "on 10,368 tets with conductivity; quad: the same on 128^2 quadrilaterals")
+                         "on 10,368 tets with conductivity; quad: the same on 256^2 quadrilaterals; hex: on "
+                         "32^3 hexahedra")
+    ap.add_argument("--size", type=int, default=0, help="global/quad/hex: cells per edge (0 = the workload's own)")
     ap.add_argument("--rate", type=float, default=40.0, help="c5: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="c5: seconds of arrivals")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
@@ -510,8 +521,8 @@ def main():
         return bench_tri(D, args)
     if args.workload == "global":
         return bench_global(D, args)
-    if args.workload == "quad":
-        return bench_global(D, args, quads=True)
+    if args.workload in ("quad", "hex"):
+        return bench_global(D, args, shape=args.workload)
     n, order = args.n, args.order
     vx, E, fbc = D.cube_mesh(n, 1.0, "pec")
     K = E.shape[0]

@@ -534,10 +534,15 @@ int batch_error(const std::exception& e)
 
 }  // namespace
 
-// Quadrilaterals run on the assembled (Global) operator only; the batch
-// advances hesthaven-operator instances.
-constexpr const char* kBatchNoQuads =
-    "quadrilaterals run on the assembled operator only (dgtd_op_create with DGTD_KERNEL_CSR), not in a batch";
+// Quadrilaterals and hexahedra run on the assembled (Global) operator only;
+// the batch advances hesthaven-operator instances.
+constexpr const char* kBatchNoTensorElements =
+    "quadrilaterals and hexahedra run on the assembled operator only (dgtd_op_create with DGTD_KERNEL_CSR), "
+    "not in a batch";
+inline bool tensor_elements(const dgtd_mesh_desc* mesh)
+{
+    return (mesh->dim == 2 && mesh->nvert == 4) || (mesh->dim != 2 && mesh->nvert == 8);
+}
 
 extern "C" {
 
@@ -545,7 +550,7 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
 {
     if (!mesh || !bytes) return dgtd::api_fail(DGTD_E_ARG, "NULL argument");
     if (mesh->order < 1 || mesh->order > dgtd::kMaxOrder) return dgtd::api_fail(DGTD_E_UNSUPPORTED, "order must be in 1..5");
-    if (mesh->nvert == 4 && mesh->dim == 2) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoQuads);
+    if (tensor_elements(mesh)) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoTensorElements);
     const int dim = mesh->dim == 2 ? 2 : 3;
     *bytes = footprint(dim, mesh->K, mesh->order, precision, tfsf_elements(mesh, dim));
     return DGTD_OK;
@@ -613,9 +618,10 @@ int dgtd_batch_add_tfsf(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double*
         if (!dgtd::planewaves_device(waves, nwaves, dwaves, err)) return dgtd::api_fail(DGTD_E_ARG, err);
     }
     const int dim = mesh->dim == 2 ? 2 : 3;
-    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)))
-        return dgtd::api_fail(DGTD_E_ARG, "nvert must be 0, or 3 or 4 with dim = 2");
-    if (mesh->nvert == 4) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoQuads);
+    if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)) &&
+        !(mesh->dim != 2 && (mesh->nvert == 4 || mesh->nvert == 8)))
+        return dgtd::api_fail(DGTD_E_ARG, "nvert must be 0, 3 or 4 with dim = 2, or 4 or 8 with dim = 0 or 3");
+    if (tensor_elements(mesh)) return dgtd::api_fail(DGTD_E_UNSUPPORTED, kBatchNoTensorElements);
     if (mesh->K < 1 || mesh->nv < 1 || !mesh->vx || !mesh->EToV)
         return dgtd::api_fail(DGTD_E_ARG, "the mesh needs at least one element, its vertices and EToV");
     if (mesh->sigma)

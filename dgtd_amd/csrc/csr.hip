@@ -1,12 +1,15 @@
 // Stage kernel over the assembled Global operator (GlobalEvolution::Mult,
 // GlobalEvolution.cpp:628-1088, out = A [E; H]) in CSR: a block of 256
-// threads takes 16 rows, streams their entries (value, 32-bit column, gathered
-// state value) into LDS products, 16 lanes per row sum them, and the lane with
+// threads takes 256 / LPR rows (LPR lanes per row: 16 for the long rows of
+// tetrahedra, 8 or 4 for the short rows of tensor-product elements and low
+// orders), streams their entries (value, 16-bit column code, gathered state
+// value) into LDS products, the LPR lanes of a row sum them, and the lane with
 // the sum applies the stage update of that DOF (rows are in the state layout).
 // HBM-bound on the matrix: per entry one value and a 16-bit element-local
-// column code (decoded through the block's element table in LDS).
+// column code, decoded through the element table of the entry's 16-DOF group
+// (kCsrRowsPerBlock) in LDS.
 //
-// Block order: a block takes 16 consecutive DOFs of one field component, and
+// Block order: a block takes consecutive DOFs of one field component, and
 // consecutive logical blocks cycle through the six components of the same
 // DOFs, so a run of logical blocks needs the state of one element range (and
 // its neighbours) only.  Hardware block b runs on XCD b % 8; logical block
@@ -21,32 +24,42 @@ namespace dgtd {
 
 namespace {
 
-constexpr int kLanesPerRow = 16;
-constexpr int kRowsPerBlock = 256 / kLanesPerRow;
-static_assert(kRowsPerBlock == kCsrRowsPerBlock, "the column codes are per block of 16 DOFs");
 constexpr int kXcds = 8;
 
 // kUnroll: entries per thread and chunk (the chunk is 256 * kUnroll entries).
-template <typename T, int MODE, int kUnroll>
+template <typename T, int MODE, int kUnroll, int LPR>
 __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
 {
+    constexpr int kRows = 256 / LPR;
+    constexpr int kGroups = kRows / kCsrRowsPerBlock;
+    static_assert(kGroups >= 1 && kRows % kCsrRowsPerBlock == 0, "whole 16-DOF groups per block");
     constexpr int kChunk = 256 * kUnroll;
     __shared__ T prod[kChunk];
-    __shared__ int tab[kCsrTab];
+    __shared__ int tab[kGroups * kCsrTab];
+    __shared__ int64_t gbound[kGroups + 1];
     const int t = threadIdx.x;
     const int64_t per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
     const int64_t lb = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     const int64_t N = a.nrows / 6;
-    const int64_t dof0 = (lb / 6) * kRowsPerBlock;
-    const int64_t nrow = dof0 < N ? (N - dof0 < kRowsPerBlock ? N - dof0 : kRowsPerBlock) : 0;
+    const int64_t dof0 = (lb / 6) * kRows;
+    const int64_t nrow = dof0 < N ? (N - dof0 < kRows ? N - dof0 : kRows) : 0;
     const int64_t row0 = (lb % 6) * N + (nrow > 0 ? dof0 : 0);
-    const int64_t dof = dof0 + t / kLanesPerRow;
+    const int64_t dof = dof0 + t / LPR;
     const bool valid = dof < N;  // padding blocks and the tail repeat the block's first row (no branch)
     const int64_t r = valid ? (lb % 6) * N + dof : row0;
-    const int sub = t % kLanesPerRow;
-    const int64_t nb = (N + kRowsPerBlock - 1) / kRowsPerBlock;
-    const int64_t db = lb / 6 < nb ? lb / 6 : nb - 1;
-    if (t < kCsrTab) tab[t] = a.etab[db * kCsrTab + t];
+    const int sub = t % LPR;
+    // the element tables of the block's groups (clamped to the last group) and,
+    // with several groups, the entry ranges where each group's rows start
+    const int64_t ng = (N + kCsrRowsPerBlock - 1) / kCsrRowsPerBlock;
+    if (t < kGroups * kCsrTab) {
+        int64_t g = dof0 / kCsrRowsPerBlock + t / kCsrTab;
+        g = g < ng ? g : ng - 1;
+        tab[t] = a.etab[g * kCsrTab + t % kCsrTab];
+    }
+    if (kGroups > 1 && t <= kGroups) {
+        const int64_t off = (int64_t)t * kCsrRowsPerBlock;
+        gbound[t] = a.rowptr[row0 + (off < nrow ? off : nrow)];
+    }
     __syncthreads();
     const int n32 = (int)N;  // 6 N < 2^31 (checked at assembly)
     // The block's rows are consecutive, so their entries are one contiguous
@@ -68,19 +81,24 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            const int col = (int)(c[u] >> kCsrCompShift & 7) * n32 + tab[c[u] >> kCsrSlotShift] * a.np +
+            const int64_t k = c0 + u * 256 + t;
+            const int64_t kk = k < b1 ? k : b1 - 1;  // the entry loaded above
+            int g = 0;
+#pragma unroll
+            for (int j = 1; j < kGroups; ++j) g += kk >= gbound[j] ? 1 : 0;
+            const int col = (int)(c[u] >> kCsrCompShift & 7) * n32 + tab[g * kCsrTab + (c[u] >> kCsrSlotShift)] * a.np +
                             (int)(c[u] & ((1u << kCsrCompShift) - 1));
             const T pr = v[u] * a.qin[col];
-            prod[u * 256 + t] = c0 + u * 256 + t < b1 ? pr : T(0);
+            prod[u * 256 + t] = k < b1 ? pr : T(0);
         }
         __syncthreads();
         const int64_t lo = rb > c0 ? rb : c0;
         const int64_t hi = re < c0 + kChunk ? re : c0 + kChunk;
-        for (int64_t k = lo + sub; k < hi; k += kLanesPerRow) acc += prod[k - c0];
+        for (int64_t k = lo + sub; k < hi; k += LPR) acc += prod[k - c0];
         __syncthreads();
     }
 #pragma unroll
-    for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kLanesPerRow);
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
     if (sub != 0 || !valid) return;
     const T rhs = acc;
     // stability test on the step's last stage (rare atomic; unstable_entry)
@@ -110,30 +128,48 @@ __global__ __launch_bounds__(256) void csr_stage(StageArgs<T> a)
     }
 }
 
+template <typename T, int MODE, int U, int LPR>
+hipError_t csr_go(const StageArgs<T>& a, hipStream_t s)
+{
+    constexpr int kRows = 256 / LPR;
+    const int64_t N = a.nrows / 6;
+    const int64_t blocks = 6 * ((N + kRows - 1) / kRows);
+    const int64_t grid = kXcds * ((blocks + kXcds - 1) / kXcds);
+    hipLaunchKernelGGL((csr_stage<T, MODE, U, LPR>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename T, int MODE>
 hipError_t csr_launch(const StageArgs<T>& a, hipStream_t s)
 {
-    // one chunk per block where the rows allow: the smallest chunk holding an
-    // average block's entries (16 rows), at most 4096 entries; short rows (2D
-    // meshes: 38 entries per row on order-3 quadrilaterals) take small chunks,
-    // so fewer lanes repeat the range's last entry
-    const double per_block = (double)a.nnz / (double)a.nrows * kRowsPerBlock;
-    const int64_t N = a.nrows / 6;
-    const int64_t blocks = 6 * ((N + kRowsPerBlock - 1) / kRowsPerBlock);
-    const int64_t grid = kXcds * ((blocks + kXcds - 1) / kXcds);
-    if (per_block <= 256 * 3)
-        hipLaunchKernelGGL((csr_stage<T, MODE, 3>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else if (per_block <= 256 * 5)
-        hipLaunchKernelGGL((csr_stage<T, MODE, 5>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else if (per_block <= 256 * 8)
-        hipLaunchKernelGGL((csr_stage<T, MODE, 8>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else if (sizeof(T) == 4 && per_block <= 256 * 10)  // FP64 measured faster with the 3072-entry chunk
-        hipLaunchKernelGGL((csr_stage<T, MODE, 10>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else if (per_block <= 256 * 12)
-        hipLaunchKernelGGL((csr_stage<T, MODE, 12>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((csr_stage<T, MODE, 16>), dim3((unsigned)grid), dim3(256), 0, s, a);
-    return hipGetLastError();
+    // Lanes per row by the average row length: short rows (tensor-product
+    // elements: 13 entries per row on order-3 quadrilaterals, 18 on order-3
+    // hexahedra; low orders) share a block among 64 or 32 rows, so a block
+    // still streams a full chunk; long rows (156 on order-3 tetrahedra) take
+    // 16 lanes.  One chunk per block where the rows allow: the smallest chunk
+    // holding an average block's entries, at most 4096 entries.
+    const double avg = (double)a.nnz / (double)a.nrows;
+    if (avg <= 24.0) {
+        const double per_block = avg * 64;
+        if (per_block <= 256 * 2) return csr_go<T, MODE, 2, 4>(a, s);
+        if (per_block <= 256 * 3) return csr_go<T, MODE, 3, 4>(a, s);
+        if (per_block <= 256 * 4) return csr_go<T, MODE, 4, 4>(a, s);
+        if (per_block <= 256 * 5) return csr_go<T, MODE, 5, 4>(a, s);
+        return csr_go<T, MODE, 6, 4>(a, s);
+    }
+    if (avg <= 48.0) {
+        const double per_block = avg * 32;
+        if (per_block <= 256 * 4) return csr_go<T, MODE, 4, 8>(a, s);
+        if (per_block <= 256 * 5) return csr_go<T, MODE, 5, 8>(a, s);
+        return csr_go<T, MODE, 6, 8>(a, s);
+    }
+    const double per_block = avg * 16;
+    if (per_block <= 256 * 5) return csr_go<T, MODE, 5, 16>(a, s);
+    if (per_block <= 256 * 8) return csr_go<T, MODE, 8, 16>(a, s);
+    if (sizeof(T) == 4 && per_block <= 256 * 10)  // FP64 measured faster with the 3072-entry chunk
+        return csr_go<T, MODE, 10, 16>(a, s);
+    if (per_block <= 256 * 12) return csr_go<T, MODE, 12, 16>(a, s);
+    return csr_go<T, MODE, 16, 16>(a, s);
 }
 
 }  // namespace

@@ -27,7 +27,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
+#include <thread>
 
 namespace dgtd {
 
@@ -106,15 +108,55 @@ void load_2d(ElemView& v, const TriMeshData& md, int e, int nf)
     v.imu = g[5 + 3 * nf];
 }
 
-// Dense element block: rows (c, i) of the element, columns (slot, k, j) with
-// slot 0 = own element, 1 + f = neighbour across face f.
-void element_block(const ElemView& v, double alpha, double sigma, std::vector<double>& B)
+// One element's block of the matrix: rows (c, i) of the element, columns
+// q = (slot * 6 + k) * Np + j with slot 0 = own element, 1 + f = neighbour
+// across face f.  Dense storage with a list of the touched entries per row,
+// so that filling, reading and clearing cost what the nonzeros cost.
+struct BlockAcc {
+    int rows = 0, ncol = 0;
+    std::vector<double> B;
+    std::vector<uint8_t> used;
+    std::vector<std::vector<int>> touched;
+    void init(int r, int c)
+    {
+        if (r == rows && c == ncol) return;
+        rows = r;
+        ncol = c;
+        B.assign((size_t)r * c, 0.0);
+        used.assign((size_t)r * c, 0);
+        touched.assign(r, {});
+    }
+    void add(int row, int q, double w)
+    {
+        const size_t x = (size_t)row * ncol + q;
+        if (!used[x]) {
+            used[x] = 1;
+            touched[row].push_back(q);
+        }
+        B[x] += w;
+    }
+    // the row's nonzero entries in column order; leaves the row cleared
+    template <typename F>
+    void drain(int row, F&& emit)
+    {
+        std::vector<int>& t = touched[row];
+        std::sort(t.begin(), t.end());
+        for (int q : t) {
+            const size_t x = (size_t)row * ncol + q;
+            if (B[x] != 0.0) emit(q, B[x]);
+            B[x] = 0.0;
+            used[x] = 0;
+        }
+        t.clear();
+    }
+};
+
+void element_block(const ElemView& v, double alpha, double sigma, BlockAcc& acc)
 {
     const int Np = v.Np, Nfp = v.Nfp, nf = v.nf;
-    const int ncol = (1 + nf) * 6 * Np;
-    B.assign((size_t)6 * Np * ncol, 0.0);
-    auto at = [&](int c, int i, int slot, int k, int j) -> double& {
-        return B[((size_t)c * Np + i) * ncol + ((size_t)slot * 6 + k) * Np + j];
+    acc.init(6 * Np, (1 + nf) * 6 * Np);
+    auto add = [&](int c, int i, int slot, int k, int j, double w) {
+        acc.add(c * Np + i, (slot * 6 + k) * Np + j, w);
     };
     // volume: D_d = sum_k G[k][d] D_k
     std::vector<double> Dd[3];
@@ -128,10 +170,15 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
         for (int i = 0; i < Np; ++i)
             for (int j = 0; j < Np; ++j) {
                 // E_x: (D_y H_z - D_z H_y)/eps;  H_x: (-D_y E_z + D_z E_y)/mu
-                at(x, i, 0, 3 + z, j) += Dd[y][(size_t)i * Np + j] * v.ieps;
-                at(x, i, 0, 3 + y, j) -= Dd[z][(size_t)i * Np + j] * v.ieps;
-                at(3 + x, i, 0, z, j) -= Dd[y][(size_t)i * Np + j] * v.imu;
-                at(3 + x, i, 0, y, j) += Dd[z][(size_t)i * Np + j] * v.imu;
+                const double dy = Dd[y][(size_t)i * Np + j], dz = Dd[z][(size_t)i * Np + j];
+                if (dy != 0.0) {
+                    add(x, i, 0, 3 + z, j, dy * v.ieps);
+                    add(3 + x, i, 0, z, j, -(dy * v.imu));
+                }
+                if (dz != 0.0) {
+                    add(x, i, 0, 3 + y, j, -(dz * v.ieps));
+                    add(3 + x, i, 0, y, j, dz * v.imu);
+                }
             }
     }
     // surface
@@ -174,10 +221,10 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
                         const double w = l * C[c][k];
                         if (w == 0.0) continue;
                         if (interior) {
-                            at(c, i, 1 + f, k, P) += w;
-                            at(c, i, 0, k, M) -= w;
+                            add(c, i, 1 + f, k, P, w);
+                            add(c, i, 0, k, M, -w);
                         } else {
-                            at(c, i, 0, k, M) += w * (k < 3 ? ce : ch);
+                            add(c, i, 0, k, M, w * (k < 3 ? ce : ch));
                         }
                     }
                 }
@@ -187,9 +234,11 @@ void element_block(const ElemView& v, double alpha, double sigma, std::vector<do
     // bulk conductivity on the E rows: -sigma/eps E
     if (sigma != 0.0)
         for (int c = 0; c < 3; ++c)
-            for (int i = 0; i < Np; ++i) at(c, i, 0, c, i) -= sigma * v.ieps;
+            for (int i = 0; i < Np; ++i) add(c, i, 0, c, i, -(sigma * v.ieps));
 }
 
+// Elements in contiguous ranges over host threads (each element writes its
+// own rows only): a count pass sizes the rows, a second pass fills them.
 template <typename View>
 void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double alpha, Csr& out)
 {
@@ -198,9 +247,9 @@ void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double
             if (!(sigma[e] >= 0.0) || !std::isfinite(sigma[e]))
                 throw std::runtime_error("conductivity must be finite and >= 0");
     const size_t N = (size_t)Np * K;
-    const int ncol = (1 + nf) * 6 * Np;
-    std::vector<double> B;
     std::vector<int64_t> count(6 * N + 1, 0);
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nth = std::max(1, std::min({hw, 16, K / 64}));
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) {
             out.rowptr.assign(6 * N + 1, 0);
@@ -209,25 +258,34 @@ void assemble(int K, int Np, int nf, View&& view_of, const double* sigma, double
             out.val.assign((size_t)out.rowptr[6 * N], 0.0);
             std::fill(count.begin(), count.end(), 0);
         }
-        for (int e = 0; e < K; ++e) {
-            ElemView v = view_of(e);
-            element_block(v, alpha, sigma ? sigma[e] : 0.0, B);
-            for (int c = 0; c < 6; ++c)
-                for (int i = 0; i < Np; ++i) {
-                    const size_t r = c * N + (size_t)e * Np + i;
-                    const double* row = &B[((size_t)c * Np + i) * ncol];
-                    for (int q = 0; q < ncol; ++q) {
-                        if (row[q] == 0.0) continue;
-                        if (pass == 1) {
-                            const int slot = q / (6 * Np), k = (q / Np) % 6, j = q % Np;
-                            const int elem = slot == 0 ? e : v.nb[slot - 1];
-                            const size_t at = (size_t)out.rowptr[r] + count[r];
-                            out.col[at] = (int32_t)(k * N + (size_t)elem * Np + j);
-                            out.val[at] = row[q];
-                        }
-                        ++count[r];
+        auto work = [&](int e0, int e1) {
+            BlockAcc acc;
+            for (int e = e0; e < e1; ++e) {
+                ElemView v = view_of(e);
+                element_block(v, alpha, sigma ? sigma[e] : 0.0, acc);
+                for (int c = 0; c < 6; ++c)
+                    for (int i = 0; i < Np; ++i) {
+                        const size_t r = c * N + (size_t)e * Np + i;
+                        acc.drain(c * Np + i, [&](int q, double val) {
+                            if (pass == 1) {
+                                const int slot = q / (6 * Np), k = (q / Np) % 6, j = q % Np;
+                                const int elem = slot == 0 ? e : v.nb[slot - 1];
+                                const size_t at = (size_t)out.rowptr[r] + count[r];
+                                out.col[at] = (int32_t)(k * N + (size_t)elem * Np + j);
+                                out.val[at] = val;
+                            }
+                            ++count[r];
+                        });
                     }
-                }
+            }
+        };
+        if (nth == 1) {
+            work(0, K);
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nth; ++t)
+                pool.emplace_back(work, (int)((int64_t)K * t / nth), (int)((int64_t)K * (t + 1) / nth));
+            for (std::thread& th : pool) th.join();
         }
     }
 }

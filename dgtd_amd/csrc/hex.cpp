@@ -43,7 +43,6 @@ RefHex build_refhex(int p)
     R.Dr.assign((size_t)Np * Np, 0.0);
     R.Ds.assign((size_t)Np * Np, 0.0);
     R.Dt.assign((size_t)Np * Np, 0.0);
-    std::vector<double> Minv((size_t)Np * Np, 0.0);
     for (int k = 0; k < n1; ++k)
         for (int j = 0; j < n1; ++j)
             for (int i = 0; i < n1; ++i) {
@@ -56,7 +55,6 @@ RefHex build_refhex(int p)
                         for (int ii = 0; ii < n1; ++ii) {
                             const size_t q = (size_t)a * Np + node(ii, jj, kk);
                             R.M[q] = g1.M[i * n1 + ii] * g1.M[j * n1 + jj] * g1.M[k * n1 + kk];
-                            Minv[q] = g1.Minv[i * n1 + ii] * g1.Minv[j * n1 + jj] * g1.Minv[k * n1 + kk];
                             if (jj == j && kk == k) R.Dr[q] = g1.D[i * n1 + ii];
                             if (ii == i && kk == k) R.Ds[q] = g1.D[j * n1 + jj];
                             if (ii == i && jj == j) R.Dt[q] = g1.D[k * n1 + kk];
@@ -77,22 +75,29 @@ RefHex build_refhex(int p)
                 R.Fmask[(size_t)f * Nfp + m + n1 * n] = node(ix[0], ix[1], ix[2]);
             }
     }
-    // LIFT = M^-1 E, E the face masses (reference face area 4): M1 x M1 on (m, n)
-    std::vector<double> E((size_t)Np * 6 * Nfp, 0.0);
-    for (int f = 0; f < 6; ++f)
-        for (int n = 0; n < n1; ++n)
-            for (int m = 0; m < n1; ++m)
-                for (int nn = 0; nn < n1; ++nn)
-                    for (int mm = 0; mm < n1; ++mm)
-                        E[(size_t)R.Fmask[(size_t)f * Nfp + m + n1 * n] * 6 * Nfp + f * Nfp + mm + n1 * nn] =
-                            g1.M[m * n1 + mm] * g1.M[n * n1 + nn];
+    // LIFT = M^-1 E, E the face masses (reference face area 4): M1 x M1 on the
+    // face's (m, n).  With the tensor mass M1 x M1 x M1 the tangential factors
+    // cancel (M1^-1 M1 = I; the Gauss-Lobatto points are symmetric, so a face
+    // traversed downwards gives the same identity), which leaves the closed
+    // form: face node q lifts onto the normal line through it, weight
+    // M1^-1[i, b] at line index i (b = 0 or p, the face's index on that line),
+    // and exactly zero elsewhere -- what the assembled matrix stores.
     R.LIFT.assign((size_t)Np * 6 * Nfp, 0.0);
-    for (int a = 0; a < Np; ++a)
-        for (int c = 0; c < Np; ++c) {
-            const double mv = Minv[(size_t)a * Np + c];
-            if (mv == 0.0) continue;
-            for (int q = 0; q < 6 * Nfp; ++q) R.LIFT[(size_t)a * 6 * Nfp + q] += mv * E[(size_t)c * 6 * Nfp + q];
+    for (int f = 0; f < 6; ++f) {
+        const int* A = kHexCorner[kHexFaces[f][0]];
+        const int* B = kHexCorner[kHexFaces[f][1]];
+        const int* Dv = kHexCorner[kHexFaces[f][3]];
+        int nd = 0;
+        while (!(A[nd] == B[nd] && A[nd] == Dv[nd])) ++nd;
+        const int b = p * A[nd];
+        const int stride = nd == 0 ? 1 : nd == 1 ? n1 : n1 * n1;
+        for (int q = 0; q < Nfp; ++q) {
+            const int c = R.Fmask[(size_t)f * Nfp + q];
+            const int base = c - b * stride;  // the line's node with index 0
+            for (int i = 0; i < n1; ++i)
+                R.LIFT[(size_t)(base + i * stride) * 6 * Nfp + f * Nfp + q] = g1.Minv[(size_t)i * n1 + b];
         }
+    }
     // neighbour table: own face node (m, n), own corners A, B, D at the
     // neighbour cycle's corners r, r + 1 (s = 0) or r - 1 (s = 1), and r - 1
     // or r + 1: the neighbour's face parameters are affine in (m, n)

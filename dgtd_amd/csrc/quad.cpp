@@ -169,7 +169,6 @@ RefQuad build_refquad(int p)
     R.M.assign((size_t)Np * Np, 0.0);
     R.Dr.assign((size_t)Np * Np, 0.0);
     R.Ds.assign((size_t)Np * Np, 0.0);
-    std::vector<double> Minv((size_t)Np * Np, 0.0);
     for (int j = 0; j < n1; ++j)
         for (int i = 0; i < n1; ++i) {
             const int a = i + n1 * j;
@@ -179,7 +178,6 @@ RefQuad build_refquad(int p)
                 for (int k = 0; k < n1; ++k) {
                     const int b = k + n1 * l;
                     R.M[(size_t)a * Np + b] = M1[(size_t)i * n1 + k] * M1[(size_t)j * n1 + l];
-                    Minv[(size_t)a * Np + b] = M1inv[(size_t)i * n1 + k] * M1inv[(size_t)j * n1 + l];
                     if (l == j) R.Dr[(size_t)a * Np + b] = D1[(size_t)i * n1 + k];
                     if (k == i) R.Ds[(size_t)a * Np + b] = D1[(size_t)j * n1 + l];
                 }
@@ -193,19 +191,23 @@ RefQuad build_refquad(int p)
         R.Fmask[2 * Nfp + b] = (p - b) + n1 * p;          // s = 1, i down
         R.Fmask[3 * Nfp + b] = n1 * (p - b);              // r = -1, j down
     }
-    // LIFT = M^-1 E with E the edge mass (each edge of reference length 2)
-    std::vector<double> E((size_t)Np * 4 * Nfp, 0.0);
-    for (int f = 0; f < 4; ++f)
-        for (int a = 0; a < Nfp; ++a)
-            for (int b = 0; b < Nfp; ++b)
-                E[(size_t)R.Fmask[f * Nfp + a] * 4 * Nfp + f * Nfp + b] = M1[(size_t)a * n1 + b];
+    // LIFT = M^-1 E with E the edge mass (each edge of reference length 2).
+    // With the tensor mass M1 x M1 the tangential factor cancels (M1^-1 M1 =
+    // I; the points are symmetric, so a reversed edge gives the same
+    // identity): edge node q lifts onto the normal line through it, weight
+    // M1^-1[i, b] at line index i (b = 0 or p), exactly zero elsewhere.
     R.LIFT.assign((size_t)Np * 4 * Nfp, 0.0);
-    for (int a = 0; a < Np; ++a)
-        for (int c = 0; c < Np; ++c) {
-            const double m = Minv[(size_t)a * Np + c];
-            if (m == 0.0) continue;
-            for (int q = 0; q < 4 * Nfp; ++q) R.LIFT[(size_t)a * 4 * Nfp + q] += m * E[(size_t)c * 4 * Nfp + q];
+    for (int f = 0; f < 4; ++f) {
+        const bool along_r = f == 1 || f == 3;  // edges r = +-1: the normal line runs along r
+        const int b = (f == 1 || f == 2) ? p : 0;
+        const int stride = along_r ? 1 : n1;
+        for (int q = 0; q < Nfp; ++q) {
+            const int c = R.Fmask[f * Nfp + q];
+            const int base = c - b * stride;
+            for (int i = 0; i < n1; ++i)
+                R.LIFT[(size_t)(base + i * stride) * 4 * Nfp + f * Nfp + q] = M1inv[(size_t)i * n1 + b];
         }
+    }
     // neighbour table: the neighbour edge g traverses the shared edge in the
     // same order (c = 0) or reversed (c = 1)
     R.edge_perm.assign(32 * (size_t)Nfp, -1);

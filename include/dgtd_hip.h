@@ -70,14 +70,15 @@ typedef struct dgtd_op dgtd_op;
 /* Mesh description; host arrays borrowed for the duration of dgtd_op_create.
  * Replaces the (ParFiniteElementSpace, Model) pair of the reference
  * constructor (HesthavenEvolution.cpp:315): affine tetrahedra, or affine
- * triangles with dim = 2. */
+ * triangles with dim = 2; quadrilaterals and hexahedra per nvert. */
 typedef struct {
-    int32_t K;              /* number of tetrahedra */
+    int32_t K;              /* number of elements */
     int32_t order;          /* polynomial order p, 1..5 */
     const double* vx;       /* 3*nv vertex coordinates (x,y,z interleaved) */
     int32_t nv;             /* number of vertices */
-    const int32_t* EToV;    /* 4*K vertex ids; local vertex order fixes node/face order */
-    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (1-3 boundary, 4-7 interior faces), NULL = none */
+    const int32_t* EToV;    /* 4*K vertex ids (nvert*K, see nvert); local vertex order fixes node/face order */
+    const int8_t* face_bc;  /* 4*K DGTD_BC_* codes (faces*K, see nvert; 1-3 boundary, 4-7 interior faces),
+                             * NULL = none */
     const double* eps_r;    /* K relative permittivity, NULL = 1 */
     const double* mu_r;     /* K relative permeability, NULL = 1 */
     int32_t dim;            /* 0 or 3: tetrahedra.  2: triangles -- the 2D hesthaven
@@ -93,14 +94,22 @@ typedef struct {
                              * NULL = 0.  Only with DGTD_KERNEL_CSR: the Global operator's
                              * -M_eps^-1 M_sigma E term (collectGlobalConductiveOperator,
                              * DGOperatorFactory.h:1351-1361), -sigma/eps_r E per element. */
-    int32_t nvert;          /* vertices per element with dim = 2: 0 or 3 triangles, 4
-                             * quadrilaterals -- parallelograms, on the assembled (Global)
-                             * operator only (DGTD_KERNEL_CSR; AUTO selects it): the
-                             * reference's default operator on its quadrilateral cases
-                             * (ExtensiveCasesTest.cpp:1003,1082).  EToV and face_bc hold 4
-                             * entries per element, edge f joins local vertices (f, f+1 mod
-                             * 4), nodes are the (p+1)^2 Gauss-Lobatto tensor points in
-                             * MFEM's lexicographic order. */
+    int32_t nvert;          /* vertices per element.  With dim = 2: 0 or 3 triangles, 4
+                             * quadrilaterals -- parallelograms; EToV and face_bc hold 4
+                             * entries per element, edge f joins local vertices (f, f+1
+                             * mod 4), nodes are the (p+1)^2 Gauss-Lobatto tensor points
+                             * in MFEM's lexicographic order.  With dim = 0 or 3: 0 or 4
+                             * tetrahedra, 8 hexahedra -- parallelepipeds, vertices in
+                             * MFEM's order (0 (0,0,0), 1 (1,0,0), 2 (1,1,0), 3 (0,1,0),
+                             * 4..7 the same at z = 1 of the unit cube); face_bc holds 6
+                             * entries per element for MFEM's faces (vertex cycles
+                             * (3,2,1,0), (0,1,5,4), (1,2,6,5), (2,3,7,6), (3,0,4,7),
+                             * (4,5,6,7)), nodes are the (p+1)^3 Gauss-Lobatto tensor
+                             * points, node i + (p+1) j + (p+1)^2 k at (x_i, x_j, x_k).
+                             * Both run on the assembled (Global) operator only
+                             * (DGTD_KERNEL_CSR; AUTO selects it), the reference's default
+                             * operator on its quadrilateral and hexahedral cases
+                             * (ExtensiveCasesTest.cpp:1003,1082, Solver3DTest.cpp:57-215). */
 } dgtd_mesh_desc;
 
 /* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)
@@ -125,7 +134,8 @@ typedef struct {
  * their upwind terms by 1 whatever alpha, as the Global operator does
  * (DGOperatorFactory.h:485-495, 521-531, 557-567; identical at alpha = 1),
  * plus bulk conductivity; one fused kernel does SpMV and stage update.
- * Chosen explicitly (never by autotune); tetrahedra and triangles. */
+ * Chosen explicitly (never by autotune) on tetrahedra and triangles; the
+ * only kernel of quadrilaterals and hexahedra (AUTO selects it there). */
 #define DGTD_KERNEL_CSR 3
 
 const char* dgtd_last_error(void);
@@ -151,7 +161,8 @@ int dgtd_op_max_fscale(const dgtd_op* op, double* out);
  * otherwise cfl * 0.75 * (2/3) * rmin * min_e vol_e / (surface_e / 2)
  * (calcMeshTimeStep and getTimeStepScale, Solver.cpp:207-300; rmin is the
  * first Gauss-Lobatto spacing of order p on [-1, 1], speed of light 1),
- * halved on quadrilaterals and divided by 0.8 on tetrahedra under Global. */
+ * halved on quadrilaterals, divided by 6 on hexahedra and by 0.8 on
+ * tetrahedra under Global. */
 int dgtd_op_estimate_time_step(const dgtd_op* op, int32_t evolution_operator, double cfl, double* dt);
 /* The same from a mesh description, on the host (no device work). */
 int dgtd_mesh_estimate_time_step(const dgtd_mesh_desc* mesh, int32_t evolution_operator, double cfl, double* dt);
@@ -380,10 +391,11 @@ typedef struct {
     int32_t snapshot_steps;       /* probes.domain_snapshot: "steps" (0 if absent), -1 = no snapshots */
     int32_t snapshot_saves;       /* probes.domain_snapshot: "saves", 0 if absent */
     int32_t nplanewaves;          /* "planewave" sources (their tags are TF/SF faces) */
-    int32_t dim;                  /* 3: tetrahedral mesh; 2: triangle or quadrilateral mesh
-                                   * (dgtd_case_mesh sets nvert = 4 for quadrilaterals, which
-                                   * run on the assembled operator; a quadrilateral case naming
-                                   * the hesthaven operator is refused) */
+    int32_t dim;                  /* 3: tetrahedral or hexahedral mesh; 2: triangle or
+                                   * quadrilateral mesh (dgtd_case_mesh sets nvert = 4 for
+                                   * quadrilaterals, 8 for hexahedra, which run on the
+                                   * assembled operator; such a case naming the hesthaven
+                                   * operator is refused) */
     int32_t nfield_probes;        /* probes.field entries */
 } dgtd_case_info;
 
@@ -471,7 +483,8 @@ int dgtd_batch_footprint(const dgtd_mesh_desc* mesh, int32_t precision, int64_t*
  * triangles of every order and tetrahedra of orders 2-5 run on the
  * matrix-core stage, order-1 tetrahedra on the per-node stage.  Conductivity
  * and state buffers of 4 GiB or more are refused (DGTD_E_UNSUPPORTED), and so
- * are quadrilateral meshes (nvert = 4: the assembled operator only); TF/SF
+ * are quadrilateral and hexahedral meshes (nvert 4 with dim 2, nvert 8: the
+ * assembled operator only); TF/SF
  * faces (code 7) inject nothing without plane waves (dgtd_batch_add_tfsf). */
 int dgtd_batch_add(dgtd_batch* b, const dgtd_mesh_desc* mesh, const double* x0, double dt,
                    double final_time, int32_t* id);
@@ -524,12 +537,14 @@ int dgtd_ref_element(int32_t order, double* rst, double* M, double* Dr, double* 
  * :721-791): element, local face, face node ascending; P = M on boundary faces
  * and on faces with an interior condition (their jump uses the own trace only).
  * face_codes (4*K, may be NULL) receives the code of each face (0 for plain
- * interior faces, 4-6 on both sides of a face with an interior condition). */
+ * interior faces, 4-6 on both sides of a face with an interior condition).
+ * Tetrahedral meshes only (DGTD_E_UNSUPPORTED otherwise). */
 int dgtd_mesh_vmaps(const dgtd_mesh_desc* mesh, double alpha, int32_t* vmapM, int32_t* vmapP,
                     int8_t* face_codes);
 
 /* Per-element geometric factors, 27 doubles per element: dr/dx..dt/dz (9),
- * per face nx ny nz fscale/2 (16), 1/eps_r, 1/mu_r. */
+ * per face nx ny nz fscale/2 (16), 1/eps_r, 1/mu_r.  Tetrahedral meshes only
+ * (DGTD_E_UNSUPPORTED otherwise). */
 int dgtd_mesh_geometry(const dgtd_mesh_desc* mesh, double alpha, double* geo);
 
 /* The assembled operator DGTD_KERNEL_CSR uploads, on the host in FP64 (the

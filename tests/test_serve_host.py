@@ -109,11 +109,11 @@ def test_footprint_counts_the_element_shape():
         assert footprint(vx, E, fbc, order, prec) == 3 * 6 * Np * K * s + K * 15 * s + K * 6 * 4
 
 
-def test_batch_refuses_quadrilaterals():
-    """The batch advances hesthaven-operator instances; quadrilaterals run
-    on the assembled operator only, so the admission measure refuses them
-    (the same check guards dgtd_batch_add)."""
-    vx, E, fbc = D.cartesian_2d(3, 2, "quad")
-    with pytest.raises(D.DgtdError, match="assembled operator only") as ei:
-        D.footprint(vx, E, fbc, 2)
-    assert ei.value.code == -4
+def test_batch_refuses_quadrilaterals_and_hexahedra():
+    """The batch advances hesthaven-operator instances; quadrilaterals and
+    hexahedra run on the assembled operator only, so the admission measure
+    refuses them (the same check guards dgtd_batch_add)."""
+    for vx, E, fbc in (D.cartesian_2d(3, 2, "quad"), D.cartesian_3d(2, 2, 1, "hex")):
+        with pytest.raises(D.DgtdError, match="assembled operator only") as ei:
+            D.footprint(vx, E, fbc, 2)
+        assert ei.value.code == -4
