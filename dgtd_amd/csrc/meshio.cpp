@@ -442,6 +442,91 @@ TetMesh refine_uniform(const TetMesh& m)
             r.tri_attr.push_back(m.tri_attr[i]);
         }
     }
+    // Hexahedra 1:8 on the 3 x 3 x 3 lattice of corners, edge midpoints, face
+    // centres and the centre; child (a, b, c) takes lattice points
+    // (a + x, b + y, c + z) at MFEM's corner order, so children keep the
+    // parent's orientation.  Boundary quadrilaterals split 1:4 on the same
+    // edge midpoints and face centres.
+    std::map<std::array<int32_t, 4>, int32_t> centre;
+    auto face_centre = [&](int32_t a, int32_t b, int32_t c, int32_t d) {
+        std::array<int32_t, 4> key{a, b, c, d};
+        std::sort(key.begin(), key.end());
+        auto it = centre.find(key);
+        if (it != centre.end()) return it->second;
+        const int32_t id = (int32_t)(r.vx.size() / 3);
+        for (int k = 0; k < 3; ++k)
+            r.vx.push_back(0.25 * (m.vx[3 * a + k] + m.vx[3 * b + k] + m.vx[3 * c + k] + m.vx[3 * d + k]));
+        centre.emplace(key, id);
+        return id;
+    };
+    static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    r.hexes.reserve(8 * m.hexes.size());
+    for (size_t e = 0; e < m.hexes.size(); ++e) {
+        const auto& h = m.hexes[e];
+        auto vert = [&](int i, int j, int k) {  // corner at lattice (2i, 2j, 2k)
+            for (int c = 0; c < 8; ++c)
+                if (corner[c][0] == i && corner[c][1] == j && corner[c][2] == k) return h[c];
+            return h[0];
+        };
+        int32_t L[3][3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    const int mid = (i == 1) + (j == 1) + (k == 1);
+                    if (mid == 0) {
+                        L[i][j][k] = vert(i / 2, j / 2, k / 2);
+                    } else if (mid == 1) {  // edge midpoint
+                        const int a0[3] = {i == 1 ? 0 : i / 2, j == 1 ? 0 : j / 2, k == 1 ? 0 : k / 2};
+                        const int a1[3] = {i == 1 ? 1 : i / 2, j == 1 ? 1 : j / 2, k == 1 ? 1 : k / 2};
+                        L[i][j][k] = midpoint(vert(a0[0], a0[1], a0[2]), vert(a1[0], a1[1], a1[2]));
+                    } else if (mid == 2) {  // face centre: the fixed coordinate spans no range
+                        int32_t q[4];
+                        int n = 0;
+                        for (int x = 0; x < 2; ++x)
+                            for (int y = 0; y < 2; ++y) {
+                                const int ci = i == 1 ? x : i / 2;
+                                const int cj = j == 1 ? (i == 1 ? y : x) : j / 2;
+                                const int ck = k == 1 ? y : k / 2;
+                                q[n++] = vert(ci, cj, ck);
+                            }
+                        L[i][j][k] = face_centre(q[0], q[1], q[2], q[3]);
+                    } else {
+                        const int32_t id = (int32_t)(r.vx.size() / 3);
+                        for (int d = 0; d < 3; ++d) {
+                            double c = 0.0;
+                            for (int v = 0; v < 8; ++v) c += m.vx[3 * h[v] + d];
+                            r.vx.push_back(0.125 * c);
+                        }
+                        L[1][1][1] = id;
+                    }
+                }
+        for (int c = 0; c < 8; ++c) {
+            const int a = corner[c][0], b = corner[c][1], z = corner[c][2];
+            std::array<int32_t, 8> kid;
+            for (int v = 0; v < 8; ++v) kid[v] = L[a + corner[v][0]][b + corner[v][1]][z + corner[v][2]];
+            r.hexes.push_back(kid);
+            r.hex_attr.push_back(m.hex_attr[e]);
+        }
+    }
+    for (size_t i = 0; i < m.quads.size(); ++i) {
+        const auto& q = m.quads[i];
+        int32_t P[3][3];
+        P[0][0] = q[0];
+        P[2][0] = q[1];
+        P[2][2] = q[2];
+        P[0][2] = q[3];
+        P[1][0] = midpoint(q[0], q[1]);
+        P[2][1] = midpoint(q[1], q[2]);
+        P[1][2] = midpoint(q[2], q[3]);
+        P[0][1] = midpoint(q[3], q[0]);
+        P[1][1] = face_centre(q[0], q[1], q[2], q[3]);
+        for (int t = 0; t < 2; ++t)
+            for (int u = 0; u < 2; ++u) {
+                r.quads.push_back({P[u][t], P[u + 1][t], P[u + 1][t + 1], P[u][t + 1]});
+                r.quad_attr.push_back(m.quad_attr[i]);
+            }
+    }
     return r;
 }
 

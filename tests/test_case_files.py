@@ -639,3 +639,28 @@ def test_hexahedral_gmsh_case_and_mfem_round_trip(tmp_path):
     with pytest.raises(D.DgtdError, match="Global operator only") as ei:
         D.Case(str(tmp_path / "c3.json"))
     assert ei.value.code == -4
+
+
+def test_hexahedral_case_uniform_refinement(tmp_path):
+    """refinement: 1 splits each hexahedron 1:8 and each boundary
+    quadrilateral 1:4 (MFEM's UniformRefinement): the 4 x 2 x 1 box becomes
+    the 8 x 4 x 2 box, every child in MFEM's vertex order and with the
+    boundary codes of the finer MakeCartesian3D mesh."""
+    write_gmsh_hexes(str(tmp_path / "h.msh"), 4, 2, 1, sx=2.0, mirror=(3,))
+    bnd = [{"tags": [1, 6], "type": "PEC"}, {"tags": [2, 4], "type": "PMC"}, {"tags": [3, 5], "type": "SMA"}]
+    p = write_case(str(tmp_path), "h.msh", bnd)
+    case = json.load(open(p))
+    case["model"]["refinement"] = 1
+    (tmp_path / "r.json").write_text(json.dumps(case))
+    c = D.Case(str(tmp_path / "r.json"))
+    assert c.hexes and c.K == 64
+    vx, E, fbc = D.cartesian_3d(8, 4, 2, "hex", sx=2.0, bc=(1, 2, 3, 2, 3, 1))
+    cent = lambda v, e: np.round(v[e].mean(axis=1), 12)
+    want = {tuple(x): i for i, x in enumerate(cent(vx, E))}
+    got = cent(c.vx, c.EToV)
+    idx = np.array([want[tuple(x)] for x in got])
+    assert sorted(idx) == list(range(64))
+    # same corner coordinates in the same (MFEM) vertex order, same face codes
+    np.testing.assert_allclose(c.vx[c.EToV], vx[E[idx]], atol=1e-14)
+    np.testing.assert_array_equal(c.face_bc, fbc[idx])
+    assert len(np.unique(np.round(c.vx, 12), axis=0)) == len(c.vx) == 9 * 5 * 3  # shared points merged
