@@ -342,3 +342,46 @@ def test_launcher_runs_quad_case(tmp_path):
     assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-10)
     rows = (out / "PointProbes" / "PointProbe0.dat").read_text().splitlines()
     assert rows[0] == "PointProbe ID 0"
+
+
+def test_launcher_runs_hexahedral_case(tmp_path):
+    """sma_upwind_hexa_1dot5D (Solver3DTest.cpp:155-215) as a case file: a
+    Gmsh mesh of 10 x 2 x 2 hexahedra on [0,1] x [0,0.4] x [0,0.4] with
+    MakeCartesian3D's side tags (PEC z, PMC y, SMA x), the Gaussian E_z pulse,
+    order 3, time step 7e-3, final time 1, point probes at x = 0 and x = 1.
+    The launcher runs it on the assembled operator; its PointProbe files show
+    the two halves leaving through the SMA ends (E_z = 0.5, H_y = +-0.5 at the
+    sample nearest t = 0.5; zero at t = 1; tolerance 1e-2), and its final norm
+    matches the library run."""
+    from test_case_files import write_gmsh_hexes
+
+    write_gmsh_hexes(str(tmp_path / "h.msh"), 10, 2, 2, sx=1.0, sy=0.4, sz=0.4)
+    case = {"solver_options": {"upwind_alpha": 1.0, "evolution_operator": "global", "time_step": 7e-3,
+                               "final_time": 1.0, "order": 3},
+            "model": {"filename": "h.msh", "materials": [{"tags": [1], "type": "vacuum"}],
+                      "boundaries": [{"tags": [1, 6], "type": "PEC"}, {"tags": [2, 4], "type": "PMC"},
+                                     {"tags": [3, 5], "type": "SMA"}]},
+            "probes": {"exporter": {"steps": 1},
+                       "point": [{"position": [0.0, 0.2, 0.2]}, {"position": [1.0, 0.2, 0.2]}]},
+            "sources": [{"type": "initial", "field_type": "electric", "center": [0.5, 0.2, 0.2],
+                         "polarization": [0.0, 0.0, 1.0], "dimension": 1,
+                         "magnitude": {"type": "gaussian", "spread": 0.1}}]}
+    path = tmp_path / "hexa.json"
+    path.write_text(json.dumps(case))
+    out = tmp_path / "exp"
+    r = subprocess.run([LAUNCH, str(path), "--out", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["K"] == 40 and summary["steps"] == 143 and summary["nonfinite"] == 0
+    assert summary["norm2"] <= 1e-2
+    x, info = D.run_case(str(path), precision=64, integrator="rk4")
+    assert summary["norm2"] == pytest.approx(np.linalg.norm(x), rel=1e-9, abs=1e-12)
+    tol = 1e-2
+    for p, sign in ((0, 1.0), (1, -1.0)):
+        lines = (out / "PointProbes" / f"PointProbe{p}.dat").read_text().splitlines()
+        rows = np.array([[float(v) for v in ln.split()] for ln in lines[4:]])
+        t = rows[:, 0] * 299792458.0
+        assert rows.shape == (144, 7) and t[-1] == pytest.approx(1.0, abs=1e-5)
+        mid = rows[np.argmin(np.abs(t - 0.5))]
+        assert mid[1 + EZ] == pytest.approx(0.5, abs=tol) and mid[1 + HY] == pytest.approx(0.5 * sign, abs=tol)
+        assert abs(rows[-1, 1 + EZ]) <= tol and abs(rows[-1, 1 + HY]) <= tol
