@@ -578,12 +578,14 @@ def test_case_without_time_step_takes_the_estimate(tmp_path):
             assert c.time_step() == want > 0
 
 
-def write_gmsh_hexes(path, nx, ny, nz, sx=1.0, sy=1.0, sz=1.0, mirror=()):
+def write_gmsh_hexes(path, nx, ny, nz, sx=1.0, sy=1.0, sz=1.0, mirror=(), x_planes=()):
     """A MakeCartesian3D-style hexahedral Gmsh 2.2 file: hexahedra (type 5,
     tag 1) and boundary quadrilaterals (type 3) tagged 1..6 as
     MakeCartesian3D's attributes (z=0, y=0, x=sx, y=sy, x=0, z=sz).
     Elements listed in `mirror` are written with a reversed (negative)
-    orientation, which the reader must correct."""
+    orientation, which the reader must correct.  x_planes: (x, tag) pairs,
+    interior quadrilaterals on the plane x tagged tag (written once, from the
+    element on the plane's low side)."""
     vx, E, fbc = D.cartesian_3d(nx, ny, nz, "hex", sx=sx, sy=sy, sz=sz, bc=(1, 2, 3, 4, 5, 6))
     from oracle import oracle_hex as OH
     lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(vx))]
@@ -595,6 +597,10 @@ def write_gmsh_hexes(path, nx, ny, nz, sx=1.0, sy=1.0, sz=1.0, mirror=()):
             if fbc[e, f]:
                 q = [int(E[e, j]) + 1 for j in OH.HEX_FACES[f]]
                 elems.append(f"3 2 {fbc[e, f]} {fbc[e, f]} " + " ".join(map(str, q)))
+            for xp, tag in x_planes:
+                if f == 2 and np.allclose(vx[E[e, list(OH.HEX_FACES[f])], 0], xp):
+                    q = [int(E[e, j]) + 1 for j in OH.HEX_FACES[f]]
+                    elems.append(f"3 2 {tag} {tag} " + " ".join(map(str, q)))
     for e in range(E.shape[0]):
         v = [int(x) + 1 for x in E[e]]
         if e in mirror:
@@ -664,3 +670,28 @@ def test_hexahedral_case_uniform_refinement(tmp_path):
     np.testing.assert_allclose(c.vx[c.EToV], vx[E[idx]], atol=1e-14)
     np.testing.assert_array_equal(c.face_bc, fbc[idx])
     assert len(np.unique(np.round(c.vx, 12), axis=0)) == len(c.vx) == 9 * 5 * 3  # shared points merged
+
+
+def test_hexahedral_tfsf_case(tmp_path):
+    """A plane wave on hexahedra: the TF/SF surface is an interior
+    quadrilateral surface (tag 7, plane x = 1) of a Gmsh hexahedral guide;
+    both sides of each of its faces carry code 7, and the automatic delay
+    starts the pulse 5 sqrt(2) spreads before the plane (driver.cpp:575-588)."""
+    write_gmsh_hexes(str(tmp_path / "g.msh"), 16, 1, 1, sx=4.0, sy=0.5, sz=0.5, x_planes=[(1.0, 7)])
+    case = {"solver_options": {"time_step": 0.0, "final_time": 2.2, "order": 3},
+            "model": {"filename": "g.msh", "materials": [{"tags": [1], "type": "vacuum"}],
+                      "boundaries": [{"tags": [1, 6], "type": "PEC"}, {"tags": [2, 4], "type": "PMC"},
+                                     {"tags": [3, 5], "type": "SMA"}]},
+            "sources": [{"type": "planewave", "polarization": [0.0, 0.0, 1.0], "propagation": [1.0, 0.0, 0.0],
+                         "tags": [7], "magnitude": {"type": "gaussian", "spread": 0.3}}]}
+    (tmp_path / "t.json").write_text(json.dumps(case))
+    c = D.Case(str(tmp_path / "t.json"))
+    assert c.hexes and c.K == 16
+    (w,) = c.planewaves
+    assert w.mean == pytest.approx(1.0 - 5.0 * 0.3 * np.sqrt(2.0), rel=1e-14)
+    tf = np.argwhere(c.face_bc == 7)
+    assert len(tf) == 2
+    from oracle import oracle_hex as OH
+    for e, f in tf:
+        np.testing.assert_allclose(c.vx[c.EToV[e, list(OH.HEX_FACES[f])], 0], 1.0)
+    np.testing.assert_array_equal(c.face_bc[:, 2][c.face_bc[:, 2] != 7], [0] * 14 + [3])

@@ -178,3 +178,41 @@ def test_solver3d_sma_upwind_hexa_1dot5d():
     assert abs(t - 0.5) < 7e-3 / 2
     assert f[0, EZ] == pytest.approx(0.5, abs=tol) and f[0, HY] == pytest.approx(0.5, abs=tol)
     assert f[1, EZ] == pytest.approx(0.5, abs=tol) and f[1, HY] == pytest.approx(-0.5, abs=tol)
+
+
+def test_hexahedral_tfsf_case_matches_oracle(tmp_path):
+    """The hexahedral TF/SF case file (interior quadrilateral surface x = 1 as
+    the plane-wave tag, automatic delay and time step) run through the case
+    reader's operator equals the oracle's RK4 run with the same wave, and
+    the scattered-field side stays empty."""
+    import json
+
+    from test_case_files import write_gmsh_hexes
+
+    write_gmsh_hexes(str(tmp_path / "g.msh"), 16, 1, 1, sx=4.0, sy=0.5, sz=0.5, x_planes=[(1.0, 7)])
+    case = {"solver_options": {"time_step": 0.0, "final_time": 2.2, "order": 3},
+            "model": {"filename": "g.msh", "materials": [{"tags": [1], "type": "vacuum"}],
+                      "boundaries": [{"tags": [1, 6], "type": "PEC"}, {"tags": [2, 4], "type": "PMC"},
+                                     {"tags": [3, 5], "type": "SMA"}]},
+            "sources": [{"type": "planewave", "polarization": [0.0, 0.0, 1.0], "propagation": [1.0, 0.0, 0.0],
+                         "tags": [7], "magnitude": {"type": "gaussian", "spread": 0.3}}]}
+    (tmp_path / "t.json").write_text(json.dumps(case))
+    c = D.Case(str(tmp_path / "t.json"))
+    (w,) = c.planewaves
+    op = c.operator(precision=64, integrator="rk4")
+    try:
+        assert op.kernel_variant() == "csr"
+        dt = c.time_step(op)
+        op.set_state(np.zeros(c.ndofs))
+        steps = op.run(dt, 2.2)
+        x = op.get_state()
+    finally:
+        op.close()
+    ref = OH.OperatorHex(c.vx, c.EToV, c.face_bc, 3)
+    ref.set_planewaves([dict(polarization=(0, 0, 1), propagation=(1, 0, 0), spread=0.3, mean=w.mean)])
+    xr, steps_ref, _ = ref.run(np.zeros(ref.ndofs()), dt, 2.2, "rk4")
+    assert steps == steps_ref
+    assert rel(x, xr) <= 1e-11
+    q = x.reshape(6, c.K, c.Np)
+    total = ref.pos[..., 0].mean(axis=1) > 1.0
+    assert np.abs(q[:, total]).max() > 0.1 and np.abs(q[:, ~total]).max() < 1e-12
