@@ -116,14 +116,19 @@ class Case:
         check(lib().dgtd_project_initial_fields(ctypes.byref(m.desc), srcs, len(self.sources), dptr(x)))
         return x
 
-    def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
-        """The case's operator; conductive materials, and SMA faces with
-        alpha != 1 under the Global or Maxwell operator (which weight SMA faces
-        as at alpha = 1), and quadrilateral and hexahedral meshes need the
-        assembled one ("csr"), which kernel="auto" then selects."""
+    def needs_assembled_operator(self):
+        """True when kernel "auto" runs the assembled operator ("csr"), as the
+        launcher decides: conductive materials, SMA faces (boundary or
+        interior) with alpha != 1 under the Global or Maxwell operator (which
+        weight SMA faces as at alpha = 1), quadrilateral and hexahedral meshes."""
         global_sma = (self.info["evolution_operator"] != "hesthaven" and self.info["alpha"] != 1.0
-                      and bool((self.face_bc == 3).any()))
-        if kernel == "auto" and (self.sigma is not None or global_sma or self.quads or self.hexes):
+                      and bool(((self.face_bc == 3) | (self.face_bc == 6)).any()))
+        return self.sigma is not None or global_sma or self.quads or self.hexes
+
+    def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
+        """The case's operator; kernel="auto" selects the assembled one ("csr")
+        where needs_assembled_operator() says so."""
+        if kernel == "auto" and self.needs_assembled_operator():
             kernel = "csr"
         op = HesthavenEvolution(self.vx, self.EToV, self.face_bc, self.order, alpha=self.info["alpha"],
                                 eps=self.eps, mu=self.mu, precision=precision, integrator=integrator,
@@ -137,7 +142,8 @@ class Case:
         (estimateTimeStep, Solver.cpp:110-115, 302-353): cfl / (max fscale *
         p^2) for the hesthaven operator on a 3D mesh, else the mesh rule
         0.75 * (2/3) * rmin * min vol/(surface/2) * cfl, halved on
-        quadrilaterals, / 0.8 for the Global operator in 3D.  (op is
+        quadrilaterals, / 6 on hexahedra, / 0.8 for the Global operator on
+        tetrahedra.  (op is
         accepted for compatibility and not needed.)"""
         dt = ctypes.c_double()
         check(lib().dgtd_case_time_step(self._h, ctypes.byref(dt)))

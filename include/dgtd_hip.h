@@ -359,12 +359,15 @@ int dgtd_project_initial_fields(const dgtd_mesh_desc* mesh, const dgtd_initial_f
 
 /* ------------------------------------------------------------------------
  * Case files: the reference's JSON launcher input (driver.cpp:531-706,
- * 940-1035, 1242-1300) as far as the hesthaven path runs it.
- *   solver_options: upwind_alpha, time_step (0 = Hesthaven dt rule), final_time,
+ * 940-1035, 1242-1300) as far as the hesthaven and assembled operators run
+ * it.
+ *   solver_options: upwind_alpha, time_step (0 = the reference's automatic rule,
+ *                   dgtd_case_time_step), final_time,
  *                   cfl, order, evolution_operator, basis_type (GaussLobatto),
  *                   ode_type (RK4)
  *   model: filename (Gmsh 2.2 ASCII .msh or MFEM v1.0 .mesh, relative to the
- *          case file; driver.cpp:1083-1188), refinement (uniform, 1:8),
+ *          case file; driver.cpp:1083-1188; tetrahedra or hexahedra, triangles
+ *          or quadrilaterals), refinement (uniform, 1:8 in 3D),
  *          materials (relative_permittivity / _permeability / bulk_conductivity
  *          per element tag; a conductive case needs DGTD_KERNEL_CSR),
  *          boundaries (PEC / PMC / SMA per tag; a tag on interior faces
@@ -384,7 +387,10 @@ typedef struct dgtd_case dgtd_case;
 typedef struct {
     int32_t K, nv, order;
     double alpha, time_step, final_time, cfl;
-    int32_t evolution_operator;   /* as named in the case; all run the hesthaven operator */
+    int32_t evolution_operator;   /* as named in the case (DGTD_OPERATOR_*; "global" when absent):
+                                   * the matrix-free hesthaven operator, or the assembled one
+                                   * for conductivity, SMA faces with alpha != 1 under Global
+                                   * or Maxwell, quadrilaterals and hexahedra */
     int32_t nsources, nprobes;
     int32_t exporter_steps;       /* probes.exporter.steps, 0 if absent */
     int32_t refinement;           /* uniform refinement levels applied */

@@ -695,3 +695,27 @@ def test_hexahedral_tfsf_case(tmp_path):
     for e, f in tf:
         np.testing.assert_allclose(c.vx[c.EToV[e, list(OH.HEX_FACES[f])], 0], 1.0)
     np.testing.assert_array_equal(c.face_bc[:, 2][c.face_bc[:, 2] != 7], [0] * 14 + [3])
+
+
+def test_case_kernel_choice_counts_interior_sma(tmp_path):
+    """kernel "auto" takes the assembled operator where the Global operator's
+    SMA weighting differs from the matrix-free one: SMA faces at alpha != 1,
+    interior SMA (code 6) as well as boundary SMA, as the launcher decides."""
+    shutil.copy(os.path.join(CASES, "2D_TFSF_IntBoundary.msh"), tmp_path / "2D_TFSF_IntBoundary.msh")
+    case = json.load(open(os.path.join(CASES, "2D_TFSF_IntBoundary.json")))
+
+    def variant(name, interior, alpha, op="global"):
+        case["model"]["boundaries"] = [{"tags": [18], "type": interior},
+                                       {"tags": [13, 15, 16, 17, 19, 20, 21, 22], "type": "PEC"}]
+        case["solver_options"].update(upwind_alpha=alpha, evolution_operator=op)
+        (tmp_path / name).write_text(json.dumps(case))
+        return D.Case(str(tmp_path / name))
+
+    c = variant("a.json", "SMA", 0.5)
+    assert (c.face_bc == 6).any() and not (c.face_bc == 3).any()
+    assert c.needs_assembled_operator()
+    assert not variant("b.json", "PEC", 0.5).needs_assembled_operator()
+    assert not variant("c.json", "SMA", 1.0).needs_assembled_operator()
+    assert not variant("d.json", "SMA", 0.5, "hesthaven").needs_assembled_operator()
+    q = D.Case(os.path.join(CASES, "2D_TFSF_Quads.json"))
+    assert q.quads and q.needs_assembled_operator()
