@@ -315,8 +315,8 @@ def bench_global(D, args, shape="tet"):
     out = {"workload": f"{mesh}, p={order}, eps_r~U[1,4], "
                        f"sigma=2 for x<1/2, Ez Gaussian, {steps} LSERK4 steps, assembled CSR operator", "dof": dof}
     states = {}
-    for prec in (32, 64):
-        op = D.GlobalEvolution(vx, E, fbc, order, eps=eps, sigma=sigma, precision=prec, integrator="lserk4")
+
+    def timed(op, prec, name):
         dt = 0.5 / (op.max_fscale() * order ** 2)
         op.set_state(x0)
         e0 = op.norms()["energy"]
@@ -328,13 +328,24 @@ def bench_global(D, args, shape="tet"):
         us, launches = op.last_kernel_us()
         nm = op.norms()
         b, f = op.work_per_element()
-        states[prec] = op.get_state()
-        out[f"fp{prec}"] = {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
-                            "kernel_avg_us": us, "kernel": f"csr_stage<{'float' if prec == 32 else 'double'},LSERK>",
-                            "algorithmic_bytes_per_launch": b * K,
-                            "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                            "energy_initial": e0, "energy_final": nm["energy"], "nonfinite": nm["nonfinite"]}
+        state = op.get_state()
         op.close()
+        return state, {"value": dof * 5 * steps / wall, "unit": "DOF*RK-stages/s", "wall_s": wall,
+                       "kernel_avg_us": us, "kernel": f"{name}<{'float' if prec == 32 else 'double'},LSERK>",
+                       "algorithmic_bytes_per_launch": b * K,
+                       "hbm_frac": b * K / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                       "energy_initial": e0, "energy_final": nm["energy"], "nonfinite": nm["nonfinite"]}
+
+    for prec in (32, 64):
+        op = D.GlobalEvolution(vx, E, fbc, order, eps=eps, sigma=sigma, precision=prec, integrator="lserk4")
+        states[prec], out[f"fp{prec}"] = timed(op, prec, "csr_stage")
+    if shape == "hex":  # the matrix-free hexahedron stage on the same operator
+        for prec in (32, 64):
+            op = D.HesthavenEvolution(vx, E, fbc, order, eps=eps, sigma=sigma, precision=prec,
+                                      integrator="lserk4", kernel="valu")
+            st, out[f"matrix_free_fp{prec}"] = timed(op, prec, "hex_stage")
+            out[f"matrix_free_fp{prec}"]["rel_l2_vs_csr"] = float(np.linalg.norm(st - states[prec])
+                                                                   / np.linalg.norm(states[prec]))
     out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(states[32] - states[64]) / np.linalg.norm(states[64]))
     if not args.no_cpu:
         import scipy.sparse as sp

@@ -20,6 +20,7 @@
 #include "fields.hpp"
 #include "kernels.hpp"
 #include "trace.hpp"
+#include "hex_dev.hpp"
 #include "tfsf_host.hpp"
 #include "mesh.hpp"
 #include "packing.hpp"
@@ -241,6 +242,7 @@ struct dgtd_op {
     hipError_t launch_on(const dgtd::StageArgs<T>& a0, int mode, int var, hipStream_t s) const
     {
         if (var == DGTD_KERNEL_CSR) return dgtd::launch_csr_stage<T>(a0, mode, s);
+        if (hex) return dgtd::launch_hex_stage<T>(H3.p, a0, mode, s);
         if (var == DGTD_KERNEL_MFMA) {
             dgtd::StageArgs<T> a = a0;
             a.mats = static_cast<const T*>(d_mats_mfma);
@@ -888,10 +890,14 @@ dgtd::MeshInput mesh_input(const dgtd_mesh_desc* mesh)
 
 // Set-up of a hexahedral-mesh operator (dim 3, nvert = 8): the assembled
 // operator only.
+// Hexahedra run the Global operator's flux, either assembled (DGTD_KERNEL_CSR)
+// or matrix-free (DGTD_KERNEL_VALU, which DGTD_KERNEL_AUTO selects:
+// hex_stage.hip reads no matrix, and the set-up assembles none).
 void create_hex(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
 {
-    if (kernel != DGTD_KERNEL_CSR && kernel != DGTD_KERNEL_AUTO)
-        throw UnsupportedError("hexahedra run on the assembled operator (DGTD_KERNEL_CSR)");
+    if (kernel != DGTD_KERNEL_CSR && kernel != DGTD_KERNEL_AUTO && kernel != DGTD_KERNEL_VALU)
+        throw UnsupportedError("hexahedra run the Global operator: assembled (DGTD_KERNEL_CSR) or matrix-free "
+                               "(DGTD_KERNEL_VALU), not on the matrix cores");
     op->dim = 3;
     op->hex = true;
     op->H3 = dgtd::build_refhex(mesh->order);
@@ -936,6 +942,54 @@ void create_hex(dgtd_op* op, const dgtd_mesh_desc* mesh, int kernel)
         op->d_tfsf_geo = static_cast<double*>(upload(r.geo));
         op->d_lift = static_cast<double*>(upload(R.LIFT));
     }
+    if (kernel != DGTD_KERNEL_CSR) {
+        // matrix-free records: geometry with sigma/eps, connectivity, D1 and
+        // the two used columns of M1^-1, face tables
+        const int K = mesh->K, n1 = R.p + 1, Np = R.Np, Nfp = R.Nfp;
+        std::vector<double> geo((size_t)dgtd::kHexGeoDev * K);
+        for (int e = 0; e < K; ++e) {
+            for (int v = 0; v < dgtd::kHexGeoStride; ++v)
+                geo[(size_t)dgtd::kHexGeoDev * e + v] = md.geo[(size_t)dgtd::kHexGeoStride * e + v];
+            const double sg = mesh->sigma ? mesh->sigma[e] : 0.0;
+            if (!(sg >= 0.0) || !std::isfinite(sg)) throw std::runtime_error("conductivity must be finite and >= 0");
+            geo[(size_t)dgtd::kHexGeoDev * e + 35] = sg * md.geo[(size_t)dgtd::kHexGeoStride * e + 33];
+        }
+        const dgtd::Gll1D g1 = dgtd::build_gll1d(R.p);
+        std::vector<double> m(g1.D.begin(), g1.D.end());
+        for (int i = 0; i < n1; ++i) {
+            m.push_back(g1.Minv[(size_t)i * n1 + 0]);
+            m.push_back(g1.Minv[(size_t)i * n1 + R.p]);
+        }
+        std::vector<int> tabs(R.Fmask.begin(), R.Fmask.end());
+        tabs.insert(tabs.end(), R.face_perm.begin(), R.face_perm.end());
+        std::vector<int> fq((size_t)6 * Np, -1);
+        for (int f = 0; f < 6; ++f) {
+            const int axis = f == 0 || f == 5 ? 2 : f == 1 || f == 3 ? 1 : 0;
+            const int stride = axis == 0 ? 1 : axis == 1 ? n1 : n1 * n1;
+            const int b = f == 2 || f == 3 || f == 5 ? R.p : 0;
+            for (int q = 0; q < Nfp; ++q) {
+                const int c = R.Fmask[(size_t)f * Nfp + q];
+                if ((c / stride) % n1 != b) throw std::logic_error("hexahedron face table");
+                for (int i = 0; i < n1; ++i) fq[(size_t)f * Np + c + (i - b) * stride] = q;
+            }
+        }
+        for (int v : fq)
+            if (v < 0) throw std::logic_error("hexahedron normal-line table");
+        tabs.insert(tabs.end(), fq.begin(), fq.end());
+        op->d_tabs = static_cast<int*>(upload(tabs));
+        op->d_conn = static_cast<int*>(upload(md.conn));
+        if (op->precision == 64) {
+            op->d_geo = upload(geo);
+            op->d_mats = upload(m);
+        } else {
+            op->d_geo = upload(dgtd::cast_vec<float>(geo));
+            op->d_mats = upload(dgtd::cast_vec<float>(m));
+        }
+        hip_check(hipStreamSynchronize(op->stream), "sync");
+        op->variant = DGTD_KERNEL_VALU;
+        op->tuning.variant = DGTD_KERNEL_VALU;
+        return;
+    }
     hip_check(hipStreamSynchronize(op->stream), "sync");
     select_variant(op, DGTD_KERNEL_CSR, [&] { upload_csr(op, dgtd::assemble_global_hex(md, R, mesh->sigma, op->alpha)); });
 }
@@ -962,7 +1016,8 @@ int dgtd_op_create(const dgtd_mesh_desc* mesh, const dgtd_opts* opts, dgtd_op** 
     if (!std::isfinite(opts->alpha)) return fail(DGTD_E_ARG, "alpha must be finite");
     if (opts->kernel < DGTD_KERNEL_AUTO || opts->kernel > DGTD_KERNEL_CSR)
         return fail(DGTD_E_ARG, "kernel must be DGTD_KERNEL_AUTO, _VALU, _MFMA or _CSR");
-    if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR)
+    const bool hexes = mesh->dim != 2 && mesh->nvert == 8;  // their matrix-free stage applies sigma too
+    if (mesh->sigma && opts->kernel != DGTD_KERNEL_CSR && !hexes)
         return fail(DGTD_E_UNSUPPORTED, "bulk conductivity needs the assembled operator (DGTD_KERNEL_CSR)");
     if (mesh->dim != 0 && mesh->dim != 2 && mesh->dim != 3) return fail(DGTD_E_ARG, "dim must be 0, 2 or 3");
     if (mesh->nvert != 0 && !(mesh->dim == 2 && (mesh->nvert == 3 || mesh->nvert == 4)) &&
@@ -1605,6 +1660,7 @@ int dgtd_op_set_candidate(dgtd_op* op, int32_t candidate)
 {
     if (!op) return fail(DGTD_E_ARG, "NULL argument");
     if (op->variant == DGTD_KERNEL_CSR) return fail(DGTD_E_ARG, "assembled operator: no stage candidates");
+    if (op->hex) return fail(DGTD_E_ARG, "hexahedra: one matrix-free stage, no candidates");
     if (candidate < 0 || candidate >= dgtd_op::kCandidates) return fail(DGTD_E_ARG, "candidate out of range");
     int var, cfg;
     bool mort;
@@ -1640,6 +1696,12 @@ int dgtd_op_work_per_element(const dgtd_op* op, double* bytes, double* flops)
         const double rows = 6.0 * Np, nnz_e = (double)op->nnz / op->K;
         if (bytes) *bytes = nnz_e * (s + 2.0) + rows * (8.0 + 8.0) + rows * 4.0 * s;
         if (flops) *flops = 2.0 * nnz_e + 4.0 * rows;
+        return DGTD_OK;
+    }
+    if (op->hex) {  // matrix-free hexahedra: state vectors, 36-value record, 12 connectivity ints
+        const double n1 = op->H3.p + 1.0;
+        if (bytes) *bytes = 24.0 * Np * s + 36.0 * s + 48.0;
+        if (flops) *flops = Np * (36.0 * n1 + 198.0) + 240.0 * Nfp;
         return DGTD_OK;
     }
     if (op->dim == 2) {  // triangles: 2 derivative directions, 3 edges, 15-value record

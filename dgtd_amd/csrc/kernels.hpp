@@ -67,6 +67,13 @@ constexpr int kCsrSlotShift = 11;
 template <typename T>
 hipError_t launch_csr_stage(const StageArgs<T>& a, int mode, hipStream_t s);
 
+// Matrix-free hexahedron stage (hex_stage.hip): geo kHexGeoDev and conn
+// kHexConnDev per element, mats = D1 (row-major (p+1)^2) then M1^-1 columns 0
+// and p ((p+1) x 2), tabs = Fmask [6 Nfp], face permutations [48 Nfp], and per
+// face the face node on each node's normal line [6 Np].
+template <typename T>
+hipError_t launch_hex_stage(int order, const StageArgs<T>& a, int mode, hipStream_t s);
+
 template <typename T>
 hipError_t launch_stage(int order, const StageArgs<T>& a, int mode, int grid, hipStream_t s);
 

@@ -56,8 +56,8 @@ def test_hex_mult_matches_oracle(order):
 
 def test_hex_kernel_choice_positions_and_probes():
     vx, E, bc = sheared_hexes(2, 2, 1)
-    op = D.HesthavenEvolution(vx, E, bc, 3)  # kernel "auto" takes the assembled operator
-    assert op.kernel_variant() == "csr"
+    op = D.HesthavenEvolution(vx, E, bc, 3)  # kernel "auto" takes the matrix-free hexahedron stage
+    assert op.kernel_variant() == "valu"
     ref = OH.OperatorHex(vx, E, bc, 3)
     pos = op.node_positions()
     for d in range(3):
@@ -72,10 +72,45 @@ def test_hex_kernel_choice_positions_and_probes():
     q = x.reshape(6, op.K, op.Np)
     want = np.einsum("pn,cpn->pc", w, q[:, elem])
     np.testing.assert_allclose(op.probe_sample(), want, atol=1e-13)
-    for kernel in ("valu", "mfma"):
-        with pytest.raises(D.DgtdError, match="assembled operator") as ei:
-            D.HesthavenEvolution(vx, E, bc, 3, kernel=kernel)
-        assert ei.value.code == -4
+    with pytest.raises(D.DgtdError, match="matrix cores") as ei:
+        D.HesthavenEvolution(vx, E, bc, 3, kernel="mfma")
+    assert ei.value.code == -4
+    with pytest.raises(D.DgtdError):
+        op.set_candidate(0)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_hex_matrix_free_matches_oracle(order):
+    """The matrix-free hexahedron stage (kernel "valu", hex_stage.hip)
+    against the oracle: every face code, random eps, mu, conductivity, both
+    fluxes; FP64 to 1e-12, FP32 to 1e-5; and its RK4 / LSERK4 steps equal
+    the assembled operator's to 1e-12."""
+    vx, E, bc, eps, mu = hex_case(order)
+    sigma = np.where(np.arange(E.shape[0]) % 3 == 0, 0.7, 0.0)
+    for alpha in (1.0, 0.0, 0.5):
+        ref = OH.OperatorHex(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu)
+        x = np.random.default_rng(order).standard_normal(ref.ndofs())
+        yr = ref.mult(x)
+        m64 = D.HesthavenEvolution(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, precision=64, kernel="valu")
+        assert m64.kernel_variant() == "valu"
+        assert np.abs(m64.mult(x) - yr).max() <= 1e-12 * np.abs(yr).max()
+        m32 = D.HesthavenEvolution(vx, E, bc, order, alpha=alpha, eps=eps, mu=mu, precision=32, kernel="valu")
+        assert rel(m32.mult(x), yr) <= 1e-5
+        m64.close()
+        m32.close()
+    for integrator in ("rk4", "lserk4"):
+        a = D.HesthavenEvolution(vx, E, bc, order, eps=eps, mu=mu, sigma=sigma, precision=64, kernel="valu",
+                                 integrator=integrator)
+        g = D.GlobalEvolution(vx, E, bc, order, eps=eps, mu=mu, sigma=sigma, precision=64, integrator=integrator)
+        x0 = np.random.default_rng(7).standard_normal(a.ndofs)
+        assert rel(a.mult(x0), g.mult(x0)) <= 1e-12  # conductivity included
+        dt = 0.3 / (a.max_fscale() * (order + 1) ** 2)
+        a.set_state(x0)
+        g.set_state(x0)
+        assert a.run(dt, 5.5 * dt) == g.run(dt, 5.5 * dt) == 6
+        assert rel(a.get_state(), g.get_state()) <= 1e-12
+        a.close()
+        g.close()
 
 
 def _hex_waveguide(nx=16, length=4.0, x_tfsf=1.0):
@@ -92,13 +127,15 @@ def _hex_waveguide(nx=16, length=4.0, x_tfsf=1.0):
     return vx, E, bc
 
 
+@pytest.mark.parametrize("kernel", ["csr", "valu"])
 @pytest.mark.parametrize("integrator", ["rk4", "lserk4"])
-def test_hex_tfsf_run_matches_oracle(integrator):
+def test_hex_tfsf_run_matches_oracle(integrator, kernel):
     vx, E, bc = _hex_waveguide()
     wave = dict(polarization=(0, 0, 1), propagation=(1, 0, 0), spread=0.3, mean=-1.0)
     ref = OH.OperatorHex(vx, E, bc, 3)
     ref.set_planewaves([wave])
-    gpu = D.GlobalEvolution(vx, E, bc, 3, precision=64, integrator=integrator)
+    gpu = D.HesthavenEvolution(vx, E, bc, 3, precision=64, integrator=integrator, kernel=kernel)
+    assert gpu.kernel_variant() == kernel
     gpu.set_planewaves([D.planewave((0, 0, 1), (1, 0, 0), 0.3, -1.0)])
     x = np.random.default_rng(2).standard_normal(ref.ndofs())
     gpu.SetTime(0.9)
