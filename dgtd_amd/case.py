@@ -120,10 +120,14 @@ class Case:
         """True when kernel "auto" runs the assembled operator ("csr"), as the
         launcher decides: conductive materials, SMA faces (boundary or
         interior) with alpha != 1 under the Global or Maxwell operator (which
-        weight SMA faces as at alpha = 1), quadrilateral and hexahedral meshes."""
+        weight SMA faces as at alpha = 1), quadrilateral meshes.  Hexahedral
+        meshes never: their matrix-free stage applies the Global operator's
+        flux and conductivity."""
+        if self.hexes:
+            return False
         global_sma = (self.info["evolution_operator"] != "hesthaven" and self.info["alpha"] != 1.0
                       and bool(((self.face_bc == 3) | (self.face_bc == 6)).any()))
-        return self.sigma is not None or global_sma or self.quads or self.hexes
+        return self.sigma is not None or global_sma or self.quads
 
     def operator(self, precision=64, integrator="rk4", kernel="auto", device=0):
         """The case's operator; kernel="auto" selects the assembled one ("csr")

@@ -108,7 +108,9 @@ int main(int argc, char** argv)
     for (int64_t i = 0; i < (int64_t)mesh.K * nfaces; ++i)
         sma = sma || mesh.face_bc[i] == DGTD_BC_SMA || mesh.face_bc[i] == DGTD_BC_INT_SMA;
     const bool global_sma = info.evolution_operator != DGTD_OPERATOR_HESTHAVEN && sma && info.alpha != 1.0;
-    o.kernel = kernel == DGTD_KERNEL_AUTO && (mesh.sigma || global_sma) ? DGTD_KERNEL_CSR : kernel;
+    // (hexahedra: AUTO takes their matrix-free stage, which applies the Global
+    // operator's flux and conductivity)
+    o.kernel = kernel == DGTD_KERNEL_AUTO && mesh.nvert != 8 && (mesh.sigma || global_sma) ? DGTD_KERNEL_CSR : kernel;
     dgtd_op* op = nullptr;
     check(dgtd_op_create(&mesh, &o, &op), "operator");
 

@@ -91,7 +91,7 @@ typedef struct {
                              * components, Np = (p+1)(p+2)/2. */
     const double* sigma;    /* K bulk conductivity, normalised as the reference's
                              * Material (sigma_SI * Z0, Material.h:30, driver.cpp:962-963),
-                             * NULL = 0.  Only with DGTD_KERNEL_CSR: the Global operator's
+                             * NULL = 0.  Only with DGTD_KERNEL_CSR (and on hexahedra): the Global operator's
                              * -M_eps^-1 M_sigma E term (collectGlobalConductiveOperator,
                              * DGOperatorFactory.h:1351-1361), -sigma/eps_r E per element. */
     int32_t nvert;          /* vertices per element.  With dim = 2: 0 or 3 triangles, 4
@@ -106,10 +106,12 @@ typedef struct {
                              * (3,2,1,0), (0,1,5,4), (1,2,6,5), (2,3,7,6), (3,0,4,7),
                              * (4,5,6,7)), nodes are the (p+1)^3 Gauss-Lobatto tensor
                              * points, node i + (p+1) j + (p+1)^2 k at (x_i, x_j, x_k).
-                             * Both run on the assembled (Global) operator only
-                             * (DGTD_KERNEL_CSR; AUTO selects it), the reference's default
+                             * Both run the Global operator's flux, the reference's default
                              * operator on its quadrilateral and hexahedral cases
-                             * (ExtensiveCasesTest.cpp:1003,1082, Solver3DTest.cpp:57-215). */
+                             * (ExtensiveCasesTest.cpp:1003,1082, Solver3DTest.cpp:57-215):
+                             * quadrilaterals assembled (DGTD_KERNEL_CSR; AUTO selects it),
+                             * hexahedra assembled (DGTD_KERNEL_CSR) or matrix-free
+                             * (DGTD_KERNEL_VALU, which AUTO selects; sigma allowed). */
 } dgtd_mesh_desc;
 
 /* Options; mirrors EvolutionOptions (src/evolution/EvolutionOptions.h:13-20)
@@ -124,7 +126,8 @@ typedef struct {
 
 /* Stage-kernel variants (both fuse gather, flux, curl, LIFT and the RK update). */
 #define DGTD_KERNEL_AUTO 0 /* time the available variants on this mesh at create, keep the fastest */
-#define DGTD_KERNEL_VALU 1 /* per-node vector FMA, reference matrices in LDS */
+#define DGTD_KERNEL_VALU 1 /* per-node vector FMA, reference matrices in LDS; on hexahedra the
+                            * matrix-free tensor-product stage with the Global operator's flux */
 #define DGTD_KERNEL_MFMA 2 /* matrix cores (v_mfma 16x16x4), 8-element wave tasks */
 /* The assembled operator: the reference's Global operator path
  * (GlobalEvolution::Mult, GlobalEvolution.cpp:628-1088, a CSR SpMV over the
@@ -134,8 +137,9 @@ typedef struct {
  * their upwind terms by 1 whatever alpha, as the Global operator does
  * (DGOperatorFactory.h:485-495, 521-531, 557-567; identical at alpha = 1),
  * plus bulk conductivity; one fused kernel does SpMV and stage update.
- * Chosen explicitly (never by autotune) on tetrahedra and triangles; the
- * only kernel of quadrilaterals and hexahedra (AUTO selects it there). */
+ * Chosen explicitly (never by autotune) on tetrahedra, triangles and
+ * hexahedra (whose AUTO takes their matrix-free stage, DGTD_KERNEL_VALU);
+ * the only kernel of quadrilaterals (AUTO selects it there). */
 #define DGTD_KERNEL_CSR 3
 
 const char* dgtd_last_error(void);

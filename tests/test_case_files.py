@@ -719,3 +719,12 @@ def test_case_kernel_choice_counts_interior_sma(tmp_path):
     assert not variant("d.json", "SMA", 0.5, "hesthaven").needs_assembled_operator()
     q = D.Case(os.path.join(CASES, "2D_TFSF_Quads.json"))
     assert q.quads and q.needs_assembled_operator()
+    # hexahedra: the matrix-free stage carries the Global operator's flux
+    write_gmsh_hexes(str(tmp_path / "h.msh"), 2, 1, 1)
+    h = json.loads(json.dumps(case))
+    h["model"].update(filename="h.msh", boundaries=[{"tags": [1, 2, 3, 4, 5, 6], "type": "SMA"}])
+    h["sources"] = []
+    h["solver_options"].update(evolution_operator="global", upwind_alpha=0.5)
+    (tmp_path / "h.json").write_text(json.dumps(h))
+    hc = D.Case(str(tmp_path / "h.json"))
+    assert hc.hexes and not hc.needs_assembled_operator()
