@@ -537,7 +537,8 @@ int batch_error(const std::exception& e)
 // Quadrilaterals and hexahedra run on the assembled (Global) operator only;
 // the batch advances hesthaven-operator instances.
 constexpr const char* kBatchNoTensorElements =
-    "quadrilaterals and hexahedra run on the assembled operator only (dgtd_op_create with DGTD_KERNEL_CSR), "
+    "quadrilaterals and hexahedra run the Global operator only (dgtd_op_create: DGTD_KERNEL_CSR, or for hexahedra "
+    "DGTD_KERNEL_AUTO / _VALU), "
     "not in a batch";
 inline bool tensor_elements(const dgtd_mesh_desc* mesh)
 {

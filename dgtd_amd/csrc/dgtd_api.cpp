@@ -128,7 +128,7 @@ struct dgtd_op {
     dgtd::RefTri T2;
     bool quad = false;
     dgtd::RefQuad Q2;
-    bool hex = false;  // hexahedra (dim 3, RefHex H3; the assembled operator only)
+    bool hex = false;  // hexahedra (dim 3, RefHex H3; the Global operator: assembled or matrix-free)
     dgtd::RefHex H3;
     int K = 0, Np = 0, Nfp = 0;
     size_t N = 0;

@@ -104,11 +104,11 @@ def test_continuous_admission_and_removal():
 
 
 def test_add_refuses_quadrilaterals_and_hexahedra():
-    """Tensor-product elements run on the assembled operator only; the batch
+    """Tensor-product elements run the Global operator only; the batch
     refuses them at admission and stays empty."""
     b = D.Batch(precision=32, max_instances=4)
     for vx, E, fbc in (D.cartesian_2d(3, 2, "quad"), D.cartesian_3d(2, 2, 1, "hex")):
-        with pytest.raises(D.DgtdError, match="assembled operator only") as ei:
+        with pytest.raises(D.DgtdError, match="Global operator only") as ei:
             b.add(vx, E, fbc, 2, final_time=0.1)
         assert ei.value.code == -4
     assert b.stats()["bytes_in_use"] == 0

@@ -8,7 +8,8 @@ FP64 RK4 / LSERK4 runs equal the oracle's to 1e-11 relative L2; FP32 agrees
 with FP64 to 1e-5; the reference's Solver3DTest cases pec_global_1dot5D
 (hexahedra and tetrahedra) and sma_upwind_hexa_1dot5D
 (Solver3DTest.cpp:57-215) meet their own tolerances.  The matrix-free
-hesthaven kernels refuse hexahedra.
+hexahedron stage (kernel "valu", the default on hexahedra) meets the same
+bars; the matrix-core kernels refuse hexahedra.
 """
 import numpy as np
 import pytest

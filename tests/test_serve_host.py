@@ -111,9 +111,9 @@ def test_footprint_counts_the_element_shape():
 
 def test_batch_refuses_quadrilaterals_and_hexahedra():
     """The batch advances hesthaven-operator instances; quadrilaterals and
-    hexahedra run on the assembled operator only, so the admission measure
+    hexahedra run the Global operator only, so the admission measure
     refuses them (the same check guards dgtd_batch_add)."""
     for vx, E, fbc in (D.cartesian_2d(3, 2, "quad"), D.cartesian_3d(2, 2, 1, "hex")):
-        with pytest.raises(D.DgtdError, match="assembled operator only") as ei:
+        with pytest.raises(D.DgtdError, match="Global operator only") as ei:
             D.footprint(vx, E, fbc, 2)
         assert ei.value.code == -4
